@@ -1,0 +1,220 @@
+/*
+ * mckl_hip.h -- C ABI of the MI355X (gfx950) implementation of MCKL's Monte
+ * Carlo hot path.
+ *
+ * Every entry point returns an int status: 0 on success, a negative value on
+ * failure (MCKL_HIP_EINVAL for invalid arguments, MCKL_HIP_EHIP - code for a
+ * HIP runtime error).  mckl_hip_last_error() returns a description of the
+ * last failure on the calling thread.  Pointers named d_* are device memory;
+ * h_* are host memory.  `stream` is a hipStream_t (NULL = default stream).
+ * Calls on distinct engine states and streams are thread-safe.  Output bits
+ * depend only on (key, counter, n, parameters), never on grid shape.
+ *
+ * Reference paths below are relative to the zhouyan/MCKL source tree.
+ */
+#ifndef MCKL_HIP_H
+#define MCKL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCKL_HIP_OK 0
+#define MCKL_HIP_EINVAL (-22)
+#define MCKL_HIP_ENOMEM (-12)
+#define MCKL_HIP_EHIP (-1000)
+
+/* engine kinds */
+#define MCKL_HIP_PHILOX4X32 1   /* random/philox.hpp:87-180, 10 rounds */
+#define MCKL_HIP_THREEFRY4X64 2 /* random/threefry.hpp:82-205, 20 rounds */
+#define MCKL_HIP_ARS 3          /* random/aes.hpp:135-214, ARS-5 */
+
+/* U01 intervals: random/u01_distribution.hpp:140-211 */
+#define MCKL_HIP_U01_CC 0
+#define MCKL_HIP_U01_CO 1
+#define MCKL_HIP_U01_OC 2
+#define MCKL_HIP_U01_OO 3
+#define MCKL_HIP_U01_CANONICAL 4
+
+/* resampling schemes: algorithm/resample.hpp:43-59 (same numbering) */
+#define MCKL_HIP_MULTINOMIAL 0
+#define MCKL_HIP_STRATIFIED 2
+#define MCKL_HIP_SYSTEMATIC 3
+
+/* models of the built-in SMC samplers */
+#define MCKL_HIP_MODEL_LG 1 /* x' = a x + sx e, y = x + sy n, x0 ~ N(0,1) */
+#define MCKL_HIP_MODEL_SV 2 /* x' = phi x + s e, y = beta exp(x/2) n */
+
+/*
+ * Counter-engine state.  Mirrors CounterEngine (random/counter.hpp:61-288):
+ * key, counter and the number of words of the buffered block already
+ * consumed (index == words per block means "empty").  The buffered block is
+ * the generator applied to `ctr` and is recomputed on the device when
+ * needed, so the state is plain host data that the host advances exactly as
+ * the reference's bulk and scalar calls advance it.
+ *   Philox4x32: key[0..1] hold the two u32 key words; ctr[0..1] used.
+ *   Threefry4x64: key[0..3] u64; ctr[0..3].
+ *   ARS: key[0..3] hold the four u32 key words; ctr[0..1].
+ * rbits is the result width (32, or 64 for the *_64 engines).
+ */
+typedef struct mckl_hip_engine {
+    uint64_t ctr[4];
+    uint64_t key[4];
+    uint32_t index;
+    int32_t kind;
+    int32_t rbits;
+    int32_t reserved;
+} mckl_hip_engine;
+
+const char *mckl_hip_last_error(void);
+
+/* Words of result_type per block (Generator::size()/sizeof(ResultType)). */
+int mckl_hip_engine_words(const mckl_hip_engine *e);
+
+/* CounterEngine(const key_type&) / seed(key): counter.hpp:112-114,282-287 */
+int mckl_hip_engine_init(
+    mckl_hip_engine *e, int kind, int rbits, const uint64_t *key);
+
+/* seed(result_type s): key = {s, 0, ...}; counter.hpp:97-103 */
+int mckl_hip_engine_seed(mckl_hip_engine *e, uint64_t s);
+
+/* ctr(const ctr_type&): counter.hpp:123-127 */
+int mckl_hip_engine_set_ctr(mckl_hip_engine *e, const uint64_t *ctr);
+
+/* discard(skip_type): counter.hpp:178-199 (host arithmetic only) */
+int mckl_hip_engine_discard(mckl_hip_engine *e, uint64_t nskip);
+
+/* Scalar operator()(): counter.hpp:134-142.  Computes the block on the
+ * device and returns the word to the host (synchronous). */
+int mckl_hip_engine_next(mckl_hip_engine *e, uint64_t *h_out);
+
+/* The buffered block G(ctr) as bytes (16 or 32), computed on the device
+ * (synchronous); used for the reference's textual state output. */
+int mckl_hip_engine_block(const mckl_hip_engine *e, void *h_block);
+
+/*
+ * Raw generator kernels with the shape of the reference's native C ABI
+ * (include/mckl/mckl.h:313-341, e.g. mckl_philox4x32_sse2_kernel): n blocks
+ * for counters ctr+1 .. ctr+n written to d_r (n * 16 or n * 32 bytes), and
+ * ctr advanced by n.  Unlike the NASM kernels the counter carries across all
+ * words (the reference's callers split at the carry to the same effect:
+ * random/internal/philox_sse2_4x32.hpp:71-92).  Asynchronous on `stream`.
+ */
+int mckl_hip_philox4x32_kernel(
+    uint64_t *ctr, size_t n, void *d_r, const uint32_t *key, void *stream);
+int mckl_hip_threefry4x64_kernel(
+    uint64_t *ctr, size_t n, void *d_r, const uint64_t *key, void *stream);
+int mckl_hip_ars_kernel(
+    uint64_t *ctr, size_t n, void *d_r, const uint32_t *key, void *stream);
+
+/* rand(rng, n, r): counter.hpp:142-165,298-303.  n words of result_type
+ * (u32 for rbits 32, u64 for rbits 64) into d_r; state advanced as the
+ * reference's bulk call advances it. */
+int mckl_hip_rand(mckl_hip_engine *e, size_t n, void *d_r, void *stream);
+
+/* rand(rng, U01xxDistribution<T>, n, r) for 32-bit engines:
+ * random/u01_distribution.hpp:79-142, random/internal/u01_generic.hpp:48-198,
+ * random/internal/common.hpp:69-86 (chunking by BufferSize<T>). */
+int mckl_hip_rand_u01_f32(
+    mckl_hip_engine *e, int interval, size_t n, float *d_r, void *stream);
+int mckl_hip_rand_u01_f64(
+    mckl_hip_engine *e, int interval, size_t n, double *d_r, void *stream);
+
+/* rand(rng, NormalDistribution<T>(mean, sd), n, r) for 32-bit engines:
+ * random/normal_distribution.hpp:50-101 (chunked Box-Muller, odd tail). */
+int mckl_hip_rand_normal_f32(mckl_hip_engine *e, size_t n, float mean,
+    float sd, float *d_r, void *stream);
+int mckl_hip_rand_normal_f64(mckl_hip_engine *e, size_t n, double mean,
+    double sd, double *d_r, void *stream);
+
+/* The same fills for `nstreams` independent engines in one launch: engine s
+ * writes n values to d_r + s*n and is advanced as if it had made the
+ * single-engine call.  (No reference counterpart: the reference loops over
+ * engines; one launch keeps the GPU full.) */
+int mckl_hip_rand_u01_f64_streams(mckl_hip_engine *e, size_t nstreams,
+    int interval, size_t n, double *d_r, void *stream);
+int mckl_hip_rand_normal_f32_streams(mckl_hip_engine *e, size_t nstreams,
+    size_t n, float mean, float sd, float *d_r, void *stream);
+int mckl_hip_rand_normal_f64_streams(mckl_hip_engine *e, size_t nstreams,
+    size_t n, double mean, double sd, double *d_r, void *stream);
+
+/*
+ * Weight (core/weight.hpp:44-228) on device-resident normalised FP64
+ * weights d_w.  d_stats (device, 2 doubles) receives {ESS, lmax+log(accw)};
+ * the second value is the log normaliser increment that the reference
+ * computes in normalize() and discards (weight.hpp:198-227).
+ */
+int mckl_hip_weight_set_equal(size_t n, double *d_w, double *d_stats,
+    void *stream); /* weight.hpp:76-80 */
+int mckl_hip_weight_set_log(size_t n, const double *d_v, double *d_w,
+    double *d_stats, void *stream); /* weight.hpp:124-129 */
+int mckl_hip_weight_add_log(size_t n, const double *d_v, double *d_w,
+    double *d_stats, void *stream); /* weight.hpp:143-148 */
+
+/*
+ * Resampling (algorithm/resample.hpp).  Replication counts and parent
+ * indices are u32.  d_u01: M sorted uniforms.
+ */
+/* u01_rand_sorted / _stratified / _systematic: resample.hpp:216-268 */
+int mckl_hip_u01_rand_sequence(
+    int scheme, mckl_hip_engine *e, size_t n, double *d_r, void *stream);
+/* resample_trans_u01_rep: resample.hpp:374-407 */
+int mckl_hip_resample_trans_u01_rep(size_t N, size_t M, const double *d_w,
+    const double *d_u01, uint32_t *d_rep, void *stream);
+/* resample_trans_rep_index: resample.hpp:416-466 */
+int mckl_hip_resample_trans_rep_index(size_t N, size_t M,
+    const uint32_t *d_rep, uint32_t *d_idx, void *stream);
+/* ResampleAlgorithm::operator() + rep->index: resample.hpp:515-576,
+ * ResampleEval::operator() resample.hpp:494-507 */
+int mckl_hip_resample(int scheme, size_t N, size_t M, mckl_hip_engine *e,
+    const double *d_w, uint32_t *d_rep, uint32_t *d_idx, void *stream);
+/* StateMatrix<RowMajor,dim,float>::select: core/state_matrix.hpp:193-265
+ * (out of place: d_dst[j] = d_src[d_idx[j]], rows of `dim` floats) */
+int mckl_hip_select_f32(size_t M, size_t dim, const uint32_t *d_idx,
+    const float *d_src, float *d_dst, void *stream);
+
+/*
+ * Plain SMC sampler (SMCSampler::iterate, algorithm/smc.hpp:352-406) for the
+ * built-in models, FP32 states and log-weights, FP64 reductions and scans.
+ * The handle owns its device buffers; run() is asynchronous and contains no
+ * host synchronisation (the ESS test is taken on the device), so it can be
+ * captured into a HIP graph.
+ */
+typedef struct mckl_hip_smc mckl_hip_smc;
+int mckl_hip_smc_create(int model, const double *theta, size_t T,
+    const double *h_y, size_t N, uint64_t seed, double threshold, int scheme,
+    mckl_hip_smc **out);
+int mckl_hip_smc_run(mckl_hip_smc *s, void *stream);
+/* loglik, and optionally per-step ESS and resampling flags (length T) */
+int mckl_hip_smc_result(mckl_hip_smc *s, double *h_loglik, double *h_ess,
+    int32_t *h_resampled);
+int mckl_hip_smc_destroy(mckl_hip_smc *s);
+
+/*
+ * Batched SMC: nfilters independent filters of uneven sizes in one
+ * persistent launch (no reference counterpart; the reference runs one
+ * SMCSampler at a time).  Filter f uses theta[3f..3f+2], y[T f .. T f+T-1],
+ * seed[f] and is bit-identical to the plain sampler with the same inputs.
+ */
+typedef struct mckl_hip_smc_batch mckl_hip_smc_batch;
+typedef struct mckl_hip_tune {
+    int32_t block;       /* threads per workgroup */
+    int32_t waves_per_eu; /* occupancy hint: 0 = default */
+} mckl_hip_tune;
+int mckl_hip_smc_batch_create(int model, size_t nfilters,
+    const uint32_t *h_N, const double *h_theta, size_t T, const double *h_y,
+    const uint64_t *h_seed, double threshold, int scheme,
+    mckl_hip_smc_batch **out);
+int mckl_hip_smc_batch_set_tune(mckl_hip_smc_batch *b, const mckl_hip_tune *t);
+int mckl_hip_smc_batch_run(mckl_hip_smc_batch *b, int use_graph, void *stream);
+int mckl_hip_smc_batch_result(mckl_hip_smc_batch *b, double *h_loglik);
+int mckl_hip_smc_batch_destroy(mckl_hip_smc_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

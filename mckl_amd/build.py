@@ -1,0 +1,86 @@
+"""Build step for libmckl_hip.so (gfx950) and the C++ host-API test program.
+
+Run as ``python -m mckl_amd.build`` or through ``__graft_entry__.build()``.
+Every HIP source is compiled with -ffp-contract=off (each floating-point
+operation rounds once, as in the reference build) and with packed-FP32
+instruction selection disabled.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmckl_hip.so")
+
+HIP_SOURCES = ["rng.hip", "smc.hip"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def hipcc():
+    cand = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc")
+    return cand if os.path.exists(cand) else "hipcc"
+
+
+def build_lib(jobs=4):
+    objs = []
+    procs = []
+    for src in HIP_SOURCES:
+        s = os.path.join(CSRC, src)
+        if not os.path.exists(s):
+            continue
+        o = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc()] + HIPCC_FLAGS + ["-c", s, "-o", o]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd), s))
+        objs.append(o)
+    for p, s in procs:
+        if p.wait() != 0:
+            raise RuntimeError("hipcc failed on %s" % s)
+    tmp = LIB + ".tmp"
+    _run([hipcc()] + HIPCC_FLAGS + ["-shared", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+def build_host_test():
+    """C++ program exercising the host-side API mirror (tests/cpp)."""
+    src = os.path.join(ROOT, "tests", "cpp", "host_api_test.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "host_api_test")
+    _run([
+        "g++", "-std=c++14", "-O2", "-Wall",
+        "-I", os.path.join(ROOT, "include"),
+        "-I", os.path.join(HERE, "include"),
+        src, "-o", out,
+        "-L", HERE, "-lmckl_hip", "-Wl,-rpath," + HERE,
+    ])
+    return out
+
+
+def main():
+    build_lib()
+    build_host_test()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,1072 @@
+// rng.hip -- bulk fills of the counter-based engines on gfx950: raw words,
+// fused U01 conversions and chunked Box-Muller Normal, plus the host-side
+// engine state machine that mirrors CounterEngine (random/counter.hpp).
+//
+// Layout: each lane computes whole counter blocks (16 or 32 bytes) and
+// stores them with 16-byte vector stores, so a wavefront writes 1-2 KiB of
+// contiguous HBM per store instruction.  A call that starts inside the
+// engine's buffered block is handled by indexing in "block space": block 0
+// is the buffered block, and the first `skip` elements are masked.
+#include "common.hpp"
+#include "engines.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace mckl_hip {
+
+std::string &last_error_ref()
+{
+    static thread_local std::string s;
+    return s;
+}
+
+__constant__ uint32_t c_aes_t0[256];
+
+// ---------------------------------------------------------------------------
+// AES T0 table from GF(2^8) arithmetic (FIPS-197 5.1.1); layout as
+// random/internal/aes_constants.hpp:41 (bytes 2S, S, S, 3S).
+// ---------------------------------------------------------------------------
+static uint8_t gf_mul(uint8_t a, uint8_t b)
+{
+    uint8_t p = 0;
+    for (int i = 0; i < 8; ++i) {
+        if (b & 1)
+            p ^= a;
+        const uint8_t hi = a & 0x80;
+        a = (uint8_t)(a << 1);
+        if (hi)
+            a ^= 0x1B;
+        b >>= 1;
+    }
+    return p;
+}
+
+static void build_t0(uint32_t *t0)
+{
+    for (int x = 0; x < 256; ++x) {
+        uint8_t inv = 0;
+        for (int y = 1; x && y < 256; ++y)
+            if (gf_mul((uint8_t)x, (uint8_t)y) == 1) {
+                inv = (uint8_t)y;
+                break;
+            }
+        const uint8_t b = inv;
+        const uint8_t s = (uint8_t)(b ^ ((b << 1) | (b >> 7)) ^
+            ((b << 2) | (b >> 6)) ^ ((b << 3) | (b >> 5)) ^
+            ((b << 4) | (b >> 4)) ^ 0x63);
+        t0[x] = (uint32_t)gf_mul(s, 2) | ((uint32_t)s << 8) |
+            ((uint32_t)s << 16) | ((uint32_t)gf_mul(s, 3) << 24);
+    }
+}
+
+int ensure_device_tables()
+{
+    static std::mutex mu;
+    static bool done[64] = {false};
+    int dev = 0;
+    MCKL_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64)
+        return set_error(MCKL_HIP_EINVAL, "device index %d unsupported", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!done[dev]) {
+        uint32_t t0[256];
+        build_t0(t0);
+        MCKL_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_aes_t0), t0, sizeof(t0)));
+        done[dev] = true;
+    }
+    return MCKL_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+constexpr int kMaxBatch = 16; // streams per launch (kernel-argument payload)
+constexpr int kBlock = 256;
+
+struct FillArg {
+    StreamArg s;
+    uint64_t skip;   // elements of block space before the call's first one
+    uint32_t vec_ok; // (out - skip) is 16-byte aligned
+    uint32_t pad;
+};
+
+struct FillBatch {
+    FillArg a[kMaxBatch];
+};
+
+template <class Gen>
+struct LdsWords {
+    static constexpr int value = 1;
+};
+template <>
+struct LdsWords<ArsGen> {
+    static constexpr int value = ARS_LDS_WORDS;
+};
+
+struct ConvU32 {
+    using T = uint32_t;
+    static constexpr int Q = 1;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return w[0];
+    }
+};
+struct ConvU64 {
+    using T = uint64_t;
+    static constexpr int Q = 2;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+    }
+};
+struct ConvU01F64 {
+    using T = double;
+    static constexpr int Q = 1;
+    int iv;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return u01_f64(iv, w[0]);
+    }
+};
+struct ConvU01F32 {
+    using T = float;
+    static constexpr int Q = 1;
+    int iv;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return u01_f32(iv, w[0]);
+    }
+};
+struct ConvCanF64 {
+    using T = double;
+    static constexpr int Q = 2;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return u01_canonical_f64(w[0], w[1]);
+    }
+};
+struct ConvCanF32 {
+    using T = float;
+    static constexpr int Q = 1;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return u01_canonical_f32(w[0]);
+    }
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void store_vec(T *p, const T (&v)[N])
+{
+    static_assert((sizeof(T) * N) % 16 == 0, "vector store granularity");
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    constexpr int NV = (int)(sizeof(T) * N / 16);
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(v);
+    u32x4 *dst = reinterpret_cast<u32x4 *>(p);
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+        __builtin_nontemporal_store(src[i], dst + i);
+}
+
+// Element-granular fill.  Block b of stream y covers elements
+// [b*EPB - skip, (b+1)*EPB - skip) of that stream's output.
+template <class Gen, class Conv>
+__global__ __launch_bounds__(kBlock) void fill_kernel(FillBatch batch,
+    Conv conv, uint64_t nblocks, uint64_t n, typename Conv::T *out)
+{
+    using T = typename Conv::T;
+    constexpr int EPB = Gen::W32 / Conv::Q;
+    __shared__ uint32_t lds[LdsWords<Gen>::value];
+    const FillArg &arg = batch.a[blockIdx.y];
+    Gen g;
+    g.init(arg.s);
+    g.prepare(lds);
+    T *o = out + (uint64_t)blockIdx.y * n;
+    const int64_t skip = (int64_t)arg.skip;
+    const bool vec_ok = arg.vec_ok != 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+         b < nblocks; b += stride) {
+        uint32_t w[Gen::W32];
+        g.block(b, w);
+        T v[EPB];
+#pragma unroll
+        for (int j = 0; j < EPB; ++j)
+            v[j] = conv(w + j * Conv::Q);
+        const int64_t e0 = (int64_t)(b * EPB) - skip;
+        if (vec_ok && e0 >= 0 && e0 + EPB <= (int64_t)n) {
+            store_vec(o + e0, v);
+        } else {
+#pragma unroll
+            for (int j = 0; j < EPB; ++j) {
+                const int64_t e = e0 + j;
+                if (e >= 0 && e < (int64_t)n)
+                    o[e] = v[j];
+            }
+        }
+    }
+}
+
+// One element per lane, for conversions whose elements straddle blocks
+// (Canonical<double> at an odd word offset).  Element e uses words
+// skipw + 2e, skipw + 2e + 1 of block space.
+template <class Gen>
+__global__ __launch_bounds__(kBlock) void fill_canonical_straddle_kernel(
+    FillBatch batch, uint64_t n, double *out)
+{
+    __shared__ uint32_t lds[LdsWords<Gen>::value];
+    const FillArg &arg = batch.a[blockIdx.y];
+    Gen g;
+    g.init(arg.s);
+    g.prepare(lds);
+    double *o = out + (uint64_t)blockIdx.y * n;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += stride) {
+        const uint64_t p = arg.skip + 2 * e; // word position in block space
+        uint32_t w0[Gen::W32], w1[Gen::W32];
+        g.block(p / Gen::W32, w0);
+        g.block(p / Gen::W32 + 1, w1);
+        const uint32_t j = (uint32_t)(p % Gen::W32);
+        const uint32_t a = w0[j];
+        const uint32_t b = j + 1 < (uint32_t)Gen::W32 ? w0[j + 1] : w1[0];
+        o[e] = u01_canonical_f64(a, b);
+    }
+}
+
+// ---- Normal (chunked Box-Muller) ------------------------------------------
+template <typename T>
+struct NormalTraits;
+template <>
+struct NormalTraits<float> {
+    static constexpr int K = 2048; // BufferSize<float>
+};
+template <>
+struct NormalTraits<double> {
+    static constexpr int K = 1024; // BufferSize<double>
+};
+
+// Box-Muller on one (s-word, theta-word) pair with the reference's op order
+// (normal_distribution.hpp:58-77).  Float transcendentals are evaluated in
+// double and rounded once, which is within 1 ulp of glibc's logf/sinf/cosf.
+__device__ __forceinline__ void box_muller(uint32_t ws, uint32_t wt,
+    float mean, float sd, float &o_sin, float &o_cos)
+{
+    const float u1 = u01_f32(2, ws);
+    const float u2 = u01_f32(2, wt);
+    float s = (float)log((double)u1);
+    s = __fmul_rn(s, -2.0f);
+    s = __fsqrt_rn(s);
+    const float th = __fmul_rn(6.283185307179586476925286766559005768394f, u2);
+    double sn, cs;
+    sincos((double)th, &sn, &cs);
+    const float a = (float)sn, b = (float)cs;
+    if (sd != 1.0f)
+        s = __fmul_rn(sd, s);
+    if (mean != 0.0f) {
+        o_sin = __fadd_rn(__fmul_rn(s, a), mean);
+        o_cos = __fadd_rn(__fmul_rn(s, b), mean);
+    } else {
+        o_sin = __fmul_rn(s, a);
+        o_cos = __fmul_rn(s, b);
+    }
+}
+
+__device__ __forceinline__ void box_muller(uint32_t ws, uint32_t wt,
+    double mean, double sd, double &o_sin, double &o_cos)
+{
+    const double u1 = u01_f64(2, ws);
+    const double u2 = u01_f64(2, wt);
+    double s = log(u1);
+    s = __dmul_rn(s, -2.0);
+    s = __dsqrt_rn(s);
+    const double th = __dmul_rn(6.283185307179586, u2);
+    double a, b;
+    sincos(th, &a, &b);
+    if (sd != 1.0)
+        s = __dmul_rn(sd, s);
+    if (mean != 0.0) {
+        o_sin = __dadd_rn(__dmul_rn(s, a), mean);
+        o_cos = __dadd_rn(__dmul_rn(s, b), mean);
+    } else {
+        o_sin = __dmul_rn(s, a);
+        o_cos = __dmul_rn(s, b);
+    }
+}
+
+// odd tail: normal_distribution.hpp:93-100
+__device__ __forceinline__ float box_muller_tail(
+    uint32_t wu, uint32_t wv, float mean, float sd)
+{
+    const float u = u01_f32(2, wu), v = u01_f32(2, wv);
+    const float l = (float)log((double)u);
+    const float r = __fsqrt_rn(__fmul_rn(-2.0f, l));
+    const float c = (float)cos(
+        (double)__fmul_rn(6.283185307179586476925286766559005768394f, v));
+    return __fadd_rn(mean, __fmul_rn(__fmul_rn(sd, r), c));
+}
+
+__device__ __forceinline__ double box_muller_tail(
+    uint32_t wu, uint32_t wv, double mean, double sd)
+{
+    const double u = u01_f64(2, wu), v = u01_f64(2, wv);
+    const double r = __dsqrt_rn(__dmul_rn(-2.0, log(u)));
+    const double c = cos(__dmul_rn(6.283185307179586, v));
+    return __dadd_rn(mean, __dmul_rn(__dmul_rn(sd, r), c));
+}
+
+// Word p (relative to the call's first word) of a stream whose first word is
+// word `skip` of block 0.
+template <class Gen>
+__device__ __forceinline__ uint32_t word_at(const Gen &g, uint64_t skip,
+    uint64_t p)
+{
+    const uint64_t q = p + skip;
+    uint32_t w[Gen::W32];
+    g.block(q / Gen::W32, w);
+    uint32_t r = w[0];
+#pragma unroll
+    for (int j = 1; j < Gen::W32; ++j)
+        if ((uint32_t)(q % Gen::W32) == (uint32_t)j)
+            r = w[j];
+    return r;
+}
+
+// Full chunks.  Task (c, g): pairs g*W .. g*W+W-1 of chunk c, W = words per
+// block.  With skip == 0 the s-words and theta-words of a task are exactly
+// one block each; otherwise each window straddles two blocks.
+template <class Gen, typename T>
+__global__ __launch_bounds__(kBlock) void normal_kernel(FillBatch batch,
+    uint64_t nchunks, uint64_t n, T mean, T sd, T *out)
+{
+    constexpr int K = NormalTraits<T>::K;
+    constexpr int H = K / 2;
+    constexpr int W = Gen::W32;
+    constexpr int G = H / W; // tasks per chunk
+    __shared__ uint32_t lds[LdsWords<Gen>::value];
+    const FillArg &arg = batch.a[blockIdx.y];
+    Gen g;
+    g.init(arg.s);
+    g.prepare(lds);
+    T *o = out + (uint64_t)blockIdx.y * n;
+    const uint32_t skip = (uint32_t)arg.skip;
+    const uint64_t ntask = nchunks * G;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+         t < ntask; t += stride) {
+        const uint64_t c = t / G;
+        const uint32_t gi = (uint32_t)(t % G);
+        const uint64_t bs = c * (K / W) + gi;  // block of first s-word
+        const uint64_t bt = bs + H / W;        // block of first theta-word
+        uint32_t ws[W], wt[W];
+        if (skip == 0) {
+            g.block(bs, ws);
+            g.block(bt, wt);
+        } else {
+            uint32_t a[W], b[W];
+            g.block(bs, a);
+            g.block(bs + 1, b);
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                ws[j] = j + skip < (uint32_t)W ? a[j + skip] : b[j + skip - W];
+            g.block(bt, a);
+            g.block(bt + 1, b);
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                wt[j] = j + skip < (uint32_t)W ? a[j + skip] : b[j + skip - W];
+        }
+        T vs[W], vc[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+            box_muller(ws[j], wt[j], mean, sd, vs[j], vc[j]);
+        T *os = o + c * K + (uint64_t)gi * W;
+        if (arg.vec_ok) {
+            store_vec(os, vs);
+            store_vec(os + H, vc);
+        } else {
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                os[j] = vs[j];
+                os[H + j] = vc[j];
+            }
+        }
+    }
+}
+
+// Last partial chunk (L = n % K values, nu = L/2 pairs) and the odd tail.
+template <class Gen, typename T>
+__global__ __launch_bounds__(kBlock) void normal_tail_kernel(FillBatch batch,
+    uint64_t nchunks, uint64_t n, T mean, T sd, T *out)
+{
+    constexpr int K = NormalTraits<T>::K;
+    __shared__ uint32_t lds[LdsWords<Gen>::value];
+    const FillArg &arg = batch.a[blockIdx.y];
+    Gen g;
+    g.init(arg.s);
+    g.prepare(lds);
+    T *o = out + (uint64_t)blockIdx.y * n;
+    const uint64_t base = nchunks * K;
+    const uint64_t L = n - base;
+    const uint64_t nu = L / 2;
+    for (uint64_t p = threadIdx.x; p <= nu; p += blockDim.x) {
+        if (p < nu) {
+            const uint32_t ws = word_at(g, arg.skip, base + p);
+            const uint32_t wt = word_at(g, arg.skip, base + nu + p);
+            T a, b;
+            box_muller(ws, wt, mean, sd, a, b);
+            o[base + p] = a;
+            o[base + nu + p] = b;
+        } else if (L % 2 == 1) {
+            const uint32_t wu = word_at(g, arg.skip, base + L);
+            const uint32_t wv = word_at(g, arg.skip, base + L + 1);
+            o[base + L - 1] = box_muller_tail(wu, wv, mean, sd);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side: engine state machine (random/counter.hpp)
+// ---------------------------------------------------------------------------
+static int ctr_words(int kind) { return kind == THREEFRY ? 4 : 2; }
+static int block_w32(int kind) { return kind == THREEFRY ? 8 : 4; }
+
+static int check_engine(const mckl_hip_engine *e)
+{
+    if (!e)
+        return set_error(MCKL_HIP_EINVAL, "engine is NULL");
+    if (e->kind < PHILOX || e->kind > ARS)
+        return set_error(MCKL_HIP_EINVAL, "unknown engine kind %d", e->kind);
+    if (e->rbits != 32 && e->rbits != 64)
+        return set_error(MCKL_HIP_EINVAL, "rbits must be 32 or 64");
+    const uint32_t M = (uint32_t)(block_w32(e->kind) * 32 / e->rbits);
+    if (e->index > M)
+        return set_error(MCKL_HIP_EINVAL, "engine index %u > %u", e->index, M);
+    return MCKL_HIP_OK;
+}
+
+static uint32_t words_per_block(const mckl_hip_engine *e)
+{
+    return (uint32_t)(block_w32(e->kind) * 32 / e->rbits);
+}
+
+// ++ctr with ripple carry: random/internal/increment_generic.hpp:73-103
+static void ctr_inc(uint64_t *c, int k)
+{
+    for (int i = 0; i < k; ++i)
+        if (++c[i] != 0)
+            return;
+}
+
+// ctr += v with full carry (the effect of v single increments)
+static void ctr_add(uint64_t *c, int k, uint64_t v)
+{
+    const uint64_t old = c[0];
+    c[0] += v;
+    if (c[0] < old)
+        ctr_inc(c + 1, k - 1);
+}
+
+// increment(ctr, nskip): random/internal/increment_generic.hpp:208-217,
+// including its strict-less carry test (used only by discard).
+static void ctr_increment_skip(uint64_t *c, int k, uint64_t nskip)
+{
+    if (c[0] < UINT64_MAX - nskip) {
+        c[0] += nskip;
+    } else {
+        c[0] += nskip;
+        ctr_inc(c + 1, k - 1);
+    }
+}
+
+// State transition of one bulk call operator()(n, r): counter.hpp:142-165
+static void bulk_call(mckl_hip_engine *e, uint64_t n)
+{
+    const uint32_t M = words_per_block(e);
+    const uint64_t remain = M - e->index;
+    if (n <= remain) {
+        e->index += (uint32_t)n;
+        return;
+    }
+    n -= remain;
+    const uint64_t m = n / M;
+    ctr_add(e->ctr, ctr_words(e->kind), m);
+    ctr_inc(e->ctr, ctr_words(e->kind));
+    e->index = (uint32_t)(n - m * M);
+}
+
+// `count` bulk calls of k words each, k a multiple of M and k > M: after the
+// first call index is fixed and each further call adds k/M to the counter.
+static void bulk_calls(mckl_hip_engine *e, uint64_t k, uint64_t count)
+{
+    if (count == 0)
+        return;
+    bulk_call(e, k);
+    if (count > 1) {
+        const uint32_t M = words_per_block(e);
+        ctr_add(e->ctr, ctr_words(e->kind), (count - 1) * (k / M));
+    }
+}
+
+// scalar operator()(): counter.hpp:134-142
+static void scalar_call(mckl_hip_engine *e)
+{
+    const uint32_t M = words_per_block(e);
+    if (e->index == M) {
+        ctr_inc(e->ctr, ctr_words(e->kind));
+        e->index = 0;
+    }
+    e->index++;
+}
+
+// Position of the next word: counter of the block holding it, and its u32
+// offset within that block.
+static void next_position(const mckl_hip_engine *e, StreamArg &s, uint64_t &skip32)
+{
+    const uint32_t M = words_per_block(e);
+    std::memcpy(s.ctr, e->ctr, sizeof(s.ctr));
+    std::memcpy(s.key, e->key, sizeof(s.key));
+    if (e->index < M) {
+        skip32 = (uint64_t)e->index * (e->rbits / 32);
+    } else {
+        ctr_inc(s.ctr, ctr_words(e->kind));
+        skip32 = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+static int grid_cap(int kind) { return kind == ARS ? 1024 : 16384; }
+
+template <class Gen, class Conv>
+static int launch_fill(const std::vector<FillArg> &args, Conv conv,
+    uint64_t nblocks, uint64_t n, typename Conv::T *out, hipStream_t st)
+{
+    for (size_t b0 = 0; b0 < args.size(); b0 += kMaxBatch) {
+        FillBatch batch;
+        std::memset(&batch, 0, sizeof(batch));
+        const size_t cnt = std::min<size_t>(kMaxBatch, args.size() - b0);
+        for (size_t i = 0; i < cnt; ++i)
+            batch.a[i] = args[b0 + i];
+        const uint64_t want = ceil_div(nblocks, kBlock);
+        const uint64_t cap = (uint64_t)grid_cap(
+                                 LdsWords<Gen>::value > 1 ? ARS : PHILOX) /
+            std::max<size_t>(1, cnt);
+        const uint32_t gx = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(want, std::max<uint64_t>(1, cap)));
+        dim3 grid(gx, (uint32_t)cnt);
+        fill_kernel<Gen, Conv><<<grid, kBlock, 0, st>>>(
+            batch, conv, nblocks, n, out + b0 * n);
+        MCKL_HIP_CHECK_LAUNCH("fill_kernel");
+    }
+    return MCKL_HIP_OK;
+}
+
+template <class Gen>
+static int launch_straddle(const std::vector<FillArg> &args, uint64_t n,
+    double *out, hipStream_t st)
+{
+    for (size_t b0 = 0; b0 < args.size(); b0 += kMaxBatch) {
+        FillBatch batch;
+        std::memset(&batch, 0, sizeof(batch));
+        const size_t cnt = std::min<size_t>(kMaxBatch, args.size() - b0);
+        for (size_t i = 0; i < cnt; ++i)
+            batch.a[i] = args[b0 + i];
+        const uint32_t gx = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(ceil_div(n, kBlock), 4096));
+        fill_canonical_straddle_kernel<Gen>
+            <<<dim3(gx, (uint32_t)cnt), kBlock, 0, st>>>(batch, n, out + b0 * n);
+        MCKL_HIP_CHECK_LAUNCH("fill_canonical_straddle_kernel");
+    }
+    return MCKL_HIP_OK;
+}
+
+template <class Gen, typename T>
+static int launch_normal(const std::vector<FillArg> &args, uint64_t n, T mean,
+    T sd, T *out, hipStream_t st)
+{
+    constexpr int K = NormalTraits<T>::K;
+    const uint64_t nchunks = n / K;
+    const uint64_t ntask = nchunks * (K / 2 / Gen::W32);
+    for (size_t b0 = 0; b0 < args.size(); b0 += kMaxBatch) {
+        FillBatch batch;
+        std::memset(&batch, 0, sizeof(batch));
+        const size_t cnt = std::min<size_t>(kMaxBatch, args.size() - b0);
+        for (size_t i = 0; i < cnt; ++i)
+            batch.a[i] = args[b0 + i];
+        if (ntask) {
+            const uint64_t cap = (LdsWords<Gen>::value > 1 ? 1024 : 16384) /
+                std::max<size_t>(1, cnt);
+            const uint32_t gx = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(ceil_div(ntask, kBlock), cap));
+            normal_kernel<Gen, T><<<dim3(gx, (uint32_t)cnt), kBlock, 0, st>>>(
+                batch, nchunks, n, mean, sd, out + b0 * n);
+            MCKL_HIP_CHECK_LAUNCH("normal_kernel");
+        }
+        if (n % K != 0) {
+            normal_tail_kernel<Gen, T><<<dim3(1, (uint32_t)cnt), kBlock, 0, st>>>(
+                batch, nchunks, n, mean, sd, out + b0 * n);
+            MCKL_HIP_CHECK_LAUNCH("normal_tail_kernel");
+        }
+    }
+    return MCKL_HIP_OK;
+}
+
+// dispatch on engine kind
+template <template <class> class Op, typename... A>
+static int by_kind(int kind, A &&...a)
+{
+    switch (kind) {
+        case PHILOX:
+            return Op<PhiloxGen>::run(std::forward<A>(a)...);
+        case THREEFRY:
+            return Op<ThreefryGen>::run(std::forward<A>(a)...);
+        case ARS:
+            return Op<ArsGen>::run(std::forward<A>(a)...);
+    }
+    return set_error(MCKL_HIP_EINVAL, "unknown engine kind %d", kind);
+}
+
+template <class Gen>
+struct FillOp {
+    template <class Conv>
+    static int run(const std::vector<FillArg> &args, Conv conv,
+        uint64_t nblocks, uint64_t n, typename Conv::T *out, hipStream_t st)
+    {
+        return launch_fill<Gen, Conv>(args, conv, nblocks, n, out, st);
+    }
+};
+
+template <class Gen>
+struct StraddleOp {
+    static int run(const std::vector<FillArg> &args, uint64_t n, double *out,
+        hipStream_t st)
+    {
+        return launch_straddle<Gen>(args, n, out, st);
+    }
+};
+
+template <class Gen>
+struct NormalOp {
+    template <typename T>
+    static int run(const std::vector<FillArg> &args, uint64_t n, T mean, T sd,
+        T *out, hipStream_t st)
+    {
+        return launch_normal<Gen, T>(args, n, mean, sd, out, st);
+    }
+};
+
+static int common_checks(const mckl_hip_engine *e, size_t nstreams)
+{
+    for (size_t s = 0; s < nstreams; ++s) {
+        int rc = check_engine(e + s);
+        if (rc)
+            return rc;
+        if (e[s].kind != e[0].kind || e[s].rbits != e[0].rbits)
+            return set_error(MCKL_HIP_EINVAL,
+                "all engines of a multi-stream call must share kind and width");
+    }
+    return MCKL_HIP_OK;
+}
+
+// Element fill over nstreams engines with Q u32 words per element (Q in
+// {1,2}), advancing each engine by the reference's chunked call sequence
+// (chunk words `kwords`, 0 = a single bulk call of n*Q result words).
+template <class Conv>
+static int element_fill(mckl_hip_engine *e, size_t nstreams, uint64_t n,
+    typename Conv::T *out, Conv conv, uint64_t kwords, hipStream_t st)
+{
+    int rc = common_checks(e, nstreams);
+    if (rc)
+        return rc;
+    if (n == 0 || nstreams == 0)
+        return MCKL_HIP_OK;
+    if (!out)
+        return set_error(MCKL_HIP_EINVAL, "output pointer is NULL");
+    rc = ensure_device_tables();
+    if (rc)
+        return rc;
+    using T = typename Conv::T;
+    const int W32 = block_w32(e[0].kind);
+    std::vector<FillArg> args(nstreams);
+    uint64_t nblocks = 0;
+    bool straddle = false;
+    for (size_t s = 0; s < nstreams; ++s) {
+        uint64_t skip32 = 0;
+        next_position(e + s, args[s].s, skip32);
+        if (skip32 % Conv::Q != 0)
+            straddle = true;
+        args[s].skip = skip32 / Conv::Q;
+        const uintptr_t base = reinterpret_cast<uintptr_t>(out + s * n) -
+            args[s].skip * sizeof(T);
+        args[s].vec_ok = (base % 16 == 0) ? 1u : 0u;
+        args[s].pad = 0;
+        nblocks = std::max<uint64_t>(nblocks,
+            ceil_div(skip32 + n * Conv::Q, (uint64_t)W32));
+    }
+    if (straddle) {
+        // only Canonical<double> (Q = 2) at an odd word offset gets here
+        for (size_t s = 0; s < nstreams; ++s) {
+            uint64_t skip32 = 0;
+            next_position(e + s, args[s].s, skip32);
+            args[s].skip = skip32;
+        }
+        rc = by_kind<StraddleOp>(e[0].kind, args, n,
+            reinterpret_cast<double *>(out), st);
+    } else {
+        rc = by_kind<FillOp>(e[0].kind, args, conv, nblocks, n, out, st);
+    }
+    if (rc)
+        return rc;
+    const uint64_t rwords = n * Conv::Q * 32 / (uint64_t)e[0].rbits;
+    for (size_t s = 0; s < nstreams; ++s) {
+        if (kwords == 0) {
+            bulk_call(e + s, rwords);
+        } else {
+            bulk_calls(e + s, kwords, rwords / kwords);
+            bulk_call(e + s, rwords % kwords);
+        }
+    }
+    return MCKL_HIP_OK;
+}
+
+template <typename T>
+static int normal_fill(mckl_hip_engine *e, size_t nstreams, uint64_t n, T mean,
+    T sd, T *out, hipStream_t st)
+{
+    int rc = common_checks(e, nstreams);
+    if (rc)
+        return rc;
+    if (e[0].rbits != 32)
+        return set_error(MCKL_HIP_EINVAL,
+            "Normal fills are provided for 32-bit engines");
+    if (!(sd > 0))
+        return set_error(MCKL_HIP_EINVAL,
+            "NormalDistribution requires stddev > 0"); // normal_distribution.hpp:45-48
+    if (n == 0 || nstreams == 0)
+        return MCKL_HIP_OK;
+    if (!out)
+        return set_error(MCKL_HIP_EINVAL, "output pointer is NULL");
+    rc = ensure_device_tables();
+    if (rc)
+        return rc;
+    constexpr int K = NormalTraits<T>::K;
+    std::vector<FillArg> args(nstreams);
+    for (size_t s = 0; s < nstreams; ++s) {
+        uint64_t skip32 = 0;
+        next_position(e + s, args[s].s, skip32);
+        args[s].skip = skip32;
+        const uintptr_t base = reinterpret_cast<uintptr_t>(out + s * n);
+        args[s].vec_ok = (base % 16 == 0) ? 1u : 0u;
+        args[s].pad = 0;
+    }
+    rc = by_kind<NormalOp>(e[0].kind, args, n, mean, sd, out, st);
+    if (rc)
+        return rc;
+    for (size_t s = 0; s < nstreams; ++s) {
+        bulk_calls(e + s, K, n / K);
+        bulk_call(e + s, n % K);
+        if (n % 2 == 1) {
+            scalar_call(e + s);
+            scalar_call(e + s);
+        }
+    }
+    return MCKL_HIP_OK;
+}
+
+// raw generator kernel: n blocks for counters ctr+1..ctr+n
+static int raw_kernel(int kind, uint64_t *ctr, size_t n, void *d_r,
+    const uint64_t *key, void *stream)
+{
+    if (!ctr || !key)
+        return set_error(MCKL_HIP_EINVAL, "ctr/key is NULL");
+    if (n == 0)
+        return MCKL_HIP_OK;
+    if (!d_r)
+        return set_error(MCKL_HIP_EINVAL, "output pointer is NULL");
+    int rc = ensure_device_tables();
+    if (rc)
+        return rc;
+    std::vector<FillArg> args(1);
+    std::memset(args.data(), 0, sizeof(FillArg));
+    std::memcpy(args[0].s.ctr, ctr, sizeof(uint64_t) * ctr_words(kind));
+    std::memcpy(args[0].s.key, key, sizeof(uint64_t) * 4);
+    ctr_inc(args[0].s.ctr, ctr_words(kind));
+    args[0].skip = 0;
+    args[0].vec_ok = (reinterpret_cast<uintptr_t>(d_r) % 16 == 0) ? 1u : 0u;
+    const int W32 = block_w32(kind);
+    rc = by_kind<FillOp>(kind, args, ConvU32(), (uint64_t)n, (uint64_t)n * W32,
+        static_cast<uint32_t *>(d_r), as_stream(stream));
+    if (rc)
+        return rc;
+    ctr_add(ctr, ctr_words(kind), n);
+    return MCKL_HIP_OK;
+}
+
+// one block on the device, copied back (synchronous)
+__global__ void one_block_kernel(int kind, StreamArg a, uint32_t *out)
+{
+    __shared__ uint32_t lds[ARS_LDS_WORDS];
+    uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (kind == PHILOX) {
+        PhiloxGen g;
+        g.init(a);
+        if (threadIdx.x == 0)
+            g.block(0, w);
+    } else if (kind == THREEFRY) {
+        ThreefryGen g;
+        g.init(a);
+        if (threadIdx.x == 0)
+            g.block(0, w);
+    } else {
+        ArsGen g;
+        g.init(a);
+        g.prepare(lds);
+        if (threadIdx.x == 0)
+            g.block(0, w);
+    }
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 8; ++j)
+            out[j] = w[j];
+}
+
+static int block_on_device(int kind, const uint64_t *ctr, const uint64_t *key,
+    uint32_t *h_out)
+{
+    int rc = ensure_device_tables();
+    if (rc)
+        return rc;
+    StreamArg a;
+    std::memset(&a, 0, sizeof(a));
+    std::memcpy(a.ctr, ctr, sizeof(a.ctr));
+    std::memcpy(a.key, key, sizeof(a.key));
+    uint32_t *d = nullptr;
+    MCKL_HIP_TRY(hipMalloc(&d, 32));
+    one_block_kernel<<<1, kBlock>>>(kind, a, d);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess)
+        err = hipMemcpy(h_out, d, 32, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return hip_status(err, "one_block_kernel");
+}
+
+} // namespace mckl_hip
+
+using namespace mckl_hip;
+
+extern "C" {
+
+const char *mckl_hip_last_error(void) { return last_error_ref().c_str(); }
+
+int mckl_hip_engine_words(const mckl_hip_engine *e)
+{
+    int rc = check_engine(e);
+    return rc ? rc : (int)words_per_block(e);
+}
+
+int mckl_hip_engine_init(
+    mckl_hip_engine *e, int kind, int rbits, const uint64_t *key)
+{
+    if (!e || !key)
+        return set_error(MCKL_HIP_EINVAL, "engine/key is NULL");
+    std::memset(e, 0, sizeof(*e));
+    e->kind = kind;
+    e->rbits = rbits;
+    int rc = check_engine(e);
+    if (rc)
+        return rc;
+    // key words: Philox4x32 two u32, ARS four u32, Threefry four u64
+    const int nk = kind == PHILOX ? 2 : 4;
+    for (int i = 0; i < nk; ++i)
+        e->key[i] = kind == THREEFRY ? key[i] : (uint32_t)key[i];
+    e->index = words_per_block(e);
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_engine_seed(mckl_hip_engine *e, uint64_t s)
+{
+    int rc = check_engine(e);
+    if (rc)
+        return rc;
+    // seed(result_type s): counter.hpp:97-103
+    const uint64_t sv = e->rbits == 32 ? (uint32_t)s : s;
+    uint64_t key[4] = {sv, 0, 0, 0};
+    return mckl_hip_engine_init(e, e->kind, e->rbits, key);
+}
+
+int mckl_hip_engine_set_ctr(mckl_hip_engine *e, const uint64_t *ctr)
+{
+    int rc = check_engine(e);
+    if (rc)
+        return rc;
+    if (!ctr)
+        return set_error(MCKL_HIP_EINVAL, "ctr is NULL");
+    std::memset(e->ctr, 0, sizeof(e->ctr));
+    std::memcpy(e->ctr, ctr, sizeof(uint64_t) * ctr_words(e->kind));
+    e->index = words_per_block(e);
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_engine_discard(mckl_hip_engine *e, uint64_t nskip)
+{
+    int rc = check_engine(e);
+    if (rc)
+        return rc;
+    if (nskip == 0)
+        return MCKL_HIP_OK;
+    const uint32_t M = words_per_block(e);
+    const uint64_t remain = M - e->index;
+    if (nskip <= remain) {
+        e->index += (uint32_t)nskip;
+        return MCKL_HIP_OK;
+    }
+    nskip -= remain;
+    e->index = M;
+    ctr_increment_skip(e->ctr, ctr_words(e->kind), nskip / M);
+    ctr_inc(e->ctr, ctr_words(e->kind));
+    e->index = (uint32_t)(nskip % M);
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_engine_block(const mckl_hip_engine *e, void *h_block)
+{
+    int rc = check_engine(e);
+    if (rc)
+        return rc;
+    if (!h_block)
+        return set_error(MCKL_HIP_EINVAL, "output is NULL");
+    uint32_t w[8];
+    rc = block_on_device(e->kind, e->ctr, e->key, w);
+    if (rc)
+        return rc;
+    std::memcpy(h_block, w, (size_t)block_w32(e->kind) * 4);
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_engine_next(mckl_hip_engine *e, uint64_t *h_out)
+{
+    int rc = check_engine(e);
+    if (rc)
+        return rc;
+    if (!h_out)
+        return set_error(MCKL_HIP_EINVAL, "output is NULL");
+    scalar_call(e);
+    uint32_t w[8];
+    rc = block_on_device(e->kind, e->ctr, e->key, w);
+    if (rc)
+        return rc;
+    const uint32_t j = e->index - 1;
+    *h_out = e->rbits == 32
+        ? (uint64_t)w[j]
+        : ((uint64_t)w[2 * j] | ((uint64_t)w[2 * j + 1] << 32));
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_philox4x32_kernel(
+    uint64_t *ctr, size_t n, void *d_r, const uint32_t *key, void *stream)
+{
+    if (!key)
+        return set_error(MCKL_HIP_EINVAL, "key is NULL");
+    uint64_t k[4] = {key[0], key[1], 0, 0};
+    return raw_kernel(PHILOX, ctr, n, d_r, k, stream);
+}
+
+int mckl_hip_threefry4x64_kernel(
+    uint64_t *ctr, size_t n, void *d_r, const uint64_t *key, void *stream)
+{
+    return raw_kernel(THREEFRY, ctr, n, d_r, key, stream);
+}
+
+int mckl_hip_ars_kernel(
+    uint64_t *ctr, size_t n, void *d_r, const uint32_t *key, void *stream)
+{
+    if (!key)
+        return set_error(MCKL_HIP_EINVAL, "key is NULL");
+    uint64_t k[4] = {key[0], key[1], key[2], key[3]};
+    return raw_kernel(ARS, ctr, n, d_r, k, stream);
+}
+
+int mckl_hip_rand(mckl_hip_engine *e, size_t n, void *d_r, void *stream)
+{
+    int rc = check_engine(e);
+    if (rc)
+        return rc;
+    if (e->rbits == 32)
+        return element_fill(e, 1, n, static_cast<uint32_t *>(d_r), ConvU32(), 0,
+            as_stream(stream));
+    return element_fill(
+        e, 1, n, static_cast<uint64_t *>(d_r), ConvU64(), 0, as_stream(stream));
+}
+
+static int u01_f64_impl(mckl_hip_engine *e, size_t ns, int interval, size_t n,
+    double *d_r, void *stream)
+{
+    if (interval < 0 || interval > 4)
+        return set_error(MCKL_HIP_EINVAL, "unknown U01 interval %d", interval);
+    for (size_t s = 0; s < ns; ++s)
+        if (e && e[s].rbits != 32)
+            return set_error(
+                MCKL_HIP_EINVAL, "U01 fills are provided for 32-bit engines");
+    // BufferSize<double> = 1024 elements per bulk call (common.hpp:69-86)
+    if (interval == MCKL_HIP_U01_CANONICAL)
+        return element_fill(e, ns, n, d_r, ConvCanF64(), 2048, as_stream(stream));
+    ConvU01F64 c;
+    c.iv = interval;
+    return element_fill(e, ns, n, d_r, c, 1024, as_stream(stream));
+}
+
+int mckl_hip_rand_u01_f64(
+    mckl_hip_engine *e, int interval, size_t n, double *d_r, void *stream)
+{
+    return u01_f64_impl(e, 1, interval, n, d_r, stream);
+}
+
+int mckl_hip_rand_u01_f64_streams(mckl_hip_engine *e, size_t nstreams,
+    int interval, size_t n, double *d_r, void *stream)
+{
+    return u01_f64_impl(e, nstreams, interval, n, d_r, stream);
+}
+
+int mckl_hip_rand_u01_f32(
+    mckl_hip_engine *e, int interval, size_t n, float *d_r, void *stream)
+{
+    if (interval < 0 || interval > 4)
+        return set_error(MCKL_HIP_EINVAL, "unknown U01 interval %d", interval);
+    if (e && e->rbits != 32)
+        return set_error(
+            MCKL_HIP_EINVAL, "U01 fills are provided for 32-bit engines");
+    if (interval == MCKL_HIP_U01_CANONICAL)
+        return element_fill(e, 1, n, d_r, ConvCanF32(), 2048, as_stream(stream));
+    ConvU01F32 c;
+    c.iv = interval;
+    return element_fill(e, 1, n, d_r, c, 2048, as_stream(stream));
+}
+
+int mckl_hip_rand_normal_f32(mckl_hip_engine *e, size_t n, float mean,
+    float sd, float *d_r, void *stream)
+{
+    return normal_fill<float>(e, 1, n, mean, sd, d_r, as_stream(stream));
+}
+
+int mckl_hip_rand_normal_f64(mckl_hip_engine *e, size_t n, double mean,
+    double sd, double *d_r, void *stream)
+{
+    return normal_fill<double>(e, 1, n, mean, sd, d_r, as_stream(stream));
+}
+
+int mckl_hip_rand_normal_f32_streams(mckl_hip_engine *e, size_t nstreams,
+    size_t n, float mean, float sd, float *d_r, void *stream)
+{
+    return normal_fill<float>(e, nstreams, n, mean, sd, d_r, as_stream(stream));
+}
+
+int mckl_hip_rand_normal_f64_streams(mckl_hip_engine *e, size_t nstreams,
+    size_t n, double mean, double sd, double *d_r, void *stream)
+{
+    return normal_fill<double>(e, nstreams, n, mean, sd, d_r, as_stream(stream));
+}
+
+} // extern "C"

@@ -1,0 +1,991 @@
+/*
+ * mckl_oracle.c -- CPU restatement of MCKL's Monte Carlo hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity checker and the CPU baseline port).
+ * The product in mckl_amd/ never links this file.  Build flags are pinned in
+ * oracle/Makefile: -O2 -ffp-contract=off, no -march, so every float/double
+ * operation rounds once, exactly as the reference does when it is built the
+ * same way (no FMA contraction, SSE2 scalar math, glibc 2.35 libm).
+ *
+ * Reference paths are relative to the zhouyan/MCKL source tree.
+ */
+#include "mckl_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ===================================================================== */
+/* Philox4x32-10                                                          */
+/* random/internal/philox_generic_4x.hpp:40-79 (round),                   */
+/* random/internal/philox_constants.hpp:52-56,82-87 (constants),          */
+/* random/internal/philox_common.hpp:44-73 (32x32->64 hi/lo).             */
+/* The counter (array<uint64,2>) is read as four little-endian u32 words. */
+/* ===================================================================== */
+void mo_philox4x32_block(
+    const uint32_t key[2], const uint64_t ctr[2], uint32_t out[4])
+{
+    uint32_t s0 = (uint32_t)ctr[0];
+    uint32_t s1 = (uint32_t)(ctr[0] >> 32);
+    uint32_t s2 = (uint32_t)ctr[1];
+    uint32_t s3 = (uint32_t)(ctr[1] >> 32);
+    uint32_t k0 = key[0];
+    uint32_t k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p2 = (uint64_t)s2 * 0xCD9E8D57u;
+        uint64_t p0 = (uint64_t)s0 * 0xD2511F53u;
+        uint32_t hi2 = (uint32_t)(p2 >> 32), lo2 = (uint32_t)p2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t rk0 = k0 + 0x9E3779B9u * (uint32_t)r;
+        uint32_t rk1 = k1 + 0xBB67AE85u * (uint32_t)r;
+        uint32_t n0 = rk0 ^ s1 ^ hi2;
+        uint32_t n2 = rk1 ^ s3 ^ hi0;
+        s0 = n0;
+        s1 = lo2;
+        s2 = n2;
+        s3 = lo0;
+    }
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+    out[3] = s3;
+}
+
+/* ===================================================================== */
+/* Threefry4x64-20                                                        */
+/* random/threefry.hpp:109-121 (parity word, zero tweak),                 */
+/* random/internal/threefry_common.hpp:212-247 (key injection),           */
+/* random/internal/threefry_generic_4x64.hpp:40-108 (round schedule),     */
+/* random/internal/threefry_constants.hpp:83-89 (rotations).              */
+/* ===================================================================== */
+static const int mo_tf_r0[8] = {14, 52, 23, 5, 25, 46, 58, 32};
+static const int mo_tf_r1[8] = {16, 57, 40, 37, 33, 12, 22, 32};
+
+static uint64_t mo_rotl64(uint64_t x, int l)
+{
+    return (x << l) | (x >> (64 - l));
+}
+
+void mo_threefry4x64_block(
+    const uint64_t key[4], const uint64_t ctr[4], uint64_t out[4])
+{
+    uint64_t par[8];
+    par[0] = key[0];
+    par[1] = key[1];
+    par[2] = key[2];
+    par[3] = key[3];
+    par[4] = 0x1BD11BDAA9FC1A22ull ^ key[0] ^ key[1] ^ key[2] ^ key[3];
+    par[5] = 0; /* tweak t0 */
+    par[6] = 0; /* tweak t1 */
+    par[7] = 0; /* t0 ^ t1 */
+    uint64_t s[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    for (int inj = 0; inj <= 5; ++inj) {
+        /* injection number S = inj, applied before rounds 4*inj+1.. */
+        const unsigned S = (unsigned)inj;
+        s[0] += par[(S + 0) % 5];
+        s[1] += par[(S + 1) % 5] + par[5 + S % 3];
+        s[2] += par[(S + 2) % 5] + par[5 + (S + 1) % 3];
+        s[3] += par[(S + 3) % 5] + (uint64_t)S;
+        if (inj == 5)
+            break;
+        for (int q = 0; q < 4; ++q) {
+            const int r = inj * 4 + q; /* round N-1 */
+            const int l0 = mo_tf_r0[r % 8];
+            const int l1 = mo_tf_r1[r % 8];
+            s[0] += s[1];
+            s[2] += s[3];
+            uint64_t t1 = mo_rotl64(s[1], l0);
+            uint64_t t3 = mo_rotl64(s[3], l1);
+            s[1] = s[2] ^ t3;
+            s[3] = s[0] ^ t1;
+        }
+    }
+    out[0] = s[0];
+    out[1] = s[1];
+    out[2] = s[2];
+    out[3] = s[3];
+}
+
+/* ===================================================================== */
+/* AES round function and ARS-5                                           */
+/* random/internal/aes_generic.hpp:254-306 (ARS round keys, Weyl),        */
+/* random/internal/aes_generic.hpp:413-491 (T-table round, last round),   */
+/* random/internal/aes_constants.hpp:41 (T0 | T1 | T2 | T3 | S layout),   */
+/* random/internal/aes_constants.hpp:289-294 (Weyl constants).            */
+/* The tables are derived here from GF(2^8) arithmetic (FIPS-197 5.1.1).  */
+/* ===================================================================== */
+static uint8_t mo_gf_mul(uint8_t a, uint8_t b)
+{
+    uint8_t p = 0;
+    for (int i = 0; i < 8; ++i) {
+        if (b & 1)
+            p ^= a;
+        uint8_t hi = (uint8_t)(a & 0x80);
+        a = (uint8_t)(a << 1);
+        if (hi)
+            a ^= 0x1B;
+        b >>= 1;
+    }
+    return p;
+}
+
+void mo_aes_tables(uint32_t t0[256], uint8_t sbox[256])
+{
+    for (int x = 0; x < 256; ++x) {
+        uint8_t inv = 0;
+        if (x != 0) {
+            for (int y = 1; y < 256; ++y) {
+                if (mo_gf_mul((uint8_t)x, (uint8_t)y) == 1) {
+                    inv = (uint8_t)y;
+                    break;
+                }
+            }
+        }
+        uint8_t b = inv;
+        uint8_t s = (uint8_t)(b ^ ((b << 1) | (b >> 7)) ^ ((b << 2) | (b >> 6)) ^
+            ((b << 3) | (b >> 5)) ^ ((b << 4) | (b >> 4)) ^ 0x63);
+        sbox[x] = s;
+        uint32_t s2 = mo_gf_mul(s, 2), s3 = mo_gf_mul(s, 3);
+        t0[x] = s2 | ((uint32_t)s << 8) | ((uint32_t)s << 16) | (s3 << 24);
+    }
+}
+
+static uint32_t mo_t0[256];
+static uint8_t mo_sbox[256];
+static int mo_tables_ready = 0;
+
+static void mo_ensure_tables(void)
+{
+    if (!mo_tables_ready) {
+        mo_aes_tables(mo_t0, mo_sbox);
+        mo_tables_ready = 1;
+    }
+}
+
+static uint32_t mo_rotl32(uint32_t x, int l)
+{
+    return l == 0 ? x : (x << l) | (x >> (32 - l));
+}
+
+static void mo_aes_enc(uint32_t s[4], const uint32_t rk[4])
+{
+    uint32_t t[4];
+    for (int j = 0; j < 4; ++j) {
+        t[j] = rk[j] ^ mo_t0[s[j] & 0xFF] ^
+            mo_rotl32(mo_t0[(s[(j + 1) & 3] >> 8) & 0xFF], 8) ^
+            mo_rotl32(mo_t0[(s[(j + 2) & 3] >> 16) & 0xFF], 16) ^
+            mo_rotl32(mo_t0[(s[(j + 3) & 3] >> 24) & 0xFF], 24);
+    }
+    memcpy(s, t, sizeof(t));
+}
+
+static void mo_aes_enclast(uint32_t s[4], const uint32_t rk[4])
+{
+    uint32_t t[4];
+    for (int j = 0; j < 4; ++j) {
+        t[j] = rk[j] ^ (uint32_t)mo_sbox[s[j] & 0xFF] ^
+            ((uint32_t)mo_sbox[(s[(j + 1) & 3] >> 8) & 0xFF] << 8) ^
+            ((uint32_t)mo_sbox[(s[(j + 2) & 3] >> 16) & 0xFF] << 16) ^
+            ((uint32_t)mo_sbox[(s[(j + 3) & 3] >> 24) & 0xFF] << 24);
+    }
+    memcpy(s, t, sizeof(t));
+}
+
+void mo_ars_block(const uint32_t key[4], const uint64_t ctr[2], uint32_t out[4])
+{
+    mo_ensure_tables();
+    const uint64_t K0 = (uint64_t)key[0] | ((uint64_t)key[1] << 32);
+    const uint64_t K1 = (uint64_t)key[2] | ((uint64_t)key[3] << 32);
+    uint32_t rk[6][4];
+    for (int r = 0; r <= 5; ++r) {
+        uint64_t k0 = K0 + 0x9E3779B97F4A7C15ull * (uint64_t)r;
+        uint64_t k1 = K1 + 0xBB67AE8584CAA73Bull * (uint64_t)r;
+        rk[r][0] = (uint32_t)k0;
+        rk[r][1] = (uint32_t)(k0 >> 32);
+        rk[r][2] = (uint32_t)k1;
+        rk[r][3] = (uint32_t)(k1 >> 32);
+    }
+    uint32_t s[4] = {(uint32_t)ctr[0], (uint32_t)(ctr[0] >> 32),
+        (uint32_t)ctr[1], (uint32_t)(ctr[1] >> 32)};
+    for (int j = 0; j < 4; ++j)
+        s[j] ^= rk[0][j];
+    for (int r = 1; r < 5; ++r)
+        mo_aes_enc(s, rk[r]);
+    mo_aes_enclast(s, rk[5]);
+    memcpy(out, s, sizeof(s));
+}
+
+/* FIPS-197 AES-128 with the same round function, used only to pin the round
+ * against the FIPS-197 Appendix C vector that example/random/src/
+ * random_aes.cpp:49-86 checks.  Key expansion: FIPS-197 5.2. */
+void mo_aes128_encrypt(const uint8_t key[16], const uint8_t in[16], uint8_t out[16])
+{
+    mo_ensure_tables();
+    uint32_t w[44];
+    for (int i = 0; i < 4; ++i)
+        w[i] = (uint32_t)key[4 * i] | ((uint32_t)key[4 * i + 1] << 8) |
+            ((uint32_t)key[4 * i + 2] << 16) | ((uint32_t)key[4 * i + 3] << 24);
+    uint8_t rcon = 1;
+    for (int i = 4; i < 44; ++i) {
+        uint32_t t = w[i - 1];
+        if (i % 4 == 0) {
+            t = (t >> 8) | (t << 24); /* RotWord on little-endian words */
+            t = (uint32_t)mo_sbox[t & 0xFF] |
+                ((uint32_t)mo_sbox[(t >> 8) & 0xFF] << 8) |
+                ((uint32_t)mo_sbox[(t >> 16) & 0xFF] << 16) |
+                ((uint32_t)mo_sbox[(t >> 24) & 0xFF] << 24);
+            t ^= rcon;
+            rcon = mo_gf_mul(rcon, 2);
+        }
+        w[i] = w[i - 4] ^ t;
+    }
+    uint32_t s[4];
+    for (int i = 0; i < 4; ++i)
+        s[i] = ((uint32_t)in[4 * i] | ((uint32_t)in[4 * i + 1] << 8) |
+                   ((uint32_t)in[4 * i + 2] << 16) |
+                   ((uint32_t)in[4 * i + 3] << 24)) ^
+            w[i];
+    for (int r = 1; r < 10; ++r)
+        mo_aes_enc(s, w + 4 * r);
+    mo_aes_enclast(s, w + 40);
+    for (int i = 0; i < 4; ++i) {
+        out[4 * i] = (uint8_t)s[i];
+        out[4 * i + 1] = (uint8_t)(s[i] >> 8);
+        out[4 * i + 2] = (uint8_t)(s[i] >> 16);
+        out[4 * i + 3] = (uint8_t)(s[i] >> 24);
+    }
+}
+
+/* ===================================================================== */
+/* Counter engine                                                         */
+/* random/counter.hpp:85-114 (seed/key), :126-165 (scalar and bulk        */
+/* draws), :178-199 (discard), :282-287 (reset).                          */
+/* random/internal/increment_generic.hpp:73-103 (increment by one),       */
+/* :208-217 (increment by n, including its carry test).                   */
+/* ===================================================================== */
+static int mo_ctr_words(int kind)
+{
+    return kind == MO_THREEFRY4X64 ? 4 : 2;
+}
+
+static unsigned mo_block_bytes(int kind)
+{
+    return kind == MO_THREEFRY4X64 ? 32u : 16u;
+}
+
+unsigned mo_engine_words(const mo_engine *e)
+{
+    return mo_block_bytes(e->kind) / (unsigned)(e->rbits / 8);
+}
+
+static void mo_increment1(uint64_t *ctr, int k)
+{
+    for (int i = 0; i < k; ++i)
+        if (++ctr[i] != 0)
+            return;
+}
+
+void mo_increment(uint64_t *ctr, int k, uint64_t nskip)
+{
+    /* increment_generic.hpp:208-217: the carry test is a strict '<'. */
+    if (ctr[0] < UINT64_MAX - nskip) {
+        ctr[0] += nskip;
+    } else {
+        ctr[0] += nskip;
+        mo_increment1(ctr + 1, k - 1);
+    }
+}
+
+static void mo_block(const mo_engine *e, const uint64_t *ctr, uint8_t *out)
+{
+    if (e->kind == MO_PHILOX4X32) {
+        uint32_t key[2] = {(uint32_t)e->key[0], (uint32_t)e->key[1]};
+        uint32_t r[4];
+        mo_philox4x32_block(key, ctr, r);
+        memcpy(out, r, 16);
+    } else if (e->kind == MO_THREEFRY4X64) {
+        uint64_t r[4];
+        mo_threefry4x64_block(e->key, ctr, r);
+        memcpy(out, r, 32);
+    } else {
+        uint32_t key[4] = {(uint32_t)e->key[0], (uint32_t)e->key[1],
+            (uint32_t)e->key[2], (uint32_t)e->key[3]};
+        uint32_t r[4];
+        mo_ars_block(key, ctr, r);
+        memcpy(out, r, 16);
+    }
+}
+
+/* generator_(ctr_, r): increment then encrypt, counter.hpp:126-134 */
+static void mo_gen_next(mo_engine *e, uint8_t *out)
+{
+    mo_increment1(e->ctr, mo_ctr_words(e->kind));
+    mo_block(e, e->ctr, out);
+}
+
+void mo_engine_init(mo_engine *e, int kind, int rbits, const uint64_t *key)
+{
+    memset(e, 0, sizeof(*e));
+    e->kind = kind;
+    e->rbits = rbits;
+    int nk = kind == MO_PHILOX4X32 ? 2 : 4;
+    for (int i = 0; i < nk; ++i)
+        e->key[i] = key[i];
+    e->index = mo_engine_words(e);
+}
+
+void mo_engine_seed(mo_engine *e, uint64_t s)
+{
+    /* counter.hpp:97-103: key = {s, 0, ...} truncated to the key word type;
+     * the argument itself is result_type. */
+    uint64_t key[4] = {0, 0, 0, 0};
+    uint64_t sv = e->rbits == 32 ? (uint32_t)s : s;
+    key[0] = e->kind == MO_THREEFRY4X64 ? sv : (uint32_t)sv;
+    mo_engine_init(e, e->kind, e->rbits, key);
+}
+
+void mo_engine_set_ctr(mo_engine *e, const uint64_t *ctr)
+{
+    for (int i = 0; i < mo_ctr_words(e->kind); ++i)
+        e->ctr[i] = ctr[i];
+    e->index = mo_engine_words(e);
+}
+
+static void mo_rand_bytes(mo_engine *e, size_t n, uint8_t *r)
+{
+    const unsigned rs = (unsigned)(e->rbits / 8);
+    const unsigned M = mo_engine_words(e);
+    const size_t remain = (size_t)(M - e->index);
+    if (n <= remain) {
+        memcpy(r, e->buf + (size_t)e->index * rs, n * rs);
+        e->index += (unsigned)n;
+        return;
+    }
+    memcpy(r, e->buf + (size_t)e->index * rs, remain * rs);
+    r += remain * rs;
+    n -= remain;
+    e->index = M;
+    const size_t m = n / M;
+    for (size_t i = 0; i < m; ++i, r += (size_t)M * rs)
+        mo_gen_next(e, r);
+    n -= m * M;
+    mo_gen_next(e, e->buf);
+    memcpy(r, e->buf, n * rs);
+    e->index = (unsigned)n;
+}
+
+void mo_rand_u32(mo_engine *e, size_t n, uint32_t *r)
+{
+    mo_rand_bytes(e, n, (uint8_t *)r);
+}
+
+void mo_rand_u64(mo_engine *e, size_t n, uint64_t *r)
+{
+    mo_rand_bytes(e, n, (uint8_t *)r);
+}
+
+uint32_t mo_next_u32(mo_engine *e)
+{
+    /* counter.hpp:134-142 */
+    const unsigned M = mo_engine_words(e);
+    if (e->index == M) {
+        mo_gen_next(e, e->buf);
+        e->index = 0;
+    }
+    uint32_t v;
+    memcpy(&v, e->buf + (size_t)e->index * 4, 4);
+    e->index++;
+    return v;
+}
+
+void mo_discard(mo_engine *e, uint64_t nskip)
+{
+    /* counter.hpp:178-199; rate = sizeof(result block)/sizeof(ctr) == 1 */
+    if (nskip == 0)
+        return;
+    const unsigned M = mo_engine_words(e);
+    const uint64_t remain = M - e->index;
+    if (nskip <= remain) {
+        e->index += (unsigned)nskip;
+        return;
+    }
+    nskip -= remain;
+    e->index = M;
+    mo_increment(e->ctr, mo_ctr_words(e->kind), nskip / M);
+    mo_gen_next(e, e->buf);
+    e->index = (unsigned)(nskip % M);
+}
+
+/* ===================================================================== */
+/* U01 conversions                                                        */
+/* random/internal/u01_generic.hpp:48-164 (CC/CO/OC/OO fixed point),      */
+/* :166-198 (Canonical); random/u01_distribution.hpp:95-113 (one u32 per  */
+/* value for 32-bit engines), :115-129 (Canonical uses two u32/double).   */
+/* ===================================================================== */
+double mo_u01_u32_f64(int interval, uint32_t u)
+{
+    const double p32 = 0x1p-32, p33 = 0x1p-33;
+    switch (interval) {
+        case MO_CC:
+            return ((double)(u & 1) + (double)u) * p32;
+        case MO_CO:
+            return (double)u * p32;
+        case MO_OC:
+            return (double)u * p32 + p32;
+        default:
+            return (double)u * p32 + p33;
+    }
+}
+
+float mo_u01_u32_f32(int interval, uint32_t u)
+{
+    switch (interval) {
+        case MO_CC: {
+            uint32_t v = (uint32_t)(u << 1) >> 7;
+            return (float)((v & 1) + v) * 0x1p-25f;
+        }
+        case MO_CO:
+            return (float)(u >> 8) * 0x1p-24f;
+        case MO_OC:
+            return (float)(u >> 8) * 0x1p-24f + 0x1p-24f;
+        default:
+            return (float)(u >> 9) * 0x1p-23f + 0x1p-24f;
+    }
+}
+
+double mo_u01_canonical_f64(uint32_t u0, uint32_t u1)
+{
+    return (double)u0 * 0x1p-64 + ((double)u1 * 0x1p-32 + 0.0);
+}
+
+float mo_u01_canonical_f32(uint32_t u)
+{
+    return (float)u * 0x1p-32f + 0.0f;
+}
+
+/* random/internal/common.hpp:69-86 (BATCH_0 chunking by BufferSize),
+ * internal/defines.hpp:62-67 (BufferSize = 8192 / sizeof(T)). */
+void mo_rand_u01_f64(mo_engine *e, int interval, size_t n, double *r)
+{
+    const size_t K = 1024;
+    uint32_t s[2 * 1024];
+    const size_t Q = interval == MO_CANONICAL ? 2 : 1;
+    for (;;) {
+        size_t k = n < K ? n : K;
+        mo_rand_u32(e, k * Q, s);
+        for (size_t i = 0; i < k; ++i)
+            r[i] = Q == 2 ? mo_u01_canonical_f64(s[2 * i], s[2 * i + 1])
+                          : mo_u01_u32_f64(interval, s[i]);
+        r += k;
+        n -= k;
+        if (k < K)
+            break;
+    }
+}
+
+void mo_rand_u01_f32(mo_engine *e, int interval, size_t n, float *r)
+{
+    const size_t K = 2048;
+    uint32_t s[2048];
+    for (;;) {
+        size_t k = n < K ? n : K;
+        mo_rand_u32(e, k, s);
+        for (size_t i = 0; i < k; ++i)
+            r[i] = interval == MO_CANONICAL ? mo_u01_canonical_f32(s[i])
+                                            : mo_u01_u32_f32(interval, s[i]);
+        r += k;
+        n -= k;
+        if (k < K)
+            break;
+    }
+}
+
+/* ===================================================================== */
+/* Normal: chunked Box-Muller                                             */
+/* random/normal_distribution.hpp:50-78 (chunk: first half s*sin, second  */
+/* half s*cos), :82-101 (chunk loop and odd tail with two scalar U01OC    */
+/* draws); math/vmf.hpp:1052-1077 (sincos), :670-722 (muladd = a*b+c      */
+/* without FMA); math/constants.hpp:39-51,111 (2*pi constant).            */
+/* ===================================================================== */
+static void mo_normal_chunk_f32(
+    mo_engine *e, size_t n, float *r, float mean, float sd)
+{
+    float s[1024];
+    const size_t nu = n / 2;
+    float *u1 = r, *u2 = r + nu;
+    mo_rand_u01_f32(e, MO_OC, n, r);
+    for (size_t i = 0; i < nu; ++i)
+        s[i] = logf(u1[i]);
+    for (size_t i = 0; i < nu; ++i)
+        s[i] = s[i] * -2.0f;
+    for (size_t i = 0; i < nu; ++i)
+        s[i] = sqrtf(s[i]);
+    for (size_t i = 0; i < nu; ++i)
+        u2[i] = 6.283185307179586476925286766559005768394f * u2[i];
+    /* separate sin and cos passes, as vmf.hpp:1071-1076 does (a fused
+     * sincos call may round differently) */
+    for (size_t i = 0; i < nu; ++i)
+        u1[i] = sinf(u2[i]);
+    for (size_t i = 0; i < nu; ++i)
+        u2[i] = cosf(u2[i]);
+    if (sd != 1.0f)
+        for (size_t i = 0; i < nu; ++i)
+            s[i] = sd * s[i];
+    if (mean != 0.0f) {
+        for (size_t i = 0; i < nu; ++i)
+            u1[i] = s[i] * u1[i] + mean;
+        for (size_t i = 0; i < nu; ++i)
+            u2[i] = s[i] * u2[i] + mean;
+    } else {
+        for (size_t i = 0; i < nu; ++i)
+            u1[i] = s[i] * u1[i];
+        for (size_t i = 0; i < nu; ++i)
+            u2[i] = s[i] * u2[i];
+    }
+}
+
+void mo_rand_normal_f32(mo_engine *e, size_t n, float mean, float sd, float *r)
+{
+    const size_t k = 2048;
+    const size_t m = n / k, l = n % k;
+    for (size_t i = 0; i < m; ++i, r += k)
+        mo_normal_chunk_f32(e, k, r, mean, sd);
+    mo_normal_chunk_f32(e, l, r, mean, sd);
+    if (n % 2 != 0) {
+        float u = mo_u01_u32_f32(MO_OC, mo_next_u32(e));
+        float v = mo_u01_u32_f32(MO_OC, mo_next_u32(e));
+        r[l - 1] = mean +
+            sd * sqrtf(-2 * logf(u)) *
+                cosf(6.283185307179586476925286766559005768394f * v);
+    }
+}
+
+static void mo_normal_chunk_f64(
+    mo_engine *e, size_t n, double *r, double mean, double sd)
+{
+    double s[512];
+    const size_t nu = n / 2;
+    double *u1 = r, *u2 = r + nu;
+    mo_rand_u01_f64(e, MO_OC, n, r);
+    for (size_t i = 0; i < nu; ++i)
+        s[i] = log(u1[i]);
+    for (size_t i = 0; i < nu; ++i)
+        s[i] = s[i] * -2.0;
+    for (size_t i = 0; i < nu; ++i)
+        s[i] = sqrt(s[i]);
+    for (size_t i = 0; i < nu; ++i)
+        u2[i] = 6.283185307179586 * u2[i];
+    for (size_t i = 0; i < nu; ++i)
+        u1[i] = sin(u2[i]);
+    for (size_t i = 0; i < nu; ++i)
+        u2[i] = cos(u2[i]);
+    if (sd != 1.0)
+        for (size_t i = 0; i < nu; ++i)
+            s[i] = sd * s[i];
+    if (mean != 0.0) {
+        for (size_t i = 0; i < nu; ++i)
+            u1[i] = s[i] * u1[i] + mean;
+        for (size_t i = 0; i < nu; ++i)
+            u2[i] = s[i] * u2[i] + mean;
+    } else {
+        for (size_t i = 0; i < nu; ++i)
+            u1[i] = s[i] * u1[i];
+        for (size_t i = 0; i < nu; ++i)
+            u2[i] = s[i] * u2[i];
+    }
+}
+
+void mo_rand_normal_f64(
+    mo_engine *e, size_t n, double mean, double sd, double *r)
+{
+    const size_t k = 1024;
+    const size_t m = n / k, l = n % k;
+    for (size_t i = 0; i < m; ++i, r += k)
+        mo_normal_chunk_f64(e, k, r, mean, sd);
+    mo_normal_chunk_f64(e, l, r, mean, sd);
+    if (n % 2 != 0) {
+        double u = mo_u01_u32_f64(MO_OC, mo_next_u32(e));
+        double v = mo_u01_u32_f64(MO_OC, mo_next_u32(e));
+        r[l - 1] = mean +
+            sd * sqrt(-2 * log(u)) * cos(6.283185307179586 * v);
+    }
+}
+
+/* ===================================================================== */
+/* Weight                                                                 */
+/* core/weight.hpp:124-148 (set_log/add_log), :178-227 (max element,      */
+/* chunked exp, sequential accumulate, inline ddot, scale by 1/accw,      */
+/* ESS = accw^2/essw); internal/cblas.hpp:252-260 (ddot loop).            */
+/* The log normaliser increment lmax + log(accw) is computed by the       */
+/* reference and discarded (weight.hpp:198-227); it is returned here.     */
+/* ===================================================================== */
+static void mo_weight_normalize(size_t n, double *w, double *ess, double *lognc)
+{
+    double lmax = -INFINITY;
+    for (size_t i = 0; i < n; ++i)
+        if (lmax < w[i])
+            lmax = w[i];
+    double accw = 0, essw = 0;
+    const size_t k = 1024;
+    for (size_t c = 0; c < n; c += k) {
+        size_t len = n - c < k ? n - c : k;
+        double *p = w + c;
+        for (size_t i = 0; i < len; ++i)
+            p[i] = p[i] - lmax;
+        for (size_t i = 0; i < len; ++i)
+            p[i] = exp(p[i]);
+        for (size_t i = 0; i < len; ++i)
+            accw = accw + p[i];
+        double d = 0;
+        for (size_t i = 0; i < len; ++i)
+            d += p[i] * p[i];
+        essw += d;
+    }
+    const double inv = 1 / accw;
+    for (size_t i = 0; i < n; ++i)
+        w[i] = inv * w[i];
+    *ess = accw * accw / essw;
+    if (lognc)
+        *lognc = lmax + log(accw);
+}
+
+void mo_weight_set_log(size_t n, const double *v, double *w, double *ess)
+{
+    memcpy(w, v, n * sizeof(double));
+    mo_weight_normalize(n, w, ess, NULL);
+}
+
+void mo_weight_add_log(size_t n, const double *v, double *w, double *ess,
+    double *lognc_inc)
+{
+    for (size_t i = 0; i < n; ++i)
+        w[i] = log(w[i]);
+    for (size_t i = 0; i < n; ++i)
+        w[i] = v[i] + w[i];
+    mo_weight_normalize(n, w, ess, lognc_inc);
+}
+
+/* ===================================================================== */
+/* Resampling                                                             */
+/* algorithm/resample.hpp:63-81 (sorted transform), :95-120 (stratified), */
+/* :122-135 (systematic), :216-268 (rand forms: Canonical uniforms, one    */
+/* per value, chunked by BufferSize<double> = 1024), :374-407 (u01 ->     */
+/* replication), :416-466 (replication -> parent index), :515-576         */
+/* (ResampleAlgorithm composition).                                       */
+/* ===================================================================== */
+void mo_u01_rand_sorted(mo_engine *e, size_t N, double *r)
+{
+    if (N == 0)
+        return;
+    const size_t k = 1024;
+    double lmax = 0;
+    double s[1024];
+    for (size_t n0 = 0; n0 < N; n0 += k, r += k) {
+        size_t n = N - n0 < k ? N - n0 : k;
+        mo_rand_u01_f64(e, MO_CANONICAL, n, r);
+        for (size_t j = 0; j < n; ++j)
+            r[j] = log(r[j]);
+        size_t m = N - n0;
+        for (size_t j = 0; j < n; ++j, --m) {
+            lmax += r[j] / (double)m;
+            s[j] = lmax;
+        }
+        for (size_t j = 0; j < n; ++j)
+            s[j] = exp(s[j]);
+        for (size_t j = 0; j < n; ++j)
+            r[j] = 1.0 - s[j];
+    }
+}
+
+void mo_u01_rand_stratified(mo_engine *e, size_t N, double *r)
+{
+    const size_t k = 1024;
+    const double delta = 1 / (double)N;
+    for (size_t n0 = 0; n0 < N; n0 += k, r += k) {
+        size_t n = N - n0 < k ? N - n0 : k;
+        mo_rand_u01_f64(e, MO_CANONICAL, n, r);
+        for (size_t j = 0; j < n; ++j)
+            r[j] = r[j] + (double)(n0 + j);
+        for (size_t j = 0; j < n; ++j)
+            r[j] = delta * r[j];
+    }
+}
+
+void mo_u01_rand_systematic(mo_engine *e, size_t N, double *r)
+{
+    uint32_t a = mo_next_u32(e);
+    uint32_t b = mo_next_u32(e);
+    double u01 = mo_u01_canonical_f64(a, b);
+    if (N == 0)
+        return;
+    const double delta = 1 / (double)N;
+    const double u = u01 * delta;
+    for (size_t i = 0; i < N; ++i)
+        r[i] = (double)i * delta + u;
+}
+
+void mo_resample_trans_u01_rep(
+    size_t N, size_t M, const double *w, const double *u01, uint64_t *rep)
+{
+    if (N == 0)
+        return;
+    if (N == 1) {
+        rep[0] = M;
+        return;
+    }
+    memset(rep, 0, N * sizeof(uint64_t));
+    if (M == 0)
+        return;
+    double accw = 0;
+    size_t j = 0;
+    for (size_t i = 0; i != N - 1; ++i) {
+        accw += w[i];
+        while (j != M && u01[j] < accw) {
+            rep[i] += 1;
+            ++j;
+        }
+    }
+    rep[N - 1] = M - j;
+}
+
+void mo_resample_trans_rep_index(
+    size_t N, size_t M, const uint64_t *rep, uint64_t *idx)
+{
+    if (N == 0 || M == 0)
+        return;
+    const size_t K = N < M ? N : M;
+    uint64_t time = 0;
+    size_t src = 0;
+    for (size_t dst = 0; dst < M; ++dst) {
+        if (dst < K && rep[dst] > 0) {
+            idx[dst] = dst;
+            continue;
+        }
+        /* seek, resample.hpp:432-446 */
+        if (src < K && rep[src] < time + 2) {
+            time = 0;
+            do {
+                ++src;
+            } while (src < K && rep[src] < time + 2);
+        }
+        if (src >= K && rep[src] < time + 1) {
+            time = 0;
+            do {
+                ++src;
+            } while (src < N && rep[src] < time + 1);
+        }
+        idx[dst] = src;
+        ++time;
+    }
+}
+
+void mo_resample(int scheme, size_t N, size_t M, mo_engine *e,
+    const double *w, uint64_t *rep, uint64_t *idx)
+{
+    double *u = (double *)malloc((M ? M : 1) * sizeof(double));
+    if (scheme == MO_MULTINOMIAL)
+        mo_u01_rand_sorted(e, M, u);
+    else if (scheme == MO_STRATIFIED)
+        mo_u01_rand_stratified(e, M, u);
+    else
+        mo_u01_rand_systematic(e, M, u);
+    mo_resample_trans_u01_rep(N, M, w, u, rep);
+    if (idx)
+        mo_resample_trans_rep_index(N, M, rep, idx);
+    free(u);
+}
+
+/* ===================================================================== */
+/* This project's SMC particle filter, restated on the CPU.               */
+/*                                                                        */
+/* Loop shape follows SMCSampler::do_iterate (algorithm/smc.hpp:389-406): */
+/* move+weight, ESS test against threshold*N (:397), resample, with the   */
+/* ResampleAlgorithm and rep->index transform above.  The random-number   */
+/* layout, the FP32 state/log-weight arithmetic and the log-likelihood    */
+/* accounting are this project's own design (the reference has no         */
+/* log-likelihood API, core/weight.hpp:198-227); the device kernels       */
+/* implement the same definitions:                                        */
+/*   eps(i,t)  = Box-Muller cos branch of Philox4x32 block at counter     */
+/*               {i, t} under key {seed lo, seed hi}; words 0,1 -> U01OC. */
+/*   resampling uniforms at step t: Philox4x32 blocks at counter          */
+/*               {2^63 + b, t}; Canonical doubles from word pairs.        */
+/*   every transcendental is evaluated in double and rounded once.        */
+/* ===================================================================== */
+static float mo_log_f(float x) { return (float)log((double)x); }
+static float mo_cos_f(float x) { return (float)cos((double)x); }
+static float mo_exp_f(float x) { return (float)exp((double)x); }
+
+float mo_smc_normal(uint64_t seed, uint64_t i, uint64_t t)
+{
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint64_t ctr[2] = {i, t};
+    uint32_t w[4];
+    mo_philox4x32_block(key, ctr, w);
+    float u1 = mo_u01_u32_f32(MO_OC, w[0]);
+    float u2 = mo_u01_u32_f32(MO_OC, w[1]);
+    float l = mo_log_f(u1);
+    float s = sqrtf(l * -2.0f);
+    float th = 6.283185307179586476925286766559005768394f * u2;
+    return s * mo_cos_f(th);
+}
+
+static double mo_smc_uniform(uint64_t seed, uint64_t t, uint64_t j)
+{
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint64_t ctr[2] = {0x8000000000000000ull + j / 2, t};
+    uint32_t w[4];
+    mo_philox4x32_block(key, ctr, w);
+    return (j & 1) ? mo_u01_canonical_f64(w[2], w[3])
+                   : mo_u01_canonical_f64(w[0], w[1]);
+}
+
+typedef struct {
+    float a, s;        /* transition x' = a x + s eps */
+    float sd0;         /* initial x0 = sd0 eps */
+    float inv_sy, cst; /* LG: observation scale; constant of log g */
+    double beta;       /* SV */
+} mo_model;
+
+static void mo_model_init(int model, const double *th, mo_model *m)
+{
+    const double l2pi = log(2.0 * 3.141592653589793);
+    if (model == MO_MODEL_LG) {
+        m->a = (float)th[0];
+        m->s = (float)th[1];
+        m->sd0 = 1.0f;
+        m->inv_sy = (float)(1.0 / th[2]);
+        m->cst = (float)(-log(th[2]) - 0.5 * l2pi);
+        m->beta = 0;
+    } else {
+        m->a = (float)th[0];
+        m->s = (float)th[1];
+        m->sd0 = (float)sqrt(th[1] * th[1] / (1.0 - th[0] * th[0]));
+        m->inv_sy = 0;
+        m->cst = (float)(-log(th[2]) - 0.5 * l2pi);
+        m->beta = th[2];
+    }
+}
+
+static float mo_model_logg(int model, const mo_model *m, double y, float ysq,
+    float yf, float x)
+{
+    if (model == MO_MODEL_LG) {
+        float d = yf - x;
+        float z = d * m->inv_sy;
+        return -0.5f * (z * z) + m->cst;
+    }
+    float e = mo_exp_f(-x);
+    float q = ysq * e;
+    (void)y;
+    return ((-0.5f * q) - (0.5f * x)) + m->cst;
+}
+
+double mo_smc_run(int model, const double *theta, const double *y, size_t T,
+    size_t N, uint64_t seed, double threshold, int scheme, int nthreads,
+    double *ess_hist, int *resampled_hist)
+{
+    mo_model m;
+    mo_model_init(model, theta, &m);
+    float *x = (float *)malloc(N * sizeof(float));
+    float *xn = (float *)malloc(N * sizeof(float));
+    float *lw = (float *)malloc(N * sizeof(float));
+    double *W = (double *)malloc(N * sizeof(double));
+    double *u = (double *)malloc(N * sizeof(double));
+    uint64_t *rep = (uint64_t *)malloc(N * sizeof(uint64_t));
+    uint64_t *idx = (uint64_t *)malloc(N * sizeof(uint64_t));
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    long long Ns = (long long)N;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < Ns; ++i) {
+        x[i] = m.sd0 * mo_smc_normal(seed, (uint64_t)i, 0);
+        lw[i] = 0.0f;
+    }
+    float mprev = 0.0f;
+    int resampled = 0;
+    double logz = 0;
+    for (size_t t = 1; t <= T; ++t) {
+        const double yt = y[t - 1];
+        const float yf = (float)yt;
+        const float ysq = model == MO_MODEL_SV
+            ? (float)(yt * yt / (m.beta * m.beta))
+            : 0.0f;
+        double C = 0;
+#pragma omp parallel for schedule(static) reduction(+ : C)
+        for (long long i = 0; i < Ns; ++i) {
+            float xp, carry;
+            if (resampled) {
+                xp = x[idx[i]];
+                carry = 0.0f;
+            } else {
+                xp = x[i];
+                carry = lw[i] - mprev;
+            }
+            float eps = mo_smc_normal(seed, (uint64_t)i, t);
+            float xv = m.a * xp + m.s * eps;
+            xn[i] = xv;
+            lw[i] = carry + mo_model_logg(model, &m, yt, ysq, yf, xv);
+            C += exp((double)carry);
+        }
+        float *tmp = x;
+        x = xn;
+        xn = tmp;
+        float mx = -INFINITY;
+        for (size_t i = 0; i < N; ++i)
+            if (mx < lw[i])
+                mx = lw[i];
+        double S = 0, S2 = 0;
+        for (size_t i = 0; i < N; ++i) {
+            double e = exp((double)lw[i] - (double)mx);
+            W[i] = e;
+            S += e;
+            S2 += e * e;
+        }
+        logz += ((double)mx + log(S)) - log(C);
+        const double ess = S * S / S2;
+        if (ess_hist)
+            ess_hist[t - 1] = ess;
+        mprev = mx;
+        resampled = ess < threshold * (double)N;
+        if (resampled_hist)
+            resampled_hist[t - 1] = resampled;
+        if (resampled) {
+            const double inv = 1 / S;
+            for (size_t i = 0; i < N; ++i)
+                W[i] = inv * W[i];
+            const double delta = 1 / (double)N;
+            if (scheme == MO_SYSTEMATIC) {
+                const double uu = mo_smc_uniform(seed, t, 0) * delta;
+                for (size_t j = 0; j < N; ++j)
+                    u[j] = (double)j * delta + uu;
+            } else if (scheme == MO_STRATIFIED) {
+                for (size_t j = 0; j < N; ++j)
+                    u[j] = delta * (mo_smc_uniform(seed, t, j) + (double)j);
+            } else {
+                double lm = 0;
+                for (size_t j = 0; j < N; ++j) {
+                    lm += log(mo_smc_uniform(seed, t, j)) / (double)(N - j);
+                    u[j] = 1.0 - exp(lm);
+                }
+            }
+            mo_resample_trans_u01_rep(N, N, W, u, rep);
+            mo_resample_trans_rep_index(N, N, rep, idx);
+        }
+    }
+    free(x);
+    free(xn);
+    free(lw);
+    free(W);
+    free(u);
+    free(rep);
+    free(idx);
+    return logz;
+}

@@ -1,0 +1,118 @@
+/*
+ * mckl_oracle.h -- CPU restatement of MCKL's Monte Carlo hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * implementation in mckl_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
+ *
+ * Every function cites the reference file:line whose behaviour it restates.
+ * Paths are relative to the zhouyan/MCKL source tree.
+ *
+ * Parity pinning (what checks each part, see tests/test_oracle_*.py):
+ *   engines        reference KAT files random_{philox4x32,threefry4x64,ars}
+ *                  [_64].data; FIPS-197 Appendix C for the AES round; the
+ *                  compiled reference (oracle/_ref) for bulk/partial/discard.
+ *   u01            closed forms (exact), edge words 0 and 2^32-1, oracle/_ref.
+ *   normal         oracle/_ref (bit-exact, glibc libm on both sides);
+ *                  moments against theory.
+ *   weight         oracle/_ref (bit-exact); closed form on tiny inputs.
+ *   resampling     oracle/_ref (bit-exact); reference invariants; brute force
+ *                  on all small replication vectors.
+ *   smc / kalman   the SMC restatement is this project's own device algorithm
+ *                  (no reference counterpart); pinned by the Kalman filter
+ *                  (closed form) within Monte Carlo error, and the Kalman
+ *                  filter is pinned by brute-force Gaussian integration.
+ */
+#ifndef MCKL_ORACLE_H
+#define MCKL_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MO_PHILOX4X32 = 1,
+    MO_THREEFRY4X64 = 2,
+    MO_ARS = 3
+};
+
+enum { MO_CC = 0, MO_CO = 1, MO_OC = 2, MO_OO = 3, MO_CANONICAL = 4 };
+
+enum { MO_MULTINOMIAL = 0, MO_STRATIFIED = 2, MO_SYSTEMATIC = 3 };
+
+/* Counter engine state: restates CounterEngine (random/counter.hpp:61-288).
+ * buf holds the buffered block as raw bytes, index counts consumed result
+ * words (M == empty). */
+typedef struct {
+    int kind;
+    int rbits;
+    uint64_t ctr[4];
+    uint64_t key[4];
+    uint8_t buf[32];
+    unsigned index;
+} mo_engine;
+
+/* ---- generators: one block for an explicit counter value ---- */
+void mo_philox4x32_block(
+    const uint32_t key[2], const uint64_t ctr[2], uint32_t out[4]);
+void mo_threefry4x64_block(
+    const uint64_t key[4], const uint64_t ctr[4], uint64_t out[4]);
+void mo_ars_block(const uint32_t key[4], const uint64_t ctr[2], uint32_t out[4]);
+void mo_aes_tables(uint32_t t0[256], uint8_t sbox[256]);
+void mo_aes128_encrypt(const uint8_t key[16], const uint8_t in[16], uint8_t out[16]);
+
+/* ---- engine ---- */
+unsigned mo_engine_words(const mo_engine *e);
+void mo_engine_init(mo_engine *e, int kind, int rbits, const uint64_t *key);
+void mo_engine_seed(mo_engine *e, uint64_t s);
+void mo_engine_set_ctr(mo_engine *e, const uint64_t *ctr);
+void mo_rand_u32(mo_engine *e, size_t n, uint32_t *r);
+void mo_rand_u64(mo_engine *e, size_t n, uint64_t *r);
+uint32_t mo_next_u32(mo_engine *e);
+void mo_discard(mo_engine *e, uint64_t nskip);
+void mo_increment(uint64_t *ctr, int k, uint64_t nskip);
+
+/* ---- u01 conversions ---- */
+double mo_u01_u32_f64(int interval, uint32_t u);
+float mo_u01_u32_f32(int interval, uint32_t u);
+double mo_u01_canonical_f64(uint32_t u0, uint32_t u1);
+float mo_u01_canonical_f32(uint32_t u);
+void mo_rand_u01_f64(mo_engine *e, int interval, size_t n, double *r);
+void mo_rand_u01_f32(mo_engine *e, int interval, size_t n, float *r);
+
+/* ---- normal ---- */
+void mo_rand_normal_f32(mo_engine *e, size_t n, float mean, float sd, float *r);
+void mo_rand_normal_f64(
+    mo_engine *e, size_t n, double mean, double sd, double *r);
+
+/* ---- weight ---- */
+void mo_weight_set_log(size_t n, const double *v, double *w, double *ess);
+void mo_weight_add_log(size_t n, const double *v, double *w, double *ess,
+    double *lognc_inc);
+
+/* ---- resampling ---- */
+void mo_u01_rand_sorted(mo_engine *e, size_t n, double *r);
+void mo_u01_rand_stratified(mo_engine *e, size_t n, double *r);
+void mo_u01_rand_systematic(mo_engine *e, size_t n, double *r);
+void mo_resample_trans_u01_rep(size_t n, size_t m, const double *w,
+    const double *u01, uint64_t *rep);
+void mo_resample_trans_rep_index(
+    size_t n, size_t m, const uint64_t *rep, uint64_t *idx);
+void mo_resample(int scheme, size_t n, size_t m, mo_engine *e,
+    const double *w, uint64_t *rep, uint64_t *idx);
+
+/* ---- this project's SMC (device algorithm restated on the CPU) ---- */
+enum { MO_MODEL_LG = 1, MO_MODEL_SV = 2 };
+float mo_smc_normal(uint64_t seed, uint64_t i, uint64_t t);
+double mo_smc_run(int model, const double *theta, const double *y, size_t T,
+    size_t N, uint64_t seed, double threshold, int scheme, int nthreads,
+    double *ess_hist, int *resampled_hist);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

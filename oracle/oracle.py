@@ -1,0 +1,263 @@
+"""ctypes binding of the oracle (test infrastructure only).
+
+``O``   -- libmckl_oracle.so, the CPU restatement (mckl_oracle.c)
+``REF`` -- oracle/_ref/libmckl_ref.so, the reference's own headers behind
+           ref_driver.cpp, when it has been built (None otherwise)
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(HERE, "libmckl_oracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libmckl_ref.so")
+
+PHILOX4X32, THREEFRY4X64, ARS = 1, 2, 3
+CC, CO, OC, OO, CANONICAL = 0, 1, 2, 3, 4
+MULTINOMIAL, STRATIFIED, SYSTEMATIC = 0, 2, 3
+MODEL_LG, MODEL_SV = 1, 2
+
+O = ctypes.CDLL(ORACLE_PATH)
+REF = ctypes.CDLL(REF_PATH) if os.path.exists(REF_PATH) else None
+
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+
+
+class Eng(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("rbits", ctypes.c_int),
+                ("ctr", ctypes.c_uint64 * 4), ("key", ctypes.c_uint64 * 4),
+                ("buf", ctypes.c_uint8 * 32), ("index", ctypes.c_uint)]
+
+
+def P(a):
+    return a.ctypes.data_as(_vp)
+
+
+O.mo_smc_normal.restype = ctypes.c_float
+O.mo_smc_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
+O.mo_smc_run.restype = ctypes.c_double
+O.mo_smc_run.argtypes = [ctypes.c_int, _vp, _vp, _sz, _sz, ctypes.c_uint64,
+                         ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp, _vp]
+O.mo_next_u32.restype = ctypes.c_uint32
+O.mo_u01_u32_f64.restype = ctypes.c_double
+O.mo_u01_u32_f32.restype = ctypes.c_float
+O.mo_u01_canonical_f64.restype = ctypes.c_double
+O.mo_u01_canonical_f32.restype = ctypes.c_float
+if REF is not None:
+    REF.ref_time_philox_u01.restype = ctypes.c_double
+
+
+def engine(kind, key, rbits=32):
+    e = Eng()
+    k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
+    O.mo_engine_init(ctypes.byref(e), kind, rbits, k)
+    return e
+
+
+def state(e):
+    n = 4 if e.kind == THREEFRY4X64 else 2
+    return list(e.ctr[:n]), int(e.index)
+
+
+def rand_u32(e, n):
+    out = np.zeros(max(n, 1), np.uint32)
+    O.mo_rand_u32(ctypes.byref(e), _sz(n), P(out))
+    return out[:n]
+
+
+def rand_u64(e, n):
+    out = np.zeros(max(n, 1), np.uint64)
+    O.mo_rand_u64(ctypes.byref(e), _sz(n), P(out))
+    return out[:n]
+
+
+def rand_u01(e, interval, n, dtype):
+    out = np.zeros(max(n, 1), dtype)
+    fn = O.mo_rand_u01_f64 if np.dtype(dtype) == np.float64 else O.mo_rand_u01_f32
+    fn(ctypes.byref(e), interval, _sz(n), P(out))
+    return out[:n]
+
+
+def rand_normal(e, n, mean, sd, dtype):
+    out = np.zeros(max(n, 1), dtype)
+    if np.dtype(dtype) == np.float64:
+        O.mo_rand_normal_f64(ctypes.byref(e), _sz(n), ctypes.c_double(mean),
+                             ctypes.c_double(sd), P(out))
+    else:
+        O.mo_rand_normal_f32(ctypes.byref(e), _sz(n), ctypes.c_float(mean),
+                             ctypes.c_float(sd), P(out))
+    return out[:n]
+
+
+def philox_block(key, ctr):
+    k = (ctypes.c_uint32 * 2)(*key)
+    c = (ctypes.c_uint64 * 2)(*ctr)
+    o = (ctypes.c_uint32 * 4)()
+    O.mo_philox4x32_block(k, c, o)
+    return list(o)
+
+
+def threefry_block(key, ctr):
+    k = (ctypes.c_uint64 * 4)(*key)
+    c = (ctypes.c_uint64 * 4)(*ctr)
+    o = (ctypes.c_uint64 * 4)()
+    O.mo_threefry4x64_block(k, c, o)
+    return list(o)
+
+
+def ars_block(key, ctr):
+    k = (ctypes.c_uint32 * 4)(*key)
+    c = (ctypes.c_uint64 * 2)(*ctr)
+    o = (ctypes.c_uint32 * 4)()
+    O.mo_ars_block(k, c, o)
+    return list(o)
+
+
+def aes128(key, block):
+    k = (ctypes.c_uint8 * 16)(*key)
+    b = (ctypes.c_uint8 * 16)(*block)
+    o = (ctypes.c_uint8 * 16)()
+    O.mo_aes128_encrypt(k, b, o)
+    return bytes(o)
+
+
+def weight_set_log(v):
+    v = np.ascontiguousarray(v, np.float64)
+    w = np.zeros_like(v)
+    ess = ctypes.c_double()
+    O.mo_weight_set_log(_sz(len(v)), P(v), P(w), ctypes.byref(ess))
+    return w, ess.value
+
+
+def weight_add_log(w, v):
+    v = np.ascontiguousarray(v, np.float64)
+    w = np.array(w, np.float64)
+    ess = ctypes.c_double()
+    lnc = ctypes.c_double()
+    O.mo_weight_add_log(_sz(len(v)), P(v), P(w), ctypes.byref(ess),
+                        ctypes.byref(lnc))
+    return w, ess.value, lnc.value
+
+
+def u01_sequence(scheme, e, n):
+    out = np.zeros(max(n, 1), np.float64)
+    fn = {MULTINOMIAL: O.mo_u01_rand_sorted, STRATIFIED: O.mo_u01_rand_stratified,
+          SYSTEMATIC: O.mo_u01_rand_systematic}[scheme]
+    fn(ctypes.byref(e), _sz(n), P(out))
+    return out[:n]
+
+
+def trans_u01_rep(w, u, N=None, M=None):
+    w = np.ascontiguousarray(w, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    N = len(w) if N is None else N
+    M = len(u) if M is None else M
+    rep = np.zeros(max(N, 1), np.uint64)
+    O.mo_resample_trans_u01_rep(_sz(N), _sz(M), P(w), P(u), P(rep))
+    return rep[:N]
+
+
+def rep_index(rep, M):
+    rep = np.ascontiguousarray(rep, np.uint64)
+    idx = np.zeros(max(M, 1), np.uint64)
+    O.mo_resample_trans_rep_index(_sz(len(rep)), _sz(M), P(rep), P(idx))
+    return idx[:M]
+
+
+def resample(scheme, e, w, M=None):
+    w = np.ascontiguousarray(w, np.float64)
+    N = len(w)
+    M = N if M is None else M
+    rep = np.zeros(max(N, 1), np.uint64)
+    idx = np.zeros(max(M, 1), np.uint64)
+    O.mo_resample(scheme, _sz(N), _sz(M), ctypes.byref(e), P(w), P(rep), P(idx))
+    return rep[:N], idx[:M]
+
+
+def smc_normal(seed, i, t):
+    return O.mo_smc_normal(seed, i, t)
+
+
+def smc_run(model, theta, y, N, seed, threshold=0.5, scheme=SYSTEMATIC,
+            nthreads=1, history=False):
+    th = np.ascontiguousarray(theta, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    T = len(y)
+    ess = np.zeros(T, np.float64)
+    res = np.zeros(T, np.int32)
+    ll = O.mo_smc_run(model, P(th), P(y), _sz(T), _sz(N), seed, threshold,
+                      scheme, nthreads, P(ess), P(res))
+    if history:
+        return ll, ess, res
+    return ll
+
+
+# ---- reference (compiled from the reference headers), when available ----
+def ref_engine_ops(kind, rbits, key, sizes, ctr0=None):
+    n = sum(s if s > 0 else (1 if s == 0 else 0) for s in sizes)
+    out = np.zeros(max(n, 1), np.uint32 if rbits == 32 else np.uint64)
+    k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
+    c0 = None if ctr0 is None else (ctypes.c_uint64 * 4)(
+        *(list(ctr0) + [0] * (4 - len(ctr0))))
+    sz = (ctypes.c_longlong * len(sizes))(*sizes)
+    ctr = (ctypes.c_uint64 * 4)()
+    idx = ctypes.c_uint()
+    REF.ref_engine_ops(kind, rbits, k, c0, len(sizes), sz, P(out), ctr,
+                       ctypes.byref(idx))
+    nc = 4 if kind == THREEFRY4X64 else 2
+    return out[:n], list(ctr[:nc]), idx.value
+
+
+def ref_u01(kind, key, interval, dtype, n, pre=0):
+    out = np.zeros(n + 1, dtype)
+    k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
+    ctr = (ctypes.c_uint64 * 4)()
+    idx = ctypes.c_uint()
+    REF.ref_u01(kind, k, interval, int(np.dtype(dtype) == np.float64), _sz(pre),
+                _sz(n), P(out), ctr, ctypes.byref(idx))
+    nc = 4 if kind == THREEFRY4X64 else 2
+    return out[:n], list(ctr[:nc]), idx.value
+
+
+def ref_normal(kind, key, dtype, n, mean, sd, pre=0):
+    out = np.zeros(n + 1, dtype)
+    k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
+    ctr = (ctypes.c_uint64 * 4)()
+    idx = ctypes.c_uint()
+    REF.ref_normal(kind, k, int(np.dtype(dtype) == np.float64), _sz(pre), _sz(n),
+                   ctypes.c_double(mean), ctypes.c_double(sd), P(out), ctr,
+                   ctypes.byref(idx))
+    nc = 4 if kind == THREEFRY4X64 else 2
+    return out[:n], list(ctr[:nc]), idx.value
+
+
+def ref_resample(scheme, kind, key, w, M=None):
+    w = np.ascontiguousarray(w, np.float64)
+    N = len(w)
+    M = N if M is None else M
+    rep = np.zeros(max(N, 1), np.uint64)
+    idx = np.zeros(max(M, 1), np.uint64)
+    k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
+    REF.ref_resample(scheme, _sz(N), _sz(M), kind, k, P(w), P(rep), P(idx))
+    return rep[:N], idx[:M]
+
+
+def ref_weight(mode, v, w=None, ess=0.0):
+    v = np.ascontiguousarray(v, np.float64)
+    w = np.zeros_like(v) if w is None else np.array(w, np.float64)
+    e = ctypes.c_double(ess)
+    REF.ref_weight(mode, _sz(len(v)), P(v), P(w), ctypes.byref(e))
+    return w, e.value
+
+
+def ref_u01_sequence(scheme, kind, key, n):
+    out = np.zeros(max(n, 1), np.float64)
+    k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
+    ctr = (ctypes.c_uint64 * 4)()
+    idx = ctypes.c_uint()
+    REF.ref_u01_sequence(scheme, kind, k, _sz(n), P(out), ctr, ctypes.byref(idx))
+    return out[:n]

@@ -1,0 +1,292 @@
+// ref_driver.cpp -- thin C entry points over the reference's own headers.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile into oracle/_ref/ from
+// the reference headers where they lie (header-only library; nothing is
+// copied).  Used to validate the CPU restatement (mckl_oracle.c), to write
+// the committed fixtures under tests/golden/, and as the "reference" CPU
+// baseline in bench.py.  The product never links it.
+//
+// Pinned flags (oracle/Makefile): -std=c++14 -O2 -ffp-contract=off, no -march
+// => SSE2 paths, no FMA in muladd (math/vmf.hpp:646-722), software AES for ARS
+// (random/aes.hpp:76-104), MCKL_REQUIRE_ENDIANNESS_NEUTURAL=1 as the
+// reference's examples are built (example/CMakeLists.txt:34).
+
+#include <mckl/algorithm/resample.hpp>
+#include <mckl/core/weight.hpp>
+#include <mckl/random/aes.hpp>
+#include <mckl/random/normal_distribution.hpp>
+#include <mckl/random/philox.hpp>
+#include <mckl/random/threefry.hpp>
+#include <mckl/random/u01_distribution.hpp>
+
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+template <typename RNG>
+void set_key(RNG &rng, const std::uint64_t *key)
+{
+    typename RNG::key_type k;
+    for (std::size_t i = 0; i != k.size(); ++i)
+        k[i] = static_cast<typename RNG::key_type::value_type>(key[i]);
+    rng.seed(k);
+}
+
+// Engine state as (ctr words, index) parsed from the reference's own
+// operator<< (random/counter.hpp:235-250): "result... ctr... gen... index".
+template <typename RNG>
+void get_state(const RNG &rng, std::uint64_t *ctr, unsigned *index)
+{
+    auto c = rng.ctr();
+    for (std::size_t i = 0; i != c.size(); ++i)
+        ctr[i] = c[i];
+    std::ostringstream os;
+    os << rng;
+    std::string s = os.str();
+    std::size_t p = s.find_last_of(' ');
+    *index = static_cast<unsigned>(std::stoul(s.substr(p + 1)));
+}
+
+template <typename RNG, typename Fn>
+int with_engine(int rbits, const std::uint64_t *key, Fn &&fn);
+
+template <typename Fn>
+int dispatch(int kind, int rbits, const std::uint64_t *key, Fn &&fn)
+{
+    switch (kind) {
+        case 1:
+            if (rbits == 32) {
+                mckl::Philox4x32 rng;
+                set_key(rng, key);
+                return fn(rng);
+            } else {
+                mckl::Philox4x32_64 rng;
+                set_key(rng, key);
+                return fn(rng);
+            }
+        case 2:
+            if (rbits == 32) {
+                mckl::Threefry4x64 rng;
+                set_key(rng, key);
+                return fn(rng);
+            } else {
+                mckl::Threefry4x64_64 rng;
+                set_key(rng, key);
+                return fn(rng);
+            }
+        case 3:
+            if (rbits == 32) {
+                mckl::ARS rng;
+                set_key(rng, key);
+                return fn(rng);
+            } else {
+                mckl::ARS_64 rng;
+                set_key(rng, key);
+                return fn(rng);
+            }
+    }
+    return -1;
+}
+
+} // namespace
+
+extern "C" {
+
+// A sequence of bulk rand calls (sizes[0..ncalls)), output concatenated.
+// ops: size > 0 is rand(rng, size, r); size < 0 is discard(-size); size == 0
+// is one scalar rng() call.  Final state returned in ctr/index.
+int ref_engine_ops(int kind, int rbits, const std::uint64_t *key,
+    const std::uint64_t *ctr0, int ncalls, const long long *sizes, void *out,
+    std::uint64_t *ctr, unsigned *index)
+{
+    return dispatch(kind, rbits, key, [&](auto &rng) {
+        using R = typename std::remove_reference_t<decltype(rng)>::result_type;
+        if (ctr0) {
+            auto c = rng.ctr();
+            for (std::size_t i = 0; i != c.size(); ++i)
+                c[i] = ctr0[i];
+            rng.ctr(c);
+        }
+        R *r = static_cast<R *>(out);
+        for (int i = 0; i != ncalls; ++i) {
+            if (sizes[i] > 0) {
+                mckl::rand(rng, static_cast<std::size_t>(sizes[i]), r);
+                r += sizes[i];
+            } else if (sizes[i] < 0) {
+                rng.discard(static_cast<typename std::remove_reference_t<
+                        decltype(rng)>::skip_type>(-sizes[i]));
+            } else {
+                *r++ = rng();
+            }
+        }
+        get_state(rng, ctr, index);
+        return 0;
+    });
+}
+
+// pre raw u32 draws, then rand(rng, U01{CC,CO,OC,OO,Canonical}<T>, n, r)
+int ref_u01(int kind, const std::uint64_t *key, int interval, int dbl,
+    std::size_t pre, std::size_t n, void *out, std::uint64_t *ctr,
+    unsigned *index)
+{
+    return dispatch(kind, 32, key, [&](auto &rng) {
+        using R = typename std::remove_reference_t<decltype(rng)>::result_type;
+        std::vector<R> skip(pre + 1);
+        mckl::rand(rng, pre, skip.data());
+        auto run = [&](auto *r) {
+            using T = std::remove_pointer_t<decltype(r)>;
+            switch (interval) {
+                case 0: {
+                    mckl::U01CCDistribution<T> d;
+                    mckl::rand(rng, d, n, r);
+                } break;
+                case 1: {
+                    mckl::U01CODistribution<T> d;
+                    mckl::rand(rng, d, n, r);
+                } break;
+                case 2: {
+                    mckl::U01OCDistribution<T> d;
+                    mckl::rand(rng, d, n, r);
+                } break;
+                case 3: {
+                    mckl::U01OODistribution<T> d;
+                    mckl::rand(rng, d, n, r);
+                } break;
+                default: {
+                    mckl::U01CanonicalDistribution<T> d;
+                    mckl::rand(rng, d, n, r);
+                } break;
+            }
+        };
+        if (dbl)
+            run(static_cast<double *>(out));
+        else
+            run(static_cast<float *>(out));
+        get_state(rng, ctr, index);
+        return 0;
+    });
+}
+
+// pre raw u32 draws, then rand(rng, NormalDistribution<T>(mean, sd), n, r)
+int ref_normal(int kind, const std::uint64_t *key, int dbl, std::size_t pre,
+    std::size_t n, double mean, double sd, void *out, std::uint64_t *ctr,
+    unsigned *index)
+{
+    return dispatch(kind, 32, key, [&](auto &rng) {
+        using R = typename std::remove_reference_t<decltype(rng)>::result_type;
+        std::vector<R> skip(pre + 1);
+        mckl::rand(rng, pre, skip.data());
+        if (dbl) {
+            mckl::NormalDistribution<double> d(mean, sd);
+            mckl::rand(rng, d, n, static_cast<double *>(out));
+        } else {
+            mckl::NormalDistribution<float> d(
+                static_cast<float>(mean), static_cast<float>(sd));
+            mckl::rand(rng, d, n, static_cast<float *>(out));
+        }
+        get_state(rng, ctr, index);
+        return 0;
+    });
+}
+
+// Weight::set_log (mode 0) or Weight::add_log (mode 1) on W (in/out).
+int ref_weight(int mode, std::size_t n, const double *v, double *w, double *ess)
+{
+    mckl::Weight weight(n);
+    if (mode == 1)
+        weight.set_exact(*ess, w);
+    if (mode == 0)
+        weight.set_log(v);
+    else
+        weight.add_log(v);
+    weight.read(w);
+    *ess = weight.ess();
+    return 0;
+}
+
+// U01 sequences: scheme 0 sorted (multinomial), 2 stratified, 3 systematic
+int ref_u01_sequence(int scheme, int kind, const std::uint64_t *key,
+    std::size_t n, double *r, std::uint64_t *ctr, unsigned *index)
+{
+    return dispatch(kind, 32, key, [&](auto &rng) {
+        if (scheme == 0)
+            mckl::u01_rand_sorted(rng, n, r);
+        else if (scheme == 2)
+            mckl::u01_rand_stratified(rng, n, r);
+        else
+            mckl::u01_rand_systematic(rng, n, r);
+        get_state(rng, ctr, index);
+        return 0;
+    });
+}
+
+int ref_resample(int scheme, std::size_t N, std::size_t M, int kind,
+    const std::uint64_t *key, const double *w, std::uint64_t *rep,
+    std::uint64_t *idx)
+{
+    return dispatch(kind, 32, key, [&](auto &rng) {
+        std::vector<std::size_t> r(N + 1), x(M + 1);
+        if (scheme == 0)
+            mckl::ResampleMultinomial()(N, M, rng, w, r.data());
+        else if (scheme == 2)
+            mckl::ResampleStratified()(N, M, rng, w, r.data());
+        else
+            mckl::ResampleSystematic()(N, M, rng, w, r.data());
+        mckl::resample_trans_rep_index(N, M, r.data(), x.data());
+        for (std::size_t i = 0; i != N; ++i)
+            rep[i] = r[i];
+        for (std::size_t i = 0; i != M; ++i)
+            idx[i] = x[i];
+        return 0;
+    });
+}
+
+int ref_trans_u01_rep(std::size_t N, std::size_t M, const double *w,
+    const double *u, std::uint64_t *rep)
+{
+    std::vector<std::size_t> r(N + 1);
+    mckl::resample_trans_u01_rep(N, M, w, u, r.data());
+    for (std::size_t i = 0; i != N; ++i)
+        rep[i] = r[i];
+    return 0;
+}
+
+int ref_rep_index(
+    std::size_t N, std::size_t M, const std::uint64_t *rep, std::uint64_t *idx)
+{
+    std::vector<std::size_t> r(rep, rep + N), x(M + 1);
+    mckl::resample_trans_rep_index(N, M, r.data(), x.data());
+    for (std::size_t i = 0; i != M; ++i)
+        idx[i] = x[i];
+    return 0;
+}
+
+// CPU baseline: time reps of `rand(Philox4x32, n, u32*)` followed by
+// rand(U01OO<double>, n) on the same stream (fresh engine each rep), single
+// thread.  Returns seconds per rep (median).
+double ref_time_philox_u01(const std::uint64_t *key, std::size_t n, int reps,
+    std::uint32_t *u32, double *f64)
+{
+    std::vector<double> t;
+    for (int k = 0; k < reps; ++k) {
+        mckl::Philox4x32 rng;
+        set_key(rng, key);
+        auto t0 = std::chrono::steady_clock::now();
+        mckl::rand(rng, n, u32);
+        mckl::Philox4x32 rng2;
+        set_key(rng2, key);
+        mckl::U01OODistribution<double> d;
+        mckl::rand(rng2, d, n, f64);
+        auto t1 = std::chrono::steady_clock::now();
+        t.push_back(std::chrono::duration<double>(t1 - t0).count());
+    }
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+}
+
+} // extern "C"
