@@ -204,3 +204,196 @@ def streams_normal(engines, n, mean, stddev, r):
     for i, e in enumerate(engines):
         e.state = arr[i]
     return r
+
+
+# ---------------------------------------------------------------------------
+# Weight and resampling (core/weight.hpp, algorithm/resample.hpp)
+# ---------------------------------------------------------------------------
+MULTINOMIAL = _lib.MULTINOMIAL
+STRATIFIED = _lib.STRATIFIED
+SYSTEMATIC = _lib.SYSTEMATIC
+MODEL_LG = _lib.MODEL_LG
+MODEL_SV = _lib.MODEL_SV
+
+
+class Weight:
+    """Device-resident normalised FP64 weights (core/weight.hpp:44-228).
+
+    ``log_nc()`` returns the log normaliser increment lmax + log(accw) of the
+    last set_log/add_log, which the reference computes and discards.
+    """
+
+    def __init__(self, n):
+        import torch
+        self.data = torch.empty(n, dtype=torch.float64, device="cuda")
+        self._stats = torch.zeros(2, dtype=torch.float64, device="cuda")
+        self.set_equal()
+
+    def size(self):
+        return self.data.numel()
+
+    def set_equal(self):
+        check(lib.mckl_hip_weight_set_equal(self.size(), _ptr(self.data),
+                                            _ptr(self._stats), _stream()))
+
+    def set_log(self, v):
+        check(lib.mckl_hip_weight_set_log(self.size(), _ptr(v), _ptr(self.data),
+                                          _ptr(self._stats), _stream()))
+
+    def add_log(self, v):
+        check(lib.mckl_hip_weight_add_log(self.size(), _ptr(v), _ptr(self.data),
+                                          _ptr(self._stats), _stream()))
+
+    def ess(self):
+        return float(self._stats[0].item())
+
+    def log_nc(self):
+        return float(self._stats[1].item())
+
+
+def u01_rand_sequence(scheme, rng, n, r):
+    check(lib.mckl_hip_u01_rand_sequence(scheme, ctypes.byref(rng.state), n,
+                                         _ptr(r), _stream()))
+    return r
+
+
+def resample_trans_u01_rep(N, M, w, u, rep):
+    check(lib.mckl_hip_resample_trans_u01_rep(N, M, _ptr(w), _ptr(u),
+                                              _ptr(rep), _stream()))
+    return rep
+
+
+def resample_trans_rep_index(N, M, rep, idx):
+    check(lib.mckl_hip_resample_trans_rep_index(N, M, _ptr(rep), _ptr(idx),
+                                                _stream()))
+    return idx
+
+
+class _ResampleAlgorithm:
+    """ResampleAlgorithm<U01Seq, false> (algorithm/resample.hpp:515-576):
+    ``alg(N, M, rng, w, rep, idx=None)`` on device arrays."""
+
+    SCHEME = None
+
+    def __call__(self, N, M, rng, w, rep, idx=None):
+        check(lib.mckl_hip_resample(self.SCHEME, N, M, ctypes.byref(rng.state),
+                                    _ptr(w), _ptr(rep),
+                                    _ptr(idx) if idx is not None else None,
+                                    _stream()))
+        return rep
+
+
+class ResampleMultinomial(_ResampleAlgorithm):
+    SCHEME = MULTINOMIAL
+
+
+class ResampleStratified(_ResampleAlgorithm):
+    SCHEME = STRATIFIED
+
+
+class ResampleSystematic(_ResampleAlgorithm):
+    SCHEME = SYSTEMATIC
+
+
+def select(idx, src, dst, dim=1):
+    """StateMatrix<RowMajor, dim, float>::select, out of place."""
+    M = idx.numel()
+    check(lib.mckl_hip_select_f32(M, dim, _ptr(idx), _ptr(src), _ptr(dst),
+                                  _stream()))
+    return dst
+
+
+# ---------------------------------------------------------------------------
+# SMC samplers (algorithm/smc.hpp:151-421 shape) for the built-in models
+# ---------------------------------------------------------------------------
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class SMCSampler:
+    """Bootstrap particle filter on the device.
+
+    model: MODEL_LG (theta = a, sx, sy) or MODEL_SV (theta = phi, sigma,
+    beta); y: host observations (length T).  ``iterate()`` enqueues all T
+    steps on the current stream without host synchronisation.
+    """
+
+    def __init__(self, model, theta, y, N, seed=1, resample_threshold=0.5,
+                 scheme=SYSTEMATIC):
+        self._th = np.ascontiguousarray(theta, np.float64)
+        self._y = np.ascontiguousarray(y, np.float64)
+        self.T = len(self._y)
+        self.N = int(N)
+        h = ctypes.c_void_p()
+        check(lib.mckl_hip_smc_create(model, _dptr(self._th), self.T,
+                                      _dptr(self._y), self.N, int(seed),
+                                      float(resample_threshold), scheme,
+                                      ctypes.byref(h)))
+        self._h = h
+
+    def iterate(self):
+        check(lib.mckl_hip_smc_run(self._h, _stream()))
+
+    def results(self):
+        ll = ctypes.c_double()
+        ess = np.zeros(self.T, np.float64)
+        res = np.zeros(self.T, np.int32)
+        check(lib.mckl_hip_smc_result(
+            self._h, ctypes.byref(ll), _dptr(ess),
+            res.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return ll.value, ess, res
+
+    def log_likelihood(self):
+        return self.results()[0]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.mckl_hip_smc_destroy(h)
+            self._h = None
+
+
+def smc_loglik(model, theta, y, N, seed=1, threshold=0.5, scheme=SYSTEMATIC):
+    s = SMCSampler(model, theta, y, N, seed, threshold, scheme)
+    s.iterate()
+    return s.log_likelihood()
+
+
+class SMCBatch:
+    """Many independent filters in one persistent launch."""
+
+    def __init__(self, model, N, theta, y, seeds, resample_threshold=0.5,
+                 scheme=STRATIFIED):
+        self._N = np.ascontiguousarray(N, np.uint32)
+        self._th = np.ascontiguousarray(theta, np.float64).reshape(-1)
+        self._y = np.ascontiguousarray(y, np.float64)
+        self._seeds = np.ascontiguousarray(seeds, np.uint64)
+        self.nfilters = len(self._N)
+        self.T = self._y.shape[-1]
+        h = ctypes.c_void_p()
+        check(lib.mckl_hip_smc_batch_create(
+            model, self.nfilters,
+            self._N.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+            _dptr(self._th), self.T, _dptr(self._y.reshape(-1)),
+            self._seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            float(resample_threshold), scheme, ctypes.byref(h)))
+        self._h = h
+
+    def set_tune(self, block=256, waves_per_eu=0):
+        t = _lib.Tune(block, waves_per_eu)
+        check(lib.mckl_hip_smc_batch_set_tune(self._h, ctypes.byref(t)))
+
+    def run(self, use_graph=True):
+        check(lib.mckl_hip_smc_batch_run(self._h, int(bool(use_graph)),
+                                         _stream()))
+
+    def log_likelihood(self):
+        out = np.zeros(self.nfilters, np.float64)
+        check(lib.mckl_hip_smc_batch_result(self._h, _dptr(out)))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.mckl_hip_smc_batch_destroy(h)
+            self._h = None

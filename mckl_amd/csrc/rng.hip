@@ -853,6 +853,13 @@ static int block_on_device(int kind, const uint64_t *ctr, const uint64_t *key,
     return hip_status(err, "one_block_kernel");
 }
 
+// U01Canonical<double> fill used by the resampling sequences
+// (u01_distribution.hpp:115-129: two u32 per double, 1024 per bulk call)
+int rand_canonical_f64(mckl_hip_engine *e, size_t n, double *d_r, hipStream_t st)
+{
+    return element_fill(e, 1, n, d_r, ConvCanF64(), 2048, st);
+}
+
 } // namespace mckl_hip
 
 using namespace mckl_hip;

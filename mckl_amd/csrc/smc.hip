@@ -1,0 +1,817 @@
+// smc.hip -- SMC samplers on gfx950.
+//
+//  * plain sampler: one workgroup per 2048-particle tile, a handful of
+//    launches per step, no host synchronisation (the ESS test is taken on the
+//    device and the resampling kernels exit when it is negative), so the
+//    whole T loop can be captured into one HIP graph;
+//  * batched sampler: thousands of independent filters of uneven sizes in one
+//    persistent launch, one workgroup per filter taken from a device queue
+//    ordered by decreasing particle count;
+// both built from the per-tile stages of smc_tiles.hpp, so a filter gives
+// the same bits in either sampler.
+#include "common.hpp"
+#include "smc_tiles.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace mckl_hip {
+
+constexpr int kScratch = 1280; // LDS bytes per 256-thread group
+
+struct GroupLds {
+    double *d;
+    float *f;
+    uint32_t *u;
+};
+
+__device__ __forceinline__ GroupLds group_lds(unsigned char *base)
+{
+    GroupLds g;
+    g.d = reinterpret_cast<double *>(base);
+    g.f = reinterpret_cast<float *>(base + 64);
+    g.u = reinterpret_cast<uint32_t *>(base + 128);
+    return g;
+}
+
+// per-filter device state of the plain sampler
+struct SmcState {
+    double logz;
+    double S, S2;   // sums of the current step
+    double C;       // normaliser of the carried weights for the next step
+    float m;        // max log weight of the current step
+    int resampled;  // current step resampled (applies to the next move)
+    uint32_t counter; // last-block counter
+    uint32_t pad;
+};
+
+// -------------------------------------------------------------------------
+// plain sampler kernels (grid = ntiles workgroups of 256 threads)
+// -------------------------------------------------------------------------
+__global__ __launch_bounds__(kGroup) void k_init(
+    SmcArgs A, float *x, float *lw, SmcState *st)
+{
+    tile_init(A, blockIdx.x, x, lw, threadIdx.x);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->logz = 0.0;
+        st->S = (double)A.N;
+        st->S2 = (double)A.N;
+        st->C = (double)A.N;
+        st->m = 0.0f;
+        st->resampled = 0;
+        st->counter = 0;
+    }
+}
+
+__global__ __launch_bounds__(kGroup) void k_move(SmcArgs A, uint32_t t,
+    const float *xin, float *xout, float *lw, const uint32_t *idx,
+    const SmcState *st, float *tmax)
+{
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    const GroupLds L = group_lds(lds);
+    const int res = st->resampled;
+    const float mprev = st->m;
+    const float mt = tile_move(A, t, blockIdx.x, xin, xout, lw, idx, res,
+        mprev, threadIdx.x, L.f);
+    if (threadIdx.x == 0)
+        tmax[blockIdx.x] = mt;
+}
+
+// sums per tile; the last workgroup to finish reduces across tiles and
+// takes the step's decisions
+__global__ __launch_bounds__(kGroup) void k_sums(SmcArgs A, uint32_t t,
+    const float *lw, const float *tmax, double *ts, double *ts2, SmcState *st,
+    double *ess_hist, int32_t *res_hist)
+{
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    __shared__ int last;
+    const GroupLds L = group_lds(lds);
+    const int gt = threadIdx.x;
+    float m = -__builtin_huge_valf();
+    for (uint32_t i = gt; i < A.ntiles; i += kGroup)
+        m = fmaxf(m, tmax[i]);
+    m = group_max(m, gt, L.f);
+    double S, S2;
+    tile_sums(A, blockIdx.x, lw, m, gt, L.d, S, S2);
+    if (gt == 0) {
+        ts[blockIdx.x] = S;
+        ts2[blockIdx.x] = S2;
+        __threadfence();
+        last = atomicAdd(&st->counter, 1u) == A.ntiles - 1;
+    }
+    __syncthreads();
+    if (!last)
+        return;
+    __threadfence();
+    double rs, rs2;
+    tiles_sum2(ts, ts2, A.ntiles, gt, L.d, rs, rs2);
+    if (gt == 0) {
+        const StepOut o = step_finalize(A, m, rs, rs2, st->C);
+        st->logz = __dadd_rn(st->logz, o.logz_inc);
+        st->S = rs;
+        st->S2 = rs2;
+        st->m = m;
+        st->resampled = o.resample;
+        st->C = o.resample ? (double)A.N : rs;
+        st->counter = 0;
+        if (ess_hist)
+            ess_hist[t - 1] = o.ess;
+        if (res_hist)
+            res_hist[t - 1] = o.resample;
+    }
+}
+
+struct RsBuf {
+    double *tdh, *tdl;   // tile dd sums
+    double *toh, *tol;   // exclusive tile offsets (ntiles + 1)
+    uint32_t *te, *tz;   // tile extra-copy / zero-slot counts
+    uint32_t *teo, *tzo; // their exclusive offsets (ntiles + 1)
+    uint32_t *cnt, *Eincl;
+};
+
+__global__ __launch_bounds__(kGroup) void k_rs_wsum(
+    SmcArgs A, const float *lw, SmcState *st, RsBuf B)
+{
+    if (!st->resampled)
+        return;
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    __shared__ int last;
+    const GroupLds L = group_lds(lds);
+    const int gt = threadIdx.x;
+    const double invS = __ddiv_rn(1.0, st->S);
+    const dd tot = tile_wsum(A, blockIdx.x, lw, st->m, invS, gt, L.d);
+    if (gt == 0) {
+        B.tdh[blockIdx.x] = tot.hi;
+        B.tdl[blockIdx.x] = tot.lo;
+        __threadfence();
+        last = atomicAdd(&st->counter, 1u) == A.ntiles - 1;
+    }
+    __syncthreads();
+    if (!last)
+        return;
+    __threadfence();
+    tiles_scan_dd(true, B.tdh, B.tdl, A.ntiles, B.toh, B.tol, gt, L.d);
+    if (gt == 0)
+        st->counter = 0;
+}
+
+__global__ __launch_bounds__(kGroup) void k_rs_counts(
+    SmcArgs A, uint32_t t, const float *lw, SmcState *st, RsBuf B)
+{
+    if (!st->resampled)
+        return;
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    __shared__ int last;
+    const GroupLds L = group_lds(lds);
+    const int gt = threadIdx.x;
+    const double invS = __ddiv_rn(1.0, st->S);
+    const UniformSeq U = make_uniform_seq(A, t);
+    uint32_t es, zs;
+    tile_counts(A, blockIdx.x, lw, st->m, invS, B.toh, B.tol, U, B.cnt, gt,
+        L.d, L.u, es, zs);
+    if (gt == 0) {
+        B.te[blockIdx.x] = es;
+        B.tz[blockIdx.x] = zs;
+        __threadfence();
+        last = atomicAdd(&st->counter, 1u) == A.ntiles - 1;
+    }
+    __syncthreads();
+    if (!last)
+        return;
+    __threadfence();
+    tiles_scan_u2(true, B.te, B.tz, A.ntiles, B.teo, B.tzo, gt, L.u);
+    if (gt == 0)
+        st->counter = 0;
+}
+
+__global__ __launch_bounds__(kGroup) void k_rs_index(
+    SmcArgs A, const SmcState *st, RsBuf B, uint32_t *idx)
+{
+    if (!st->resampled)
+        return;
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    const GroupLds L = group_lds(lds);
+    tile_index(A, blockIdx.x, B.cnt, B.teo[blockIdx.x], B.tzo[blockIdx.x],
+        B.Eincl, idx, threadIdx.x, L.u);
+}
+
+__global__ __launch_bounds__(kGroup) void k_rs_resolve(
+    SmcArgs A, const SmcState *st, RsBuf B, uint32_t *idx)
+{
+    if (!st->resampled)
+        return;
+    tile_resolve(A, blockIdx.x, B.Eincl, idx, threadIdx.x);
+}
+
+// -------------------------------------------------------------------------
+// batched sampler: one workgroup (G groups of 256 threads) per filter
+// -------------------------------------------------------------------------
+constexpr int kMaxTiles = 64; // N <= 131072 per filter in the batched form
+
+struct FilterDesc {
+    SmcArgs A;
+    uint64_t off; // particle offset into the shared buffers
+};
+
+struct BatchLds {
+    float tmax[kMaxTiles];
+    double ts[kMaxTiles], ts2[kMaxTiles];
+    double tdh[kMaxTiles], tdl[kMaxTiles];
+    double toh[kMaxTiles + 1], tol[kMaxTiles + 1];
+    uint32_t te[kMaxTiles], tz[kMaxTiles];
+    uint32_t teo[kMaxTiles + 1], tzo[kMaxTiles + 1];
+    int flag;
+    int filter;
+};
+
+template <int G>
+__global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
+    const uint32_t *order, uint32_t nfilters, uint32_t *queue, float *xa,
+    float *xb, float *lwb, uint32_t *idxb, uint32_t *cntb, uint32_t *Eb,
+    double *logz_out)
+{
+    __shared__ __align__(16) unsigned char scratch[G * kScratch];
+    __shared__ BatchLds S;
+    const int grp = threadIdx.x / kGroup;
+    const int gt = threadIdx.x % kGroup;
+    const GroupLds L = group_lds(scratch + grp * kScratch);
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const uint32_t q = atomicAdd(queue, 1u);
+            S.filter = q < nfilters ? (int)order[q] : -1;
+        }
+        __syncthreads();
+        const int f = S.filter;
+        __syncthreads();
+        if (f < 0)
+            return;
+        const SmcArgs A = fd[f].A;
+        const uint64_t off = fd[f].off;
+        float *x0 = xa + off, *x1 = xb + off, *lw = lwb + off;
+        uint32_t *idx = idxb + off, *cnt = cntb + off, *Eincl = Eb + off;
+        const int nt = (int)A.ntiles;
+        const int rounds = (nt + G - 1) / G;
+
+        for (int k = grp; k < nt; k += G)
+            tile_init(A, (uint32_t)k, x0, lw, gt);
+        __syncthreads();
+        double logz = 0.0, C = (double)A.N;
+        float mprev = 0.0f;
+        int resampled = 0;
+        // Groups without a tile in a round take tile index nt (past the
+        // end): every particle index is >= N, so they touch no memory but
+        // still meet every barrier of the group functions.
+        for (uint32_t t = 1; t <= A.T; ++t) {
+            const float *xin = (t & 1) ? x0 : x1;
+            float *xout = (t & 1) ? x1 : x0;
+            // move + weight
+            for (int r = 0; r < rounds; ++r) {
+                const int k = r * G + grp;
+                const uint32_t kk = (uint32_t)(k < nt ? k : nt);
+                const float mt = tile_move(A, t, kk, xin, xout, lw, idx,
+                    resampled, mprev, gt, L.f);
+                if (gt == 0 && k < nt)
+                    S.tmax[k] = mt;
+                __syncthreads();
+            }
+            float m = -__builtin_huge_valf();
+            for (int i = 0; i < nt; ++i)
+                m = fmaxf(m, S.tmax[i]);
+            for (int r = 0; r < rounds; ++r) {
+                const int k = r * G + grp;
+                double s, s2;
+                tile_sums(A, (uint32_t)(k < nt ? k : nt), lw, m, gt, L.d, s, s2);
+                if (gt == 0 && k < nt) {
+                    S.ts[k] = s;
+                    S.ts2[k] = s2;
+                }
+                __syncthreads();
+            }
+            double rs, rs2;
+            tiles_sum2(S.ts, S.ts2, nt, gt, L.d, rs, rs2);
+            const StepOut o = step_finalize(A, m, rs, rs2, C);
+            logz = __dadd_rn(logz, o.logz_inc);
+            C = o.resample ? (double)A.N : rs;
+            mprev = m;
+            resampled = o.resample;
+            if (resampled) {
+                const double invS = __ddiv_rn(1.0, rs);
+                for (int r = 0; r < rounds; ++r) {
+                    const int k = r * G + grp;
+                    const dd tot = tile_wsum(A, (uint32_t)(k < nt ? k : nt),
+                        lw, m, invS, gt, L.d);
+                    if (gt == 0 && k < nt) {
+                        S.tdh[k] = tot.hi;
+                        S.tdl[k] = tot.lo;
+                    }
+                    __syncthreads();
+                }
+                tiles_scan_dd(grp == 0, S.tdh, S.tdl, nt, S.toh, S.tol, gt, L.d);
+                const UniformSeq U = make_uniform_seq(A, t);
+                for (int r = 0; r < rounds; ++r) {
+                    const int k = r * G + grp;
+                    uint32_t es, zs;
+                    tile_counts(A, (uint32_t)(k < nt ? k : nt), lw, m, invS,
+                        S.toh, S.tol, U, cnt, gt, L.d, L.u, es, zs);
+                    if (gt == 0 && k < nt) {
+                        S.te[k] = es;
+                        S.tz[k] = zs;
+                    }
+                    __syncthreads();
+                }
+                tiles_scan_u2(grp == 0, S.te, S.tz, nt, S.teo, S.tzo, gt, L.u);
+                for (int r = 0; r < rounds; ++r) {
+                    const int k = r * G + grp;
+                    const int kk = k < nt ? k : nt;
+                    tile_index(A, (uint32_t)kk, cnt, S.teo[kk], S.tzo[kk],
+                        Eincl, idx, gt, L.u);
+                    __syncthreads();
+                }
+                for (int r = 0; r < rounds; ++r) {
+                    const int k = r * G + grp;
+                    if (k < nt)
+                        tile_resolve(A, (uint32_t)k, Eincl, idx, gt);
+                }
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0)
+            logz_out[f] = logz;
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        p[i] = 0;
+}
+
+// -------------------------------------------------------------------------
+// host side
+// -------------------------------------------------------------------------
+static int model_const(int model, const double *th, ModelConst &mc)
+{
+    if (model != 1 && model != 2)
+        return set_error(MCKL_HIP_EINVAL, "unknown model %d", model);
+    if (!th)
+        return set_error(MCKL_HIP_EINVAL, "theta is NULL");
+    if (!(th[2] > 0) || !(th[1] > 0))
+        return set_error(MCKL_HIP_EINVAL, "scale parameters must be > 0");
+    if (model == 2 && !(std::fabs(th[0]) < 1))
+        return set_error(MCKL_HIP_EINVAL, "SV requires |phi| < 1");
+    const double l2pi = std::log(2.0 * 3.141592653589793);
+    mc.model = model;
+    mc.a = (float)th[0];
+    mc.s = (float)th[1];
+    mc.cst = (float)(-std::log(th[2]) - 0.5 * l2pi);
+    if (model == 1) {
+        mc.sd0 = 1.0f;
+        mc.inv_sy = (float)(1.0 / th[2]);
+    } else {
+        mc.sd0 = (float)std::sqrt(th[1] * th[1] / (1.0 - th[0] * th[0]));
+        mc.inv_sy = 0.0f;
+    }
+    return MCKL_HIP_OK;
+}
+
+static void obs_const(int model, const double *th, const double *y, size_t T,
+    float *out)
+{
+    for (size_t t = 0; t < T; ++t)
+        out[t] = model == 1 ? (float)y[t] : (float)(y[t] * y[t] / (th[2] * th[2]));
+}
+
+static int check_common(size_t N, size_t T, double threshold, int scheme)
+{
+    if (N == 0 || N >= (size_t)0x7FFFFFFF)
+        return set_error(MCKL_HIP_EINVAL, "particle count %zu out of range", N);
+    if (T == 0)
+        return set_error(MCKL_HIP_EINVAL, "T must be > 0");
+    if (scheme != MCKL_HIP_STRATIFIED && scheme != MCKL_HIP_SYSTEMATIC)
+        return set_error(MCKL_HIP_EINVAL,
+            "the samplers resample with the stratified or systematic scheme");
+    if (std::isnan(threshold))
+        return set_error(MCKL_HIP_EINVAL, "threshold is NaN");
+    return MCKL_HIP_OK;
+}
+
+template <typename T>
+static int dev_alloc(T **p, size_t n)
+{
+    *p = nullptr;
+    MCKL_HIP_TRY(hipMalloc(reinterpret_cast<void **>(p),
+        std::max<size_t>(n, 1) * sizeof(T)));
+    return MCKL_HIP_OK;
+}
+
+} // namespace mckl_hip
+
+using namespace mckl_hip;
+
+struct mckl_hip_smc {
+    SmcArgs A;
+    size_t T;
+    float *x[2] = {nullptr, nullptr};
+    float *lw = nullptr, *obs = nullptr, *tmax = nullptr;
+    uint32_t *idx = nullptr;
+    double *ts = nullptr, *ts2 = nullptr;
+    RsBuf B;
+    SmcState *st = nullptr;
+    double *ess = nullptr;
+    int32_t *res = nullptr;
+};
+
+static void smc_free(mckl_hip_smc *s)
+{
+    void *ptrs[] = {s->x[0], s->x[1], s->lw, s->obs, s->tmax, s->idx, s->ts,
+        s->ts2, s->B.tdh, s->B.tdl, s->B.toh, s->B.tol, s->B.te, s->B.tz,
+        s->B.teo, s->B.tzo, s->B.cnt, s->B.Eincl, s->st, s->ess, s->res};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    delete s;
+}
+
+extern "C" {
+
+int mckl_hip_smc_create(int model, const double *theta, size_t T,
+    const double *h_y, size_t N, uint64_t seed, double threshold, int scheme,
+    mckl_hip_smc **out)
+{
+    if (!out || !h_y)
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    *out = nullptr;
+    int rc = check_common(N, T, threshold, scheme);
+    if (rc)
+        return rc;
+    ModelConst mc;
+    rc = model_const(model, theta, mc);
+    if (rc)
+        return rc;
+    mckl_hip_smc *s = new mckl_hip_smc();
+    std::memset(&s->B, 0, sizeof(s->B));
+    s->T = T;
+    s->A.mc = mc;
+    s->A.k0 = (uint32_t)seed;
+    s->A.k1 = (uint32_t)(seed >> 32);
+    s->A.N = (uint32_t)N;
+    s->A.ntiles = (uint32_t)ceil_div(N, kTile);
+    s->A.tauN = threshold * (double)N;
+    s->A.scheme = scheme;
+    s->A.T = (uint32_t)T;
+    const size_t nt = s->A.ntiles;
+    const size_t Np = nt * kTile; // padded: vector accesses stay in bounds
+#define MCKL_ALLOC(p, n)                                                      \
+    do {                                                                      \
+        rc = dev_alloc(&(p), (n));                                            \
+        if (rc) {                                                             \
+            smc_free(s);                                                      \
+            return rc;                                                        \
+        }                                                                     \
+    } while (0)
+    MCKL_ALLOC(s->x[0], Np);
+    MCKL_ALLOC(s->x[1], Np);
+    MCKL_ALLOC(s->lw, Np);
+    MCKL_ALLOC(s->idx, Np);
+    MCKL_ALLOC(s->obs, T);
+    MCKL_ALLOC(s->tmax, nt);
+    MCKL_ALLOC(s->ts, nt);
+    MCKL_ALLOC(s->ts2, nt);
+    MCKL_ALLOC(s->B.tdh, nt);
+    MCKL_ALLOC(s->B.tdl, nt);
+    MCKL_ALLOC(s->B.toh, nt + 1);
+    MCKL_ALLOC(s->B.tol, nt + 1);
+    MCKL_ALLOC(s->B.te, nt);
+    MCKL_ALLOC(s->B.tz, nt);
+    MCKL_ALLOC(s->B.teo, nt + 1);
+    MCKL_ALLOC(s->B.tzo, nt + 1);
+    MCKL_ALLOC(s->B.cnt, Np);
+    MCKL_ALLOC(s->B.Eincl, Np);
+    MCKL_ALLOC(s->st, 1);
+    MCKL_ALLOC(s->ess, T);
+    MCKL_ALLOC(s->res, T);
+#undef MCKL_ALLOC
+    std::vector<float> ob(T);
+    obs_const(model, theta, h_y, T, ob.data());
+    hipError_t err = hipMemcpy(s->obs, ob.data(), T * sizeof(float),
+        hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemset(s->st, 0, sizeof(SmcState));
+    if (err != hipSuccess) {
+        rc = hip_status(err, "mckl_hip_smc_create");
+        smc_free(s);
+        return rc;
+    }
+    s->A.obs = s->obs;
+    *out = s;
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_smc_run(mckl_hip_smc *s, void *stream)
+{
+    if (!s)
+        return set_error(MCKL_HIP_EINVAL, "sampler is NULL");
+    hipStream_t st = as_stream(stream);
+    const dim3 grid(s->A.ntiles);
+    k_init<<<grid, kGroup, 0, st>>>(s->A, s->x[0], s->lw, s->st);
+    MCKL_HIP_CHECK_LAUNCH("k_init");
+    for (uint32_t t = 1; t <= s->T; ++t) {
+        const float *xin = s->x[(t - 1) & 1];
+        float *xout = s->x[t & 1];
+        k_move<<<grid, kGroup, 0, st>>>(
+            s->A, t, xin, xout, s->lw, s->idx, s->st, s->tmax);
+        k_sums<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->tmax, s->ts, s->ts2,
+            s->st, s->ess, s->res);
+        k_rs_wsum<<<grid, kGroup, 0, st>>>(s->A, s->lw, s->st, s->B);
+        k_rs_counts<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->st, s->B);
+        k_rs_index<<<grid, kGroup, 0, st>>>(s->A, s->st, s->B, s->idx);
+        k_rs_resolve<<<grid, kGroup, 0, st>>>(s->A, s->st, s->B, s->idx);
+    }
+    MCKL_HIP_CHECK_LAUNCH("smc step kernels");
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_smc_result(mckl_hip_smc *s, double *h_loglik, double *h_ess,
+    int32_t *h_resampled)
+{
+    if (!s)
+        return set_error(MCKL_HIP_EINVAL, "sampler is NULL");
+    MCKL_HIP_TRY(hipDeviceSynchronize());
+    if (h_loglik) {
+        SmcState hs;
+        MCKL_HIP_TRY(hipMemcpy(&hs, s->st, sizeof(hs), hipMemcpyDeviceToHost));
+        *h_loglik = hs.logz;
+    }
+    if (h_ess)
+        MCKL_HIP_TRY(hipMemcpy(h_ess, s->ess, s->T * sizeof(double),
+            hipMemcpyDeviceToHost));
+    if (h_resampled)
+        MCKL_HIP_TRY(hipMemcpy(h_resampled, s->res, s->T * sizeof(int32_t),
+            hipMemcpyDeviceToHost));
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_smc_destroy(mckl_hip_smc *s)
+{
+    if (s)
+        smc_free(s);
+    return MCKL_HIP_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched sampler host side
+// ---------------------------------------------------------------------------
+struct mckl_hip_smc_batch {
+    size_t nfilters = 0;
+    size_t T = 0;
+    FilterDesc *fd = nullptr;
+    uint32_t *order = nullptr;
+    uint32_t *queue = nullptr;
+    float *xa = nullptr, *xb = nullptr, *lw = nullptr, *obs = nullptr;
+    uint32_t *idx = nullptr, *cnt = nullptr, *E = nullptr;
+    double *logz = nullptr;
+    mckl_hip_tune tune{256, 0};
+    int persistent = 0; // workgroups in the persistent grid
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    int graph_tune = -1;
+};
+
+static void batch_free(mckl_hip_smc_batch *b)
+{
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    void *ptrs[] = {b->fd, b->order, b->queue, b->xa, b->xb, b->lw, b->obs,
+        b->idx, b->cnt, b->E, b->logz};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    delete b;
+}
+
+static int batch_launch(mckl_hip_smc_batch *b, hipStream_t st)
+{
+    k_zero_u32<<<1, 64, 0, st>>>(b->queue, 1);
+    const int G = std::max(1, b->tune.block / kGroup);
+    const uint32_t nf = (uint32_t)b->nfilters;
+    const dim3 grid((uint32_t)b->persistent);
+    switch (G) {
+        case 1:
+            k_batch<1><<<grid, kGroup, 0, st>>>(b->fd, b->order, nf, b->queue,
+                b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->logz);
+            break;
+        case 2:
+            k_batch<2><<<grid, 2 * kGroup, 0, st>>>(b->fd, b->order, nf,
+                b->queue, b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->logz);
+            break;
+        default:
+            k_batch<4><<<grid, 4 * kGroup, 0, st>>>(b->fd, b->order, nf,
+                b->queue, b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->logz);
+            break;
+    }
+    MCKL_HIP_CHECK_LAUNCH("k_batch");
+    return MCKL_HIP_OK;
+}
+
+static int set_persistent(mckl_hip_smc_batch *b)
+{
+    int dev = 0, ncu = 0;
+    MCKL_HIP_TRY(hipGetDevice(&dev));
+    MCKL_HIP_TRY(hipDeviceGetAttribute(
+        &ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int G = std::max(1, b->tune.block / kGroup);
+    int occ = 0;
+    hipError_t err;
+    switch (G) {
+        case 1:
+            err = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occ, k_batch<1>, kGroup, 0);
+            break;
+        case 2:
+            err = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occ, k_batch<2>, 2 * kGroup, 0);
+            break;
+        default:
+            err = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occ, k_batch<4>, 4 * kGroup, 0);
+            break;
+    }
+    MCKL_HIP_TRY(err);
+    int per_cu = std::max(1, occ);
+    if (b->tune.waves_per_eu > 0) // cap resident workgroups per CU
+        per_cu = std::min(per_cu,
+            std::max(1, b->tune.waves_per_eu * 4 / (G * kGroup / 64)));
+    b->persistent = (int)std::min<size_t>((size_t)ncu * per_cu, b->nfilters);
+    return MCKL_HIP_OK;
+}
+
+extern "C" {
+
+int mckl_hip_smc_batch_create(int model, size_t nfilters, const uint32_t *h_N,
+    const double *h_theta, size_t T, const double *h_y, const uint64_t *h_seed,
+    double threshold, int scheme, mckl_hip_smc_batch **out)
+{
+    if (!out || !h_N || !h_theta || !h_y || !h_seed)
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (nfilters == 0 || nfilters >= (1u << 31))
+        return set_error(MCKL_HIP_EINVAL, "nfilters out of range");
+    std::vector<FilterDesc> fd(nfilters);
+    std::vector<float> obs(nfilters * T);
+    uint64_t total = 0;
+    for (size_t f = 0; f < nfilters; ++f) {
+        const size_t N = h_N[f];
+        int rc = check_common(N, T, threshold, scheme);
+        if (rc)
+            return rc;
+        if (N > (size_t)kMaxTiles * kTile)
+            return set_error(MCKL_HIP_EINVAL,
+                "batched filters hold at most %d particles", kMaxTiles * kTile);
+        ModelConst mc;
+        rc = model_const(model, h_theta + 3 * f, mc);
+        if (rc)
+            return rc;
+        SmcArgs &A = fd[f].A;
+        A.mc = mc;
+        A.k0 = (uint32_t)h_seed[f];
+        A.k1 = (uint32_t)(h_seed[f] >> 32);
+        A.N = (uint32_t)N;
+        A.ntiles = (uint32_t)ceil_div(N, kTile);
+        A.tauN = threshold * (double)N;
+        A.scheme = scheme;
+        A.T = (uint32_t)T;
+        A.obs = nullptr; // set after upload
+        fd[f].off = total;
+        total += (uint64_t)A.ntiles * kTile;
+        obs_const(model, h_theta + 3 * f, h_y + T * f, T, obs.data() + T * f);
+    }
+    // largest filters first (longest-processing-time order)
+    std::vector<uint32_t> order(nfilters);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+        [&](uint32_t a, uint32_t b) { return h_N[a] > h_N[b]; });
+
+    mckl_hip_smc_batch *b = new mckl_hip_smc_batch();
+    b->nfilters = nfilters;
+    b->T = T;
+    int rc = MCKL_HIP_OK;
+#define MCKL_ALLOC(p, n)                                                      \
+    do {                                                                      \
+        rc = dev_alloc(&(p), (n));                                            \
+        if (rc) {                                                             \
+            batch_free(b);                                                    \
+            return rc;                                                        \
+        }                                                                     \
+    } while (0)
+    MCKL_ALLOC(b->fd, nfilters);
+    MCKL_ALLOC(b->order, nfilters);
+    MCKL_ALLOC(b->queue, 1);
+    MCKL_ALLOC(b->xa, total);
+    MCKL_ALLOC(b->xb, total);
+    MCKL_ALLOC(b->lw, total);
+    MCKL_ALLOC(b->idx, total);
+    MCKL_ALLOC(b->cnt, total);
+    MCKL_ALLOC(b->E, total);
+    MCKL_ALLOC(b->obs, nfilters * T);
+    MCKL_ALLOC(b->logz, nfilters);
+#undef MCKL_ALLOC
+    for (size_t f = 0; f < nfilters; ++f)
+        fd[f].A.obs = b->obs + T * f;
+    hipError_t err = hipMemcpy(b->fd, fd.data(), nfilters * sizeof(FilterDesc),
+        hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(b->order, order.data(), nfilters * sizeof(uint32_t),
+            hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(b->obs, obs.data(), obs.size() * sizeof(float),
+            hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+        rc = hip_status(err, "mckl_hip_smc_batch_create");
+        batch_free(b);
+        return rc;
+    }
+    rc = set_persistent(b);
+    if (rc) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_smc_batch_set_tune(mckl_hip_smc_batch *b, const mckl_hip_tune *t)
+{
+    if (!b || !t)
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    if (t->block != 256 && t->block != 512 && t->block != 1024)
+        return set_error(MCKL_HIP_EINVAL, "block must be 256, 512 or 1024");
+    if (t->waves_per_eu < 0 || t->waves_per_eu > 8)
+        return set_error(MCKL_HIP_EINVAL, "waves_per_eu must be in 0..8");
+    b->tune = *t;
+    return set_persistent(b);
+}
+
+int mckl_hip_smc_batch_run(mckl_hip_smc_batch *b, int use_graph, void *stream)
+{
+    if (!b)
+        return set_error(MCKL_HIP_EINVAL, "batch is NULL");
+    hipStream_t st = as_stream(stream);
+    if (!use_graph)
+        return batch_launch(b, st);
+    const int key = b->tune.block * 16 + b->tune.waves_per_eu;
+    if (!b->exec || b->graph_tune != key) {
+        if (b->exec) {
+            (void)hipGraphExecDestroy(b->exec);
+            b->exec = nullptr;
+        }
+        if (b->graph) {
+            (void)hipGraphDestroy(b->graph);
+            b->graph = nullptr;
+        }
+        hipStream_t cap;
+        MCKL_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipError_t err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        int rc = MCKL_HIP_OK;
+        if (err == hipSuccess) {
+            rc = batch_launch(b, cap);
+            err = hipStreamEndCapture(cap, &b->graph);
+        }
+        if (err == hipSuccess && rc == MCKL_HIP_OK)
+            err = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(cap);
+        if (rc)
+            return rc;
+        MCKL_HIP_TRY(err);
+        b->graph_tune = key;
+    }
+    MCKL_HIP_TRY(hipGraphLaunch(b->exec, st));
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_smc_batch_result(mckl_hip_smc_batch *b, double *h_loglik)
+{
+    if (!b || !h_loglik)
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    MCKL_HIP_TRY(hipDeviceSynchronize());
+    MCKL_HIP_TRY(hipMemcpy(h_loglik, b->logz, b->nfilters * sizeof(double),
+        hipMemcpyDeviceToHost));
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_smc_batch_destroy(mckl_hip_smc_batch *b)
+{
+    if (b)
+        batch_free(b);
+    return MCKL_HIP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,388 @@
+// smc_math.hpp -- device building blocks of the SMC samplers.
+//
+//  * float log/exp/sin-cos(2 pi u) with explicit fused multiply-adds in a
+//    fixed order: the same definitions as the CPU oracle, so both compute
+//    identical bits (no dependence on a vendor math library);
+//  * double-double (TwoSum) accumulation for the FP64 prefix sums;
+//  * the canonical per-tile reductions and scans.  A tile is 2048 particles
+//    handled by one group of 256 threads (4 wavefronts), 8 contiguous
+//    particles per thread.  Every sampler (plain, batched) reduces and scans
+//    through these functions and combines tiles with tiles_sum2 /
+//    tiles_scan_*, whose order depends only on the tile count, so results do
+//    not depend on how many tiles a launch processes concurrently.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "engines.hpp"
+
+namespace mckl_hip {
+
+constexpr int kGroup = 256;          // threads per tile group
+constexpr int kItems = 8;            // particles per thread
+constexpr int kTile = kGroup * kItems; // particles per tile
+constexpr int kWaves = kGroup / 64;
+
+__device__ __forceinline__ float f_as(uint32_t b) { return __uint_as_float(b); }
+__device__ __forceinline__ uint32_t u_as(float f) { return __float_as_uint(f); }
+
+// log(x), x positive normal.  x = 2^e m, m in [sqrt(1/2), sqrt(2)),
+// log(1+f) = f + f^2 P(f).
+__device__ __forceinline__ float smc_log_f(float x)
+{
+    const uint32_t b = u_as(x);
+    int e = (int)(b >> 23) - 127;
+    uint32_t mb = (b & 0x7FFFFFu) | 0x3F800000u;
+    if (mb > 0x3FB504F3u) {
+        mb -= 0x00800000u;
+        e += 1;
+    }
+    const float f = __fsub_rn(f_as(mb), 1.0f);
+    float p = -0x1.4237fcp-4f;
+    p = __fmaf_rn(p, f, 0x1.0696e4p-3f);
+    p = __fmaf_rn(p, f, -0x1.0c524cp-3f);
+    p = __fmaf_rn(p, f, 0x1.22973ap-3f);
+    p = __fmaf_rn(p, f, -0x1.548882p-3f);
+    p = __fmaf_rn(p, f, 0x1.99a3ecp-3f);
+    p = __fmaf_rn(p, f, -0x1.000206p-2f);
+    p = __fmaf_rn(p, f, 0x1.55554ep-2f);
+    p = __fmaf_rn(p, f, -0x1.fffffep-2f);
+    const float f2 = __fmul_rn(f, f);
+    const float r = __fmaf_rn(f2, p, f);
+    const float ef = (float)e;
+    return __fmaf_rn(ef, 0x1.62e430p-1f, __fmaf_rn(ef, -0x1.05c610p-29f, r));
+}
+
+// exp(x); 0 below -87 (deterministic underflow), +inf above 88.
+__device__ __forceinline__ float smc_exp_f(float x)
+{
+    if (x < -87.0f)
+        return 0.0f;
+    if (x > 88.0f)
+        return __builtin_huge_valf();
+    const float k = rintf(__fmul_rn(x, 0x1.715476p+0f));
+    float r = __fmaf_rn(-k, 0x1.62e430p-1f, x);
+    r = __fmaf_rn(-k, -0x1.05c610p-29f, r);
+    float q = 0x1.a1516cp-13f;
+    q = __fmaf_rn(q, r, 0x1.6d431ap-10f);
+    q = __fmaf_rn(q, r, 0x1.1110c6p-7f);
+    q = __fmaf_rn(q, r, 0x1.5554eap-5f);
+    q = __fmaf_rn(q, r, 0x1.555556p-3f);
+    q = __fmaf_rn(q, r, 0x1.000000p-1f);
+    const float p = __fadd_rn(1.0f, __fmaf_rn(__fmul_rn(r, r), q, r));
+    return __fmul_rn(p, f_as((uint32_t)((int)k + 127) << 23));
+}
+
+// sin(2 pi u), cos(2 pi u), u in [0, 1]
+__device__ __forceinline__ void smc_sincos2pi_f(float u, float &s, float &c)
+{
+    const float q = rintf(__fmul_rn(u, 4.0f));
+    const float r = __fsub_rn(u, __fmul_rn(q, 0.25f));
+    const float ph = __fmul_rn(r, 0x1.921fb6p+2f);
+    const float p2 = __fmul_rn(ph, ph);
+    float sp = 0x1.6dbd24p-19f;
+    sp = __fmaf_rn(sp, p2, -0x1.a013a4p-13f);
+    sp = __fmaf_rn(sp, p2, 0x1.11110ep-7f);
+    sp = __fmaf_rn(sp, p2, -0x1.555556p-3f);
+    const float sn = __fmaf_rn(__fmul_rn(ph, p2), sp, ph);
+    float cp = -0x1.2469bap-22f;
+    cp = __fmaf_rn(cp, p2, 0x1.a0125ap-16f);
+    cp = __fmaf_rn(cp, p2, -0x1.6c16bap-10f);
+    cp = __fmaf_rn(cp, p2, 0x1.555556p-5f);
+    cp = __fmaf_rn(cp, p2, -0x1.000000p-1f);
+    const float cs = __fmaf_rn(p2, cp, 1.0f);
+    switch ((int)q & 3) {
+        case 0:
+            s = sn;
+            c = cs;
+            break;
+        case 1:
+            s = cs;
+            c = -sn;
+            break;
+        case 2:
+            s = -sn;
+            c = -cs;
+            break;
+        default:
+            s = -cs;
+            c = sn;
+            break;
+    }
+}
+
+// Normal pair of particles 2k, 2k+1 at step t: Philox4x32 block at counter
+// {k, t} under key {seed lo, seed hi}; even = r cos, odd = r sin.
+__device__ __forceinline__ void smc_normal_pair(
+    uint32_t k0, uint32_t k1, uint64_t k, uint64_t t, float &e0, float &e1)
+{
+    uint32_t s0 = (uint32_t)k, s1 = (uint32_t)(k >> 32);
+    uint32_t s2 = (uint32_t)t, s3 = (uint32_t)(t >> 32);
+    philox4x32_10(k0, k1, s0, s1, s2, s3);
+    const float u1 = u01_f32(2, s0);
+    const float u2 = u01_f32(2, s1);
+    const float rad = __fsqrt_rn(__fmul_rn(-2.0f, smc_log_f(u1)));
+    float sn, cs;
+    smc_sincos2pi_f(u2, sn, cs);
+    e0 = __fmul_rn(rad, cs);
+    e1 = __fmul_rn(rad, sn);
+}
+
+// Canonical uniform j of step t (resampling): counter {2^63 + j/2, t}
+__device__ __forceinline__ double smc_uniform(
+    uint32_t k0, uint32_t k1, uint64_t t, uint64_t j)
+{
+    const uint64_t c = 0x8000000000000000ull + (j >> 1);
+    uint32_t s0 = (uint32_t)c, s1 = (uint32_t)(c >> 32);
+    uint32_t s2 = (uint32_t)t, s3 = (uint32_t)(t >> 32);
+    philox4x32_10(k0, k1, s0, s1, s2, s3);
+    return (j & 1) ? u01_canonical_f64(s2, s3) : u01_canonical_f64(s0, s1);
+}
+
+// ---- model arithmetic (float, one rounding per operation) ----
+struct ModelConst {
+    int model;      // 1 LG, 2 SV
+    float a, s;     // transition
+    float sd0;      // initial scale
+    float inv_sy;   // LG observation precision scale
+    float cst;      // constant of log g
+};
+
+__device__ __forceinline__ float smc_move(const ModelConst &m, float x, float e)
+{
+    return __fadd_rn(__fmul_rn(m.a, x), __fmul_rn(m.s, e));
+}
+
+// obs = (float)y for LG, (float)(y^2/beta^2) for SV
+__device__ __forceinline__ float smc_logg(const ModelConst &m, float obs, float x)
+{
+    if (m.model == 1) {
+        const float d = __fsub_rn(obs, x);
+        const float z = __fmul_rn(d, m.inv_sy);
+        return __fadd_rn(__fmul_rn(-0.5f, __fmul_rn(z, z)), m.cst);
+    }
+    const float q = __fmul_rn(obs, smc_exp_f(-x));
+    return __fadd_rn(__fsub_rn(__fmul_rn(-0.5f, q), __fmul_rn(0.5f, x)), m.cst);
+}
+
+// ---- double-double ----
+struct dd {
+    double hi, lo;
+};
+
+__device__ __forceinline__ dd dd_add(dd a, dd b)
+{
+    const double s = __dadd_rn(a.hi, b.hi);
+    const double bb = __dsub_rn(s, a.hi);
+    const double err = __dadd_rn(__dsub_rn(a.hi, __dsub_rn(s, bb)),
+        __dsub_rn(b.hi, bb));
+    const double lo = __dadd_rn(err, __dadd_rn(a.lo, b.lo));
+    const double h = __dadd_rn(s, lo);
+    return dd{h, __dsub_rn(lo, __dsub_rn(h, s))};
+}
+
+__device__ __forceinline__ dd dd_shfl_up(dd v, int d)
+{
+    return dd{__shfl_up(v.hi, d, 64), __shfl_up(v.lo, d, 64)};
+}
+
+// ---- group (256-thread) reductions; `lds` is this group's scratch ----
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1)
+        v = fmaxf(v, __shfl_xor(v, k, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1)
+        v = __dadd_rn(v, __shfl_xor(v, k, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v)
+{
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1)
+        v += __shfl_xor(v, k, 64);
+    return v;
+}
+
+// group max; result valid in all threads of the group (gt = thread index
+// within the group, lds >= 4 floats)
+__device__ __forceinline__ float group_max(float v, int gt, float *lds)
+{
+    v = wave_max(v);
+    if ((gt & 63) == 0)
+        lds[gt >> 6] = v;
+    __syncthreads();
+    const float r = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
+    __syncthreads();
+    return r;
+}
+
+// group sum of two doubles in the canonical order: sequential per thread
+// (done by the caller), butterfly per wave, (w0 + w1) + (w2 + w3).
+__device__ __forceinline__ void group_sum2(
+    double &a, double &b, int gt, double *lds)
+{
+    a = wave_sum_d(a);
+    b = wave_sum_d(b);
+    if ((gt & 63) == 0) {
+        lds[gt >> 6] = a;
+        lds[4 + (gt >> 6)] = b;
+    }
+    __syncthreads();
+    a = __dadd_rn(__dadd_rn(lds[0], lds[1]), __dadd_rn(lds[2], lds[3]));
+    b = __dadd_rn(__dadd_rn(lds[4], lds[5]), __dadd_rn(lds[6], lds[7]));
+    __syncthreads();
+}
+
+// exclusive scan of a u32 per thread over the group; returns the exclusive
+// prefix, `total` receives the group sum.  lds >= 4 u32.
+__device__ __forceinline__ uint32_t group_excl_scan_u(
+    uint32_t v, int gt, uint32_t *lds, uint32_t &total)
+{
+    const int lane = gt & 63;
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d)
+            incl += o;
+    }
+    if (lane == 63)
+        lds[gt >> 6] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+    const int w = gt >> 6;
+    for (int i = 0; i < w; ++i)
+        off += lds[i];
+    total = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return off + incl - v;
+}
+
+// exclusive dd scan per thread value over the group (fixed Hillis-Steele
+// per wave, waves in order).  lds >= 8 doubles.
+__device__ __forceinline__ dd group_excl_scan_dd(
+    dd v, int gt, double *lds, dd &total)
+{
+    const int lane = gt & 63;
+    dd incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const dd o = dd_shfl_up(incl, d);
+        if (lane >= d)
+            incl = dd_add(o, incl);
+    }
+    // exclusive within the wave: the inclusive value of lane-1
+    dd ex = dd_shfl_up(incl, 1);
+    if (lane == 0)
+        ex = dd{0.0, 0.0};
+    if (lane == 63) {
+        lds[2 * (gt >> 6)] = incl.hi;
+        lds[2 * (gt >> 6) + 1] = incl.lo;
+    }
+    __syncthreads();
+    const int w = gt >> 6;
+    dd off{0.0, 0.0};
+    for (int i = 0; i < w; ++i)
+        off = dd_add(off, dd{lds[2 * i], lds[2 * i + 1]});
+    dd tot{0.0, 0.0};
+    for (int i = 0; i < kWaves; ++i)
+        tot = dd_add(tot, dd{lds[2 * i], lds[2 * i + 1]});
+    total = tot;
+    __syncthreads();
+    return w == 0 ? ex : (lane == 0 ? off : dd_add(off, ex));
+}
+
+// ---- cross-tile combination, canonical for a given tile count ----
+// Tiles are dealt to the 256 threads of one group in contiguous chunks of
+// ceil(n/256); each thread combines its chunk sequentially, then the group
+// combines thread results with the group functions above.  Plain (many
+// workgroups) and batched (one workgroup per filter) samplers both call these
+// with the same n, so they produce the same bits.
+
+// sums of (s, s2) over n tiles; result in every thread of the group
+__device__ __forceinline__ void tiles_sum2(const double *s, const double *s2,
+    int n, int gt, double *lds, double &rs, double &rs2)
+{
+    const int chunk = (n + kGroup - 1) / kGroup;
+    const int b = gt * chunk;
+    const int e = b + chunk < n ? b + chunk : n;
+    double a = 0.0, c = 0.0;
+    for (int i = b; i < e; ++i) {
+        a = __dadd_rn(a, s[i]);
+        c = __dadd_rn(c, s2[i]);
+    }
+    group_sum2(a, c, gt, lds);
+    rs = a;
+    rs2 = c;
+}
+
+// exclusive dd scan over n tile totals (hi/lo arrays); writes off[0..n],
+// off[n] = total.  Arrays may live in LDS or global memory.
+__device__ __forceinline__ void tiles_scan_dd(bool write, const double *hi,
+    const double *lo, int n, double *ohi, double *olo, int gt, double *lds)
+{
+    const int chunk = (n + kGroup - 1) / kGroup;
+    const int b = gt * chunk;
+    const int e = b + chunk < n ? b + chunk : n;
+    dd loc{0.0, 0.0};
+    for (int i = b; i < e; ++i)
+        loc = dd_add(loc, dd{hi[i], lo[i]});
+    dd total;
+    dd off = group_excl_scan_dd(loc, gt, lds, total);
+    if (write) {
+        for (int i = b; i < e; ++i) {
+            ohi[i] = off.hi;
+            olo[i] = off.lo;
+            off = dd_add(off, dd{hi[i], lo[i]});
+        }
+        if (gt == 0) {
+            ohi[n] = total.hi;
+            olo[n] = total.lo;
+        }
+    }
+    __syncthreads();
+}
+
+// exclusive u32 scans of two arrays over n tiles; off arrays get n+1 entries
+__device__ __forceinline__ void tiles_scan_u2(bool write, const uint32_t *a,
+    const uint32_t *b2, int n, uint32_t *oa, uint32_t *ob, int gt,
+    uint32_t *lds)
+{
+    const int chunk = (n + kGroup - 1) / kGroup;
+    const int b = gt * chunk;
+    const int e = b + chunk < n ? b + chunk : n;
+    uint32_t la = 0, lb = 0;
+    for (int i = b; i < e; ++i) {
+        la += a[i];
+        lb += b2[i];
+    }
+    uint32_t ta, tb;
+    uint32_t xa = group_excl_scan_u(la, gt, lds, ta);
+    uint32_t xb = group_excl_scan_u(lb, gt, lds + 4, tb);
+    if (write) {
+        for (int i = b; i < e; ++i) {
+            oa[i] = xa;
+            ob[i] = xb;
+            xa += a[i];
+            xb += b2[i];
+        }
+        if (gt == 0) {
+            oa[n] = ta;
+            ob[n] = tb;
+        }
+    }
+    __syncthreads();
+}
+
+} // namespace mckl_hip

@@ -1,0 +1,392 @@
+// smc_tiles.hpp -- per-tile stages of one SMC step, shared by the plain
+// (one workgroup per tile, many launches) and the batched (one workgroup per
+// filter, one launch) samplers.  Each function is executed by one group of
+// 256 threads; `gt` is the thread index within the group and `lds` points to
+// that group's scratch.  A tile is particles [tile*2048, tile*2048+2048),
+// 8 contiguous particles per thread.
+//
+// Step t (t = 1..T) of the sampler, following SMCSampler::do_iterate
+// (algorithm/smc.hpp:389-406):
+//   move     x_i = a x_{p(i)} + s eps(i,t), lw_i = carry_i + log g(y_t | x_i)
+//            p(i) = idx_i and carry_i = 0 after a resampling step, else
+//            p(i) = i and carry_i = lw_i - m_{t-1}            -> tile max
+//   sums     e_i = exp_f(lw_i - m_t), S = sum e, S2 = sum e^2 -> tile sums
+//   finalize logZ += m_t + log S_t - log C_t, ESS = S^2/S2 (smc.hpp:397)
+//   resample (ESS < threshold N) W_i = e_i / S, FP64 prefix sums in
+//            double-double, replication counts (resample.hpp:374-407) and
+//            the stable rep->index transform (resample.hpp:416-466).
+#pragma once
+
+#include "smc_math.hpp"
+
+namespace mckl_hip {
+
+struct SmcArgs {
+    ModelConst mc;
+    uint32_t k0, k1; // particle RNG key: seed lo, seed hi
+    uint32_t N;
+    uint32_t ntiles;
+    double tauN; // resampling threshold * N
+    int scheme;  // 2 stratified, 3 systematic
+    uint32_t T;
+    const float *obs; // T observation constants
+};
+
+constexpr uint32_t kZeroFlag = 0x80000000u;
+
+__device__ __forceinline__ uint64_t tile_first(uint32_t tile, int gt)
+{
+    return (uint64_t)tile * kTile + (uint64_t)gt * kItems;
+}
+
+// t = 0: x_i = sd0 eps(i, 0), lw_i = 0
+__device__ __forceinline__ void tile_init(
+    const SmcArgs &A, uint32_t tile, float *x, float *lw, int gt)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+#pragma unroll
+    for (int k = 0; k < kItems; k += 2) {
+        const uint64_t i = i0 + k;
+        if (i >= A.N)
+            break;
+        float e0, e1;
+        smc_normal_pair(A.k0, A.k1, i >> 1, 0, e0, e1);
+        x[i] = __fmul_rn(A.mc.sd0, e0);
+        lw[i] = 0.0f;
+        if (i + 1 < A.N) {
+            x[i + 1] = __fmul_rn(A.mc.sd0, e1);
+            lw[i + 1] = 0.0f;
+        }
+    }
+}
+
+// move + weight of one tile; returns the tile max of lw (all threads)
+__device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
+    uint32_t tile, const float *xin, float *xout, float *lw,
+    const uint32_t *idx, int resampled, float mprev, int gt, float *ldsf)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    const float obs = A.obs[t - 1];
+    float lmax = -__builtin_huge_valf();
+    if (i0 + kItems <= A.N) {
+        float xp[kItems], lo[kItems];
+        if (resampled) {
+            const uint4 a0 = *reinterpret_cast<const uint4 *>(idx + i0);
+            const uint4 a1 = *reinterpret_cast<const uint4 *>(idx + i0 + 4);
+            const uint32_t a[kItems] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y,
+                a1.z, a1.w};
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                xp[k] = xin[a[k]];
+                lo[k] = 0.0f;
+            }
+        } else {
+            const float4 v0 = *reinterpret_cast<const float4 *>(xin + i0);
+            const float4 v1 = *reinterpret_cast<const float4 *>(xin + i0 + 4);
+            const float4 w0 = *reinterpret_cast<const float4 *>(lw + i0);
+            const float4 w1 = *reinterpret_cast<const float4 *>(lw + i0 + 4);
+            const float xs[kItems] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y,
+                v1.z, v1.w};
+            const float ws[kItems] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y,
+                w1.z, w1.w};
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                xp[k] = xs[k];
+                lo[k] = __fsub_rn(ws[k], mprev);
+            }
+        }
+        float xn[kItems], ln[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k += 2) {
+            float e0, e1;
+            smc_normal_pair(A.k0, A.k1, (i0 + k) >> 1, t, e0, e1);
+            xn[k] = smc_move(A.mc, xp[k], e0);
+            xn[k + 1] = smc_move(A.mc, xp[k + 1], e1);
+            ln[k] = __fadd_rn(lo[k], smc_logg(A.mc, obs, xn[k]));
+            ln[k + 1] = __fadd_rn(lo[k + 1], smc_logg(A.mc, obs, xn[k + 1]));
+            lmax = fmaxf(lmax, fmaxf(ln[k], ln[k + 1]));
+        }
+        *reinterpret_cast<float4 *>(xout + i0) =
+            make_float4(xn[0], xn[1], xn[2], xn[3]);
+        *reinterpret_cast<float4 *>(xout + i0 + 4) =
+            make_float4(xn[4], xn[5], xn[6], xn[7]);
+        *reinterpret_cast<float4 *>(lw + i0) =
+            make_float4(ln[0], ln[1], ln[2], ln[3]);
+        *reinterpret_cast<float4 *>(lw + i0 + 4) =
+            make_float4(ln[4], ln[5], ln[6], ln[7]);
+    } else {
+        for (int k = 0; k < kItems; k += 2) {
+            const uint64_t i = i0 + k;
+            if (i >= A.N)
+                break;
+            float e[2];
+            smc_normal_pair(A.k0, A.k1, i >> 1, t, e[0], e[1]);
+            for (int q = 0; q < 2; ++q) {
+                const uint64_t p = i + q;
+                if (p >= A.N)
+                    break;
+                float xv, carry;
+                if (resampled) {
+                    xv = xin[idx[p]];
+                    carry = 0.0f;
+                } else {
+                    xv = xin[p];
+                    carry = __fsub_rn(lw[p], mprev);
+                }
+                const float xnew = smc_move(A.mc, xv, e[q]);
+                const float l = __fadd_rn(carry, smc_logg(A.mc, obs, xnew));
+                xout[p] = xnew;
+                lw[p] = l;
+                lmax = fmaxf(lmax, l);
+            }
+        }
+    }
+    return group_max(lmax, gt, ldsf);
+}
+
+// S = sum e_i, S2 = sum e_i^2 over the tile, e_i = exp_f(lw_i - m)
+__device__ __forceinline__ void tile_sums(const SmcArgs &A, uint32_t tile,
+    const float *lw, float m, int gt, double *ldsd, double &S, double &S2)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    double s = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i < A.N) {
+            const double e = (double)smc_exp_f(__fsub_rn(lw[i], m));
+            s = __dadd_rn(s, e);
+            s2 = __dadd_rn(s2, __dmul_rn(e, e));
+        }
+    }
+    group_sum2(s, s2, gt, ldsd);
+    S = s;
+    S2 = s2;
+}
+
+// normalised weight W_i = (double)e_i * (1/S)  (core/weight.hpp:221)
+__device__ __forceinline__ double smc_weight(float lwi, float m, double invS)
+{
+    return __dmul_rn((double)smc_exp_f(__fsub_rn(lwi, m)), invS);
+}
+
+// dd total of W over the tile
+__device__ __forceinline__ dd tile_wsum(const SmcArgs &A, uint32_t tile,
+    const float *lw, float m, double invS, int gt, double *ldsd)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    dd loc{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i < A.N)
+            loc = dd_add(loc, dd{smc_weight(lw[i], m, invS), 0.0});
+    }
+    dd total;
+    (void)group_excl_scan_dd(loc, gt, ldsd, total);
+    return total;
+}
+
+// Uniform sequences of step t (sorted, M = N):
+//   systematic u_j = j*delta + u0d  (resample.hpp:122-135), u0d = u01*delta
+//   stratified u_j = delta*(c_j + j) (resample.hpp:95-120)
+struct UniformSeq {
+    int scheme;
+    uint32_t k0, k1;
+    uint64_t t;
+    uint32_t M;
+    double delta, u0d;
+
+    __device__ __forceinline__ double operator()(uint64_t j) const
+    {
+        if (scheme == 3)
+            return __dadd_rn(__dmul_rn((double)j, delta), u0d);
+        return __dmul_rn(delta, __dadd_rn(smc_uniform(k0, k1, t, j), (double)j));
+    }
+
+    // #{j < M : u_j < S}
+    __device__ __forceinline__ uint32_t count_less(double S) const
+    {
+        double g = floor(__dmul_rn(S, (double)M));
+        uint32_t c = g <= 0.0 ? 0u : (g >= (double)M ? M : (uint32_t)g);
+        while (c > 0 && (*this)(c - 1) >= S)
+            --c;
+        while (c < M && (*this)(c) < S)
+            ++c;
+        return c;
+    }
+};
+
+// Prefix sums S_j of W within the tile plus the tile offset, and the counts
+// cnt_j = #{u < S_j}.  The last particle of a tile takes S from the next tile
+// offset so that neighbouring tiles agree on boundary values; the last
+// particle overall takes cnt = M (resample.hpp:405).  Returns the tile sums
+// of the extra-copy and zero-slot counts through esum/zsum.
+__device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
+    const float *lw, float m, double invS, const double *toffh,
+    const double *toffl, const UniformSeq &U, uint32_t *cnt, int gt,
+    double *ldsd, uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    const uint64_t tend = (uint64_t)(tile + 1) * kTile < A.N
+        ? (uint64_t)(tile + 1) * kTile
+        : A.N;
+    double W[kItems];
+    dd loc{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        W[k] = i < A.N ? smc_weight(lw[i], m, invS) : 0.0;
+        loc = dd_add(loc, dd{W[k], 0.0});
+    }
+    dd total;
+    const dd toff = group_excl_scan_dd(loc, gt, ldsd, total);
+    const dd base = dd_add(dd{toffh[tile], toffl[tile]}, toff);
+    uint32_t c[kItems];
+    dd run{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        run = dd_add(run, dd{W[k], 0.0});
+        c[k] = 0;
+        if (i < A.N) {
+            if (i == (uint64_t)A.N - 1) {
+                c[k] = U.M;
+            } else {
+                const double S = i == tend - 1 ? toffh[tile + 1]
+                                               : dd_add(base, run).hi;
+                c[k] = U.count_less(S);
+            }
+            cnt[i] = c[k];
+        }
+    }
+    // count before the first particle of this thread
+    ldsu[gt] = c[kItems - 1];
+    __syncthreads();
+    uint32_t prev;
+    if (gt > 0)
+        prev = ldsu[gt - 1];
+    else
+        prev = tile == 0 ? 0u : U.count_less(toffh[tile]);
+    __syncthreads();
+    uint32_t es = 0, zs = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i < A.N) {
+            const uint32_t rep = c[k] - prev;
+            es += rep > 1 ? rep - 1 : 0;
+            zs += rep == 0 ? 1 : 0;
+        }
+        prev = c[k];
+    }
+    uint32_t te, tz;
+    (void)group_excl_scan_u(es, gt, ldsu, te);
+    (void)group_excl_scan_u(zs, gt, ldsu + 4, tz);
+    esum = te;
+    zsum = tz;
+}
+
+// Stable rep->index (resample.hpp:416-466, M = N): idx_i = i when rep_i > 0;
+// zero slots take extra copies in ascending source order.  Writes the
+// inclusive extra-copy prefix E_i and, for zero slots, kZeroFlag | rank.
+__device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
+    const uint32_t *cnt, uint32_t eoff, uint32_t zoff, uint32_t *Eincl,
+    uint32_t *idx, int gt, uint32_t *ldsu)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    uint32_t rep[kItems];
+    uint32_t es = 0, zs = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        rep[k] = 0;
+        if (i < A.N) {
+            const uint32_t p = i == 0 ? 0u : cnt[i - 1];
+            rep[k] = cnt[i] - p;
+            es += rep[k] > 1 ? rep[k] - 1 : 0;
+            zs += rep[k] == 0 ? 1 : 0;
+        }
+    }
+    uint32_t te, tz;
+    uint32_t ex = group_excl_scan_u(es, gt, ldsu, te) + eoff;
+    uint32_t zx = group_excl_scan_u(zs, gt, ldsu + 4, tz) + zoff;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i < A.N) {
+            ex += rep[k] > 1 ? rep[k] - 1 : 0;
+            Eincl[i] = ex;
+            if (rep[k] > 0) {
+                idx[i] = (uint32_t)i;
+            } else {
+                idx[i] = kZeroFlag | zx;
+                ++zx;
+            }
+        }
+    }
+}
+
+// first j with E[j] > z (E nondecreasing, length n)
+__device__ __forceinline__ uint32_t upper_bound_u(
+    const uint32_t *E, uint32_t n, uint32_t z)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (E[mid] > z)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void tile_resolve(
+    const SmcArgs &A, uint32_t tile, const uint32_t *Eincl, uint32_t *idx,
+    int gt)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i < A.N) {
+            const uint32_t v = idx[i];
+            if (v & kZeroFlag)
+                idx[i] = upper_bound_u(Eincl, A.N, v & ~kZeroFlag);
+        }
+    }
+}
+
+// finalize one step from the reduced sums; returns the resampling flag
+struct StepOut {
+    double logz_inc, ess;
+    int resample;
+};
+
+__device__ __forceinline__ StepOut step_finalize(const SmcArgs &A, float m,
+    double S, double S2, double C)
+{
+    StepOut o;
+    o.logz_inc = __dsub_rn(__dadd_rn((double)m, log(S)), log(C));
+    o.ess = __ddiv_rn(__dmul_rn(S, S), S2);
+    o.resample = o.ess < A.tauN ? 1 : 0;
+    return o;
+}
+
+__device__ __forceinline__ UniformSeq make_uniform_seq(
+    const SmcArgs &A, uint64_t t)
+{
+    UniformSeq U;
+    U.scheme = A.scheme;
+    U.k0 = A.k0;
+    U.k1 = A.k1;
+    U.t = t;
+    U.M = A.N;
+    U.delta = __ddiv_rn(1.0, (double)A.N);
+    U.u0d = A.scheme == 3 ? __dmul_rn(smc_uniform(A.k0, A.k1, t, 0), U.delta)
+                          : 0.0;
+    return U;
+}
+
+} // namespace mckl_hip

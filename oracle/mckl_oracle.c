@@ -803,37 +803,138 @@ void mo_resample(int scheme, size_t N, size_t M, mo_engine *e,
 /* This project's SMC particle filter, restated on the CPU.               */
 /*                                                                        */
 /* Loop shape follows SMCSampler::do_iterate (algorithm/smc.hpp:389-406): */
-/* move+weight, ESS test against threshold*N (:397), resample, with the   */
-/* ResampleAlgorithm and rep->index transform above.  The random-number   */
-/* layout, the FP32 state/log-weight arithmetic and the log-likelihood    */
+/* move+weight, ESS test against threshold*N (:397), resample with the    */
+/* ResampleAlgorithm and rep->index transform restated above.  The        */
+/* random-number layout, the FP32 arithmetic and the log-likelihood       */
 /* accounting are this project's own design (the reference has no         */
 /* log-likelihood API, core/weight.hpp:198-227); the device kernels       */
 /* implement the same definitions:                                        */
-/*   eps(i,t)  = Box-Muller cos branch of Philox4x32 block at counter     */
-/*               {i, t} under key {seed lo, seed hi}; words 0,1 -> U01OC. */
-/*   resampling uniforms at step t: Philox4x32 blocks at counter          */
-/*               {2^63 + b, t}; Canonical doubles from word pairs.        */
-/*   every transcendental is evaluated in double and rounded once.        */
+/*   eps(i,t): Philox4x32 block at counter {i/2, t}, key {seed lo, hi};   */
+/*     u1 = U01OC(w0), u2 = U01OC(w1); r = sqrt(-2 log_f(u1));            */
+/*     eps = r cos(2 pi u2) for even i, r sin(2 pi u2) for odd i.         */
+/*   resampling uniforms of step t: Philox4x32 blocks at counter          */
+/*     {2^63 + j/2, t}, Canonical doubles from words (0,1) / (2,3).       */
+/*   log_f, exp_f, sincos2pi_f: float polynomials with explicit fused     */
+/*     multiply-adds in a fixed order (identical bits on CPU and GPU).    */
+/*   logZ += m_t + log S_t - log C_t, C_t = N after a resampling step     */
+/*     (and at t = 1), else S_{t-1}; S_t = sum exp_f(lw_i - m_t).         */
 /* ===================================================================== */
-static float mo_log_f(float x) { return (float)log((double)x); }
-static float mo_cos_f(float x) { return (float)cos((double)x); }
-static float mo_exp_f(float x) { return (float)exp((double)x); }
+static float mo_asf(uint32_t b)
+{
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+static uint32_t mo_asu(float f)
+{
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    return b;
+}
+
+/* log(x) for positive normal x: x = 2^e m, m in [sqrt(1/2), sqrt(2)),
+ * log(1+f) = f + f^2 P(f), P degree 8 (least squares on Chebyshev nodes). */
+float mo_log_f(float x)
+{
+    uint32_t b = mo_asu(x);
+    int e = (int)(b >> 23) - 127;
+    uint32_t mb = (b & 0x7FFFFFu) | 0x3F800000u;
+    if (mb > 0x3FB504F3u) {
+        mb -= 0x00800000u;
+        e += 1;
+    }
+    const float f = mo_asf(mb) - 1.0f;
+    float p = -0x1.4237fcp-4f;
+    p = fmaf(p, f, 0x1.0696e4p-3f);
+    p = fmaf(p, f, -0x1.0c524cp-3f);
+    p = fmaf(p, f, 0x1.22973ap-3f);
+    p = fmaf(p, f, -0x1.548882p-3f);
+    p = fmaf(p, f, 0x1.99a3ecp-3f);
+    p = fmaf(p, f, -0x1.000206p-2f);
+    p = fmaf(p, f, 0x1.55554ep-2f);
+    p = fmaf(p, f, -0x1.fffffep-2f);
+    const float f2 = f * f;
+    const float r = fmaf(f2, p, f);
+    const float ef = (float)e;
+    return fmaf(ef, 0x1.62e430p-1f, fmaf(ef, -0x1.05c610p-29f, r));
+}
+
+/* exp(x): k = rint(x log2 e), r = x - k ln2 (two fmas),
+ * exp(r) = 1 + r + r^2 Q(r), Q degree 5; underflow to 0 below -87. */
+float mo_exp_f(float x)
+{
+    if (x < -87.0f)
+        return 0.0f;
+    if (x > 88.0f)
+        return INFINITY;
+    const float k = rintf(x * 0x1.715476p+0f);
+    float r = fmaf(-k, 0x1.62e430p-1f, x);
+    r = fmaf(-k, -0x1.05c610p-29f, r);
+    float q = 0x1.a1516cp-13f;
+    q = fmaf(q, r, 0x1.6d431ap-10f);
+    q = fmaf(q, r, 0x1.1110c6p-7f);
+    q = fmaf(q, r, 0x1.5554eap-5f);
+    q = fmaf(q, r, 0x1.555556p-3f);
+    q = fmaf(q, r, 0x1.000000p-1f);
+    const float p = 1.0f + fmaf(r * r, q, r);
+    return p * mo_asf((uint32_t)((int)k + 127) << 23);
+}
+
+/* sin(2 pi u), cos(2 pi u) for u in [0, 1]: exact quadrant reduction
+ * u = q/4 + r, |r| <= 1/8, then odd/even polynomials in phi = 2 pi r. */
+void mo_sincos2pi_f(float u, float *s, float *c)
+{
+    const float q = rintf(u * 4.0f);
+    const float r = u - q * 0.25f;
+    const float ph = r * 0x1.921fb6p+2f;
+    const float p2 = ph * ph;
+    float sp = 0x1.6dbd24p-19f;
+    sp = fmaf(sp, p2, -0x1.a013a4p-13f);
+    sp = fmaf(sp, p2, 0x1.11110ep-7f);
+    sp = fmaf(sp, p2, -0x1.555556p-3f);
+    const float sn = fmaf(ph * p2, sp, ph);
+    float cp = -0x1.2469bap-22f;
+    cp = fmaf(cp, p2, 0x1.a0125ap-16f);
+    cp = fmaf(cp, p2, -0x1.6c16bap-10f);
+    cp = fmaf(cp, p2, 0x1.555556p-5f);
+    cp = fmaf(cp, p2, -0x1.000000p-1f);
+    const float cs = fmaf(p2, cp, 1.0f);
+    switch ((int)q & 3) {
+        case 0:
+            *s = sn;
+            *c = cs;
+            break;
+        case 1:
+            *s = cs;
+            *c = -sn;
+            break;
+        case 2:
+            *s = -sn;
+            *c = -cs;
+            break;
+        default:
+            *s = -cs;
+            *c = sn;
+            break;
+    }
+}
 
 float mo_smc_normal(uint64_t seed, uint64_t i, uint64_t t)
 {
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-    uint64_t ctr[2] = {i, t};
+    uint64_t ctr[2] = {i >> 1, t};
     uint32_t w[4];
     mo_philox4x32_block(key, ctr, w);
-    float u1 = mo_u01_u32_f32(MO_OC, w[0]);
-    float u2 = mo_u01_u32_f32(MO_OC, w[1]);
-    float l = mo_log_f(u1);
-    float s = sqrtf(l * -2.0f);
-    float th = 6.283185307179586476925286766559005768394f * u2;
-    return s * mo_cos_f(th);
+    const float u1 = mo_u01_u32_f32(MO_OC, w[0]);
+    const float u2 = mo_u01_u32_f32(MO_OC, w[1]);
+    const float rad = sqrtf(-2.0f * mo_log_f(u1));
+    float s, c;
+    mo_sincos2pi_f(u2, &s, &c);
+    return (i & 1) ? rad * s : rad * c;
 }
 
-static double mo_smc_uniform(uint64_t seed, uint64_t t, uint64_t j)
+double mo_smc_uniform(uint64_t seed, uint64_t t, uint64_t j)
 {
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint64_t ctr[2] = {0x8000000000000000ull + j / 2, t};
@@ -844,44 +945,41 @@ static double mo_smc_uniform(uint64_t seed, uint64_t t, uint64_t j)
 }
 
 typedef struct {
-    float a, s;        /* transition x' = a x + s eps */
-    float sd0;         /* initial x0 = sd0 eps */
-    float inv_sy, cst; /* LG: observation scale; constant of log g */
-    double beta;       /* SV */
+    float a, s, sd0, inv_sy, cst;
 } mo_model;
 
 static void mo_model_init(int model, const double *th, mo_model *m)
 {
     const double l2pi = log(2.0 * 3.141592653589793);
+    m->a = (float)th[0];
+    m->s = (float)th[1];
+    m->cst = (float)(-log(th[2]) - 0.5 * l2pi);
     if (model == MO_MODEL_LG) {
-        m->a = (float)th[0];
-        m->s = (float)th[1];
         m->sd0 = 1.0f;
         m->inv_sy = (float)(1.0 / th[2]);
-        m->cst = (float)(-log(th[2]) - 0.5 * l2pi);
-        m->beta = 0;
     } else {
-        m->a = (float)th[0];
-        m->s = (float)th[1];
         m->sd0 = (float)sqrt(th[1] * th[1] / (1.0 - th[0] * th[0]));
-        m->inv_sy = 0;
-        m->cst = (float)(-log(th[2]) - 0.5 * l2pi);
-        m->beta = th[2];
+        m->inv_sy = 0.0f;
     }
 }
 
-static float mo_model_logg(int model, const mo_model *m, double y, float ysq,
-    float yf, float x)
+/* per-step observation constant: LG (float)y, SV (float)(y^2 / beta^2) */
+float mo_smc_obs(int model, const double *th, double y)
 {
+    return model == MO_MODEL_LG ? (float)y : (float)(y * y / (th[2] * th[2]));
+}
+
+float mo_smc_logg(int model, const double *th, float obs, float x)
+{
+    mo_model m;
+    mo_model_init(model, th, &m);
     if (model == MO_MODEL_LG) {
-        float d = yf - x;
-        float z = d * m->inv_sy;
-        return -0.5f * (z * z) + m->cst;
+        const float d = obs - x;
+        const float z = d * m.inv_sy;
+        return -0.5f * (z * z) + m.cst;
     }
-    float e = mo_exp_f(-x);
-    float q = ysq * e;
-    (void)y;
-    return ((-0.5f * q) - (0.5f * x)) + m->cst;
+    const float q = obs * mo_exp_f(-x);
+    return ((-0.5f * q) - (0.5f * x)) + m.cst;
 }
 
 double mo_smc_run(int model, const double *theta, const double *y, size_t T,
@@ -893,6 +991,7 @@ double mo_smc_run(int model, const double *theta, const double *y, size_t T,
     float *x = (float *)malloc(N * sizeof(float));
     float *xn = (float *)malloc(N * sizeof(float));
     float *lw = (float *)malloc(N * sizeof(float));
+    float *e = (float *)malloc(N * sizeof(float));
     double *W = (double *)malloc(N * sizeof(double));
     double *u = (double *)malloc(N * sizeof(double));
     uint64_t *rep = (uint64_t *)malloc(N * sizeof(uint64_t));
@@ -903,23 +1002,21 @@ double mo_smc_run(int model, const double *theta, const double *y, size_t T,
 #else
     (void)nthreads;
 #endif
-    long long Ns = (long long)N;
+    const long long Ns = (long long)N;
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < Ns; ++i) {
         x[i] = m.sd0 * mo_smc_normal(seed, (uint64_t)i, 0);
         lw[i] = 0.0f;
     }
+    /* t = 1 carries lw = 0 with m_0 = 0, so C_1 = S_0 = N */
     float mprev = 0.0f;
     int resampled = 0;
+    double Sprev = (double)N;
     double logz = 0;
     for (size_t t = 1; t <= T; ++t) {
-        const double yt = y[t - 1];
-        const float yf = (float)yt;
-        const float ysq = model == MO_MODEL_SV
-            ? (float)(yt * yt / (m.beta * m.beta))
-            : 0.0f;
-        double C = 0;
-#pragma omp parallel for schedule(static) reduction(+ : C)
+        const float obs = mo_smc_obs(model, theta, y[t - 1]);
+        const double C = resampled ? (double)N : Sprev;
+#pragma omp parallel for schedule(static)
         for (long long i = 0; i < Ns; ++i) {
             float xp, carry;
             if (resampled) {
@@ -929,11 +1026,10 @@ double mo_smc_run(int model, const double *theta, const double *y, size_t T,
                 xp = x[i];
                 carry = lw[i] - mprev;
             }
-            float eps = mo_smc_normal(seed, (uint64_t)i, t);
-            float xv = m.a * xp + m.s * eps;
+            const float eps = mo_smc_normal(seed, (uint64_t)i, t);
+            const float xv = m.a * xp + m.s * eps;
             xn[i] = xv;
-            lw[i] = carry + mo_model_logg(model, &m, yt, ysq, yf, xv);
-            C += exp((double)carry);
+            lw[i] = carry + mo_smc_logg(model, theta, obs, xv);
         }
         float *tmp = x;
         x = xn;
@@ -942,26 +1038,29 @@ double mo_smc_run(int model, const double *theta, const double *y, size_t T,
         for (size_t i = 0; i < N; ++i)
             if (mx < lw[i])
                 mx = lw[i];
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < Ns; ++i)
+            e[i] = mo_exp_f(lw[i] - mx);
         double S = 0, S2 = 0;
         for (size_t i = 0; i < N; ++i) {
-            double e = exp((double)lw[i] - (double)mx);
-            W[i] = e;
-            S += e;
-            S2 += e * e;
+            const double d = (double)e[i];
+            S += d;
+            S2 += d * d;
         }
         logz += ((double)mx + log(S)) - log(C);
         const double ess = S * S / S2;
         if (ess_hist)
             ess_hist[t - 1] = ess;
         mprev = mx;
+        Sprev = S;
         resampled = ess < threshold * (double)N;
         if (resampled_hist)
             resampled_hist[t - 1] = resampled;
         if (resampled) {
-            const double inv = 1 / S;
+            const double inv = 1.0 / S;
             for (size_t i = 0; i < N; ++i)
-                W[i] = inv * W[i];
-            const double delta = 1 / (double)N;
+                W[i] = (double)e[i] * inv;
+            const double delta = 1.0 / (double)N;
             if (scheme == MO_SYSTEMATIC) {
                 const double uu = mo_smc_uniform(seed, t, 0) * delta;
                 for (size_t j = 0; j < N; ++j)
@@ -983,6 +1082,7 @@ double mo_smc_run(int model, const double *theta, const double *y, size_t T,
     free(x);
     free(xn);
     free(lw);
+    free(e);
     free(W);
     free(u);
     free(rep);

@@ -106,7 +106,13 @@ void mo_resample(int scheme, size_t n, size_t m, mo_engine *e,
 
 /* ---- this project's SMC (device algorithm restated on the CPU) ---- */
 enum { MO_MODEL_LG = 1, MO_MODEL_SV = 2 };
+float mo_log_f(float x);
+float mo_exp_f(float x);
+void mo_sincos2pi_f(float u, float *s, float *c);
 float mo_smc_normal(uint64_t seed, uint64_t i, uint64_t t);
+double mo_smc_uniform(uint64_t seed, uint64_t t, uint64_t j);
+float mo_smc_obs(int model, const double *theta, double y);
+float mo_smc_logg(int model, const double *theta, float obs, float x);
 double mo_smc_run(int model, const double *theta, const double *y, size_t T,
     size_t N, uint64_t seed, double threshold, int scheme, int nthreads,
     double *ess_hist, int *resampled_hist);

@@ -1,0 +1,159 @@
+"""GPU Weight and resampling primitives against the compiled-reference
+fixtures and the oracle.
+
+Replication counts come from FP64 prefix sums.  The device sums in
+double-double (correctly rounded), the reference sums sequentially in FP64;
+a count can therefore differ only where a uniform lies within the
+reference's own rounding error of a cumulative-weight boundary.  Each case
+reports the number of such differences and asserts every one is explained
+that way; the rep->index transform must then be bit-exact given rep.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _m():
+    import mckl_amd as m
+    return m
+
+
+def _t(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def boundary_explained(w, u, rep_ref, rep_gpu):
+    """Every replication difference is a uniform within the sequential
+    sum's error bound (N * eps * S) of an exact cumulative weight."""
+    diff = np.nonzero(rep_ref.astype(np.int64) != rep_gpu.astype(np.int64))[0]
+    if len(diff) == 0:
+        return 0, 0
+    exact = np.cumsum(np.asarray(w, np.longdouble))
+    N = len(w)
+    tol = N * np.finfo(np.float64).eps
+    near = 0
+    within_1e15 = 0
+    for i in diff:
+        S = float(exact[min(i, N - 1)])
+        Sp = float(exact[i - 1]) if i > 0 else 0.0
+        d = min(np.min(np.abs(u - S)) / S if S else 1.0,
+                np.min(np.abs(u - Sp)) / Sp if Sp else 1.0)
+        assert d <= tol, (i, d)
+        near += 1
+        within_1e15 += d <= 1e-15
+    return near, within_1e15
+
+
+def test_resample_fixtures(dev):
+    import torch
+    m = _m()
+    fx = np.load(os.path.join(GOLDEN, "resample_cases.npz"))
+    k = 0
+    total_near = 0
+    while "c%d_meta" % k in fx:
+        p = "c%d" % k
+        scheme, N, M, k0 = [int(v) for v in fx[p + "_meta"]]
+        w = _t(fx[p + "_w"], dev)
+        rep = torch.zeros(N, dtype=torch.int32, device=dev)
+        idx = torch.zeros(M, dtype=torch.int32, device=dev)
+        rng = m.Philox4x32(key=[k0, 0])
+        alg = {0: m.ResampleMultinomial, 2: m.ResampleStratified,
+               3: m.ResampleSystematic}[scheme]()
+        alg(N, M, rng, w, rep, idx)
+        # uniform sequence itself
+        rng2 = m.Philox4x32(key=[k0, 0])
+        u = torch.zeros(max(M, 1), dtype=torch.float64, device=dev)
+        m.u01_rand_sequence(scheme, rng2, M, u)
+        ug = u.cpu().numpy()[:M]
+        if scheme == 0:
+            # sorted uniforms: prefix of log(u)/(N-j), rounding may differ
+            assert np.allclose(ug, fx[p + "_u"], rtol=1e-12, atol=1e-15)
+        else:
+            assert np.array_equal(ug, fx[p + "_u"])
+        rg = rep.cpu().numpy().astype(np.uint64)
+        near, _ = boundary_explained(fx[p + "_w"], fx[p + "_u"], fx[p + "_rep"],
+                                     rg)
+        total_near += near
+        if near == 0:
+            assert np.array_equal(rg, fx[p + "_rep"])
+            assert np.array_equal(idx.cpu().numpy().astype(np.uint64),
+                                  fx[p + "_idx"])
+        k += 1
+    print("replication differences at cumulative-weight boundaries:",
+          total_near)
+
+
+@pytest.mark.parametrize("N,M", [(1, 1), (1, 5), (5, 1), (2, 2), (1000, 1000),
+                                 (4097, 5000), (5000, 4097), (70000, 70000)])
+def test_rep_index_vs_oracle(dev, N, M):
+    import torch
+    m = _m()
+    rng = np.random.default_rng(N * 31 + M)
+    w = rng.random(N) ** 5
+    rep = rng.multinomial(M, w / w.sum()).astype(np.uint32)
+    idx = torch.zeros(M, dtype=torch.int32, device=dev)
+    m.resample_trans_rep_index(N, M, _t(rep.view(np.int32), dev), idx)
+    want = orc.rep_index(rep.astype(np.uint64), M)
+    assert np.array_equal(idx.cpu().numpy().view(np.uint32).astype(np.uint64),
+                          want)
+
+
+def test_trans_u01_rep_vs_oracle(dev):
+    import torch
+    m = _m()
+    rng = np.random.default_rng(11)
+    for N, M in ((2, 2), (1000, 1000), (3000, 7000), (7000, 3000), (1, 9)):
+        w = rng.random(N)
+        w /= w.sum()
+        u = np.sort(rng.random(M))
+        rep = torch.zeros(N, dtype=torch.int32, device=dev)
+        m.resample_trans_u01_rep(N, M, _t(w, dev), _t(u, dev), rep)
+        rg = rep.cpu().numpy().astype(np.uint64)
+        ro = orc.trans_u01_rep(w, u)
+        near, _ = boundary_explained(w, u, ro, rg)
+        assert int(rg.sum()) == M
+        if near == 0:
+            assert np.array_equal(rg, ro)
+
+
+def test_weight_fixtures(dev):
+    m = _m()
+    fx = np.load(os.path.join(GOLDEN, "weight_cases.npz"))
+    k = 0
+    while "w%d_v" % k in fx:
+        v = fx["w%d_v" % k]
+        n = len(v)
+        W = m.Weight(n)
+        W.set_log(_t(v, dev))
+        got = W.data.cpu().numpy()
+        assert np.allclose(got, fx["w%d_W" % k], rtol=1e-13, atol=0)
+        assert abs(W.ess() - fx["w%d_ess" % k][0]) <= 1e-12 * n
+        W.add_log(_t(fx["w%d_v2" % k], dev))
+        got2 = W.data.cpu().numpy()
+        assert np.allclose(got2, fx["w%d_W2" % k], rtol=1e-12, atol=0)
+        assert abs(W.ess() - fx["w%d_ess2" % k][0]) <= 1e-11 * n
+        _, _, lnc = orc.weight_add_log(fx["w%d_W" % k], fx["w%d_v2" % k])
+        assert abs(W.log_nc() - lnc) < 1e-12 * max(1.0, abs(lnc))
+        k += 1
+
+
+def test_select(dev):
+    import torch
+    m = _m()
+    src = torch.arange(3000, dtype=torch.float32, device=dev)
+    idx = torch.tensor(np.random.default_rng(1).integers(0, 1000, 777),
+                       dtype=torch.int32, device=dev)
+    dst = torch.empty(777 * 3, dtype=torch.float32, device=dev)
+    m.select(idx, src, dst, dim=3)
+    i = idx.cpu().numpy()
+    want = np.stack([3 * i, 3 * i + 1, 3 * i + 2], 1).reshape(-1)
+    assert np.array_equal(dst.cpu().numpy(), want.astype(np.float32))

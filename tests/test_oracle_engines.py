@@ -1,0 +1,123 @@
+"""Pin the oracle's engines: the reference's own KAT files, FIPS-197, and the
+compiled reference (oracle/_ref) for ragged bulk calls, scalar calls and
+discard.  CPU only."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+ENGINES = [
+    ("philox4x32", orc.PHILOX4X32, 32),
+    ("philox4x32_64", orc.PHILOX4X32, 64),
+    ("threefry4x64", orc.THREEFRY4X64, 32),
+    ("threefry4x64_64", orc.THREEFRY4X64, 64),
+    ("ars", orc.ARS, 32),
+    ("ars_64", orc.ARS, 64),
+]
+
+needs_ref = pytest.mark.skipif(orc.REF is None,
+                               reason="oracle/_ref not built here")
+
+
+@pytest.mark.parametrize("name,kind,rbits", ENGINES)
+def test_kat_files(name, kind, rbits):
+    """example/random/include/random_rng.hpp:80-104: a default engine
+    (key {1,0,..}, counter 0) bulk-fills exactly the KAT words."""
+    kat = np.array([int(l) for l in open(os.path.join(
+        GOLDEN, "random_%s.data" % name))], dtype=np.uint64)
+    assert len(kat) == 1000
+    e = orc.engine(kind, [1], rbits)
+    got = orc.rand_u32(e, 1000) if rbits == 32 else orc.rand_u64(e, 1000)
+    assert np.array_equal(got.astype(np.uint64), kat)
+
+
+def test_aes_round_fips197():
+    """FIPS-197 Appendix C.1 (the vector example/random/src/random_aes.cpp
+    checks): AES-128 built from the same round function as ARS."""
+    key = bytes(range(16))
+    pt = bytes.fromhex("00112233445566778899aabbccddeeff")
+    assert orc.aes128(key, pt).hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
+
+
+def test_aes_tables_closed_form():
+    t0 = (ctypes.c_uint32 * 256)()
+    sb = (ctypes.c_uint8 * 256)()
+    orc.O.mo_aes_tables(t0, sb)
+    # S-box spot values (FIPS-197 Figure 7) and bijectivity
+    assert sb[0x00] == 0x63 and sb[0x53] == 0xED and sb[0xFF] == 0x16
+    assert len(set(sb)) == 256
+    # T0 = (2S, S, S, 3S) little-endian; aes_constants.hpp:42 starts 0xA56363C6
+    assert t0[0] == 0xA56363C6 and t0[1] == 0x847C7CF8
+
+
+def test_philox_counter_words():
+    """The 128-bit counter enters as four little-endian u32 words: a counter
+    in the high word must differ from the same value in the low word."""
+    a = orc.philox_block([1, 2], [5, 0])
+    b = orc.philox_block([1, 2], [0, 5])
+    assert a != b
+    e = orc.engine(orc.PHILOX4X32, [1, 2])
+    assert list(orc.rand_u32(e, 4)) == orc.philox_block([1, 2], [1, 0])
+
+
+@needs_ref
+@pytest.mark.parametrize("name,kind,rbits", ENGINES)
+@pytest.mark.parametrize("seq", [
+    [1, 1, 1, 7, 0, 0, 4096, 3],
+    [5, -3, 0, 17, -1000, 2, 1],
+    [0, 0, 0, 0, 0, 9, -7, 65],
+    [12345, -2 ** 20, 77],
+])
+def test_engine_ops_vs_reference(name, kind, rbits, seq):
+    key = [0xA5A5A5A5, 0x5A5A5A5A, 7, 9]
+    ref, rctr, ridx = orc.ref_engine_ops(kind, rbits, key, seq)
+    e = orc.engine(kind, key, rbits)
+    out = []
+    for s in seq:
+        if s > 0:
+            out.append(orc.rand_u32(e, s) if rbits == 32 else orc.rand_u64(e, s))
+        elif s < 0:
+            orc.O.mo_discard(ctypes.byref(e), ctypes.c_uint64(-s))
+        else:
+            out.append(orc.rand_u32(e, 1) if rbits == 32 else orc.rand_u64(e, 1))
+    got = np.concatenate(out) if out else np.zeros(0)
+    assert np.array_equal(got, ref)
+    assert orc.state(e) == (rctr, ridx)
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", [orc.PHILOX4X32, orc.THREEFRY4X64, orc.ARS])
+def test_counter_carry_vs_reference(kind):
+    """Bulk fills that cross 2^64 in the low counter word (the reference
+    splits its SIMD loop there: random/internal/philox_sse2_4x32.hpp:71-92)."""
+    key = [3, 1, 4, 1]
+    start = [2 ** 64 - 5, 2 ** 64 - 1, 1, 0]
+    ref, rctr, ridx = orc.ref_engine_ops(kind, 32, key, [3, 64, 1], ctr0=start)
+    e = orc.engine(kind, key)
+    c = (ctypes.c_uint64 * 4)(*start)
+    orc.O.mo_engine_set_ctr(ctypes.byref(e), c)
+    got = np.concatenate([orc.rand_u32(e, 3), orc.rand_u32(e, 64),
+                          orc.rand_u32(e, 1)])
+    assert np.array_equal(got, ref)
+    assert orc.state(e) == (rctr, ridx)
+
+
+@needs_ref
+def test_discard_carry_quirk_vs_reference():
+    """discard's increment(ctr, n) carries when ctr[0] == max - n exactly
+    (increment_generic.hpp:208-217 tests with '<'): the oracle matches."""
+    key = [1, 2]
+    M = 4
+    for lo in (2 ** 64 - 1 - 3, 2 ** 64 - 1 - 2, 2 ** 64 - 1):
+        ref, rctr, ridx = orc.ref_engine_ops(orc.PHILOX4X32, 32, key,
+                                             [-(3 * M + 1), 2], ctr0=[lo, 0])
+        e = orc.engine(orc.PHILOX4X32, key)
+        orc.O.mo_engine_set_ctr(ctypes.byref(e), (ctypes.c_uint64 * 4)(lo, 0))
+        orc.O.mo_discard(ctypes.byref(e), ctypes.c_uint64(3 * M + 1))
+        got = orc.rand_u32(e, 2)
+        assert np.array_equal(got, ref) and orc.state(e) == (rctr, ridx)
