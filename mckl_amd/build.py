@@ -67,12 +67,17 @@ def build_host_test():
     if not os.path.exists(src):
         return None
     out = os.path.join(ROOT, "tests", "cpp", "host_api_test")
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     _run([
         "g++", "-std=c++14", "-O2", "-Wall",
+        "-D__HIP_PLATFORM_AMD__",
         "-I", os.path.join(ROOT, "include"),
         "-I", os.path.join(HERE, "include"),
+        "-I", os.path.join(rocm, "include"),
         src, "-o", out,
-        "-L", HERE, "-lmckl_hip", "-Wl,-rpath," + HERE,
+        "-L", HERE, "-lmckl_hip", "-Wl,-rpath,$ORIGIN/../../mckl_amd",
+        "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+        "-Wl,-rpath," + os.path.join(rocm, "lib"),
     ])
     return out
 

@@ -276,6 +276,13 @@ struct ArsGen {
     }
 };
 
+// Correctly rounded square roots.  (HIP's __fsqrt_rn is the approximate
+// hardware square root unless OCML's rounded operations are enabled; the
+// builtins lower to the correctly rounded sequences under the default
+// -fhip-fp32-correctly-rounded-divide-sqrt.)
+__device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double sqrt_rn(double x) { return __builtin_sqrt(x); }
+
 // ------------------------------------------------------------------------
 // U01 conversions, exact fixed point: random/internal/u01_generic.hpp:48-164
 // (CC/CO/OC/OO) and :166-198 (Canonical, two u32 per double).

@@ -66,28 +66,38 @@ __global__ __launch_bounds__(kGroup) void k_init(
     }
 }
 
+struct RsBuf {
+    double *tdh, *tdl;   // tile dd totals of e
+    double *toh, *tol;   // exclusive tile offsets (ntiles + 1)
+    uint32_t *te, *tz;   // tile extra-copy / zero-slot counts
+    uint32_t *teo, *tzo; // their exclusive offsets (ntiles + 1)
+    uint32_t *cnt, *Eincl;
+};
+
 __global__ __launch_bounds__(kGroup) void k_move(SmcArgs A, uint32_t t,
     const float *xin, float *xout, float *lw, const uint32_t *idx,
-    const SmcState *st, float *tmax)
+    const uint32_t *Eincl, const SmcState *st, float *tmax)
 {
     __shared__ __align__(16) unsigned char lds[kScratch];
     const GroupLds L = group_lds(lds);
     const int res = st->resampled;
     const float mprev = st->m;
-    const float mt = tile_move(A, t, blockIdx.x, xin, xout, lw, idx, res,
-        mprev, threadIdx.x, L.f);
+    const float mt = tile_move(A, t, blockIdx.x, xin, xout, lw, idx, Eincl,
+        res, mprev, threadIdx.x, L.f);
     if (threadIdx.x == 0)
         tmax[blockIdx.x] = mt;
 }
 
-// sums per tile; the last workgroup to finish reduces across tiles and
-// takes the step's decisions
+// Sums per tile.  The last workgroup to finish reduces across tiles, takes
+// the step's decisions and, when the step resamples, scans the tile totals
+// of e for the cumulative weights.
 __global__ __launch_bounds__(kGroup) void k_sums(SmcArgs A, uint32_t t,
-    const float *lw, const float *tmax, double *ts, double *ts2, SmcState *st,
-    double *ess_hist, int32_t *res_hist)
+    const float *lw, const float *tmax, double *ts, double *ts2, RsBuf B,
+    SmcState *st, double *ess_hist, int32_t *res_hist)
 {
     __shared__ __align__(16) unsigned char lds[kScratch];
     __shared__ int last;
+    __shared__ int resample;
     const GroupLds L = group_lds(lds);
     const int gt = threadIdx.x;
     float m = -__builtin_huge_valf();
@@ -95,10 +105,13 @@ __global__ __launch_bounds__(kGroup) void k_sums(SmcArgs A, uint32_t t,
         m = fmaxf(m, tmax[i]);
     m = group_max(m, gt, L.f);
     double S, S2;
-    tile_sums(A, blockIdx.x, lw, m, gt, L.d, S, S2);
+    dd et;
+    tile_sums(A, blockIdx.x, lw, m, gt, L.d, S, S2, et);
     if (gt == 0) {
         ts[blockIdx.x] = S;
         ts2[blockIdx.x] = S2;
+        B.tdh[blockIdx.x] = et.hi;
+        B.tdl[blockIdx.x] = et.lo;
         __threadfence();
         last = atomicAdd(&st->counter, 1u) == A.ntiles - 1;
     }
@@ -117,45 +130,15 @@ __global__ __launch_bounds__(kGroup) void k_sums(SmcArgs A, uint32_t t,
         st->resampled = o.resample;
         st->C = o.resample ? (double)A.N : rs;
         st->counter = 0;
+        resample = o.resample;
         if (ess_hist)
             ess_hist[t - 1] = o.ess;
         if (res_hist)
             res_hist[t - 1] = o.resample;
     }
-}
-
-struct RsBuf {
-    double *tdh, *tdl;   // tile dd sums
-    double *toh, *tol;   // exclusive tile offsets (ntiles + 1)
-    uint32_t *te, *tz;   // tile extra-copy / zero-slot counts
-    uint32_t *teo, *tzo; // their exclusive offsets (ntiles + 1)
-    uint32_t *cnt, *Eincl;
-};
-
-__global__ __launch_bounds__(kGroup) void k_rs_wsum(
-    SmcArgs A, const float *lw, SmcState *st, RsBuf B)
-{
-    if (!st->resampled)
-        return;
-    __shared__ __align__(16) unsigned char lds[kScratch];
-    __shared__ int last;
-    const GroupLds L = group_lds(lds);
-    const int gt = threadIdx.x;
-    const double invS = __ddiv_rn(1.0, st->S);
-    const dd tot = tile_wsum(A, blockIdx.x, lw, st->m, invS, gt, L.d);
-    if (gt == 0) {
-        B.tdh[blockIdx.x] = tot.hi;
-        B.tdl[blockIdx.x] = tot.lo;
-        __threadfence();
-        last = atomicAdd(&st->counter, 1u) == A.ntiles - 1;
-    }
     __syncthreads();
-    if (!last)
-        return;
-    __threadfence();
-    tiles_scan_dd(true, B.tdh, B.tdl, A.ntiles, B.toh, B.tol, gt, L.d);
-    if (gt == 0)
-        st->counter = 0;
+    if (resample)
+        tiles_scan_dd(true, B.tdh, B.tdl, A.ntiles, B.toh, B.tol, gt, L.d);
 }
 
 __global__ __launch_bounds__(kGroup) void k_rs_counts(
@@ -196,14 +179,6 @@ __global__ __launch_bounds__(kGroup) void k_rs_index(
     const GroupLds L = group_lds(lds);
     tile_index(A, blockIdx.x, B.cnt, B.teo[blockIdx.x], B.tzo[blockIdx.x],
         B.Eincl, idx, threadIdx.x, L.u);
-}
-
-__global__ __launch_bounds__(kGroup) void k_rs_resolve(
-    SmcArgs A, const SmcState *st, RsBuf B, uint32_t *idx)
-{
-    if (!st->resampled)
-        return;
-    tile_resolve(A, blockIdx.x, B.Eincl, idx, threadIdx.x);
 }
 
 // -------------------------------------------------------------------------
@@ -271,7 +246,7 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
             for (int r = 0; r < rounds; ++r) {
                 const int k = r * G + grp;
                 const uint32_t kk = (uint32_t)(k < nt ? k : nt);
-                const float mt = tile_move(A, t, kk, xin, xout, lw, idx,
+                const float mt = tile_move(A, t, kk, xin, xout, lw, idx, Eincl,
                     resampled, mprev, gt, L.f);
                 if (gt == 0 && k < nt)
                     S.tmax[k] = mt;
@@ -283,10 +258,14 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
             for (int r = 0; r < rounds; ++r) {
                 const int k = r * G + grp;
                 double s, s2;
-                tile_sums(A, (uint32_t)(k < nt ? k : nt), lw, m, gt, L.d, s, s2);
+                dd et;
+                tile_sums(A, (uint32_t)(k < nt ? k : nt), lw, m, gt, L.d, s, s2,
+                    et);
                 if (gt == 0 && k < nt) {
                     S.ts[k] = s;
                     S.ts2[k] = s2;
+                    S.tdh[k] = et.hi;
+                    S.tdl[k] = et.lo;
                 }
                 __syncthreads();
             }
@@ -299,16 +278,6 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
             resampled = o.resample;
             if (resampled) {
                 const double invS = __ddiv_rn(1.0, rs);
-                for (int r = 0; r < rounds; ++r) {
-                    const int k = r * G + grp;
-                    const dd tot = tile_wsum(A, (uint32_t)(k < nt ? k : nt),
-                        lw, m, invS, gt, L.d);
-                    if (gt == 0 && k < nt) {
-                        S.tdh[k] = tot.hi;
-                        S.tdl[k] = tot.lo;
-                    }
-                    __syncthreads();
-                }
                 tiles_scan_dd(grp == 0, S.tdh, S.tdl, nt, S.toh, S.tol, gt, L.d);
                 const UniformSeq U = make_uniform_seq(A, t);
                 for (int r = 0; r < rounds; ++r) {
@@ -330,12 +299,6 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
                         Eincl, idx, gt, L.u);
                     __syncthreads();
                 }
-                for (int r = 0; r < rounds; ++r) {
-                    const int k = r * G + grp;
-                    if (k < nt)
-                        tile_resolve(A, (uint32_t)k, Eincl, idx, gt);
-                }
-                __syncthreads();
             }
         }
         if (threadIdx.x == 0)
@@ -423,10 +386,16 @@ struct mckl_hip_smc {
     SmcState *st = nullptr;
     double *ess = nullptr;
     int32_t *res = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 static void smc_free(mckl_hip_smc *s)
 {
+    if (s->exec)
+        (void)hipGraphExecDestroy(s->exec);
+    if (s->graph)
+        (void)hipGraphDestroy(s->graph);
     void *ptrs[] = {s->x[0], s->x[1], s->lw, s->obs, s->tmax, s->idx, s->ts,
         s->ts2, s->B.tdh, s->B.tdl, s->B.toh, s->B.tol, s->B.te, s->B.tz,
         s->B.teo, s->B.tzo, s->B.cnt, s->B.Eincl, s->st, s->ess, s->res};
@@ -511,11 +480,10 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
     return MCKL_HIP_OK;
 }
 
-int mckl_hip_smc_run(mckl_hip_smc *s, void *stream)
+// Enqueue the whole run: init, then per step move, sums (+decision), counts
+// and index (the last two exit at once on steps that do not resample).
+static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
 {
-    if (!s)
-        return set_error(MCKL_HIP_EINVAL, "sampler is NULL");
-    hipStream_t st = as_stream(stream);
     const dim3 grid(s->A.ntiles);
     k_init<<<grid, kGroup, 0, st>>>(s->A, s->x[0], s->lw, s->st);
     MCKL_HIP_CHECK_LAUNCH("k_init");
@@ -523,15 +491,43 @@ int mckl_hip_smc_run(mckl_hip_smc *s, void *stream)
         const float *xin = s->x[(t - 1) & 1];
         float *xout = s->x[t & 1];
         k_move<<<grid, kGroup, 0, st>>>(
-            s->A, t, xin, xout, s->lw, s->idx, s->st, s->tmax);
+            s->A, t, xin, xout, s->lw, s->idx, s->B.Eincl, s->st, s->tmax);
         k_sums<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->tmax, s->ts, s->ts2,
-            s->st, s->ess, s->res);
-        k_rs_wsum<<<grid, kGroup, 0, st>>>(s->A, s->lw, s->st, s->B);
+            s->B, s->st, s->ess, s->res);
         k_rs_counts<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->st, s->B);
         k_rs_index<<<grid, kGroup, 0, st>>>(s->A, s->st, s->B, s->idx);
-        k_rs_resolve<<<grid, kGroup, 0, st>>>(s->A, s->st, s->B, s->idx);
     }
     MCKL_HIP_CHECK_LAUNCH("smc step kernels");
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_smc_run(mckl_hip_smc *s, void *stream)
+{
+    if (!s)
+        return set_error(MCKL_HIP_EINVAL, "sampler is NULL");
+    hipStream_t st = as_stream(stream);
+    // The T-step launch sequence is captured once into a HIP graph and
+    // replayed; a run contains no host synchronisation.
+    if (!s->exec) {
+        hipStream_t cap;
+        MCKL_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipError_t err =
+            hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        int rc = MCKL_HIP_OK;
+        if (err == hipSuccess) {
+            rc = smc_enqueue(s, cap);
+            hipGraph_t g = nullptr;
+            err = hipStreamEndCapture(cap, &g);
+            s->graph = g;
+        }
+        if (err == hipSuccess && rc == MCKL_HIP_OK)
+            err = hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(cap);
+        if (rc)
+            return rc;
+        MCKL_HIP_TRY(err);
+    }
+    MCKL_HIP_TRY(hipGraphLaunch(s->exec, st));
     return MCKL_HIP_OK;
 }
 

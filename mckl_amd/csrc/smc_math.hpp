@@ -122,7 +122,7 @@ __device__ __forceinline__ void smc_normal_pair(
     philox4x32_10(k0, k1, s0, s1, s2, s3);
     const float u1 = u01_f32(2, s0);
     const float u2 = u01_f32(2, s1);
-    const float rad = __fsqrt_rn(__fmul_rn(-2.0f, smc_log_f(u1)));
+    const float rad = sqrt_rn(__fmul_rn(-2.0f, smc_log_f(u1)));
     float sn, cs;
     smc_sincos2pi_f(u2, sn, cs);
     e0 = __fmul_rn(rad, cs);

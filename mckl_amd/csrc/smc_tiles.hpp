@@ -60,10 +60,36 @@ __device__ __forceinline__ void tile_init(
     }
 }
 
+// first j with E[j] > z (E nondecreasing, length n)
+__device__ __forceinline__ uint32_t upper_bound_u(
+    const uint32_t *E, uint32_t n, uint32_t z)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (E[mid] > z)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// Parent of slot i after the stable rep->index transform: the slot itself
+// when it kept a copy, else the source of the extra copy with the slot's
+// rank among the zero slots (tile_index writes kZeroFlag | rank).
+__device__ __forceinline__ uint32_t parent_of(
+    uint32_t code, const uint32_t *Eincl, uint32_t N)
+{
+    return (code & kZeroFlag) ? upper_bound_u(Eincl, N, code & ~kZeroFlag)
+                              : code;
+}
+
 // move + weight of one tile; returns the tile max of lw (all threads)
 __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
     uint32_t tile, const float *xin, float *xout, float *lw,
-    const uint32_t *idx, int resampled, float mprev, int gt, float *ldsf)
+    const uint32_t *idx, const uint32_t *Eincl, int resampled, float mprev,
+    int gt, float *ldsf)
 {
     const uint64_t i0 = tile_first(tile, gt);
     const float obs = A.obs[t - 1];
@@ -77,7 +103,7 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
                 a1.z, a1.w};
 #pragma unroll
             for (int k = 0; k < kItems; ++k) {
-                xp[k] = xin[a[k]];
+                xp[k] = xin[parent_of(a[k], Eincl, A.N)];
                 lo[k] = 0.0f;
             }
         } else {
@@ -127,7 +153,7 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
                     break;
                 float xv, carry;
                 if (resampled) {
-                    xv = xin[idx[p]];
+                    xv = xin[parent_of(idx[p], Eincl, A.N)];
                     carry = 0.0f;
                 } else {
                     xv = xin[p];
@@ -144,12 +170,15 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
     return group_max(lmax, gt, ldsf);
 }
 
-// S = sum e_i, S2 = sum e_i^2 over the tile, e_i = exp_f(lw_i - m)
+// e_i = exp_f(lw_i - m); the tile's S = sum e, S2 = sum e^2 (canonical
+// order) and the double-double total of e for the resampling prefix sums
 __device__ __forceinline__ void tile_sums(const SmcArgs &A, uint32_t tile,
-    const float *lw, float m, int gt, double *ldsd, double &S, double &S2)
+    const float *lw, float m, int gt, double *ldsd, double &S, double &S2,
+    dd &etot)
 {
     const uint64_t i0 = tile_first(tile, gt);
     double s = 0.0, s2 = 0.0;
+    dd loc{0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
@@ -157,34 +186,13 @@ __device__ __forceinline__ void tile_sums(const SmcArgs &A, uint32_t tile,
             const double e = (double)smc_exp_f(__fsub_rn(lw[i], m));
             s = __dadd_rn(s, e);
             s2 = __dadd_rn(s2, __dmul_rn(e, e));
+            loc = dd_add(loc, dd{e, 0.0});
         }
     }
     group_sum2(s, s2, gt, ldsd);
     S = s;
     S2 = s2;
-}
-
-// normalised weight W_i = (double)e_i * (1/S)  (core/weight.hpp:221)
-__device__ __forceinline__ double smc_weight(float lwi, float m, double invS)
-{
-    return __dmul_rn((double)smc_exp_f(__fsub_rn(lwi, m)), invS);
-}
-
-// dd total of W over the tile
-__device__ __forceinline__ dd tile_wsum(const SmcArgs &A, uint32_t tile,
-    const float *lw, float m, double invS, int gt, double *ldsd)
-{
-    const uint64_t i0 = tile_first(tile, gt);
-    dd loc{0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = i0 + k;
-        if (i < A.N)
-            loc = dd_add(loc, dd{smc_weight(lw[i], m, invS), 0.0});
-    }
-    dd total;
-    (void)group_excl_scan_dd(loc, gt, ldsd, total);
-    return total;
+    (void)group_excl_scan_dd(loc, gt, ldsd, etot);
 }
 
 // Uniform sequences of step t (sorted, M = N):
@@ -217,11 +225,13 @@ struct UniformSeq {
     }
 };
 
-// Prefix sums S_j of W within the tile plus the tile offset, and the counts
-// cnt_j = #{u < S_j}.  The last particle of a tile takes S from the next tile
-// offset so that neighbouring tiles agree on boundary values; the last
-// particle overall takes cnt = M (resample.hpp:405).  Returns the tile sums
-// of the extra-copy and zero-slot counts through esum/zsum.
+// Cumulative normalised weights S_j = round(prefix_j(e)) * (1/S), prefix in
+// double-double over tiles (tile offsets from tiles_scan_dd of the tile
+// totals of tile_sums), and the counts cnt_j = #{u < S_j}.  The last particle
+// of a tile takes its prefix from the next tile offset so that neighbouring
+// tiles agree on boundary values; the last particle overall takes cnt = M
+// (resample.hpp:405).  Returns the tile sums of the extra-copy and zero-slot
+// counts through esum/zsum.
 __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
     const float *lw, float m, double invS, const double *toffh,
     const double *toffl, const UniformSeq &U, uint32_t *cnt, int gt,
@@ -236,7 +246,7 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        W[k] = i < A.N ? smc_weight(lw[i], m, invS) : 0.0;
+        W[k] = i < A.N ? (double)smc_exp_f(__fsub_rn(lw[i], m)) : 0.0;
         loc = dd_add(loc, dd{W[k], 0.0});
     }
     dd total;
@@ -253,9 +263,9 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
             if (i == (uint64_t)A.N - 1) {
                 c[k] = U.M;
             } else {
-                const double S = i == tend - 1 ? toffh[tile + 1]
+                const double P = i == tend - 1 ? toffh[tile + 1]
                                                : dd_add(base, run).hi;
-                c[k] = U.count_less(S);
+                c[k] = U.count_less(__dmul_rn(P, invS));
             }
             cnt[i] = c[k];
         }
@@ -267,7 +277,7 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
     if (gt > 0)
         prev = ldsu[gt - 1];
     else
-        prev = tile == 0 ? 0u : U.count_less(toffh[tile]);
+        prev = tile == 0 ? 0u : U.count_less(__dmul_rn(toffh[tile], invS));
     __syncthreads();
     uint32_t es = 0, zs = 0;
 #pragma unroll
@@ -323,37 +333,6 @@ __device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
                 idx[i] = kZeroFlag | zx;
                 ++zx;
             }
-        }
-    }
-}
-
-// first j with E[j] > z (E nondecreasing, length n)
-__device__ __forceinline__ uint32_t upper_bound_u(
-    const uint32_t *E, uint32_t n, uint32_t z)
-{
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (E[mid] > z)
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ void tile_resolve(
-    const SmcArgs &A, uint32_t tile, const uint32_t *Eincl, uint32_t *idx,
-    int gt)
-{
-    const uint64_t i0 = tile_first(tile, gt);
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = i0 + k;
-        if (i < A.N) {
-            const uint32_t v = idx[i];
-            if (v & kZeroFlag)
-                idx[i] = upper_bound_u(Eincl, A.N, v & ~kZeroFlag);
         }
     }
 }

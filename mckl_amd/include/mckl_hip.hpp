@@ -1,0 +1,351 @@
+// mckl_hip.hpp -- C++ host API of the MI355X hot path, mirroring the
+// reference's interface (namespace mckl, same type and function names) over
+// the C ABI in include/mckl_hip.h.  Header-only; link libmckl_hip.so.
+//
+// Differences from the reference, all forced by the device boundary:
+//  * output and weight arrays are device pointers;
+//  * the scalar engine call operator()() and ess() synchronise (the value is
+//    produced on the device);
+//  * errors raise mckl::RuntimeAssert carrying the C ABI's message, as the
+//    reference does with MCKL_RUNTIME_ASSERT_AS_EXCEPTION
+//    (internal/assert.hpp:49-84).
+#ifndef MCKL_HIP_HPP
+#define MCKL_HIP_HPP
+
+#include <mckl_hip.h>
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mckl {
+
+class RuntimeAssert : public std::runtime_error {
+  public:
+    explicit RuntimeAssert(const std::string &msg) : std::runtime_error(msg) {}
+};
+
+inline void hip_check(int rc)
+{
+    if (rc != MCKL_HIP_OK)
+        throw RuntimeAssert(std::string("mckl_hip: ") + mckl_hip_last_error());
+}
+
+// stream used by all calls of this thread (hipStream_t; nullptr = default)
+inline void *&current_stream()
+{
+    static thread_local void *s = nullptr;
+    return s;
+}
+
+/// CounterEngine<ResultType, Generator> (random/counter.hpp:61-288)
+template <int Kind, typename ResultType>
+class CounterEngineHIP
+{
+  public:
+    using result_type = ResultType;
+    using skip_type = std::uint64_t;
+    using key_type = typename std::conditional<Kind == MCKL_HIP_THREEFRY4X64,
+        std::array<std::uint64_t, 4>,
+        typename std::conditional<Kind == MCKL_HIP_PHILOX4X32,
+            std::array<std::uint32_t, 2>,
+            std::array<std::uint32_t, 4>>::type>::type;
+    using ctr_type = typename std::conditional<Kind == MCKL_HIP_THREEFRY4X64,
+        std::array<std::uint64_t, 4>, std::array<std::uint64_t, 2>>::type;
+
+    explicit CounterEngineHIP(result_type s = 1)
+    {
+        key_type k{};
+        reset(k);
+        seed(s);
+    }
+
+    explicit CounterEngineHIP(const key_type &k) { reset(k); }
+
+    void seed(result_type s) { hip_check(mckl_hip_engine_seed(&state_, s)); }
+
+    void seed(const key_type &k) { reset(k); }
+
+    key_type key() const
+    {
+        key_type k;
+        for (std::size_t i = 0; i != k.size(); ++i)
+            k[i] = static_cast<typename key_type::value_type>(state_.key[i]);
+        return k;
+    }
+
+    ctr_type ctr() const
+    {
+        ctr_type c;
+        for (std::size_t i = 0; i != c.size(); ++i)
+            c[i] = state_.ctr[i];
+        return c;
+    }
+
+    void ctr(const ctr_type &c)
+    {
+        std::uint64_t w[4] = {0, 0, 0, 0};
+        for (std::size_t i = 0; i != c.size(); ++i)
+            w[i] = c[i];
+        hip_check(mckl_hip_engine_set_ctr(&state_, w));
+    }
+
+    result_type operator()()
+    {
+        std::uint64_t v = 0;
+        hip_check(mckl_hip_engine_next(&state_, &v));
+        return static_cast<result_type>(v);
+    }
+
+    void discard(skip_type n) { hip_check(mckl_hip_engine_discard(&state_, n)); }
+
+    static constexpr result_type min() { return 0; }
+    static constexpr result_type max() { return ~result_type(0); }
+
+    mckl_hip_engine &state() { return state_; }
+    const mckl_hip_engine &state() const { return state_; }
+
+    friend bool operator==(const CounterEngineHIP &a, const CounterEngineHIP &b)
+    {
+        return a.state_.index == b.state_.index && a.ctr() == b.ctr() &&
+            a.key() == b.key();
+    }
+    friend bool operator!=(const CounterEngineHIP &a, const CounterEngineHIP &b)
+    {
+        return !(a == b);
+    }
+
+  private:
+    mckl_hip_engine state_;
+
+    void reset(const key_type &k)
+    {
+        std::uint64_t w[4] = {0, 0, 0, 0};
+        for (std::size_t i = 0; i != k.size(); ++i)
+            w[i] = k[i];
+        hip_check(mckl_hip_engine_init(
+            &state_, Kind, int(sizeof(ResultType) * 8), w));
+    }
+};
+
+using Philox4x32 = CounterEngineHIP<MCKL_HIP_PHILOX4X32, std::uint32_t>;
+using Philox4x32_64 = CounterEngineHIP<MCKL_HIP_PHILOX4X32, std::uint64_t>;
+using Threefry4x64 = CounterEngineHIP<MCKL_HIP_THREEFRY4X64, std::uint32_t>;
+using Threefry4x64_64 = CounterEngineHIP<MCKL_HIP_THREEFRY4X64, std::uint64_t>;
+using ARS = CounterEngineHIP<MCKL_HIP_ARS, std::uint32_t>;
+using ARS_64 = CounterEngineHIP<MCKL_HIP_ARS, std::uint64_t>;
+
+/// rand(rng, n, r): random/counter.hpp:298-303 (r in device memory)
+template <int K, typename R>
+inline void rand(CounterEngineHIP<K, R> &rng, std::size_t n, R *d_r)
+{
+    hip_check(mckl_hip_rand(&rng.state(), n, d_r, current_stream()));
+}
+
+template <typename RealType, int Interval>
+class U01DistributionHIP
+{
+  public:
+    using result_type = RealType;
+    template <int K, typename R>
+    void operator()(CounterEngineHIP<K, R> &rng, std::size_t n, RealType *d_r)
+    {
+        fill(rng.state(), n, d_r);
+    }
+
+  private:
+    static void fill(mckl_hip_engine &e, std::size_t n, double *r)
+    {
+        hip_check(mckl_hip_rand_u01_f64(&e, Interval, n, r, current_stream()));
+    }
+    static void fill(mckl_hip_engine &e, std::size_t n, float *r)
+    {
+        hip_check(mckl_hip_rand_u01_f32(&e, Interval, n, r, current_stream()));
+    }
+};
+
+template <typename T>
+using U01CCDistribution = U01DistributionHIP<T, MCKL_HIP_U01_CC>;
+template <typename T>
+using U01CODistribution = U01DistributionHIP<T, MCKL_HIP_U01_CO>;
+template <typename T>
+using U01OCDistribution = U01DistributionHIP<T, MCKL_HIP_U01_OC>;
+template <typename T>
+using U01OODistribution = U01DistributionHIP<T, MCKL_HIP_U01_OO>;
+template <typename T>
+using U01CanonicalDistribution = U01DistributionHIP<T, MCKL_HIP_U01_CANONICAL>;
+
+/// NormalDistribution<RealType> (random/normal_distribution.hpp:113-154)
+template <typename RealType>
+class NormalDistribution
+{
+  public:
+    using result_type = RealType;
+    explicit NormalDistribution(RealType mean = 0, RealType stddev = 1)
+        : mean_(mean), stddev_(stddev)
+    {
+    }
+    RealType mean() const { return mean_; }
+    RealType stddev() const { return stddev_; }
+
+    template <int K, typename R>
+    void operator()(CounterEngineHIP<K, R> &rng, std::size_t n, RealType *d_r)
+    {
+        fill(rng.state(), n, d_r);
+    }
+
+  private:
+    RealType mean_, stddev_;
+    void fill(mckl_hip_engine &e, std::size_t n, float *r)
+    {
+        hip_check(mckl_hip_rand_normal_f32(
+            &e, n, mean_, stddev_, r, current_stream()));
+    }
+    void fill(mckl_hip_engine &e, std::size_t n, double *r)
+    {
+        hip_check(mckl_hip_rand_normal_f64(
+            &e, n, mean_, stddev_, r, current_stream()));
+    }
+};
+
+/// rand(rng, dist, n, r): random/internal/common.hpp:706-712
+template <int K, typename R, typename Dist>
+inline void rand(CounterEngineHIP<K, R> &rng, Dist &dist, std::size_t n,
+    typename Dist::result_type *d_r)
+{
+    dist(rng, n, d_r);
+}
+
+/// Weight on device memory (core/weight.hpp:44-228).  The caller owns the
+/// device array of n doubles and a device array of 2 doubles for
+/// {ESS, log normaliser increment}.
+class Weight
+{
+  public:
+    Weight(std::size_t n, double *d_w, double *d_stats)
+        : n_(n), w_(d_w), stats_(d_stats)
+    {
+        set_equal();
+    }
+    std::size_t size() const { return n_; }
+    double *data() { return w_; }
+    void set_equal()
+    {
+        hip_check(mckl_hip_weight_set_equal(n_, w_, stats_, current_stream()));
+    }
+    void set_log(const double *d_v)
+    {
+        hip_check(mckl_hip_weight_set_log(n_, d_v, w_, stats_, current_stream()));
+    }
+    void add_log(const double *d_v)
+    {
+        hip_check(mckl_hip_weight_add_log(n_, d_v, w_, stats_, current_stream()));
+    }
+    const double *stats() const { return stats_; }
+
+  private:
+    std::size_t n_;
+    double *w_;
+    double *stats_;
+};
+
+enum ResampleScheme {
+    Multinomial = MCKL_HIP_MULTINOMIAL,
+    Stratified = MCKL_HIP_STRATIFIED,
+    Systematic = MCKL_HIP_SYSTEMATIC
+};
+
+/// ResampleAlgorithm (algorithm/resample.hpp:515-576) on device arrays
+template <ResampleScheme Scheme>
+class ResampleAlgorithmHIP
+{
+  public:
+    template <int K, typename R>
+    void operator()(std::size_t N, std::size_t M, CounterEngineHIP<K, R> &rng,
+        const double *d_w, std::uint32_t *d_rep,
+        std::uint32_t *d_idx = nullptr) const
+    {
+        hip_check(mckl_hip_resample(Scheme, N, M, &rng.state(), d_w, d_rep,
+            d_idx, current_stream()));
+    }
+};
+
+using ResampleMultinomial = ResampleAlgorithmHIP<Multinomial>;
+using ResampleStratified = ResampleAlgorithmHIP<Stratified>;
+using ResampleSystematic = ResampleAlgorithmHIP<Systematic>;
+
+inline void resample_trans_u01_rep(std::size_t N, std::size_t M,
+    const double *d_w, const double *d_u01, std::uint32_t *d_rep)
+{
+    hip_check(mckl_hip_resample_trans_u01_rep(
+        N, M, d_w, d_u01, d_rep, current_stream()));
+}
+
+inline void resample_trans_rep_index(std::size_t N, std::size_t M,
+    const std::uint32_t *d_rep, std::uint32_t *d_idx)
+{
+    hip_check(mckl_hip_resample_trans_rep_index(
+        N, M, d_rep, d_idx, current_stream()));
+}
+
+/// SMCSampler (algorithm/smc.hpp:151-421) for the built-in models
+class SMCSamplerHIP
+{
+  public:
+    SMCSamplerHIP(int model, const std::array<double, 3> &theta,
+        const std::vector<double> &y, std::size_t N, std::uint64_t seed = 1)
+        : model_(model), theta_(theta), y_(y), N_(N), seed_(seed)
+    {
+    }
+    ~SMCSamplerHIP()
+    {
+        if (s_)
+            mckl_hip_smc_destroy(s_);
+    }
+    SMCSamplerHIP(const SMCSamplerHIP &) = delete;
+    SMCSamplerHIP &operator=(const SMCSamplerHIP &) = delete;
+
+    void resample_threshold(double t) { threshold_ = t; }
+    double resample_threshold() const { return threshold_; }
+    void resample(ResampleScheme s) { scheme_ = s; }
+
+    /// all T steps, asynchronous on the current stream
+    void iterate()
+    {
+        if (!s_)
+            hip_check(mckl_hip_smc_create(model_, theta_.data(), y_.size(),
+                y_.data(), N_, seed_, threshold_, scheme_, &s_));
+        hip_check(mckl_hip_smc_run(s_, current_stream()));
+    }
+
+    double log_likelihood()
+    {
+        double ll = 0;
+        hip_check(mckl_hip_smc_result(s_, &ll, nullptr, nullptr));
+        return ll;
+    }
+
+    std::vector<double> ess_history()
+    {
+        std::vector<double> e(y_.size());
+        hip_check(mckl_hip_smc_result(s_, nullptr, e.data(), nullptr));
+        return e;
+    }
+
+  private:
+    int model_;
+    std::array<double, 3> theta_;
+    std::vector<double> y_;
+    std::size_t N_;
+    std::uint64_t seed_;
+    double threshold_ = 0.5;
+    int scheme_ = MCKL_HIP_SYSTEMATIC;
+    mckl_hip_smc *s_ = nullptr;
+};
+
+} // namespace mckl
+
+#endif

@@ -1,0 +1,143 @@
+// host_api_test.cpp -- exercises the C++ host API (mckl_hip.hpp) the way the
+// reference's example tests exercise theirs: KAT comparison
+// (example/random/include/random_rng.hpp:80-104), bulk vs scalar draws
+// (:234-331), resampling invariants and a sampler run.
+// Usage: host_api_test <path to random_philox4x32.data>
+#include <mckl_hip.hpp>
+
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <vector>
+
+static int failures = 0;
+
+#define CHECK(cond, what)                                                     \
+    do {                                                                      \
+        std::printf("%-48s %s\n", what, (cond) ? "Passed" : "Failed");        \
+        if (!(cond))                                                          \
+            ++failures;                                                       \
+    } while (0)
+
+template <typename T>
+struct Dev {
+    T *p = nullptr;
+    explicit Dev(std::size_t n) { (void)hipMalloc(&p, n * sizeof(T)); }
+    ~Dev() { (void)hipFree(p); }
+    std::vector<T> host(std::size_t n) const
+    {
+        std::vector<T> h(n);
+        (void)hipMemcpy(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost);
+        return h;
+    }
+};
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s random_philox4x32.data\n", argv[0]);
+        return 2;
+    }
+    // 1. KAT of a default-constructed engine
+    std::vector<std::uint32_t> kat;
+    {
+        std::ifstream f(argv[1]);
+        std::uint32_t x;
+        while (f >> x)
+            kat.push_back(x);
+    }
+    {
+        mckl::Philox4x32 rng;
+        Dev<std::uint32_t> r(kat.size());
+        mckl::rand(rng, kat.size(), r.p);
+        CHECK(!kat.empty() && r.host(kat.size()) == kat, "Philox4x32 KAT");
+        // seed(key) round trip (random_rng.hpp:97-103)
+        mckl::Philox4x32::key_type k = {{rng(), rng()}};
+        rng.seed(k);
+        CHECK(rng.key() == k, "Philox4x32 seed(key) round trip");
+    }
+    // 2. scalar draws equal the bulk stream, across partial blocks
+    {
+        mckl::Threefry4x64 a(7), b(7);
+        Dev<std::uint32_t> r(37);
+        mckl::rand(a, 3, r.p);
+        mckl::rand(a, 34, r.p + 3);
+        std::vector<std::uint32_t> bulk = r.host(37), scal(37);
+        for (auto &v : scal)
+            v = b();
+        CHECK(bulk == scal && a == b, "Threefry4x64 scalar == bulk");
+        mckl::ARS c(3), d(3);
+        c.discard(1001);
+        for (int i = 0; i < 1001; ++i)
+            d();
+        CHECK(c() == d(), "ARS discard == scalar draws");
+    }
+    // 3. Normal<double> moments
+    {
+        mckl::ARS rng(11);
+        mckl::NormalDistribution<double> normal(2.5, 0.3);
+        const std::size_t n = 1 << 20;
+        Dev<double> r(n);
+        mckl::rand(rng, normal, n, r.p);
+        std::vector<double> h = r.host(n);
+        double s = 0, s2 = 0;
+        for (double v : h) {
+            s += v;
+            s2 += v * v;
+        }
+        const double mu = s / n, var = s2 / n - mu * mu;
+        CHECK(std::fabs(mu - 2.5) < 5 * 0.3 / std::sqrt(double(n)) &&
+                std::fabs(var - 0.09) < 0.001,
+            "NormalDistribution<double>(2.5, 0.3) moments");
+    }
+    // 4. Weight and systematic resampling of equal weights
+    {
+        const std::size_t n = 5000;
+        Dev<double> w(n), st(2), v(n);
+        std::vector<double> zeros(n, 0.0);
+        (void)hipMemcpy(v.p, zeros.data(), n * sizeof(double),
+            hipMemcpyHostToDevice);
+        mckl::Weight weight(n, w.p, st.p);
+        weight.set_log(v.p);
+        std::vector<double> s = st.host(2);
+        CHECK(std::fabs(s[0] - double(n)) < 1e-9, "Weight::set_log ESS");
+        mckl::Philox4x32 rng(5);
+        Dev<std::uint32_t> rep(n), idx(n);
+        mckl::ResampleSystematic()(n, n, rng, w.p, rep.p, idx.p);
+        std::vector<std::uint32_t> r = rep.host(n), x = idx.host(n);
+        bool ok = true;
+        for (std::size_t i = 0; i != n; ++i)
+            ok = ok && r[i] == 1 && x[i] == i;
+        CHECK(ok, "ResampleSystematic on equal weights");
+    }
+    // 5. errors surface as exceptions
+    {
+        bool thrown = false;
+        try {
+            mckl::Philox4x32 rng;
+            mckl::NormalDistribution<float> bad(0.0f, -1.0f);
+            Dev<float> r(4);
+            mckl::rand(rng, bad, 4, r.p);
+        } catch (const mckl::RuntimeAssert &) {
+            thrown = true;
+        }
+        CHECK(thrown, "invalid stddev raises RuntimeAssert");
+    }
+    // 6. plain sampler on the linear-Gaussian model
+    {
+        std::vector<double> y(30);
+        for (std::size_t t = 0; t != y.size(); ++t)
+            y[t] = std::sin(0.3 * t);
+        mckl::SMCSamplerHIP sampler(MCKL_HIP_MODEL_LG, {{0.9, 1.0, 0.5}}, y,
+            1 << 14, 3);
+        sampler.resample_threshold(0.5);
+        sampler.resample(mckl::Systematic);
+        sampler.iterate();
+        const double ll = sampler.log_likelihood();
+        CHECK(std::isfinite(ll) && ll < 0, "SMCSamplerHIP log likelihood");
+    }
+    std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
+    return failures ? 1 : 0;
+}

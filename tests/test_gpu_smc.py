@@ -67,6 +67,25 @@ def test_plain_repeatable(dev):
     assert a == b
 
 
+def test_plain_graph_replays(dev):
+    """The sampler replays one captured HIP graph; every replay, with eager
+    device work and synchronisations in between, gives the first result."""
+    import torch
+    m = _m()
+    y = _obs()["lg_y"][:80]
+    s = m.SMCSampler(1, (0.9, 1.0, 0.5), y, 50000, 21, 0.5, 3)
+    s.iterate()
+    first = s.results()
+    for k in range(3):
+        junk = torch.randn(1 << 20, device=dev).sum()
+        torch.cuda.synchronize()
+        s.iterate()
+        ll, ess, res = s.results()
+        assert ll == first[0]
+        assert np.array_equal(ess, first[1]) and np.array_equal(res, first[2])
+        del junk
+
+
 def test_lg_config_vs_kalman(dev):
     """Config: N = 2^20 FP32 particles, T = 100 observations simulated from
     seed 7, systematic resampling when ESS < N/2."""
