@@ -213,6 +213,39 @@ int mckl_hip_smc_batch_run(mckl_hip_smc_batch *b, int use_graph, void *stream);
 int mckl_hip_smc_batch_result(mckl_hip_smc_batch *b, double *h_loglik);
 int mckl_hip_smc_batch_destroy(mckl_hip_smc_batch *b);
 
+/*
+ * Gradient of the SMC log-likelihood estimate with respect to theta by a
+ * hand-written adjoint (no reference counterpart: the reference exposes
+ * neither the log-likelihood, core/weight.hpp:198-227, nor derivatives).
+ * nbatch parameter points, N <= 131072 particles each, FP64 states; the
+ * ancestors and resampling decisions are those of the forward pass at theta
+ * and stay fixed in the reverse sweep; forward states are checkpointed every
+ * K steps and recomputed segment by segment.  y holds T observations shared
+ * by all points (y_per_point = 0) or T per point.  theta, grad: 3 per point
+ * (LG: a, sx, sy; SV: phi, sigma, beta).
+ * With tape = 1 run() also records the ancestors, normalised log-weights and
+ * decisions (nbatch * (T+1) * N * 12 bytes), and replay() evaluates the
+ * differentiated function at another theta with them held fixed: the
+ * function whose central differences the gradient must match.
+ */
+typedef struct mckl_hip_smc_grad mckl_hip_smc_grad;
+int mckl_hip_smc_grad_create(int model, size_t nbatch, size_t N, size_t T,
+    const double *h_y, int y_per_point, const uint64_t *h_seed,
+    double threshold, int scheme, size_t K, int tape,
+    mckl_hip_smc_grad **out);
+/* forward + adjoint: h_loglik[nbatch], h_grad[3 nbatch] (either may be NULL);
+ * synchronises the stream */
+int mckl_hip_smc_grad_run(mckl_hip_smc_grad *g, const double *h_theta,
+    double *h_loglik, double *h_grad, void *stream);
+int mckl_hip_smc_grad_replay(mckl_hip_smc_grad *g, const double *h_theta,
+    double *h_value, void *stream);
+int mckl_hip_smc_grad_destroy(mckl_hip_smc_grad *g);
+/* one-shot form: create, run, destroy (no tape) */
+int mckl_hip_smc_loglik_grad(int model, size_t nbatch, const double *h_theta,
+    size_t N, size_t T, const double *h_y, int y_per_point,
+    const uint64_t *h_seed, double threshold, int scheme, size_t K,
+    double *h_loglik, double *h_grad, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

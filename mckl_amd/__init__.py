@@ -397,3 +397,78 @@ class SMCBatch:
         if h is not None and h.value:
             lib.mckl_hip_smc_batch_destroy(h)
             self._h = None
+
+
+class SMCGradient:
+    """Gradient of the SMC log-likelihood estimate by the hand-written
+    adjoint, for a batch of parameter points.
+
+    theta has shape (nbatch, 3); y has shape (T,) (shared) or (nbatch, T);
+    seeds one per point.  With tape=True, run() records the ancestors and
+    decisions so that replay(theta) evaluates the differentiated function at
+    another theta with them held fixed (central finite differences of
+    replay() are what the adjoint's gradient must reproduce).
+    """
+
+    def __init__(self, model, nbatch, N, y, seeds, resample_threshold=0.5,
+                 scheme=SYSTEMATIC, K=50, tape=False):
+        self.nbatch = int(nbatch)
+        self._y = np.ascontiguousarray(y, np.float64)
+        per_point = self._y.ndim == 2
+        self.T = self._y.shape[-1]
+        self._seeds = np.ascontiguousarray(
+            np.broadcast_to(np.asarray(seeds, np.uint64), (self.nbatch,)))
+        h = ctypes.c_void_p()
+        check(lib.mckl_hip_smc_grad_create(
+            model, self.nbatch, int(N), self.T, _dptr(self._y.reshape(-1)),
+            int(per_point),
+            self._seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            float(resample_threshold), scheme, int(K), int(bool(tape)),
+            ctypes.byref(h)))
+        self._h = h
+
+    def _theta(self, theta):
+        th = np.ascontiguousarray(theta, np.float64).reshape(-1, 3)
+        if th.shape[0] != self.nbatch:
+            th = np.ascontiguousarray(np.broadcast_to(th, (self.nbatch, 3)))
+        return th
+
+    def run(self, theta):
+        """Returns (loglik[nbatch], grad[nbatch, 3])."""
+        th = self._theta(theta)
+        ll = np.zeros(self.nbatch, np.float64)
+        g = np.zeros((self.nbatch, 3), np.float64)
+        check(lib.mckl_hip_smc_grad_run(self._h, _dptr(th), _dptr(ll),
+                                        _dptr(g), _stream()))
+        return ll, g
+
+    def replay(self, theta):
+        th = self._theta(theta)
+        v = np.zeros(self.nbatch, np.float64)
+        check(lib.mckl_hip_smc_grad_replay(self._h, _dptr(th), _dptr(v),
+                                           _stream()))
+        return v
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.mckl_hip_smc_grad_destroy(h)
+            self._h = None
+
+
+def smc_loglik_grad(model, theta, y, N, seeds, resample_threshold=0.5,
+                    scheme=SYSTEMATIC, K=50):
+    """One-shot batched log-likelihood and gradient (no tape)."""
+    th = np.ascontiguousarray(theta, np.float64).reshape(-1, 3)
+    nb = th.shape[0]
+    yy = np.ascontiguousarray(y, np.float64)
+    sd = np.ascontiguousarray(np.broadcast_to(np.asarray(seeds, np.uint64),
+                                              (nb,)))
+    ll = np.zeros(nb, np.float64)
+    g = np.zeros((nb, 3), np.float64)
+    check(lib.mckl_hip_smc_loglik_grad(
+        model, nb, _dptr(th), int(N), yy.shape[-1], _dptr(yy.reshape(-1)),
+        int(yy.ndim == 2), sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+        float(resample_threshold), scheme, int(K), _dptr(ll), _dptr(g),
+        _stream()))
+    return ll, g

@@ -102,6 +102,19 @@ SIGNATURES = {
     "mckl_hip_smc_batch_run": (_int, [_vp, _int, _vp]),
     "mckl_hip_smc_batch_result": (_int, [_vp, ctypes.POINTER(_dbl)]),
     "mckl_hip_smc_batch_destroy": (_int, [_vp]),
+    "mckl_hip_smc_grad_create": (
+        _int, [_int, _sz, _sz, _sz, ctypes.POINTER(_dbl), _int, _U64P, _dbl,
+               _int, _sz, _int, ctypes.POINTER(_vp)]),
+    "mckl_hip_smc_grad_run": (
+        _int, [_vp, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl),
+               ctypes.POINTER(_dbl), _vp]),
+    "mckl_hip_smc_grad_replay": (
+        _int, [_vp, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl), _vp]),
+    "mckl_hip_smc_grad_destroy": (_int, [_vp]),
+    "mckl_hip_smc_loglik_grad": (
+        _int, [_int, _sz, ctypes.POINTER(_dbl), _sz, _sz, ctypes.POINTER(_dbl),
+               _int, _U64P, _dbl, _int, _sz, ctypes.POINTER(_dbl),
+               ctypes.POINTER(_dbl), _vp]),
 }
 
 MISSING = []

@@ -20,23 +20,6 @@
 
 namespace mckl_hip {
 
-constexpr int kScratch = 1280; // LDS bytes per 256-thread group
-
-struct GroupLds {
-    double *d;
-    float *f;
-    uint32_t *u;
-};
-
-__device__ __forceinline__ GroupLds group_lds(unsigned char *base)
-{
-    GroupLds g;
-    g.d = reinterpret_cast<double *>(base);
-    g.f = reinterpret_cast<float *>(base + 64);
-    g.u = reinterpret_cast<uint32_t *>(base + 128);
-    return g;
-}
-
 // per-filter device state of the plain sampler
 struct SmcState {
     double logz;
@@ -359,15 +342,6 @@ static int check_common(size_t N, size_t T, double threshold, int scheme)
             "the samplers resample with the stratified or systematic scheme");
     if (std::isnan(threshold))
         return set_error(MCKL_HIP_EINVAL, "threshold is NaN");
-    return MCKL_HIP_OK;
-}
-
-template <typename T>
-static int dev_alloc(T **p, size_t n)
-{
-    *p = nullptr;
-    MCKL_HIP_TRY(hipMalloc(reinterpret_cast<void **>(p),
-        std::max<size_t>(n, 1) * sizeof(T)));
     return MCKL_HIP_OK;
 }
 

@@ -1,0 +1,267 @@
+// smc_grad.hpp -- batched gradient of the SMC log-likelihood with a
+// hand-written adjoint (no reference counterpart: the reference exposes no
+// log-likelihood, core/weight.hpp:198-227, and no derivatives).
+//
+// One workgroup per parameter point, FP64 particle states and log-weights.
+// The forward pass follows the samplers of smc.hip step for step (same noise
+// streams, same resampling decisions and stable rep->index transform) and
+// stores a checkpoint every K steps.  The backward pass walks the segments in
+// reverse: it recomputes a segment from its checkpoint (the kernels are
+// deterministic, so ancestors come back identical without being stored) and
+// sweeps the adjoint through it.
+//
+// The differentiated function, with ancestors and resampling decisions held
+// at their values at theta0 (the point of the forward pass):
+//   F(theta) = sum_t [ LSE(lw_t) - LSE(carry_t) ]
+//   lw_t,i   = carry_t,i + log g(y_t | x_t,i)
+//   x_t,i    = a x_{t-1,p(i)} + s eps_t,i,  x_0,i = sd0 eps_0,i
+//   carry    = lw_{t-1,i} - m_{t-1}                  (no resampling at t-1)
+//            = LWn_{t-1,p(i)}(theta) - LWn_{t-1,p(i)}(theta0)  (resampled)
+// with LWn the normalised log-weights.  F(theta0) is the estimate log Z-hat;
+// the resampled weights carry their dependence on theta (value 1, gradient
+// d log W), which makes grad F a consistent estimate of the score.  The
+// replay entry point evaluates F at any theta with a tape recorded at
+// theta0, for finite-difference checks.
+#pragma once
+
+#include "common.hpp"
+#include "smc_tiles.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace mckl_hip {
+
+constexpr int kMaxTilesG = 64; // N <= 131072 per point
+
+struct GradArgs {
+    int model;                  // 1 LG (a, sx, sy), 2 SV (phi, sigma, beta)
+    double a, s, c;             // theta
+    double sd0, dsd0_a, dsd0_s; // initial scale and its derivatives
+    double cst;                 // -log c - log(2 pi)/2
+    uint32_t k0, k1;            // particle RNG key
+    uint32_t N, ntiles, T, K;
+    double tauN;
+    int scheme;
+    const double *y;
+};
+
+// Per-slot layout: one slot holds the state after step t (x, lw, and the
+// resampling structures that give the parents of step t+1).
+struct SlotPtr {
+    double *x, *lw;
+    uint32_t *code, *E, *zpos;
+    double *scal; // m, S, lse, R
+};
+
+__device__ __forceinline__ SlotPtr slot_at(double *base, size_t slot_bytes,
+    uint32_t k, uint32_t Np)
+{
+    char *p = reinterpret_cast<char *>(base) + (size_t)k * slot_bytes;
+    SlotPtr s;
+    s.x = reinterpret_cast<double *>(p);
+    s.lw = s.x + Np;
+    s.code = reinterpret_cast<uint32_t *>(s.lw + Np);
+    s.E = s.code + Np;
+    s.zpos = s.E + Np;
+    s.scal = reinterpret_cast<double *>(s.zpos + Np);
+    return s;
+}
+
+__device__ __forceinline__ void gmodel(const GradArgs &G, double y, double x,
+    double &g, double &gx, double &gc)
+{
+    if (G.model == 1) {
+        const double ic = 1.0 / G.c;
+        const double z = (y - x) * ic;
+        g = -0.5 * z * z + G.cst;
+        gx = z * ic;
+        gc = (z * z - 1.0) * ic;
+    } else {
+        const double q = y * y * exp(-x) / (G.c * G.c);
+        g = -0.5 * q - 0.5 * x + G.cst;
+        gx = 0.5 * q - 0.5;
+        gc = (q - 1.0) / G.c;
+    }
+}
+
+__device__ __forceinline__ double eps_at(const GradArgs &G, uint64_t i,
+    uint64_t t)
+{
+    float e0, e1;
+    smc_normal_pair(G.k0, G.k1, i >> 1, t, e0, e1);
+    return (double)((i & 1) ? e1 : e0);
+}
+
+// ---- forward pieces (FP64 counterparts of smc_tiles.hpp) ----
+__device__ __forceinline__ double gtile_move(const GradArgs &G, uint32_t t,
+    uint32_t tile, const SlotPtr &prev, const SlotPtr &cur, int Rprev,
+    double mprev, const double *lwn_ref, const uint32_t *tpar, int gt,
+    double *ldsd)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    const double y = G.y[t - 1];
+    double lmax = -__builtin_huge_val();
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i >= G.N)
+            break;
+        double xp, carry;
+        if (Rprev) {
+            const uint32_t p = tpar ? tpar[i]
+                                    : parent_of(prev.code[i], prev.E, G.N);
+            xp = prev.x[p];
+            // value of the carried weight: LWn(theta)[p] - LWn(theta0)[p]
+            carry = lwn_ref ? (prev.lw[p] - prev.scal[2]) - lwn_ref[p] : 0.0;
+        } else {
+            xp = prev.x[i];
+            carry = prev.lw[i] - mprev;
+        }
+        const double x = G.a * xp + G.s * eps_at(G, i, t);
+        double g, gx, gc;
+        gmodel(G, y, x, g, gx, gc);
+        const double l = carry + g;
+        cur.x[i] = x;
+        cur.lw[i] = l;
+        lmax = lmax < l ? l : lmax;
+    }
+    // group max of doubles
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const double o = __shfl_xor(lmax, d, 64);
+        lmax = lmax < o ? o : lmax;
+    }
+    if ((gt & 63) == 0)
+        ldsd[gt >> 6] = lmax;
+    __syncthreads();
+    double r = ldsd[0];
+    for (int w = 1; w < kWaves; ++w)
+        r = r < ldsd[w] ? ldsd[w] : r;
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ void gtile_sums(const GradArgs &G, uint32_t tile,
+    const double *lw, double m, int gt, double *ldsd, double &S, double &S2,
+    dd &etot)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    double s = 0.0, s2 = 0.0;
+    dd loc{0.0, 0.0};
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i < G.N) {
+            const double e = exp(lw[i] - m);
+            s += e;
+            s2 += e * e;
+            loc = dd_add(loc, dd{e, 0.0});
+        }
+    }
+    group_sum2(s, s2, gt, ldsd);
+    S = s;
+    S2 = s2;
+    (void)group_excl_scan_dd(loc, gt, ldsd, etot);
+}
+
+__device__ __forceinline__ void gtile_counts(const GradArgs &G, uint32_t tile,
+    const double *lw, double m, double invS, const double *toffh,
+    const double *toffl, const UniformSeq &U, uint32_t *cnt, int gt,
+    double *ldsd, uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    const uint64_t tend = (uint64_t)(tile + 1) * kTile < G.N
+        ? (uint64_t)(tile + 1) * kTile
+        : G.N;
+    double W[kItems];
+    dd loc{0.0, 0.0};
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        W[k] = i < G.N ? exp(lw[i] - m) : 0.0;
+        loc = dd_add(loc, dd{W[k], 0.0});
+    }
+    dd total;
+    const dd toff = group_excl_scan_dd(loc, gt, ldsd, total);
+    const dd base = dd_add(dd{toffh[tile], toffl[tile]}, toff);
+    uint32_t c[kItems];
+    dd run{0.0, 0.0};
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        run = dd_add(run, dd{W[k], 0.0});
+        c[k] = 0;
+        if (i < G.N) {
+            if (i == (uint64_t)G.N - 1) {
+                c[k] = U.M;
+            } else {
+                const double P = i == tend - 1 ? toffh[tile + 1]
+                                               : dd_add(base, run).hi;
+                c[k] = U.count_less(P * invS);
+            }
+            cnt[i] = c[k];
+        }
+    }
+    ldsu[gt] = c[kItems - 1];
+    __syncthreads();
+    uint32_t prev;
+    if (gt > 0)
+        prev = ldsu[gt - 1];
+    else
+        prev = tile == 0 ? 0u : U.count_less(toffh[tile] * invS);
+    __syncthreads();
+    uint32_t es = 0, zs = 0;
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i < G.N) {
+            const uint32_t rep = c[k] - prev;
+            es += rep > 1 ? rep - 1 : 0;
+            zs += rep == 0 ? 1 : 0;
+        }
+        prev = c[k];
+    }
+    uint32_t te, tz;
+    (void)group_excl_scan_u(es, gt, ldsu, te);
+    (void)group_excl_scan_u(zs, gt, ldsu + 4, tz);
+    esum = te;
+    zsum = tz;
+}
+
+// tile_index plus the positions of the zero slots by rank (for the adjoint's
+// gather over children)
+__device__ __forceinline__ void gtile_index(const GradArgs &G, uint32_t tile,
+    const uint32_t *cnt, uint32_t eoff, uint32_t zoff, const SlotPtr &sl,
+    int gt, uint32_t *ldsu)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    uint32_t rep[kItems];
+    uint32_t es = 0, zs = 0;
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        rep[k] = 0;
+        if (i < G.N) {
+            const uint32_t p = i == 0 ? 0u : cnt[i - 1];
+            rep[k] = cnt[i] - p;
+            es += rep[k] > 1 ? rep[k] - 1 : 0;
+            zs += rep[k] == 0 ? 1 : 0;
+        }
+    }
+    uint32_t te, tz;
+    uint32_t ex = group_excl_scan_u(es, gt, ldsu, te) + eoff;
+    uint32_t zx = group_excl_scan_u(zs, gt, ldsu + 4, tz) + zoff;
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i < G.N) {
+            ex += rep[k] > 1 ? rep[k] - 1 : 0;
+            sl.E[i] = ex;
+            if (rep[k] > 0) {
+                sl.code[i] = (uint32_t)i;
+            } else {
+                sl.code[i] = kZeroFlag | zx;
+                sl.zpos[zx] = (uint32_t)i;
+                ++zx;
+            }
+        }
+    }
+}
+
+} // namespace mckl_hip

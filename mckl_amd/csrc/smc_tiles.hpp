@@ -17,9 +17,38 @@
 //            the stable rep->index transform (resample.hpp:416-466).
 #pragma once
 
+#include <algorithm>
+
+#include "common.hpp"
 #include "smc_math.hpp"
 
 namespace mckl_hip {
+
+constexpr int kScratch = 1280; // LDS bytes per 256-thread group
+
+struct GroupLds {
+    double *d;
+    float *f;
+    uint32_t *u;
+};
+
+__device__ __forceinline__ GroupLds group_lds(unsigned char *base)
+{
+    GroupLds g;
+    g.d = reinterpret_cast<double *>(base);
+    g.f = reinterpret_cast<float *>(base + 64);
+    g.u = reinterpret_cast<uint32_t *>(base + 128);
+    return g;
+}
+
+template <typename T>
+static int dev_alloc(T **p, size_t n)
+{
+    *p = nullptr;
+    MCKL_HIP_TRY(hipMalloc(reinterpret_cast<void **>(p),
+        std::max<size_t>(n, 1) * sizeof(T)));
+    return MCKL_HIP_OK;
+}
 
 struct SmcArgs {
     ModelConst mc;
