@@ -10,9 +10,11 @@ inside the timed region.
 Also reported (in "secondary"): the 2^28-word saturation fill, the
 distribution fills (Threefry4x64 / ARS x Normal<float>(0,1) /
 Normal<double>(2.5,0.3), 64 streams x 2^22), the plain SMC on the
-linear-Gaussian configuration (N = 2^20, T = 100), and the batched SMC on
+linear-Gaussian configuration (N = 2^20, T = 100), the batched SMC on
 4096 stochastic-volatility filters (T = 50) against one plain launch
-sequence per filter.
+sequence per filter, and the SMC log-likelihood gradient (256 points x
+N = 2^16 x T = 500, with its finite-difference and Kalman checks).
+`--only LEG` runs one of these legs alone (for profiling).
 
 roofline: the dominant kernel (the U01OO<double> fill) timed with HIP
 events on its own stream inside the timed region; algorithmic bytes =
@@ -39,45 +41,63 @@ def _events():
 
 
 def bench_config1(m, torch, steps, warmup):
+    """One step = rand(rng, n, u32) then rand(rng, U01OO<double>, n, f64),
+    each from a fresh Philox4x32 with the configuration key, through the C
+    ABI (the calls a C++ caller makes; ctypes arguments prepared once).
+    The timed region holds only the steps; per-kernel durations come from a
+    separate back-to-back loop of each fill between two HIP events on the
+    launch stream."""
+    from mckl_amd import _lib
     n = 1 << 24
     u32 = torch.empty(n, dtype=torch.int32, device="cuda")
     f64 = torch.empty(n, dtype=torch.float64, device="cuda")
-    dist = m.U01OODistribution("float64")
-    k_raw = []
-    k_u01 = []
+    lib = _lib.lib
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    key = (ctypes.c_uint64 * 4)(KEY1[0], KEY1[1], 0, 0)
+    eng = _lib.Engine()
+    pe = ctypes.byref(eng)
+    pu = ctypes.c_void_p(u32.data_ptr())
+    pf = ctypes.c_void_p(f64.data_ptr())
+    cn = ctypes.c_size_t(n)
 
-    def step(record):
-        rng = m.Philox4x32(key=KEY1)
-        if record:
-            a, b = _events()
-            a.record()
-        m.rand(rng, n, u32)
-        if record:
-            b.record()
-            k_raw.append((a, b))
-        rng = m.Philox4x32(key=KEY1)
-        if record:
-            a, b = _events()
-            a.record()
-        m.rand(rng, dist, n, f64)
-        if record:
-            b.record()
-            k_u01.append((a, b))
+    def raw():
+        lib.mckl_hip_engine_init(pe, _lib.PHILOX4X32, 32, key)
+        lib.mckl_hip_rand(pe, cn, pu, st)
+
+    def u01():
+        lib.mckl_hip_engine_init(pe, _lib.PHILOX4X32, 32, key)
+        lib.mckl_hip_rand_u01_f64(pe, _lib.U01_OO, cn, pf, st)
 
     for _ in range(warmup):
-        step(False)
+        raw()
+        u01()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        step(True)
+        raw()
+        u01()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    raw_ms = sum(a.elapsed_time(b) for a, b in k_raw) / steps
-    u01_ms = sum(a.elapsed_time(b) for a, b in k_u01) / steps
+    wall = time.perf_counter() - t0
+    # the result of the timed steps is the configuration's stream
+    rng = m.Philox4x32(key=KEY1)
+    chk = torch.empty(n, dtype=torch.int32, device="cuda")
+    m.rand(rng, n, chk)
+    assert torch.equal(chk, u32)
+
+    def kernel_ms(fn, reps=50):
+        fn()
+        torch.cuda.synchronize()
+        a, b = _events()
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
     return {
         "n": n, "wall_s": wall, "bytes_per_step": 12 * n,
-        "raw_ms": raw_ms, "u01_ms": u01_ms,
+        "raw_ms": kernel_ms(raw), "u01_ms": kernel_ms(u01),
     }
 
 
@@ -210,6 +230,90 @@ def bench_smc(m, torch):
     return res
 
 
+def bench_grad(m, torch):
+    """Gradient configuration: SV (0.95, 0.2, 0.7), T = 500 observations
+    from seed 11, N = 2^16, 256 jittered parameter points, checkpoints every
+    50 steps; FD check on two points with the ancestor tape; LG Kalman check
+    (64 seeds, T = 100 from seed 7)."""
+    from mckl_amd import workloads as wl
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import kalman
+    res = {}
+    y = wl.sv_observations(500, 11)
+    th = wl.sv_grad_points(256)
+    N, T, nb = 1 << 16, 500, 256
+    g = m.SMCGradient(m.MODEL_SV, nb, N, y, np.arange(1, nb + 1), 0.5,
+                      m.STRATIFIED, K=50)
+    g.run(th)
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ll, gr = g.run(th)
+    t = (time.perf_counter() - t0) / reps
+    del g
+    ps = float(nb) * N * T
+    # FD check: two points, ancestor tape, central differences h = 1e-4
+    gt = m.SMCGradient(m.MODEL_SV, 2, N, y, [1, 2], 0.5, m.STRATIFIED, K=50,
+                       tape=True)
+    _, g2 = gt.run(th[:2])
+
+    def fd(h):
+        out = np.zeros((2, 3))
+        for j in range(3):
+            tp = th[:2].copy()
+            tm = th[:2].copy()
+            tp[:, j] += h
+            tm[:, j] -= h
+            out[:, j] = (gt.replay(tp) - gt.replay(tm)) / (2 * h)
+        return out
+
+    f1 = fd(1e-4)
+    f2 = fd(5e-5)
+    rich = (4 * f2 - f1) / 3
+    del gt
+    # LG vs Kalman
+    yl = wl.lg_observations(100, 7)
+    lg = (0.9, 1.0, 0.5)
+    gl = m.SMCGradient(m.MODEL_LG, 64, N, yl, np.arange(1, 65), 0.5,
+                       m.SYSTEMATIC, K=50)
+    _, gg = gl.run(lg)
+    del gl
+    ex = kalman.grad(yl, *lg)
+    mean = gg.mean(0)
+    se = gg.std(0, ddof=1) / np.sqrt(64)
+    res["smc_grad_sv_256x2^16_T500"] = {
+        "ms_per_gradient_batch": round(t * 1e3, 2),
+        "forward_particle_steps_per_s": ps / t,
+        "note": "forward + checkpoint recompute + reverse sweep; FP64 states",
+        "max_rel_err_vs_fd_h1e-4": float(np.max(np.abs(f1 - g2) /
+                                              np.abs(g2))),
+        "max_rel_err_vs_richardson_fd": float(np.max(np.abs(rich - g2) /
+                                                   np.abs(g2))),
+        "mean_grad": gr.mean(0).tolist(),
+        "lg_kalman_grad": ex.tolist(), "lg_mean_grad_64_seeds": mean.tolist(),
+        "lg_rel_err": (np.abs(mean - ex) / np.abs(ex)).tolist(),
+        "lg_z": ((mean - ex) / se).tolist(),
+    }
+    return res
+
+
+def _traffic():
+    """HBM bytes per launch of the U01OO<double> fill (FETCH_SIZE +
+    WRITE_SIZE, each scaled by the same counter on a copy of known size),
+    from the rocprofv3 counter summary kept with the profiles; None if that
+    summary is absent."""
+    path = os.path.join(ROOT, "profiles", "u01f64_fill_config1_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return int(float(line.split("=")[1]))
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(steps=10):
     """The reference's own bulk fills (compiled from its headers), one
     thread; else the oracle port.  Sample: the full config-1 step."""
@@ -248,6 +352,10 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--only", choices=["fills", "saturation", "normals", "smc",
+                                       "grad"],
+                    help="run one secondary leg alone and print its JSON "
+                         "(for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the fills and samplers shard by "
@@ -257,6 +365,22 @@ def main():
     import torch
     import mckl_amd as m
 
+    if args.only == "fills":
+        # the config-1 fills plus a device copy of known size (2^24 doubles
+        # read and written), the scale for the HBM byte counters
+        c1 = bench_config1(m, torch, args.steps, args.warmup)
+        src = torch.ones(1 << 24, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({k: v for k, v in c1.items()}), flush=True)
+        return
+    if args.only:
+        leg = {"saturation": bench_saturation, "normals": bench_normals,
+               "smc": bench_smc, "grad": bench_grad}[args.only]
+        print(json.dumps(leg(m, torch)), flush=True)
+        return
     c1 = bench_config1(m, torch, args.steps, args.warmup)
     value = c1["bytes_per_step"] * args.steps / c1["wall_s"] / 1e9
     u01_gbs = 8 * c1["n"] / (c1["u01_ms"] / 1e3) / 1e9
@@ -266,9 +390,13 @@ def main():
                  "u01oo_f64_fill_GBps": round(u01_gbs, 1),
                  "u01oo_f64_fill_ms": round(c1["u01_ms"], 4)}
     if not args.no_secondary:
-        secondary.update(bench_saturation(m, torch))
-        secondary.update(bench_normals(m, torch))
-        secondary.update(bench_smc(m, torch))
+        for name, leg in (("saturation", bench_saturation),
+                          ("normals", bench_normals), ("smc", bench_smc),
+                          ("grad", bench_grad)):
+            t0 = time.perf_counter()
+            secondary.update(leg(m, torch))
+            print("bench: %s leg done in %.1f s" % (name,
+                  time.perf_counter() - t0), file=sys.stderr, flush=True)
     out = {
         "metric": "Philox GB/s vs HBM peak; SMC particle-steps/s plain vs "
                   "batched+adjoint",
@@ -296,7 +424,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(u01_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": _traffic(),
         },
         "cpu_baseline": cpu_baseline(),
         "secondary": secondary,

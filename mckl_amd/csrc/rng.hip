@@ -192,6 +192,51 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(FillBatch batch,
     T *o = out + (uint64_t)blockIdx.y * n;
     const int64_t skip = (int64_t)arg.skip;
     const bool vec_ok = arg.vec_ok != 0;
+    if constexpr (sizeof(T) * EPB == 32) {
+        // 32 output bytes per block.  A wave covers 64 consecutive blocks
+        // (2 KiB of output) per iteration; lane pair (2j, 2j+1) computes
+        // blocks j and 32 + j and the pair swaps one 16-byte half, so that
+        // each of the two stores writes 1 KiB lane-contiguously.
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const int lane = threadIdx.x & 63;
+        const int odd = lane & 1;
+        const uint64_t wave =
+            ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+        const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+        for (uint64_t wb = wave * 64; wb < nblocks; wb += nwaves * 64) {
+            const uint64_t b = wb + (uint64_t)(odd * 32 + (lane >> 1));
+            uint32_t w[Gen::W32];
+            g.block(b, w); // arithmetic only: harmless past nblocks
+            T v[EPB];
+#pragma unroll
+            for (int j = 0; j < EPB; ++j)
+                v[j] = conv(w + j * Conv::Q);
+            const int64_t ew = (int64_t)(wb * EPB) - skip;
+            if (vec_ok && ew >= 0 && ew + 64 * EPB <= (int64_t)n) {
+                // wave-uniform branch: every lane takes part in the swap
+                const u32x4 *src = reinterpret_cast<const u32x4 *>(v);
+                const u32x4 own = odd ? src[1] : src[0];
+                const u32x4 give = odd ? src[0] : src[1];
+                u32x4 recv;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    recv[c] = (uint32_t)__builtin_amdgcn_mov_dpp(
+                        (int)give[c], 0xB1, 0xF, 0xF, false); // quad_perm 1032
+                u32x4 *dst = reinterpret_cast<u32x4 *>(o + ew);
+                dst[lane] = odd ? recv : own;
+                dst[64 + lane] = odd ? own : recv;
+            } else if (b < nblocks) {
+                const int64_t e0 = (int64_t)(b * EPB) - skip;
+#pragma unroll
+                for (int j = 0; j < EPB; ++j) {
+                    const int64_t e = e0 + j;
+                    if (e >= 0 && e < (int64_t)n)
+                        o[e] = v[j];
+                }
+            }
+        }
+        return;
+    }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
          b < nblocks; b += stride) {

@@ -94,3 +94,16 @@ def sv_batch(nfilters=4096, T=50):
         y[f] = sv_from_z(z.cpu().numpy(), T, phi, s, beta)
     seeds = np.arange(1, nfilters + 1, dtype=np.uint64)
     return theta, N, y, seeds
+
+
+def sv_grad_points(nb=256, theta=(0.95, 0.2, 0.7), rel=0.02):
+    """Gradient batch: nb parameter points jittered around theta,
+    theta_p,j = theta_j (1 + rel (u_{3p+j} - 1/2)) with u the U01OO<double>
+    stream of Philox4x32 key {0x6D1A7, 0}, counter 0."""
+    import mckl_amd as m
+    torch = _torch()
+    rng = m.Philox4x32(key=[0x6D1A7, 0])
+    u = torch.empty(3 * nb, dtype=torch.float64, device="cuda")
+    m.rand(rng, m.U01OODistribution("float64"), 3 * nb, u)
+    u = u.cpu().numpy().reshape(nb, 3)
+    return np.asarray(theta, np.float64) * (1.0 + rel * (u - 0.5))
