@@ -13,6 +13,7 @@
 #define MCKL_HIP_HPP
 
 #include <mckl_hip.h>
+#include <hip/hip_runtime_api.h>
 
 #include <array>
 #include <cstddef>
@@ -245,6 +246,17 @@ class Weight
         hip_check(mckl_hip_weight_add_log(n_, d_v, w_, stats_, current_stream()));
     }
     const double *stats() const { return stats_; }
+    /// ESS of the last set/add (weight.hpp:64); synchronises the stream
+    double ess() const
+    {
+        double s[2];
+        if (hipStreamSynchronize(static_cast<hipStream_t>(current_stream())) !=
+                hipSuccess ||
+            hipMemcpy(s, stats_, sizeof(s), hipMemcpyDeviceToHost) !=
+                hipSuccess)
+            throw RuntimeAssert("mckl_hip: reading the ESS failed");
+        return s[0];
+    }
 
   private:
     std::size_t n_;

@@ -103,6 +103,7 @@ int main(int argc, char **argv)
         weight.set_log(v.p);
         std::vector<double> s = st.host(2);
         CHECK(std::fabs(s[0] - double(n)) < 1e-9, "Weight::set_log ESS");
+        CHECK(weight.ess() == s[0], "Weight::ess");
         mckl::Philox4x32 rng(5);
         Dev<std::uint32_t> rep(n), idx(n);
         mckl::ResampleSystematic()(n, n, rng, w.p, rep.p, idx.p);

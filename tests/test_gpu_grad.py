@@ -147,3 +147,41 @@ def test_grad_errors(dev):
         g.replay(SV)
     with pytest.raises(m.MCKLError):
         g.run((1.2, 0.2, 0.7))
+
+
+def test_torch_autograd_wrapper(dev):
+    """The adjoint behind a torch.autograd.Function (test harness only):
+    backward of sum(w * loglik) gives w[:, None] * grad, and autograd's own
+    numerical check of the replayed function agrees with it."""
+    import torch
+    m = _m()
+    y = _obs()["sv_y"][:30]
+    g = m.SMCGradient(2, 2, 2000, y, [5, 6], 0.5, 2, K=8, tape=True)
+
+    class LogLik(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta):
+            ll, gr = g.run(theta.detach().cpu().numpy())
+            ctx.save_for_backward(torch.from_numpy(gr))
+            return torch.from_numpy(ll)
+
+        @staticmethod
+        def backward(ctx, go):
+            (gr,) = ctx.saved_tensors
+            return go[:, None] * gr
+
+    th = torch.tensor([SV, (0.9, 0.25, 0.6)], dtype=torch.float64,
+                      requires_grad=True)
+    w = torch.tensor([0.3, -1.7], dtype=torch.float64)
+    (w * LogLik.apply(th)).sum().backward()
+    _, gr = g.run(th.detach().numpy())
+    assert torch.allclose(th.grad, w[:, None] * torch.from_numpy(gr),
+                          rtol=0, atol=0)
+    # central differences of the replayed function through torch
+    h = 1e-5
+    for j in range(3):
+        e = np.zeros((2, 3))
+        e[:, j] = h
+        base = th.detach().numpy()
+        d = (g.replay(base + e) - g.replay(base - e)) / (2 * h)
+        assert np.allclose(d, gr[:, j], rtol=1e-5, atol=1e-6)
