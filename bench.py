@@ -148,8 +148,13 @@ def bench_normals(m, torch, reps=3):
 
 
 def bench_smc(m, torch):
+    res = bench_smc_plain(m, torch)
+    res.update(bench_smc_batched(m, torch))
+    return res
+
+
+def bench_smc_plain(m, torch):
     from mckl_amd import workloads as wl
-    import numpy as np
     res = {}
     # plain SMC, linear-Gaussian, N = 2^20, T = 100, systematic
     y = wl.lg_observations(100, 7)
@@ -169,7 +174,13 @@ def bench_smc(m, torch):
     res["smc_plain_lg_N2^20_T100"] = {
         "particle_steps_per_s": N * T / t, "ms": round(t * 1e3, 3),
         "loglik": ll, "resampling_steps": int(rs.sum())}
+    return res
 
+
+def bench_smc_batched(m, torch):
+    from mckl_amd import workloads as wl
+    import numpy as np
+    res = {}
     # batched SV filters
     theta, Nf, yb, seeds = wl.sv_batch(4096, 50)
     T = 50
@@ -353,7 +364,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--only", choices=["fills", "saturation", "normals", "smc",
-                                       "grad"],
+                                       "smc_plain", "smc_batched", "grad"],
                     help="run one secondary leg alone and print its JSON "
                          "(for profiling)")
     args = ap.parse_args()
@@ -378,7 +389,9 @@ def main():
         return
     if args.only:
         leg = {"saturation": bench_saturation, "normals": bench_normals,
-               "smc": bench_smc, "grad": bench_grad}[args.only]
+               "smc": bench_smc, "smc_plain": bench_smc_plain,
+               "smc_batched": bench_smc_batched,
+               "grad": bench_grad}[args.only]
         print(json.dumps(leg(m, torch)), flush=True)
         return
     c1 = bench_config1(m, torch, args.steps, args.warmup)

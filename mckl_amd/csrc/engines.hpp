@@ -61,6 +61,7 @@ __device__ __forceinline__ void philox4x32_10(uint32_t k0, uint32_t k1,
 }
 
 struct PhiloxGen {
+    static constexpr int KIND = PHILOX;
     static constexpr int W32 = 4; // u32 words per block
     uint32_t k0, k1;
     uint64_t c0, c1;
@@ -93,12 +94,37 @@ struct PhiloxGen {
 // random/internal/threefry_generic_4x64.hpp:40-108 (schedule),
 // random/internal/threefry_constants.hpp:83-89 (rotations).
 // ------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int l)
+// 64-bit add as a 32-bit add and add-with-carry (v_add_co_u32 +
+// v_addc_co_u32); measured faster in the Threefry rounds than the single
+// 64-bit v_lshl_add_u64 the compiler otherwise selects.
+__device__ __forceinline__ uint64_t add64(uint64_t a, uint64_t b)
 {
-    return (x << l) | (x >> (64 - l));
+    unsigned c, c2;
+    const uint32_t lo = __builtin_addc((uint32_t)a, (uint32_t)b, 0u, &c);
+    const uint32_t hi =
+        __builtin_addc((uint32_t)(a >> 32), (uint32_t)(b >> 32), c, &c2);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// 64-bit rotate left by a constant as two 32-bit funnel shifts
+// (v_alignbit_b32); a rotation by 32 or more first swaps the halves.
+template <int L>
+__device__ __forceinline__ uint64_t rotl64(uint64_t x)
+{
+    static_assert(L > 0 && L < 64, "rotation");
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    const uint32_t a = L >= 32 ? lo : hi;
+    const uint32_t b = L >= 32 ? hi : lo;
+    constexpr int R = L & 31;
+    if (R == 0)
+        return ((uint64_t)a << 32) | b;
+    const uint32_t nh = __builtin_amdgcn_alignbit(a, b, 32 - R);
+    const uint32_t nl = __builtin_amdgcn_alignbit(b, a, 32 - R);
+    return ((uint64_t)nh << 32) | nl;
 }
 
 struct ThreefryGen {
+    static constexpr int KIND = THREEFRY;
     static constexpr int W32 = 8;
     uint64_t k[5];
     uint64_t c[4];
@@ -121,20 +147,20 @@ struct ThreefryGen {
     __device__ __forceinline__ void inject(
         uint64_t &s0, uint64_t &s1, uint64_t &s2, uint64_t &s3) const
     {
-        s0 += k[(S + 0) % 5];
-        s1 += k[(S + 1) % 5];
-        s2 += k[(S + 2) % 5];
-        s3 += k[(S + 3) % 5] + (uint64_t)S;
+        s0 = add64(s0, k[(S + 0) % 5]);
+        s1 = add64(s1, k[(S + 1) % 5]);
+        s2 = add64(s2, k[(S + 2) % 5]);
+        s3 = add64(s3, k[(S + 3) % 5] + (uint64_t)S);
     }
 
     template <int R0, int R1>
     __device__ __forceinline__ static void round(
         uint64_t &s0, uint64_t &s1, uint64_t &s2, uint64_t &s3)
     {
-        s0 += s1;
-        s2 += s3;
-        const uint64_t t1 = rotl64(s1, R0);
-        const uint64_t t3 = rotl64(s3, R1);
+        s0 = add64(s0, s1);
+        s2 = add64(s2, s3);
+        const uint64_t t1 = rotl64<R0>(s1);
+        const uint64_t t3 = rotl64<R1>(s3);
         s1 = s2 ^ t3;
         s3 = s0 ^ t1;
     }
@@ -198,6 +224,7 @@ constexpr int ARS_LDS_WORDS = 256 * ARS_REPL;
 extern __constant__ uint32_t c_aes_t0[256];
 
 struct ArsGen {
+    static constexpr int KIND = ARS;
     static constexpr int W32 = 4;
     uint64_t K0, K1;
     uint64_t c0, c1;

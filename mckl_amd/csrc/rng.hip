@@ -86,6 +86,7 @@ int ensure_device_tables()
 // ---------------------------------------------------------------------------
 constexpr int kMaxBatch = 16; // streams per launch (kernel-argument payload)
 constexpr int kBlock = 256;
+constexpr int kUnroll = 2; // blocks per lane per iteration (16-byte path)
 
 struct FillArg {
     StreamArg s;
@@ -192,6 +193,9 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(FillBatch batch,
     T *o = out + (uint64_t)blockIdx.y * n;
     const int64_t skip = (int64_t)arg.skip;
     const bool vec_ok = arg.vec_ok != 0;
+    // first block holding an output element: lanes start there, so a call
+    // that begins on a block boundary stores lane-aligned
+    const uint64_t bf = (uint64_t)skip / EPB;
     if constexpr (sizeof(T) * EPB == 32) {
         // 32 output bytes per block.  A wave covers 64 consecutive blocks
         // (2 KiB of output) per iteration; lane pair (2j, 2j+1) computes
@@ -200,10 +204,11 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(FillBatch batch,
         typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
         const int lane = threadIdx.x & 63;
         const int odd = lane & 1;
-        const uint64_t wave =
-            ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-        const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-        for (uint64_t wb = wave * 64; wb < nblocks; wb += nwaves * 64) {
+        // the wave index is uniform: keep it (and the loop bounds) scalar
+        const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / 64) +
+            (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / 64);
+        for (uint64_t wb = bf + wave * 64; wb < nblocks; wb += nwaves * 64) {
             const uint64_t b = wb + (uint64_t)(odd * 32 + (lane >> 1));
             uint32_t w[Gen::W32];
             g.block(b, w); // arithmetic only: harmless past nblocks
@@ -237,24 +242,37 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(FillBatch batch,
         }
         return;
     }
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-         b < nblocks; b += stride) {
-        uint32_t w[Gen::W32];
-        g.block(b, w);
-        T v[EPB];
+    // 16 output bytes per block (or 64 for Threefry doubles): each lane
+    // takes kUnroll blocks kBlock apart per iteration, so a workgroup's
+    // stores stay lane-contiguous and two blocks are in flight per lane
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock * kUnroll;
+    for (uint64_t b0 = bf + (uint64_t)blockIdx.x * kBlock * kUnroll +
+             threadIdx.x;
+         b0 < nblocks; b0 += stride) {
+        T v[kUnroll][EPB];
 #pragma unroll
-        for (int j = 0; j < EPB; ++j)
-            v[j] = conv(w + j * Conv::Q);
-        const int64_t e0 = (int64_t)(b * EPB) - skip;
-        if (vec_ok && e0 >= 0 && e0 + EPB <= (int64_t)n) {
-            store_vec(o + e0, v);
-        } else {
+        for (int u = 0; u < kUnroll; ++u) {
+            uint32_t w[Gen::W32];
+            g.block(b0 + (uint64_t)u * kBlock, w);
 #pragma unroll
-            for (int j = 0; j < EPB; ++j) {
-                const int64_t e = e0 + j;
-                if (e >= 0 && e < (int64_t)n)
-                    o[e] = v[j];
+            for (int j = 0; j < EPB; ++j)
+                v[u][j] = conv(w + j * Conv::Q);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint64_t b = b0 + (uint64_t)u * kBlock;
+            if (b >= nblocks)
+                break;
+            const int64_t e0 = (int64_t)(b * EPB) - skip;
+            if (vec_ok && e0 >= 0 && e0 + EPB <= (int64_t)n) {
+                store_vec(o + e0, v[u]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < EPB; ++j) {
+                    const int64_t e = e0 + j;
+                    if (e >= 0 && e < (int64_t)n)
+                        o[e] = v[u][j];
+                }
             }
         }
     }
@@ -604,9 +622,15 @@ static int launch_fill(const std::vector<FillArg> &args, Conv conv,
         const size_t cnt = std::min<size_t>(kMaxBatch, args.size() - b0);
         for (size_t i = 0; i < cnt; ++i)
             batch.a[i] = args[b0 + i];
-        const uint64_t want = ceil_div(nblocks, kBlock);
-        const uint64_t cap = (uint64_t)grid_cap(
-                                 LdsWords<Gen>::value > 1 ? ARS : PHILOX) /
+        constexpr uint64_t EPB = Gen::W32 / Conv::Q;
+        uint64_t bf = nblocks;
+        for (size_t i = 0; i < cnt; ++i)
+            bf = std::min<uint64_t>(bf, batch.a[i].skip / EPB);
+        const uint64_t per_group = (sizeof(typename Conv::T) * EPB == 32)
+            ? (uint64_t)kBlock
+            : (uint64_t)kBlock * kUnroll;
+        const uint64_t want = ceil_div(nblocks - bf, per_group);
+        const uint64_t cap = (uint64_t)grid_cap(Gen::KIND) /
             std::max<size_t>(1, cnt);
         const uint32_t gx = (uint32_t)std::max<uint64_t>(
             1, std::min<uint64_t>(want, std::max<uint64_t>(1, cap)));
@@ -652,7 +676,7 @@ static int launch_normal(const std::vector<FillArg> &args, uint64_t n, T mean,
             batch.a[i] = args[b0 + i];
         if (ntask) {
             const uint64_t cap = std::max<uint64_t>(1,
-                (uint64_t)grid_cap(LdsWords<Gen>::value > 1 ? ARS : PHILOX) /
+                (uint64_t)grid_cap(Gen::KIND) /
                     std::max<size_t>(1, cnt));
             const uint32_t gx = (uint32_t)std::max<uint64_t>(
                 1, std::min<uint64_t>(ceil_div(ntask, kBlock), cap));

@@ -20,19 +20,23 @@
 
 namespace mckl_hip {
 
-// per-filter device state of the plain sampler
+// Device state of the plain sampler.  Per-step values live in two slots
+// indexed by step parity: step t reads slot (t-1)&1 and writes slot t&1, so
+// the workgroups of one launch that read the previous values never race the
+// one that writes the new ones.
 struct SmcState {
     double logz;
-    double S, S2;   // sums of the current step
-    double C;       // normaliser of the carried weights for the next step
-    float m;        // max log weight of the current step
-    int resampled;  // current step resampled (applies to the next move)
-    uint32_t counter; // last-block counter
-    uint32_t pad;
+    double C[2];      // normaliser of the carried weights for the next step
+    float m[2];       // max log weight of the step
+    int resampled[2]; // the step resampled (applies to the next move)
 };
 
 // -------------------------------------------------------------------------
-// plain sampler kernels (grid = ntiles workgroups of 256 threads)
+// plain sampler kernels (grid = ntiles workgroups of 256 threads).  No
+// device-wide fences or atomics: a value every tile needs (max, sums, the
+// resampling decision, tile offsets) is recomputed by each workgroup from
+// the per-tile arrays with the canonical cross-tile functions, and only
+// workgroup 0 writes it to the state.
 // -------------------------------------------------------------------------
 __global__ __launch_bounds__(kGroup) void k_init(
     SmcArgs A, float *x, float *lw, SmcState *st)
@@ -40,53 +44,48 @@ __global__ __launch_bounds__(kGroup) void k_init(
     tile_init(A, blockIdx.x, x, lw, threadIdx.x);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->logz = 0.0;
-        st->S = (double)A.N;
-        st->S2 = (double)A.N;
-        st->C = (double)A.N;
-        st->m = 0.0f;
-        st->resampled = 0;
-        st->counter = 0;
+        st->C[0] = (double)A.N;
+        st->m[0] = 0.0f;
+        st->resampled[0] = 0;
     }
 }
 
 struct RsBuf {
-    double *tdh, *tdl;   // tile dd totals of e
-    double *toh, *tol;   // exclusive tile offsets (ntiles + 1)
-    uint32_t *te, *tz;   // tile extra-copy / zero-slot counts
-    uint32_t *teo, *tzo; // their exclusive offsets (ntiles + 1)
-    uint32_t *cnt, *Eincl;
+    double *tdh, *tdl; // tile dd totals of e
+    uint32_t *te, *tz; // tile extra-copy / zero-slot counts
+    uint32_t *cnt, *Eincl, *zpos;
 };
 
 __global__ __launch_bounds__(kGroup) void k_move(SmcArgs A, uint32_t t,
     const float *xin, float *xout, float *lw, const uint32_t *idx,
-    const uint32_t *Eincl, const SmcState *st, float *tmax)
+    const SmcState *st, float *tmax)
 {
     __shared__ __align__(16) unsigned char lds[kScratch];
     const GroupLds L = group_lds(lds);
-    const int res = st->resampled;
-    const float mprev = st->m;
-    const float mt = tile_move(A, t, blockIdx.x, xin, xout, lw, idx, Eincl,
-        res, mprev, threadIdx.x, L.f);
+    const int p = (t - 1) & 1;
+    const float mt = tile_move(A, t, blockIdx.x, xin, xout, lw, idx,
+        st->resampled[p], st->m[p], threadIdx.x, L.f);
     if (threadIdx.x == 0)
         tmax[blockIdx.x] = mt;
 }
 
-// Sums per tile.  The last workgroup to finish reduces across tiles, takes
-// the step's decisions and, when the step resamples, scans the tile totals
-// of e for the cumulative weights.
-__global__ __launch_bounds__(kGroup) void k_sums(SmcArgs A, uint32_t t,
-    const float *lw, const float *tmax, double *ts, double *ts2, RsBuf B,
-    SmcState *st, double *ess_hist, int32_t *res_hist)
+__device__ __forceinline__ float tiles_max(
+    const float *tmax, uint32_t nt, int gt, float *ldsf)
+{
+    float m = -__builtin_huge_valf();
+    for (uint32_t i = gt; i < nt; i += kGroup)
+        m = fmaxf(m, tmax[i]);
+    return group_max(m, gt, ldsf);
+}
+
+// tile sums of e = exp_f(lw - m) and their double-double totals
+__global__ __launch_bounds__(kGroup) void k_sums(SmcArgs A, const float *lw,
+    const float *tmax, double *ts, double *ts2, RsBuf B)
 {
     __shared__ __align__(16) unsigned char lds[kScratch];
-    __shared__ int last;
-    __shared__ int resample;
     const GroupLds L = group_lds(lds);
     const int gt = threadIdx.x;
-    float m = -__builtin_huge_valf();
-    for (uint32_t i = gt; i < A.ntiles; i += kGroup)
-        m = fmaxf(m, tmax[i]);
-    m = group_max(m, gt, L.f);
+    const float m = tiles_max(tmax, A.ntiles, gt, L.f);
     double S, S2;
     dd et;
     tile_sums(A, blockIdx.x, lw, m, gt, L.d, S, S2, et);
@@ -95,73 +94,88 @@ __global__ __launch_bounds__(kGroup) void k_sums(SmcArgs A, uint32_t t,
         ts2[blockIdx.x] = S2;
         B.tdh[blockIdx.x] = et.hi;
         B.tdl[blockIdx.x] = et.lo;
-        __threadfence();
-        last = atomicAdd(&st->counter, 1u) == A.ntiles - 1;
     }
-    __syncthreads();
-    if (!last)
-        return;
-    __threadfence();
+}
+
+// Every workgroup reduces the tile sums and takes the step's decision;
+// workgroup 0 records it.  When the step resamples, each workgroup derives
+// its tile's prefix offsets and counts the uniforms below its cumulative
+// weights.
+__global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
+    const float *lw, const float *tmax, const double *ts, const double *ts2,
+    RsBuf B, SmcState *st, double *ess_hist, int32_t *res_hist)
+{
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    const GroupLds L = group_lds(lds);
+    const int gt = threadIdx.x;
+    const int nt = (int)A.ntiles;
+    const float m = tiles_max(tmax, A.ntiles, gt, L.f);
     double rs, rs2;
-    tiles_sum2(ts, ts2, A.ntiles, gt, L.d, rs, rs2);
-    if (gt == 0) {
-        const StepOut o = step_finalize(A, m, rs, rs2, st->C);
+    tiles_sum2(ts, ts2, nt, gt, L.d, rs, rs2);
+    const int p = (t - 1) & 1, c = t & 1;
+    const StepOut o = step_finalize(A, m, rs, rs2, st->C[p]);
+    if (blockIdx.x == 0 && gt == 0) {
         st->logz = __dadd_rn(st->logz, o.logz_inc);
-        st->S = rs;
-        st->S2 = rs2;
-        st->m = m;
-        st->resampled = o.resample;
-        st->C = o.resample ? (double)A.N : rs;
-        st->counter = 0;
-        resample = o.resample;
+        st->m[c] = m;
+        st->resampled[c] = o.resample;
+        st->C[c] = o.resample ? (double)A.N : rs;
         if (ess_hist)
             ess_hist[t - 1] = o.ess;
         if (res_hist)
             res_hist[t - 1] = o.resample;
     }
-    __syncthreads();
-    if (resample)
-        tiles_scan_dd(true, B.tdh, B.tdl, A.ntiles, B.toh, B.tol, gt, L.d);
-}
-
-__global__ __launch_bounds__(kGroup) void k_rs_counts(
-    SmcArgs A, uint32_t t, const float *lw, SmcState *st, RsBuf B)
-{
-    if (!st->resampled)
+    if (!o.resample)
         return;
-    __shared__ __align__(16) unsigned char lds[kScratch];
-    __shared__ int last;
-    const GroupLds L = group_lds(lds);
-    const int gt = threadIdx.x;
-    const double invS = __ddiv_rn(1.0, st->S);
+    const uint32_t k = blockIdx.x;
+    dd off, next;
+    tiles_offset_dd(B.tdh, B.tdl, nt, (int)k, (int)k + 1, gt, L.d, off, next);
+    const double invS = __ddiv_rn(1.0, rs);
     const UniformSeq U = make_uniform_seq(A, t);
     uint32_t es, zs;
-    tile_counts(A, blockIdx.x, lw, st->m, invS, B.toh, B.tol, U, B.cnt, gt,
-        L.d, L.u, es, zs);
+    tile_counts(A, k, lw, m, invS, off, next.hi, U, B.cnt, gt, L.d, L.u, es,
+        zs);
     if (gt == 0) {
-        B.te[blockIdx.x] = es;
-        B.tz[blockIdx.x] = zs;
-        __threadfence();
-        last = atomicAdd(&st->counter, 1u) == A.ntiles - 1;
+        B.te[k] = es;
+        B.tz[k] = zs;
     }
-    __syncthreads();
-    if (!last)
-        return;
-    __threadfence();
-    tiles_scan_u2(true, B.te, B.tz, A.ntiles, B.teo, B.tzo, gt, L.u);
-    if (gt == 0)
-        st->counter = 0;
 }
 
-__global__ __launch_bounds__(kGroup) void k_rs_index(
-    SmcArgs A, const SmcState *st, RsBuf B, uint32_t *idx)
+// exclusive prefix of a u32 tile array up to tile k (all threads)
+__device__ __forceinline__ void tiles_prefix_u2(const uint32_t *a,
+    const uint32_t *b, uint32_t k, int gt, uint32_t *ldsu, uint32_t &pa,
+    uint32_t &pb)
 {
-    if (!st->resampled)
+    uint32_t sa = 0, sb = 0;
+    for (uint32_t i = gt; i < k; i += kGroup) {
+        sa += a[i];
+        sb += b[i];
+    }
+    uint32_t ta, tb;
+    (void)group_excl_scan_u(sa, gt, ldsu, ta);
+    (void)group_excl_scan_u(sb, gt, ldsu + 4, tb);
+    pa = ta;
+    pb = tb;
+}
+
+__global__ __launch_bounds__(kGroup) void k_rs_index(SmcArgs A, uint32_t t,
+    const SmcState *st, RsBuf B, uint32_t *idx)
+{
+    if (!st->resampled[t & 1])
         return;
     __shared__ __align__(16) unsigned char lds[kScratch];
     const GroupLds L = group_lds(lds);
-    tile_index(A, blockIdx.x, B.cnt, B.teo[blockIdx.x], B.tzo[blockIdx.x],
-        B.Eincl, idx, threadIdx.x, L.u);
+    uint32_t eoff, zoff;
+    tiles_prefix_u2(B.te, B.tz, blockIdx.x, threadIdx.x, L.u, eoff, zoff);
+    tile_index(A, blockIdx.x, B.cnt, eoff, zoff, B.Eincl, idx, B.zpos,
+        threadIdx.x, L.u);
+}
+
+__global__ __launch_bounds__(kGroup) void k_rs_scatter(SmcArgs A, uint32_t t,
+    const SmcState *st, RsBuf B, uint32_t *idx)
+{
+    if (!st->resampled[t & 1])
+        return;
+    tile_scatter(A, blockIdx.x, B.cnt, B.Eincl, B.zpos, idx, threadIdx.x);
 }
 
 // -------------------------------------------------------------------------
@@ -189,7 +203,7 @@ template <int G>
 __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
     const uint32_t *order, uint32_t nfilters, uint32_t *queue, float *xa,
     float *xb, float *lwb, uint32_t *idxb, uint32_t *cntb, uint32_t *Eb,
-    double *logz_out)
+    uint32_t *zb, double *logz_out)
 {
     __shared__ __align__(16) unsigned char scratch[G * kScratch];
     __shared__ BatchLds S;
@@ -210,6 +224,7 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
         const uint64_t off = fd[f].off;
         float *x0 = xa + off, *x1 = xb + off, *lw = lwb + off;
         uint32_t *idx = idxb + off, *cnt = cntb + off, *Eincl = Eb + off;
+        uint32_t *zpos = zb + off;
         const int nt = (int)A.ntiles;
         const int rounds = (nt + G - 1) / G;
 
@@ -229,7 +244,7 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
             for (int r = 0; r < rounds; ++r) {
                 const int k = r * G + grp;
                 const uint32_t kk = (uint32_t)(k < nt ? k : nt);
-                const float mt = tile_move(A, t, kk, xin, xout, lw, idx, Eincl,
+                const float mt = tile_move(A, t, kk, xin, xout, lw, idx,
                     resampled, mprev, gt, L.f);
                 if (gt == 0 && k < nt)
                     S.tmax[k] = mt;
@@ -266,8 +281,10 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
                 for (int r = 0; r < rounds; ++r) {
                     const int k = r * G + grp;
                     uint32_t es, zs;
-                    tile_counts(A, (uint32_t)(k < nt ? k : nt), lw, m, invS,
-                        S.toh, S.tol, U, cnt, gt, L.d, L.u, es, zs);
+                    const int kk = k < nt ? k : nt;
+                    tile_counts(A, (uint32_t)kk, lw, m, invS,
+                        dd{S.toh[kk], S.tol[kk]}, kk < nt ? S.toh[kk + 1] : 0.0,
+                        U, cnt, gt, L.d, L.u, es, zs);
                     if (gt == 0 && k < nt) {
                         S.te[k] = es;
                         S.tz[k] = zs;
@@ -279,9 +296,12 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
                     const int k = r * G + grp;
                     const int kk = k < nt ? k : nt;
                     tile_index(A, (uint32_t)kk, cnt, S.teo[kk], S.tzo[kk],
-                        Eincl, idx, gt, L.u);
+                        Eincl, idx, zpos, gt, L.u);
                     __syncthreads();
                 }
+                for (int k = grp; k < nt; k += G)
+                    tile_scatter(A, (uint32_t)k, cnt, Eincl, zpos, idx, gt);
+                __syncthreads();
             }
         }
         if (threadIdx.x == 0)
@@ -371,8 +391,8 @@ static void smc_free(mckl_hip_smc *s)
     if (s->graph)
         (void)hipGraphDestroy(s->graph);
     void *ptrs[] = {s->x[0], s->x[1], s->lw, s->obs, s->tmax, s->idx, s->ts,
-        s->ts2, s->B.tdh, s->B.tdl, s->B.toh, s->B.tol, s->B.te, s->B.tz,
-        s->B.teo, s->B.tzo, s->B.cnt, s->B.Eincl, s->st, s->ess, s->res};
+        s->ts2, s->B.tdh, s->B.tdl, s->B.te, s->B.tz, s->B.cnt, s->B.Eincl,
+        s->B.zpos, s->st, s->ess, s->res};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -426,14 +446,11 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
     MCKL_ALLOC(s->ts2, nt);
     MCKL_ALLOC(s->B.tdh, nt);
     MCKL_ALLOC(s->B.tdl, nt);
-    MCKL_ALLOC(s->B.toh, nt + 1);
-    MCKL_ALLOC(s->B.tol, nt + 1);
     MCKL_ALLOC(s->B.te, nt);
     MCKL_ALLOC(s->B.tz, nt);
-    MCKL_ALLOC(s->B.teo, nt + 1);
-    MCKL_ALLOC(s->B.tzo, nt + 1);
     MCKL_ALLOC(s->B.cnt, Np);
     MCKL_ALLOC(s->B.Eincl, Np);
+    MCKL_ALLOC(s->B.zpos, Np);
     MCKL_ALLOC(s->st, 1);
     MCKL_ALLOC(s->ess, T);
     MCKL_ALLOC(s->res, T);
@@ -454,8 +471,9 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
     return MCKL_HIP_OK;
 }
 
-// Enqueue the whole run: init, then per step move, sums (+decision), counts
-// and index (the last two exit at once on steps that do not resample).
+// Enqueue the whole run: init, then per step move, sums, the decision with
+// the resampling counts, the rep->index transform and the zero-slot scatter
+// (the last three exit at once on steps that do not resample).
 static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
 {
     const dim3 grid(s->A.ntiles);
@@ -465,11 +483,13 @@ static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
         const float *xin = s->x[(t - 1) & 1];
         float *xout = s->x[t & 1];
         k_move<<<grid, kGroup, 0, st>>>(
-            s->A, t, xin, xout, s->lw, s->idx, s->B.Eincl, s->st, s->tmax);
-        k_sums<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->tmax, s->ts, s->ts2,
-            s->B, s->st, s->ess, s->res);
-        k_rs_counts<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->st, s->B);
-        k_rs_index<<<grid, kGroup, 0, st>>>(s->A, s->st, s->B, s->idx);
+            s->A, t, xin, xout, s->lw, s->idx, s->st, s->tmax);
+        k_sums<<<grid, kGroup, 0, st>>>(
+            s->A, s->lw, s->tmax, s->ts, s->ts2, s->B);
+        k_rs_counts<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->tmax, s->ts,
+            s->ts2, s->B, s->st, s->ess, s->res);
+        k_rs_index<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);
+        k_rs_scatter<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);
     }
     MCKL_HIP_CHECK_LAUNCH("smc step kernels");
     return MCKL_HIP_OK;
@@ -544,7 +564,7 @@ struct mckl_hip_smc_batch {
     uint32_t *order = nullptr;
     uint32_t *queue = nullptr;
     float *xa = nullptr, *xb = nullptr, *lw = nullptr, *obs = nullptr;
-    uint32_t *idx = nullptr, *cnt = nullptr, *E = nullptr;
+    uint32_t *idx = nullptr, *cnt = nullptr, *E = nullptr, *zpos = nullptr;
     double *logz = nullptr;
     mckl_hip_tune tune{256, 0};
     int persistent = 0; // workgroups in the persistent grid
@@ -561,7 +581,7 @@ static void batch_free(mckl_hip_smc_batch *b)
     if (b->graph)
         (void)hipGraphDestroy(b->graph);
     void *ptrs[] = {b->fd, b->order, b->queue, b->xa, b->xb, b->lw, b->obs,
-        b->idx, b->cnt, b->E, b->logz};
+        b->idx, b->cnt, b->E, b->zpos, b->logz};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -577,15 +597,17 @@ static int batch_launch(mckl_hip_smc_batch *b, hipStream_t st)
     switch (G) {
         case 1:
             k_batch<1><<<grid, kGroup, 0, st>>>(b->fd, b->order, nf, b->queue,
-                b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->logz);
+                b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->zpos, b->logz);
             break;
         case 2:
             k_batch<2><<<grid, 2 * kGroup, 0, st>>>(b->fd, b->order, nf,
-                b->queue, b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->logz);
+                b->queue, b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->zpos,
+                b->logz);
             break;
         default:
             k_batch<4><<<grid, 4 * kGroup, 0, st>>>(b->fd, b->order, nf,
-                b->queue, b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->logz);
+                b->queue, b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->zpos,
+                b->logz);
             break;
     }
     MCKL_HIP_CHECK_LAUNCH("k_batch");
@@ -691,6 +713,7 @@ int mckl_hip_smc_batch_create(int model, size_t nfilters, const uint32_t *h_N,
     MCKL_ALLOC(b->idx, total);
     MCKL_ALLOC(b->cnt, total);
     MCKL_ALLOC(b->E, total);
+    MCKL_ALLOC(b->zpos, total);
     MCKL_ALLOC(b->obs, nfilters * T);
     MCKL_ALLOC(b->logz, nfilters);
 #undef MCKL_ALLOC

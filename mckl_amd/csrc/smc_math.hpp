@@ -354,6 +354,54 @@ __device__ __forceinline__ void tiles_scan_dd(bool write, const double *hi,
     __syncthreads();
 }
 
+// The entries j0 and j1 (0 <= j <= n) of the offsets tiles_scan_dd writes,
+// computed with the same operations in the same order (the thread owning
+// tile j adds its chunk's tiles before j to its exclusive prefix; j = n is
+// the total), without writing the array.  Lets every workgroup of a launch
+// derive its own tile offsets.  lds >= 8 doubles + 4 doubles of exchange.
+__device__ __forceinline__ void tiles_offset_dd(const double *hi,
+    const double *lo, int n, int j0, int j1, int gt, double *lds, dd &o0,
+    dd &o1)
+{
+    const int chunk = (n + kGroup - 1) / kGroup;
+    const int b = gt * chunk;
+    const int e = b + chunk < n ? b + chunk : n;
+    dd loc{0.0, 0.0};
+    for (int i = b; i < e; ++i)
+        loc = dd_add(loc, dd{hi[i], lo[i]});
+    dd total;
+    const dd off = group_excl_scan_dd(loc, gt, lds, total);
+    double *x = lds + 8;
+    if (b < n || gt == 0) {
+        dd run = off;
+        for (int i = b; i < e; ++i) {
+            if (i == j0) {
+                x[0] = run.hi;
+                x[1] = run.lo;
+            }
+            if (i == j1) {
+                x[2] = run.hi;
+                x[3] = run.lo;
+            }
+            run = dd_add(run, dd{hi[i], lo[i]});
+        }
+    }
+    if (gt == 0) {
+        if (j0 == n) {
+            x[0] = total.hi;
+            x[1] = total.lo;
+        }
+        if (j1 == n) {
+            x[2] = total.hi;
+            x[3] = total.lo;
+        }
+    }
+    __syncthreads();
+    o0 = dd{x[0], x[1]};
+    o1 = dd{x[2], x[3]};
+    __syncthreads();
+}
+
 // exclusive u32 scans of two arrays over n tiles; off arrays get n+1 entries
 __device__ __forceinline__ void tiles_scan_u2(bool write, const uint32_t *a,
     const uint32_t *b2, int n, uint32_t *oa, uint32_t *ob, int gt,

@@ -104,9 +104,10 @@ __device__ __forceinline__ uint32_t upper_bound_u(
     return lo;
 }
 
-// Parent of slot i after the stable rep->index transform: the slot itself
-// when it kept a copy, else the source of the extra copy with the slot's
-// rank among the zero slots (tile_index writes kZeroFlag | rank).
+// Parent of slot i after the stable rep->index transform, from a code that
+// is the slot itself when it kept a copy, else kZeroFlag | the slot's rank
+// among the zero slots (the source of that extra copy is the first j with
+// E_j > rank).  Used by the gradient sampler, which keeps these codes.
 __device__ __forceinline__ uint32_t parent_of(
     uint32_t code, const uint32_t *Eincl, uint32_t N)
 {
@@ -114,11 +115,11 @@ __device__ __forceinline__ uint32_t parent_of(
                               : code;
 }
 
-// move + weight of one tile; returns the tile max of lw (all threads)
+// move + weight of one tile; returns the tile max of lw (all threads).
+// After a resampling step idx holds every slot's resolved parent.
 __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
     uint32_t tile, const float *xin, float *xout, float *lw,
-    const uint32_t *idx, const uint32_t *Eincl, int resampled, float mprev,
-    int gt, float *ldsf)
+    const uint32_t *idx, int resampled, float mprev, int gt, float *ldsf)
 {
     const uint64_t i0 = tile_first(tile, gt);
     const float obs = A.obs[t - 1];
@@ -132,7 +133,7 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
                 a1.z, a1.w};
 #pragma unroll
             for (int k = 0; k < kItems; ++k) {
-                xp[k] = xin[parent_of(a[k], Eincl, A.N)];
+                xp[k] = xin[a[k]];
                 lo[k] = 0.0f;
             }
         } else {
@@ -182,7 +183,7 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
                     break;
                 float xv, carry;
                 if (resampled) {
-                    xv = xin[parent_of(idx[p], Eincl, A.N)];
+                    xv = xin[idx[p]];
                     carry = 0.0f;
                 } else {
                     xv = xin[p];
@@ -256,15 +257,16 @@ struct UniformSeq {
 
 // Cumulative normalised weights S_j = round(prefix_j(e)) * (1/S), prefix in
 // double-double over tiles (tile offsets from tiles_scan_dd of the tile
-// totals of tile_sums), and the counts cnt_j = #{u < S_j}.  The last particle
-// of a tile takes its prefix from the next tile offset so that neighbouring
-// tiles agree on boundary values; the last particle overall takes cnt = M
+// totals of tile_sums: toff for this tile, next_hi the high part of the next
+// tile's), and the counts cnt_j = #{u < S_j}.  The last particle of a tile
+// takes its prefix from the next tile offset so that neighbouring tiles agree
+// on boundary values; the last particle overall takes cnt = M
 // (resample.hpp:405).  Returns the tile sums of the extra-copy and zero-slot
 // counts through esum/zsum.
 __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
-    const float *lw, float m, double invS, const double *toffh,
-    const double *toffl, const UniformSeq &U, uint32_t *cnt, int gt,
-    double *ldsd, uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
+    const float *lw, float m, double invS, dd tile_off, double next_hi,
+    const UniformSeq &U, uint32_t *cnt, int gt, double *ldsd, uint32_t *ldsu,
+    uint32_t &esum, uint32_t &zsum)
 {
     const uint64_t i0 = tile_first(tile, gt);
     const uint64_t tend = (uint64_t)(tile + 1) * kTile < A.N
@@ -280,7 +282,7 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
     }
     dd total;
     const dd toff = group_excl_scan_dd(loc, gt, ldsd, total);
-    const dd base = dd_add(dd{toffh[tile], toffl[tile]}, toff);
+    const dd base = dd_add(tile_off, toff);
     uint32_t c[kItems];
     dd run{0.0, 0.0};
 #pragma unroll
@@ -292,8 +294,8 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
             if (i == (uint64_t)A.N - 1) {
                 c[k] = U.M;
             } else {
-                const double P = i == tend - 1 ? toffh[tile + 1]
-                                               : dd_add(base, run).hi;
+                const double P =
+                    i == tend - 1 ? next_hi : dd_add(base, run).hi;
                 c[k] = U.count_less(__dmul_rn(P, invS));
             }
             cnt[i] = c[k];
@@ -306,7 +308,7 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
     if (gt > 0)
         prev = ldsu[gt - 1];
     else
-        prev = tile == 0 ? 0u : U.count_less(__dmul_rn(toffh[tile], invS));
+        prev = tile == 0 ? 0u : U.count_less(__dmul_rn(tile_off.hi, invS));
     __syncthreads();
     uint32_t es = 0, zs = 0;
 #pragma unroll
@@ -328,10 +330,12 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
 
 // Stable rep->index (resample.hpp:416-466, M = N): idx_i = i when rep_i > 0;
 // zero slots take extra copies in ascending source order.  Writes the
-// inclusive extra-copy prefix E_i and, for zero slots, kZeroFlag | rank.
+// inclusive extra-copy prefix E_i, idx_i = i for the slots that keep a copy,
+// and zpos[r] = i for the zero slot of rank r; tile_scatter then fills the
+// zero slots.
 __device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
     const uint32_t *cnt, uint32_t eoff, uint32_t zoff, uint32_t *Eincl,
-    uint32_t *idx, int gt, uint32_t *ldsu)
+    uint32_t *idx, uint32_t *zpos, int gt, uint32_t *ldsu)
 {
     const uint64_t i0 = tile_first(tile, gt);
     uint32_t rep[kItems];
@@ -359,8 +363,30 @@ __device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
             if (rep[k] > 0) {
                 idx[i] = (uint32_t)i;
             } else {
-                idx[i] = kZeroFlag | zx;
+                zpos[zx] = (uint32_t)i;
                 ++zx;
+            }
+        }
+    }
+}
+
+// Source j with rep_j > 1 owns the extra copies of ranks
+// [E_j - (rep_j - 1), E_j) and writes itself into those zero slots.
+__device__ __forceinline__ void tile_scatter(const SmcArgs &A, uint32_t tile,
+    const uint32_t *cnt, const uint32_t *Eincl, const uint32_t *zpos,
+    uint32_t *idx, int gt)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t j = i0 + k;
+        if (j < A.N) {
+            const uint32_t p = j == 0 ? 0u : cnt[j - 1];
+            const uint32_t rep = cnt[j] - p;
+            if (rep > 1) {
+                const uint32_t e = Eincl[j];
+                for (uint32_t r = e - (rep - 1); r < e; ++r)
+                    idx[zpos[r]] = (uint32_t)j;
             }
         }
     }
