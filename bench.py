@@ -325,6 +325,56 @@ def _traffic():
     return None
 
 
+def _guarded(fn, *a):
+    try:
+        return fn(*a)
+    except Exception as exc:  # a CPU side figure must not sink the GPU line
+        return {"error": repr(exc)}
+
+
+def cpu_reference_points(threads):
+    """CPU figures beside the secondary GPU legs, on this host: the
+    reference compiled from its headers (Normal fills) and the oracle port
+    (SMC; the reference has no counterpart of these samplers).  Bounded
+    samples: 2^22 normals, the plain configuration over its first 10 steps,
+    the first 64 filters of the batched configuration."""
+    import numpy as np
+    from mckl_amd import workloads as wl
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    res = {"threads": threads}
+    if orc.REF is not None:
+        n = 1 << 22
+        for ename, kind, dname, dt, mean, sd in (
+                ("threefry4x64", orc.THREEFRY4X64, "f32", np.float32, 0.0, 1.0),
+                ("ars", orc.ARS, "f64", np.float64, 2.5, 0.3)):
+            t0 = time.perf_counter()
+            orc.ref_normal(kind, [1, 0, 0, 0], dt, n, mean, sd)
+            t = time.perf_counter() - t0
+            res["normal_%s_%s_reference_1thread_Gsamples_per_s" % (
+                ename, dname)] = round(n / t / 1e9, 4)
+    y = wl.lg_observations(100, 7)[:10]
+    N = 1 << 20
+    for nt in (1, threads):
+        t0 = time.perf_counter()
+        orc.smc_run(1, (0.9, 1.0, 0.5), y, N, 1, 0.5, orc.SYSTEMATIC,
+                    nthreads=nt)
+        t = time.perf_counter() - t0
+        res["smc_plain_lg_port_%dthreads_particle_steps_per_s" % nt] = \
+            N * len(y) / t
+    theta, Nf, yb, seeds = wl.sv_batch(4096, 50)
+    ps = 0.0
+    t0 = time.perf_counter()
+    for f in range(64):
+        orc.smc_run(2, theta[f], yb[f], int(Nf[f]), int(seeds[f]), 0.5,
+                    orc.STRATIFIED, nthreads=threads)
+        ps += float(Nf[f]) * 50
+    t = time.perf_counter() - t0
+    res["smc_batched_sv_port_%dthreads_particle_steps_per_s" % threads] = \
+        ps / t
+    return res
+
+
 def cpu_baseline(steps=10):
     """The reference's own bulk fills (compiled from its headers), one
     thread; else the oracle port.  Sample: the full config-1 step."""
@@ -403,9 +453,13 @@ def main():
                  "u01oo_f64_fill_GBps": round(u01_gbs, 1),
                  "u01oo_f64_fill_ms": round(c1["u01_ms"], 4)}
     if not args.no_secondary:
+        threads = max(1, min(16, os.cpu_count() or 1))
         for name, leg in (("saturation", bench_saturation),
                           ("normals", bench_normals), ("smc", bench_smc),
-                          ("grad", bench_grad)):
+                          ("grad", bench_grad),
+                          ("cpu", lambda m_, t_: {
+                              "cpu": _guarded(cpu_reference_points,
+                                              threads)})):
             t0 = time.perf_counter()
             secondary.update(leg(m, torch))
             print("bench: %s leg done in %.1f s" % (name,

@@ -175,7 +175,9 @@ __global__ __launch_bounds__(kGroup) void k_rs_scatter(SmcArgs A, uint32_t t,
 {
     if (!st->resampled[t & 1])
         return;
-    tile_scatter(A, blockIdx.x, B.cnt, B.Eincl, B.zpos, idx, threadIdx.x);
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    const GroupLds L = group_lds(lds);
+    tile_scatter(A, blockIdx.x, B.cnt, B.Eincl, B.zpos, idx, threadIdx.x, L.u);
 }
 
 // -------------------------------------------------------------------------
@@ -299,9 +301,11 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
                         Eincl, idx, zpos, gt, L.u);
                     __syncthreads();
                 }
-                for (int k = grp; k < nt; k += G)
-                    tile_scatter(A, (uint32_t)k, cnt, Eincl, zpos, idx, gt);
-                __syncthreads();
+                for (int r = 0; r < rounds; ++r) {
+                    const int k = r * G + grp;
+                    tile_scatter(A, (uint32_t)(k < nt ? k : nt), cnt, Eincl,
+                        zpos, idx, gt, L.u);
+                }
             }
         }
         if (threadIdx.x == 0)

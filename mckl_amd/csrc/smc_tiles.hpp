@@ -68,6 +68,51 @@ __device__ __forceinline__ uint64_t tile_first(uint32_t tile, int gt)
     return (uint64_t)tile * kTile + (uint64_t)gt * kItems;
 }
 
+// The 8 values of a thread in a tile row: two 16-byte loads when all are
+// below N (rows are padded to whole tiles and 16-byte aligned), else
+// element loads with `fill` past N.
+__device__ __forceinline__ void load8(const float *p, uint64_t i0, uint64_t N,
+    float (&v)[kItems], float fill)
+{
+    if (i0 + kItems <= N) {
+        const float4 a = *reinterpret_cast<const float4 *>(p + i0);
+        const float4 b = *reinterpret_cast<const float4 *>(p + i0 + 4);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = a.w;
+        v[4] = b.x;
+        v[5] = b.y;
+        v[6] = b.z;
+        v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            v[k] = i0 + k < N ? p[i0 + k] : fill;
+    }
+}
+
+__device__ __forceinline__ void load8(const uint32_t *p, uint64_t i0,
+    uint64_t N, uint32_t (&v)[kItems], uint32_t fill)
+{
+    if (i0 + kItems <= N) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(p + i0);
+        const uint4 b = *reinterpret_cast<const uint4 *>(p + i0 + 4);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = a.w;
+        v[4] = b.x;
+        v[5] = b.y;
+        v[6] = b.z;
+        v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            v[k] = i0 + k < N ? p[i0 + k] : fill;
+    }
+}
+
 // t = 0: x_i = sd0 eps(i, 0), lw_i = 0
 __device__ __forceinline__ void tile_init(
     const SmcArgs &A, uint32_t tile, float *x, float *lw, int gt)
@@ -209,11 +254,13 @@ __device__ __forceinline__ void tile_sums(const SmcArgs &A, uint32_t tile,
     const uint64_t i0 = tile_first(tile, gt);
     double s = 0.0, s2 = 0.0;
     dd loc{0.0, 0.0};
+    float l[kItems];
+    load8(lw, i0, A.N, l, 0.0f);
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         if (i < A.N) {
-            const double e = (double)smc_exp_f(__fsub_rn(lw[i], m));
+            const double e = (double)smc_exp_f(__fsub_rn(l[k], m));
             s = __dadd_rn(s, e);
             s2 = __dadd_rn(s2, __dmul_rn(e, e));
             loc = dd_add(loc, dd{e, 0.0});
@@ -242,15 +289,31 @@ struct UniformSeq {
         return __dmul_rn(delta, __dadd_rn(smc_uniform(k0, k1, t, j), (double)j));
     }
 
-    // #{j < M : u_j < S}
+    // #{j < M : u_j < S}, from the guess floor(S M) by walking the
+    // non-decreasing sequence.  For stratified uniforms
+    // fl(delta j) <= u_j <= fl(delta (j+1)) (rounding is monotone and the
+    // stratum variate lies in [0, 1)), so a step whose answer those bounds
+    // already decide skips generating u_j: typically one evaluation per
+    // count instead of two or three.  The count is the same either way.
     __device__ __forceinline__ uint32_t count_less(double S) const
     {
         double g = floor(__dmul_rn(S, (double)M));
         uint32_t c = g <= 0.0 ? 0u : (g >= (double)M ? M : (uint32_t)g);
-        while (c > 0 && (*this)(c - 1) >= S)
+        const bool strat = scheme == 2;
+        while (c > 0) {
+            if (strat && __dmul_rn(delta, (double)c) < S)
+                break; // u_{c-1} <= fl(delta c) < S
+            if ((*this)(c - 1) < S)
+                break;
             --c;
-        while (c < M && (*this)(c) < S)
+        }
+        while (c < M) {
+            if (strat && __dmul_rn(delta, (double)c) >= S)
+                break; // u_c >= fl(delta c) >= S
+            if ((*this)(c) >= S)
+                break;
             ++c;
+        }
         return c;
     }
 };
@@ -274,10 +337,12 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
         : A.N;
     double W[kItems];
     dd loc{0.0, 0.0};
+    float l[kItems];
+    load8(lw, i0, A.N, l, 0.0f);
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        W[k] = i < A.N ? (double)smc_exp_f(__fsub_rn(lw[i], m)) : 0.0;
+        W[k] = i < A.N ? (double)smc_exp_f(__fsub_rn(l[k], m)) : 0.0;
         loc = dd_add(loc, dd{W[k], 0.0});
     }
     dd total;
@@ -340,13 +405,16 @@ __device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
     const uint64_t i0 = tile_first(tile, gt);
     uint32_t rep[kItems];
     uint32_t es = 0, zs = 0;
+    uint32_t c[kItems];
+    load8(cnt, i0, A.N, c, 0u);
+    uint32_t prev = (i0 == 0 || i0 > A.N) ? 0u : cnt[i0 - 1];
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         rep[k] = 0;
         if (i < A.N) {
-            const uint32_t p = i == 0 ? 0u : cnt[i - 1];
-            rep[k] = cnt[i] - p;
+            rep[k] = c[k] - prev;
+            prev = c[k];
             es += rep[k] > 1 ? rep[k] - 1 : 0;
             zs += rep[k] == 0 ? 1 : 0;
         }
@@ -371,25 +439,60 @@ __device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
 }
 
 // Source j with rep_j > 1 owns the extra copies of ranks
-// [E_j - (rep_j - 1), E_j) and writes itself into those zero slots.
+// [E_j - (rep_j - 1), E_j) and writes itself into those zero slots.  A source
+// with more than kHeavy extra copies is queued in LDS and its copies are
+// written by the whole group, so one heavily replicated particle does not
+// serialise on a single lane.  ldsu needs 4 + 3 * kMaxHeavy words.
+constexpr uint32_t kHeavy = 16;
+constexpr uint32_t kMaxHeavy = 64;
+
 __device__ __forceinline__ void tile_scatter(const SmcArgs &A, uint32_t tile,
-    const uint32_t *cnt, const uint32_t *Eincl, const uint32_t *zpos,
-    uint32_t *idx, int gt)
+    const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ Eincl,
+    const uint32_t *__restrict__ zpos, uint32_t *__restrict__ idx, int gt,
+    uint32_t *ldsu)
 {
     const uint64_t i0 = tile_first(tile, gt);
+    uint32_t c[kItems], E[kItems];
+    load8(cnt, i0, A.N, c, 0u);
+    load8(Eincl, i0, A.N, E, 0u);
+    uint32_t prev = (i0 == 0 || i0 > A.N) ? 0u : cnt[i0 - 1];
+    uint32_t *nheavy = ldsu;
+    uint32_t *list = ldsu + 4;
+    if (gt == 0)
+        *nheavy = 0;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t j = i0 + k;
         if (j < A.N) {
-            const uint32_t p = j == 0 ? 0u : cnt[j - 1];
-            const uint32_t rep = cnt[j] - p;
+            const uint32_t rep = c[k] - prev;
+            prev = c[k];
             if (rep > 1) {
-                const uint32_t e = Eincl[j];
-                for (uint32_t r = e - (rep - 1); r < e; ++r)
-                    idx[zpos[r]] = (uint32_t)j;
+                const uint32_t end = E[k], beg = end - (rep - 1);
+                uint32_t slot = kMaxHeavy;
+                if (rep - 1 > kHeavy) {
+                    slot = atomicAdd(nheavy, 1u);
+                    if (slot < kMaxHeavy) {
+                        list[3 * slot] = (uint32_t)j;
+                        list[3 * slot + 1] = beg;
+                        list[3 * slot + 2] = end;
+                    }
+                }
+                if (slot >= kMaxHeavy)
+                    for (uint32_t r = beg; r < end; ++r)
+                        idx[zpos[r]] = (uint32_t)j;
             }
         }
     }
+    __syncthreads();
+    const uint32_t nh = *nheavy < kMaxHeavy ? *nheavy : kMaxHeavy;
+    for (uint32_t h = 0; h < nh; ++h) {
+        const uint32_t j = list[3 * h];
+        const uint32_t end = list[3 * h + 2];
+        for (uint32_t r = list[3 * h + 1] + gt; r < end; r += kGroup)
+            idx[zpos[r]] = j;
+    }
+    __syncthreads();
 }
 
 // finalize one step from the reduced sums; returns the resampling flag
