@@ -96,6 +96,32 @@ int mckl_hip_engine_next(mckl_hip_engine *e, uint64_t *h_out);
 int mckl_hip_engine_block(const mckl_hip_engine *e, void *h_block);
 
 /*
+ * SeedGenerator<key_type>::get() called n times (random/seed.hpp:54-341) as
+ * RNGSetVector::reset() uses it (random/rng_set.hpp:81-120; the seed type of
+ * a counter engine is its key_type, random/counter.hpp:291-296): starting
+ * from internal seed `seed` (the reference starts at 1), with partition
+ * (np, rank) (seed.hpp:171-199) and Threefry randomization when `randomize`
+ * (MCKL_SEED_RANDOMIZE, the reference's default).  Writes n keys of 4 u64 in
+ * mckl_hip_engine.key layout to device memory.  Asynchronous.
+ */
+int mckl_hip_seed_keys(int kind, uint64_t seed, uint64_t np, uint64_t rank,
+    int randomize, size_t n, uint64_t *d_keys, void *stream);
+
+/*
+ * Fresh per-engine streams of an RNG set (RNGSetVector<RNG>(n) after reset,
+ * random/rng_set.hpp:81-120): engine e has key d_keys[4e..4e+3] and counter
+ * 0, and writes its first `per` outputs to d_r[e * per ..]: rand(rng[e], per,
+ * r) (counter.hpp:298-303) for 32-bit engines, or rand(rng[e],
+ * U01xxDistribution<T>, per, r) (u01_distribution.hpp:79-142).
+ */
+int mckl_hip_rngset_rand(int kind, size_t nengines, const uint64_t *d_keys,
+    size_t per, uint32_t *d_r, void *stream);
+int mckl_hip_rngset_u01_f32(int kind, int interval, size_t nengines,
+    const uint64_t *d_keys, size_t per, float *d_r, void *stream);
+int mckl_hip_rngset_u01_f64(int kind, int interval, size_t nengines,
+    const uint64_t *d_keys, size_t per, double *d_r, void *stream);
+
+/*
  * Raw generator kernels with the shape of the reference's native C ABI
  * (include/mckl/mckl.h:313-341, e.g. mckl_philox4x32_sse2_kernel): n blocks
  * for counters ctr+1 .. ctr+n written to d_r (n * 16 or n * 32 bytes), and

@@ -472,3 +472,68 @@ def smc_loglik_grad(model, theta, y, N, seeds, resample_threshold=0.5,
         float(resample_threshold), scheme, int(K), _dptr(ll), _dptr(g),
         _stream()))
     return ll, g
+
+
+class SeedGenerator:
+    """SeedGenerator<key_type of an engine> (random/seed.hpp:54-341): the
+    keys RNGSetVector::reset() hands to its engines (rng_set.hpp:81-120).
+    `keys(n)` returns the next n keys as a (n, 4) uint64 device tensor in
+    engine-key layout and advances the internal seed by n, as n get() calls
+    would."""
+
+    def __init__(self, engine_cls=None, randomize=True):
+        self.kind = (engine_cls or Philox4x32).KIND
+        self.randomize = bool(randomize)
+        self.seed = 1
+        self.np = 1
+        self.rank = 0
+
+    def set(self, s):
+        self.seed = int(s)
+
+    def partition(self, np_, rank):
+        if np_ < 1 or rank >= np_:
+            raise MCKLError("partition needs np > 0 and rank < np")
+        self.np, self.rank = int(np_), int(rank)
+
+    def keys(self, n):
+        import torch
+        out = torch.empty((n, 4), dtype=torch.int64, device="cuda")
+        check(lib.mckl_hip_seed_keys(self.kind, self.seed, self.np, self.rank,
+                                     int(self.randomize), n, _ptr(out),
+                                     _stream()))
+        self.seed += n
+        return out
+
+
+class RNGSetVector:
+    """RNGSetVector<RNG>(n) (random/rng_set.hpp:81-120) on the device: n
+    engines keyed by the SeedGenerator, counters 0.  rand/u01 give each
+    engine's first `per` outputs (engine e in rows e of the output)."""
+
+    def __init__(self, engine_cls, n, seed_generator=None):
+        if engine_cls.RBITS != 32:
+            raise MCKLError("RNG sets are provided for 32-bit engines")
+        self.engine_cls = engine_cls
+        self.n = int(n)
+        sg = seed_generator or SeedGenerator(engine_cls)
+        self.keys = sg.keys(self.n)
+
+    def engine(self, e):
+        """The host engine of member e (for further calls of its own)."""
+        k = [int(v) & 0xFFFFFFFFFFFFFFFF for v in self.keys[e].cpu().tolist()]
+        return self.engine_cls(key=k[:self.engine_cls.KEY_WORDS])
+
+    def rand(self, per, out):
+        check(lib.mckl_hip_rngset_rand(self.engine_cls.KIND, self.n,
+                                       _ptr(self.keys), per, _ptr(out),
+                                       _stream()))
+        return out
+
+    def u01(self, per, out, interval=_lib.U01_OO):
+        import torch
+        fn = (lib.mckl_hip_rngset_u01_f64 if out.dtype == torch.float64
+              else lib.mckl_hip_rngset_u01_f32)
+        check(fn(self.engine_cls.KIND, interval, self.n, _ptr(self.keys),
+                 per, _ptr(out), _stream()))
+        return out

@@ -102,6 +102,11 @@ SIGNATURES = {
     "mckl_hip_smc_batch_run": (_int, [_vp, _int, _vp]),
     "mckl_hip_smc_batch_result": (_int, [_vp, ctypes.POINTER(_dbl)]),
     "mckl_hip_smc_batch_destroy": (_int, [_vp]),
+    "mckl_hip_seed_keys": (
+        _int, [_int, _u64, _u64, _u64, _int, _sz, _vp, _vp]),
+    "mckl_hip_rngset_rand": (_int, [_int, _sz, _vp, _sz, _vp, _vp]),
+    "mckl_hip_rngset_u01_f32": (_int, [_int, _int, _sz, _vp, _sz, _vp, _vp]),
+    "mckl_hip_rngset_u01_f64": (_int, [_int, _int, _sz, _vp, _sz, _vp, _vp]),
     "mckl_hip_smc_grad_create": (
         _int, [_int, _sz, _sz, _sz, ctypes.POINTER(_dbl), _int, _U64P, _dbl,
                _int, _sz, _int, ctypes.POINTER(_vp)]),

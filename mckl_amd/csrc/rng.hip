@@ -1155,3 +1155,234 @@ int mckl_hip_rand_normal_f64_streams(mckl_hip_engine *e, size_t nstreams,
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// SeedGenerator keys and fresh per-engine streams (RNGSetVector)
+// random/seed.hpp:54-341, random/rng_set.hpp:81-120
+// ---------------------------------------------------------------------------
+namespace mckl_hip {
+
+// Threefry2x32-20 / Threefry2x64-20 of one block under key 0 with zero tweak
+// (random/internal/threefry_common.hpp:212-247: word 0 += par[S % 3],
+// word 1 += par[(S + 1) % 3] + S; round s0 += s1, s1 = rotl(s1, R) ^ s0;
+// rotations random/internal/threefry_constants.hpp:62-80).
+__device__ __forceinline__ void threefry2x32_key0(uint32_t &s0, uint32_t &s1)
+{
+    constexpr int R[8] = {13, 15, 26, 6, 17, 29, 16, 24};
+    const uint32_t par[3] = {0u, 0u, 0x1BD11BDAu};
+#pragma unroll
+    for (int S = 0; S <= 5; ++S) {
+        s0 += par[S % 3];
+        s1 += par[(S + 1) % 3] + (uint32_t)S;
+        if (S == 5)
+            break;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s0 += s1;
+            const int l = R[(S * 4 + q) % 8];
+            s1 = __builtin_amdgcn_alignbit(s1, s1, 32 - l) ^ s0;
+        }
+    }
+}
+
+__device__ __forceinline__ void threefry2x64_key0(uint64_t &s0, uint64_t &s1)
+{
+    const uint64_t par[3] = {0ull, 0ull, 0x1BD11BDAA9FC1A22ull};
+#define MCKL_TF2_ROUND(L)                                                     \
+    s0 = add64(s0, s1);                                                       \
+    s1 = rotl64<L>(s1) ^ s0;
+#pragma unroll
+    for (int S = 0; S <= 5; ++S) {
+        s0 = add64(s0, par[S % 3]);
+        s1 = add64(s1, par[(S + 1) % 3] + (uint64_t)S);
+        if (S == 5)
+            break;
+        if (S % 2 == 0) {
+            MCKL_TF2_ROUND(16)
+            MCKL_TF2_ROUND(42)
+            MCKL_TF2_ROUND(12)
+            MCKL_TF2_ROUND(31)
+        } else {
+            MCKL_TF2_ROUND(16)
+            MCKL_TF2_ROUND(32)
+            MCKL_TF2_ROUND(24)
+            MCKL_TF2_ROUND(21)
+        }
+    }
+#undef MCKL_TF2_ROUND
+}
+
+// n consecutive get() from internal seed `seed`; keys in engine layout
+__global__ __launch_bounds__(kBlock) void k_seed_keys(int kind, uint64_t seed,
+    uint64_t np, uint64_t rank, uint64_t maxs, int randomize, uint64_t n,
+    uint64_t *keys)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    uint64_t s = seed + i;
+    uint64_t k[4] = {0, 0, 0, 0};
+    if (kind == PHILOX) {
+        s = (s % maxs) * np + rank;
+        uint32_t w0 = (uint32_t)s, w1 = (uint32_t)(s >> 32);
+        if (randomize)
+            threefry2x32_key0(w0, w1);
+        k[0] = w0;
+        k[1] = w1;
+    } else if (kind == ARS) {
+        uint64_t w0 = s, w1 = rank;
+        if (randomize)
+            threefry2x64_key0(w0, w1);
+        k[0] = (uint32_t)w0;
+        k[1] = w0 >> 32;
+        k[2] = (uint32_t)w1;
+        k[3] = w1 >> 32;
+    } else {
+        k[0] = s;
+        k[3] = rank;
+        if (randomize) {
+            StreamArg a{};
+            a.ctr[0] = s;
+            a.ctr[3] = rank;
+            ThreefryGen g;
+            g.init(a); // key 0
+            uint32_t w[8];
+            g.block(0, w);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                k[j] = (uint64_t)w[2 * j] | ((uint64_t)w[2 * j + 1] << 32);
+        }
+    }
+    uint64_t *o = keys + 4 * i;
+    o[0] = k[0];
+    o[1] = k[1];
+    o[2] = k[2];
+    o[3] = k[3];
+}
+
+// Engine e (key keys[4e..4e+3], counter 0, nothing buffered) produces `per`
+// elements: stream words from block ctr + 1 on, one lane per (engine,
+// block).  out[e * per + j] is element j of engine e.
+template <class Gen, class Conv>
+__global__ __launch_bounds__(kBlock) void k_rngset_fill(const uint64_t *keys,
+    uint64_t nengines, uint64_t per, uint64_t bpe, Conv conv,
+    typename Conv::T *out)
+{
+    using T = typename Conv::T;
+    constexpr int EPB = Gen::W32 / Conv::Q;
+    __shared__ uint32_t lds[LdsWords<Gen>::value];
+    const uint64_t total = nengines * bpe;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    Gen g;
+    StreamArg a{};
+    g.init(a);
+    g.prepare(lds);
+    for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < total;
+         t += stride) {
+        const uint64_t e = t / bpe, b = t % bpe;
+        a.key[0] = keys[4 * e];
+        a.key[1] = keys[4 * e + 1];
+        a.key[2] = keys[4 * e + 2];
+        a.key[3] = keys[4 * e + 3];
+        Gen ge = g;
+        ge.init(a);
+        uint32_t w[Gen::W32];
+        ge.block(b + 1, w);
+        T *o = out + e * per;
+#pragma unroll
+        for (int j = 0; j < EPB; ++j) {
+            const uint64_t k = b * EPB + j;
+            if (k < per)
+                o[k] = conv(w + j * Conv::Q);
+        }
+    }
+}
+
+template <class Gen>
+struct RngSetOp {
+    template <class Conv>
+    static int run(const uint64_t *keys, uint64_t nengines, uint64_t per,
+        Conv conv, typename Conv::T *out, hipStream_t st)
+    {
+        constexpr int EPB = Gen::W32 / Conv::Q;
+        const uint64_t bpe = ceil_div(per, (uint64_t)EPB);
+        const uint64_t total = nengines * bpe;
+        const uint64_t cap = (uint64_t)grid_cap(Gen::KIND);
+        const uint32_t gx = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(ceil_div(total, kBlock), cap));
+        k_rngset_fill<Gen, Conv><<<gx, kBlock, 0, st>>>(
+            keys, nengines, per, bpe, conv, out);
+        MCKL_HIP_CHECK_LAUNCH("k_rngset_fill");
+        return MCKL_HIP_OK;
+    }
+};
+
+template <class Conv>
+static int rngset_fill(int kind, size_t nengines, const uint64_t *d_keys,
+    size_t per, typename Conv::T *out, Conv conv, hipStream_t st)
+{
+    if (kind != PHILOX && kind != THREEFRY && kind != ARS)
+        return set_error(MCKL_HIP_EINVAL, "unknown engine kind %d", kind);
+    if (nengines == 0 || per == 0)
+        return MCKL_HIP_OK;
+    if (!d_keys || !out)
+        return set_error(MCKL_HIP_EINVAL, "NULL device pointer");
+    int rc = ensure_device_tables();
+    if (rc)
+        return rc;
+    return by_kind<RngSetOp>(kind, d_keys, (uint64_t)nengines, (uint64_t)per,
+        conv, out, st);
+}
+
+} // namespace mckl_hip
+
+extern "C" {
+
+int mckl_hip_seed_keys(int kind, uint64_t seed, uint64_t np, uint64_t rank,
+    int randomize, size_t n, uint64_t *d_keys, void *stream)
+{
+    if (kind != PHILOX && kind != THREEFRY && kind != ARS)
+        return set_error(MCKL_HIP_EINVAL, "unknown engine kind %d", kind);
+    if (np == 0 || rank >= np)
+        return set_error(MCKL_HIP_EINVAL,
+            "partition needs np > 0 and rank < np"); // seed.hpp:173-184
+    if (n == 0)
+        return MCKL_HIP_OK;
+    if (!d_keys)
+        return set_error(MCKL_HIP_EINVAL, "NULL device pointer");
+    const uint64_t maxs = np > 1 ? (UINT64_MAX - rank) / np + 1 : UINT64_MAX;
+    const uint64_t grid = ceil_div((uint64_t)n, (uint64_t)kBlock);
+    k_seed_keys<<<(uint32_t)grid, kBlock, 0, as_stream(stream)>>>(kind, seed,
+        np, rank, maxs, randomize, (uint64_t)n, d_keys);
+    MCKL_HIP_CHECK_LAUNCH("k_seed_keys");
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_rngset_rand(int kind, size_t nengines, const uint64_t *d_keys,
+    size_t per, uint32_t *d_r, void *stream)
+{
+    return rngset_fill(kind, nengines, d_keys, per, d_r, ConvU32(),
+        as_stream(stream));
+}
+
+int mckl_hip_rngset_u01_f64(int kind, int interval, size_t nengines,
+    const uint64_t *d_keys, size_t per, double *d_r, void *stream)
+{
+    if (interval < MCKL_HIP_U01_CC || interval > MCKL_HIP_U01_OO)
+        return set_error(MCKL_HIP_EINVAL, "interval %d", interval);
+    ConvU01F64 c;
+    c.iv = interval;
+    return rngset_fill(kind, nengines, d_keys, per, d_r, c, as_stream(stream));
+}
+
+int mckl_hip_rngset_u01_f32(int kind, int interval, size_t nengines,
+    const uint64_t *d_keys, size_t per, float *d_r, void *stream)
+{
+    if (interval < MCKL_HIP_U01_CC || interval > MCKL_HIP_U01_OO)
+        return set_error(MCKL_HIP_EINVAL, "interval %d", interval);
+    ConvU01F32 c;
+    c.iv = interval;
+    return rngset_fill(kind, nengines, d_keys, per, d_r, c, as_stream(stream));
+}
+
+} // extern "C"

@@ -111,6 +111,119 @@ void mo_threefry4x64_block(
 }
 
 /* ===================================================================== */
+/* Threefry2x32-20 and Threefry2x64-20 (the SeedGenerator's randomizers)  */
+/* random/threefry.hpp:109-121 (parity word, zero tweak),                 */
+/* random/internal/threefry_common.hpp:212-247 (key box: word 0 gets      */
+/* par[S % 3], word 1 par[(S + 1) % 3] + S; the tweak is zero),           */
+/* random/internal/threefry_generic_2x64.hpp:40-80 (round: s0 += s1;      */
+/* s1 = rotl(s1, R[(N-1) % 8]) ^ s0),                                     */
+/* random/internal/threefry_constants.hpp:46-80 (parity, rotations).      */
+/* ===================================================================== */
+static const int mo_tf2x32_r[8] = {13, 15, 26, 6, 17, 29, 16, 24};
+static const int mo_tf2x64_r[8] = {16, 42, 12, 31, 16, 32, 24, 21};
+
+static uint32_t mo_rotl32_(uint32_t x, int l)
+{
+    return (x << l) | (x >> (32 - l));
+}
+
+void mo_threefry2x32_block(
+    const uint32_t key[2], const uint32_t in[2], uint32_t out[2])
+{
+    const uint32_t par[3] = {key[0], key[1], 0x1BD11BDAu ^ key[0] ^ key[1]};
+    uint32_t s0 = in[0], s1 = in[1];
+    for (unsigned S = 0; S <= 5; ++S) {
+        s0 += par[S % 3];
+        s1 += par[(S + 1) % 3] + S;
+        if (S == 5)
+            break;
+        for (int q = 0; q < 4; ++q) {
+            s0 += s1;
+            s1 = mo_rotl32_(s1, mo_tf2x32_r[(S * 4 + q) % 8]);
+            s1 ^= s0;
+        }
+    }
+    out[0] = s0;
+    out[1] = s1;
+}
+
+void mo_threefry2x64_block(
+    const uint64_t key[2], const uint64_t in[2], uint64_t out[2])
+{
+    const uint64_t par[3] = {
+        key[0], key[1], 0x1BD11BDAA9FC1A22ull ^ key[0] ^ key[1]};
+    uint64_t s0 = in[0], s1 = in[1];
+    for (unsigned S = 0; S <= 5; ++S) {
+        s0 += par[S % 3];
+        s1 += par[(S + 1) % 3] + S;
+        if (S == 5)
+            break;
+        for (int q = 0; q < 4; ++q) {
+            s0 += s1;
+            s1 = mo_rotl64(s1, mo_tf2x64_r[(S * 4 + q) % 8]);
+            s1 ^= s0;
+        }
+    }
+    out[0] = s0;
+    out[1] = s1;
+}
+
+/* ===================================================================== */
+/* SeedGenerator<key_type>::get(), as used by RNGSetVector::reset()       */
+/* (random/seed.hpp:54-341, random/rng_set.hpp:81-120; SeedType of a      */
+/* counter engine is its key_type, random/counter.hpp:291-296).           */
+/*   Philox4x32  key u32[2]: seed_type u64, one word: s %= maxs; s *= np; */
+/*               s += rank; randomized by Threefry2x32 (key 0) of the     */
+/*               little-endian words (s_lo, s_hi)           (:190-199)    */
+/*   ARS         key u32[4]: two u64 words {s, rank}, Threefry2x64        */
+/*   Threefry4x64 key u64[4]: {s, 0, 0, rank}, Threefry4x64   (:180-188)  */
+/* The internal seed starts at 1 (:165) and is post-incremented per get.  */
+/* Output: n keys of 4 u64 in mo_engine.key layout (u32 words zero-       */
+/* extended for the 32-bit key types).                                    */
+/* ===================================================================== */
+void mo_seed_keys(int kind, uint64_t seed, uint64_t np, uint64_t rank,
+    int randomize, size_t n, uint64_t *keys)
+{
+    const uint64_t zero[4] = {0, 0, 0, 0};
+    uint64_t maxs = UINT64_MAX;
+    if (kind == MO_PHILOX4X32 && np > 1)
+        maxs = (UINT64_MAX - rank) / np + 1;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t s = seed + (uint64_t)i;
+        uint64_t *k = keys + 4 * i;
+        k[0] = k[1] = k[2] = k[3] = 0;
+        if (kind == MO_PHILOX4X32) {
+            s %= maxs;
+            s *= np;
+            s += rank;
+            uint32_t w[2] = {(uint32_t)s, (uint32_t)(s >> 32)};
+            if (randomize) {
+                const uint32_t z[2] = {0, 0};
+                mo_threefry2x32_block(z, w, w);
+            }
+            k[0] = w[0];
+            k[1] = w[1];
+        } else if (kind == MO_ARS) {
+            uint64_t w[2] = {s, rank};
+            if (randomize)
+                mo_threefry2x64_block(zero, w, w);
+            k[0] = (uint32_t)w[0];
+            k[1] = (uint32_t)(w[0] >> 32);
+            k[2] = (uint32_t)w[1];
+            k[3] = (uint32_t)(w[1] >> 32);
+        } else {
+            uint64_t w[4] = {s, 0, 0, rank};
+            if (randomize)
+                mo_threefry4x64_block(zero, w, w);
+            k[0] = w[0];
+            k[1] = w[1];
+            k[2] = w[2];
+            k[3] = w[3];
+        }
+    }
+}
+
+/* ===================================================================== */
 /* AES round function and ARS-5                                           */
 /* random/internal/aes_generic.hpp:254-306 (ARS round keys, Weyl),        */
 /* random/internal/aes_generic.hpp:413-491 (T-table round, last round),   */

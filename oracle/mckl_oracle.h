@@ -9,6 +9,8 @@
  * Paths are relative to the zhouyan/MCKL source tree.
  *
  * Parity pinning (what checks each part, see tests/test_oracle_*.py):
+ *   seed keys      reference KAT files random_seed_{2x32,1x64,4x32,2x64,
+ *                  4x64,8x32}.data (plain and randomized columns).
  *   engines        reference KAT files random_{philox4x32,threefry4x64,ars}
  *                  [_64].data; FIPS-197 Appendix C for the AES round; the
  *                  compiled reference (oracle/_ref) for bulk/partial/discard.
@@ -61,6 +63,14 @@ void mo_philox4x32_block(
 void mo_threefry4x64_block(
     const uint64_t key[4], const uint64_t ctr[4], uint64_t out[4]);
 void mo_ars_block(const uint32_t key[4], const uint64_t ctr[2], uint32_t out[4]);
+void mo_threefry2x32_block(
+    const uint32_t key[2], const uint32_t in[2], uint32_t out[2]);
+void mo_threefry2x64_block(
+    const uint64_t key[2], const uint64_t in[2], uint64_t out[2]);
+
+/* ---- SeedGenerator keys for RNGSetVector (n keys, 4 u64 each) ---- */
+void mo_seed_keys(int kind, uint64_t seed, uint64_t np, uint64_t rank,
+    int randomize, size_t n, uint64_t *keys);
 void mo_aes_tables(uint32_t t0[256], uint8_t sbox[256]);
 void mo_aes128_encrypt(const uint8_t key[16], const uint8_t in[16], uint8_t out[16]);
 

@@ -178,6 +178,15 @@ def resample(scheme, e, w, M=None):
     return rep[:N], idx[:M]
 
 
+def seed_keys(kind, seed, n, np_=1, rank=0, randomize=True):
+    """SeedGenerator<key_type>::get() n times from internal seed `seed`:
+    (n, 4) uint64 keys in engine-key layout."""
+    out = np.zeros((n, 4), np.uint64)
+    O.mo_seed_keys(kind, ctypes.c_uint64(seed), ctypes.c_uint64(np_),
+                   ctypes.c_uint64(rank), int(bool(randomize)), _sz(n), P(out))
+    return out
+
+
 def smc_normal(seed, i, t):
     return O.mo_smc_normal(seed, i, t)
 
