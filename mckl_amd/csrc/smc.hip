@@ -54,6 +54,8 @@ struct RsBuf {
     double *tdh, *tdl; // tile dd totals of e
     uint32_t *te, *tz; // tile extra-copy / zero-slot counts
     uint32_t *cnt, *Eincl, *zpos;
+    double *tmh, *tml; // multinomial: tile dd totals of the log spacings
+    double *u;         // multinomial: the step's sorted uniforms
 };
 
 __global__ __launch_bounds__(kGroup) void k_move(SmcArgs A, uint32_t t,
@@ -127,10 +129,65 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     if (!o.resample)
         return;
     const uint32_t k = blockIdx.x;
+    if (A.scheme == 0) {
+        // multinomial: tile totals of the log spacings; the uniforms and
+        // the counts follow in k_rs_uniforms and k_rs_mcounts
+        dd mt;
+        tile_mterms(A, k, t, gt, L.d, mt);
+        if (gt == 0) {
+            B.tmh[k] = mt.hi;
+            B.tml[k] = mt.lo;
+        }
+        return;
+    }
     dd off, next;
     tiles_offset_dd(B.tdh, B.tdl, nt, (int)k, (int)k + 1, gt, L.d, off, next);
     const double invS = __ddiv_rn(1.0, rs);
     const UniformSeq U = make_uniform_seq(A, t);
+    uint32_t es, zs;
+    tile_counts(A, k, lw, m, invS, off, next.hi, U, B.cnt, gt, L.d, L.u, es,
+        zs);
+    if (gt == 0) {
+        B.te[k] = es;
+        B.tz[k] = zs;
+    }
+}
+
+// multinomial: each tile's sorted uniforms from the prefix of log spacings
+__global__ __launch_bounds__(kGroup) void k_rs_uniforms(SmcArgs A, uint32_t t,
+    const SmcState *st, RsBuf B)
+{
+    if (!st->resampled[t & 1])
+        return;
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    const GroupLds L = group_lds(lds);
+    const uint32_t k = blockIdx.x;
+    dd off, next;
+    tiles_offset_dd(B.tmh, B.tml, (int)A.ntiles, (int)k, (int)k + 1,
+        threadIdx.x, L.d, off, next);
+    tile_muniforms(A, k, t, off, B.u, threadIdx.x, L.d);
+}
+
+// multinomial: counts of the sorted uniforms below each cumulative weight
+__global__ __launch_bounds__(kGroup) void k_rs_mcounts(SmcArgs A, uint32_t t,
+    const float *lw, const double *ts, const double *ts2, const SmcState *st,
+    RsBuf B)
+{
+    const int c = t & 1;
+    if (!st->resampled[c])
+        return;
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    const GroupLds L = group_lds(lds);
+    const int gt = threadIdx.x;
+    const int nt = (int)A.ntiles;
+    const float m = st->m[c];
+    double rs, rs2;
+    tiles_sum2(ts, ts2, nt, gt, L.d, rs, rs2);
+    const uint32_t k = blockIdx.x;
+    dd off, next;
+    tiles_offset_dd(B.tdh, B.tdl, nt, (int)k, (int)k + 1, gt, L.d, off, next);
+    const double invS = __ddiv_rn(1.0, rs);
+    const UniformSeq U = make_uniform_seq(A, t, B.u);
     uint32_t es, zs;
     tile_counts(A, k, lw, m, invS, off, next.hi, U, B.cnt, gt, L.d, L.u, es,
         zs);
@@ -197,6 +254,7 @@ struct BatchLds {
     double toh[kMaxTiles + 1], tol[kMaxTiles + 1];
     uint32_t te[kMaxTiles], tz[kMaxTiles];
     uint32_t teo[kMaxTiles + 1], tzo[kMaxTiles + 1];
+    double tmh[kMaxTiles], tml[kMaxTiles];
     int flag;
     int filter;
 };
@@ -205,7 +263,7 @@ template <int G>
 __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
     const uint32_t *order, uint32_t nfilters, uint32_t *queue, float *xa,
     float *xb, float *lwb, uint32_t *idxb, uint32_t *cntb, uint32_t *Eb,
-    uint32_t *zb, double *logz_out)
+    uint32_t *zb, double *ub, double *logz_out)
 {
     __shared__ __align__(16) unsigned char scratch[G * kScratch];
     __shared__ BatchLds S;
@@ -278,8 +336,32 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
             resampled = o.resample;
             if (resampled) {
                 const double invS = __ddiv_rn(1.0, rs);
+                double *u = ub ? ub + off : nullptr;
+                if (A.scheme == 0) {
+                    // multinomial: sorted uniforms of the step first
+                    for (int r = 0; r < rounds; ++r) {
+                        const int k = r * G + grp;
+                        dd mt;
+                        tile_mterms(A, (uint32_t)(k < nt ? k : nt), t, gt, L.d,
+                            mt);
+                        if (gt == 0 && k < nt) {
+                            S.tmh[k] = mt.hi;
+                            S.tml[k] = mt.lo;
+                        }
+                        __syncthreads();
+                    }
+                    tiles_scan_dd(
+                        grp == 0, S.tmh, S.tml, nt, S.toh, S.tol, gt, L.d);
+                    for (int r = 0; r < rounds; ++r) {
+                        const int k = r * G + grp;
+                        const int kk = k < nt ? k : nt;
+                        tile_muniforms(A, (uint32_t)kk, t,
+                            dd{S.toh[kk], S.tol[kk]}, u, gt, L.d);
+                    }
+                    __syncthreads();
+                }
                 tiles_scan_dd(grp == 0, S.tdh, S.tdl, nt, S.toh, S.tol, gt, L.d);
-                const UniformSeq U = make_uniform_seq(A, t);
+                const UniformSeq U = make_uniform_seq(A, t, u);
                 for (int r = 0; r < rounds; ++r) {
                     const int k = r * G + grp;
                     uint32_t es, zs;
@@ -361,9 +443,10 @@ static int check_common(size_t N, size_t T, double threshold, int scheme)
         return set_error(MCKL_HIP_EINVAL, "particle count %zu out of range", N);
     if (T == 0)
         return set_error(MCKL_HIP_EINVAL, "T must be > 0");
-    if (scheme != MCKL_HIP_STRATIFIED && scheme != MCKL_HIP_SYSTEMATIC)
-        return set_error(MCKL_HIP_EINVAL,
-            "the samplers resample with the stratified or systematic scheme");
+    if (scheme != MCKL_HIP_MULTINOMIAL && scheme != MCKL_HIP_STRATIFIED &&
+        scheme != MCKL_HIP_SYSTEMATIC)
+        return set_error(MCKL_HIP_EINVAL, "unknown resampling scheme %d",
+            scheme);
     if (std::isnan(threshold))
         return set_error(MCKL_HIP_EINVAL, "threshold is NaN");
     return MCKL_HIP_OK;
@@ -396,7 +479,7 @@ static void smc_free(mckl_hip_smc *s)
         (void)hipGraphDestroy(s->graph);
     void *ptrs[] = {s->x[0], s->x[1], s->lw, s->obs, s->tmax, s->idx, s->ts,
         s->ts2, s->B.tdh, s->B.tdl, s->B.te, s->B.tz, s->B.cnt, s->B.Eincl,
-        s->B.zpos, s->st, s->ess, s->res};
+        s->B.zpos, s->B.tmh, s->B.tml, s->B.u, s->st, s->ess, s->res};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -455,6 +538,11 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
     MCKL_ALLOC(s->B.cnt, Np);
     MCKL_ALLOC(s->B.Eincl, Np);
     MCKL_ALLOC(s->B.zpos, Np);
+    if (scheme == MCKL_HIP_MULTINOMIAL) {
+        MCKL_ALLOC(s->B.tmh, nt);
+        MCKL_ALLOC(s->B.tml, nt);
+        MCKL_ALLOC(s->B.u, Np);
+    }
     MCKL_ALLOC(s->st, 1);
     MCKL_ALLOC(s->ess, T);
     MCKL_ALLOC(s->res, T);
@@ -492,6 +580,11 @@ static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
             s->A, s->lw, s->tmax, s->ts, s->ts2, s->B);
         k_rs_counts<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->tmax, s->ts,
             s->ts2, s->B, s->st, s->ess, s->res);
+        if (s->A.scheme == MCKL_HIP_MULTINOMIAL) {
+            k_rs_uniforms<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B);
+            k_rs_mcounts<<<grid, kGroup, 0, st>>>(
+                s->A, t, s->lw, s->ts, s->ts2, s->st, s->B);
+        }
         k_rs_index<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);
         k_rs_scatter<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);
     }
@@ -569,6 +662,7 @@ struct mckl_hip_smc_batch {
     uint32_t *queue = nullptr;
     float *xa = nullptr, *xb = nullptr, *lw = nullptr, *obs = nullptr;
     uint32_t *idx = nullptr, *cnt = nullptr, *E = nullptr, *zpos = nullptr;
+    double *u = nullptr; // multinomial uniforms
     double *logz = nullptr;
     mckl_hip_tune tune{256, 0};
     int persistent = 0; // workgroups in the persistent grid
@@ -585,7 +679,7 @@ static void batch_free(mckl_hip_smc_batch *b)
     if (b->graph)
         (void)hipGraphDestroy(b->graph);
     void *ptrs[] = {b->fd, b->order, b->queue, b->xa, b->xb, b->lw, b->obs,
-        b->idx, b->cnt, b->E, b->zpos, b->logz};
+        b->idx, b->cnt, b->E, b->zpos, b->u, b->logz};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -601,17 +695,18 @@ static int batch_launch(mckl_hip_smc_batch *b, hipStream_t st)
     switch (G) {
         case 1:
             k_batch<1><<<grid, kGroup, 0, st>>>(b->fd, b->order, nf, b->queue,
-                b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->zpos, b->logz);
+                b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->zpos, b->u,
+                b->logz);
             break;
         case 2:
             k_batch<2><<<grid, 2 * kGroup, 0, st>>>(b->fd, b->order, nf,
                 b->queue, b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->zpos,
-                b->logz);
+                b->u, b->logz);
             break;
         default:
             k_batch<4><<<grid, 4 * kGroup, 0, st>>>(b->fd, b->order, nf,
                 b->queue, b->xa, b->xb, b->lw, b->idx, b->cnt, b->E, b->zpos,
-                b->logz);
+                b->u, b->logz);
             break;
     }
     MCKL_HIP_CHECK_LAUNCH("k_batch");
@@ -718,6 +813,8 @@ int mckl_hip_smc_batch_create(int model, size_t nfilters, const uint32_t *h_N,
     MCKL_ALLOC(b->cnt, total);
     MCKL_ALLOC(b->E, total);
     MCKL_ALLOC(b->zpos, total);
+    if (scheme == MCKL_HIP_MULTINOMIAL)
+        MCKL_ALLOC(b->u, total);
     MCKL_ALLOC(b->obs, nfilters * T);
     MCKL_ALLOC(b->logz, nfilters);
 #undef MCKL_ALLOC

@@ -56,7 +56,7 @@ struct SmcArgs {
     uint32_t N;
     uint32_t ntiles;
     double tauN; // resampling threshold * N
-    int scheme;  // 2 stratified, 3 systematic
+    int scheme;  // 0 multinomial, 2 stratified, 3 systematic
     uint32_t T;
     const float *obs; // T observation constants
 };
@@ -281,6 +281,7 @@ struct UniformSeq {
     uint64_t t;
     uint32_t M;
     double delta, u0d;
+    const double *u; // multinomial: the sorted uniforms of the step
 
     __device__ __forceinline__ double operator()(uint64_t j) const
     {
@@ -297,6 +298,18 @@ struct UniformSeq {
     // count instead of two or three.  The count is the same either way.
     __device__ __forceinline__ uint32_t count_less(double S) const
     {
+        if (scheme == 0) {
+            // first j with u_j >= S (binary search: monotone in S)
+            uint32_t lo = 0, hi = M;
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                if (u[mid] < S)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            return lo;
+        }
         double g = floor(__dmul_rn(S, (double)M));
         uint32_t c = g <= 0.0 ? 0u : (g >= (double)M ? M : (uint32_t)g);
         const bool strat = scheme == 2;
@@ -495,6 +508,54 @@ __device__ __forceinline__ void tile_scatter(const SmcArgs &A, uint32_t tile,
     __syncthreads();
 }
 
+// Multinomial uniforms of step t (u01_rand_sorted, resample.hpp:63-81,
+// 216-235, on the step's uniform stream U_j = smc_uniform(t, j)):
+// lm_j = sum_{i <= j} log(U_i) / (N - i), u_j = 1 - exp(lm_j), ascending.
+// The sum is a double-double prefix over tiles (tile totals here, offsets by
+// tiles_offset_dd/tiles_scan_dd), rounded once per element.
+__device__ __forceinline__ void tile_mterms(const SmcArgs &A, uint32_t tile,
+    uint64_t t, int gt, double *ldsd, dd &ttot)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    dd loc{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t j = i0 + k;
+        if (j < A.N) {
+            const double a = __ddiv_rn(
+                log(smc_uniform(A.k0, A.k1, t, j)), (double)(A.N - j));
+            loc = dd_add(loc, dd{a, 0.0});
+        }
+    }
+    (void)group_excl_scan_dd(loc, gt, ldsd, ttot);
+}
+
+__device__ __forceinline__ void tile_muniforms(const SmcArgs &A,
+    uint32_t tile, uint64_t t, dd tile_off, double *u, int gt, double *ldsd)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    double a[kItems];
+    dd loc{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t j = i0 + k;
+        a[k] = j < A.N ? __ddiv_rn(log(smc_uniform(A.k0, A.k1, t, j)),
+                             (double)(A.N - j))
+                       : 0.0;
+        loc = dd_add(loc, dd{a[k], 0.0});
+    }
+    dd tot;
+    const dd base = dd_add(tile_off, group_excl_scan_dd(loc, gt, ldsd, tot));
+    dd run{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t j = i0 + k;
+        run = dd_add(run, dd{a[k], 0.0});
+        if (j < A.N)
+            u[j] = __dsub_rn(1.0, exp(dd_add(base, run).hi));
+    }
+}
+
 // finalize one step from the reduced sums; returns the resampling flag
 struct StepOut {
     double logz_inc, ess;
@@ -512,9 +573,10 @@ __device__ __forceinline__ StepOut step_finalize(const SmcArgs &A, float m,
 }
 
 __device__ __forceinline__ UniformSeq make_uniform_seq(
-    const SmcArgs &A, uint64_t t)
+    const SmcArgs &A, uint64_t t, const double *u = nullptr)
 {
     UniformSeq U;
+    U.u = u;
     U.scheme = A.scheme;
     U.k0 = A.k0;
     U.k1 = A.k1;

@@ -40,6 +40,10 @@ CASES = [
     (1, (0.9, 1.0, 0.5), 2049, 8, 3),
     (2, (0.95, 0.2, 0.7), 4096, 9, 2),
     (2, (0.95, 0.2, 0.7), 70001, 10, 3),
+    # multinomial: sorted uniforms from the log-spacing prefix
+    (1, (0.9, 1.0, 0.5), 4096, 11, 0),
+    (2, (0.95, 0.2, 0.7), 5000, 12, 0),
+    (1, (0.9, 1.0, 0.5), 2049, 13, 0),
 ]
 
 
@@ -126,6 +130,28 @@ def test_batched_equals_plain(dev):
     b.run(use_graph=True)
     b.run(use_graph=True)
     assert np.array_equal(b.log_likelihood(), lb)
+
+
+def test_multinomial_batched_and_kalman(dev):
+    """Multinomial resampling: the batched sampler equals the plain one bit
+    for bit, and the LG estimate is within Monte Carlo error of Kalman."""
+    m = _m()
+    ob = _obs()
+    theta = ob["batch64_theta"][:8]
+    N = ob["batch64_N"][:8]
+    y = ob["batch64_y"][:8]
+    seeds = np.arange(40, 48, dtype=np.uint64)
+    b = m.SMCBatch(2, N, theta, y, seeds, 0.5, 0)
+    b.run()
+    lb = b.log_likelihood()
+    for f in range(8):
+        lp = m.smc_loglik(2, theta[f], y[f], int(N[f]), int(seeds[f]), 0.5, 0)
+        assert lb[f] == lp, (f, lb[f], lp)
+    yl = ob["lg_y"]
+    exact = kalman.loglik(yl, 0.9, 1.0, 0.5)
+    ll = m.smc_loglik(1, (0.9, 1.0, 0.5), yl, 1 << 18, 3, 0.5, 0)
+    print("kalman %.6f multinomial %.6f" % (exact, ll))
+    assert abs(ll - exact) < 0.25
 
 
 def test_batched_vs_oracle(dev):
