@@ -87,10 +87,12 @@ __global__ __launch_bounds__(kGroup) void k_sums(SmcArgs A, const float *lw,
     __shared__ __align__(16) unsigned char lds[kScratch];
     const GroupLds L = group_lds(lds);
     const int gt = threadIdx.x;
+    float l[kItems];
+    load8(lw, tile_first(blockIdx.x, gt), A.N, l, 0.0f);
     const float m = tiles_max(tmax, A.ntiles, gt, L.f);
     double S, S2;
     dd et;
-    tile_sums(A, blockIdx.x, lw, m, gt, L.d, S, S2, et);
+    tile_sums_l(A, blockIdx.x, l, m, gt, L.d, S, S2, et);
     if (gt == 0) {
         ts[blockIdx.x] = S;
         ts2[blockIdx.x] = S2;
@@ -102,7 +104,10 @@ __global__ __launch_bounds__(kGroup) void k_sums(SmcArgs A, const float *lw,
 // Every workgroup reduces the tile sums and takes the step's decision;
 // workgroup 0 records it.  When the step resamples, each workgroup derives
 // its tile's prefix offsets and counts the uniforms below its cumulative
-// weights.
+// weights.  All global loads the workgroup needs are issued up front (its
+// tile's log weights, its chunk of every per-tile array), so the kernel
+// waits on memory once; the reductions are those of tiles_max, tiles_sum2
+// and tiles_offset_dd, operation for operation.
 __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     const float *lw, const float *tmax, const double *ts, const double *ts2,
     RsBuf B, SmcState *st, double *ess_hist, int32_t *res_hist)
@@ -111,12 +116,34 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     const GroupLds L = group_lds(lds);
     const int gt = threadIdx.x;
     const int nt = (int)A.ntiles;
-    const float m = tiles_max(tmax, A.ntiles, gt, L.f);
-    double rs, rs2;
-    tiles_sum2(ts, ts2, nt, gt, L.d, rs, rs2);
+    const uint32_t k = blockIdx.x;
     const int p = (t - 1) & 1, c = t & 1;
-    const StepOut o = step_finalize(A, m, rs, rs2, st->C[p]);
+    float l[kItems];
+    load8(lw, tile_first(k, gt), A.N, l, 0.0f);
+    const double Cprev = st->C[p];
+    const int chunk = (nt + kGroup - 1) / kGroup;
+    const int b = gt * chunk;
+    const int e = b + chunk < nt ? b + chunk : nt;
+    float mx = -__builtin_huge_valf();
+    for (int i = gt; i < nt; i += kGroup)
+        mx = fmaxf(mx, tmax[i]);
+    double sa = 0.0, sb = 0.0;
+    dd loc{0.0, 0.0};
+    for (int i = b; i < e; ++i) {
+        sa = __dadd_rn(sa, ts[i]);
+        sb = __dadd_rn(sb, ts2[i]);
+        loc = dd_add(loc, dd{B.tdh[i], B.tdl[i]});
+    }
+    const float m = group_max(mx, gt, L.f);
+    group_sum2(sa, sb, gt, L.d);
+    const double rs = sa, rs2 = sb;
+    // the decision in every workgroup; the log Z increment (two logs) only
+    // where it is recorded
+    StepOut o;
+    o.ess = __ddiv_rn(__dmul_rn(rs, rs), rs2);
+    o.resample = o.ess < A.tauN ? 1 : 0;
     if (blockIdx.x == 0 && gt == 0) {
+        o = step_finalize(A, m, rs, rs2, Cprev);
         st->logz = __dadd_rn(st->logz, o.logz_inc);
         st->m[c] = m;
         st->resampled[c] = o.resample;
@@ -128,7 +155,6 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     }
     if (!o.resample)
         return;
-    const uint32_t k = blockIdx.x;
     if (A.scheme == 0) {
         // multinomial: tile totals of the log spacings; the uniforms and
         // the counts follow in k_rs_uniforms and k_rs_mcounts
@@ -140,12 +166,32 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
         }
         return;
     }
-    dd off, next;
-    tiles_offset_dd(B.tdh, B.tdl, nt, (int)k, (int)k + 1, gt, L.d, off, next);
+    // offsets of tiles k and k + 1 (tiles_offset_dd on the chunk sums)
+    dd total;
+    const dd off = group_excl_scan_dd(loc, gt, L.d, total);
+    double *x = L.d + 8;
+    {
+        dd run = off;
+        for (int i = b; i < e; ++i) {
+            if (i == (int)k) {
+                x[0] = run.hi;
+                x[1] = run.lo;
+            }
+            if (i == (int)k + 1)
+                x[2] = run.hi;
+            run = dd_add(run, dd{B.tdh[i], B.tdl[i]});
+        }
+        if (gt == 0 && (int)k + 1 == nt)
+            x[2] = total.hi;
+    }
+    __syncthreads();
+    const dd toff{x[0], x[1]};
+    const double next_hi = x[2];
+    __syncthreads();
     const double invS = __ddiv_rn(1.0, rs);
     const UniformSeq U = make_uniform_seq(A, t);
     uint32_t es, zs;
-    tile_counts(A, k, lw, m, invS, off, next.hi, U, B.cnt, gt, L.d, L.u, es,
+    tile_counts_l(A, k, l, m, invS, toff, next_hi, U, B.cnt, gt, L.d, L.u, es,
         zs);
     if (gt == 0) {
         B.te[k] = es;
@@ -207,11 +253,9 @@ __device__ __forceinline__ void tiles_prefix_u2(const uint32_t *a,
         sa += a[i];
         sb += b[i];
     }
-    uint32_t ta, tb;
-    (void)group_excl_scan_u(sa, gt, ldsu, ta);
-    (void)group_excl_scan_u(sb, gt, ldsu + 4, tb);
-    pa = ta;
-    pb = tb;
+    group_sum2_u(sa, sb, gt, ldsu);
+    pa = sa;
+    pb = sb;
 }
 
 __global__ __launch_bounds__(kGroup) void k_rs_index(SmcArgs A, uint32_t t,
@@ -221,9 +265,11 @@ __global__ __launch_bounds__(kGroup) void k_rs_index(SmcArgs A, uint32_t t,
         return;
     __shared__ __align__(16) unsigned char lds[kScratch];
     const GroupLds L = group_lds(lds);
+    uint32_t c[kItems], prev;
+    load_counts(A, blockIdx.x, B.cnt, threadIdx.x, c, prev);
     uint32_t eoff, zoff;
     tiles_prefix_u2(B.te, B.tz, blockIdx.x, threadIdx.x, L.u, eoff, zoff);
-    tile_index(A, blockIdx.x, B.cnt, eoff, zoff, B.Eincl, idx, B.zpos,
+    tile_index_c(A, blockIdx.x, c, prev, eoff, zoff, B.Eincl, idx, B.zpos,
         threadIdx.x, L.u);
 }
 

@@ -29,7 +29,7 @@ __global__ void k_grad_reset(uint32_t *queue)
         *queue = 0;
 }
 
-constexpr int kGradGroups = 2; // 512 threads: 256 VGPRs without spills
+constexpr int kGradGroups = 2; // 512 threads: 4 groups (1024) would spill at 128 VGPRs
 
 // modes
 constexpr int kGradRun = 0;    // forward + adjoint
@@ -42,7 +42,7 @@ __device__ void copy_slot(const SlotPtr &d, const SlotPtr &s, uint32_t Np)
     for (uint32_t i = threadIdx.x; i < Np; i += G * kGroup) {
         d.x[i] = s.x[i];
         d.lw[i] = s.lw[i];
-        d.code[i] = s.code[i];
+        d.par[i] = s.par[i];
         d.E[i] = s.E[i];
         d.zpos[i] = s.zpos[i];
     }
@@ -164,9 +164,17 @@ __device__ double fwd_step(const GradArgs &A, uint32_t t, const SlotPtr &prev,
             gtile_index(A, (uint32_t)kk, cnt, S.teo[kk], S.tzo[kk], cur, gt, L.u);
             __syncthreads();
         }
+        // zero slots take their sources (resolved parents, no search later)
+        SmcArgs sa{};
+        sa.N = A.N;
+        for (int r = 0; r < rounds; ++r) {
+            const int k = r * G + grp;
+            tile_scatter(sa, (uint32_t)(k < nt ? k : nt), cnt, cur.E, cur.zpos,
+                cur.par, gt, L.u);
+        }
         if (mode == kGradRecord) {
             for (uint32_t i = threadIdx.x; i < A.N; i += G * kGroup) {
-                tape_par_out[i] = parent_of(cur.code[i], cur.E, A.N);
+                tape_par_out[i] = cur.par[i];
                 tape_lwn_out[i] = cur.lw[i] - lse;
             }
             __syncthreads();
@@ -293,7 +301,7 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
                         const double xbar = xb[i] + lwbar * gx;
                         tc += lwbar * gc;
                         const uint32_t par =
-                            Rprev ? parent_of(prev.code[i], prev.E, A.N)
+                            Rprev ? prev.par[i]
                                   : (uint32_t)i;
                         ta += xbar * prev.x[par];
                         tsg += xbar * eps_at(A, i, t);
@@ -330,7 +338,7 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
                         const uint64_t j = i0 + q;
                         if (j >= A.N)
                             break;
-                        const bool own = (prev.code[j] & kZeroFlag) == 0;
+                        const bool own = prev.par[j] == (uint32_t)j;
                         double xs = own ? xpb[j] : 0.0;
                         double cs = own ? cb[j] : 0.0;
                         const uint32_t Ej = prev.E[j];

@@ -52,7 +52,7 @@ struct GradArgs {
 // resampling structures that give the parents of step t+1).
 struct SlotPtr {
     double *x, *lw;
-    uint32_t *code, *E, *zpos;
+    uint32_t *par, *E, *zpos; // resolved parents of step t+1's slots
     double *scal; // m, S, lse, R
 };
 
@@ -63,8 +63,8 @@ __device__ __forceinline__ SlotPtr slot_at(double *base, size_t slot_bytes,
     SlotPtr s;
     s.x = reinterpret_cast<double *>(p);
     s.lw = s.x + Np;
-    s.code = reinterpret_cast<uint32_t *>(s.lw + Np);
-    s.E = s.code + Np;
+    s.par = reinterpret_cast<uint32_t *>(s.lw + Np);
+    s.E = s.par + Np;
     s.zpos = s.E + Np;
     s.scal = reinterpret_cast<double *>(s.zpos + Np);
     return s;
@@ -111,7 +111,7 @@ __device__ __forceinline__ double gtile_move(const GradArgs &G, uint32_t t,
         double xp, carry;
         if (Rprev) {
             const uint32_t p = tpar ? tpar[i]
-                                    : parent_of(prev.code[i], prev.E, G.N);
+                                    : prev.par[i];
             xp = prev.x[p];
             // value of the carried weight: LWn(theta)[p] - LWn(theta0)[p]
             carry = lwn_ref ? (prev.lw[p] - prev.scal[2]) - lwn_ref[p] : 0.0;
@@ -226,8 +226,9 @@ __device__ __forceinline__ void gtile_counts(const GradArgs &G, uint32_t tile,
     zsum = tz;
 }
 
-// tile_index plus the positions of the zero slots by rank (for the adjoint's
-// gather over children)
+// tile_index on a slot: E, par (own index; zero slots are filled by
+// tile_scatter) and the positions of the zero slots by rank, which the
+// adjoint's gather over children also uses
 __device__ __forceinline__ void gtile_index(const GradArgs &G, uint32_t tile,
     const uint32_t *cnt, uint32_t eoff, uint32_t zoff, const SlotPtr &sl,
     int gt, uint32_t *ldsu)
@@ -253,10 +254,8 @@ __device__ __forceinline__ void gtile_index(const GradArgs &G, uint32_t tile,
         if (i < G.N) {
             ex += rep[k] > 1 ? rep[k] - 1 : 0;
             sl.E[i] = ex;
-            if (rep[k] > 0) {
-                sl.code[i] = (uint32_t)i;
-            } else {
-                sl.code[i] = kZeroFlag | zx;
+            sl.par[i] = (uint32_t)i; // zero slots: overwritten by the scatter
+            if (rep[k] == 0) {
                 sl.zpos[zx] = (uint32_t)i;
                 ++zx;
             }

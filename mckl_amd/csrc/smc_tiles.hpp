@@ -113,6 +113,38 @@ __device__ __forceinline__ void load8(const uint32_t *p, uint64_t i0,
     }
 }
 
+// Store the 8 values of a thread: two 16-byte stores when all are below N.
+__device__ __forceinline__ void store8(uint32_t *p, uint64_t i0, uint64_t N,
+    const uint32_t (&v)[kItems])
+{
+    if (i0 + kItems <= N) {
+        *reinterpret_cast<uint4 *>(p + i0) = make_uint4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<uint4 *>(p + i0 + 4) =
+            make_uint4(v[4], v[5], v[6], v[7]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            if (i0 + k < N)
+                p[i0 + k] = v[k];
+    }
+}
+
+// group sums of two u32 per thread; results in every thread
+__device__ __forceinline__ void group_sum2_u(
+    uint32_t &a, uint32_t &b, int gt, uint32_t *ldsu)
+{
+    a = wave_sum_u(a);
+    b = wave_sum_u(b);
+    if ((gt & 63) == 0) {
+        ldsu[gt >> 6] = a;
+        ldsu[4 + (gt >> 6)] = b;
+    }
+    __syncthreads();
+    a = ldsu[0] + ldsu[1] + ldsu[2] + ldsu[3];
+    b = ldsu[4] + ldsu[5] + ldsu[6] + ldsu[7];
+    __syncthreads();
+}
+
 // t = 0: x_i = sd0 eps(i, 0), lw_i = 0
 __device__ __forceinline__ void tile_init(
     const SmcArgs &A, uint32_t tile, float *x, float *lw, int gt)
@@ -247,15 +279,13 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
 
 // e_i = exp_f(lw_i - m); the tile's S = sum e, S2 = sum e^2 (canonical
 // order) and the double-double total of e for the resampling prefix sums
-__device__ __forceinline__ void tile_sums(const SmcArgs &A, uint32_t tile,
-    const float *lw, float m, int gt, double *ldsd, double &S, double &S2,
-    dd &etot)
+__device__ __forceinline__ void tile_sums_l(const SmcArgs &A, uint32_t tile,
+    const float (&l)[kItems], float m, int gt, double *ldsd, double &S,
+    double &S2, dd &etot)
 {
     const uint64_t i0 = tile_first(tile, gt);
     double s = 0.0, s2 = 0.0;
     dd loc{0.0, 0.0};
-    float l[kItems];
-    load8(lw, i0, A.N, l, 0.0f);
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
@@ -270,6 +300,15 @@ __device__ __forceinline__ void tile_sums(const SmcArgs &A, uint32_t tile,
     S = s;
     S2 = s2;
     (void)group_excl_scan_dd(loc, gt, ldsd, etot);
+}
+
+__device__ __forceinline__ void tile_sums(const SmcArgs &A, uint32_t tile,
+    const float *lw, float m, int gt, double *ldsd, double &S, double &S2,
+    dd &etot)
+{
+    float l[kItems];
+    load8(lw, tile_first(tile, gt), A.N, l, 0.0f);
+    tile_sums_l(A, tile, l, m, gt, ldsd, S, S2, etot);
 }
 
 // Uniform sequences of step t (sorted, M = N):
@@ -339,10 +378,11 @@ struct UniformSeq {
 // on boundary values; the last particle overall takes cnt = M
 // (resample.hpp:405).  Returns the tile sums of the extra-copy and zero-slot
 // counts through esum/zsum.
-__device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
-    const float *lw, float m, double invS, dd tile_off, double next_hi,
-    const UniformSeq &U, uint32_t *cnt, int gt, double *ldsd, uint32_t *ldsu,
-    uint32_t &esum, uint32_t &zsum)
+// (l: the thread's 8 log weights, loaded by the caller)
+__device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
+    const float (&l)[kItems], float m, double invS, dd tile_off,
+    double next_hi, const UniformSeq &U, uint32_t *cnt, int gt, double *ldsd,
+    uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
 {
     const uint64_t i0 = tile_first(tile, gt);
     const uint64_t tend = (uint64_t)(tile + 1) * kTile < A.N
@@ -350,8 +390,6 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
         : A.N;
     double W[kItems];
     dd loc{0.0, 0.0};
-    float l[kItems];
-    load8(lw, i0, A.N, l, 0.0f);
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
@@ -376,9 +414,9 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
                     i == tend - 1 ? next_hi : dd_add(base, run).hi;
                 c[k] = U.count_less(__dmul_rn(P, invS));
             }
-            cnt[i] = c[k];
         }
     }
+    store8(cnt, i0, A.N, c);
     // count before the first particle of this thread
     ldsu[gt] = c[kItems - 1];
     __syncthreads();
@@ -399,28 +437,43 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
         }
         prev = c[k];
     }
-    uint32_t te, tz;
-    (void)group_excl_scan_u(es, gt, ldsu, te);
-    (void)group_excl_scan_u(zs, gt, ldsu + 4, tz);
-    esum = te;
-    zsum = tz;
+    group_sum2_u(es, zs, gt, ldsu);
+    esum = es;
+    zsum = zs;
+}
+
+__device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
+    const float *lw, float m, double invS, dd tile_off, double next_hi,
+    const UniformSeq &U, uint32_t *cnt, int gt, double *ldsd, uint32_t *ldsu,
+    uint32_t &esum, uint32_t &zsum)
+{
+    float l[kItems];
+    load8(lw, tile_first(tile, gt), A.N, l, 0.0f);
+    tile_counts_l(A, tile, l, m, invS, tile_off, next_hi, U, cnt, gt, ldsd,
+        ldsu, esum, zsum);
 }
 
 // Stable rep->index (resample.hpp:416-466, M = N): idx_i = i when rep_i > 0;
 // zero slots take extra copies in ascending source order.  Writes the
-// inclusive extra-copy prefix E_i, idx_i = i for the slots that keep a copy,
-// and zpos[r] = i for the zero slot of rank r; tile_scatter then fills the
-// zero slots.
-__device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
-    const uint32_t *cnt, uint32_t eoff, uint32_t zoff, uint32_t *Eincl,
-    uint32_t *idx, uint32_t *zpos, int gt, uint32_t *ldsu)
+// inclusive extra-copy prefix E_i, idx_i = i for every slot, and
+// zpos[r] = i for the zero slot of rank r; tile_scatter then overwrites the
+// zero slots with their sources.
+// the thread's 8 counts and the count before them
+__device__ __forceinline__ void load_counts(const SmcArgs &A, uint32_t tile,
+    const uint32_t *cnt, int gt, uint32_t (&c)[kItems], uint32_t &prev)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    load8(cnt, i0, A.N, c, 0u);
+    prev = (i0 == 0 || i0 > A.N) ? 0u : cnt[i0 - 1];
+}
+
+__device__ __forceinline__ void tile_index_c(const SmcArgs &A, uint32_t tile,
+    const uint32_t (&c)[kItems], uint32_t prev, uint32_t eoff, uint32_t zoff,
+    uint32_t *Eincl, uint32_t *idx, uint32_t *zpos, int gt, uint32_t *ldsu)
 {
     const uint64_t i0 = tile_first(tile, gt);
     uint32_t rep[kItems];
     uint32_t es = 0, zs = 0;
-    uint32_t c[kItems];
-    load8(cnt, i0, A.N, c, 0u);
-    uint32_t prev = (i0 == 0 || i0 > A.N) ? 0u : cnt[i0 - 1];
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
@@ -435,20 +488,29 @@ __device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
     uint32_t te, tz;
     uint32_t ex = group_excl_scan_u(es, gt, ldsu, te) + eoff;
     uint32_t zx = group_excl_scan_u(zs, gt, ldsu + 4, tz) + zoff;
+    uint32_t E[kItems], own[kItems];
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        if (i < A.N) {
-            ex += rep[k] > 1 ? rep[k] - 1 : 0;
-            Eincl[i] = ex;
-            if (rep[k] > 0) {
-                idx[i] = (uint32_t)i;
-            } else {
-                zpos[zx] = (uint32_t)i;
-                ++zx;
-            }
+        ex += rep[k] > 1 ? rep[k] - 1 : 0;
+        E[k] = ex;
+        own[k] = (uint32_t)i; // zero slots are overwritten by tile_scatter
+        if (i < A.N && rep[k] == 0) {
+            zpos[zx] = (uint32_t)i;
+            ++zx;
         }
     }
+    store8(Eincl, i0, A.N, E);
+    store8(idx, i0, A.N, own);
+}
+
+__device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
+    const uint32_t *cnt, uint32_t eoff, uint32_t zoff, uint32_t *Eincl,
+    uint32_t *idx, uint32_t *zpos, int gt, uint32_t *ldsu)
+{
+    uint32_t c[kItems], prev;
+    load_counts(A, tile, cnt, gt, c, prev);
+    tile_index_c(A, tile, c, prev, eoff, zoff, Eincl, idx, zpos, gt, ldsu);
 }
 
 // Source j with rep_j > 1 owns the extra copies of ranks
