@@ -97,6 +97,52 @@ class _CounterEngine:
     def words(self):
         return lib.mckl_hip_engine_words(ctypes.byref(self.state))
 
+    # operator<< (random/counter.hpp:235-249): result_, ctr_, generator_,
+    # index_, each array as "size v0 v1 ..."; the buffered block result_ is
+    # recomputed on the device from (key, ctr)
+    def state_string(self):
+        st = self.state
+        nbytes = 32 if self.KIND == _lib.THREEFRY4X64 else 16
+        blk = (ctypes.c_uint8 * nbytes)()
+        check(lib.mckl_hip_engine_block(ctypes.byref(st), blk))
+        raw = bytes(blk)
+        rb = self.RBITS // 8
+        res = [int.from_bytes(raw[i:i + rb], "little")
+               for i in range(0, nbytes, rb)]
+        K = 4 if self.KIND == _lib.THREEFRY4X64 else 2
+        parts = [str(len(res))] + [str(v) for v in res]
+        parts += [str(K)] + [str(int(v)) for v in st.ctr[:K]]
+        key = [int(v) for v in st.key]
+        if self.KIND == _lib.PHILOX4X32:
+            parts += ["2"] + [str(v & 0xFFFFFFFF) for v in key[:2]]
+        elif self.KIND == _lib.ARS:
+            parts += ["4"] + [str(v & 0xFFFFFFFF) for v in key]
+        else:
+            par = 0x1BD11BDAA9FC1A22
+            for v in key:
+                par ^= v
+            parts += ["8"] + [str(v) for v in key] + [str(par), "0", "0", "0"]
+        parts.append(str(int(st.index)))
+        return " ".join(parts)
+
+    # operator>> (random/counter.hpp:251-271)
+    def load_state_string(self, text):
+        v = [int(x) for x in text.split()]
+        m = v[0]
+        pos = 1 + m
+        K = v[pos]
+        ctr = v[pos + 1:pos + 1 + K]
+        pos += 1 + K
+        nk = v[pos]
+        key = v[pos + 1:pos + 1 + nk][:self.KEY_WORDS]
+        pos += 1 + nk
+        index = v[pos]
+        if len(v) != pos + 1 or index > m:
+            raise MCKLError("malformed engine state")
+        self.seed_key(key)
+        self.set_ctr(ctr)
+        self.state.index = index
+
 
 class Philox4x32(_CounterEngine):
     KIND, RBITS, KEY_WORDS = _lib.PHILOX4X32, 32, 2

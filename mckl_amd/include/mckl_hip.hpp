@@ -18,6 +18,9 @@
 #include <array>
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
+#include <istream>
+#include <ostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -117,6 +120,89 @@ class CounterEngineHIP
     friend bool operator!=(const CounterEngineHIP &a, const CounterEngineHIP &b)
     {
         return !(a == b);
+    }
+
+    /// operator<< (random/counter.hpp:235-249): result_ ctr_ generator_
+    /// index_, arrays as "size v0 v1 ..."; result_ is the buffered block
+    /// G(ctr), computed on the device
+    template <typename CharT, typename Traits>
+    friend std::basic_ostream<CharT, Traits> &operator<<(
+        std::basic_ostream<CharT, Traits> &os, const CounterEngineHIP &eng)
+    {
+        if (!os)
+            return os;
+        constexpr std::size_t nbytes = Kind == MCKL_HIP_THREEFRY4X64 ? 32 : 16;
+        unsigned char blk[nbytes];
+        hip_check(mckl_hip_engine_block(&eng.state_, blk));
+        constexpr std::size_t M = nbytes / sizeof(ResultType);
+        os << M;
+        for (std::size_t i = 0; i != M; ++i) {
+            ResultType v;
+            std::memcpy(&v, blk + i * sizeof(ResultType), sizeof(v));
+            os << ' ' << +v;
+        }
+        const ctr_type c = eng.ctr();
+        os << ' ' << c.size();
+        for (auto v : c)
+            os << ' ' << v;
+        const key_type k = eng.key();
+        if (Kind == MCKL_HIP_THREEFRY4X64) {
+            std::uint64_t par = 0x1BD11BDAA9FC1A22ULL;
+            for (auto v : k)
+                par ^= v;
+            os << ' ' << 8;
+            for (auto v : k)
+                os << ' ' << v;
+            os << ' ' << par << " 0 0 0";
+        } else {
+            os << ' ' << k.size();
+            for (auto v : k)
+                os << ' ' << v;
+        }
+        os << ' ' << eng.state_.index;
+        return os;
+    }
+
+    /// operator>> (random/counter.hpp:251-271)
+    template <typename CharT, typename Traits>
+    friend std::basic_istream<CharT, Traits> &operator>>(
+        std::basic_istream<CharT, Traits> &is, CounterEngineHIP &eng)
+    {
+        if (!is)
+            return is;
+        std::size_t m = 0, kc = 0, kk = 0;
+        unsigned index = 0;
+        is >> std::ws >> m;
+        for (std::size_t i = 0; i != m && is; ++i) {
+            unsigned long long r;
+            is >> std::ws >> r;
+        }
+        ctr_type c{};
+        is >> std::ws >> kc;
+        for (std::size_t i = 0; i != kc && is; ++i) {
+            unsigned long long v;
+            is >> std::ws >> v;
+            if (i < c.size())
+                c[i] = v;
+        }
+        key_type k{};
+        is >> std::ws >> kk;
+        for (std::size_t i = 0; i != kk && is; ++i) {
+            unsigned long long v;
+            is >> std::ws >> v;
+            if (i < k.size())
+                k[i] = static_cast<typename key_type::value_type>(v);
+        }
+        is >> std::ws >> index;
+        if (is && kc == c.size() && index <= m) {
+            CounterEngineHIP tmp(k);
+            tmp.ctr(c);
+            tmp.state_.index = index;
+            eng = tmp;
+        } else {
+            is.setstate(std::ios_base::failbit);
+        }
+        return is;
     }
 
   private:

@@ -12,6 +12,7 @@
 #include "mckl_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -393,6 +394,54 @@ static unsigned mo_block_bytes(int kind)
 unsigned mo_engine_words(const mo_engine *e)
 {
     return mo_block_bytes(e->kind) / (unsigned)(e->rbits / 8);
+}
+
+/* Textual state, CounterEngine operator<< (random/counter.hpp:235-249):
+ * result_ (M words), ctr_, generator_, index_, each std::array printed as
+ * "size v0 v1 ..." (internal/iostream.hpp:45-63).  The generators print
+ * Philox key_ (philox.hpp:145-156), Threefry par_ = key, parity, three
+ * tweak words (threefry.hpp:170-181, :109-121), ARS key_ (aes_key_seq.hpp:
+ * 162-173).  Returns the length written (NUL-terminated) or -1. */
+int mo_engine_text(const mo_engine *e, char *text, size_t cap)
+{
+    char tmp[1024];
+    int n = 0;
+    const unsigned M = mo_engine_words(e);
+    n += snprintf(tmp + n, sizeof(tmp) - n, "%u", M);
+    for (unsigned i = 0; i < M; ++i) {
+        uint64_t v = 0;
+        memcpy(&v, e->buf + i * (e->rbits / 8), (size_t)(e->rbits / 8));
+        n += snprintf(tmp + n, sizeof(tmp) - n, " %llu", (unsigned long long)v);
+    }
+    const int K = e->kind == MO_THREEFRY4X64 ? 4 : 2;
+    n += snprintf(tmp + n, sizeof(tmp) - n, " %d", K);
+    for (int i = 0; i < K; ++i)
+        n += snprintf(tmp + n, sizeof(tmp) - n, " %llu",
+            (unsigned long long)e->ctr[i]);
+    if (e->kind == MO_PHILOX4X32) {
+        n += snprintf(tmp + n, sizeof(tmp) - n, " 2 %llu %llu",
+            (unsigned long long)(uint32_t)e->key[0],
+            (unsigned long long)(uint32_t)e->key[1]);
+    } else if (e->kind == MO_ARS) {
+        n += snprintf(tmp + n, sizeof(tmp) - n, " 4");
+        for (int i = 0; i < 4; ++i)
+            n += snprintf(tmp + n, sizeof(tmp) - n, " %llu",
+                (unsigned long long)(uint32_t)e->key[i]);
+    } else {
+        const uint64_t par = 0x1BD11BDAA9FC1A22ull ^ e->key[0] ^ e->key[1] ^
+            e->key[2] ^ e->key[3];
+        n += snprintf(tmp + n, sizeof(tmp) - n, " 8");
+        for (int i = 0; i < 4; ++i)
+            n += snprintf(tmp + n, sizeof(tmp) - n, " %llu",
+                (unsigned long long)e->key[i]);
+        n += snprintf(tmp + n, sizeof(tmp) - n, " %llu 0 0 0",
+            (unsigned long long)par);
+    }
+    n += snprintf(tmp + n, sizeof(tmp) - n, " %u", e->index);
+    if ((size_t)n + 1 > cap)
+        return -1;
+    memcpy(text, tmp, (size_t)n + 1);
+    return n;
 }
 
 static void mo_increment1(uint64_t *ctr, int k)

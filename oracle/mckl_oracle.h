@@ -78,6 +78,7 @@ void mo_aes128_encrypt(const uint8_t key[16], const uint8_t in[16], uint8_t out[
 unsigned mo_engine_words(const mo_engine *e);
 void mo_engine_init(mo_engine *e, int kind, int rbits, const uint64_t *key);
 void mo_engine_seed(mo_engine *e, uint64_t s);
+int mo_engine_text(const mo_engine *e, char *text, size_t cap);
 void mo_engine_set_ctr(mo_engine *e, const uint64_t *ctr);
 void mo_rand_u32(mo_engine *e, size_t n, uint32_t *r);
 void mo_rand_u64(mo_engine *e, size_t n, uint64_t *r);

@@ -178,6 +178,13 @@ def resample(scheme, e, w, M=None):
     return rep[:N], idx[:M]
 
 
+def engine_text(e):
+    buf = ctypes.create_string_buffer(1024)
+    n = O.mo_engine_text(ctypes.byref(e), buf, _sz(1024))
+    assert n > 0
+    return buf.value.decode()
+
+
 def seed_keys(kind, seed, n, np_=1, rank=0, randomize=True):
     """SeedGenerator<key_type>::get() n times from internal seed `seed`:
     (n, 4) uint64 keys in engine-key layout."""

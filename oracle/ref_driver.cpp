@@ -129,6 +129,36 @@ int ref_engine_ops(int kind, int rbits, const std::uint64_t *key,
     });
 }
 
+// The engine's textual state (operator<<, random/counter.hpp:235-249)
+// after the same call sequence as ref_engine_ops; written NUL-terminated to
+// `text` (capacity `cap`).  Returns the length, or -1 if it does not fit.
+int ref_engine_text(int kind, int rbits, const std::uint64_t *key, int ncalls,
+    const long long *sizes, char *text, std::size_t cap)
+{
+    return dispatch(kind, rbits, key, [&](auto &rng) {
+        using R = typename std::remove_reference_t<decltype(rng)>::result_type;
+        std::vector<R> buf(1);
+        for (int i = 0; i != ncalls; ++i) {
+            if (sizes[i] > 0) {
+                buf.resize(static_cast<std::size_t>(sizes[i]));
+                mckl::rand(rng, static_cast<std::size_t>(sizes[i]), buf.data());
+            } else if (sizes[i] < 0) {
+                rng.discard(static_cast<typename std::remove_reference_t<
+                        decltype(rng)>::skip_type>(-sizes[i]));
+            } else {
+                (void)rng();
+            }
+        }
+        std::ostringstream os;
+        os << rng;
+        const std::string s = os.str();
+        if (s.size() + 1 > cap)
+            return -1;
+        std::memcpy(text, s.c_str(), s.size() + 1);
+        return static_cast<int>(s.size());
+    });
+}
+
 // pre raw u32 draws, then rand(rng, U01{CC,CO,OC,OO,Canonical}<T>, n, r)
 int ref_u01(int kind, const std::uint64_t *key, int interval, int dbl,
     std::size_t pre, std::size_t n, void *out, std::uint64_t *ctr,

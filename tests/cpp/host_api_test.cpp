@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <sstream>
 #include <vector>
 
 static int failures = 0;
@@ -73,6 +74,24 @@ int main(int argc, char **argv)
         for (int i = 0; i < 1001; ++i)
             d();
         CHECK(c() == d(), "ARS discard == scalar draws");
+        std::stringstream ss;
+        ss << c;
+        mckl::ARS e(77);
+        ss >> e;
+        CHECK(!ss.fail() && e == c && e() == c(),
+            "ARS operator<< / operator>> round trip");
+        // the reference's text for Philox4x32 key {7, 9} after rand(5),
+        // one scalar draw and discard(3) (tests/golden/engine_text.json)
+        mckl::Philox4x32 p(std::array<std::uint32_t, 2>{{7, 9}});
+        Dev<std::uint32_t> five(5);
+        mckl::rand(p, 5, five.p);
+        (void)p();
+        p.discard(3);
+        std::stringstream ps;
+        ps << p;
+        CHECK(ps.str() == "4 3170914008 3725014840 3378673226 1820808664 2 3 0 "
+                          "2 7 9 1",
+            "Philox4x32 operator<< matches the reference text");
     }
     // 3. Normal<double> moments
     {

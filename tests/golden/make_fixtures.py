@@ -17,6 +17,11 @@ Fixtures:
   weight_cases.npz     Weight::set_log / add_log inputs and outputs.
   observations.npz     simulated observations of the SMC configurations (the
                        recipes of mckl_amd/workloads.py on reference draws).
+  engine_text.json     CounterEngine's textual state (operator<<) after call
+                       sequences (bulk n > 0, scalar 0, discard -n), all six
+                       engines.
+
+`python tests/golden/make_fixtures.py NAME ...` writes only the named ones.
 """
 import hashlib
 import json
@@ -135,7 +140,41 @@ def observations():
     np.savez_compressed(os.path.join(HERE, "observations.npz"), **out)
 
 
+ENGINE_TEXT_SEQS = [
+    [5, 0, -3], [1], [4097], [0], [3, -100, 2], [-1], [4], [8, 0, 0, 0],
+    [2, -(1 << 40), 1], [1000, -7, 0, 11],
+]
+
+
+def engine_text():
+    import ctypes
+    cases = []
+    for kind, rbits, key in ((orc.PHILOX4X32, 32, [7, 9]),
+                             (orc.PHILOX4X32, 64, [0xDEADBEEF, 3]),
+                             (orc.THREEFRY4X64, 32, [1, 2, 3, 4]),
+                             (orc.THREEFRY4X64, 64, [2 ** 63 + 5, 0, 7, 1]),
+                             (orc.ARS, 32, [5, 6, 7, 8]),
+                             (orc.ARS, 64, [0xFFFFFFFF, 1, 2, 3])):
+        for seq in ENGINE_TEXT_SEQS:
+            k = (ctypes.c_uint64 * 4)(*(key + [0] * (4 - len(key))))
+            sz = (ctypes.c_longlong * len(seq))(*seq)
+            buf = ctypes.create_string_buffer(4096)
+            n = orc.REF.ref_engine_text(kind, rbits, k, len(seq), sz, buf,
+                                        4096)
+            assert n > 0
+            cases.append({"kind": kind, "rbits": rbits, "key": key,
+                          "calls": seq, "text": buf.value.decode()})
+    with open(os.path.join(HERE, "engine_text.json"), "w") as f:
+        json.dump(cases, f, indent=1)
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1:
+        if orc.REF is None:
+            sys.exit("oracle/_ref/libmckl_ref.so is not built")
+        for name in sys.argv[1:]:
+            globals()[name]()
+        sys.exit(0)
     if orc.REF is None:
         sys.exit("oracle/_ref/libmckl_ref.so is not built (make -C oracle ref)")
     # the workload recipes, imported without importing the GPU package

@@ -359,3 +359,26 @@ def test_engine_errors():
     with pytest.raises(m.MCKLError):
         m.rand(rng, m.NormalDistribution(0.0, -1.0, "float32"), 4,
                __import__("torch").empty(4, device="cuda"))
+
+
+def test_engine_text_state(dev):
+    """operator<< / operator>> (random/counter.hpp:235-271) against the
+    compiled reference's text after mixed call sequences, and a round trip
+    that continues the stream."""
+    m = _m()
+    cases = json.load(open(os.path.join(GOLDEN, "engine_text.json")))
+    for c in cases:
+        rng = _cls(c["kind"], c["rbits"])(key=c["key"])
+        for s in c["calls"]:
+            if s > 0:
+                out = _tensor(s, c["rbits"] // 8, dev)
+                m.rand(rng, s, out)
+            elif s < 0:
+                rng.discard(-s)
+            else:
+                rng()
+        assert rng.state_string() == c["text"], c
+        other = _cls(c["kind"], c["rbits"])(key=[1])
+        other.load_state_string(c["text"])
+        assert other == rng
+        assert other() == rng()

@@ -121,3 +121,27 @@ def test_discard_carry_quirk_vs_reference():
         orc.O.mo_discard(ctypes.byref(e), ctypes.c_uint64(3 * M + 1))
         got = orc.rand_u32(e, 2)
         assert np.array_equal(got, ref) and orc.state(e) == (rctr, ridx)
+
+
+def test_engine_text_fixture():
+    """CounterEngine operator<< (random/counter.hpp:235-249) from the
+    compiled reference after mixed call sequences (bulk, scalar, discard):
+    the oracle's restatement prints the same text."""
+    import json
+    cases = json.load(open(os.path.join(GOLDEN, "engine_text.json")))
+    for c in cases:
+        e = orc.engine(c["kind"], c["key"], c["rbits"])
+        for s in c["calls"]:
+            if s > 0:
+                if c["rbits"] == 32:
+                    orc.rand_u32(e, s)
+                else:
+                    orc.rand_u64(e, s)
+            elif s < 0:
+                orc.O.mo_discard(ctypes.byref(e), ctypes.c_uint64(-s))
+            else:
+                if c["rbits"] == 32:
+                    orc.O.mo_next_u32(ctypes.byref(e))
+                else:
+                    orc.rand_u64(e, 1)
+        assert orc.engine_text(e) == c["text"], c
