@@ -39,10 +39,14 @@ extern "C" {
 #define MCKL_HIP_U01_OO 3
 #define MCKL_HIP_U01_CANONICAL 4
 
-/* resampling schemes: algorithm/resample.hpp:43-59 (same numbering) */
+/* resampling schemes: ResampleScheme, algorithm/resample.hpp:43-50 (same
+ * numbering) */
 #define MCKL_HIP_MULTINOMIAL 0
-#define MCKL_HIP_STRATIFIED 2
-#define MCKL_HIP_SYSTEMATIC 3
+#define MCKL_HIP_STRATIFIED 1
+#define MCKL_HIP_SYSTEMATIC 2
+#define MCKL_HIP_RESIDUAL 3
+#define MCKL_HIP_RESIDUAL_STRATIFIED 4
+#define MCKL_HIP_RESIDUAL_SYSTEMATIC 5
 
 /* models of the built-in SMC samplers */
 #define MCKL_HIP_MODEL_LG 1 /* x' = a x + sx e, y = x + sy n, x0 ~ N(0,1) */

@@ -258,6 +258,9 @@ def streams_normal(engines, n, mean, stddev, r):
 MULTINOMIAL = _lib.MULTINOMIAL
 STRATIFIED = _lib.STRATIFIED
 SYSTEMATIC = _lib.SYSTEMATIC
+RESIDUAL = _lib.RESIDUAL
+RESIDUAL_STRATIFIED = _lib.RESIDUAL_STRATIFIED
+RESIDUAL_SYSTEMATIC = _lib.RESIDUAL_SYSTEMATIC
 MODEL_LG = _lib.MODEL_LG
 MODEL_SV = _lib.MODEL_SV
 
@@ -339,6 +342,19 @@ class ResampleStratified(_ResampleAlgorithm):
 
 class ResampleSystematic(_ResampleAlgorithm):
     SCHEME = SYSTEMATIC
+
+
+# ResampleAlgorithm<U01Seq, true>: residual family (resample.hpp:581-593)
+class ResampleResidual(_ResampleAlgorithm):
+    SCHEME = RESIDUAL
+
+
+class ResampleResidualStratified(_ResampleAlgorithm):
+    SCHEME = RESIDUAL_STRATIFIED
+
+
+class ResampleResidualSystematic(_ResampleAlgorithm):
+    SCHEME = RESIDUAL_SYSTEMATIC
 
 
 def select(idx, src, dst, dim=1):

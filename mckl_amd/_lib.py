@@ -18,7 +18,8 @@ ARS = 3
 
 U01_CC, U01_CO, U01_OC, U01_OO, U01_CANONICAL = 0, 1, 2, 3, 4
 
-MULTINOMIAL, STRATIFIED, SYSTEMATIC = 0, 2, 3
+MULTINOMIAL, STRATIFIED, SYSTEMATIC = 0, 1, 2
+RESIDUAL, RESIDUAL_STRATIFIED, RESIDUAL_SYSTEMATIC = 3, 4, 5
 
 MODEL_LG, MODEL_SV = 1, 2
 

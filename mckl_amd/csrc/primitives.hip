@@ -569,6 +569,142 @@ int check_scheme(int scheme)
     return MCKL_HIP_OK;
 }
 
+// ---- residual family: resample_trans_residual (resample.hpp:333-364) ----
+// resid_i = modf(M W_i, &integ_i); tile sums: resid in double-double,
+// integ exactly
+__global__ __launch_bounds__(kB) void k_residual_tiles(uint64_t N, double Md,
+    const double *w, double *resid, uint32_t *integ, double *th, double *tl,
+    unsigned long long *ti)
+{
+    __shared__ __align__(16) double sd[16];
+    __shared__ unsigned long long su[4];
+    const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
+    dd loc{0.0, 0.0};
+    unsigned long long isum = 0;
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i < N) {
+            double ip;
+            const double r = modf(__dmul_rn(Md, w[i]), &ip);
+            resid[i] = r;
+            integ[i] = (uint32_t)ip;
+            loc = dd_add(loc, dd{r, 0.0});
+            isum += (unsigned long long)ip;
+        }
+    }
+    dd tot;
+    (void)group_excl_scan_dd(loc, threadIdx.x, sd, tot);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        isum += __shfl_xor(isum, d, 64);
+    if ((threadIdx.x & 63) == 0)
+        su[threadIdx.x >> 6] = isum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        th[blockIdx.x] = tot.hi;
+        tl[blockIdx.x] = tot.lo;
+        ti[blockIdx.x] = su[0] + su[1] + su[2] + su[3];
+    }
+}
+
+// out[0] = 1 / sum(resid) (the sum rounded once from double-double),
+// out[1] = M - sum(integ) (as a double-exact integer < 2^31)
+__global__ __launch_bounds__(kB) void k_residual_total(const double *th,
+    const double *tl, const unsigned long long *ti, uint32_t nt, uint64_t M,
+    double *mul, unsigned long long *R)
+{
+    __shared__ __align__(16) double sd[16];
+    __shared__ unsigned long long su[4];
+    const int gt = threadIdx.x;
+    const int chunk = ((int)nt + kB - 1) / kB;
+    const int b = gt * chunk;
+    const int e = b + chunk < (int)nt ? b + chunk : (int)nt;
+    dd loc{0.0, 0.0};
+    unsigned long long isum = 0;
+    for (int i = b; i < e; ++i) {
+        loc = dd_add(loc, dd{th[i], tl[i]});
+        isum += ti[i];
+    }
+    dd tot;
+    (void)group_excl_scan_dd(loc, gt, sd, tot);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        isum += __shfl_xor(isum, d, 64);
+    if ((gt & 63) == 0)
+        su[gt >> 6] = isum;
+    __syncthreads();
+    if (gt == 0) {
+        *mul = __ddiv_rn(1.0, tot.hi);
+        *R = M - (su[0] + su[1] + su[2] + su[3]);
+    }
+}
+
+__global__ void k_residual_scale(uint64_t N, double *resid, const double *mul)
+{
+    const double s = *mul;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        resid[i] = __dmul_rn(resid[i], s);
+}
+
+__global__ void k_add_integ(uint64_t N, uint32_t *rep, const uint32_t *integ)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        rep[i] += integ[i];
+}
+
+// ResampleAlgorithm<U01Seq, true>::eval (resample.hpp:555-576): residuals,
+// R uniforms of the base scheme over them, plus the integral parts.  R is
+// read back to the host (the uniform sequence advances the engine by R).
+int residual_rep(int base, size_t N, size_t M, mckl_hip_engine *e,
+    const double *w, uint32_t *rep, hipStream_t st)
+{
+    const uint32_t nt = (uint32_t)ceil_div(N, kTile);
+    Scratch s(st);
+    const size_t o = align256(nt * sizeof(double));
+    const size_t on = align256(N * sizeof(double));
+    const size_t oi = align256(N * sizeof(uint32_t));
+    int rc = s.alloc(3 * o + on + oi + 512);
+    if (rc)
+        return rc;
+    char *base_p = static_cast<char *>(s.p);
+    double *th = reinterpret_cast<double *>(base_p);
+    double *tl = reinterpret_cast<double *>(base_p + o);
+    unsigned long long *ti =
+        reinterpret_cast<unsigned long long *>(base_p + 2 * o);
+    double *resid = reinterpret_cast<double *>(base_p + 3 * o);
+    uint32_t *integ = reinterpret_cast<uint32_t *>(base_p + 3 * o + on);
+    double *mul = reinterpret_cast<double *>(base_p + 3 * o + on + oi);
+    unsigned long long *dR = reinterpret_cast<unsigned long long *>(
+        base_p + 3 * o + on + oi + 256);
+    k_residual_tiles<<<nt, kB, 0, st>>>(
+        N, (double)M, w, resid, integ, th, tl, ti);
+    k_residual_total<<<1, kB, 0, st>>>(th, tl, ti, nt, M, mul, dR);
+    k_residual_scale<<<grid_for(N), 256, 0, st>>>(N, resid, mul);
+    MCKL_HIP_CHECK_LAUNCH("residual kernels");
+    unsigned long long R = 0;
+    MCKL_HIP_TRY(hipMemcpyAsync(&R, dR, sizeof(R), hipMemcpyDeviceToHost, st));
+    MCKL_HIP_TRY(hipStreamSynchronize(st));
+    if (R > M)
+        return set_error(MCKL_HIP_EINVAL,
+            "weights do not sum to one (integral parts exceed M)");
+    Scratch su(st);
+    rc = su.alloc(align256(std::max<size_t>(R, 1) * sizeof(double)));
+    if (rc)
+        return rc;
+    double *u = static_cast<double *>(su.p);
+    rc = sequence(base, e, R, u, st);
+    if (rc)
+        return rc;
+    rc = trans_u01_rep(N, R, resid, u, rep, st);
+    if (rc)
+        return rc;
+    k_add_integ<<<grid_for(N), 256, 0, st>>>(N, rep, integ);
+    MCKL_HIP_CHECK_LAUNCH("k_add_integ");
+    return MCKL_HIP_OK;
+}
+
 } // namespace
 } // namespace mckl_hip
 
@@ -644,7 +780,9 @@ int mckl_hip_resample_trans_rep_index(size_t N, size_t M,
 int mckl_hip_resample(int scheme, size_t N, size_t M, mckl_hip_engine *e,
     const double *d_w, uint32_t *d_rep, uint32_t *d_idx, void *stream)
 {
-    int rc = check_scheme(scheme);
+    const bool residual = scheme >= MCKL_HIP_RESIDUAL &&
+        scheme <= MCKL_HIP_RESIDUAL_SYSTEMATIC;
+    int rc = check_scheme(residual ? scheme - MCKL_HIP_RESIDUAL : scheme);
     if (rc)
         return rc;
     if (!e || !d_rep)
@@ -652,6 +790,16 @@ int mckl_hip_resample(int scheme, size_t N, size_t M, mckl_hip_engine *e,
     if (N >= 0x7FFFFFFFu || M >= 0x7FFFFFFFu)
         return set_error(MCKL_HIP_EINVAL, "sizes must be < 2^31");
     hipStream_t st = as_stream(stream);
+    if (residual) {
+        if (N == 0)
+            return MCKL_HIP_OK;
+        rc = residual_rep(scheme - MCKL_HIP_RESIDUAL, N, M, e, d_w, d_rep, st);
+        if (rc)
+            return rc;
+        if (d_idx)
+            rc = rep_index(N, M, d_rep, d_idx, st);
+        return rc;
+    }
     Scratch s(st);
     rc = s.alloc(align256(std::max<size_t>(M, 1) * sizeof(double)));
     if (rc)

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     }
     if (!o.resample)
         return;
-    if (A.scheme == 0) {
+    if (A.scheme == MCKL_HIP_MULTINOMIAL) {
         // multinomial: tile totals of the log spacings; the uniforms and
         // the counts follow in k_rs_uniforms and k_rs_mcounts
         dd mt;
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
             if (resampled) {
                 const double invS = __ddiv_rn(1.0, rs);
                 double *u = ub ? ub + off : nullptr;
-                if (A.scheme == 0) {
+                if (A.scheme == MCKL_HIP_MULTINOMIAL) {
                     // multinomial: sorted uniforms of the step first
                     for (int r = 0; r < rounds; ++r) {
                         const int k = r * G + grp;

@@ -145,7 +145,7 @@ __device__ double fwd_step(const GradArgs &A, uint32_t t, const SlotPtr &prev,
         U.t = t;
         U.M = A.N;
         U.delta = 1.0 / (double)A.N;
-        U.u0d = A.scheme == 3 ? smc_uniform(A.k0, A.k1, t, 0) * U.delta : 0.0;
+        U.u0d = A.scheme == MCKL_HIP_SYSTEMATIC ? smc_uniform(A.k0, A.k1, t, 0) * U.delta : 0.0;
         for (int r = 0; r < rounds; ++r) {
             const int k = r * G + grp;
             uint32_t es, zs;

@@ -56,7 +56,7 @@ struct SmcArgs {
     uint32_t N;
     uint32_t ntiles;
     double tauN; // resampling threshold * N
-    int scheme;  // 0 multinomial, 2 stratified, 3 systematic
+    int scheme;  // MCKL_HIP_MULTINOMIAL, _STRATIFIED or _SYSTEMATIC
     uint32_t T;
     const float *obs; // T observation constants
 };
@@ -324,7 +324,7 @@ struct UniformSeq {
 
     __device__ __forceinline__ double operator()(uint64_t j) const
     {
-        if (scheme == 3)
+        if (scheme == MCKL_HIP_SYSTEMATIC)
             return __dadd_rn(__dmul_rn((double)j, delta), u0d);
         return __dmul_rn(delta, __dadd_rn(smc_uniform(k0, k1, t, j), (double)j));
     }
@@ -337,7 +337,7 @@ struct UniformSeq {
     // count instead of two or three.  The count is the same either way.
     __device__ __forceinline__ uint32_t count_less(double S) const
     {
-        if (scheme == 0) {
+        if (scheme == MCKL_HIP_MULTINOMIAL) {
             // first j with u_j >= S (binary search: monotone in S)
             uint32_t lo = 0, hi = M;
             while (lo < hi) {
@@ -351,7 +351,7 @@ struct UniformSeq {
         }
         double g = floor(__dmul_rn(S, (double)M));
         uint32_t c = g <= 0.0 ? 0u : (g >= (double)M ? M : (uint32_t)g);
-        const bool strat = scheme == 2;
+        const bool strat = scheme == MCKL_HIP_STRATIFIED;
         while (c > 0) {
             if (strat && __dmul_rn(delta, (double)c) < S)
                 break; // u_{c-1} <= fl(delta c) < S
@@ -645,7 +645,7 @@ __device__ __forceinline__ UniformSeq make_uniform_seq(
     U.t = t;
     U.M = A.N;
     U.delta = __ddiv_rn(1.0, (double)A.N);
-    U.u0d = A.scheme == 3 ? __dmul_rn(smc_uniform(A.k0, A.k1, t, 0), U.delta)
+    U.u0d = A.scheme == MCKL_HIP_SYSTEMATIC ? __dmul_rn(smc_uniform(A.k0, A.k1, t, 0), U.delta)
                           : 0.0;
     return U;
 }

@@ -353,7 +353,10 @@ class Weight
 enum ResampleScheme {
     Multinomial = MCKL_HIP_MULTINOMIAL,
     Stratified = MCKL_HIP_STRATIFIED,
-    Systematic = MCKL_HIP_SYSTEMATIC
+    Systematic = MCKL_HIP_SYSTEMATIC,
+    Residual = MCKL_HIP_RESIDUAL,
+    ResidualStratified = MCKL_HIP_RESIDUAL_STRATIFIED,
+    ResidualSystematic = MCKL_HIP_RESIDUAL_SYSTEMATIC
 };
 
 /// ResampleAlgorithm (algorithm/resample.hpp:515-576) on device arrays
@@ -373,6 +376,9 @@ class ResampleAlgorithmHIP
 
 using ResampleMultinomial = ResampleAlgorithmHIP<Multinomial>;
 using ResampleStratified = ResampleAlgorithmHIP<Stratified>;
+using ResampleResidual = ResampleAlgorithmHIP<Residual>;
+using ResampleResidualStratified = ResampleAlgorithmHIP<ResidualStratified>;
+using ResampleResidualSystematic = ResampleAlgorithmHIP<ResidualSystematic>;
 using ResampleSystematic = ResampleAlgorithmHIP<Systematic>;
 
 inline void resample_trans_u01_rep(std::size_t N, std::size_t M,

@@ -945,20 +945,64 @@ void mo_resample_trans_rep_index(
     }
 }
 
+/* resample_trans_residual (algorithm/resample.hpp:333-364): integral parts
+ * and normalised residuals of M W_i; returns R = M - sum of integral parts */
+size_t mo_resample_trans_residual(size_t N, size_t M, const double *w,
+    double *resid, uint64_t *integ)
+{
+    double sum_resid = 0;
+    uint64_t sum_integ = 0;
+    const double coeff = (double)M;
+    for (size_t i = 0; i != N; ++i) {
+        const double x = coeff * w[i];
+        double ip;
+        resid[i] = modf(x, &ip);
+        integ[i] = (uint64_t)ip;
+        sum_resid += resid[i];
+        sum_integ += integ[i];
+    }
+    const double mul = 1 / sum_resid;
+    for (size_t i = 0; i != N; ++i)
+        resid[i] *= mul;
+    return M - (size_t)sum_integ;
+}
+
+static void mo_u01_sequence(int scheme, mo_engine *e, size_t n, double *u)
+{
+    if (scheme == MO_MULTINOMIAL || scheme == MO_RESIDUAL)
+        mo_u01_rand_sorted(e, n, u);
+    else if (scheme == MO_STRATIFIED || scheme == MO_RESIDUAL_STRATIFIED)
+        mo_u01_rand_stratified(e, n, u);
+    else
+        mo_u01_rand_systematic(e, n, u);
+}
+
+/* ResampleAlgorithm::eval (algorithm/resample.hpp:538-576): the plain
+ * schemes draw M uniforms; the residual ones keep the integral parts and
+ * draw R uniforms over the normalised residuals (schemes :581-593). */
 void mo_resample(int scheme, size_t N, size_t M, mo_engine *e,
     const double *w, uint64_t *rep, uint64_t *idx)
 {
-    double *u = (double *)malloc((M ? M : 1) * sizeof(double));
-    if (scheme == MO_MULTINOMIAL)
-        mo_u01_rand_sorted(e, M, u);
-    else if (scheme == MO_STRATIFIED)
-        mo_u01_rand_stratified(e, M, u);
-    else
-        mo_u01_rand_systematic(e, M, u);
-    mo_resample_trans_u01_rep(N, M, w, u, rep);
+    if (scheme >= MO_RESIDUAL) {
+        double *resid = (double *)malloc((N ? N : 1) * sizeof(double));
+        uint64_t *integ = (uint64_t *)malloc((N ? N : 1) * sizeof(uint64_t));
+        const size_t R = mo_resample_trans_residual(N, M, w, resid, integ);
+        double *u = (double *)malloc((R ? R : 1) * sizeof(double));
+        mo_u01_sequence(scheme, e, R, u);
+        mo_resample_trans_u01_rep(N, R, resid, u, rep);
+        for (size_t i = 0; i != N; ++i)
+            rep[i] += integ[i];
+        free(u);
+        free(integ);
+        free(resid);
+    } else {
+        double *u = (double *)malloc((M ? M : 1) * sizeof(double));
+        mo_u01_sequence(scheme, e, M, u);
+        mo_resample_trans_u01_rep(N, M, w, u, rep);
+        free(u);
+    }
     if (idx)
         mo_resample_trans_rep_index(N, M, rep, idx);
-    free(u);
 }
 
 /* ===================================================================== */

@@ -43,7 +43,15 @@ enum {
 
 enum { MO_CC = 0, MO_CO = 1, MO_OC = 2, MO_OO = 3, MO_CANONICAL = 4 };
 
-enum { MO_MULTINOMIAL = 0, MO_STRATIFIED = 2, MO_SYSTEMATIC = 3 };
+/* ResampleScheme numbering, algorithm/resample.hpp:43-50 */
+enum {
+    MO_MULTINOMIAL = 0,
+    MO_STRATIFIED = 1,
+    MO_SYSTEMATIC = 2,
+    MO_RESIDUAL = 3,
+    MO_RESIDUAL_STRATIFIED = 4,
+    MO_RESIDUAL_SYSTEMATIC = 5
+};
 
 /* Counter engine state: restates CounterEngine (random/counter.hpp:61-288).
  * buf holds the buffered block as raw bytes, index counts consumed result
@@ -112,6 +120,8 @@ void mo_resample_trans_u01_rep(size_t n, size_t m, const double *w,
     const double *u01, uint64_t *rep);
 void mo_resample_trans_rep_index(
     size_t n, size_t m, const uint64_t *rep, uint64_t *idx);
+size_t mo_resample_trans_residual(size_t N, size_t M, const double *w,
+    double *resid, uint64_t *integ);
 void mo_resample(int scheme, size_t n, size_t m, mo_engine *e,
     const double *w, uint64_t *rep, uint64_t *idx);
 

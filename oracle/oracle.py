@@ -17,7 +17,8 @@ REF_PATH = os.path.join(HERE, "_ref", "libmckl_ref.so")
 
 PHILOX4X32, THREEFRY4X64, ARS = 1, 2, 3
 CC, CO, OC, OO, CANONICAL = 0, 1, 2, 3, 4
-MULTINOMIAL, STRATIFIED, SYSTEMATIC = 0, 2, 3
+MULTINOMIAL, STRATIFIED, SYSTEMATIC = 0, 1, 2
+RESIDUAL, RESIDUAL_STRATIFIED, RESIDUAL_SYSTEMATIC = 3, 4, 5
 MODEL_LG, MODEL_SV = 1, 2
 
 O = ctypes.CDLL(ORACLE_PATH)
@@ -146,7 +147,7 @@ def weight_add_log(w, v):
 def u01_sequence(scheme, e, n):
     out = np.zeros(max(n, 1), np.float64)
     fn = {MULTINOMIAL: O.mo_u01_rand_sorted, STRATIFIED: O.mo_u01_rand_stratified,
-          SYSTEMATIC: O.mo_u01_rand_systematic}[scheme]
+          SYSTEMATIC: O.mo_u01_rand_systematic}[scheme % 3]
     fn(ctypes.byref(e), _sz(n), P(out))
     return out[:n]
 

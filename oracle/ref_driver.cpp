@@ -239,14 +239,16 @@ int ref_weight(int mode, std::size_t n, const double *v, double *w, double *ess)
     return 0;
 }
 
-// U01 sequences: scheme 0 sorted (multinomial), 2 stratified, 3 systematic
+// U01 sequences: scheme 0 sorted (multinomial), 1 stratified, 2 systematic
+// (ResampleScheme numbering)
 int ref_u01_sequence(int scheme, int kind, const std::uint64_t *key,
     std::size_t n, double *r, std::uint64_t *ctr, unsigned *index)
 {
+    const int base = scheme >= 3 ? scheme - 3 : scheme; // residual family
     return dispatch(kind, 32, key, [&](auto &rng) {
-        if (scheme == 0)
+        if (base == 0)
             mckl::u01_rand_sorted(rng, n, r);
-        else if (scheme == 2)
+        else if (base == 1)
             mckl::u01_rand_stratified(rng, n, r);
         else
             mckl::u01_rand_systematic(rng, n, r);
@@ -261,12 +263,25 @@ int ref_resample(int scheme, std::size_t N, std::size_t M, int kind,
 {
     return dispatch(kind, 32, key, [&](auto &rng) {
         std::vector<std::size_t> r(N + 1), x(M + 1);
-        if (scheme == 0)
-            mckl::ResampleMultinomial()(N, M, rng, w, r.data());
-        else if (scheme == 2)
-            mckl::ResampleStratified()(N, M, rng, w, r.data());
-        else
-            mckl::ResampleSystematic()(N, M, rng, w, r.data());
+        switch (scheme) {
+            case 0:
+                mckl::ResampleMultinomial()(N, M, rng, w, r.data());
+                break;
+            case 1:
+                mckl::ResampleStratified()(N, M, rng, w, r.data());
+                break;
+            case 2:
+                mckl::ResampleSystematic()(N, M, rng, w, r.data());
+                break;
+            case 3:
+                mckl::ResampleResidual()(N, M, rng, w, r.data());
+                break;
+            case 4:
+                mckl::ResampleResidualStratified()(N, M, rng, w, r.data());
+                break;
+            default:
+                mckl::ResampleResidualSystematic()(N, M, rng, w, r.data());
+        }
         mckl::resample_trans_rep_index(N, M, r.data(), x.data());
         for (std::size_t i = 0; i != N; ++i)
             rep[i] = r[i];

@@ -13,7 +13,8 @@ Fixtures:
                        Normal<double>(2.5,0.3)}, seeds 1, 2, 64, 2^22 values:
                        three windows each plus the final engine state.
   resample_cases.npz   W, rep, idx from ResampleAlgorithm + rep->index for the
-                       three schemes at several (N, M).
+                       six schemes (ResampleScheme numbering) at several
+                       (N, M); u is the scheme's uniform sequence over M.
   weight_cases.npz     Weight::set_log / add_log inputs and outputs.
   observations.npz     simulated observations of the SMC configurations (the
                        recipes of mckl_amd/workloads.py on reference draws).
@@ -82,7 +83,9 @@ def resample_cases():
     rng = np.random.default_rng(20240601)
     out = {}
     k = 0
-    for scheme in (orc.MULTINOMIAL, orc.STRATIFIED, orc.SYSTEMATIC):
+    for scheme in (orc.MULTINOMIAL, orc.STRATIFIED, orc.SYSTEMATIC,
+                   orc.RESIDUAL, orc.RESIDUAL_STRATIFIED,
+                   orc.RESIDUAL_SYSTEMATIC):
         for N, M in ((1, 1), (2, 2), (3, 3), (1000, 1000), (4097, 4097),
                      (300, 700), (700, 300), (65536, 65536)):
             w = rng.random(N) ** 4

@@ -19,6 +19,8 @@ import kalman
 
 pytestmark = pytest.mark.gpu
 
+STRAT, SYS = 1, 2  # ResampleScheme numbering
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 LG = (0.9, 1.0, 0.5)
 SV = (0.95, 0.2, 0.7)
@@ -47,7 +49,8 @@ def _fd(g, th, h=1e-4):
 
 
 @pytest.mark.parametrize("model,N,K,scheme", [
-    (1, 3000, 7, 3), (2, 4096, 10, 2), (2, 2049, 60, 3), (1, 5, 3, 3)])
+    (1, 3000, 7, SYS), (2, 4096, 10, STRAT), (2, 2049, 60, SYS),
+    (1, 5, 3, SYS)])
 def test_replay_and_fd(dev, model, N, K, scheme):
     m = _m()
     y = _obs()["lg_y"][:60] if model == 1 else _obs()["sv_y"][:60]
@@ -78,7 +81,7 @@ def test_checkpoint_interval_invariance(dev):
     y = _obs()["sv_y"][:40]
     res = []
     for K in (1, 3, 13, 40, 100):
-        g = m.SMCGradient(2, 2, 5000, y, [8, 9], 0.5, 3, K=K)
+        g = m.SMCGradient(2, 2, 5000, y, [8, 9], 0.5, SYS, K=K)
         res.append(g.run([SV, (0.9, 0.25, 0.6)]))
     for ll, gr in res[1:]:
         assert np.array_equal(ll, res[0][0])
@@ -91,12 +94,12 @@ def test_per_point_observations_and_one_shot(dev):
     th = ob["batch64_theta"][:5]
     y = ob["batch64_y"][:5]
     seeds = np.arange(30, 35, dtype=np.uint64)
-    g = m.SMCGradient(2, 5, 3000, y, seeds, 0.5, 2, K=10)
+    g = m.SMCGradient(2, 5, 3000, y, seeds, 0.5, STRAT, K=10)
     ll, gr = g.run(th)
-    ll1, gr1 = m.smc_loglik_grad(2, th, y, 3000, seeds, 0.5, 2, K=10)
+    ll1, gr1 = m.smc_loglik_grad(2, th, y, 3000, seeds, 0.5, STRAT, K=10)
     assert np.array_equal(ll, ll1) and np.array_equal(gr, gr1)
     for p in range(5):
-        gp = m.SMCGradient(2, 1, 3000, y[p], [seeds[p]], 0.5, 2, K=25)
+        gp = m.SMCGradient(2, 1, 3000, y[p], [seeds[p]], 0.5, STRAT, K=25)
         llp, grp = gp.run(th[p])
         assert llp[0] == ll[p] and np.array_equal(grp[0], gr[p])
 
@@ -111,9 +114,9 @@ def test_forward_close_to_plain_sampler(dev):
     m = _m()
     y = _obs()["lg_y"][:15]
     N, seed = 4096, 1
-    g = m.SMCGradient(1, 1, N, y, [seed], 0.5, 3, K=5)
+    g = m.SMCGradient(1, 1, N, y, [seed], 0.5, SYS, K=5)
     ll, _ = g.run(LG)
-    lp = m.smc_loglik(1, LG, y, N, seed, 0.5, 3)
+    lp = m.smc_loglik(1, LG, y, N, seed, 0.5, SYS)
     print("N=%d fp64 %.9f fp32 %.9f" % (N, ll[0], lp))
     assert abs(ll[0] - lp) < 1e-4
 
@@ -124,7 +127,7 @@ def test_lg_mean_gradient_vs_kalman(dev):
     m = _m()
     y = _obs()["lg_y"]
     nb = 64
-    g = m.SMCGradient(1, nb, 1 << 16, y, np.arange(1, nb + 1), 0.5, 3, K=50)
+    g = m.SMCGradient(1, nb, 1 << 16, y, np.arange(1, nb + 1), 0.5, SYS, K=50)
     ll, gr = g.run(LG)
     ex = kalman.grad(y, *LG)
     mean = gr.mean(0)
@@ -156,7 +159,7 @@ def test_torch_autograd_wrapper(dev):
     import torch
     m = _m()
     y = _obs()["sv_y"][:30]
-    g = m.SMCGradient(2, 2, 2000, y, [5, 6], 0.5, 2, K=8, tape=True)
+    g = m.SMCGradient(2, 2, 2000, y, [5, 6], 0.5, STRAT, K=8, tape=True)
 
     class LogLik(torch.autograd.Function):
         @staticmethod

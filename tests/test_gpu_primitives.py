@@ -66,20 +66,30 @@ def test_resample_fixtures(dev):
         rep = torch.zeros(N, dtype=torch.int32, device=dev)
         idx = torch.zeros(M, dtype=torch.int32, device=dev)
         rng = m.Philox4x32(key=[k0, 0])
-        alg = {0: m.ResampleMultinomial, 2: m.ResampleStratified,
-               3: m.ResampleSystematic}[scheme]()
+        alg = {0: m.ResampleMultinomial, 1: m.ResampleStratified,
+               2: m.ResampleSystematic, 3: m.ResampleResidual,
+               4: m.ResampleResidualStratified,
+               5: m.ResampleResidualSystematic}[scheme]()
         alg(N, M, rng, w, rep, idx)
         # uniform sequence itself
         rng2 = m.Philox4x32(key=[k0, 0])
         u = torch.zeros(max(M, 1), dtype=torch.float64, device=dev)
-        m.u01_rand_sequence(scheme, rng2, M, u)
+        m.u01_rand_sequence(scheme % 3, rng2, M, u)
         ug = u.cpu().numpy()[:M]
-        if scheme == 0:
+        if scheme % 3 == 0:
             # sorted uniforms: prefix of log(u)/(N-j), rounding may differ
             assert np.allclose(ug, fx[p + "_u"], rtol=1e-12, atol=1e-15)
         else:
             assert np.array_equal(ug, fx[p + "_u"])
         rg = rep.cpu().numpy().astype(np.uint64)
+        if scheme >= 3:
+            # residual family: integral parts exact, R uniforms over the
+            # normalised residuals
+            assert np.array_equal(rg, fx[p + "_rep"]), p
+            assert np.array_equal(idx.cpu().numpy().astype(np.uint64),
+                                  fx[p + "_idx"]), p
+            k += 1
+            continue
         near, _ = boundary_explained(fx[p + "_w"], fx[p + "_u"], fx[p + "_rep"],
                                      rg)
         total_near += near

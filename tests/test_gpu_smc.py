@@ -20,6 +20,8 @@ import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
+MULT, STRAT, SYS = orc.MULTINOMIAL, orc.STRATIFIED, orc.SYSTEMATIC
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -34,16 +36,16 @@ def _obs():
 
 CASES = [
     # model, theta, N, seed, scheme
-    (1, (0.9, 1.0, 0.5), 4096, 5, 3),
-    (1, (0.9, 1.0, 0.5), 5000, 6, 2),
-    (1, (0.9, 1.0, 0.5), 3, 7, 3),
-    (1, (0.9, 1.0, 0.5), 2049, 8, 3),
-    (2, (0.95, 0.2, 0.7), 4096, 9, 2),
-    (2, (0.95, 0.2, 0.7), 70001, 10, 3),
+    (1, (0.9, 1.0, 0.5), 4096, 5, SYS),
+    (1, (0.9, 1.0, 0.5), 5000, 6, STRAT),
+    (1, (0.9, 1.0, 0.5), 3, 7, SYS),
+    (1, (0.9, 1.0, 0.5), 2049, 8, SYS),
+    (2, (0.95, 0.2, 0.7), 4096, 9, STRAT),
+    (2, (0.95, 0.2, 0.7), 70001, 10, SYS),
     # multinomial: sorted uniforms from the log-spacing prefix
-    (1, (0.9, 1.0, 0.5), 4096, 11, 0),
-    (2, (0.95, 0.2, 0.7), 5000, 12, 0),
-    (1, (0.9, 1.0, 0.5), 2049, 13, 0),
+    (1, (0.9, 1.0, 0.5), 4096, 11, MULT),
+    (2, (0.95, 0.2, 0.7), 5000, 12, MULT),
+    (1, (0.9, 1.0, 0.5), 2049, 13, MULT),
 ]
 
 
@@ -66,8 +68,8 @@ def test_plain_vs_oracle(dev, case):
 def test_plain_repeatable(dev):
     m = _m()
     y = _obs()["sv_y"][:50]
-    a = m.smc_loglik(2, (0.95, 0.2, 0.7), y, 30000, 4, 0.5, 2)
-    b = m.smc_loglik(2, (0.95, 0.2, 0.7), y, 30000, 4, 0.5, 2)
+    a = m.smc_loglik(2, (0.95, 0.2, 0.7), y, 30000, 4, 0.5, STRAT)
+    b = m.smc_loglik(2, (0.95, 0.2, 0.7), y, 30000, 4, 0.5, STRAT)
     assert a == b
 
 
@@ -77,7 +79,7 @@ def test_plain_graph_replays(dev):
     import torch
     m = _m()
     y = _obs()["lg_y"][:80]
-    s = m.SMCSampler(1, (0.9, 1.0, 0.5), y, 50000, 21, 0.5, 3)
+    s = m.SMCSampler(1, (0.9, 1.0, 0.5), y, 50000, 21, 0.5, SYS)
     s.iterate()
     first = s.results()
     for k in range(3):
@@ -98,7 +100,7 @@ def test_lg_config_vs_kalman(dev):
     exact = kalman.loglik(y, 0.9, 1.0, 0.5)
     ests = []
     for seed in range(1, 5):
-        s = m.SMCSampler(1, (0.9, 1.0, 0.5), y, 1 << 20, seed, 0.5, 3)
+        s = m.SMCSampler(1, (0.9, 1.0, 0.5), y, 1 << 20, seed, 0.5, SYS)
         s.iterate()
         ll, ess, res = s.results()
         ests.append(ll)
@@ -119,13 +121,13 @@ def test_batched_equals_plain(dev):
     N[2] = 2049
     y = ob["batch64_y"][:24]
     seeds = np.arange(1, 25, dtype=np.uint64)
-    b = m.SMCBatch(2, N, theta, y, seeds, 0.5, 2)
+    b = m.SMCBatch(2, N, theta, y, seeds, 0.5, STRAT)
     for block in (256, 512, 1024):
         b.set_tune(block, 0)
         b.run(use_graph=False)
         lb = b.log_likelihood()
         for f in range(len(N)):
-            lp = m.smc_loglik(2, theta[f], y[f], int(N[f]), int(seeds[f]), 0.5, 2)
+            lp = m.smc_loglik(2, theta[f], y[f], int(N[f]), int(seeds[f]), 0.5, STRAT)
             assert lb[f] == lp, (block, f, lb[f], lp)
     b.run(use_graph=True)
     b.run(use_graph=True)
@@ -141,15 +143,15 @@ def test_multinomial_batched_and_kalman(dev):
     N = ob["batch64_N"][:8]
     y = ob["batch64_y"][:8]
     seeds = np.arange(40, 48, dtype=np.uint64)
-    b = m.SMCBatch(2, N, theta, y, seeds, 0.5, 0)
+    b = m.SMCBatch(2, N, theta, y, seeds, 0.5, MULT)
     b.run()
     lb = b.log_likelihood()
     for f in range(8):
-        lp = m.smc_loglik(2, theta[f], y[f], int(N[f]), int(seeds[f]), 0.5, 0)
+        lp = m.smc_loglik(2, theta[f], y[f], int(N[f]), int(seeds[f]), 0.5, MULT)
         assert lb[f] == lp, (f, lb[f], lp)
     yl = ob["lg_y"]
     exact = kalman.loglik(yl, 0.9, 1.0, 0.5)
-    ll = m.smc_loglik(1, (0.9, 1.0, 0.5), yl, 1 << 18, 3, 0.5, 0)
+    ll = m.smc_loglik(1, (0.9, 1.0, 0.5), yl, 1 << 18, 3, 0.5, MULT)
     print("kalman %.6f multinomial %.6f" % (exact, ll))
     assert abs(ll - exact) < 0.25
 
@@ -161,11 +163,11 @@ def test_batched_vs_oracle(dev):
     N = ob["batch64_N"][:6]
     y = ob["batch64_y"][:6]
     seeds = np.arange(11, 17, dtype=np.uint64)
-    b = m.SMCBatch(2, N, theta, y, seeds, 0.5, 2)
+    b = m.SMCBatch(2, N, theta, y, seeds, 0.5, STRAT)
     b.run()
     lb = b.log_likelihood()
     for f in range(6):
-        lo = orc.smc_run(2, theta[f], y[f], int(N[f]), int(seeds[f]), 0.5, 2)
+        lo = orc.smc_run(2, theta[f], y[f], int(N[f]), int(seeds[f]), 0.5, STRAT)
         assert abs(lb[f] - lo) < 1e-10 * max(1.0, abs(lo))
 
 

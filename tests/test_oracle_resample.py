@@ -23,7 +23,8 @@ def _cases():
 
 def test_resample_fixtures():
     """ResampleAlgorithm + resample_trans_rep_index from the compiled
-    reference, for the three schemes at several (N, M)."""
+    reference, for the six schemes (the residual family through
+    resample_trans_residual) at several (N, M)."""
     n = 0
     for fx, p in _cases():
         scheme, N, M, k0 = [int(v) for v in fx[p + "_meta"]]
@@ -35,7 +36,7 @@ def test_resample_fixtures():
         u = orc.u01_sequence(scheme, e, M)
         assert np.array_equal(u, fx[p + "_u"]), p
         n += 1
-    assert n == 24
+    assert n == 48
 
 
 def check_rep_index(rep, idx, M):
