@@ -179,3 +179,46 @@ def test_smc_errors(dev):
         m.SMCSampler(2, (1.5, 0.2, 0.7), np.zeros(3), 100, 1)
     with pytest.raises(m.MCKLError):
         m.SMCSampler(1, (0.9, 1.0, 0.5), np.zeros(3), 100, 1, 0.5, 7)
+
+
+@pytest.mark.parametrize("N,T,scheme", [((1 << 21) + 77, 8, SYS),
+                                        (1, 12, STRAT), (2, 12, MULT),
+                                        (2048, 1, SYS)])
+def test_plain_sizes_vs_oracle(dev, N, T, scheme):
+    """Sizes the configurations do not reach: more than 1024 tiles (several
+    tiles per thread in every cross-tile reduction), one and two particles,
+    a single step."""
+    m = _m()
+    y = _obs()["lg_y"][:T]
+    th = (0.9, 1.0, 0.5)
+    # threshold 0.99: every step with unequal weights resamples
+    s = m.SMCSampler(1, th, y, N, 31, 0.99, scheme)
+    s.iterate()
+    ll, ess, res = s.results()
+    oll, oess, ores = orc.smc_run(1, th, y, N, 31, 0.99, scheme, nthreads=16,
+                                  history=True)
+    if N > 2:
+        assert res.sum() == T
+    print("N=%d ll=%.12f oracle=%.12f resamples=%d" % (N, ll, oll, res.sum()))
+    assert np.array_equal(res, ores)
+    assert np.allclose(ess, oess, rtol=1e-11, atol=0)
+    assert abs(ll - oll) < 1e-9 * max(1.0, abs(oll))
+
+
+def test_batched_extreme_sizes(dev):
+    """Filters of 1, 2 and the maximum 131072 particles in one batch equal
+    their plain runs bit for bit."""
+    m = _m()
+    ob = _obs()
+    N = np.array([1, 2, 131072, 2047, 2048], np.uint32)
+    theta = ob["batch64_theta"][:5]
+    y = ob["batch64_y"][:5, :20]
+    seeds = np.arange(90, 95, dtype=np.uint64)
+    for scheme in (MULT, STRAT, SYS):
+        b = m.SMCBatch(2, N, theta, y, seeds, 0.5, scheme)
+        b.run()
+        lb = b.log_likelihood()
+        for f in range(5):
+            lp = m.smc_loglik(2, theta[f], y[f], int(N[f]), int(seeds[f]), 0.5,
+                              scheme)
+            assert lb[f] == lp, (scheme, f, lb[f], lp)
