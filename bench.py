@@ -12,8 +12,9 @@ distribution fills (Threefry4x64 / ARS x Normal<float>(0,1) /
 Normal<double>(2.5,0.3), 64 streams x 2^22), the plain SMC on the
 linear-Gaussian configuration (N = 2^20, T = 100), the batched SMC on
 4096 stochastic-volatility filters (T = 50) against one plain launch
-sequence per filter, and the SMC log-likelihood gradient (256 points x
-N = 2^16 x T = 500, with its finite-difference and Kalman checks).
+sequence per filter, the SMC log-likelihood gradient (256 points x
+N = 2^16 x T = 500, with its finite-difference and Kalman checks) and the
+ResampleIndex trace-back (N = 2^20 x 256 iterations).
 `--only LEG` runs one of these legs alone (for profiling).
 
 roofline: the dominant kernel (the U01OO<double> fill) timed with HIP
@@ -309,6 +310,44 @@ def bench_grad(m, torch):
     return res
 
 
+def bench_resample_index(m, torch, N=1 << 20, R=256, reps=5):
+    """ResampleIndex::read_index_matrix over R = 256 iterations of N = 2^20
+    particles, both layouts, for a genealogy of multinomial resampling
+    (random parents) and one of systematic resampling of near-equal weights
+    (parents mostly the particle itself, as in a well-mixed sampler).
+    Algorithmic bytes: 4 written + 4 read per matrix entry."""
+    res = {}
+    g = torch.Generator(device="cpu").manual_seed(3)
+    rng = m.Philox4x32(3)
+    rep = torch.empty(N, dtype=torch.int32, device="cuda")
+    for kind, alg, spread in (("multinomial", m.ResampleMultinomial(), 1.0),
+                              ("systematic_flat", m.ResampleSystematic(),
+                               0.05)):
+        ri = m.ResampleIndex()
+        for _ in range(R):
+            w = 1.0 + spread * torch.rand(N, generator=g, dtype=torch.float64)
+            w = (w / w.sum()).cuda()
+            idx = torch.empty(N, dtype=torch.int32, device="cuda")
+            alg(N, N, rng, w, rep, idx)
+            ri.push_back(N, idx)
+        out = torch.empty(N * R, dtype=torch.int32, device="cuda")
+        for layout, lname in ((m.RowMajor, "row"), (m.ColMajor, "col")):
+            ri.read_index_matrix(layout, out)
+            a, b = _events()
+            a.record()
+            for _ in range(reps):
+                ri.read_index_matrix(layout, out)
+            b.record()
+            b.synchronize()
+            t = a.elapsed_time(b) / reps / 1e3
+            res["resample_index_%s_%s_GBps" % (kind, lname)] = round(
+                8.0 * N * R / t / 1e9, 1)
+            res["resample_index_%s_%s_ms" % (kind, lname)] = round(t * 1e3, 3)
+        del ri, out
+    res["resample_index_config"] = "N=2^20, R=256, u32 entries"
+    return res
+
+
 def _traffic():
     """HBM bytes per launch of the U01OO<double> fill (FETCH_SIZE +
     WRITE_SIZE, each scaled by the same counter on a copy of known size),
@@ -337,7 +376,8 @@ def cpu_reference_points(threads):
     reference compiled from its headers (Normal fills) and the oracle port
     (SMC; the reference has no counterpart of these samplers).  Bounded
     samples: 2^22 normals, the plain configuration over its first 10 steps,
-    the first 64 filters of the batched configuration."""
+    the first 64 filters of the batched configuration, the trace-back of 64
+    iterations of 2^20 particles."""
     import numpy as np
     from mckl_amd import workloads as wl
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -372,6 +412,23 @@ def cpu_reference_points(threads):
     t = time.perf_counter() - t0
     res["smc_batched_sv_port_%dthreads_particle_steps_per_s" % threads] = \
         ps / t
+    if orc.REF is not None:
+        # the reference's read_index_matrix (column-major) over 64 iterations
+        # of 2^20 uniformly random parents, 8 algorithmic bytes per entry
+        N, R = 1 << 20, 64
+        g = np.random.default_rng(5)
+        orc.REF.ref_ri_new()
+        for _ in range(R):
+            v = g.integers(0, N, N).astype(np.uint32)
+            orc.REF.ref_ri_push(ctypes.c_size_t(N), orc.P(v))
+        buf = np.zeros(N * R, np.uint32)
+        t0 = time.perf_counter()
+        orc.REF.ref_ri_matrix(102, ctypes.c_size_t(R - 1), ctypes.c_size_t(0),
+                              orc.P(buf))
+        t = time.perf_counter() - t0
+        res["resample_index_col_reference_1thread_GBps"] = round(
+            8.0 * N * R / t / 1e9, 3)
+        orc.REF.ref_ri_new()
     return res
 
 
@@ -414,7 +471,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--only", choices=["fills", "saturation", "normals", "smc",
-                                       "smc_plain", "smc_batched", "grad"],
+                                       "smc_plain", "smc_batched", "grad",
+                                       "resample_index"],
                     help="run one secondary leg alone and print its JSON "
                          "(for profiling)")
     args = ap.parse_args()
@@ -441,7 +499,8 @@ def main():
         leg = {"saturation": bench_saturation, "normals": bench_normals,
                "smc": bench_smc, "smc_plain": bench_smc_plain,
                "smc_batched": bench_smc_batched,
-               "grad": bench_grad}[args.only]
+               "grad": bench_grad,
+               "resample_index": bench_resample_index}[args.only]
         print(json.dumps(leg(m, torch)), flush=True)
         return
     c1 = bench_config1(m, torch, args.steps, args.warmup)
@@ -457,6 +516,7 @@ def main():
         for name, leg in (("saturation", bench_saturation),
                           ("normals", bench_normals), ("smc", bench_smc),
                           ("grad", bench_grad),
+                          ("resample_index", bench_resample_index),
                           ("cpu", lambda m_, t_: {
                               "cpu": _guarded(cpu_reference_points,
                                               threads)})):

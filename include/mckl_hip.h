@@ -207,6 +207,52 @@ int mckl_hip_select_f32(size_t M, size_t dim, const uint32_t *d_idx,
     const float *d_src, float *d_dst, void *stream);
 
 /*
+ * ResampleIndex<uint32_t> (algorithm/resample.hpp:597-832): the history of
+ * resampling indices, one device vector per iteration, and its trace-back.
+ * Bookkeeping follows the reference's member by member: reset() zeroes the
+ * iteration count and keeps the vectors for reuse, clear() frees them and
+ * keeps the count, push_back() stores iteration num_iter() into vector
+ * num_iter()-1 when that vector exists and appends a vector otherwise.
+ * Queries the reference guards with runtime_assert return MCKL_HIP_EINVAL,
+ * and so do queries that would read a vector clear() has freed.  An entry
+ * that does not index the vector it is read against traces to 0xFFFFFFFF.
+ * Calls on one handle are not thread-safe.  Layouts: MatrixLayout,
+ * internal/defines.hpp:46.
+ */
+#define MCKL_HIP_ROW_MAJOR 101
+#define MCKL_HIP_COL_MAJOR 102
+typedef struct mckl_hip_resample_index mckl_hip_resample_index;
+int mckl_hip_resample_index_create(mckl_hip_resample_index **out);
+int mckl_hip_resample_index_destroy(mckl_hip_resample_index *h);
+/* num_iter(): resample.hpp:606 */
+int mckl_hip_resample_index_num_iter(
+    const mckl_hip_resample_index *h, size_t *out);
+/* size() (iter = SIZE_MAX) and size(iter): resample.hpp:609-625 */
+int mckl_hip_resample_index_size(
+    const mckl_hip_resample_index *h, size_t iter, size_t *out);
+/* reset(), clear(): resample.hpp:628-631 */
+int mckl_hip_resample_index_reset(mckl_hip_resample_index *h);
+int mckl_hip_resample_index_clear(mckl_hip_resample_index *h);
+/* push_back(N) (d_idx NULL: identity) and push_back(N, first):
+ * resample.hpp:634-656; d_idx is copied on `stream` */
+int mckl_hip_resample_index_push_back(mckl_hip_resample_index *h, size_t N,
+    const uint32_t *d_idx, void *stream);
+/* index(id, iter_back, iter) for n ids at once: resample.hpp:658-681.
+ * As in the reference the trace starts from the last vector held (not from
+ * vector iter_back) and then walks vectors iter_back-1 down to iter. */
+int mckl_hip_resample_index_index(const mckl_hip_resample_index *h,
+    size_t n, const uint32_t *d_id, size_t iter_back, size_t iter,
+    uint32_t *d_out, void *stream);
+/* index_matrix_nrow / ncol: resample.hpp:683-715 */
+int mckl_hip_resample_index_matrix_shape(const mckl_hip_resample_index *h,
+    size_t iter_back, size_t iter, size_t *nrow, size_t *ncol);
+/* read_index_matrix(layout, first, iter_back, iter): resample.hpp:741-810;
+ * d_out holds nrow * ncol entries, element (i, c) = trace of particle i of
+ * iteration iter_back back to iteration iter + c */
+int mckl_hip_resample_index_read_matrix(const mckl_hip_resample_index *h,
+    int layout, size_t iter_back, size_t iter, uint32_t *d_out, void *stream);
+
+/*
  * Plain SMC sampler (SMCSampler::iterate, algorithm/smc.hpp:352-406) for the
  * built-in models, FP32 states and log-weights, FP64 reductions and scans.
  * The handle owns its device buffers; run() is asynchronous and contains no

@@ -365,6 +365,103 @@ def select(idx, src, dst, dim=1):
     return dst
 
 
+RowMajor = _lib.ROW_MAJOR
+ColMajor = _lib.COL_MAJOR
+_SIZE_MAX = 2 ** 64 - 1
+
+
+class ResampleIndex:
+    """ResampleIndex<uint32_t> (algorithm/resample.hpp:597-832) with the
+    history in device memory.  Index vectors are int32/uint32 device tensors
+    (u32 entries); the trace-back runs on the device.  Iteration defaults
+    follow the reference: ``iter_back`` = num_iter() - 1, ``iter`` = 0."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib.mckl_hip_resample_index_create(ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.mckl_hip_resample_index_destroy(h)
+            self._h = None
+
+    def num_iter(self):
+        n = ctypes.c_size_t()
+        check(lib.mckl_hip_resample_index_num_iter(self._h, ctypes.byref(n)))
+        return n.value
+
+    def size(self, iter=None):
+        n = ctypes.c_size_t()
+        check(lib.mckl_hip_resample_index_size(
+            self._h, _SIZE_MAX if iter is None else iter, ctypes.byref(n)))
+        return n.value
+
+    def reset(self):
+        check(lib.mckl_hip_resample_index_reset(self._h))
+
+    def clear(self):
+        check(lib.mckl_hip_resample_index_clear(self._h))
+
+    def push_back(self, N, idx=None):
+        """push_back(N) (identity) or push_back(N, idx) with idx a device
+        tensor of at least N 32-bit entries."""
+        if idx is not None and idx.numel() < N:
+            raise MCKLError("push_back: index vector shorter than N")
+        check(lib.mckl_hip_resample_index_push_back(
+            self._h, N, _ptr(idx) if idx is not None else None, _stream()))
+
+    def _back(self, iter_back):
+        return (self.num_iter() - 1) % (2 ** 64) if iter_back is None \
+            else iter_back
+
+    def index(self, id, iter_back=None, iter=0):
+        """index(id, iter_back, iter): an int for an int id; for a device
+        tensor of ids, a device tensor of traced indices."""
+        import torch
+        ib = self._back(iter_back)
+        scalar = not torch.is_tensor(id)
+        ids = torch.tensor([id], dtype=torch.int32, device="cuda") \
+            if scalar else id
+        out = torch.empty(ids.numel(), dtype=torch.int32, device=ids.device)
+        check(lib.mckl_hip_resample_index_index(
+            self._h, ids.numel(), _ptr(ids), ib, iter, _ptr(out), _stream()))
+        if scalar:
+            return int(out.cpu().numpy().view(np.uint32)[0])
+        return out
+
+    def _shape(self, iter_back, iter):
+        nr, nc = ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib.mckl_hip_resample_index_matrix_shape(
+            self._h, iter_back, iter, ctypes.byref(nr), ctypes.byref(nc)))
+        return nr.value, nc.value
+
+    def index_matrix_nrow(self, iter_back=None):
+        return self._shape(self._back(iter_back), 0)[0]
+
+    def index_matrix_ncol(self, iter_back=None, iter=0):
+        return self._shape(self._back(iter_back), iter)[1]
+
+    def read_index_matrix(self, layout, out, iter_back=None, iter=0):
+        ib = self._back(iter_back)
+        nr, nc = self._shape(ib, iter)
+        if out.numel() < nr * nc:
+            raise MCKLError("read_index_matrix: output smaller than %d x %d"
+                            % (nr, nc))
+        check(lib.mckl_hip_resample_index_read_matrix(
+            self._h, layout, ib, iter, _ptr(out), _stream()))
+        return out
+
+    def index_matrix(self, layout, iter_back=None, iter=0):
+        import torch
+        ib = self._back(iter_back)
+        nr, nc = self._shape(ib, iter)
+        out = torch.empty(max(nr * nc, 1), dtype=torch.int32, device="cuda")
+        self.read_index_matrix(layout, out, ib, iter)
+        return out[:nr * nc]
+
+
 # ---------------------------------------------------------------------------
 # SMC samplers (algorithm/smc.hpp:151-421 shape) for the built-in models
 # ---------------------------------------------------------------------------

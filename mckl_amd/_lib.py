@@ -23,6 +23,8 @@ RESIDUAL, RESIDUAL_STRATIFIED, RESIDUAL_SYSTEMATIC = 3, 4, 5
 
 MODEL_LG, MODEL_SV = 1, 2
 
+ROW_MAJOR, COL_MAJOR = 101, 102
+
 
 class Engine(ctypes.Structure):
     """mckl_hip_engine: key, counter and consumed-word index."""
@@ -88,6 +90,18 @@ SIGNATURES = {
     "mckl_hip_resample_trans_rep_index": (_int, [_sz, _sz, _vp, _vp, _vp]),
     "mckl_hip_resample": (_int, [_int, _sz, _sz, _EP, _vp, _vp, _vp, _vp]),
     "mckl_hip_select_f32": (_int, [_sz, _sz, _vp, _vp, _vp, _vp]),
+    "mckl_hip_resample_index_create": (_int, [ctypes.POINTER(_vp)]),
+    "mckl_hip_resample_index_destroy": (_int, [_vp]),
+    "mckl_hip_resample_index_num_iter": (_int, [_vp, ctypes.POINTER(_sz)]),
+    "mckl_hip_resample_index_size": (_int, [_vp, _sz, ctypes.POINTER(_sz)]),
+    "mckl_hip_resample_index_reset": (_int, [_vp]),
+    "mckl_hip_resample_index_clear": (_int, [_vp]),
+    "mckl_hip_resample_index_push_back": (_int, [_vp, _sz, _vp, _vp]),
+    "mckl_hip_resample_index_index": (_int, [_vp, _sz, _vp, _sz, _sz, _vp, _vp]),
+    "mckl_hip_resample_index_matrix_shape": (
+        _int, [_vp, _sz, _sz, ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
+    "mckl_hip_resample_index_read_matrix": (
+        _int, [_vp, _int, _sz, _sz, _vp, _vp]),
     "mckl_hip_smc_create": (
         _int, [_int, ctypes.POINTER(_dbl), _sz, ctypes.POINTER(_dbl), _sz,
                _u64, _dbl, _int, ctypes.POINTER(_vp)]),

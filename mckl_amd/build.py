@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmckl_hip.so")
 
-HIP_SOURCES = ["rng.hip", "smc.hip", "primitives.hip", "smc_grad.hip"]
+HIP_SOURCES = ["rng.hip", "smc.hip", "primitives.hip", "smc_grad.hip",
+               "resample_index.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -395,6 +395,146 @@ inline void resample_trans_rep_index(std::size_t N, std::size_t M,
         N, M, d_rep, d_idx, current_stream()));
 }
 
+enum class MatrixLayout { RowMajor = MCKL_HIP_ROW_MAJOR,
+    ColMajor = MCKL_HIP_COL_MAJOR };
+constexpr MatrixLayout RowMajor = MatrixLayout::RowMajor;
+constexpr MatrixLayout ColMajor = MatrixLayout::ColMajor;
+
+/// ResampleIndex<std::uint32_t> (algorithm/resample.hpp:597-832); index
+/// vectors and matrices are device arrays, index(id, ...) returns to the host
+class ResampleIndexHIP
+{
+  public:
+    using index_type = std::uint32_t;
+
+    ResampleIndexHIP() { hip_check(mckl_hip_resample_index_create(&h_)); }
+    ~ResampleIndexHIP() { mckl_hip_resample_index_destroy(h_); }
+    ResampleIndexHIP(const ResampleIndexHIP &) = delete;
+    ResampleIndexHIP &operator=(const ResampleIndexHIP &) = delete;
+
+    std::size_t num_iter() const
+    {
+        std::size_t n = 0;
+        hip_check(mckl_hip_resample_index_num_iter(h_, &n));
+        return n;
+    }
+
+    std::size_t size() const { return size(SIZE_MAX); }
+
+    std::size_t size(std::size_t iter) const
+    {
+        std::size_t n = 0;
+        hip_check(mckl_hip_resample_index_size(h_, iter, &n));
+        return n;
+    }
+
+    void reset() { hip_check(mckl_hip_resample_index_reset(h_)); }
+    void clear() { hip_check(mckl_hip_resample_index_clear(h_)); }
+
+    void push_back(std::size_t N)
+    {
+        hip_check(mckl_hip_resample_index_push_back(
+            h_, N, nullptr, current_stream()));
+    }
+
+    void push_back(std::size_t N, const index_type *d_first)
+    {
+        hip_check(mckl_hip_resample_index_push_back(
+            h_, N, d_first, current_stream()));
+    }
+
+    index_type index(std::size_t id) const
+    {
+        return index(id, num_iter() - 1, 0);
+    }
+
+    index_type index(std::size_t id, std::size_t iter_back) const
+    {
+        return index(id, iter_back, 0);
+    }
+
+    index_type index(
+        std::size_t id, std::size_t iter_back, std::size_t iter) const
+    {
+        index_type *d = nullptr;
+        index_type r = static_cast<index_type>(id);
+        if (hipMalloc(reinterpret_cast<void **>(&d), 2 * sizeof(r)) !=
+            hipSuccess)
+            throw RuntimeAssert("mckl_hip: hipMalloc failed");
+        int rc = hipMemcpy(d, &r, sizeof(r), hipMemcpyHostToDevice) ==
+                hipSuccess ?
+            mckl_hip_resample_index_index(
+                h_, 1, d, iter_back, iter, d + 1, current_stream()) :
+            MCKL_HIP_EHIP;
+        if (rc == MCKL_HIP_OK &&
+            hipMemcpy(&r, d + 1, sizeof(r), hipMemcpyDeviceToHost) !=
+                hipSuccess)
+            rc = MCKL_HIP_EHIP;
+        hipFree(d);
+        hip_check(rc);
+        return r;
+    }
+
+    std::size_t index_matrix_nrow() const
+    {
+        return index_matrix_nrow(num_iter() - 1);
+    }
+
+    std::size_t index_matrix_nrow(std::size_t iter_back) const
+    {
+        std::size_t r = 0, c = 0;
+        hip_check(mckl_hip_resample_index_matrix_shape(h_, iter_back, 0, &r, &c));
+        return r;
+    }
+
+    std::size_t index_matrix_ncol() const
+    {
+        return index_matrix_ncol(num_iter() - 1, 0);
+    }
+
+    std::size_t index_matrix_ncol(std::size_t iter_back) const
+    {
+        return index_matrix_ncol(iter_back, 0);
+    }
+
+    std::size_t index_matrix_ncol(
+        std::size_t iter_back, std::size_t iter) const
+    {
+        std::size_t r = 0, c = 0;
+        hip_check(
+            mckl_hip_resample_index_matrix_shape(h_, iter_back, iter, &r, &c));
+        return c;
+    }
+
+    /// writes nrow * ncol entries to d_first, returns d_first + nrow * ncol
+    index_type *read_index_matrix(MatrixLayout layout, index_type *d_first,
+        std::size_t iter_back, std::size_t iter) const
+    {
+        hip_check(mckl_hip_resample_index_read_matrix(h_,
+            static_cast<int>(layout), iter_back, iter, d_first,
+            current_stream()));
+        return d_first +
+            index_matrix_nrow(iter_back) * index_matrix_ncol(iter_back, iter);
+    }
+
+    index_type *read_index_matrix(MatrixLayout layout, index_type *d_first,
+        std::size_t iter_back) const
+    {
+        return read_index_matrix(layout, d_first, iter_back, 0);
+    }
+
+    index_type *read_index_matrix(
+        MatrixLayout layout, index_type *d_first) const
+    {
+        return read_index_matrix(layout, d_first, num_iter() - 1, 0);
+    }
+
+  private:
+    mckl_hip_resample_index *h_ = nullptr;
+};
+
+using ResampleIndex = ResampleIndexHIP;
+
 /// SMCSampler (algorithm/smc.hpp:151-421) for the built-in models
 class SMCSamplerHIP
 {

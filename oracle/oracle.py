@@ -278,3 +278,131 @@ def ref_u01_sequence(scheme, kind, key, n):
     idx = ctypes.c_uint()
     REF.ref_u01_sequence(scheme, kind, k, _sz(n), P(out), ctr, ctypes.byref(idx))
     return out[:n]
+
+
+class ResampleIndex:
+    """Restatement of ResampleIndex<IntType> (algorithm/resample.hpp:597-832)
+    on numpy uint32 vectors, including its bookkeeping: reset() keeps the
+    vectors (:628), clear() frees them and keeps num_iter (:631), push_back
+    writes vector num_iter-1 when it exists and appends otherwise
+    (:634-656), index() starts from the last vector held whatever iter_back
+    is (:675).  Queries the reference guards with runtime_assert raise
+    ValueError; reading a vector clear() has freed (undefined in the
+    reference) raises IndexError.  An entry that does not index the vector
+    it is read against traces to 0xFFFFFFFF (undefined in the reference)."""
+
+    BAD = 0xFFFFFFFF
+
+    def __init__(self):
+        self.num_iter = 0
+        self.vec = []
+
+    def size(self, iter=None):
+        if iter is None:                                   # :609-615
+            return len(self.vec[-1]) if self.vec else 0
+        return len(self.vec[iter]) if iter < len(self.vec) else 0  # :618-625
+
+    def reset(self):
+        self.num_iter = 0
+
+    def clear(self):
+        self.vec = []
+
+    def push_back(self, N, idx=None):
+        self.num_iter += 1
+        v = np.arange(N, dtype=np.uint32) if idx is None else \
+            np.array(idx[:N], dtype=np.uint32)
+        if len(self.vec) < self.num_iter:
+            self.vec.append(v)
+        else:
+            self.vec[self.num_iter - 1] = v
+
+    @staticmethod
+    def _gather(v, a):
+        a = np.asarray(a, dtype=np.uint64)
+        ok = a < len(v)
+        out = np.full(a.shape, ResampleIndex.BAD, dtype=np.uint32)
+        out[ok] = v[a[ok]]
+        return out
+
+    def _check(self, iter_back, iter):
+        if not (iter <= iter_back < self.num_iter):
+            raise ValueError("iteration numbers out of range")
+
+    def index(self, ids, iter_back=None, iter=0):
+        """index(id, iter_back, iter) (:669-681) for an array of ids."""
+        iter_back = self.num_iter - 1 if iter_back is None else iter_back
+        self._check(iter_back, iter)
+        if not self.vec or (iter_back > iter and iter_back - 1 >= len(self.vec)):
+            raise IndexError("vector released by clear()")
+        a = self._gather(self.vec[-1], ids)
+        while iter_back > iter:
+            iter_back -= 1
+            a = self._gather(self.vec[iter_back], a)
+        return a
+
+    def shape(self, iter_back=None, iter=0):
+        iter_back = self.num_iter - 1 if iter_back is None else iter_back
+        self._check(iter_back, iter)
+        if iter_back >= len(self.vec):
+            raise IndexError("vector released by clear()")
+        return len(self.vec[iter_back]), iter_back - iter + 1   # :683-715
+
+    def index_matrix(self, layout, iter_back=None, iter=0):
+        """read_index_matrix (:741-810): column R-1 is vector iter_back, column
+        c is vector iter_back-(R-1-c) gathered at column c+1; returned flat
+        in the requested layout (101 row-major, 102 column-major)."""
+        iter_back = self.num_iter - 1 if iter_back is None else iter_back
+        N, R = self.shape(iter_back, iter)
+        cols = [None] * R
+        cols[R - 1] = self.vec[iter_back].copy()
+        for r in range(1, R):
+            cols[R - 1 - r] = self._gather(self.vec[iter_back - r],
+                                           cols[R - r])
+        m = np.stack(cols, axis=1) if R else np.zeros((N, 0), np.uint32)
+        return (m if layout == 101 else m.T).reshape(-1)
+
+
+def ref_resample_index(script):
+    """Run a ResampleIndex script through the compiled reference.  script:
+    list of ("push", N, idx-or-None) / ("reset",) / ("clear",) /
+    ("index", ids, iter_back, iter) / ("matrix", layout, iter_back, iter) /
+    ("size", iter-or-None) / ("num_iter",).  Returns the query results."""
+    out = []
+    REF.ref_ri_new()
+    for op in script:
+        if op[0] == "push":
+            N, idx = op[1], op[2]
+            if idx is None:
+                REF.ref_ri_push(_sz(N), None)
+            else:
+                a = np.ascontiguousarray(idx[:N], np.uint32)
+                REF.ref_ri_push(_sz(N), P(a))
+        elif op[0] == "reset":
+            REF.ref_ri_reset()
+        elif op[0] == "clear":
+            REF.ref_ri_clear()
+        elif op[0] == "num_iter":
+            out.append(REF.ref_ri_size(ctypes.c_int(-2), _sz(0)))
+        elif op[0] == "size":
+            it = op[1]
+            out.append(REF.ref_ri_size(ctypes.c_int(-1 if it is None else 0),
+                                       _sz(0 if it is None else it)))
+        elif op[0] == "index":
+            ids = np.ascontiguousarray(op[1], np.uint32)
+            r = np.zeros(max(len(ids), 1), np.uint32)
+            REF.ref_ri_index(_sz(len(ids)), P(ids), _sz(op[2]), _sz(op[3]),
+                             P(r))
+            out.append(r[:len(ids)])
+        elif op[0] == "matrix":
+            nr = REF.ref_ri_nrow(_sz(op[2]))
+            nc = op[2] - op[3] + 1
+            r = np.zeros(max(nr * nc, 1), np.uint32)
+            REF.ref_ri_matrix(op[1], _sz(op[2]), _sz(op[3]), P(r))
+            out.append(r[:nr * nc])
+    return out
+
+
+if REF is not None:
+    REF.ref_ri_size.restype = ctypes.c_size_t
+    REF.ref_ri_nrow.restype = ctypes.c_size_t

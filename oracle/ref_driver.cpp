@@ -311,6 +311,67 @@ int ref_rep_index(
     return 0;
 }
 
+// ResampleIndex<std::uint32_t> (algorithm/resample.hpp:597-832) driven op
+// by op; callers pass only arguments the reference's runtime_asserts accept.
+static mckl::ResampleIndex<std::uint32_t> *g_ri = nullptr;
+
+int ref_ri_new()
+{
+    delete g_ri;
+    g_ri = new mckl::ResampleIndex<std::uint32_t>();
+    return 0;
+}
+
+int ref_ri_push(std::size_t N, const std::uint32_t *idx)
+{
+    if (idx)
+        g_ri->push_back(N, idx);
+    else
+        g_ri->push_back(N);
+    return 0;
+}
+
+int ref_ri_reset()
+{
+    g_ri->reset();
+    return 0;
+}
+
+int ref_ri_clear()
+{
+    g_ri->clear();
+    return 0;
+}
+
+// which = -2: num_iter(); -1: size(); 0: size(iter)
+std::size_t ref_ri_size(int which, std::size_t iter)
+{
+    if (which == -2)
+        return g_ri->num_iter();
+    return which == -1 ? g_ri->size() : g_ri->size(iter);
+}
+
+std::size_t ref_ri_nrow(std::size_t iter_back)
+{
+    return g_ri->index_matrix_nrow(iter_back);
+}
+
+int ref_ri_index(std::size_t n, const std::uint32_t *id,
+    std::size_t iter_back, std::size_t iter, std::uint32_t *out)
+{
+    for (std::size_t j = 0; j != n; ++j)
+        out[j] = g_ri->index(id[j], iter_back, iter);
+    return 0;
+}
+
+int ref_ri_matrix(
+    int layout, std::size_t iter_back, std::size_t iter, std::uint32_t *out)
+{
+    g_ri->read_index_matrix(
+        layout == 101 ? mckl::RowMajor : mckl::ColMajor, out, iter_back, iter);
+    return 0;
+}
+
 // CPU baseline: time reps of `rand(Philox4x32, n, u32*)` followed by
 // rand(U01OO<double>, n) on the same stream (fresh engine each rep), single
 // thread.  Returns seconds per rep (median).

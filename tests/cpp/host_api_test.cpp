@@ -158,6 +158,58 @@ int main(int argc, char **argv)
         const double ll = sampler.log_likelihood();
         CHECK(std::isfinite(ll) && ll < 0, "SMCSamplerHIP log likelihood");
     }
+    // 7. ResampleIndex trace-back equals the direct copy of the genealogy
+    //    (example/algorithm/include/algorithm_resample_index.hpp:37-70,
+    //    :108-135), both layouts
+    {
+        const std::size_t N = 1000, dim = 70;
+        mckl::Philox4x32 rng(9);
+        std::vector<std::vector<std::uint32_t>> index(dim);
+        std::vector<std::uint32_t> draws(N * dim);
+        {
+            Dev<std::uint32_t> d(N * dim);
+            mckl::rand(rng, N * dim, d.p);
+            draws = d.host(N * dim);
+        }
+        mckl::ResampleIndex ri;
+        Dev<std::uint32_t> di(N);
+        for (std::size_t d = 0; d != dim; ++d) {
+            index[d].resize(N);
+            for (std::size_t i = 0; i != N; ++i)
+                index[d][i] = draws[d * N + i] % N;
+            (void)hipMemcpy(di.p, index[d].data(), N * 4,
+                hipMemcpyHostToDevice);
+            ri.push_back(N, di.p);
+        }
+        // direct copy: column d holds the value d*N + i, then every column
+        // up to d is gathered through index[d]
+        std::vector<std::uint32_t> state(N * dim), tmp(N * dim);
+        for (std::size_t d = 0; d != dim; ++d) {
+            for (std::size_t i = 0; i != N; ++i)
+                state[i * dim + d] = static_cast<std::uint32_t>(d * N + i);
+            tmp = state;
+            for (std::size_t i = 0; i != N; ++i)
+                for (std::size_t j = 0; j <= d; ++j)
+                    state[i * dim + j] = tmp[index[d][i] * dim + j];
+        }
+        bool ok = ri.num_iter() == dim && ri.index_matrix_nrow() == N &&
+            ri.index_matrix_ncol() == dim;
+        for (int pass = 0; pass != 2; ++pass) {
+            const auto layout = pass == 0 ? mckl::RowMajor : mckl::ColMajor;
+            Dev<std::uint32_t> m(N * dim);
+            std::uint32_t *end = ri.read_index_matrix(layout, m.p);
+            std::vector<std::uint32_t> h = m.host(N * dim);
+            ok = ok && end == m.p + N * dim;
+            for (std::size_t i = 0; i != N; ++i)
+                for (std::size_t j = 0; j != dim; ++j) {
+                    const std::uint32_t e =
+                        pass == 0 ? h[i * dim + j] : h[j * N + i];
+                    ok = ok && state[i * dim + j] == j * N + e;
+                }
+        }
+        ok = ok && ri.index(17) == state[17 * dim] % N;
+        CHECK(ok, "ResampleIndex trace == direct copy");
+    }
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;
 }

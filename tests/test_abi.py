@@ -86,3 +86,41 @@ def test_null_and_invalid_arguments():
     # n == 0 is a no-op, output may be NULL
     assert m.lib.mckl_hip_rand(ctypes.byref(e), 0, None, None) == 0
     assert e.index == 4
+
+
+def test_resample_index_bookkeeping_host_only():
+    """ResampleIndex bookkeeping and its range checks (resample.hpp:606-715)
+    with empty vectors only, so no device memory is touched."""
+    import mckl_amd as m
+    L = m.lib
+    h = ctypes.c_void_p()
+    m.check(L.mckl_hip_resample_index_create(ctypes.byref(h)))
+    n, r, c = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+    SMAX = 2 ** 64 - 1
+    for _ in range(3):
+        m.check(L.mckl_hip_resample_index_push_back(h, 0, None, None))
+    m.check(L.mckl_hip_resample_index_num_iter(h, ctypes.byref(n)))
+    assert n.value == 3
+    m.check(L.mckl_hip_resample_index_size(h, SMAX, ctypes.byref(n)))
+    assert n.value == 0
+    m.check(L.mckl_hip_resample_index_matrix_shape(h, 2, 1, ctypes.byref(r),
+                                                   ctypes.byref(c)))
+    assert (r.value, c.value) == (0, 2)
+    E = m._lib.EINVAL
+    assert L.mckl_hip_resample_index_matrix_shape(
+        h, 3, 0, ctypes.byref(r), ctypes.byref(c)) == E
+    assert b"out of range" in L.mckl_hip_last_error()
+    assert L.mckl_hip_resample_index_index(h, 0, None, 1, 2, None, None) == E
+    assert L.mckl_hip_resample_index_read_matrix(h, 5, 2, 0, None, None) == E
+    # n == 0 ids and N == 0 rows are no-ops
+    assert L.mckl_hip_resample_index_index(h, 0, None, 2, 0, None, None) == 0
+    assert L.mckl_hip_resample_index_read_matrix(h, 101, 2, 0, None,
+                                                 None) == 0
+    m.check(L.mckl_hip_resample_index_reset(h))
+    m.check(L.mckl_hip_resample_index_num_iter(h, ctypes.byref(n)))
+    assert n.value == 0
+    m.check(L.mckl_hip_resample_index_size(h, 1, ctypes.byref(n)))
+    assert n.value == 0
+    m.check(L.mckl_hip_resample_index_clear(h))
+    m.check(L.mckl_hip_resample_index_push_back(h, 0, None, None))
+    m.check(L.mckl_hip_resample_index_destroy(h))
