@@ -48,6 +48,10 @@ extern "C" {
 #define MCKL_HIP_RESIDUAL_STRATIFIED 4
 #define MCKL_HIP_RESIDUAL_SYSTEMATIC 5
 
+/* matrix layouts: MatrixLayout, internal/defines.hpp:46 */
+#define MCKL_HIP_ROW_MAJOR 101
+#define MCKL_HIP_COL_MAJOR 102
+
 /* models of the built-in SMC samplers */
 #define MCKL_HIP_MODEL_LG 1 /* x' = a x + sx e, y = x + sy n, x0 ~ N(0,1) */
 #define MCKL_HIP_MODEL_SV 2 /* x' = phi x + s e, y = beta exp(x/2) n */
@@ -160,6 +164,15 @@ int mckl_hip_rand_normal_f32(mckl_hip_engine *e, size_t n, float mean,
 int mckl_hip_rand_normal_f64(mckl_hip_engine *e, size_t n, double mean,
     double sd, double *d_r, void *stream);
 
+/* rand(rng, UniformRealDistribution<T>(a, b), n, r) for 32-bit engines:
+ * random/uniform_real_distribution.hpp:48-58 (U01CO, then r*(b-a)+a rounded
+ * twice, the reference's muladd without FMA, math/vmf.hpp:697-722); a < b
+ * (:43-46) or MCKL_HIP_EINVAL. */
+int mckl_hip_rand_uniform_real_f32(mckl_hip_engine *e, size_t n, float a,
+    float b, float *d_r, void *stream);
+int mckl_hip_rand_uniform_real_f64(mckl_hip_engine *e, size_t n, double a,
+    double b, double *d_r, void *stream);
+
 /* The same fills for `nstreams` independent engines in one launch: engine s
  * writes n values to d_r + s*n and is advanced as if it had made the
  * single-engine call.  (No reference counterpart: the reference loops over
@@ -201,6 +214,17 @@ int mckl_hip_resample_trans_rep_index(size_t N, size_t M,
  * ResampleEval::operator() resample.hpp:494-507 */
 int mckl_hip_resample(int scheme, size_t N, size_t M, mckl_hip_engine *e,
     const double *d_w, uint32_t *d_rep, uint32_t *d_idx, void *stream);
+/* SMCEstimator::estimate's weighted sums (algorithm/smc.hpp:89-118, a dgemv
+ * in the reference): d_r[j] = sum_i d_w[i] u(i, j), j < dim, with u(i, j) =
+ * d_u[i*dim + j] (MCKL_HIP_ROW_MAJOR) or d_u[j*n + i] (MCKL_HIP_COL_MAJOR);
+ * the f32 form converts u to double first, as SMCEstimator<T, float> does
+ * (smc.hpp:128-135).  Sums are double-double, so d_r[j] is the exact sum
+ * rounded once, to ~2^-104 relative to sum |w u|.  d_r is device memory. */
+int mckl_hip_estimate_f64(size_t n, size_t dim, int layout, const double *d_u,
+    const double *d_w, double *d_r, void *stream);
+int mckl_hip_estimate_f32(size_t n, size_t dim, int layout, const float *d_u,
+    const double *d_w, double *d_r, void *stream);
+
 /* StateMatrix<RowMajor,dim,float>::select: core/state_matrix.hpp:193-265
  * (out of place: d_dst[j] = d_src[d_idx[j]], rows of `dim` floats) */
 int mckl_hip_select_f32(size_t M, size_t dim, const uint32_t *d_idx,
@@ -216,11 +240,8 @@ int mckl_hip_select_f32(size_t M, size_t dim, const uint32_t *d_idx,
  * Queries the reference guards with runtime_assert return MCKL_HIP_EINVAL,
  * and so do queries that would read a vector clear() has freed.  An entry
  * that does not index the vector it is read against traces to 0xFFFFFFFF.
- * Calls on one handle are not thread-safe.  Layouts: MatrixLayout,
- * internal/defines.hpp:46.
+ * Calls on one handle are not thread-safe.
  */
-#define MCKL_HIP_ROW_MAJOR 101
-#define MCKL_HIP_COL_MAJOR 102
 typedef struct mckl_hip_resample_index mckl_hip_resample_index;
 int mckl_hip_resample_index_create(mckl_hip_resample_index **out);
 int mckl_hip_resample_index_destroy(mckl_hip_resample_index *h);
@@ -268,6 +289,14 @@ int mckl_hip_smc_run(mckl_hip_smc *s, void *stream);
 int mckl_hip_smc_result(mckl_hip_smc *s, double *h_loglik, double *h_ess,
     int32_t *h_resampled);
 int mckl_hip_smc_destroy(mckl_hip_smc *s);
+/* Selection estimator (SMCEstimator at step 0, algorithm/smc.hpp:89-118,
+ * :389-394) of the moments x^1..x^k, 0 <= k <= 4 (0 removes it): after the
+ * move and weighting of each step, est[t][j-1] = sum_i W_i x_i^j with the
+ * normalised weights, computed in double-double from the FP32 states and
+ * weights of the run.  Call before run(); estimates() returns T x k values
+ * (row-major, one row per step) and synchronises. */
+int mckl_hip_smc_estimate_moments(mckl_hip_smc *s, int k);
+int mckl_hip_smc_estimates(mckl_hip_smc *s, double *h_est);
 
 /*
  * Batched SMC: nfilters independent filters of uneven sizes in one

@@ -18,7 +18,7 @@ __all__ = [
     "Philox4x32", "Philox4x32_64", "Threefry4x64", "Threefry4x64_64", "ARS",
     "ARS_64", "rand", "U01CCDistribution", "U01CODistribution",
     "U01OCDistribution", "U01OODistribution", "U01CanonicalDistribution",
-    "NormalDistribution", "MCKLError",
+    "NormalDistribution", "UniformRealDistribution", "MCKLError",
 ]
 
 
@@ -219,6 +219,31 @@ class NormalDistribution:
                 _stream()))
 
 
+class UniformRealDistribution:
+    """UniformRealDistribution<RealType> (random/uniform_real_distribution.hpp:
+    65-95): U01CO scaled to [a, b) as r * (b - a) + a, two roundings."""
+
+    def __init__(self, a=0.0, b=1.0, dtype="float64"):
+        if not a < b:
+            raise MCKLError("**UniformRealDistribution** constructed with "
+                            "invalid arguments")
+        self.a = a
+        self.b = b
+        self.dtype = np.dtype(dtype)
+
+    def min(self):
+        return self.a
+
+    def max(self):
+        return self.b
+
+    def fill(self, rng, n, r):
+        fn = (lib.mckl_hip_rand_uniform_real_f64 if self.dtype == np.float64
+              else lib.mckl_hip_rand_uniform_real_f32)
+        check(fn(ctypes.byref(rng.state), n, self.a, self.b, _ptr(r),
+                 _stream()))
+
+
 def rand(rng, *args):
     """rand(rng, n, r) or rand(rng, dist, n, r); r is a device tensor.
 
@@ -355,6 +380,24 @@ class ResampleResidualStratified(_ResampleAlgorithm):
 
 class ResampleResidualSystematic(_ResampleAlgorithm):
     SCHEME = RESIDUAL_SYSTEMATIC
+
+
+def estimate(u, w, dim, layout=None, out=None):
+    """SMCEstimator::estimate's weighted sums (algorithm/smc.hpp:89-118):
+    out[j] = sum_i w[i] u(i, j) for u holding n x dim values (float64 or
+    float32) in RowMajor (u[i*dim + j], the default) or ColMajor layout;
+    w float64.  Returns a float64 device tensor of dim values."""
+    import torch
+    layout = _lib.ROW_MAJOR if layout is None else layout
+    n = w.numel()
+    if u.numel() < n * dim:
+        raise MCKLError("estimate: u holds fewer than n * dim values")
+    if out is None:
+        out = torch.empty(max(dim, 1), dtype=torch.float64, device=w.device)
+    fn = (lib.mckl_hip_estimate_f64 if u.dtype == torch.float64
+          else lib.mckl_hip_estimate_f32)
+    check(fn(n, dim, layout, _ptr(u), _ptr(w), _ptr(out), _stream()))
+    return out[:dim]
 
 
 def select(idx, src, dst, dim=1):
@@ -504,6 +547,19 @@ class SMCSampler:
 
     def log_likelihood(self):
         return self.results()[0]
+
+    def estimate_moments(self, k):
+        """Selection estimator of x^1..x^k (SMCEstimator at step 0,
+        algorithm/smc.hpp:89-118); k = 0 removes it.  Call before iterate()."""
+        check(lib.mckl_hip_smc_estimate_moments(self._h, int(k)))
+        self._k = int(k)
+
+    def estimates(self):
+        """(T, k) array: row t holds sum_i W_i x_i^j, j = 1..k, of step t."""
+        k = getattr(self, "_k", 0)
+        out = np.zeros((self.T, max(k, 1)), np.float64)
+        check(lib.mckl_hip_smc_estimates(self._h, _dptr(out)))
+        return out[:, :k]
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -77,6 +77,8 @@ SIGNATURES = {
     "mckl_hip_rand_u01_f64": (_int, [_EP, _int, _sz, _vp, _vp]),
     "mckl_hip_rand_normal_f32": (_int, [_EP, _sz, _flt, _flt, _vp, _vp]),
     "mckl_hip_rand_normal_f64": (_int, [_EP, _sz, _dbl, _dbl, _vp, _vp]),
+    "mckl_hip_rand_uniform_real_f32": (_int, [_EP, _sz, _flt, _flt, _vp, _vp]),
+    "mckl_hip_rand_uniform_real_f64": (_int, [_EP, _sz, _dbl, _dbl, _vp, _vp]),
     "mckl_hip_rand_u01_f64_streams": (_int, [_EP, _sz, _int, _sz, _vp, _vp]),
     "mckl_hip_rand_normal_f32_streams": (
         _int, [_EP, _sz, _sz, _flt, _flt, _vp, _vp]),
@@ -110,6 +112,10 @@ SIGNATURES = {
         _int, [_vp, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl),
                ctypes.POINTER(ctypes.c_int32)]),
     "mckl_hip_smc_destroy": (_int, [_vp]),
+    "mckl_hip_smc_estimate_moments": (_int, [_vp, _int]),
+    "mckl_hip_smc_estimates": (_int, [_vp, ctypes.POINTER(_dbl)]),
+    "mckl_hip_estimate_f64": (_int, [_sz, _sz, _int, _vp, _vp, _vp, _vp]),
+    "mckl_hip_estimate_f32": (_int, [_sz, _sz, _int, _vp, _vp, _vp, _vp]),
     "mckl_hip_smc_batch_create": (
         _int, [_int, _sz, _U32P, ctypes.POINTER(_dbl), _sz,
                ctypes.POINTER(_dbl), _U64P, _dbl, _int, ctypes.POINTER(_vp)]),

@@ -705,6 +705,116 @@ int residual_rep(int base, size_t N, size_t M, mckl_hip_engine *e,
     return MCKL_HIP_OK;
 }
 
+// ---- SMCEstimator::estimate weighted sums (algorithm/smc.hpp:89-118) ----
+// r_j = sum_i w_i u(i, j).  Each product is split exactly (two_prod) and
+// summed in double-double, per tile of 2048 rows and then over tiles in a
+// fixed order, so r_j is the exactly rounded sum up to ~2^-104 relative to
+// sum |w_i u(i, j)|, whatever the grid.
+constexpr int kEstCols = 8; // columns per workgroup
+
+template <typename U>
+__global__ __launch_bounds__(kB) void k_est_tiles(uint64_t n, uint32_t dim,
+    int row_major, const U *__restrict__ u, const double *__restrict__ w,
+    double *phi, double *plo)
+{
+    __shared__ double sm[2 * kEstCols * (kB / 64)];
+    const uint32_t j0 = blockIdx.y * kEstCols;
+    const uint32_t nj = min((uint32_t)kEstCols, dim - j0);
+    dd acc[kEstCols];
+#pragma unroll
+    for (int j = 0; j < kEstCols; ++j)
+        acc[j] = dd{0.0, 0.0};
+    const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x;
+#pragma unroll 2
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + (uint64_t)k * kB;
+        if (i >= n)
+            break;
+        const double wi = w[i];
+#pragma unroll
+        for (int j = 0; j < kEstCols; ++j) {
+            if ((uint32_t)j < nj) {
+                const double v = row_major ? (double)u[i * dim + j0 + j]
+                                           : (double)u[(uint64_t)(j0 + j) * n + i];
+                acc[j] = dd_add(acc[j], two_prod(wi, v));
+            }
+        }
+    }
+    const int wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kEstCols; ++j) {
+        const dd t = wave_sum_dd(acc[j]);
+        if ((threadIdx.x & 63) == 0) {
+            sm[2 * (j * (kB / 64) + wv)] = t.hi;
+            sm[2 * (j * (kB / 64) + wv) + 1] = t.lo;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nj) {
+        const int j = threadIdx.x;
+        dd t{0.0, 0.0};
+        for (int q = 0; q < kB / 64; ++q)
+            t = dd_add(t, dd{sm[2 * (j * (kB / 64) + q)],
+                               sm[2 * (j * (kB / 64) + q) + 1]});
+        const uint64_t o = (uint64_t)(j0 + j) * gridDim.x + blockIdx.x;
+        phi[o] = t.hi;
+        plo[o] = t.lo;
+    }
+}
+
+// one workgroup per column: dd sum of the column's tile partials in tile
+// order (chunked per thread, then across the group)
+__global__ __launch_bounds__(kB) void k_est_final(uint32_t nt,
+    const double *phi, const double *plo, double *r)
+{
+    __shared__ double sm[2 * (kB / 64)];
+    const uint64_t base = (uint64_t)blockIdx.x * nt;
+    dd t{0.0, 0.0};
+    for (uint32_t q = threadIdx.x; q < nt; q += kB)
+        t = dd_add(t, dd{phi[base + q], plo[base + q]});
+    t = wave_sum_dd(t);
+    if ((threadIdx.x & 63) == 0) {
+        sm[2 * (threadIdx.x >> 6)] = t.hi;
+        sm[2 * (threadIdx.x >> 6) + 1] = t.lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        dd s{0.0, 0.0};
+        for (int q = 0; q < kB / 64; ++q)
+            s = dd_add(s, dd{sm[2 * q], sm[2 * q + 1]});
+        r[blockIdx.x] = __dadd_rn(s.hi, s.lo);
+    }
+}
+
+template <typename U>
+int estimate_impl(size_t n, size_t dim, int layout, const U *u,
+    const double *w, double *r, hipStream_t st)
+{
+    if (layout != MCKL_HIP_ROW_MAJOR && layout != MCKL_HIP_COL_MAJOR)
+        return set_error(MCKL_HIP_EINVAL, "unknown matrix layout %d", layout);
+    if (dim == 0)
+        return MCKL_HIP_OK;
+    if (!r || (n && (!u || !w)))
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    if (dim > 65535u * kEstCols || n > ((uint64_t)1 << 40))
+        return set_error(MCKL_HIP_EINVAL, "estimate: size out of range");
+    const uint32_t nt = (uint32_t)std::max<uint64_t>(1, ceil_div(n, kTile));
+    Scratch s(st);
+    const size_t half = align256(nt * dim * sizeof(double));
+    int rc = s.alloc(2 * half);
+    if (rc)
+        return rc;
+    double *phi = static_cast<double *>(s.p);
+    double *plo = reinterpret_cast<double *>(static_cast<char *>(s.p) + half);
+    const dim3 grid(nt, (uint32_t)ceil_div(dim, kEstCols));
+    k_est_tiles<U><<<grid, kB, 0, st>>>(
+        n, (uint32_t)dim, layout == MCKL_HIP_ROW_MAJOR, u, w, phi, plo);
+    MCKL_HIP_CHECK_LAUNCH("k_est_tiles");
+    k_est_final<<<(uint32_t)dim, kB, 0, st>>>(nt, phi, plo, r);
+    MCKL_HIP_CHECK_LAUNCH("k_est_final");
+    return MCKL_HIP_OK;
+}
+
 } // namespace
 } // namespace mckl_hip
 
@@ -814,6 +924,18 @@ int mckl_hip_resample(int scheme, size_t N, size_t M, mckl_hip_engine *e,
     if (d_idx)
         rc = rep_index(N, M, d_rep, d_idx, st);
     return rc;
+}
+
+int mckl_hip_estimate_f64(size_t n, size_t dim, int layout, const double *d_u,
+    const double *d_w, double *d_r, void *stream)
+{
+    return estimate_impl(n, dim, layout, d_u, d_w, d_r, as_stream(stream));
+}
+
+int mckl_hip_estimate_f32(size_t n, size_t dim, int layout, const float *d_u,
+    const double *d_w, double *d_r, void *stream)
+{
+    return estimate_impl(n, dim, layout, d_u, d_w, d_r, as_stream(stream));
 }
 
 int mckl_hip_select_f32(size_t M, size_t dim, const uint32_t *d_idx,

@@ -142,6 +142,26 @@ struct ConvU01F32 {
         return u01_f32(iv, w[0]);
     }
 };
+// UniformReal: U01CO then muladd(u, b - a, a) without FMA, two roundings
+// (random/uniform_real_distribution.hpp:48-58, math/vmf.hpp:697-722)
+struct ConvUniformF64 {
+    using T = double;
+    static constexpr int Q = 1;
+    double a, d; // d = b - a
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return __dadd_rn(__dmul_rn(u01_f64(MCKL_HIP_U01_CO, w[0]), d), a);
+    }
+};
+struct ConvUniformF32 {
+    using T = float;
+    static constexpr int Q = 1;
+    float a, d;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return __fadd_rn(__fmul_rn(u01_f32(MCKL_HIP_U01_CO, w[0]), d), a);
+    }
+};
 struct ConvCanF64 {
     using T = double;
     static constexpr int Q = 2;
@@ -1127,6 +1147,36 @@ int mckl_hip_rand_u01_f32(
         return element_fill(e, 1, n, d_r, ConvCanF32(), 2048, as_stream(stream));
     ConvU01F32 c;
     c.iv = interval;
+    return element_fill(e, 1, n, d_r, c, 2048, as_stream(stream));
+}
+
+int mckl_hip_rand_uniform_real_f64(mckl_hip_engine *e, size_t n, double a,
+    double b, double *d_r, void *stream)
+{
+    if (!(a < b))
+        return set_error(MCKL_HIP_EINVAL,
+            "**UniformRealDistribution** constructed with invalid arguments");
+    if (e && e->rbits != 32)
+        return set_error(MCKL_HIP_EINVAL,
+            "UniformReal fills are provided for 32-bit engines");
+    ConvUniformF64 c;
+    c.a = a;
+    c.d = b - a;
+    return element_fill(e, 1, n, d_r, c, 1024, as_stream(stream));
+}
+
+int mckl_hip_rand_uniform_real_f32(mckl_hip_engine *e, size_t n, float a,
+    float b, float *d_r, void *stream)
+{
+    if (!(a < b))
+        return set_error(MCKL_HIP_EINVAL,
+            "**UniformRealDistribution** constructed with invalid arguments");
+    if (e && e->rbits != 32)
+        return set_error(MCKL_HIP_EINVAL,
+            "UniformReal fills are provided for 32-bit engines");
+    ConvUniformF32 c;
+    c.a = a;
+    c.d = b - a;
     return element_fill(e, 1, n, d_r, c, 2048, as_stream(stream));
 }
 

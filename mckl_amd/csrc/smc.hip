@@ -442,6 +442,99 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
     }
 }
 
+// Selection estimator of the moments x^1..x^K (SMCEstimator at step 0,
+// algorithm/smc.hpp:89-118 and :389-394): after the move and weighting of
+// step t, per tile the double-double sums of e_i and of e_i x_i^j with
+// e_i = exp_f(lw_i - m) exactly as the weight sums take it; products split
+// exactly (two_prod).  Partials: [(t-1) (K+1) + q][tile].
+constexpr int kMaxMoments = 4;
+
+__global__ __launch_bounds__(kGroup) void k_est_moments(SmcArgs A, uint32_t t,
+    const float *x, const float *lw, const float *tmax, int K, double *eph,
+    double *epl)
+{
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    const GroupLds L = group_lds(lds);
+    const int gt = threadIdx.x;
+    float l[kItems], xv[kItems];
+    load8(lw, tile_first(blockIdx.x, gt), A.N, l, 0.0f);
+    load8(x, tile_first(blockIdx.x, gt), A.N, xv, 0.0f);
+    const float m = tiles_max(tmax, A.ntiles, gt, L.f);
+    dd acc[kMaxMoments + 1];
+#pragma unroll
+    for (int q = 0; q <= kMaxMoments; ++q)
+        acc[q] = dd{0.0, 0.0};
+    const uint64_t i0 = tile_first(blockIdx.x, gt);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (i0 + k < A.N) {
+            const double e = (double)smc_exp_f(__fsub_rn(l[k], m));
+            acc[0] = dd_add(acc[0], dd{e, 0.0});
+            const double xd = (double)xv[k];
+            double p = 1.0;
+#pragma unroll
+            for (int q = 1; q <= kMaxMoments; ++q) {
+                if (q <= K) {
+                    p = __dmul_rn(p, xd);
+                    acc[q] = dd_add(acc[q], two_prod(e, p));
+                }
+            }
+        }
+    }
+    double *sm = L.d; // 2 * (K+1) * 4 doubles
+    const int wv = gt >> 6;
+#pragma unroll
+    for (int q = 0; q <= kMaxMoments; ++q) {
+        if (q <= K) {
+            const dd s = wave_sum_dd(acc[q]);
+            if ((gt & 63) == 0) {
+                sm[2 * (q * kWaves + wv)] = s.hi;
+                sm[2 * (q * kWaves + wv) + 1] = s.lo;
+            }
+        }
+    }
+    __syncthreads();
+    if (gt <= K) {
+        dd s{0.0, 0.0};
+        for (int w = 0; w < kWaves; ++w)
+            s = dd_add(s, dd{sm[2 * (gt * kWaves + w)],
+                             sm[2 * (gt * kWaves + w) + 1]});
+        const uint64_t o =
+            ((uint64_t)(t - 1) * (K + 1) + gt) * A.ntiles + blockIdx.x;
+        eph[o] = s.hi;
+        epl[o] = s.lo;
+    }
+}
+
+// one workgroup per step: totals over tiles, est[t][j-1] = sum e x^j / sum e
+__global__ __launch_bounds__(kGroup) void k_est_moments_final(uint32_t nt,
+    int K, const double *eph, const double *epl, double *est)
+{
+    __shared__ double sm[2 * kWaves];
+    const uint32_t t = blockIdx.x;
+    double tot0 = 1.0;
+    for (int q = 0; q <= K; ++q) {
+        const uint64_t base = ((uint64_t)t * (K + 1) + q) * nt;
+        dd s{0.0, 0.0};
+        for (uint32_t i = threadIdx.x; i < nt; i += kGroup)
+            s = dd_add(s, dd{eph[base + i], epl[base + i]});
+        s = wave_sum_dd(s);
+        if ((threadIdx.x & 63) == 0) {
+            sm[2 * (threadIdx.x >> 6)] = s.hi;
+            sm[2 * (threadIdx.x >> 6) + 1] = s.lo;
+        }
+        __syncthreads();
+        dd g{0.0, 0.0};
+        for (int w = 0; w < kWaves; ++w)
+            g = dd_add(g, dd{sm[2 * w], sm[2 * w + 1]});
+        __syncthreads();
+        if (q == 0)
+            tot0 = __dadd_rn(g.hi, g.lo);
+        else if (threadIdx.x == 0)
+            est[(uint64_t)t * K + q - 1] = dd_div_d(g, tot0);
+    }
+}
+
 __global__ void k_zero_u32(uint32_t *p, uint32_t n)
 {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
@@ -513,19 +606,29 @@ struct mckl_hip_smc {
     SmcState *st = nullptr;
     double *ess = nullptr;
     int32_t *res = nullptr;
+    int kmom = 0;                                  // estimated moments
+    double *eph = nullptr, *epl = nullptr, *est = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
-static void smc_free(mckl_hip_smc *s)
+static void smc_drop_graph(mckl_hip_smc *s)
 {
     if (s->exec)
         (void)hipGraphExecDestroy(s->exec);
     if (s->graph)
         (void)hipGraphDestroy(s->graph);
+    s->exec = nullptr;
+    s->graph = nullptr;
+}
+
+static void smc_free(mckl_hip_smc *s)
+{
+    smc_drop_graph(s);
     void *ptrs[] = {s->x[0], s->x[1], s->lw, s->obs, s->tmax, s->idx, s->ts,
         s->ts2, s->B.tdh, s->B.tdl, s->B.te, s->B.tz, s->B.cnt, s->B.Eincl,
-        s->B.zpos, s->B.tmh, s->B.tml, s->B.u, s->st, s->ess, s->res};
+        s->B.zpos, s->B.tmh, s->B.tml, s->B.u, s->st, s->ess, s->res, s->eph,
+        s->epl, s->est};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -622,6 +725,9 @@ static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
         float *xout = s->x[t & 1];
         k_move<<<grid, kGroup, 0, st>>>(
             s->A, t, xin, xout, s->lw, s->idx, s->st, s->tmax);
+        if (s->kmom)
+            k_est_moments<<<grid, kGroup, 0, st>>>(s->A, t, xout, s->lw,
+                s->tmax, s->kmom, s->eph, s->epl);
         k_sums<<<grid, kGroup, 0, st>>>(
             s->A, s->lw, s->tmax, s->ts, s->ts2, s->B);
         k_rs_counts<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->tmax, s->ts,
@@ -634,7 +740,52 @@ static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
         k_rs_index<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);
         k_rs_scatter<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);
     }
+    if (s->kmom)
+        k_est_moments_final<<<(uint32_t)s->T, kGroup, 0, st>>>(
+            s->A.ntiles, s->kmom, s->eph, s->epl, s->est);
     MCKL_HIP_CHECK_LAUNCH("smc step kernels");
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_smc_estimate_moments(mckl_hip_smc *s, int k)
+{
+    if (!s)
+        return set_error(MCKL_HIP_EINVAL, "sampler is NULL");
+    if (k < 0 || k > kMaxMoments)
+        return set_error(MCKL_HIP_EINVAL,
+            "moments: k = %d outside 0..%d", k, kMaxMoments);
+    MCKL_HIP_TRY(hipDeviceSynchronize());
+    smc_drop_graph(s);
+    for (double **p : {&s->eph, &s->epl, &s->est}) {
+        if (*p)
+            MCKL_HIP_TRY(hipFree(*p));
+        *p = nullptr;
+    }
+    s->kmom = 0;
+    if (k == 0)
+        return MCKL_HIP_OK;
+    const size_t np = s->T * (size_t)(k + 1) * s->A.ntiles;
+    int rc = dev_alloc(&s->eph, np);
+    if (rc == MCKL_HIP_OK)
+        rc = dev_alloc(&s->epl, np);
+    if (rc == MCKL_HIP_OK)
+        rc = dev_alloc(&s->est, s->T * (size_t)k);
+    if (rc)
+        return rc;
+    s->kmom = k;
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_smc_estimates(mckl_hip_smc *s, double *h_est)
+{
+    if (!s || !h_est)
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    if (!s->kmom)
+        return set_error(MCKL_HIP_EINVAL,
+            "no estimator: call mckl_hip_smc_estimate_moments first");
+    MCKL_HIP_TRY(hipDeviceSynchronize());
+    MCKL_HIP_TRY(hipMemcpy(h_est, s->est, s->T * s->kmom * sizeof(double),
+        hipMemcpyDeviceToHost));
     return MCKL_HIP_OK;
 }
 

@@ -182,6 +182,32 @@ __device__ __forceinline__ dd dd_add(dd a, dd b)
     return dd{h, __dsub_rn(lo, __dsub_rn(h, s))};
 }
 
+// exact product a*b = hi + lo (one fused multiply-add for the error term)
+__device__ __forceinline__ dd two_prod(double a, double b)
+{
+    const double p = __dmul_rn(a, b);
+    return dd{p, __fma_rn(a, b, -p)};
+}
+
+// a / b for dd a and double b > 0, to about 2^-104 relative before the
+// final rounding of hi + lo
+__device__ __forceinline__ double dd_div_d(dd a, double b)
+{
+    const double q = __ddiv_rn(a.hi, b);
+    const dd qb = two_prod(q, b);
+    const double r = __dadd_rn(__dsub_rn(__dsub_rn(a.hi, qb.hi), qb.lo), a.lo);
+    return __dadd_rn(q, __ddiv_rn(r, b));
+}
+
+// wave-wide dd sum (every lane gets the total)
+__device__ __forceinline__ dd wave_sum_dd(dd v)
+{
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1)
+        v = dd_add(v, dd{__shfl_xor(v.hi, k, 64), __shfl_xor(v.lo, k, 64)});
+    return v;
+}
+
 __device__ __forceinline__ dd dd_shfl_up(dd v, int d)
 {
     return dd{__shfl_up(v.hi, d, 64), __shfl_up(v.lo, d, 64)};

@@ -298,6 +298,45 @@ class NormalDistribution
     }
 };
 
+/// UniformRealDistribution<RealType> (random/uniform_real_distribution.hpp:
+/// 65-95); a < b, else RuntimeAssert at construction
+template <typename RealType>
+class UniformRealDistribution
+{
+  public:
+    using result_type = RealType;
+    explicit UniformRealDistribution(RealType a = 0, RealType b = 1)
+        : a_(a), b_(b)
+    {
+        if (!(a < b))
+            throw RuntimeAssert("**UniformRealDistribution** constructed with "
+                                "invalid arguments");
+    }
+    RealType a() const { return a_; }
+    RealType b() const { return b_; }
+    RealType min() const { return a_; }
+    RealType max() const { return b_; }
+
+    template <int K, typename R>
+    void operator()(CounterEngineHIP<K, R> &rng, std::size_t n, RealType *d_r)
+    {
+        fill(rng.state(), n, d_r);
+    }
+
+  private:
+    RealType a_, b_;
+    void fill(mckl_hip_engine &e, std::size_t n, float *r)
+    {
+        hip_check(mckl_hip_rand_uniform_real_f32(
+            &e, n, a_, b_, r, current_stream()));
+    }
+    void fill(mckl_hip_engine &e, std::size_t n, double *r)
+    {
+        hip_check(mckl_hip_rand_uniform_real_f64(
+            &e, n, a_, b_, r, current_stream()));
+    }
+};
+
 /// rand(rng, dist, n, r): random/internal/common.hpp:706-712
 template <int K, typename R, typename Dist>
 inline void rand(CounterEngineHIP<K, R> &rng, Dist &dist, std::size_t n,
@@ -559,9 +598,7 @@ class SMCSamplerHIP
     /// all T steps, asynchronous on the current stream
     void iterate()
     {
-        if (!s_)
-            hip_check(mckl_hip_smc_create(model_, theta_.data(), y_.size(),
-                y_.data(), N_, seed_, threshold_, scheme_, &s_));
+        create();
         hip_check(mckl_hip_smc_run(s_, current_stream()));
     }
 
@@ -579,6 +616,23 @@ class SMCSamplerHIP
         return e;
     }
 
+    /// selection estimator of x^1..x^k (SMCEstimator at step 0); set before
+    /// iterate()
+    void estimate_moments(int k)
+    {
+        create();
+        hip_check(mckl_hip_smc_estimate_moments(s_, k));
+        k_ = k;
+    }
+
+    /// T x k estimates, row-major (one row per iteration)
+    std::vector<double> estimates()
+    {
+        std::vector<double> e(y_.size() * static_cast<std::size_t>(k_));
+        hip_check(mckl_hip_smc_estimates(s_, e.data()));
+        return e;
+    }
+
   private:
     int model_;
     std::array<double, 3> theta_;
@@ -587,8 +641,32 @@ class SMCSamplerHIP
     std::uint64_t seed_;
     double threshold_ = 0.5;
     int scheme_ = MCKL_HIP_SYSTEMATIC;
+    int k_ = 0;
     mckl_hip_smc *s_ = nullptr;
+
+    void create()
+    {
+        if (!s_)
+            hip_check(mckl_hip_smc_create(model_, theta_.data(), y_.size(),
+                y_.data(), N_, seed_, threshold_, scheme_, &s_));
+    }
 };
+
+/// SMCEstimator::estimate's weighted sums (algorithm/smc.hpp:89-118) on
+/// device arrays: d_r[j] = sum_i d_w[i] u(i, j)
+inline void estimate(std::size_t n, std::size_t dim, MatrixLayout layout,
+    const double *d_u, const double *d_w, double *d_r)
+{
+    hip_check(mckl_hip_estimate_f64(
+        n, dim, static_cast<int>(layout), d_u, d_w, d_r, current_stream()));
+}
+
+inline void estimate(std::size_t n, std::size_t dim, MatrixLayout layout,
+    const float *d_u, const double *d_w, double *d_r)
+{
+    hip_check(mckl_hip_estimate_f32(
+        n, dim, static_cast<int>(layout), d_u, d_w, d_r, current_stream()));
+}
 
 } // namespace mckl
 

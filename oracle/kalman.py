@@ -75,3 +75,45 @@ def grad(y, a, sx, sy, m0=0.0, v0=1.0):
         dP = -dK * P + (1.0 - K) * dP
         P = (1.0 - K) * P
     return g
+
+
+def filter_moments(y, a, sx, sy, m0=0.0, v0=1.0):
+    """Filtering means E[x_t | y_1..y_t] and variances, t = 1..T, by the
+    predict/update recursion of ``loglik``."""
+    m, P = m0, v0
+    ms, Ps = [], []
+    for yt in y:
+        m = a * m
+        P = a * a * P + sx * sx
+        S = P + sy * sy
+        K = P / S
+        m = m + K * (yt - m)
+        P = (1.0 - K) * P
+        ms.append(m)
+        Ps.append(P)
+    return np.array(ms), np.array(Ps)
+
+
+def filter_moments_dense(y, a, sx, sy, m0=0.0, v0=1.0):
+    """The same moments by conditioning the joint Gaussian of (x_t, y_1..y_t)
+    directly: E = mu_x + c' C^-1 (y - mu_y), Var = var_x - c' C^-1 c."""
+    T = len(y)
+    var = np.empty(T + 1)
+    var[0] = v0
+    for s in range(1, T + 1):
+        var[s] = a * a * var[s - 1] + sx * sx
+
+    def cov(s, t):  # Cov(x_s, x_t)
+        s, t = min(s, t), max(s, t)
+        return a ** (t - s) * var[s]
+
+    ms, Ps = [], []
+    for t in range(1, T + 1):
+        C = np.array([[cov(i, j) for j in range(1, t + 1)]
+                      for i in range(1, t + 1)]) + np.eye(t) * sy * sy
+        c = np.array([cov(t, j) for j in range(1, t + 1)])
+        mu_y = np.array([a ** j * m0 for j in range(1, t + 1)])
+        sol = np.linalg.solve(C, np.asarray(y[:t]) - mu_y)
+        ms.append(a ** t * m0 + c @ sol)
+        Ps.append(var[t] - c @ np.linalg.solve(C, c))
+    return np.array(ms), np.array(Ps)

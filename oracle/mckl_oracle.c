@@ -666,6 +666,31 @@ void mo_rand_u01_f32(mo_engine *e, int interval, size_t n, float *r)
     }
 }
 
+/* UniformReal: random/uniform_real_distribution.hpp:48-58 -- U01CO in
+ * chunks of BufferSize<T>, then muladd(r, b - a, a) without FMA
+ * (math/vmf.hpp:697-722): r * (b - a) rounded, then + a rounded. */
+void mo_rand_uniform_real_f64(mo_engine *e, size_t n, double a, double b,
+    double *r)
+{
+    const double d = b - a;
+    mo_rand_u01_f64(e, MO_CO, n, r);
+    for (size_t i = 0; i < n; ++i) {
+        volatile double t = r[i] * d;
+        r[i] = t + a;
+    }
+}
+
+void mo_rand_uniform_real_f32(mo_engine *e, size_t n, float a, float b,
+    float *r)
+{
+    const float d = b - a;
+    mo_rand_u01_f32(e, MO_CO, n, r);
+    for (size_t i = 0; i < n; ++i) {
+        volatile float t = r[i] * d;
+        r[i] = t + a;
+    }
+}
+
 /* ===================================================================== */
 /* Normal: chunked Box-Muller                                             */
 /* random/normal_distribution.hpp:50-78 (chunk: first half s*sin, second  */
@@ -1188,9 +1213,13 @@ float mo_smc_logg(int model, const double *th, float obs, float x)
     return ((-0.5f * q) - (0.5f * x)) + m.cst;
 }
 
-double mo_smc_run(int model, const double *theta, const double *y, size_t T,
-    size_t N, uint64_t seed, double threshold, int scheme, int nthreads,
-    double *ess_hist, int *resampled_hist)
+/* kmom > 0: the selection estimator of x^1..x^kmom after each step's move
+ * and weighting (SMCEstimator at step 0, algorithm/smc.hpp:89-118,
+ * :389-394): est[(t-1) kmom + j-1] = sum_i W_i x_i^j, W_i = e_i * (1/S) as
+ * the resampling weights below, summed in order (the reference's dgemv). */
+double mo_smc_run_est(int model, const double *theta, const double *y,
+    size_t T, size_t N, uint64_t seed, double threshold, int scheme,
+    int nthreads, double *ess_hist, int *resampled_hist, int kmom, double *est)
 {
     mo_model m;
     mo_model_init(model, theta, &m);
@@ -1254,6 +1283,19 @@ double mo_smc_run(int model, const double *theta, const double *y, size_t T,
             S2 += d * d;
         }
         logz += ((double)mx + log(S)) - log(C);
+        if (kmom > 0) {
+            const double inv = 1.0 / S;
+            for (int j = 1; j <= kmom; ++j) {
+                double acc = 0;
+                for (size_t i = 0; i < N; ++i) {
+                    double p = 1.0;
+                    for (int q = 0; q < j; ++q)
+                        p = p * (double)x[i];
+                    acc += ((double)e[i] * inv) * p;
+                }
+                est[(t - 1) * (size_t)kmom + (size_t)(j - 1)] = acc;
+            }
+        }
         const double ess = S * S / S2;
         if (ess_hist)
             ess_hist[t - 1] = ess;
@@ -1294,4 +1336,12 @@ double mo_smc_run(int model, const double *theta, const double *y, size_t T,
     free(rep);
     free(idx);
     return logz;
+}
+
+double mo_smc_run(int model, const double *theta, const double *y, size_t T,
+    size_t N, uint64_t seed, double threshold, int scheme, int nthreads,
+    double *ess_hist, int *resampled_hist)
+{
+    return mo_smc_run_est(model, theta, y, T, N, seed, threshold, scheme,
+        nthreads, ess_hist, resampled_hist, 0, NULL);
 }

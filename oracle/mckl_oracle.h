@@ -101,6 +101,10 @@ double mo_u01_canonical_f64(uint32_t u0, uint32_t u1);
 float mo_u01_canonical_f32(uint32_t u);
 void mo_rand_u01_f64(mo_engine *e, int interval, size_t n, double *r);
 void mo_rand_u01_f32(mo_engine *e, int interval, size_t n, float *r);
+void mo_rand_uniform_real_f64(mo_engine *e, size_t n, double a, double b,
+    double *r);
+void mo_rand_uniform_real_f32(mo_engine *e, size_t n, float a, float b,
+    float *r);
 
 /* ---- normal ---- */
 void mo_rand_normal_f32(mo_engine *e, size_t n, float mean, float sd, float *r);
@@ -137,6 +141,10 @@ float mo_smc_logg(int model, const double *theta, float obs, float x);
 double mo_smc_run(int model, const double *theta, const double *y, size_t T,
     size_t N, uint64_t seed, double threshold, int scheme, int nthreads,
     double *ess_hist, int *resampled_hist);
+double mo_smc_run_est(int model, const double *theta, const double *y,
+    size_t T, size_t N, uint64_t seed, double threshold, int scheme,
+    int nthreads, double *ess_hist, int *resampled_hist, int kmom,
+    double *est);
 
 #ifdef __cplusplus
 }

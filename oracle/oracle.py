@@ -43,6 +43,10 @@ O.mo_smc_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
 O.mo_smc_run.restype = ctypes.c_double
 O.mo_smc_run.argtypes = [ctypes.c_int, _vp, _vp, _sz, _sz, ctypes.c_uint64,
                          ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp, _vp]
+O.mo_smc_run_est.restype = ctypes.c_double
+O.mo_smc_run_est.argtypes = [ctypes.c_int, _vp, _vp, _sz, _sz, ctypes.c_uint64,
+                             ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp,
+                             _vp, ctypes.c_int, _vp]
 O.mo_next_u32.restype = ctypes.c_uint32
 O.mo_u01_u32_f64.restype = ctypes.c_double
 O.mo_u01_u32_f32.restype = ctypes.c_float
@@ -81,6 +85,29 @@ def rand_u01(e, interval, n, dtype):
     fn = O.mo_rand_u01_f64 if np.dtype(dtype) == np.float64 else O.mo_rand_u01_f32
     fn(ctypes.byref(e), interval, _sz(n), P(out))
     return out[:n]
+
+
+def rand_uniform_real(e, n, a, b, dtype):
+    out = np.zeros(max(n, 1), dtype)
+    if np.dtype(dtype) == np.float64:
+        O.mo_rand_uniform_real_f64(ctypes.byref(e), _sz(n), ctypes.c_double(a),
+                                   ctypes.c_double(b), P(out))
+    else:
+        O.mo_rand_uniform_real_f32(ctypes.byref(e), _sz(n), ctypes.c_float(a),
+                                   ctypes.c_float(b), P(out))
+    return out[:n]
+
+
+def ref_uniform_real(kind, key, dtype, n, a, b, pre=0):
+    out = np.zeros(n + 1, dtype)
+    k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
+    ctr = (ctypes.c_uint64 * 4)()
+    idx = ctypes.c_uint()
+    REF.ref_uniform_real(kind, k, int(np.dtype(dtype) == np.float64), _sz(pre),
+                         _sz(n), ctypes.c_double(a), ctypes.c_double(b), P(out),
+                         ctr, ctypes.byref(idx))
+    nc = 4 if kind == THREEFRY4X64 else 2
+    return out[:n], list(ctr[:nc]), idx.value
 
 
 def rand_normal(e, n, mean, sd, dtype):
@@ -200,14 +227,19 @@ def smc_normal(seed, i, t):
 
 
 def smc_run(model, theta, y, N, seed, threshold=0.5, scheme=SYSTEMATIC,
-            nthreads=1, history=False):
+            nthreads=1, history=False, moments=0):
+    """loglik (and with history=True the ESS and resampling flags per step;
+    with moments=k > 0 also the (T, k) selection estimates of x^1..x^k)."""
     th = np.ascontiguousarray(theta, np.float64)
     y = np.ascontiguousarray(y, np.float64)
     T = len(y)
     ess = np.zeros(T, np.float64)
     res = np.zeros(T, np.int32)
-    ll = O.mo_smc_run(model, P(th), P(y), _sz(T), _sz(N), seed, threshold,
-                      scheme, nthreads, P(ess), P(res))
+    est = np.zeros((T, max(moments, 1)), np.float64)
+    ll = O.mo_smc_run_est(model, P(th), P(y), _sz(T), _sz(N), seed, threshold,
+                          scheme, nthreads, P(ess), P(res), moments, P(est))
+    if moments:
+        return ll, ess, res, est[:, :moments]
     if history:
         return ll, ess, res
     return ll

@@ -18,6 +18,7 @@
 #include <mckl/random/philox.hpp>
 #include <mckl/random/threefry.hpp>
 #include <mckl/random/u01_distribution.hpp>
+#include <mckl/random/uniform_real_distribution.hpp>
 
 #include <chrono>
 #include <cstdint>
@@ -217,6 +218,28 @@ int ref_normal(int kind, const std::uint64_t *key, int dbl, std::size_t pre,
         } else {
             mckl::NormalDistribution<float> d(
                 static_cast<float>(mean), static_cast<float>(sd));
+            mckl::rand(rng, d, n, static_cast<float *>(out));
+        }
+        get_state(rng, ctr, index);
+        return 0;
+    });
+}
+
+// pre raw u32 draws, then rand(rng, UniformRealDistribution<T>(a, b), n, r)
+int ref_uniform_real(int kind, const std::uint64_t *key, int dbl,
+    std::size_t pre, std::size_t n, double a, double b, void *out,
+    std::uint64_t *ctr, unsigned *index)
+{
+    return dispatch(kind, 32, key, [&](auto &rng) {
+        using R = typename std::remove_reference_t<decltype(rng)>::result_type;
+        std::vector<R> skip(pre + 1);
+        mckl::rand(rng, pre, skip.data());
+        if (dbl) {
+            mckl::UniformRealDistribution<double> d(a, b);
+            mckl::rand(rng, d, n, static_cast<double *>(out));
+        } else {
+            mckl::UniformRealDistribution<float> d(
+                static_cast<float>(a), static_cast<float>(b));
             mckl::rand(rng, d, n, static_cast<float *>(out));
         }
         get_state(rng, ctr, index);

@@ -239,6 +239,37 @@ def test_u01_fills(dev, kind, dtype, interval):
         assert rng.ctr() == orc.state(e)[0] and rng.index == orc.state(e)[1]
 
 
+@pytest.mark.parametrize("kind", [1, 2, 3])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_uniform_real_fills(dev, kind, dtype):
+    """rand(rng, UniformRealDistribution<T>(a, b), n, r) bit-exact against
+    the oracle (pinned to the compiled reference), after pre-drawn words."""
+    m = _m()
+    isz = np.dtype(dtype).itemsize
+    for pre, n, a, b in ((0, 5000, 0.0, 1.0), (3, 2049, -2.5, 7.25),
+                         (1, 1, 1e-3, 1e-3 + 1e-9), (0, 40961, -1e6, -3.0),
+                         (5, 0, 0.0, 1.0), (2, 3001, 100.0, 1e8)):
+        key = [pre + 3, 1, 4, 1][:2 if kind == 1 else 4]
+        rng = _cls(kind, 32)(key=key)
+        e = orc.engine(kind, key)
+        if pre:
+            tmp = _tensor(pre, 4, dev)
+            m.rand(rng, pre, tmp)
+            orc.rand_u32(e, pre)
+        out = _tensor(n, isz, dev, fp=True)
+        m.rand(rng, m.UniformRealDistribution(a, b, dtype), n, out)
+        got = out.cpu().numpy()[:n]
+        want = orc.rand_uniform_real(e, n, a, b, dtype)
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), \
+            (pre, n, a, b)
+        assert rng.ctr() == orc.state(e)[0] and rng.index == orc.state(e)[1]
+    with pytest.raises(m.MCKLError):
+        m.UniformRealDistribution(1.0, 1.0)
+    rng = _cls(kind, 32)(key=[1, 2, 3, 4][:2 if kind == 1 else 4])
+    assert m.lib.mckl_hip_rand_uniform_real_f64(
+        ctypes.byref(rng.state), 4, 2.0, 1.0, None, None) == m._lib.EINVAL
+
+
 def test_misaligned_output(dev):
     """Output pointer not 16-byte aligned: element-wise stores."""
     m = _m()

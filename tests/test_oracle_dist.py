@@ -55,6 +55,30 @@ def test_u01_vs_reference(kind, dtype, interval):
         assert orc.state(e) == (rctr, ridx)
 
 
+UNIFORM_CASES = ((0, 5000, 0.0, 1.0), (3, 2049, -2.5, 7.25),
+                 (1, 1, 1e-3, 1e-3 + 1e-9), (0, 4097, -1e6, -3.0),
+                 (7, 0, 0.0, 1.0), (2, 3001, 100.0, 1e8))
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", [orc.PHILOX4X32, orc.THREEFRY4X64, orc.ARS])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_uniform_real_vs_reference(kind, dtype):
+    """UniformRealDistribution (uniform_real_distribution.hpp:48-95) bit-exact
+    against the compiled reference: chunked U01CO, r * (b - a) + a rounded
+    twice, engine state after the call."""
+    for pre, n, a, b in UNIFORM_CASES:
+        ref, rctr, ridx = orc.ref_uniform_real(kind, [6], dtype, n, a, b, pre)
+        e = orc.engine(kind, [6])
+        orc.rand_u32(e, pre)
+        got = orc.rand_uniform_real(e, n, a, b, dtype)
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), \
+            (pre, n, a, b)
+        assert orc.state(e) == (rctr, ridx)
+        if n:
+            assert got.min() >= dtype(a) and got.max() <= dtype(b)
+
+
 @needs_ref
 @pytest.mark.parametrize("kind", [orc.PHILOX4X32, orc.THREEFRY4X64, orc.ARS])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])

@@ -109,3 +109,34 @@ def test_smc_oracle_schemes_consistent(scheme):
                                0.5, scheme, nthreads=4, history=True)
     assert abs(ll - exact) < 0.3
     assert res.sum() > 0 and np.all(ess <= (1 << 13) + 1e-6)
+
+
+def test_kalman_filter_moments_match_dense_gaussian():
+    rng = np.random.default_rng(2)
+    for T in (1, 3, 25):
+        y = rng.normal(size=T)
+        for th in ((0.9, 1.0, 0.5), (-0.6, 0.4, 1.5)):
+            m, P = kalman.filter_moments(y, *th)
+            md, Pd = kalman.filter_moments_dense(y, *th)
+            assert np.allclose(m, md, rtol=1e-9, atol=1e-10)
+            assert np.allclose(P, Pd, rtol=1e-9, atol=1e-12)
+
+
+def test_smc_oracle_moments_vs_kalman():
+    """Selection estimates of E[x_t | y_1..t] and E[x_t^2 | y_1..t] from the
+    oracle sampler within Monte Carlo error of the Kalman filter."""
+    y = np.load(os.path.join(GOLDEN, "observations.npz"))["lg_y"][:40]
+    th = (0.9, 1.0, 0.5)
+    m, P = kalman.filter_moments(y, *th)
+    N = 1 << 15
+    for seed in (1, 2):
+        ll, ess, res, est = orc.smc_run(orc.MODEL_LG, th, y, N, seed, 0.5,
+                                        orc.SYSTEMATIC, nthreads=4, moments=2)
+        assert est.shape == (40, 2)
+        se = np.sqrt(P / ess)
+        assert np.all(np.abs(est[:, 0] - m) < 8 * se + 1e-3)
+        assert np.all(np.abs(est[:, 1] - (P + m * m)) <
+                      8 * se * (2 * np.abs(m) + 2 * np.sqrt(P)) + 1e-3)
+        # the estimator does not disturb the run
+        assert ll == orc.smc_run(orc.MODEL_LG, th, y, N, seed, 0.5,
+                                 orc.SYSTEMATIC, nthreads=4)
