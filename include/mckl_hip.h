@@ -316,6 +316,13 @@ int mckl_hip_smc_batch_create(int model, size_t nfilters,
 int mckl_hip_smc_batch_set_tune(mckl_hip_smc_batch *b, const mckl_hip_tune *t);
 int mckl_hip_smc_batch_run(mckl_hip_smc_batch *b, int use_graph, void *stream);
 int mckl_hip_smc_batch_result(mckl_hip_smc_batch *b, double *h_loglik);
+/* New parameters theta[3f..3f+2] and seeds for every filter of an existing
+ * batch (same model, sizes, observations, threshold and scheme); the next
+ * run uses them, with or without the captured graph.  Waits for runs in
+ * flight.  Used by the PMCMC drivers, which re-run the same filters at
+ * proposed parameters (algorithm/pmcmc.hpp:138-142). */
+int mckl_hip_smc_batch_set_params(mckl_hip_smc_batch *b,
+    const double *h_theta, const uint64_t *h_seed);
 int mckl_hip_smc_batch_destroy(mckl_hip_smc_batch *b);
 
 /*

@@ -607,6 +607,16 @@ class SMCBatch:
         check(lib.mckl_hip_smc_batch_result(self._h, _dptr(out)))
         return out
 
+    def set_params(self, theta, seeds):
+        """New theta (nfilters x 3) and seeds for the next run."""
+        th = np.ascontiguousarray(theta, np.float64).reshape(-1)
+        sd = np.ascontiguousarray(seeds, np.uint64)
+        if th.size != 3 * self.nfilters or sd.size != self.nfilters:
+            raise MCKLError("set_params: one theta row and seed per filter")
+        check(lib.mckl_hip_smc_batch_set_params(
+            self._h, _dptr(th),
+            sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
@@ -752,3 +762,7 @@ class RNGSetVector:
         check(fn(self.engine_cls.KIND, interval, self.n, _ptr(self.keys),
                  per, _ptr(out), _stream()))
         return out
+
+
+from . import pmcmc  # noqa: E402,F401  (PMCMCMutation, PMCMCBatch)
+from .pmcmc import PMCMCMutation, PMCMCBatch  # noqa: E402,F401

@@ -123,6 +123,8 @@ SIGNATURES = {
     "mckl_hip_smc_batch_run": (_int, [_vp, _int, _vp]),
     "mckl_hip_smc_batch_result": (_int, [_vp, ctypes.POINTER(_dbl)]),
     "mckl_hip_smc_batch_destroy": (_int, [_vp]),
+    "mckl_hip_smc_batch_set_params": (
+        _int, [_vp, ctypes.POINTER(_dbl), _U64P]),
     "mckl_hip_seed_keys": (
         _int, [_int, _u64, _u64, _u64, _int, _sz, _vp, _vp]),
     "mckl_hip_rngset_rand": (_int, [_int, _sz, _vp, _sz, _vp, _vp]),

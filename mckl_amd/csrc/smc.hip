@@ -867,6 +867,10 @@ struct mckl_hip_smc_batch {
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
     int graph_tune = -1;
+    // host copies for set_params: descriptors, model, observations
+    int model = 0;
+    std::vector<FilterDesc> fdh;
+    std::vector<double> y;
 };
 
 static void batch_free(mckl_hip_smc_batch *b)
@@ -991,6 +995,8 @@ int mckl_hip_smc_batch_create(int model, size_t nfilters, const uint32_t *h_N,
     mckl_hip_smc_batch *b = new mckl_hip_smc_batch();
     b->nfilters = nfilters;
     b->T = T;
+    b->model = model;
+    b->y.assign(h_y, h_y + nfilters * T);
     int rc = MCKL_HIP_OK;
 #define MCKL_ALLOC(p, n)                                                      \
     do {                                                                      \
@@ -1035,7 +1041,38 @@ int mckl_hip_smc_batch_create(int model, size_t nfilters, const uint32_t *h_N,
         batch_free(b);
         return rc;
     }
+    b->fdh = std::move(fd);
     *out = b;
+    return MCKL_HIP_OK;
+}
+
+int mckl_hip_smc_batch_set_params(mckl_hip_smc_batch *b,
+    const double *h_theta, const uint64_t *h_seed)
+{
+    if (!b || !h_theta || !h_seed)
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    const size_t nf = b->nfilters, T = b->T;
+    std::vector<ModelConst> mc(nf);
+    for (size_t f = 0; f < nf; ++f) {
+        const int rc = model_const(b->model, h_theta + 3 * f, mc[f]);
+        if (rc)
+            return rc;
+    }
+    std::vector<float> obs(nf * T);
+    for (size_t f = 0; f < nf; ++f) {
+        SmcArgs &A = b->fdh[f].A;
+        A.mc = mc[f];
+        A.k0 = (uint32_t)h_seed[f];
+        A.k1 = (uint32_t)(h_seed[f] >> 32);
+        obs_const(b->model, h_theta + 3 * f, b->y.data() + T * f, T,
+            obs.data() + T * f);
+    }
+    // a run still in flight reads the descriptors: wait for it
+    MCKL_HIP_TRY(hipDeviceSynchronize());
+    MCKL_HIP_TRY(hipMemcpy(b->fd, b->fdh.data(), nf * sizeof(FilterDesc),
+        hipMemcpyHostToDevice));
+    MCKL_HIP_TRY(hipMemcpy(b->obs, obs.data(), obs.size() * sizeof(float),
+        hipMemcpyHostToDevice));
     return MCKL_HIP_OK;
 }
 

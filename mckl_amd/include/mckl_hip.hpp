@@ -16,9 +16,11 @@
 #include <hip/hip_runtime_api.h>
 
 #include <array>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <istream>
 #include <ostream>
 #include <stdexcept>
@@ -649,6 +651,107 @@ class SMCSamplerHIP
         if (!s_)
             hip_check(mckl_hip_smc_create(model_, theta_.data(), y_.size(),
                 y_.data(), N_, seed_, threshold_, scheme_, &s_));
+    }
+};
+
+/// PMCMCMutation<Param, T> (algorithm/pmcmc.hpp:71-158) for the built-in
+/// models, param = theta.  The particle filter is a one-filter batch (one
+/// persistent launch per run) whose log-likelihood is the reference's
+/// log_nc; update objects and the acceptance draw (log of
+/// U01CanonicalDistribution<double>, the reference's default
+/// U01Distribution) use rng().  A proposal whose prior is -inf is rejected
+/// without a run.  Filter run r uses seed + 2^32 (r + 1).
+class PMCMCMutationHIP
+{
+  public:
+    using param_type = std::array<double, 3>;
+    using eval_type = std::function<double(Philox4x32 &, param_type &)>;
+    using prior_type = std::function<double(const param_type &)>;
+
+    PMCMCMutationHIP(std::size_t N, std::size_t M, prior_type prior,
+        int model, const std::vector<double> &y, std::uint32_t seed = 1,
+        double threshold = 0.5, int scheme = MCKL_HIP_STRATIFIED)
+        : prior_(std::move(prior)), rng_(seed), seed_(seed)
+    {
+        if (y.size() < M)
+            throw RuntimeAssert("PMCMCMutationHIP: fewer than M observations");
+        const std::uint32_t n = static_cast<std::uint32_t>(N);
+        const double init[3] = {0.5, 1.0, 1.0};
+        const std::uint64_t s = seed;
+        hip_check(mckl_hip_smc_batch_create(model, 1, &n, init, M, y.data(),
+            &s, threshold, scheme, &pf_));
+    }
+    ~PMCMCMutationHIP() { mckl_hip_smc_batch_destroy(pf_); }
+    PMCMCMutationHIP(const PMCMCMutationHIP &) = delete;
+    PMCMCMutationHIP &operator=(const PMCMCMutationHIP &) = delete;
+
+    void prior(prior_type p) { prior_ = std::move(p); }
+    std::size_t update(eval_type e)
+    {
+        eval_.push_back(std::move(e));
+        return eval_.size() - 1;
+    }
+    void reset() { eval_.clear(); }
+    Philox4x32 &rng() { return rng_; }
+    double log_nc() const { return log_nc_; }
+
+    std::size_t operator()(std::size_t iter, param_type &param)
+    {
+        if (iter == 0) {
+            log_nc_ = loglik(param);
+            return 0;
+        }
+        const double lnc = log_nc_;
+        double prob = -prior_(param) - lnc;
+        param_type p(param);
+        for (auto &e : eval_)
+            prob += e(rng_, p);
+        prob += prior_(p);
+        double lnew = -HUGE_VAL;
+        if (prob > -HUGE_VAL) {
+            lnew = loglik(p);
+            prob += lnew;
+        } else {
+            ++runs_;
+        }
+        double *d = nullptr;
+        double u = 0;
+        if (hipMalloc(reinterpret_cast<void **>(&d), sizeof(double)) !=
+            hipSuccess)
+            throw RuntimeAssert("mckl_hip: hipMalloc failed");
+        int rc = mckl_hip_rand_u01_f64(&rng_.state(), MCKL_HIP_U01_CANONICAL,
+            1, d, current_stream());
+        if (rc == MCKL_HIP_OK &&
+            hipMemcpy(&u, d, sizeof(u), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = MCKL_HIP_EHIP;
+        hipFree(d);
+        hip_check(rc);
+        if (std::log(u) < prob) {
+            param = p;
+            log_nc_ = lnew;
+            return 1;
+        }
+        return 0;
+    }
+
+  private:
+    prior_type prior_;
+    std::vector<eval_type> eval_;
+    Philox4x32 rng_;
+    std::uint64_t seed_;
+    std::uint64_t runs_ = 0;
+    double log_nc_ = 0;
+    mckl_hip_smc_batch *pf_ = nullptr;
+
+    double loglik(const param_type &p)
+    {
+        const std::uint64_t s = seed_ + ((runs_ + 1) << 32);
+        ++runs_;
+        hip_check(mckl_hip_smc_batch_set_params(pf_, p.data(), &s));
+        hip_check(mckl_hip_smc_batch_run(pf_, 1, current_stream()));
+        double ll = 0;
+        hip_check(mckl_hip_smc_batch_result(pf_, &ll));
+        return ll;
     }
 };
 

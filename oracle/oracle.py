@@ -438,3 +438,64 @@ def ref_resample_index(script):
 if REF is not None:
     REF.ref_ri_size.restype = ctypes.c_size_t
     REF.ref_ri_nrow.restype = ctypes.c_size_t
+
+
+def pmmh(model, y, N, M, prior, updates, params, iters, seed=1,
+         threshold=0.5, scheme=STRATIFIED):
+    """Restatement of PMCMCMutation::operator() (algorithm/pmcmc.hpp:125-158)
+    for C chains side by side (C = 1 is the single mutation): params (C, 3)
+    is moved in place over iterations 0..iters-1; returns the acceptance
+    flags (iters, C) and the final log normalising constants.  prior maps
+    (C, 3) to C log densities; updates are f(engine, P) -> C log proposal
+    ratios drawing from the chains' shared Philox4x32(seed) engine as the
+    device drivers do; the filter of run r, chain c is smc_run with seed
+    seed + 2^32 (r C + c + 1) on y[:M].  A proposal with prior -inf is
+    rejected without a run.  The acceptance draw is log U01Canonical<double>
+    (random/u01_distribution.hpp:42,225-233)."""
+    P0 = np.asarray(params, np.float64)
+    C = P0.shape[0]
+    e = engine(PHILOX4X32, [seed & 0xFFFFFFFF, 0])
+    runs = [0]
+
+    def loglik(P, ok):
+        base = runs[0] * C
+        runs[0] += 1
+        out = np.full(C, -np.inf)
+        for c in range(C):
+            if ok[c]:
+                s = (seed & 0xFFFFFFFF) + ((base + 1 + c) << 32)
+                out[c] = smc_run(model, P[c], y[:M], N, s, threshold, scheme,
+                                 nthreads=4)
+        return out
+
+    acc_hist = np.zeros((iters, C), np.int32)
+    lnc = loglik(P0, np.ones(C, bool))
+    for it in range(1, iters):
+        prob = -np.asarray(prior(P0), np.float64) - lnc
+        P = P0.copy()
+        for f in updates:
+            prob = prob + np.asarray(f(e, P), np.float64)
+        prob = prob + np.asarray(prior(P), np.float64)
+        ok = prob > -np.inf
+        new = loglik(P, ok)
+        prob = prob + new
+        u = np.log(rand_u01(e, CANONICAL, C, np.float64))
+        acc = u < prob
+        P0[acc] = P[acc]
+        lnc = np.where(acc, new, lnc)
+        acc_hist[it] = acc
+    params[...] = P0
+    return acc_hist, lnc
+
+
+def pmmh_random_walk(scales):
+    """Oracle twin of the random-walk update: p += scales * z with z from
+    rand(engine, Normal<double>(0, 1), C * 3)."""
+    s = np.asarray(scales, np.float64)
+
+    def f(e, P):
+        z = rand_normal(e, P.size, 0.0, 1.0, np.float64).reshape(P.shape)
+        P += s * z
+        return np.zeros(P.shape[0])
+
+    return f

@@ -210,6 +210,62 @@ int main(int argc, char **argv)
         ok = ok && ri.index(17) == state[17 * dim] % N;
         CHECK(ok, "ResampleIndex trace == direct copy");
     }
+    // 8. SMCEstimator weighted sums and the sampler's moment estimator
+    {
+        const std::size_t n = 3000, d = 2;
+        std::vector<double> u(n * d), w(n, 1.0 / n);
+        for (std::size_t i = 0; i != n; ++i) {
+            u[i * d] = double(i);
+            u[i * d + 1] = 1.0;
+        }
+        Dev<double> du(n * d), dw(n), dr(d);
+        (void)hipMemcpy(du.p, u.data(), n * d * 8, hipMemcpyHostToDevice);
+        (void)hipMemcpy(dw.p, w.data(), n * 8, hipMemcpyHostToDevice);
+        mckl::estimate(n, d, mckl::RowMajor, du.p, dw.p, dr.p);
+        std::vector<double> r = dr.host(d);
+        // sum_i i / n = (n - 1) / 2; sum_i 1 / n = 1 up to the rounding of
+        // the weights 1/n
+        CHECK(std::fabs(r[0] - 1499.5) < 1e-9 && std::fabs(r[1] - 1) < 1e-12,
+            "estimate (weighted sums)");
+        std::vector<double> y(20);
+        for (std::size_t t = 0; t != y.size(); ++t)
+            y[t] = std::cos(0.2 * t);
+        mckl::SMCSamplerHIP s(MCKL_HIP_MODEL_LG, {{0.9, 1.0, 0.5}}, y, 4096, 5);
+        s.estimate_moments(2);
+        s.iterate();
+        std::vector<double> e = s.estimates();
+        bool ok = e.size() == 40;
+        for (std::size_t t = 0; ok && t != 20; ++t)
+            ok = e[2 * t + 1] >= e[2 * t] * e[2 * t];
+        CHECK(ok, "SMCSamplerHIP moment estimates (E x^2 >= (E x)^2)");
+    }
+    // 9. PMMH with the device log-likelihood
+    {
+        std::vector<double> y(20);
+        for (std::size_t t = 0; t != y.size(); ++t)
+            y[t] = std::sin(0.3 * t);
+        auto prior = [](const std::array<double, 3> &p) {
+            return std::fabs(p[0]) < 1 && p[1] > 0 && p[2] > 0 ? 0.0 :
+                                                                  -HUGE_VAL;
+        };
+        mckl::PMCMCMutationHIP mut(256, 20, prior, MCKL_HIP_MODEL_LG, y, 3);
+        mut.update([](mckl::Philox4x32 &rng, std::array<double, 3> &p) {
+            mckl::NormalDistribution<double> normal(0, 0.05);
+            Dev<double> z(3);
+            mckl::rand(rng, normal, 3, z.p);
+            std::vector<double> h = z.host(3);
+            for (int k = 0; k != 3; ++k)
+                p[k] += h[k];
+            return 0.0;
+        });
+        std::array<double, 3> param{{0.8, 1.0, 0.5}};
+        std::size_t acc = 0;
+        for (std::size_t it = 0; it != 30; ++it)
+            acc += mut(it, param);
+        CHECK(acc > 0 && acc < 29 && std::isfinite(mut.log_nc()) &&
+                std::fabs(param[0]) < 1,
+            "PMCMCMutationHIP chain");
+    }
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;
 }
