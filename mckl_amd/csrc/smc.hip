@@ -278,9 +278,8 @@ __global__ __launch_bounds__(kGroup) void k_rs_scatter(SmcArgs A, uint32_t t,
 {
     if (!st->resampled[t & 1])
         return;
-    __shared__ __align__(16) unsigned char lds[kScratch];
-    const GroupLds L = group_lds(lds);
-    tile_scatter(A, blockIdx.x, B.cnt, B.Eincl, B.zpos, idx, threadIdx.x, L.u);
+    __shared__ uint32_t Es[kTile];
+    tile_scatter_ranks(A, blockIdx.x, B.Eincl, B.zpos, idx, threadIdx.x, Es);
 }
 
 // -------------------------------------------------------------------------
@@ -313,6 +312,7 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
 {
     __shared__ __align__(16) unsigned char scratch[G * kScratch];
     __shared__ BatchLds S;
+    __shared__ uint32_t Es[G * kTile]; // tile_scatter_ranks, one per group
     const int grp = threadIdx.x / kGroup;
     const int gt = threadIdx.x % kGroup;
     const GroupLds L = group_lds(scratch + grp * kScratch);
@@ -431,8 +431,8 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
                 }
                 for (int r = 0; r < rounds; ++r) {
                     const int k = r * G + grp;
-                    tile_scatter(A, (uint32_t)(k < nt ? k : nt), cnt, Eincl,
-                        zpos, idx, gt, L.u);
+                    tile_scatter_ranks(A, (uint32_t)(k < nt ? k : nt), Eincl,
+                        zpos, idx, gt, Es + grp * kTile);
                 }
             }
         }

@@ -57,7 +57,8 @@ template <int G>
 __device__ double fwd_step(const GradArgs &A, uint32_t t, const SlotPtr &prev,
     const SlotPtr &cur, uint32_t *cnt, int mode, const double *lwn_row,
     const uint32_t *par_row, int32_t *tape_r, double *tape_lwn_out,
-    uint32_t *tape_par_out, GLds &S, const GroupLds &L, int grp, int gt)
+    uint32_t *tape_par_out, GLds &S, const GroupLds &L, int grp, int gt,
+    uint32_t *Es)
 {
     const int nt = (int)A.ntiles;
     const int rounds = (nt + G - 1) / G;
@@ -169,8 +170,8 @@ __device__ double fwd_step(const GradArgs &A, uint32_t t, const SlotPtr &prev,
         sa.N = A.N;
         for (int r = 0; r < rounds; ++r) {
             const int k = r * G + grp;
-            tile_scatter(sa, (uint32_t)(k < nt ? k : nt), cnt, cur.E, cur.zpos,
-                cur.par, gt, L.u);
+            tile_scatter_ranks(sa, (uint32_t)(k < nt ? k : nt), cur.E,
+                cur.zpos, cur.par, gt, Es);
         }
         if (mode == kGradRecord) {
             for (uint32_t i = threadIdx.x; i < A.N; i += G * kGroup) {
@@ -192,6 +193,7 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
 {
     __shared__ __align__(16) unsigned char scratch[G * kScratch];
     __shared__ GLds S;
+    __shared__ uint32_t Es[G * kTile]; // tile_scatter_ranks, one per group
     const int grp = threadIdx.x / kGroup;
     const int gt = threadIdx.x % kGroup;
     const GroupLds L = group_lds(scratch + grp * kScratch);
@@ -248,7 +250,8 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
                 tl ? tl + (size_t)(t - 1) * rowN : nullptr,
                 tp ? tp + (size_t)(t - 1) * rowN : nullptr, tr,
                 tl ? tl + (size_t)t * rowN : nullptr,
-                tp ? tp + (size_t)t * rowN : nullptr, S, L, grp, gt);
+                tp ? tp + (size_t)t * rowN : nullptr, S, L, grp, gt,
+                Es + grp * kTile);
             if (mode != kGradReplay && t % A.K == 0 && t < A.T)
                 copy_slot<G>(slot_at(ck, slot_bytes, t / A.K, Np), cur, Np);
         }
@@ -274,7 +277,8 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
             for (uint32_t t = t0 + 1; t <= t1; ++t)
                 (void)fwd_step<G>(A, t, slot_at(seg, slot_bytes, t - t0 - 1, Np),
                     slot_at(seg, slot_bytes, t - t0, Np), cnt, kGradRun, nullptr,
-                    nullptr, nullptr, nullptr, nullptr, S, L, grp, gt);
+                    nullptr, nullptr, nullptr, nullptr, S, L, grp, gt,
+                    Es + grp * kTile);
             for (uint32_t t = t1; t > t0; --t) {
                 const SlotPtr cur = slot_at(seg, slot_bytes, t - t0, Np);
                 const SlotPtr prev = slot_at(seg, slot_bytes, t - t0 - 1, Np);

@@ -227,7 +227,7 @@ __device__ __forceinline__ void gtile_counts(const GradArgs &G, uint32_t tile,
 }
 
 // tile_index on a slot: E, par (own index; zero slots are filled by
-// tile_scatter) and the positions of the zero slots by rank, which the
+// tile_scatter_ranks) and the positions of the zero slots by rank, which the
 // adjoint's gather over children also uses
 __device__ __forceinline__ void gtile_index(const GradArgs &G, uint32_t tile,
     const uint32_t *cnt, uint32_t eoff, uint32_t zoff, const SlotPtr &sl,

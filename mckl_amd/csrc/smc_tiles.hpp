@@ -456,7 +456,7 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
 // Stable rep->index (resample.hpp:416-466, M = N): idx_i = i when rep_i > 0;
 // zero slots take extra copies in ascending source order.  Writes the
 // inclusive extra-copy prefix E_i, idx_i = i for every slot, and
-// zpos[r] = i for the zero slot of rank r; tile_scatter then overwrites the
+// zpos[r] = i for the zero slot of rank r; tile_scatter_ranks then overwrites the
 // zero slots with their sources.
 // the thread's 8 counts and the count before them
 __device__ __forceinline__ void load_counts(const SmcArgs &A, uint32_t tile,
@@ -494,7 +494,7 @@ __device__ __forceinline__ void tile_index_c(const SmcArgs &A, uint32_t tile,
         const uint64_t i = i0 + k;
         ex += rep[k] > 1 ? rep[k] - 1 : 0;
         E[k] = ex;
-        own[k] = (uint32_t)i; // zero slots are overwritten by tile_scatter
+        own[k] = (uint32_t)i; // zero slots: overwritten by tile_scatter_ranks
         if (i < A.N && rep[k] == 0) {
             zpos[zx] = (uint32_t)i;
             ++zx;
@@ -513,59 +513,44 @@ __device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
     tile_index_c(A, tile, c, prev, eoff, zoff, Eincl, idx, zpos, gt, ldsu);
 }
 
-// Source j with rep_j > 1 owns the extra copies of ranks
-// [E_j - (rep_j - 1), E_j) and writes itself into those zero slots.  A source
-// with more than kHeavy extra copies is queued in LDS and its copies are
-// written by the whole group, so one heavily replicated particle does not
-// serialise on a single lane.  ldsu needs 4 + 3 * kMaxHeavy words.
-constexpr uint32_t kHeavy = 16;
-constexpr uint32_t kMaxHeavy = 64;
-
-__device__ __forceinline__ void tile_scatter(const SmcArgs &A, uint32_t tile,
-    const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ Eincl,
+// Zero slots take their sources.  Source j with rep_j > 1 owns the extra
+// copies of ranks [E_j - (rep_j - 1), E_j), so rank r of the tile's range
+// [E_{t0-1}, E_{t0+n-1}) belongs to the first particle j of the tile with
+// E_j > r, and idx[zpos[r]] = j.  The ranks go round-robin to the group's
+// threads and each finds its source by binary search in the tile's E held in
+// LDS (Es, kTile words), so the tile's work is spread evenly over its
+// threads however the copies are distributed among its particles, and the
+// zpos loads of consecutive threads are contiguous.  A tile at or past the
+// end (the batched forms' padding rounds) writes nothing.  Contains two
+// workgroup barriers: every thread of the workgroup calls it.
+__device__ __forceinline__ void tile_scatter_ranks(const SmcArgs &A,
+    uint32_t tile, const uint32_t *__restrict__ Eincl,
     const uint32_t *__restrict__ zpos, uint32_t *__restrict__ idx, int gt,
-    uint32_t *ldsu)
+    uint32_t *Es)
 {
+    const uint64_t t0 = (uint64_t)tile * kTile;
+    const bool live = t0 < A.N;
+    const uint32_t n = !live ? 0u
+        : (uint32_t)(A.N - t0 < (uint64_t)kTile ? A.N - t0 : kTile);
     const uint64_t i0 = tile_first(tile, gt);
-    uint32_t c[kItems], E[kItems];
-    load8(cnt, i0, A.N, c, 0u);
+    uint32_t E[kItems];
     load8(Eincl, i0, A.N, E, 0u);
-    uint32_t prev = (i0 == 0 || i0 > A.N) ? 0u : cnt[i0 - 1];
-    uint32_t *nheavy = ldsu;
-    uint32_t *list = ldsu + 4;
-    if (gt == 0)
-        *nheavy = 0;
-    __syncthreads();
+    const uint32_t beg = (!live || t0 == 0) ? 0u : Eincl[t0 - 1];
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t j = i0 + k;
-        if (j < A.N) {
-            const uint32_t rep = c[k] - prev;
-            prev = c[k];
-            if (rep > 1) {
-                const uint32_t end = E[k], beg = end - (rep - 1);
-                uint32_t slot = kMaxHeavy;
-                if (rep - 1 > kHeavy) {
-                    slot = atomicAdd(nheavy, 1u);
-                    if (slot < kMaxHeavy) {
-                        list[3 * slot] = (uint32_t)j;
-                        list[3 * slot + 1] = beg;
-                        list[3 * slot + 2] = end;
-                    }
-                }
-                if (slot >= kMaxHeavy)
-                    for (uint32_t r = beg; r < end; ++r)
-                        idx[zpos[r]] = (uint32_t)j;
-            }
-        }
-    }
+    for (int k = 0; k < kItems; ++k)
+        Es[gt * kItems + k] = E[k];
     __syncthreads();
-    const uint32_t nh = *nheavy < kMaxHeavy ? *nheavy : kMaxHeavy;
-    for (uint32_t h = 0; h < nh; ++h) {
-        const uint32_t j = list[3 * h];
-        const uint32_t end = list[3 * h + 2];
-        for (uint32_t r = list[3 * h + 1] + gt; r < end; r += kGroup)
-            idx[zpos[r]] = j;
+    const uint32_t end = live ? Es[n - 1] : 0u;
+    for (uint32_t r = beg + gt; r < end; r += kGroup) {
+        uint32_t lo = 0, hi = n - 1; // Es[n - 1] = end > r
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (Es[mid] > r)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        idx[zpos[r]] = (uint32_t)(t0 + lo);
     }
     __syncthreads();
 }
