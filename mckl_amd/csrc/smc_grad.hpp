@@ -196,7 +196,7 @@ __device__ __forceinline__ void gtile_counts(const GradArgs &G, uint32_t tile,
             } else {
                 const double P = i == tend - 1 ? toffh[tile + 1]
                                                : dd_add(base, run).hi;
-                c[k] = U.count_less(P * invS);
+                c[k] = U.count_fast(P * invS);
             }
             cnt[i] = c[k];
         }
@@ -207,7 +207,7 @@ __device__ __forceinline__ void gtile_counts(const GradArgs &G, uint32_t tile,
     if (gt > 0)
         prev = ldsu[gt - 1];
     else
-        prev = tile == 0 ? 0u : U.count_less(toffh[tile] * invS);
+        prev = tile == 0 ? 0u : U.count_fast(toffh[tile] * invS);
     __syncthreads();
     uint32_t es = 0, zs = 0;
     for (int k = 0; k < kItems; ++k) {

@@ -368,6 +368,37 @@ struct UniformSeq {
         }
         return c;
     }
+
+    // The same count without the walk in all but rounding-edge cases.  The
+    // uniforms satisfy fl(delta j) <= u_j <= fl(delta (j+1)) (systematic:
+    // u0d < delta), so in exact arithmetic the count is the guess
+    // c = floor(S M) or c + 1; it is c + [u_c < S] whenever u_{c-1} < S and
+    // u_{c+1} >= S, which the bounds above often decide without evaluating
+    // u.  When either check fails (S M within rounding of an integer) the
+    // walk gives the count.  Same result as count_less, always.
+    __device__ __forceinline__ uint32_t count_fast(double S) const
+    {
+        if (scheme == MCKL_HIP_MULTINOMIAL)
+            return count_less(S);
+        const double g = floor(__dmul_rn(S, (double)M));
+        const uint32_t c = g <= 0.0 ? 0u : (g >= (double)M ? M : (uint32_t)g);
+        const bool strat = scheme == MCKL_HIP_STRATIFIED;
+        const double blo = __dmul_rn(delta, (double)c);       // <= u_c
+        const double bhi = __dmul_rn(delta, (double)c + 1.0); // u_c <= bhi
+        const bool lo_ok = c == 0 || (strat && blo < S) || (*this)(c - 1) < S;
+        const bool hi_ok = c + 1 >= M || (strat && bhi >= S) ||
+            (*this)(c + 1) >= S;                            // u_{c+1} >= bhi
+        uint32_t in = 0;
+        if (c < M) {
+            if (strat && bhi < S)
+                in = 1;
+            else if (!(strat && blo >= S))
+                in = (*this)(c) < S ? 1u : 0u;
+        }
+        if (lo_ok && hi_ok)
+            return c + in;
+        return count_less(S);
+    }
 };
 
 // Cumulative normalised weights S_j = round(prefix_j(e)) * (1/S), prefix in
@@ -412,7 +443,7 @@ __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
             } else {
                 const double P =
                     i == tend - 1 ? next_hi : dd_add(base, run).hi;
-                c[k] = U.count_less(__dmul_rn(P, invS));
+                c[k] = U.count_fast(__dmul_rn(P, invS));
             }
         }
     }
@@ -424,7 +455,7 @@ __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
     if (gt > 0)
         prev = ldsu[gt - 1];
     else
-        prev = tile == 0 ? 0u : U.count_less(__dmul_rn(tile_off.hi, invS));
+        prev = tile == 0 ? 0u : U.count_fast(__dmul_rn(tile_off.hi, invS));
     __syncthreads();
     uint32_t es = 0, zs = 0;
 #pragma unroll
