@@ -51,24 +51,37 @@ __global__ __launch_bounds__(kGroup) void k_init(
 }
 
 struct RsBuf {
-    double *tdh, *tdl; // tile dd totals of e
+    double *tdh, *tdl; // tile dd totals of a (unscaled)
     uint32_t *te, *tz; // tile extra-copy / zero-slot counts
     uint32_t *cnt, *Eincl, *zpos;
     double *tmh, *tml; // multinomial: tile dd totals of the log spacings
     double *u;         // multinomial: the step's sorted uniforms
+    double *uh, *ul;   // multinomial: scaled tile dd totals r T
 };
 
+// move + weight of a tile, then the tile's sums of a_i = exp_f(lw_i - m_k)
+// about its own max m_k (the weights' scale r_k needs the max over tiles and
+// is applied by the next kernel)
 __global__ __launch_bounds__(kGroup) void k_move(SmcArgs A, uint32_t t,
     const float *xin, float *xout, float *lw, const uint32_t *idx,
-    const SmcState *st, float *tmax)
+    const SmcState *st, float *tmax, double *ts, double *ts2, RsBuf B)
 {
     __shared__ __align__(16) unsigned char lds[kScratch];
     const GroupLds L = group_lds(lds);
     const int p = (t - 1) & 1;
+    float l[kItems];
     const float mt = tile_move(A, t, blockIdx.x, xin, xout, lw, idx,
-        st->resampled[p], st->m[p], threadIdx.x, L.f);
-    if (threadIdx.x == 0)
+        st->resampled[p], st->m[p], threadIdx.x, L.f, l);
+    double S, S2;
+    dd et;
+    tile_sums_l(A, blockIdx.x, l, mt, threadIdx.x, L.d, S, S2, et);
+    if (threadIdx.x == 0) {
         tmax[blockIdx.x] = mt;
+        ts[blockIdx.x] = S;
+        ts2[blockIdx.x] = S2;
+        B.tdh[blockIdx.x] = et.hi;
+        B.tdl[blockIdx.x] = et.lo;
+    }
 }
 
 __device__ __forceinline__ float tiles_max(
@@ -78,27 +91,6 @@ __device__ __forceinline__ float tiles_max(
     for (uint32_t i = gt; i < nt; i += kGroup)
         m = fmaxf(m, tmax[i]);
     return group_max(m, gt, ldsf);
-}
-
-// tile sums of e = exp_f(lw - m) and their double-double totals
-__global__ __launch_bounds__(kGroup) void k_sums(SmcArgs A, const float *lw,
-    const float *tmax, double *ts, double *ts2, RsBuf B)
-{
-    __shared__ __align__(16) unsigned char lds[kScratch];
-    const GroupLds L = group_lds(lds);
-    const int gt = threadIdx.x;
-    float l[kItems];
-    load8(lw, tile_first(blockIdx.x, gt), A.N, l, 0.0f);
-    const float m = tiles_max(tmax, A.ntiles, gt, L.f);
-    double S, S2;
-    dd et;
-    tile_sums_l(A, blockIdx.x, l, m, gt, L.d, S, S2, et);
-    if (gt == 0) {
-        ts[blockIdx.x] = S;
-        ts2[blockIdx.x] = S2;
-        B.tdh[blockIdx.x] = et.hi;
-        B.tdl[blockIdx.x] = et.lo;
-    }
 }
 
 // Every workgroup reduces the tile sums and takes the step's decision;
@@ -127,14 +119,18 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     float mx = -__builtin_huge_valf();
     for (int i = gt; i < nt; i += kGroup)
         mx = fmaxf(mx, tmax[i]);
+    const float m = group_max(mx, gt, L.f);
+    // sums of the scaled tile sums: r_i S_i and r_i^2 S2_i (the decision,
+    // log Z and the next normaliser), r_i T_i in double-double (the offsets
+    // and, rounded once, the total that normalises the cumulative weights)
     double sa = 0.0, sb = 0.0;
     dd loc{0.0, 0.0};
     for (int i = b; i < e; ++i) {
-        sa = __dadd_rn(sa, ts[i]);
-        sb = __dadd_rn(sb, ts2[i]);
-        loc = dd_add(loc, dd{B.tdh[i], B.tdl[i]});
+        const double r = tile_scale(tmax[i], m);
+        sa = __dadd_rn(sa, __dmul_rn(r, ts[i]));
+        sb = __dadd_rn(sb, __dmul_rn(__dmul_rn(r, r), ts2[i]));
+        loc = dd_add(loc, dd_mul_d(dd{B.tdh[i], B.tdl[i]}, r));
     }
-    const float m = group_max(mx, gt, L.f);
     group_sum2(sa, sb, gt, L.d);
     const double rs = sa, rs2 = sb;
     // the decision in every workgroup; the log Z increment (two logs) only
@@ -155,14 +151,19 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     }
     if (!o.resample)
         return;
+    const double rk = tile_scale(tmax[k], m);
     if (A.scheme == MCKL_HIP_MULTINOMIAL) {
         // multinomial: tile totals of the log spacings; the uniforms and
-        // the counts follow in k_rs_uniforms and k_rs_mcounts
+        // the counts follow in k_rs_uniforms and k_rs_mcounts, which read
+        // this tile's scaled total from B.uh/B.ul
         dd mt;
         tile_mterms(A, k, t, gt, L.d, mt);
         if (gt == 0) {
             B.tmh[k] = mt.hi;
             B.tml[k] = mt.lo;
+            const dd u = dd_mul_d(dd{B.tdh[k], B.tdl[k]}, rk);
+            B.uh[k] = u.hi;
+            B.ul[k] = u.lo;
         }
         return;
     }
@@ -179,7 +180,8 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
             }
             if (i == (int)k + 1)
                 x[2] = run.hi;
-            run = dd_add(run, dd{B.tdh[i], B.tdl[i]});
+            run = dd_add(run,
+                dd_mul_d(dd{B.tdh[i], B.tdl[i]}, tile_scale(tmax[i], m)));
         }
         if (gt == 0 && (int)k + 1 == nt)
             x[2] = total.hi;
@@ -188,11 +190,11 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     const dd toff{x[0], x[1]};
     const double next_hi = x[2];
     __syncthreads();
-    const double invS = __ddiv_rn(1.0, rs);
+    const double invS = __ddiv_rn(1.0, total.hi);
     const UniformSeq U = make_uniform_seq(A, t);
     uint32_t es, zs;
-    tile_counts_l(A, k, l, m, invS, toff, next_hi, U, B.cnt, gt, L.d, L.u, es,
-        zs);
+    tile_counts_l(A, k, l, tmax[k], rk, invS, toff, next_hi, U, B.cnt, gt,
+        L.d, L.u, es, zs);
     if (gt == 0) {
         B.te[k] = es;
         B.tz[k] = zs;
@@ -216,8 +218,7 @@ __global__ __launch_bounds__(kGroup) void k_rs_uniforms(SmcArgs A, uint32_t t,
 
 // multinomial: counts of the sorted uniforms below each cumulative weight
 __global__ __launch_bounds__(kGroup) void k_rs_mcounts(SmcArgs A, uint32_t t,
-    const float *lw, const double *ts, const double *ts2, const SmcState *st,
-    RsBuf B)
+    const float *lw, const float *tmax, const SmcState *st, RsBuf B)
 {
     const int c = t & 1;
     if (!st->resampled[c])
@@ -227,16 +228,15 @@ __global__ __launch_bounds__(kGroup) void k_rs_mcounts(SmcArgs A, uint32_t t,
     const int gt = threadIdx.x;
     const int nt = (int)A.ntiles;
     const float m = st->m[c];
-    double rs, rs2;
-    tiles_sum2(ts, ts2, nt, gt, L.d, rs, rs2);
+    const double rs = tiles_total_dd(B.uh, B.ul, nt, gt, L.d).hi;
     const uint32_t k = blockIdx.x;
     dd off, next;
-    tiles_offset_dd(B.tdh, B.tdl, nt, (int)k, (int)k + 1, gt, L.d, off, next);
+    tiles_offset_dd(B.uh, B.ul, nt, (int)k, (int)k + 1, gt, L.d, off, next);
     const double invS = __ddiv_rn(1.0, rs);
     const UniformSeq U = make_uniform_seq(A, t, B.u);
     uint32_t es, zs;
-    tile_counts(A, k, lw, m, invS, off, next.hi, U, B.cnt, gt, L.d, L.u, es,
-        zs);
+    tile_counts(A, k, lw, tmax[k], tile_scale(tmax[k], m), invS, off, next.hi,
+        U, B.cnt, gt, L.d, L.u, es, zs);
     if (gt == 0) {
         B.te[k] = es;
         B.tz[k] = zs;
@@ -294,6 +294,7 @@ struct FilterDesc {
 
 struct BatchLds {
     float tmax[kMaxTiles];
+    double r[kMaxTiles]; // tile scales exp_f(m_k - m)
     double ts[kMaxTiles], ts2[kMaxTiles];
     double tdh[kMaxTiles], tdl[kMaxTiles];
     double toh[kMaxTiles + 1], tol[kMaxTiles + 1];
@@ -305,7 +306,7 @@ struct BatchLds {
 };
 
 template <int G>
-__global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
+__global__ __launch_bounds__(G *kGroup, G == 1 ? 2 : 1) void k_batch(const FilterDesc *fd,
     const uint32_t *order, uint32_t nfilters, uint32_t *queue, float *xa,
     float *xb, float *lwb, uint32_t *idxb, uint32_t *cntb, uint32_t *Eb,
     uint32_t *zb, double *ub, double *logz_out)
@@ -346,26 +347,18 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
         for (uint32_t t = 1; t <= A.T; ++t) {
             const float *xin = (t & 1) ? x0 : x1;
             float *xout = (t & 1) ? x1 : x0;
-            // move + weight
+            // move + weight, with the tile sums about each tile's max
             for (int r = 0; r < rounds; ++r) {
                 const int k = r * G + grp;
                 const uint32_t kk = (uint32_t)(k < nt ? k : nt);
+                float l[kItems];
                 const float mt = tile_move(A, t, kk, xin, xout, lw, idx,
-                    resampled, mprev, gt, L.f);
-                if (gt == 0 && k < nt)
-                    S.tmax[k] = mt;
-                __syncthreads();
-            }
-            float m = -__builtin_huge_valf();
-            for (int i = 0; i < nt; ++i)
-                m = fmaxf(m, S.tmax[i]);
-            for (int r = 0; r < rounds; ++r) {
-                const int k = r * G + grp;
+                    resampled, mprev, gt, L.f, l);
                 double s, s2;
                 dd et;
-                tile_sums(A, (uint32_t)(k < nt ? k : nt), lw, m, gt, L.d, s, s2,
-                    et);
+                tile_sums_l(A, kk, l, mt, gt, L.d, s, s2, et);
                 if (gt == 0 && k < nt) {
+                    S.tmax[k] = mt;
                     S.ts[k] = s;
                     S.ts2[k] = s2;
                     S.tdh[k] = et.hi;
@@ -373,6 +366,21 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
                 }
                 __syncthreads();
             }
+            float m = -__builtin_huge_valf();
+            for (int i = 0; i < nt; ++i)
+                m = fmaxf(m, S.tmax[i]);
+            // scale the tile sums by r_k = exp_f(m_k - m), as the plain
+            // sampler's counting kernel does on the fly
+            for (int i = threadIdx.x; i < nt; i += G * kGroup) {
+                const double rr = tile_scale(S.tmax[i], m);
+                S.r[i] = rr;
+                S.ts[i] = __dmul_rn(rr, S.ts[i]);
+                S.ts2[i] = __dmul_rn(__dmul_rn(rr, rr), S.ts2[i]);
+                const dd u = dd_mul_d(dd{S.tdh[i], S.tdl[i]}, rr);
+                S.tdh[i] = u.hi;
+                S.tdl[i] = u.lo;
+            }
+            __syncthreads();
             double rs, rs2;
             tiles_sum2(S.ts, S.ts2, nt, gt, L.d, rs, rs2);
             const StepOut o = step_finalize(A, m, rs, rs2, C);
@@ -381,7 +389,6 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
             mprev = m;
             resampled = o.resample;
             if (resampled) {
-                const double invS = __ddiv_rn(1.0, rs);
                 double *u = ub ? ub + off : nullptr;
                 if (A.scheme == MCKL_HIP_MULTINOMIAL) {
                     // multinomial: sorted uniforms of the step first
@@ -407,12 +414,16 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
                     __syncthreads();
                 }
                 tiles_scan_dd(grp == 0, S.tdh, S.tdl, nt, S.toh, S.tol, gt, L.d);
+                // the weight total rounded once normalises the cumulative
+                // weights (the plain sampler's counting kernel does the same)
+                const double invS = __ddiv_rn(1.0, S.toh[nt]);
                 const UniformSeq U = make_uniform_seq(A, t, u);
                 for (int r = 0; r < rounds; ++r) {
                     const int k = r * G + grp;
                     uint32_t es, zs;
                     const int kk = k < nt ? k : nt;
-                    tile_counts(A, (uint32_t)kk, lw, m, invS,
+                    tile_counts(A, (uint32_t)kk, lw, kk < nt ? S.tmax[kk] : 0.0f,
+                        kk < nt ? S.r[kk] : 0.0, invS,
                         dd{S.toh[kk], S.tol[kk]}, kk < nt ? S.toh[kk + 1] : 0.0,
                         U, cnt, gt, L.d, L.u, es, zs);
                     if (gt == 0 && k < nt) {
@@ -444,9 +455,9 @@ __global__ __launch_bounds__(G *kGroup) void k_batch(const FilterDesc *fd,
 
 // Selection estimator of the moments x^1..x^K (SMCEstimator at step 0,
 // algorithm/smc.hpp:89-118 and :389-394): after the move and weighting of
-// step t, per tile the double-double sums of e_i and of e_i x_i^j with
-// e_i = exp_f(lw_i - m) exactly as the weight sums take it; products split
-// exactly (two_prod).  Partials: [(t-1) (K+1) + q][tile].
+// step t, per tile the double-double sums of e_i and of e_i x_i^j with the
+// samplers' weights e_i = a_i r_k; products split exactly (two_prod).
+// Partials: [(t-1) (K+1) + q][tile].
 constexpr int kMaxMoments = 4;
 
 __global__ __launch_bounds__(kGroup) void k_est_moments(SmcArgs A, uint32_t t,
@@ -460,6 +471,8 @@ __global__ __launch_bounds__(kGroup) void k_est_moments(SmcArgs A, uint32_t t,
     load8(lw, tile_first(blockIdx.x, gt), A.N, l, 0.0f);
     load8(x, tile_first(blockIdx.x, gt), A.N, xv, 0.0f);
     const float m = tiles_max(tmax, A.ntiles, gt, L.f);
+    const float mk = tmax[blockIdx.x];
+    const double rk = tile_scale(mk, m);
     dd acc[kMaxMoments + 1];
 #pragma unroll
     for (int q = 0; q <= kMaxMoments; ++q)
@@ -468,7 +481,8 @@ __global__ __launch_bounds__(kGroup) void k_est_moments(SmcArgs A, uint32_t t,
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         if (i0 + k < A.N) {
-            const double e = (double)smc_exp_f(__fsub_rn(l[k], m));
+            const double e = __dmul_rn((double)smc_exp_f(__fsub_rn(l[k], mk)),
+                rk); // exact: a product of two floats
             acc[0] = dd_add(acc[0], dd{e, 0.0});
             const double xd = (double)xv[k];
             double p = 1.0;
@@ -627,8 +641,8 @@ static void smc_free(mckl_hip_smc *s)
     smc_drop_graph(s);
     void *ptrs[] = {s->x[0], s->x[1], s->lw, s->obs, s->tmax, s->idx, s->ts,
         s->ts2, s->B.tdh, s->B.tdl, s->B.te, s->B.tz, s->B.cnt, s->B.Eincl,
-        s->B.zpos, s->B.tmh, s->B.tml, s->B.u, s->st, s->ess, s->res, s->eph,
-        s->epl, s->est};
+        s->B.zpos, s->B.tmh, s->B.tml, s->B.u, s->B.uh, s->B.ul, s->st, s->ess,
+        s->res, s->eph, s->epl, s->est};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -691,6 +705,8 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
         MCKL_ALLOC(s->B.tmh, nt);
         MCKL_ALLOC(s->B.tml, nt);
         MCKL_ALLOC(s->B.u, Np);
+        MCKL_ALLOC(s->B.uh, nt);
+        MCKL_ALLOC(s->B.ul, nt);
     }
     MCKL_ALLOC(s->st, 1);
     MCKL_ALLOC(s->ess, T);
@@ -723,19 +739,17 @@ static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
     for (uint32_t t = 1; t <= s->T; ++t) {
         const float *xin = s->x[(t - 1) & 1];
         float *xout = s->x[t & 1];
-        k_move<<<grid, kGroup, 0, st>>>(
-            s->A, t, xin, xout, s->lw, s->idx, s->st, s->tmax);
+        k_move<<<grid, kGroup, 0, st>>>(s->A, t, xin, xout, s->lw, s->idx,
+            s->st, s->tmax, s->ts, s->ts2, s->B);
         if (s->kmom)
             k_est_moments<<<grid, kGroup, 0, st>>>(s->A, t, xout, s->lw,
                 s->tmax, s->kmom, s->eph, s->epl);
-        k_sums<<<grid, kGroup, 0, st>>>(
-            s->A, s->lw, s->tmax, s->ts, s->ts2, s->B);
         k_rs_counts<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->tmax, s->ts,
             s->ts2, s->B, s->st, s->ess, s->res);
         if (s->A.scheme == MCKL_HIP_MULTINOMIAL) {
             k_rs_uniforms<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B);
             k_rs_mcounts<<<grid, kGroup, 0, st>>>(
-                s->A, t, s->lw, s->ts, s->ts2, s->st, s->B);
+                s->A, t, s->lw, s->tmax, s->st, s->B);
         }
         k_rs_index<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);
         k_rs_scatter<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);

@@ -189,6 +189,15 @@ __device__ __forceinline__ dd two_prod(double a, double b)
     return dd{p, __fma_rn(a, b, -p)};
 }
 
+// a * b for dd a and double b (error about 2^-104 relative)
+__device__ __forceinline__ dd dd_mul_d(dd a, double b)
+{
+    const dd p = two_prod(a.hi, b);
+    const double lo = __fma_rn(a.lo, b, p.lo);
+    const double h = __dadd_rn(p.hi, lo);
+    return dd{h, __dsub_rn(lo, __dsub_rn(h, p.hi))};
+}
+
 // a / b for dd a and double b > 0, to about 2^-104 relative before the
 // final rounding of hi + lo
 __device__ __forceinline__ double dd_div_d(dd a, double b)
@@ -295,6 +304,32 @@ __device__ __forceinline__ uint32_t group_excl_scan_u(
     return off + incl - v;
 }
 
+// exclusive max-scan of a u32 per thread over the group (init for thread 0).
+// lds >= 4 u32.
+__device__ __forceinline__ uint32_t group_excl_max_u(
+    uint32_t v, uint32_t init, int gt, uint32_t *lds)
+{
+    const int lane = gt & 63;
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d)
+            incl = incl > o ? incl : o;
+    }
+    uint32_t ex = __shfl_up(incl, 1, 64);
+    if (lane == 0)
+        ex = 0;
+    if (lane == 63)
+        lds[gt >> 6] = incl;
+    __syncthreads();
+    uint32_t base = init;
+    for (int w = 0; w < (gt >> 6); ++w)
+        base = base > lds[w] ? base : lds[w];
+    __syncthreads();
+    return base > ex ? base : ex;
+}
+
 // exclusive dd scan per thread value over the group (fixed Hillis-Steele
 // per wave, waves in order).  lds >= 8 doubles.
 __device__ __forceinline__ dd group_excl_scan_dd(
@@ -351,6 +386,22 @@ __device__ __forceinline__ void tiles_sum2(const double *s, const double *s2,
     group_sum2(a, c, gt, lds);
     rs = a;
     rs2 = c;
+}
+
+// dd total of n tile totals, the same operations as the total of
+// tiles_scan_dd (chunk sums, then the group scan's total)
+__device__ __forceinline__ dd tiles_total_dd(
+    const double *hi, const double *lo, int n, int gt, double *lds)
+{
+    const int chunk = (n + kGroup - 1) / kGroup;
+    const int b = gt * chunk;
+    const int e = b + chunk < n ? b + chunk : n;
+    dd loc{0.0, 0.0};
+    for (int i = b; i < e; ++i)
+        loc = dd_add(loc, dd{hi[i], lo[i]});
+    dd total;
+    (void)group_excl_scan_dd(loc, gt, lds, total);
+    return total;
 }
 
 // exclusive dd scan over n tile totals (hi/lo arrays); writes off[0..n],

@@ -192,15 +192,20 @@ __device__ __forceinline__ uint32_t parent_of(
                               : code;
 }
 
-// move + weight of one tile; returns the tile max of lw (all threads).
-// After a resampling step idx holds every slot's resolved parent.
+// move + weight of one tile; returns the tile max of lw (all threads) and
+// the thread's new log weights in lout (0 past the end).  After a
+// resampling step idx holds every slot's resolved parent.
 __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
     uint32_t tile, const float *xin, float *xout, float *lw,
-    const uint32_t *idx, int resampled, float mprev, int gt, float *ldsf)
+    const uint32_t *idx, int resampled, float mprev, int gt, float *ldsf,
+    float (&lout)[kItems])
 {
     const uint64_t i0 = tile_first(tile, gt);
     const float obs = A.obs[t - 1];
     float lmax = -__builtin_huge_valf();
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        lout[k] = 0.0f;
     if (i0 + kItems <= A.N) {
         float xp[kItems], lo[kItems];
         if (resampled) {
@@ -239,6 +244,9 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
             ln[k + 1] = __fadd_rn(lo[k + 1], smc_logg(A.mc, obs, xn[k + 1]));
             lmax = fmaxf(lmax, fmaxf(ln[k], ln[k + 1]));
         }
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            lout[k] = ln[k];
         *reinterpret_cast<float4 *>(xout + i0) =
             make_float4(xn[0], xn[1], xn[2], xn[3]);
         *reinterpret_cast<float4 *>(xout + i0 + 4) =
@@ -270,6 +278,7 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
                 const float l = __fadd_rn(carry, smc_logg(A.mc, obs, xnew));
                 xout[p] = xnew;
                 lw[p] = l;
+                lout[k + q] = l;
                 lmax = fmaxf(lmax, l);
             }
         }
@@ -277,8 +286,18 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
     return group_max(lmax, gt, ldsf);
 }
 
-// e_i = exp_f(lw_i - m); the tile's S = sum e, S2 = sum e^2 (canonical
-// order) and the double-double total of e for the resampling prefix sums
+// Weights.  With m_k the tile's max of lw and m the max over tiles, particle
+// i of tile k weighs e_i = a_i r_k, a_i = exp_f(lw_i - m_k), r_k =
+// exp_f(m_k - m) (tile_scale); the product of the two floats is exact in
+// double.  The tile's S = sum a, S2 = sum a^2 (canonical order) and the
+// double-double total of a are taken right after the move, before m is
+// known; the samplers scale them by r_k (S, dd total) and r_k^2 (S2).
+__device__ __forceinline__ double tile_scale(float mk, float m)
+{
+    return (double)smc_exp_f(__fsub_rn(mk, m));
+}
+
+// the tile's sums of a_i = exp_f(lw_i - mk) (see above)
 __device__ __forceinline__ void tile_sums_l(const SmcArgs &A, uint32_t tile,
     const float (&l)[kItems], float m, int gt, double *ldsd, double &S,
     double &S2, dd &etot)
@@ -300,15 +319,6 @@ __device__ __forceinline__ void tile_sums_l(const SmcArgs &A, uint32_t tile,
     S = s;
     S2 = s2;
     (void)group_excl_scan_dd(loc, gt, ldsd, etot);
-}
-
-__device__ __forceinline__ void tile_sums(const SmcArgs &A, uint32_t tile,
-    const float *lw, float m, int gt, double *ldsd, double &S, double &S2,
-    dd &etot)
-{
-    float l[kItems];
-    load8(lw, tile_first(tile, gt), A.N, l, 0.0f);
-    tile_sums_l(A, tile, l, m, gt, ldsd, S, S2, etot);
 }
 
 // Uniform sequences of step t (sorted, M = N):
@@ -402,16 +412,19 @@ struct UniformSeq {
 };
 
 // Cumulative normalised weights S_j = round(prefix_j(e)) * (1/S), prefix in
-// double-double over tiles (tile offsets from tiles_scan_dd of the tile
-// totals of tile_sums: toff for this tile, next_hi the high part of the next
-// tile's), and the counts cnt_j = #{u < S_j}.  The last particle of a tile
-// takes its prefix from the next tile offset so that neighbouring tiles agree
-// on boundary values; the last particle overall takes cnt = M
-// (resample.hpp:405).  Returns the tile sums of the extra-copy and zero-slot
-// counts through esum/zsum.
-// (l: the thread's 8 log weights, loaded by the caller)
+// double-double: the tile's offset (tile_off, from the exclusive scan of the
+// scaled tile totals) plus r_k times the in-tile prefix of a_i; the last
+// particle of a tile takes the next tile's offset (next_hi) so that
+// neighbouring tiles agree on boundary values, and the last particle overall
+// takes cnt = M (resample.hpp:405).  The counts cnt_j = #{u < S_j} are then
+// made non-decreasing and confined to [count(tile_off), count(next_hi)], the
+// values the neighbouring tiles use at the shared boundaries, so the
+// replication numbers are never negative and sum to M whatever the rounding
+// of the prefix sums (the clamp changes nothing when those are monotone).
+// Returns the tile sums of the extra-copy and zero-slot counts through
+// esum/zsum.  (l: the thread's 8 log weights, loaded by the caller)
 __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
-    const float (&l)[kItems], float m, double invS, dd tile_off,
+    const float (&l)[kItems], float mk, double rk, double invS, dd tile_off,
     double next_hi, const UniformSeq &U, uint32_t *cnt, int gt, double *ldsd,
     uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
 {
@@ -419,44 +432,51 @@ __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
     const uint64_t tend = (uint64_t)(tile + 1) * kTile < A.N
         ? (uint64_t)(tile + 1) * kTile
         : A.N;
-    double W[kItems];
+    // boundary counts shared with the neighbouring tiles
+    const uint32_t clo =
+        tile == 0 ? 0u : U.count_fast(__dmul_rn(tile_off.hi, invS));
+    const uint32_t chi =
+        tend == A.N ? U.M : U.count_fast(__dmul_rn(next_hi, invS));
     dd loc{0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        W[k] = i < A.N ? (double)smc_exp_f(__fsub_rn(l[k], m)) : 0.0;
-        loc = dd_add(loc, dd{W[k], 0.0});
+        const double w = i < A.N ? (double)smc_exp_f(__fsub_rn(l[k], mk)) : 0.0;
+        loc = dd_add(loc, dd{w, 0.0});
     }
     dd total;
     const dd toff = group_excl_scan_dd(loc, gt, ldsd, total);
-    const dd base = dd_add(tile_off, toff);
     uint32_t c[kItems];
-    dd run{0.0, 0.0};
+    dd run = toff;
+    uint32_t cmax = clo;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        run = dd_add(run, dd{W[k], 0.0});
+        // the same a_i as above (recomputed: fewer live registers)
+        const double w = i < A.N ? (double)smc_exp_f(__fsub_rn(l[k], mk)) : 0.0;
+        run = dd_add(run, dd{w, 0.0});
         c[k] = 0;
         if (i < A.N) {
-            if (i == (uint64_t)A.N - 1) {
-                c[k] = U.M;
+            uint32_t v;
+            if (i == tend - 1) {
+                v = chi;
             } else {
-                const double P =
-                    i == tend - 1 ? next_hi : dd_add(base, run).hi;
-                c[k] = U.count_fast(__dmul_rn(P, invS));
+                const double P = dd_add(tile_off, dd_mul_d(run, rk)).hi;
+                v = U.count_fast(__dmul_rn(P, invS));
             }
+            v = v < clo ? clo : (v > chi ? chi : v);
+            cmax = v > cmax ? v : cmax;
+            c[k] = cmax;
         }
     }
+    // counts of the threads before this one: make the sequence monotone
+    const uint32_t before = group_excl_max_u(cmax, clo, gt, ldsu);
+    uint32_t prev = before;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        if (i0 + k < A.N && c[k] < before)
+            c[k] = before;
     store8(cnt, i0, A.N, c);
-    // count before the first particle of this thread
-    ldsu[gt] = c[kItems - 1];
-    __syncthreads();
-    uint32_t prev;
-    if (gt > 0)
-        prev = ldsu[gt - 1];
-    else
-        prev = tile == 0 ? 0u : U.count_fast(__dmul_rn(tile_off.hi, invS));
-    __syncthreads();
     uint32_t es = 0, zs = 0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
@@ -474,14 +494,14 @@ __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
 }
 
 __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
-    const float *lw, float m, double invS, dd tile_off, double next_hi,
-    const UniformSeq &U, uint32_t *cnt, int gt, double *ldsd, uint32_t *ldsu,
-    uint32_t &esum, uint32_t &zsum)
+    const float *lw, float mk, double rk, double invS, dd tile_off,
+    double next_hi, const UniformSeq &U, uint32_t *cnt, int gt, double *ldsd,
+    uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
 {
     float l[kItems];
     load8(lw, tile_first(tile, gt), A.N, l, 0.0f);
-    tile_counts_l(A, tile, l, m, invS, tile_off, next_hi, U, cnt, gt, ldsd,
-        ldsu, esum, zsum);
+    tile_counts_l(A, tile, l, mk, rk, invS, tile_off, next_hi, U, cnt, gt,
+        ldsd, ldsu, esum, zsum);
 }
 
 // Stable rep->index (resample.hpp:416-466, M = N): idx_i = i when rep_i > 0;
@@ -571,9 +591,12 @@ __device__ __forceinline__ void tile_scatter_ranks(const SmcArgs &A,
     for (int k = 0; k < kItems; ++k)
         Es[gt * kItems + k] = E[k];
     __syncthreads();
-    const uint32_t end = live ? Es[n - 1] : 0u;
+    // ranks and slots stay inside the arrays even if the counts were not a
+    // replication vector (they are: the counts are clamped monotone)
+    uint32_t end = live ? Es[n - 1] : 0u;
+    end = end < A.N ? end : A.N;
     for (uint32_t r = beg + gt; r < end; r += kGroup) {
-        uint32_t lo = 0, hi = n - 1; // Es[n - 1] = end > r
+        uint32_t lo = 0, hi = n - 1; // Es[n - 1] >= end > r
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
             if (Es[mid] > r)
@@ -581,7 +604,9 @@ __device__ __forceinline__ void tile_scatter_ranks(const SmcArgs &A,
             else
                 lo = mid + 1;
         }
-        idx[zpos[r]] = (uint32_t)(t0 + lo);
+        const uint32_t z = zpos[r];
+        if (z < A.N)
+            idx[z] = (uint32_t)(t0 + lo);
     }
     __syncthreads();
 }

@@ -1213,6 +1213,23 @@ float mo_smc_logg(int model, const double *th, float obs, float x)
     return ((-0.5f * q) - (0.5f * x)) + m.cst;
 }
 
+/* (hi, lo) += d in double-double: two-sum of the high parts, then the
+ * low parts, renormalised (the device's dd_add with a zero low part) */
+static void mo_dd_add_d(double *hi, double *lo, double d)
+{
+    const double s = *hi + d;
+    const double bb = s - *hi;
+    const double err = (*hi - (s - bb)) + (d - bb);
+    const double l = err + *lo;
+    const double h = s + l;
+    *lo = l - (h - s);
+    *hi = h;
+}
+
+/* particles per tile of the device samplers: the weights are scaled per
+ * tile (see below), so the oracle uses the same tiling */
+#define MO_SMC_TILE 2048
+
 /* kmom > 0: the selection estimator of x^1..x^kmom after each step's move
  * and weighting (SMCEstimator at step 0, algorithm/smc.hpp:89-118,
  * :389-394): est[(t-1) kmom + j-1] = sum_i W_i x_i^j, W_i = e_i * (1/S) as
@@ -1226,8 +1243,9 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
     float *x = (float *)malloc(N * sizeof(float));
     float *xn = (float *)malloc(N * sizeof(float));
     float *lw = (float *)malloc(N * sizeof(float));
-    float *e = (float *)malloc(N * sizeof(float));
-    double *W = (double *)malloc(N * sizeof(double));
+    double *e = (double *)malloc(N * sizeof(double));
+    float *tmax = (float *)malloc(((N + MO_SMC_TILE - 1) / MO_SMC_TILE + 1) *
+        sizeof(float));
     double *u = (double *)malloc(N * sizeof(double));
     uint64_t *rep = (uint64_t *)malloc(N * sizeof(uint64_t));
     uint64_t *idx = (uint64_t *)malloc(N * sizeof(uint64_t));
@@ -1269,17 +1287,34 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
         float *tmp = x;
         x = xn;
         xn = tmp;
+        /* weights e_i = a_i r_k for particle i of tile k (2048 particles):
+         * a_i = exp_f(lw_i - m_k), r_k = exp_f(m_k - m), m_k the tile max of
+         * lw and m the max over tiles; the product of the two floats is
+         * exact in double */
+        const size_t ntile = (N + MO_SMC_TILE - 1) / MO_SMC_TILE;
         float mx = -INFINITY;
-        for (size_t i = 0; i < N; ++i)
-            if (mx < lw[i])
-                mx = lw[i];
+        for (size_t k = 0; k < ntile; ++k) {
+            float mk = -INFINITY;
+            const size_t b = k * MO_SMC_TILE;
+            const size_t en = b + MO_SMC_TILE < N ? b + MO_SMC_TILE : N;
+            for (size_t i = b; i < en; ++i)
+                if (mk < lw[i])
+                    mk = lw[i];
+            tmax[k] = mk;
+            if (mx < mk)
+                mx = mk;
+        }
 #pragma omp parallel for schedule(static)
-        for (long long i = 0; i < Ns; ++i)
-            e[i] = mo_exp_f(lw[i] - mx);
-        double S = 0, S2 = 0;
+        for (long long i = 0; i < Ns; ++i) {
+            const float mk = tmax[(size_t)i / MO_SMC_TILE];
+            e[i] = (double)mo_exp_f(lw[i] - mk) * (double)mo_exp_f(mk - mx);
+        }
+        /* S: the total rounded once (the device sums in double-double);
+         * S2 in plain double */
+        double S = 0, Slo = 0, S2 = 0;
         for (size_t i = 0; i < N; ++i) {
-            const double d = (double)e[i];
-            S += d;
+            const double d = e[i];
+            mo_dd_add_d(&S, &Slo, d);
             S2 += d * d;
         }
         logz += ((double)mx + log(S)) - log(C);
@@ -1291,7 +1326,7 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
                     double p = 1.0;
                     for (int q = 0; q < j; ++q)
                         p = p * (double)x[i];
-                    acc += ((double)e[i] * inv) * p;
+                    acc += (e[i] * inv) * p;
                 }
                 est[(t - 1) * (size_t)kmom + (size_t)(j - 1)] = acc;
             }
@@ -1306,8 +1341,6 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
             resampled_hist[t - 1] = resampled;
         if (resampled) {
             const double inv = 1.0 / S;
-            for (size_t i = 0; i < N; ++i)
-                W[i] = (double)e[i] * inv;
             const double delta = 1.0 / (double)N;
             if (scheme == MO_SYSTEMATIC) {
                 const double uu = mo_smc_uniform(seed, t, 0) * delta;
@@ -1323,7 +1356,29 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
                     u[j] = 1.0 - exp(lm);
                 }
             }
-            mo_resample_trans_u01_rep(N, N, W, u, rep);
+            /* counts from the cumulative weights rounded once: the device
+             * sums the weights in double-double (within tiles, then over
+             * tile offsets), so its cumulative weight of particle i is the
+             * double nearest the exact prefix; a double-double running sum
+             * gives the same double here (up to ties at 2^-104).
+             * cnt_i = #{u_j < fl(P_i * (1/S))}, the last particle takes N. */
+            {
+                double hi = 0.0, lo = 0.0;
+                size_t j = 0;
+                uint64_t prevc = 0;
+                for (size_t i = 0; i < N; ++i) {
+                    mo_dd_add_d(&hi, &lo, e[i]);
+                    uint64_t c = N;
+                    if (i + 1 < N) {
+                        const double Si = hi * inv;
+                        while (j < N && u[j] < Si)
+                            ++j;
+                        c = j;
+                    }
+                    rep[i] = c - prevc;
+                    prevc = c;
+                }
+            }
             mo_resample_trans_rep_index(N, N, rep, idx);
         }
     }
@@ -1331,7 +1386,7 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
     free(xn);
     free(lw);
     free(e);
-    free(W);
+    free(tmax);
     free(u);
     free(rep);
     free(idx);

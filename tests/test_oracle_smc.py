@@ -91,13 +91,14 @@ def test_smc_oracle_vs_kalman():
     y = np.load(os.path.join(GOLDEN, "observations.npz"))["lg_y"]
     exact = kalman.loglik(y, 0.9, 1.0, 0.5)
     ests = [orc.smc_run(orc.MODEL_LG, (0.9, 1.0, 0.5), y, 1 << 14, seed,
-                        0.5, orc.SYSTEMATIC, nthreads=4) for seed in range(1, 9)]
+                        0.5, orc.SYSTEMATIC, nthreads=4) for seed in range(1, 17)]
     ests = np.array(ests)
     sd = ests.std(ddof=1)
     # log Z-hat has a small negative bias (about var/2); allow 4 standard
-    # errors plus that bias
+    # errors plus that bias.  Its spread at N = 2^14 is about 0.19 (40
+    # seeds); 16 seeds estimate it to within about 0.035.
     assert abs(ests.mean() - exact) < 4 * sd / math.sqrt(len(ests)) + sd * sd
-    assert sd < 0.2
+    assert sd < 0.3
 
 
 @pytest.mark.parametrize("scheme", [orc.SYSTEMATIC, orc.STRATIFIED,
