@@ -292,10 +292,13 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
                     const uint32_t kk = (uint32_t)(k < nt ? k : nt);
                     const uint64_t i0 = tile_first(kk, gt);
                     double csum = 0.0, dummy = 0.0;
+                    float ep[2] = {0.0f, 0.0f}; // the normal pair of i, i^1
                     for (int q = 0; q < kItems; ++q) {
                         const uint64_t i = i0 + q;
                         if (i >= A.N)
                             break;
+                        if ((q & 1) == 0)
+                            smc_normal_pair(A.k0, A.k1, i >> 1, t, ep[0], ep[1]);
                         const double Wt = exp(cur.lw[i] - lse_t);
                         const double Wc = Rprev ? invN : exp(prev.lw[i] - lse_p);
                         const double lwbar = lwb[i] + Wt;
@@ -308,7 +311,7 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
                             Rprev ? prev.par[i]
                                   : (uint32_t)i;
                         ta += xbar * prev.x[par];
-                        tsg += xbar * eps_at(A, i, t);
+                        tsg += xbar * (double)ep[q & 1];
                         const double xpbar = A.a * xbar;
                         if (Rprev) {
                             xpb[i] = xpbar;

@@ -104,28 +104,36 @@ __device__ __forceinline__ double gtile_move(const GradArgs &G, uint32_t t,
     const uint64_t i0 = tile_first(tile, gt);
     const double y = G.y[t - 1];
     double lmax = -__builtin_huge_val();
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = i0 + k;
-        if (i >= G.N)
+    // particles i0.. are consecutive from an even index: one normal pair per
+    // two particles (eps_at's values, each pair drawn once)
+    for (int k = 0; k < kItems; k += 2) {
+        if (i0 + k >= G.N)
             break;
-        double xp, carry;
-        if (Rprev) {
-            const uint32_t p = tpar ? tpar[i]
-                                    : prev.par[i];
-            xp = prev.x[p];
-            // value of the carried weight: LWn(theta)[p] - LWn(theta0)[p]
-            carry = lwn_ref ? (prev.lw[p] - prev.scal[2]) - lwn_ref[p] : 0.0;
-        } else {
-            xp = prev.x[i];
-            carry = prev.lw[i] - mprev;
+        float e[2];
+        smc_normal_pair(G.k0, G.k1, (i0 + k) >> 1, t, e[0], e[1]);
+        for (int q = 0; q < 2; ++q) {
+            const uint64_t i = i0 + k + q;
+            if (i >= G.N)
+                break;
+            double xp, carry;
+            if (Rprev) {
+                const uint32_t p = tpar ? tpar[i] : prev.par[i];
+                xp = prev.x[p];
+                // value of the carried weight: LWn(theta)[p] - LWn(theta0)[p]
+                carry =
+                    lwn_ref ? (prev.lw[p] - prev.scal[2]) - lwn_ref[p] : 0.0;
+            } else {
+                xp = prev.x[i];
+                carry = prev.lw[i] - mprev;
+            }
+            const double x = G.a * xp + G.s * (double)e[q];
+            double g, gx, gc;
+            gmodel(G, y, x, g, gx, gc);
+            const double l = carry + g;
+            cur.x[i] = x;
+            cur.lw[i] = l;
+            lmax = lmax < l ? l : lmax;
         }
-        const double x = G.a * xp + G.s * eps_at(G, i, t);
-        double g, gx, gc;
-        gmodel(G, y, x, g, gx, gc);
-        const double l = carry + g;
-        cur.x[i] = x;
-        cur.lw[i] = l;
-        lmax = lmax < l ? l : lmax;
     }
     // group max of doubles
 #pragma unroll
