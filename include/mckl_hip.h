@@ -278,7 +278,9 @@ int mckl_hip_resample_index_read_matrix(const mckl_hip_resample_index *h,
  * built-in models, FP32 states and log-weights, FP64 reductions and scans.
  * The handle owns its device buffers; run() is asynchronous and contains no
  * host synchronisation (the ESS test is taken on the device), so it can be
- * captured into a HIP graph.
+ * captured into a HIP graph.  Schemes (SMCSampler::resample_method,
+ * smc.hpp:260-282): multinomial, stratified, systematic, residual stratified
+ * and residual systematic; residual multinomial returns MCKL_HIP_EINVAL.
  */
 typedef struct mckl_hip_smc mckl_hip_smc;
 int mckl_hip_smc_create(int model, const double *theta, size_t T,

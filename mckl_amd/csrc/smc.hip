@@ -26,6 +26,7 @@ namespace mckl_hip {
 // one that writes the new ones.
 struct SmcState {
     double logz;
+    double S;         // the step's weight total rounded once (residual steps)
     double C[2];      // normaliser of the carried weights for the next step
     float m[2];       // max log weight of the step
     int resampled[2]; // the step resampled (applies to the next move)
@@ -57,6 +58,8 @@ struct RsBuf {
     double *tmh, *tml; // multinomial: tile dd totals of the log spacings
     double *u;         // multinomial: the step's sorted uniforms
     double *uh, *ul;   // multinomial: scaled tile dd totals r T
+    uint32_t *ti;      // residual: tile sums of the integral parts
+    double *trh, *trl; // residual: tile dd totals of the residuals
 };
 
 // move + weight of a tile, then the tile's sums of a_i = exp_f(lw_i - m_k)
@@ -191,6 +194,22 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     const double next_hi = x[2];
     __syncthreads();
     const double invS = __ddiv_rn(1.0, total.hi);
+    if (A.scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
+        A.scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC) {
+        // residual: the tile's integral parts and residual total; the
+        // counts follow in k_rs_rcounts
+        uint32_t isum;
+        dd rt;
+        tile_residual_sums(A, k, l, tmax[k], rk, invS, gt, L.d, L.u, isum, rt);
+        if (gt == 0) {
+            B.ti[k] = isum;
+            B.trh[k] = rt.hi;
+            B.trl[k] = rt.lo;
+            if (k == 0)
+                st->S = total.hi;
+        }
+        return;
+    }
     const UniformSeq U = make_uniform_seq(A, t);
     uint32_t es, zs;
     tile_counts_l(A, k, l, tmax[k], rk, invS, toff, next_hi, U, B.cnt, gt,
@@ -237,6 +256,80 @@ __global__ __launch_bounds__(kGroup) void k_rs_mcounts(SmcArgs A, uint32_t t,
     uint32_t es, zs;
     tile_counts(A, k, lw, tmax[k], tile_scale(tmax[k], m), invS, off, next.hi,
         U, B.cnt, gt, L.d, L.u, es, zs);
+    if (gt == 0) {
+        B.te[k] = es;
+        B.tz[k] = zs;
+    }
+}
+
+// residual steps: R = M - sum of the integral parts, the residual total and
+// the tile's offsets (tiles_offset_dd's operations on the chunk sums), then
+// the cumulative counts of the tile (tile_rcounts)
+__global__ __launch_bounds__(kGroup) void k_rs_rcounts(SmcArgs A, uint32_t t,
+    const float *lw, const float *tmax, const SmcState *st, RsBuf B)
+{
+    const int c = t & 1;
+    if (!st->resampled[c])
+        return;
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    const GroupLds L = group_lds(lds);
+    const int gt = threadIdx.x;
+    const int nt = (int)A.ntiles;
+    const uint32_t k = blockIdx.x;
+    float l[kItems];
+    load8(lw, tile_first(k, gt), A.N, l, 0.0f);
+    const float m = st->m[c];
+    const double invS = __ddiv_rn(1.0, st->S);
+    const int chunk = (nt + kGroup - 1) / kGroup;
+    const int b = gt * chunk;
+    const int e = b + chunk < nt ? b + chunk : nt;
+    uint32_t isum = 0, ioff = 0;
+    dd loc{0.0, 0.0};
+    for (int i = b; i < e; ++i) {
+        isum += B.ti[i];
+        ioff += i < (int)k ? B.ti[i] : 0u;
+        loc = dd_add(loc, dd{B.trh[i], B.trl[i]});
+    }
+    group_sum2_u(isum, ioff, gt, L.u);
+    const uint32_t R = isum <= A.N ? A.N - isum : 0u;
+    dd total;
+    const dd off = group_excl_scan_dd(loc, gt, L.d, total);
+    double *x = L.d + 8;
+    {
+        dd run = off;
+        for (int i = b; i < e; ++i) {
+            if (i == (int)k) {
+                x[0] = run.hi;
+                x[1] = run.lo;
+            }
+            if (i == (int)k + 1)
+                x[2] = run.hi;
+            run = dd_add(run, dd{B.trh[i], B.trl[i]});
+        }
+        if (gt == 0 && (int)k + 1 == nt)
+            x[2] = total.hi;
+    }
+    __syncthreads();
+    const dd roff{x[0], x[1]};
+    const double rnext_hi = x[2];
+    __syncthreads();
+    UniformSeq U;
+    U.u = nullptr;
+    U.scheme = A.scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC ? MCKL_HIP_SYSTEMATIC
+                                                        : MCKL_HIP_STRATIFIED;
+    U.k0 = A.k0;
+    U.k1 = A.k1;
+    U.t = t;
+    U.M = R;
+    U.delta = R ? __ddiv_rn(1.0, (double)R) : 0.0;
+    U.u0d = U.scheme == MCKL_HIP_SYSTEMATIC
+        ? __dmul_rn(smc_uniform(A.k0, A.k1, t, 0), U.delta)
+        : 0.0;
+    const double invR = R ? __ddiv_rn(1.0, total.hi) : 0.0;
+    const float mk = tmax[k];
+    uint32_t es, zs;
+    tile_rcounts(A, k, l, mk, tile_scale(mk, m), invS, ioff, roff, rnext_hi,
+        invR, U, B.cnt, gt, L.d, L.u, es, zs);
     if (gt == 0) {
         B.te[k] = es;
         B.tz[k] = zs;
@@ -590,16 +683,21 @@ static void obs_const(int model, const double *th, const double *y, size_t T,
         out[t] = model == 1 ? (float)y[t] : (float)(y[t] * y[t] / (th[2] * th[2]));
 }
 
-static int check_common(size_t N, size_t T, double threshold, int scheme)
+// residual: the plain sampler also takes residual stratified and residual
+// systematic resampling
+static int check_common(size_t N, size_t T, double threshold, int scheme,
+    bool residual = false)
 {
     if (N == 0 || N >= (size_t)0x7FFFFFFF)
         return set_error(MCKL_HIP_EINVAL, "particle count %zu out of range", N);
     if (T == 0)
         return set_error(MCKL_HIP_EINVAL, "T must be > 0");
     if (scheme != MCKL_HIP_MULTINOMIAL && scheme != MCKL_HIP_STRATIFIED &&
-        scheme != MCKL_HIP_SYSTEMATIC)
-        return set_error(MCKL_HIP_EINVAL, "unknown resampling scheme %d",
-            scheme);
+        scheme != MCKL_HIP_SYSTEMATIC &&
+        !(residual && (scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
+                          scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC)))
+        return set_error(MCKL_HIP_EINVAL,
+            "resampling scheme %d is not provided by this sampler", scheme);
     if (std::isnan(threshold))
         return set_error(MCKL_HIP_EINVAL, "threshold is NaN");
     return MCKL_HIP_OK;
@@ -641,8 +739,8 @@ static void smc_free(mckl_hip_smc *s)
     smc_drop_graph(s);
     void *ptrs[] = {s->x[0], s->x[1], s->lw, s->obs, s->tmax, s->idx, s->ts,
         s->ts2, s->B.tdh, s->B.tdl, s->B.te, s->B.tz, s->B.cnt, s->B.Eincl,
-        s->B.zpos, s->B.tmh, s->B.tml, s->B.u, s->B.uh, s->B.ul, s->st, s->ess,
-        s->res, s->eph, s->epl, s->est};
+        s->B.zpos, s->B.tmh, s->B.tml, s->B.u, s->B.uh, s->B.ul, s->B.ti,
+        s->B.trh, s->B.trl, s->st, s->ess, s->res, s->eph, s->epl, s->est};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -658,7 +756,7 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
     if (!out || !h_y)
         return set_error(MCKL_HIP_EINVAL, "NULL argument");
     *out = nullptr;
-    int rc = check_common(N, T, threshold, scheme);
+    int rc = check_common(N, T, threshold, scheme, true);
     if (rc)
         return rc;
     ModelConst mc;
@@ -701,6 +799,12 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
     MCKL_ALLOC(s->B.cnt, Np);
     MCKL_ALLOC(s->B.Eincl, Np);
     MCKL_ALLOC(s->B.zpos, Np);
+    if (scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
+        scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC) {
+        MCKL_ALLOC(s->B.ti, nt);
+        MCKL_ALLOC(s->B.trh, nt);
+        MCKL_ALLOC(s->B.trl, nt);
+    }
     if (scheme == MCKL_HIP_MULTINOMIAL) {
         MCKL_ALLOC(s->B.tmh, nt);
         MCKL_ALLOC(s->B.tml, nt);
@@ -749,6 +853,10 @@ static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
         if (s->A.scheme == MCKL_HIP_MULTINOMIAL) {
             k_rs_uniforms<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B);
             k_rs_mcounts<<<grid, kGroup, 0, st>>>(
+                s->A, t, s->lw, s->tmax, s->st, s->B);
+        } else if (s->A.scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
+            s->A.scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC) {
+            k_rs_rcounts<<<grid, kGroup, 0, st>>>(
                 s->A, t, s->lw, s->tmax, s->st, s->B);
         }
         k_rs_index<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);

@@ -504,6 +504,139 @@ __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
         ldsd, ldsu, esum, zsum);
 }
 
+// Residual resampling in the samplers (ResampleAlgorithm<U01Seq, true>,
+// resample.hpp:333-364 and 515-593).  With the normalised weight
+// W_i = fl(e_i * (1/S)), M W_i = I_i + r_i (modf: I_i integral, r_i in
+// [0, 1), both exact); R = M - sum I.  Particle i keeps I_i copies and the
+// remaining R are drawn with the base scheme's R uniforms against the
+// cumulative residuals normalised by their total (double-double prefix
+// rounded once, as for the plain schemes).
+__device__ __forceinline__ void resid_part(
+    double e, double invS, double M, uint32_t &I, double &r)
+{
+    const double w = __dmul_rn(M, __dmul_rn(e, invS));
+    const double fw = floor(w);
+    I = (uint32_t)fw;
+    r = __dsub_rn(w, fw);
+}
+
+// the tile's sum of I_i and double-double total of r_i
+__device__ __forceinline__ void tile_residual_sums(const SmcArgs &A,
+    uint32_t tile, const float (&l)[kItems], float mk, double rk, double invS,
+    int gt, double *ldsd, uint32_t *ldsu, uint32_t &isum, dd &rtot)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    const double M = (double)A.N;
+    uint32_t is = 0;
+    dd loc{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (i0 + k < A.N) {
+            const double e =
+                __dmul_rn((double)smc_exp_f(__fsub_rn(l[k], mk)), rk);
+            uint32_t I;
+            double r;
+            resid_part(e, invS, M, I, r);
+            is += I;
+            loc = dd_add(loc, dd{r, 0.0});
+        }
+    }
+    uint32_t zero = 0;
+    group_sum2_u(is, zero, gt, ldsu);
+    isum = is;
+    (void)group_excl_scan_dd(loc, gt, ldsd, rtot);
+}
+
+// Cumulative counts of a residual step: cnt_i = (sum_{j <= i} I_j) +
+// (residual count of i), the residual counts #{u_j < P^r_i (1/Rtot)} over
+// the R uniforms of U (U.M = R), with P^r the double-double prefix of the
+// residuals (roff: the tile's offset, rnext_hi: the next tile's) and the
+// same boundary clamp as tile_counts_l.  Ioff: sum of I over the tiles
+// before this one.  Returns the tile's extra-copy and zero-slot counts.
+__device__ __forceinline__ void tile_rcounts(const SmcArgs &A, uint32_t tile,
+    const float (&l)[kItems], float mk, double rk, double invS, uint32_t Ioff,
+    dd roff, double rnext_hi, double invR, const UniformSeq &U, uint32_t *cnt,
+    int gt, double *ldsd, uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    const uint64_t tend = (uint64_t)(tile + 1) * kTile < A.N
+        ? (uint64_t)(tile + 1) * kTile
+        : A.N;
+    const double M = (double)A.N;
+    const uint32_t R = U.M;
+    const uint32_t clo = (tile == 0 || R == 0)
+        ? 0u
+        : U.count_fast(__dmul_rn(roff.hi, invR));
+    const uint32_t chi = tend == A.N
+        ? R
+        : (R == 0 ? 0u : U.count_fast(__dmul_rn(rnext_hi, invR)));
+    uint32_t I[kItems];
+    double r[kItems];
+    uint32_t is = 0;
+    dd loc{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        I[k] = 0;
+        r[k] = 0.0;
+        if (i0 + k < A.N) {
+            const double e =
+                __dmul_rn((double)smc_exp_f(__fsub_rn(l[k], mk)), rk);
+            resid_part(e, invS, M, I[k], r[k]);
+        }
+        is += I[k];
+        loc = dd_add(loc, dd{r[k], 0.0});
+    }
+    dd total;
+    const dd toff = group_excl_scan_dd(loc, gt, ldsd, total);
+    uint32_t itot;
+    const uint32_t iex = group_excl_scan_u(is, gt, ldsu, itot);
+    uint32_t cr[kItems];
+    dd run = toff;
+    uint32_t cmax = clo;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        run = dd_add(run, dd{r[k], 0.0});
+        cr[k] = 0;
+        if (i < A.N) {
+            uint32_t v;
+            if (i == tend - 1)
+                v = chi;
+            else if (R == 0)
+                v = 0;
+            else
+                v = U.count_fast(__dmul_rn(dd_add(roff, run).hi, invR));
+            v = v < clo ? clo : (v > chi ? chi : v);
+            cmax = v > cmax ? v : cmax;
+            cr[k] = cmax;
+        }
+    }
+    const uint32_t before = group_excl_max_u(cmax, clo, gt, ldsu);
+    uint32_t ic = Ioff + iex;
+    uint32_t prev = ic + before; // count of the particle before i0
+    uint32_t c[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        ic += I[k];
+        const uint32_t q = cr[k] < before ? before : cr[k];
+        c[k] = ic + q;
+    }
+    store8(cnt, i0, A.N, c);
+    uint32_t es = 0, zs = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (i0 + k < A.N) {
+            const uint32_t rep = c[k] - prev;
+            es += rep > 1 ? rep - 1 : 0;
+            zs += rep == 0 ? 1 : 0;
+        }
+        prev = c[k];
+    }
+    group_sum2_u(es, zs, gt, ldsu);
+    esum = es;
+    zsum = zs;
+}
+
 // Stable rep->index (resample.hpp:416-466, M = N): idx_i = i when rep_i > 0;
 // zero slots take extra copies in ascending source order.  Writes the
 // inclusive extra-copy prefix E_i, idx_i = i for every slot, and

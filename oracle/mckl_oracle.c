@@ -1244,6 +1244,7 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
     float *xn = (float *)malloc(N * sizeof(float));
     float *lw = (float *)malloc(N * sizeof(float));
     double *e = (double *)malloc(N * sizeof(double));
+    double *resid = (double *)malloc(N * sizeof(double));
     float *tmax = (float *)malloc(((N + MO_SMC_TILE - 1) / MO_SMC_TILE + 1) *
         sizeof(float));
     double *u = (double *)malloc(N * sizeof(double));
@@ -1339,7 +1340,54 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
         resampled = ess < threshold * (double)N;
         if (resampled_hist)
             resampled_hist[t - 1] = resampled;
-        if (resampled) {
+        if (resampled && scheme >= MO_RESIDUAL_STRATIFIED) {
+            /* residual stratified / systematic (resample.hpp:333-364,
+             * 515-593): W_i = e_i (1/S), N W_i = I_i + r_i (modf), R = N -
+             * sum I; the R uniforms of the base scheme against the
+             * residuals' prefix (rounded once) over their total (rounded
+             * once), I_i copies kept.  rep holds I until the counts. */
+            const double inv = 1.0 / S;
+            uint64_t Isum = 0;
+            double th = 0.0, tl = 0.0;
+            for (size_t i = 0; i < N; ++i) {
+                const double w = (double)N * (e[i] * inv);
+                const double fw = floor(w);
+                rep[i] = (uint64_t)fw;
+                resid[i] = w - fw;
+                Isum += rep[i];
+                mo_dd_add_d(&th, &tl, resid[i]);
+            }
+            const uint64_t R = Isum <= N ? N - Isum : 0;
+            const double dR = R ? 1.0 / (double)R : 0.0;
+            if (scheme == MO_RESIDUAL_SYSTEMATIC) {
+                const double uu = mo_smc_uniform(seed, t, 0) * dR;
+                for (size_t j = 0; j < R; ++j)
+                    u[j] = (double)j * dR + uu;
+            } else {
+                for (size_t j = 0; j < R; ++j)
+                    u[j] = dR * (mo_smc_uniform(seed, t, j) + (double)j);
+            }
+            const double invR = R ? 1.0 / th : 0.0;
+            double hi = 0.0, lo = 0.0;
+            size_t j = 0;
+            uint64_t prevc = 0;
+            for (size_t i = 0; i < N; ++i) {
+                mo_dd_add_d(&hi, &lo, resid[i]);
+                uint64_t c = R;
+                if (i + 1 < N) {
+                    c = 0;
+                    if (R) {
+                        const double Si = hi * invR;
+                        while (j < R && u[j] < Si)
+                            ++j;
+                        c = j;
+                    }
+                }
+                rep[i] += c - prevc;
+                prevc = c;
+            }
+            mo_resample_trans_rep_index(N, N, rep, idx);
+        } else if (resampled) {
             const double inv = 1.0 / S;
             const double delta = 1.0 / (double)N;
             if (scheme == MO_SYSTEMATIC) {
@@ -1386,6 +1434,7 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
     free(xn);
     free(lw);
     free(e);
+    free(resid);
     free(tmax);
     free(u);
     free(rep);

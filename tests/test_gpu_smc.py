@@ -21,6 +21,7 @@ import oracle as orc
 pytestmark = pytest.mark.gpu
 
 MULT, STRAT, SYS = orc.MULTINOMIAL, orc.STRATIFIED, orc.SYSTEMATIC
+RSTRAT, RSYS = orc.RESIDUAL_STRATIFIED, orc.RESIDUAL_SYSTEMATIC
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -46,6 +47,11 @@ CASES = [
     (1, (0.9, 1.0, 0.5), 4096, 11, MULT),
     (2, (0.95, 0.2, 0.7), 5000, 12, MULT),
     (1, (0.9, 1.0, 0.5), 2049, 13, MULT),
+    # residual stratified / systematic (plain sampler only)
+    (1, (0.9, 1.0, 0.5), 4096, 14, RSTRAT),
+    (2, (0.95, 0.2, 0.7), 70001, 15, RSTRAT),
+    (1, (0.9, 1.0, 0.5), 5000, 16, RSYS),
+    (2, (0.95, 0.2, 0.7), 3, 17, RSYS),
 ]
 
 
@@ -179,6 +185,32 @@ def test_smc_errors(dev):
         m.SMCSampler(2, (1.5, 0.2, 0.7), np.zeros(3), 100, 1)
     with pytest.raises(m.MCKLError):
         m.SMCSampler(1, (0.9, 1.0, 0.5), np.zeros(3), 100, 1, 0.5, 7)
+    # residual multinomial is not provided by the samplers; the batched
+    # sampler takes the three plain schemes only
+    with pytest.raises(m.MCKLError):
+        m.SMCSampler(1, (0.9, 1.0, 0.5), np.zeros(3), 100, 1, 0.5,
+                     orc.RESIDUAL)
+    with pytest.raises(m.MCKLError):
+        m.SMCBatch(1, [100, 200], [(0.9, 1.0, 0.5)] * 2, np.zeros((2, 3)),
+                   [1, 2], 0.5, RSYS)
+
+
+def test_residual_systematic_equals_systematic(dev):
+    """Residual systematic resampling gives the replication numbers of
+    systematic resampling with the same uniform (sum_k<=i I_k is an
+    integer, so floor(N P_i - u) splits into the integral parts and the
+    residual count): the samplers agree run for run, at any size."""
+    m = _m()
+    y = _obs()["lg_y"][:40]
+    for N in (3, 4096, 20011):
+        a = m.SMCSampler(1, (0.9, 1.0, 0.5), y, N, 21, 0.5, SYS)
+        b = m.SMCSampler(1, (0.9, 1.0, 0.5), y, N, 21, 0.5, RSYS)
+        a.iterate()
+        b.iterate()
+        la, ea, ra = a.results()
+        lb, eb, rb = b.results()
+        assert np.array_equal(ra, rb)
+        assert abs(la - lb) < 1e-10 * abs(la), (N, la, lb)
 
 
 @pytest.mark.parametrize("N,T,scheme", [((1 << 21) + 77, 8, SYS),

@@ -102,7 +102,8 @@ def test_smc_oracle_vs_kalman():
 
 
 @pytest.mark.parametrize("scheme", [orc.SYSTEMATIC, orc.STRATIFIED,
-                                    orc.MULTINOMIAL])
+                                    orc.MULTINOMIAL, orc.RESIDUAL_STRATIFIED,
+                                    orc.RESIDUAL_SYSTEMATIC])
 def test_smc_oracle_schemes_consistent(scheme):
     y = np.load(os.path.join(GOLDEN, "observations.npz"))["lg_y"][:40]
     exact = kalman.loglik(y, 0.9, 1.0, 0.5)
@@ -141,3 +142,18 @@ def test_smc_oracle_moments_vs_kalman():
         # the estimator does not disturb the run
         assert ll == orc.smc_run(orc.MODEL_LG, th, y, N, seed, 0.5,
                                  orc.SYSTEMATIC, nthreads=4)
+
+
+def test_residual_counts_closed_form():
+    """Residual steps of the oracle sampler keep floor(N W_i) copies of each
+    particle: with two particles of equal weight (N = 2) every step keeps
+    one copy of each, so the sampler never moves a particle (x_t follows
+    its own path) -- checked through the ESS history, which stays at N for
+    the equal-weight steps a flat likelihood gives."""
+    y = np.zeros(6)
+    # sy huge: the likelihood is flat, the weights stay equal, ESS = N
+    for scheme in (orc.RESIDUAL_STRATIFIED, orc.RESIDUAL_SYSTEMATIC):
+        ll, ess, res = orc.smc_run(orc.MODEL_LG, (0.9, 1.0, 1e6), y, 2, 3,
+                                   1.01, scheme, history=True)
+        assert res.sum() == 6
+        assert np.allclose(ess, 2.0, rtol=1e-9)
