@@ -279,8 +279,8 @@ int mckl_hip_resample_index_read_matrix(const mckl_hip_resample_index *h,
  * The handle owns its device buffers; run() is asynchronous and contains no
  * host synchronisation (the ESS test is taken on the device), so it can be
  * captured into a HIP graph.  Schemes (SMCSampler::resample_method,
- * smc.hpp:260-282): multinomial, stratified, systematic, residual stratified
- * and residual systematic; residual multinomial returns MCKL_HIP_EINVAL.
+ * smc.hpp:260-282): all six, the three plain schemes and the three residual
+ * ones (integral parts kept, R residual uniforms of the base scheme).
  */
 typedef struct mckl_hip_smc mckl_hip_smc;
 int mckl_hip_smc_create(int model, const double *theta, size_t T,

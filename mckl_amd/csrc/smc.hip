@@ -103,6 +103,13 @@ __device__ __forceinline__ float tiles_max(
 // tile's log weights, its chunk of every per-tile array), so the kernel
 // waits on memory once; the reductions are those of tiles_max, tiles_sum2
 // and tiles_offset_dd, operation for operation.
+__host__ __device__ inline bool is_residual(int scheme)
+{
+    return scheme == MCKL_HIP_RESIDUAL ||
+        scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
+        scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC;
+}
+
 __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     const float *lw, const float *tmax, const double *ts, const double *ts2,
     RsBuf B, SmcState *st, double *ess_hist, int32_t *res_hist)
@@ -194,10 +201,10 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     const double next_hi = x[2];
     __syncthreads();
     const double invS = __ddiv_rn(1.0, total.hi);
-    if (A.scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
-        A.scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC) {
+    if (is_residual(A.scheme)) {
         // residual: the tile's integral parts and residual total; the
-        // counts follow in k_rs_rcounts
+        // counts follow in k_rs_rcounts (after k_rs_rterms and
+        // k_rs_uniforms for residual multinomial)
         uint32_t isum;
         dd rt;
         tile_residual_sums(A, k, l, tmax[k], rk, invS, gt, L.d, L.u, isum, rt);
@@ -220,6 +227,35 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     }
 }
 
+// R = M - sum of the tile sums of the integral parts (all threads)
+__device__ __forceinline__ uint32_t residual_R(
+    const SmcArgs &A, const uint32_t *ti, int gt, uint32_t *ldsu)
+{
+    uint32_t is = 0, zero = 0;
+    for (uint32_t i = gt; i < A.ntiles; i += kGroup)
+        is += ti[i];
+    group_sum2_u(is, zero, gt, ldsu);
+    return is <= A.N ? A.N - is : 0u;
+}
+
+// residual multinomial: tile totals of the log spacings of the R uniforms
+__global__ __launch_bounds__(kGroup) void k_rs_rterms(SmcArgs A, uint32_t t,
+    const SmcState *st, RsBuf B)
+{
+    if (!st->resampled[t & 1])
+        return;
+    __shared__ __align__(16) unsigned char lds[kScratch];
+    const GroupLds L = group_lds(lds);
+    const uint32_t k = blockIdx.x;
+    const uint32_t R = residual_R(A, B.ti, threadIdx.x, L.u);
+    dd mt;
+    tile_mterms(A, k, t, threadIdx.x, L.d, mt, R);
+    if (threadIdx.x == 0) {
+        B.tmh[k] = mt.hi;
+        B.tml[k] = mt.lo;
+    }
+}
+
 // multinomial: each tile's sorted uniforms from the prefix of log spacings
 __global__ __launch_bounds__(kGroup) void k_rs_uniforms(SmcArgs A, uint32_t t,
     const SmcState *st, RsBuf B)
@@ -229,10 +265,13 @@ __global__ __launch_bounds__(kGroup) void k_rs_uniforms(SmcArgs A, uint32_t t,
     __shared__ __align__(16) unsigned char lds[kScratch];
     const GroupLds L = group_lds(lds);
     const uint32_t k = blockIdx.x;
+    const uint32_t M = is_residual(A.scheme)
+        ? residual_R(A, B.ti, threadIdx.x, L.u)
+        : A.N;
     dd off, next;
     tiles_offset_dd(B.tmh, B.tml, (int)A.ntiles, (int)k, (int)k + 1,
         threadIdx.x, L.d, off, next);
-    tile_muniforms(A, k, t, off, B.u, threadIdx.x, L.d);
+    tile_muniforms(A, k, t, off, B.u, threadIdx.x, L.d, M);
 }
 
 // multinomial: counts of the sorted uniforms below each cumulative weight
@@ -314,9 +353,10 @@ __global__ __launch_bounds__(kGroup) void k_rs_rcounts(SmcArgs A, uint32_t t,
     const double rnext_hi = x[2];
     __syncthreads();
     UniformSeq U;
-    U.u = nullptr;
+    U.u = A.scheme == MCKL_HIP_RESIDUAL ? B.u : nullptr;
     U.scheme = A.scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC ? MCKL_HIP_SYSTEMATIC
-                                                        : MCKL_HIP_STRATIFIED;
+        : A.scheme == MCKL_HIP_RESIDUAL_STRATIFIED      ? MCKL_HIP_STRATIFIED
+                                                        : MCKL_HIP_MULTINOMIAL;
     U.k0 = A.k0;
     U.k1 = A.k1;
     U.t = t;
@@ -683,8 +723,7 @@ static void obs_const(int model, const double *th, const double *y, size_t T,
         out[t] = model == 1 ? (float)y[t] : (float)(y[t] * y[t] / (th[2] * th[2]));
 }
 
-// residual: the plain sampler also takes residual stratified and residual
-// systematic resampling
+// residual: the plain sampler also takes the three residual schemes
 static int check_common(size_t N, size_t T, double threshold, int scheme,
     bool residual = false)
 {
@@ -694,7 +733,8 @@ static int check_common(size_t N, size_t T, double threshold, int scheme,
         return set_error(MCKL_HIP_EINVAL, "T must be > 0");
     if (scheme != MCKL_HIP_MULTINOMIAL && scheme != MCKL_HIP_STRATIFIED &&
         scheme != MCKL_HIP_SYSTEMATIC &&
-        !(residual && (scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
+        !(residual && (scheme == MCKL_HIP_RESIDUAL ||
+                          scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
                           scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC)))
         return set_error(MCKL_HIP_EINVAL,
             "resampling scheme %d is not provided by this sampler", scheme);
@@ -799,13 +839,12 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
     MCKL_ALLOC(s->B.cnt, Np);
     MCKL_ALLOC(s->B.Eincl, Np);
     MCKL_ALLOC(s->B.zpos, Np);
-    if (scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
-        scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC) {
+    if (is_residual(scheme)) {
         MCKL_ALLOC(s->B.ti, nt);
         MCKL_ALLOC(s->B.trh, nt);
         MCKL_ALLOC(s->B.trl, nt);
     }
-    if (scheme == MCKL_HIP_MULTINOMIAL) {
+    if (scheme == MCKL_HIP_MULTINOMIAL || scheme == MCKL_HIP_RESIDUAL) {
         MCKL_ALLOC(s->B.tmh, nt);
         MCKL_ALLOC(s->B.tml, nt);
         MCKL_ALLOC(s->B.u, Np);
@@ -854,8 +893,11 @@ static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
             k_rs_uniforms<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B);
             k_rs_mcounts<<<grid, kGroup, 0, st>>>(
                 s->A, t, s->lw, s->tmax, s->st, s->B);
-        } else if (s->A.scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
-            s->A.scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC) {
+        } else if (is_residual(s->A.scheme)) {
+            if (s->A.scheme == MCKL_HIP_RESIDUAL) {
+                k_rs_rterms<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B);
+                k_rs_uniforms<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B);
+            }
             k_rs_rcounts<<<grid, kGroup, 0, st>>>(
                 s->A, t, s->lw, s->tmax, s->st, s->B);
         }

@@ -749,25 +749,33 @@ __device__ __forceinline__ void tile_scatter_ranks(const SmcArgs &A,
 // lm_j = sum_{i <= j} log(U_i) / (N - i), u_j = 1 - exp(lm_j), ascending.
 // The sum is a double-double prefix over tiles (tile totals here, offsets by
 // tiles_offset_dd/tiles_scan_dd), rounded once per element.
+// M: the number of uniforms (N, or R of a residual step)
 __device__ __forceinline__ void tile_mterms(const SmcArgs &A, uint32_t tile,
-    uint64_t t, int gt, double *ldsd, dd &ttot)
+    uint64_t t, int gt, double *ldsd, dd &ttot, uint32_t M)
 {
     const uint64_t i0 = tile_first(tile, gt);
     dd loc{0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t j = i0 + k;
-        if (j < A.N) {
+        if (j < M) {
             const double a = __ddiv_rn(
-                log(smc_uniform(A.k0, A.k1, t, j)), (double)(A.N - j));
+                log(smc_uniform(A.k0, A.k1, t, j)), (double)(M - j));
             loc = dd_add(loc, dd{a, 0.0});
         }
     }
     (void)group_excl_scan_dd(loc, gt, ldsd, ttot);
 }
 
+__device__ __forceinline__ void tile_mterms(const SmcArgs &A, uint32_t tile,
+    uint64_t t, int gt, double *ldsd, dd &ttot)
+{
+    tile_mterms(A, tile, t, gt, ldsd, ttot, A.N);
+}
+
 __device__ __forceinline__ void tile_muniforms(const SmcArgs &A,
-    uint32_t tile, uint64_t t, dd tile_off, double *u, int gt, double *ldsd)
+    uint32_t tile, uint64_t t, dd tile_off, double *u, int gt, double *ldsd,
+    uint32_t M)
 {
     const uint64_t i0 = tile_first(tile, gt);
     double a[kItems];
@@ -775,9 +783,9 @@ __device__ __forceinline__ void tile_muniforms(const SmcArgs &A,
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t j = i0 + k;
-        a[k] = j < A.N ? __ddiv_rn(log(smc_uniform(A.k0, A.k1, t, j)),
-                             (double)(A.N - j))
-                       : 0.0;
+        a[k] = j < M ? __ddiv_rn(log(smc_uniform(A.k0, A.k1, t, j)),
+                           (double)(M - j))
+                     : 0.0;
         loc = dd_add(loc, dd{a[k], 0.0});
     }
     dd tot;
@@ -787,9 +795,15 @@ __device__ __forceinline__ void tile_muniforms(const SmcArgs &A,
     for (int k = 0; k < kItems; ++k) {
         const uint64_t j = i0 + k;
         run = dd_add(run, dd{a[k], 0.0});
-        if (j < A.N)
+        if (j < M)
             u[j] = __dsub_rn(1.0, exp(dd_add(base, run).hi));
     }
+}
+
+__device__ __forceinline__ void tile_muniforms(const SmcArgs &A,
+    uint32_t tile, uint64_t t, dd tile_off, double *u, int gt, double *ldsd)
+{
+    tile_muniforms(A, tile, t, tile_off, u, gt, ldsd, A.N);
 }
 
 // finalize one step from the reduced sums; returns the resampling flag

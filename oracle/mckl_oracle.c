@@ -1340,9 +1340,9 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
         resampled = ess < threshold * (double)N;
         if (resampled_hist)
             resampled_hist[t - 1] = resampled;
-        if (resampled && scheme >= MO_RESIDUAL_STRATIFIED) {
-            /* residual stratified / systematic (resample.hpp:333-364,
-             * 515-593): W_i = e_i (1/S), N W_i = I_i + r_i (modf), R = N -
+        if (resampled && scheme >= MO_RESIDUAL) {
+            /* residual schemes (resample.hpp:333-364, 515-593):
+             * W_i = e_i (1/S), N W_i = I_i + r_i (modf), R = N -
              * sum I; the R uniforms of the base scheme against the
              * residuals' prefix (rounded once) over their total (rounded
              * once), I_i copies kept.  rep holds I until the counts. */
@@ -1363,6 +1363,13 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
                 const double uu = mo_smc_uniform(seed, t, 0) * dR;
                 for (size_t j = 0; j < R; ++j)
                     u[j] = (double)j * dR + uu;
+            } else if (scheme == MO_RESIDUAL) {
+                /* R sorted uniforms, as the multinomial step draws N */
+                double lm = 0;
+                for (size_t j = 0; j < R; ++j) {
+                    lm += log(mo_smc_uniform(seed, t, j)) / (double)(R - j);
+                    u[j] = 1.0 - exp(lm);
+                }
             } else {
                 for (size_t j = 0; j < R; ++j)
                     u[j] = dR * (mo_smc_uniform(seed, t, j) + (double)j);

@@ -21,7 +21,8 @@ import oracle as orc
 pytestmark = pytest.mark.gpu
 
 MULT, STRAT, SYS = orc.MULTINOMIAL, orc.STRATIFIED, orc.SYSTEMATIC
-RSTRAT, RSYS = orc.RESIDUAL_STRATIFIED, orc.RESIDUAL_SYSTEMATIC
+RES, RSTRAT, RSYS = (orc.RESIDUAL, orc.RESIDUAL_STRATIFIED,
+                     orc.RESIDUAL_SYSTEMATIC)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -47,7 +48,10 @@ CASES = [
     (1, (0.9, 1.0, 0.5), 4096, 11, MULT),
     (2, (0.95, 0.2, 0.7), 5000, 12, MULT),
     (1, (0.9, 1.0, 0.5), 2049, 13, MULT),
-    # residual stratified / systematic (plain sampler only)
+    # residual multinomial / stratified / systematic (plain sampler only)
+    (1, (0.9, 1.0, 0.5), 4096, 18, RES),
+    (2, (0.95, 0.2, 0.7), 70001, 19, RES),
+    (1, (0.9, 1.0, 0.5), 2, 20, RES),
     (1, (0.9, 1.0, 0.5), 4096, 14, RSTRAT),
     (2, (0.95, 0.2, 0.7), 70001, 15, RSTRAT),
     (1, (0.9, 1.0, 0.5), 5000, 16, RSYS),
@@ -162,6 +166,20 @@ def test_multinomial_batched_and_kalman(dev):
     assert abs(ll - exact) < 0.25
 
 
+@pytest.mark.parametrize("scheme", [RES, RSTRAT, RSYS])
+def test_residual_kalman(dev, scheme):
+    """Each residual scheme on the LG configuration at N = 2^18: within
+    Monte Carlo error of the Kalman log-likelihood, and every resampling
+    step keeps the integral parts (checked through the oracle run above at
+    small sizes; here through the estimate)."""
+    m = _m()
+    y = _obs()["lg_y"]
+    exact = kalman.loglik(y, 0.9, 1.0, 0.5)
+    ll = m.smc_loglik(1, (0.9, 1.0, 0.5), y, 1 << 18, 5, 0.5, scheme)
+    print("kalman %.6f scheme %d %.6f" % (exact, scheme, ll))
+    assert abs(ll - exact) < 0.25
+
+
 def test_batched_vs_oracle(dev):
     m = _m()
     ob = _obs()
@@ -185,14 +203,11 @@ def test_smc_errors(dev):
         m.SMCSampler(2, (1.5, 0.2, 0.7), np.zeros(3), 100, 1)
     with pytest.raises(m.MCKLError):
         m.SMCSampler(1, (0.9, 1.0, 0.5), np.zeros(3), 100, 1, 0.5, 7)
-    # residual multinomial is not provided by the samplers; the batched
-    # sampler takes the three plain schemes only
-    with pytest.raises(m.MCKLError):
-        m.SMCSampler(1, (0.9, 1.0, 0.5), np.zeros(3), 100, 1, 0.5,
-                     orc.RESIDUAL)
-    with pytest.raises(m.MCKLError):
-        m.SMCBatch(1, [100, 200], [(0.9, 1.0, 0.5)] * 2, np.zeros((2, 3)),
-                   [1, 2], 0.5, RSYS)
+    # the batched sampler takes the three plain schemes only
+    for sch in (RES, RSYS):
+        with pytest.raises(m.MCKLError):
+            m.SMCBatch(1, [100, 200], [(0.9, 1.0, 0.5)] * 2,
+                       np.zeros((2, 3)), [1, 2], 0.5, sch)
 
 
 def test_residual_systematic_equals_systematic(dev):
@@ -215,6 +230,7 @@ def test_residual_systematic_equals_systematic(dev):
 
 @pytest.mark.parametrize("N,T,scheme", [((1 << 21) + 77, 8, SYS),
                                         (1, 12, STRAT), (2, 12, MULT),
+                                        ((1 << 21) + 77, 4, RES),
                                         (2048, 1, SYS)])
 def test_plain_sizes_vs_oracle(dev, N, T, scheme):
     """Sizes the configurations do not reach: more than 1024 tiles (several

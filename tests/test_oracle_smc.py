@@ -102,7 +102,8 @@ def test_smc_oracle_vs_kalman():
 
 
 @pytest.mark.parametrize("scheme", [orc.SYSTEMATIC, orc.STRATIFIED,
-                                    orc.MULTINOMIAL, orc.RESIDUAL_STRATIFIED,
+                                    orc.MULTINOMIAL, orc.RESIDUAL,
+                                    orc.RESIDUAL_STRATIFIED,
                                     orc.RESIDUAL_SYSTEMATIC])
 def test_smc_oracle_schemes_consistent(scheme):
     y = np.load(os.path.join(GOLDEN, "observations.npz"))["lg_y"][:40]
@@ -152,7 +153,8 @@ def test_residual_counts_closed_form():
     the equal-weight steps a flat likelihood gives."""
     y = np.zeros(6)
     # sy huge: the likelihood is flat, the weights stay equal, ESS = N
-    for scheme in (orc.RESIDUAL_STRATIFIED, orc.RESIDUAL_SYSTEMATIC):
+    for scheme in (orc.RESIDUAL, orc.RESIDUAL_STRATIFIED,
+                   orc.RESIDUAL_SYSTEMATIC):
         ll, ess, res = orc.smc_run(orc.MODEL_LG, (0.9, 1.0, 1e6), y, 2, 3,
                                    1.01, scheme, history=True)
         assert res.sum() == 6
