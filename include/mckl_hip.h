@@ -305,6 +305,7 @@ int mckl_hip_smc_estimates(mckl_hip_smc *s, double *h_est);
  * persistent launch (no reference counterpart; the reference runs one
  * SMCSampler at a time).  Filter f uses theta[3f..3f+2], y[T f .. T f+T-1],
  * seed[f] and is bit-identical to the plain sampler with the same inputs.
+ * All six schemes; at most 131072 particles per filter.
  */
 typedef struct mckl_hip_smc_batch mckl_hip_smc_batch;
 typedef struct mckl_hip_tune {

@@ -434,9 +434,108 @@ struct BatchLds {
     uint32_t te[kMaxTiles], tz[kMaxTiles];
     uint32_t teo[kMaxTiles + 1], tzo[kMaxTiles + 1];
     double tmh[kMaxTiles], tml[kMaxTiles];
+    uint32_t ti[kMaxTiles];              // residual: integral parts
+    double trh[kMaxTiles], trl[kMaxTiles]; // residual: residual totals
     int flag;
     int filter;
 };
+
+// Sorted uniforms of step t for one filter (M of them: N, or R of a
+// residual step): tile totals of the log spacings, their offsets, then the
+// uniforms (the plain sampler's k_rs_counts/k_rs_rterms, k_rs_uniforms).
+// Leaves S.toh/S.tol overwritten.  Every thread of the workgroup calls it.
+template <int G>
+__device__ __forceinline__ void batch_muniforms(const SmcArgs &A, uint32_t t,
+    int nt, int rounds, int grp, int gt, const GroupLds &L, BatchLds &S,
+    double *u, uint32_t M)
+{
+    for (int r = 0; r < rounds; ++r) {
+        const int k = r * G + grp;
+        dd mt;
+        tile_mterms(A, (uint32_t)(k < nt ? k : nt), t, gt, L.d, mt, M);
+        if (gt == 0 && k < nt) {
+            S.tmh[k] = mt.hi;
+            S.tml[k] = mt.lo;
+        }
+        __syncthreads();
+    }
+    tiles_scan_dd(grp == 0, S.tmh, S.tml, nt, S.toh, S.tol, gt, L.d);
+    for (int r = 0; r < rounds; ++r) {
+        const int k = r * G + grp;
+        const int kk = k < nt ? k : nt;
+        tile_muniforms(
+            A, (uint32_t)kk, t, dd{S.toh[kk], S.tol[kk]}, u, gt, L.d, M);
+    }
+    __syncthreads();
+}
+
+// Counts of a residual step for one filter, the operations of the plain
+// sampler's k_rs_counts (residual branch), k_rs_rterms, k_rs_uniforms and
+// k_rs_rcounts: per tile the integral parts and residual totals, R, the R
+// sorted uniforms for residual multinomial, the residual offsets, then the
+// tile's cumulative counts.  Leaves S.te/S.tz set and S.toh/S.tol holding the
+// residual offsets.  Every thread of the workgroup calls it.
+template <int G>
+__device__ __forceinline__ void batch_rcounts(const SmcArgs &A, uint32_t t,
+    int nt, int rounds, int grp, int gt, const GroupLds &L, BatchLds &S,
+    const float *lw, double *u, uint32_t *cnt, double invS)
+{
+    for (int r = 0; r < rounds; ++r) {
+        const int k = r * G + grp;
+        const int kk = k < nt ? k : nt;
+        float l[kItems];
+        load8(lw, tile_first((uint32_t)kk, gt), A.N, l, 0.0f);
+        uint32_t isum;
+        dd rt;
+        tile_residual_sums(A, (uint32_t)kk, l, kk < nt ? S.tmax[kk] : 0.0f,
+            kk < nt ? S.r[kk] : 0.0, invS, gt, L.d, L.u, isum, rt);
+        if (gt == 0 && k < nt) {
+            S.ti[k] = isum;
+            S.trh[k] = rt.hi;
+            S.trl[k] = rt.lo;
+        }
+        __syncthreads();
+    }
+    uint32_t isum = 0;
+    for (int i = 0; i < nt; ++i)
+        isum += S.ti[i];
+    const uint32_t R = isum <= A.N ? A.N - isum : 0u;
+    if (A.scheme == MCKL_HIP_RESIDUAL)
+        batch_muniforms<G>(A, t, nt, rounds, grp, gt, L, S, u, R);
+    tiles_scan_dd(grp == 0, S.trh, S.trl, nt, S.toh, S.tol, gt, L.d);
+    UniformSeq U;
+    U.u = A.scheme == MCKL_HIP_RESIDUAL ? u : nullptr;
+    U.scheme = A.scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC ? MCKL_HIP_SYSTEMATIC
+        : A.scheme == MCKL_HIP_RESIDUAL_STRATIFIED      ? MCKL_HIP_STRATIFIED
+                                                        : MCKL_HIP_MULTINOMIAL;
+    U.k0 = A.k0;
+    U.k1 = A.k1;
+    U.t = t;
+    U.M = R;
+    U.delta = R ? __ddiv_rn(1.0, (double)R) : 0.0;
+    U.u0d = U.scheme == MCKL_HIP_SYSTEMATIC
+        ? __dmul_rn(smc_uniform(A.k0, A.k1, t, 0), U.delta)
+        : 0.0;
+    const double invR = R ? __ddiv_rn(1.0, S.toh[nt]) : 0.0;
+    for (int r = 0; r < rounds; ++r) {
+        const int k = r * G + grp;
+        const int kk = k < nt ? k : nt;
+        uint32_t ioff = 0;
+        for (int i = 0; i < kk; ++i)
+            ioff += S.ti[i];
+        float l[kItems];
+        load8(lw, tile_first((uint32_t)kk, gt), A.N, l, 0.0f);
+        uint32_t es, zs;
+        tile_rcounts(A, (uint32_t)kk, l, kk < nt ? S.tmax[kk] : 0.0f,
+            kk < nt ? S.r[kk] : 0.0, invS, ioff, dd{S.toh[kk], S.tol[kk]},
+            kk < nt ? S.toh[kk + 1] : 0.0, invR, U, cnt, gt, L.d, L.u, es, zs);
+        if (gt == 0 && k < nt) {
+            S.te[k] = es;
+            S.tz[k] = zs;
+        }
+        __syncthreads();
+    }
+}
 
 template <int G>
 __global__ __launch_bounds__(G *kGroup, G == 1 ? 2 : 1) void k_batch(const FilterDesc *fd,
@@ -523,47 +622,33 @@ __global__ __launch_bounds__(G *kGroup, G == 1 ? 2 : 1) void k_batch(const Filte
             resampled = o.resample;
             if (resampled) {
                 double *u = ub ? ub + off : nullptr;
-                if (A.scheme == MCKL_HIP_MULTINOMIAL) {
-                    // multinomial: sorted uniforms of the step first
-                    for (int r = 0; r < rounds; ++r) {
-                        const int k = r * G + grp;
-                        dd mt;
-                        tile_mterms(A, (uint32_t)(k < nt ? k : nt), t, gt, L.d,
-                            mt);
-                        if (gt == 0 && k < nt) {
-                            S.tmh[k] = mt.hi;
-                            S.tml[k] = mt.lo;
-                        }
-                        __syncthreads();
-                    }
-                    tiles_scan_dd(
-                        grp == 0, S.tmh, S.tml, nt, S.toh, S.tol, gt, L.d);
-                    for (int r = 0; r < rounds; ++r) {
-                        const int k = r * G + grp;
-                        const int kk = k < nt ? k : nt;
-                        tile_muniforms(A, (uint32_t)kk, t,
-                            dd{S.toh[kk], S.tol[kk]}, u, gt, L.d);
-                    }
-                    __syncthreads();
-                }
+                if (A.scheme == MCKL_HIP_MULTINOMIAL) // sorted uniforms first
+                    batch_muniforms<G>(A, t, nt, rounds, grp, gt, L, S, u, A.N);
                 tiles_scan_dd(grp == 0, S.tdh, S.tdl, nt, S.toh, S.tol, gt, L.d);
                 // the weight total rounded once normalises the cumulative
                 // weights (the plain sampler's counting kernel does the same)
                 const double invS = __ddiv_rn(1.0, S.toh[nt]);
-                const UniformSeq U = make_uniform_seq(A, t, u);
-                for (int r = 0; r < rounds; ++r) {
-                    const int k = r * G + grp;
-                    uint32_t es, zs;
-                    const int kk = k < nt ? k : nt;
-                    tile_counts(A, (uint32_t)kk, lw, kk < nt ? S.tmax[kk] : 0.0f,
-                        kk < nt ? S.r[kk] : 0.0, invS,
-                        dd{S.toh[kk], S.tol[kk]}, kk < nt ? S.toh[kk + 1] : 0.0,
-                        U, cnt, gt, L.d, L.u, es, zs);
-                    if (gt == 0 && k < nt) {
-                        S.te[k] = es;
-                        S.tz[k] = zs;
+                if (is_residual(A.scheme)) {
+                    batch_rcounts<G>(A, t, nt, rounds, grp, gt, L, S, lw, u,
+                        cnt, invS);
+                } else {
+                    const UniformSeq U = make_uniform_seq(A, t, u);
+                    for (int r = 0; r < rounds; ++r) {
+                        const int k = r * G + grp;
+                        uint32_t es, zs;
+                        const int kk = k < nt ? k : nt;
+                        tile_counts(A, (uint32_t)kk, lw,
+                            kk < nt ? S.tmax[kk] : 0.0f,
+                            kk < nt ? S.r[kk] : 0.0, invS,
+                            dd{S.toh[kk], S.tol[kk]},
+                            kk < nt ? S.toh[kk + 1] : 0.0, U, cnt, gt, L.d,
+                            L.u, es, zs);
+                        if (gt == 0 && k < nt) {
+                            S.te[k] = es;
+                            S.tz[k] = zs;
+                        }
+                        __syncthreads();
                     }
-                    __syncthreads();
                 }
                 tiles_scan_u2(grp == 0, S.te, S.tz, nt, S.teo, S.tzo, gt, L.u);
                 for (int r = 0; r < rounds; ++r) {
@@ -723,9 +808,7 @@ static void obs_const(int model, const double *th, const double *y, size_t T,
         out[t] = model == 1 ? (float)y[t] : (float)(y[t] * y[t] / (th[2] * th[2]));
 }
 
-// residual: the plain sampler also takes the three residual schemes
-static int check_common(size_t N, size_t T, double threshold, int scheme,
-    bool residual = false)
+static int check_common(size_t N, size_t T, double threshold, int scheme)
 {
     if (N == 0 || N >= (size_t)0x7FFFFFFF)
         return set_error(MCKL_HIP_EINVAL, "particle count %zu out of range", N);
@@ -733,9 +816,7 @@ static int check_common(size_t N, size_t T, double threshold, int scheme,
         return set_error(MCKL_HIP_EINVAL, "T must be > 0");
     if (scheme != MCKL_HIP_MULTINOMIAL && scheme != MCKL_HIP_STRATIFIED &&
         scheme != MCKL_HIP_SYSTEMATIC &&
-        !(residual && (scheme == MCKL_HIP_RESIDUAL ||
-                          scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
-                          scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC)))
+        !is_residual(scheme))
         return set_error(MCKL_HIP_EINVAL,
             "resampling scheme %d is not provided by this sampler", scheme);
     if (std::isnan(threshold))
@@ -796,7 +877,7 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
     if (!out || !h_y)
         return set_error(MCKL_HIP_EINVAL, "NULL argument");
     *out = nullptr;
-    int rc = check_common(N, T, threshold, scheme, true);
+    int rc = check_common(N, T, threshold, scheme);
     if (rc)
         return rc;
     ModelConst mc;
@@ -1180,7 +1261,7 @@ int mckl_hip_smc_batch_create(int model, size_t nfilters, const uint32_t *h_N,
     MCKL_ALLOC(b->cnt, total);
     MCKL_ALLOC(b->E, total);
     MCKL_ALLOC(b->zpos, total);
-    if (scheme == MCKL_HIP_MULTINOMIAL)
+    if (scheme == MCKL_HIP_MULTINOMIAL || scheme == MCKL_HIP_RESIDUAL)
         MCKL_ALLOC(b->u, total);
     MCKL_ALLOC(b->obs, nfilters * T);
     MCKL_ALLOC(b->logz, nfilters);

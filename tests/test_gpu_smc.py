@@ -144,6 +144,33 @@ def test_batched_equals_plain(dev):
     assert np.array_equal(b.log_likelihood(), lb)
 
 
+@pytest.mark.parametrize("scheme", [RES, RSTRAT, RSYS])
+def test_residual_batched_equals_plain(dev, scheme):
+    """Residual schemes in the batched sampler: bit for bit the plain
+    sampler's log-likelihood, for one and two groups per filter (the
+    two-group launch has a padding round when a filter's tile count is
+    odd) and for filters of 3, 2048, 2049, 6145 and eight more of the
+    configuration's sizes (up to 36425 particles)."""
+    m = _m()
+    ob = _obs()
+    theta = ob["batch64_theta"][:12]
+    N = ob["batch64_N"][:12].copy()
+    N[0] = 3
+    N[1] = 2048
+    N[2] = 2049
+    N[3] = 6145
+    y = ob["batch64_y"][:12]
+    seeds = np.arange(60, 72, dtype=np.uint64)
+    b = m.SMCBatch(2, N, theta, y, seeds, 0.5, scheme)
+    lp = np.array([m.smc_loglik(2, theta[f], y[f], int(N[f]), int(seeds[f]),
+                                0.5, scheme) for f in range(len(N))])
+    for block in (256, 512):
+        b.set_tune(block, 0)
+        b.run(use_graph=False)
+        lb = b.log_likelihood()
+        assert np.array_equal(lb, lp), (block, lb - lp)
+
+
 def test_multinomial_batched_and_kalman(dev):
     """Multinomial resampling: the batched sampler equals the plain one bit
     for bit, and the LG estimate is within Monte Carlo error of Kalman."""
@@ -203,11 +230,9 @@ def test_smc_errors(dev):
         m.SMCSampler(2, (1.5, 0.2, 0.7), np.zeros(3), 100, 1)
     with pytest.raises(m.MCKLError):
         m.SMCSampler(1, (0.9, 1.0, 0.5), np.zeros(3), 100, 1, 0.5, 7)
-    # the batched sampler takes the three plain schemes only
-    for sch in (RES, RSYS):
-        with pytest.raises(m.MCKLError):
-            m.SMCBatch(1, [100, 200], [(0.9, 1.0, 0.5)] * 2,
-                       np.zeros((2, 3)), [1, 2], 0.5, sch)
+    with pytest.raises(m.MCKLError):
+        m.SMCBatch(1, [100, 200], [(0.9, 1.0, 0.5)] * 2, np.zeros((2, 3)),
+                   [1, 2], 0.5, 6)
 
 
 def test_residual_systematic_equals_systematic(dev):
