@@ -225,10 +225,13 @@ int mckl_hip_estimate_f64(size_t n, size_t dim, int layout, const double *d_u,
 int mckl_hip_estimate_f32(size_t n, size_t dim, int layout, const float *d_u,
     const double *d_w, double *d_r, void *stream);
 
-/* StateMatrix<RowMajor,dim,float>::select: core/state_matrix.hpp:193-265
- * (out of place: d_dst[j] = d_src[d_idx[j]], rows of `dim` floats) */
+/* StateMatrix<RowMajor,dim,T>::select: core/state_matrix.hpp:193-265
+ * (out of place: d_dst[j] = d_src[d_idx[j]], rows of `dim` values; the
+ * indices must address rows of d_src) */
 int mckl_hip_select_f32(size_t M, size_t dim, const uint32_t *d_idx,
     const float *d_src, float *d_dst, void *stream);
+int mckl_hip_select_f64(size_t M, size_t dim, const uint32_t *d_idx,
+    const double *d_src, double *d_dst, void *stream);
 
 /*
  * ResampleIndex<uint32_t> (algorithm/resample.hpp:597-832): the history of

@@ -19,6 +19,7 @@ __all__ = [
     "ARS_64", "rand", "U01CCDistribution", "U01CODistribution",
     "U01OCDistribution", "U01OODistribution", "U01CanonicalDistribution",
     "NormalDistribution", "UniformRealDistribution", "MCKLError",
+    "Particle", "ResampleEval", "SMCEstimator", "GenericSMCSampler",
 ]
 
 
@@ -766,3 +767,5 @@ class RNGSetVector:
 
 from . import pmcmc  # noqa: E402,F401  (PMCMCMutation, PMCMCBatch)
 from .pmcmc import PMCMCMutation, PMCMCBatch  # noqa: E402,F401
+from .sampler import (Particle, ResampleEval, SMCEstimator,  # noqa: E402,F401
+                      GenericSMCSampler)

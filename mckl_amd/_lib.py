@@ -92,6 +92,7 @@ SIGNATURES = {
     "mckl_hip_resample_trans_rep_index": (_int, [_sz, _sz, _vp, _vp, _vp]),
     "mckl_hip_resample": (_int, [_int, _sz, _sz, _EP, _vp, _vp, _vp, _vp]),
     "mckl_hip_select_f32": (_int, [_sz, _sz, _vp, _vp, _vp, _vp]),
+    "mckl_hip_select_f64": (_int, [_sz, _sz, _vp, _vp, _vp, _vp]),
     "mckl_hip_resample_index_create": (_int, [ctypes.POINTER(_vp)]),
     "mckl_hip_resample_index_destroy": (_int, [_vp]),
     "mckl_hip_resample_index_num_iter": (_int, [_vp, ctypes.POINTER(_sz)]),

@@ -382,8 +382,9 @@ __global__ void k_stratified(uint64_t n, double *r)
         r[j] = __dmul_rn(delta, __dadd_rn(r[j], (double)j));
 }
 
-__global__ void k_select_f32(uint64_t M, uint64_t dim, const uint32_t *idx,
-    const float *src, float *dst)
+template <typename T>
+__global__ void k_select(uint64_t M, uint64_t dim, const uint32_t *idx,
+    const T *src, T *dst)
 {
     const uint64_t total = M * dim;
     for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -786,6 +787,23 @@ __global__ __launch_bounds__(kB) void k_est_final(uint32_t nt,
     }
 }
 
+template <typename T>
+int select_impl(size_t M, size_t dim, const uint32_t *idx, const T *src,
+    T *dst, hipStream_t st)
+{
+    if (M == 0 || dim == 0)
+        return MCKL_HIP_OK;
+    if (!idx || !src || !dst)
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    if (src == dst)
+        return set_error(MCKL_HIP_EINVAL, "select is out of place");
+    if (dim > ((uint64_t)1 << 40) / M)
+        return set_error(MCKL_HIP_EINVAL, "select: size out of range");
+    k_select<T><<<grid_for(M * dim), 256, 0, st>>>(M, dim, idx, src, dst);
+    MCKL_HIP_CHECK_LAUNCH("k_select");
+    return MCKL_HIP_OK;
+}
+
 template <typename U>
 int estimate_impl(size_t n, size_t dim, int layout, const U *u,
     const double *w, double *r, hipStream_t st)
@@ -941,16 +959,13 @@ int mckl_hip_estimate_f32(size_t n, size_t dim, int layout, const float *d_u,
 int mckl_hip_select_f32(size_t M, size_t dim, const uint32_t *d_idx,
     const float *d_src, float *d_dst, void *stream)
 {
-    if (M == 0 || dim == 0)
-        return MCKL_HIP_OK;
-    if (!d_idx || !d_src || !d_dst)
-        return set_error(MCKL_HIP_EINVAL, "NULL argument");
-    if (d_src == d_dst)
-        return set_error(MCKL_HIP_EINVAL, "select is out of place");
-    k_select_f32<<<grid_for(M * dim), 256, 0, as_stream(stream)>>>(
-        M, dim, d_idx, d_src, d_dst);
-    MCKL_HIP_CHECK_LAUNCH("k_select_f32");
-    return MCKL_HIP_OK;
+    return select_impl(M, dim, d_idx, d_src, d_dst, as_stream(stream));
+}
+
+int mckl_hip_select_f64(size_t M, size_t dim, const uint32_t *d_idx,
+    const double *d_src, double *d_dst, void *stream)
+{
+    return select_impl(M, dim, d_idx, d_src, d_dst, as_stream(stream));
 }
 
 } // extern "C"

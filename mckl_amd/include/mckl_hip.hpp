@@ -22,9 +22,13 @@
 #include <cstring>
 #include <functional>
 #include <istream>
+#include <limits>
+#include <memory>
 #include <ostream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace mckl {
@@ -770,6 +774,481 @@ inline void estimate(std::size_t n, std::size_t dim, MatrixLayout layout,
     hip_check(mckl_hip_estimate_f32(
         n, dim, static_cast<int>(layout), d_u, d_w, d_r, current_stream()));
 }
+
+/// Device array owned by the host objects below (hipMalloc / hipFree)
+template <typename T>
+class DeviceArray
+{
+  public:
+    DeviceArray() = default;
+    explicit DeviceArray(std::size_t n) { resize(n); }
+    ~DeviceArray() { release(); }
+    DeviceArray(const DeviceArray &) = delete;
+    DeviceArray &operator=(const DeviceArray &) = delete;
+    DeviceArray(DeviceArray &&o) noexcept : p_(o.p_), n_(o.n_)
+    {
+        o.p_ = nullptr;
+        o.n_ = 0;
+    }
+    DeviceArray &operator=(DeviceArray &&o) noexcept
+    {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+        return *this;
+    }
+    void resize(std::size_t n)
+    {
+        if (n == n_)
+            return;
+        release();
+        void *q = nullptr;
+        if (n && hipMalloc(&q, n * sizeof(T)) != hipSuccess)
+            throw RuntimeAssert("mckl_hip: device allocation failed");
+        p_ = static_cast<T *>(q);
+        n_ = n;
+    }
+    T *data() const { return p_; }
+    std::size_t size() const { return n_; }
+
+  private:
+    T *p_ = nullptr;
+    std::size_t n_ = 0;
+
+    void release()
+    {
+        if (p_)
+            (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+};
+
+inline void select_rows(std::size_t n, std::size_t dim,
+    const std::uint32_t *d_idx, const float *src, float *dst)
+{
+    hip_check(mckl_hip_select_f32(n, dim, d_idx, src, dst, current_stream()));
+}
+
+inline void select_rows(std::size_t n, std::size_t dim,
+    const std::uint32_t *d_idx, const double *src, double *dst)
+{
+    hip_check(mckl_hip_select_f64(n, dim, d_idx, src, dst, current_stream()));
+}
+
+/// Particle<T> (core/particle.hpp:277-360) in device memory: the state is
+/// N rows of dim values of T (float or double), row-major as
+/// StateMatrix<RowMajor, Dynamic, T>; weight() is the device Weight; rng()
+/// is the engine ResampleEval draws from.
+template <typename T, typename RNG = Philox4x32>
+class Particle
+{
+    static_assert(std::is_same<T, float>::value ||
+            std::is_same<T, double>::value,
+        "Particle: the device state type is float or double");
+
+  public:
+    using size_type = std::size_t;
+    using state_type = T;
+    using rng_type = RNG;
+
+    explicit Particle(std::size_t N, std::size_t dim = 1, RNG rng = RNG())
+        : dim_(dim), rng_(rng)
+    {
+        if (N == 0 || dim == 0)
+            throw RuntimeAssert("Particle: N and dim must be positive");
+        alloc(N);
+    }
+    Particle(const Particle &) = delete;
+    Particle &operator=(const Particle &) = delete;
+
+    size_type size() const { return n_; }
+    std::size_t dim() const { return dim_; }
+    /// device pointer to the N x dim state, row-major
+    T *state() { return x_.data(); }
+    const T *state() const { return x_.data(); }
+    Weight &weight() { return weight_; }
+    const Weight &weight() const { return weight_; }
+    RNG &rng() { return rng_; }
+
+    /// Particle::select (core/particle.hpp:324-331): row j of the new state
+    /// is row d_idx[j] of the old one (j < n; d_idx a device array of row
+    /// numbers below size()), then equal weights of size n
+    void select(size_type n, const std::uint32_t *d_idx)
+    {
+        if (n == n_) {
+            select_rows(n, dim_, d_idx, x_.data(), spare_.data());
+            std::swap(x_, spare_);
+            weight_.set_equal();
+            return;
+        }
+        DeviceArray<T> nx(n * dim_);
+        select_rows(n, dim_, d_idx, x_.data(), nx.data());
+        x_ = std::move(nx);
+        spare_.resize(n * dim_);
+        w_.resize(n);
+        n_ = n;
+        weight_ = Weight(n, w_.data(), stats_.data());
+    }
+
+  private:
+    std::size_t n_ = 0;
+    std::size_t dim_;
+    RNG rng_;
+    DeviceArray<T> x_, spare_;
+    DeviceArray<double> w_, stats_;
+    Weight weight_{0, nullptr, nullptr};
+
+    void alloc(std::size_t N)
+    {
+        x_.resize(N * dim_);
+        spare_.resize(N * dim_);
+        w_.resize(N);
+        stats_.resize(2);
+        n_ = N;
+        if (hipMemset(x_.data(), 0, N * dim_ * sizeof(T)) != hipSuccess)
+            throw RuntimeAssert("mckl_hip: clearing the state failed");
+        weight_ = Weight(N, w_.data(), stats_.data());
+    }
+};
+
+/// ResampleEval<T> (algorithm/resample.hpp:470-511) of a built-in scheme:
+/// replication numbers from Particle::rng() and the weights, the stable
+/// rep -> index transform, Particle::select
+template <typename T, typename RNG = Philox4x32>
+class ResampleEval
+{
+  public:
+    explicit ResampleEval(ResampleScheme scheme)
+        : scheme_(scheme), buf_(std::make_shared<Buffers>())
+    {
+    }
+
+    void operator()(std::size_t, Particle<T, RNG> &particle) const
+    {
+        const std::size_t N = particle.size();
+        buf_->rep.resize(N);
+        buf_->idx.resize(N);
+        hip_check(mckl_hip_resample(scheme_, N, N, &particle.rng().state(),
+            particle.weight().data(), buf_->rep.data(), buf_->idx.data(),
+            current_stream()));
+        particle.select(N, buf_->idx.data());
+    }
+
+  private:
+    struct Buffers {
+        DeviceArray<std::uint32_t> rep, idx;
+    };
+    int scheme_;
+    std::shared_ptr<Buffers> buf_;
+};
+
+/// SMCEstimator<T, double> (algorithm/smc.hpp:48-136).  eval(iter, dim,
+/// particle, d_u) fills the device array d_u: N x dim values (RowMajor),
+/// dim x N (ColMajor), or the dim values of the estimate itself
+/// (record_only).  Otherwise estimate() stores r_j = sum_i W_i u(i, j),
+/// each sum exactly rounded (the reference calls an external BLAS dgemv).
+/// Estimates are kept on the host, one row per call.
+template <typename T, typename RNG = Philox4x32>
+class SMCEstimator
+{
+  public:
+    using eval_type = std::function<void(
+        std::size_t, std::size_t, Particle<T, RNG> &, double *)>;
+
+    SMCEstimator(std::size_t dim, eval_type eval,
+        MatrixLayout layout = MatrixLayout::RowMajor, bool record_only = false)
+        : dim_(dim), eval_(std::move(eval)), layout_(layout),
+          record_only_(record_only), buf_(std::make_shared<Buffers>())
+    {
+        if (dim == 0)
+            throw RuntimeAssert("SMCEstimator: dim must be positive");
+    }
+
+    std::size_t dim() const { return dim_; }
+    std::size_t num_iter() const { return rows_.size() / dim_; }
+    bool record_only() const { return record_only_; }
+    void clear() { rows_.clear(); }
+    void reserve(std::size_t n) { rows_.reserve(rows_.size() + n * dim_); }
+
+    /// SMCEstimator::estimate(eval, layout, record_only) (smc.hpp:76-84)
+    void estimate(eval_type eval,
+        MatrixLayout layout = MatrixLayout::RowMajor, bool record_only = false)
+    {
+        eval_ = std::move(eval);
+        layout_ = layout;
+        record_only_ = record_only;
+    }
+
+    /// SMCEstimator::estimate(iter, particle) (smc.hpp:86-118); reads the
+    /// result to the host (synchronises the stream)
+    void estimate(std::size_t iter, Particle<T, RNG> &particle)
+    {
+        if (!eval_)
+            throw RuntimeAssert(
+                "**Estimator::eval** used with an invalid evaluation object");
+        const std::size_t n = particle.size();
+        buf_->r.resize(dim_);
+        if (record_only_) {
+            eval_(iter, dim_, particle, buf_->r.data());
+        } else {
+            buf_->u.resize(n * dim_);
+            eval_(iter, dim_, particle, buf_->u.data());
+            mckl::estimate(n, dim_, layout_, buf_->u.data(),
+                particle.weight().data(), buf_->r.data());
+        }
+        const std::size_t k = rows_.size();
+        rows_.resize(k + dim_);
+        if (hipStreamSynchronize(static_cast<hipStream_t>(current_stream())) !=
+                hipSuccess ||
+            hipMemcpy(rows_.data() + k, buf_->r.data(), dim_ * sizeof(double),
+                hipMemcpyDeviceToHost) != hipSuccess)
+            throw RuntimeAssert("mckl_hip: reading an estimate failed");
+    }
+
+    /// row i (iteration i) of the estimates
+    const double *row_data(std::size_t i) const
+    {
+        return rows_.data() + i * dim_;
+    }
+
+    /// variable j over the iterations (EstimateMatrix::variable)
+    template <typename OutputIter>
+    OutputIter read_estimate(std::size_t j, OutputIter first) const
+    {
+        for (std::size_t i = 0; i != num_iter(); ++i)
+            *first++ = rows_[i * dim_ + j];
+        return first;
+    }
+
+    /// Estimator::average (core/estimator.hpp:62-93): rows cut, cut + thin,
+    /// ... summed and, as the reference does, divided by the number of
+    /// stored iterations; row cut itself when fewer than thin rows follow
+    template <typename OutputIter>
+    OutputIter average(
+        OutputIter first, std::size_t cut = 0, std::size_t thin = 1) const
+    {
+        const std::size_t n = num_iter();
+        if (cut >= n)
+            return first;
+        thin = thin < 1 ? 1 : thin;
+        if (n - cut < thin)
+            return std::copy(row_data(cut), row_data(cut) + dim_, first);
+        std::vector<double> sum(dim_, 0.0);
+        for (; cut < n; cut += thin)
+            for (std::size_t j = 0; j != dim_; ++j)
+                sum[j] += rows_[cut * dim_ + j];
+        for (auto &v : sum)
+            v *= 1.0 / static_cast<double>(n);
+        return std::copy(sum.begin(), sum.end(), first);
+    }
+
+  private:
+    struct Buffers {
+        DeviceArray<double> u, r;
+    };
+    std::size_t dim_;
+    eval_type eval_;
+    MatrixLayout layout_;
+    bool record_only_;
+    std::shared_ptr<Buffers> buf_;
+    std::vector<double> rows_;
+};
+
+/// SMCSampler<T> (algorithm/smc.hpp:151-421) on a device Particle<T>.
+/// Evaluation objects eval(iter, particle) enqueue their own device work on
+/// current_stream().  iterate(n) runs do_iterate (smc.hpp:389-406) n times:
+/// selection evals, selection estimators, the size and ESS recorded (the
+/// ESS read to the host), resample evals when ESS < N * threshold,
+/// resample estimators, mutation evals, mutation estimators.
+template <typename T, typename RNG = Philox4x32>
+class SMCSampler
+{
+  public:
+    using state_type = T;
+    using size_type = std::size_t;
+    using eval_type = std::function<void(std::size_t, Particle<T, RNG> &)>;
+    using estimator_type = SMCEstimator<T, RNG>;
+
+    explicit SMCSampler(std::size_t N, std::size_t dim = 1, RNG rng = RNG())
+        : particle_(N, dim, rng)
+    {
+    }
+
+    size_type size() const { return particle_.size(); }
+    std::size_t num_iter() const { return size_history_.size(); }
+    void reserve(std::size_t n)
+    {
+        for (auto &slot : est_)
+            for (auto &e : slot)
+                e.reserve(n);
+        size_history_.reserve(size_history_.size() + n);
+        ess_history_.reserve(ess_history_.size() + n);
+    }
+
+    /// remove every evaluation object and estimator, clear the history
+    void reset()
+    {
+        for (auto &slot : eval_)
+            slot.clear();
+        for (auto &slot : est_)
+            slot.clear();
+        clear();
+    }
+
+    /// clear the history (smc.hpp:205-212)
+    void clear()
+    {
+        for (auto &slot : est_)
+            for (auto &e : slot)
+                e.clear();
+        iter_ = 0;
+        size_history_.clear();
+        ess_history_.clear();
+    }
+
+    double resample_threshold() const { return threshold_; }
+    void resample_threshold(double t) { threshold_ = t; }
+    static double resample_threshold_never()
+    {
+        return -std::numeric_limits<double>::infinity();
+    }
+    static double resample_threshold_always()
+    {
+        return std::numeric_limits<double>::infinity();
+    }
+
+    std::size_t selection(eval_type e) { return add(0, std::move(e)); }
+    std::size_t resample(eval_type e) { return add(1, std::move(e)); }
+    /// the built-in ResampleEval of a scheme (smc.hpp:246-270)
+    std::size_t resample(ResampleScheme scheme)
+    {
+        return add(1, ResampleEval<T, RNG>(scheme));
+    }
+    std::size_t mutation(eval_type e) { return add(2, std::move(e)); }
+    eval_type &selection(std::size_t k) { return eval_[0].at(k); }
+    eval_type &resample(std::size_t k) { return eval_[1].at(k); }
+    eval_type &mutation(std::size_t k) { return eval_[2].at(k); }
+
+    std::size_t selection_estimator(estimator_type e)
+    {
+        return add_est(0, std::move(e));
+    }
+    std::size_t resample_estimator(estimator_type e)
+    {
+        return add_est(1, std::move(e));
+    }
+    std::size_t mutation_estimator(estimator_type e)
+    {
+        return add_est(2, std::move(e));
+    }
+    estimator_type &selection_estimator(std::size_t k)
+    {
+        return est_[0].at(k);
+    }
+    estimator_type &resample_estimator(std::size_t k)
+    {
+        return est_[1].at(k);
+    }
+    estimator_type &mutation_estimator(std::size_t k)
+    {
+        return est_[2].at(k);
+    }
+
+    void iterate(std::size_t n = 1)
+    {
+        if (n > 1)
+            reserve(n);
+        for (std::size_t i = 0; i != n; ++i)
+            do_iterate();
+    }
+
+    Particle<T, RNG> &particle() { return particle_; }
+
+    template <typename OutputIter>
+    OutputIter read_size_history(OutputIter first) const
+    {
+        return std::copy(size_history_.begin(), size_history_.end(), first);
+    }
+
+    template <typename OutputIter>
+    OutputIter read_ess_history(OutputIter first) const
+    {
+        return std::copy(ess_history_.begin(), ess_history_.end(), first);
+    }
+
+    /// Sampler::summary (core/sampler.hpp:80-118): every estimator's
+    /// estimates side by side, num_iter() rows, in the given layout
+    std::vector<double> summary(
+        MatrixLayout layout = MatrixLayout::RowMajor) const
+    {
+        std::size_t ncol = 0;
+        for (auto &slot : est_)
+            for (auto &e : slot)
+                ncol += e.dim();
+        const std::size_t nrow = num_iter();
+        std::vector<double> m(nrow * ncol);
+        std::size_t c = 0;
+        for (auto &slot : est_) {
+            for (auto &e : slot) {
+                for (std::size_t i = 0; i != nrow && i != e.num_iter(); ++i)
+                    for (std::size_t j = 0; j != e.dim(); ++j) {
+                        const double v = e.row_data(i)[j];
+                        if (layout == MatrixLayout::RowMajor)
+                            m[i * ncol + c + j] = v;
+                        else
+                            m[(c + j) * nrow + i] = v;
+                    }
+                c += e.dim();
+            }
+        }
+        return m;
+    }
+
+  private:
+    Particle<T, RNG> particle_;
+    std::vector<eval_type> eval_[3];
+    std::vector<estimator_type> est_[3];
+    std::size_t iter_ = 0;
+    double threshold_ = resample_threshold_never();
+    std::vector<size_type> size_history_;
+    std::vector<double> ess_history_;
+
+    std::size_t add(int slot, eval_type e)
+    {
+        if (!e)
+            throw RuntimeAssert("SMCSampler: invalid evaluation object");
+        eval_[slot].push_back(std::move(e));
+        return eval_[slot].size() - 1;
+    }
+
+    std::size_t add_est(int slot, estimator_type e)
+    {
+        est_[slot].push_back(std::move(e));
+        return est_[slot].size() - 1;
+    }
+
+    void do_iterate()
+    {
+        for (auto &e : eval_[0])
+            e(iter_, particle_);
+        for (auto &e : est_[0])
+            e.estimate(iter_, particle_);
+        size_history_.push_back(size());
+        ess_history_.push_back(particle_.weight().ess());
+        if (ess_history_.back() <
+            static_cast<double>(size()) * threshold_) {
+            for (auto &e : eval_[1])
+                e(iter_, particle_);
+        }
+        for (auto &e : est_[1])
+            e.estimate(iter_, particle_);
+        for (auto &e : eval_[2])
+            e(iter_, particle_);
+        for (auto &e : est_[2])
+            e.estimate(iter_, particle_);
+        ++iter_;
+    }
+};
 
 } // namespace mckl
 

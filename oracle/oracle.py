@@ -3,6 +3,9 @@
 ``O``   -- libmckl_oracle.so, the CPU restatement (mckl_oracle.c)
 ``REF`` -- oracle/_ref/libmckl_ref.so, the reference's own headers behind
            ref_driver.cpp, when it has been built (None otherwise)
+``REF_SMC`` -- oracle/_ref/libmckl_ref_smc.so, the reference's SMCSampler
+           behind ref_smc.cpp (lazy binding: its BLAS symbol is never
+           called), when built
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
 """
@@ -14,6 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_PATH = os.path.join(HERE, "libmckl_oracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libmckl_ref.so")
+REF_SMC_PATH = os.path.join(HERE, "_ref", "libmckl_ref_smc.so")
 
 PHILOX4X32, THREEFRY4X64, ARS = 1, 2, 3
 CC, CO, OC, OO, CANONICAL = 0, 1, 2, 3, 4
@@ -23,6 +27,8 @@ MODEL_LG, MODEL_SV = 1, 2
 
 O = ctypes.CDLL(ORACLE_PATH)
 REF = ctypes.CDLL(REF_PATH) if os.path.exists(REF_PATH) else None
+REF_SMC = (ctypes.CDLL(REF_SMC_PATH, mode=os.RTLD_LAZY)
+           if os.path.exists(REF_SMC_PATH) else None)
 
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
@@ -499,3 +505,128 @@ def pmmh_random_walk(scales):
         return np.zeros(P.shape[0])
 
     return f
+
+
+# ---------------------------------------------------------------------------
+# SMCSampler<T> with evaluation objects (algorithm/smc.hpp:151-421)
+# ---------------------------------------------------------------------------
+
+class Particle:
+    """Particle<T> (core/particle.hpp:277-360) over the oracle's primitives:
+    state x (N x dim float64, row-major), weights W with the ESS and log
+    normaliser increment of the last weight operation, engine rng (the
+    one ResampleEval draws from, Philox4x32 keyed by key)."""
+
+    def __init__(self, N, dim=1, key=(1, 0)):
+        self.x = np.zeros((N, dim))
+        self.rng = engine(PHILOX4X32, list(key))
+        self.set_equal(N)
+
+    def size(self):
+        return len(self.W)
+
+    # Weight::set_equal (core/weight.hpp:76-80)
+    def set_equal(self, n=None):
+        n = self.size() if n is None else n
+        self.W = np.full(n, 1.0 / n)
+        self.ess = float(n)
+        self.lnc = 0.0
+
+    def set_log(self, v):
+        self.W, self.ess = weight_set_log(v)
+        self.lnc = 0.0
+
+    def add_log(self, v):
+        self.W, self.ess, self.lnc = weight_add_log(self.W, v)
+
+    # Particle::select (core/particle.hpp:324-331)
+    def select(self, idx):
+        idx = np.asarray(idx, np.int64)
+        self.x = self.x[idx].copy()
+        self.set_equal(len(idx))
+
+
+def resample_eval(scheme):
+    """ResampleEval (algorithm/resample.hpp:470-511) of a built-in scheme:
+    replication numbers from Particle::rng() and the weights, the stable
+    rep -> index transform, Particle::select."""
+
+    def ev(it, p):
+        _, idx = resample(scheme, p.rng, p.W)
+        p.select(idx)
+
+    return ev
+
+
+class GenericSMC:
+    """SMCSampler<T>::iterate/do_iterate (algorithm/smc.hpp:352-406):
+    selection evals, selection estimators, record the size and the ESS,
+    resample evals when ESS < N * threshold, resample estimators, mutation
+    evals, mutation estimators.  Estimators are callables est(iter, p)
+    whose results are appended to self.estimates[slot]."""
+
+    def __init__(self, N, dim=1, key=(1, 0)):
+        self.particle = Particle(N, dim, key)
+        self.evals = [[], [], []]
+        self.estimators = [[], [], []]
+        self.estimates = [[], [], []]
+        self.threshold = -np.inf
+        self.size_history = []
+        self.ess_history = []
+        self.iter = 0
+
+    def iterate(self, n=1):
+        p = self.particle
+        for _ in range(n):
+            for ev in self.evals[0]:
+                ev(self.iter, p)
+            self._estimate(0)
+            self.size_history.append(p.size())
+            self.ess_history.append(p.ess)
+            if p.ess < p.size() * self.threshold:
+                for ev in self.evals[1]:
+                    ev(self.iter, p)
+            self._estimate(1)
+            for ev in self.evals[2]:
+                ev(self.iter, p)
+            self._estimate(2)
+            self.iter += 1
+
+    def _estimate(self, slot):
+        for est in self.estimators[slot]:
+            self.estimates[slot].append(est(self.iter, self.particle))
+
+
+def ref_smc_lg(N, T, scheme, threshold, key, a, sx, sy, mut, y, z):
+    """The reference's own SMCSampler on the table-noise LG model of
+    ref_smc.cpp; returns (ess history, size history, final x)."""
+    y = np.ascontiguousarray(y, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    ess = np.zeros(T)
+    size = np.zeros(T, np.uint64)
+    x = np.zeros(N)
+    k = (ctypes.c_uint64 * 2)(*[int(v) for v in key])
+    REF_SMC.ref_smc_lg(_sz(N), _sz(T), scheme, ctypes.c_double(threshold), k,
+                       ctypes.c_double(a), ctypes.c_double(sx),
+                       ctypes.c_double(sy), ctypes.c_double(mut), P(y), P(z),
+                       P(ess), P(size), P(x))
+    return ess, size, x
+
+
+def lg_table_model(smc, y, z, a, sx, sy, mut=1.0):
+    """The table-noise LG evaluation objects of ref_smc.cpp on a GenericSMC
+    (selection: move and add_log; mutation: x *= mut unless mut = 1)."""
+    N = smc.particle.size()
+
+    def sel(t, p):
+        x = p.x[:, 0]
+        x = z[:N].copy() if t == 0 else a * x + sx * z[t * N:(t + 1) * N]
+        p.x[:, 0] = x
+        d = (y[t] - x) / sy
+        p.add_log(-0.5 * (d * d))
+
+    smc.evals[0].append(sel)
+    if mut != 1.0:
+        def mu(t, p):
+            p.x *= mut
+        smc.evals[2].append(mu)

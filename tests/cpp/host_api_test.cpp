@@ -266,6 +266,83 @@ int main(int argc, char **argv)
                 std::fabs(param[0]) < 1,
             "PMCMCMutationHIP chain");
     }
+    // 10. SMCSampler<T> with evaluation objects equals the same steps done
+    // by hand with the primitives (Weight, ResampleSystematic, select)
+    {
+        const std::size_t N = 5000, T = 9;
+        mckl::NormalDistribution<double> normal(0, 1);
+        mckl::Philox4x32 noise(77);
+        Dev<double> v(N);
+        mckl::SMCSampler<double> s(N, 1, mckl::Philox4x32(5));
+        std::vector<int> order;
+        s.selection([&](std::size_t t, mckl::Particle<double> &p) {
+            if (t == 0)
+                mckl::rand(noise, normal, N, p.state());
+            mckl::rand(noise, normal, N, v.p);
+            p.weight().add_log(v.p);
+            order.push_back(0);
+        });
+        s.resample(mckl::Systematic);
+        s.resample([&](std::size_t, mckl::Particle<double> &) {
+            order.push_back(1);
+        });
+        s.mutation([&](std::size_t, mckl::Particle<double> &) {
+            order.push_back(2);
+        });
+        s.selection_estimator(mckl::SMCEstimator<double>(1,
+            [](std::size_t, std::size_t, mckl::Particle<double> &p,
+                double *u) {
+                (void)hipMemcpy(u, p.state(), p.size() * sizeof(double),
+                    hipMemcpyDeviceToDevice);
+            }));
+        // log weights N(0, 1) per step: the ESS of one increment is about
+        // N / e, of two about N / e^2, so every other step resamples
+        s.resample_threshold(0.3);
+        s.iterate(T);
+
+        mckl::Philox4x32 noise2(77), rng2(5);
+        Dev<double> v2(N), x(N), xb(N), w(N), st(2);
+        Dev<std::uint32_t> rep(N), idx(N);
+        Dev<double> r(1);
+        mckl::Weight W(N, w.p, st.p);
+        mckl::rand(noise2, normal, N, x.p);
+        std::vector<double> ess, est;
+        std::vector<int> want;
+        for (std::size_t t = 0; t != T; ++t) {
+            mckl::rand(noise2, normal, N, v2.p);
+            W.add_log(v2.p);
+            want.push_back(0);
+            mckl::estimate(N, 1, mckl::MatrixLayout::RowMajor, x.p, w.p, r.p);
+            est.push_back(r.host(1)[0]);
+            ess.push_back(W.ess());
+            if (ess.back() < 0.3 * N) {
+                mckl::ResampleSystematic()(N, N, rng2, w.p, rep.p, idx.p);
+                mckl::select_rows(N, 1, idx.p, x.p, xb.p);
+                std::swap(x.p, xb.p);
+                W.set_equal();
+                want.push_back(1);
+            }
+            want.push_back(2);
+        }
+        std::vector<double> gess(T);
+        s.read_ess_history(gess.begin());
+        std::vector<double> gx(N);
+        (void)hipMemcpy(gx.data(), s.particle().state(), N * sizeof(double),
+            hipMemcpyDeviceToHost);
+        const std::vector<double> sm = s.summary();
+        bool same_est = sm.size() == T;
+        for (std::size_t t = 0; same_est && t != T; ++t)
+            same_est = sm[t] == est[t];
+        std::size_t nres = 0;
+        for (int o : want)
+            nres += o == 1;
+        CHECK(gess == ess && s.num_iter() == T,
+            "SMCSampler<double> ESS history == the primitives'");
+        CHECK(order == want && nres > 0 && nres < T,
+            "SMCSampler<double> evaluation order");
+        CHECK(gx == x.host(N), "SMCSampler<double> state == the primitives'");
+        CHECK(same_est, "SMCSampler<double> estimates == the primitives'");
+    }
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;
 }
