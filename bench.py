@@ -10,7 +10,9 @@ inside the timed region.
 Also reported (in "secondary"): the 2^28-word saturation fill, the
 distribution fills (Threefry4x64 / ARS x Normal<float>(0,1) /
 Normal<double>(2.5,0.3), 64 streams x 2^22), the plain SMC on the
-linear-Gaussian configuration (N = 2^20, T = 100), the batched SMC on
+linear-Gaussian configuration (N = 2^20, T = 100), the same filter
+written as evaluation objects of the generic SMCSampler<T>, the batched
+SMC on
 4096 stochastic-volatility filters (T = 50) against one plain launch
 sequence per filter, the SMC log-likelihood gradient (256 points x
 N = 2^16 x T = 500, with its finite-difference and Kalman checks) and the
@@ -25,6 +27,7 @@ own headers (oracle/_ref) or else the oracle port, on this host, one thread.
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -150,6 +153,7 @@ def bench_normals(m, torch, reps=3):
 
 def bench_smc(m, torch):
     res = bench_smc_plain(m, torch)
+    res.update(bench_smc_generic(m, torch))
     res.update(bench_smc_batched(m, torch))
     return res
 
@@ -176,6 +180,88 @@ def bench_smc_plain(m, torch):
         "particle_steps_per_s": N * T / t, "ms": round(t * 1e3, 3),
         "loglik": ll, "resampling_steps": int(rs.sum())}
     return res
+
+
+def bench_smc_generic(m, torch):
+    """The plain configuration (LG (0.9, 1, 0.5), N = 2^20, T = 100,
+    systematic when ESS < N/2) as a user would write it for the reference's
+    SMCSampler<T>: evaluation objects of torch operations on the device
+    state (FP64), the library's Normal fill, Weight, ResampleEval and
+    select, the ESS read to the host once per iteration."""
+    from mckl_amd import workloads as wl
+    y = wl.lg_observations(100, 7)
+    N, T = 1 << 20, 100
+    a, sx, sy = 0.9, 1.0, 0.5
+    c = -math.log(sy) - 0.5 * math.log(2 * math.pi)
+
+    def run(seed):
+        s = m.GenericSMCSampler(N, 1, torch.float64, seed=seed)
+        eng = m.Philox4x32(seed + 1)
+        z = torch.empty(N, dtype=torch.float64, device="cuda")
+        dist = m.NormalDistribution(0.0, 1.0, "float64")
+        lnc = []
+
+        def sel(t, p):
+            if t == 0:
+                m.rand(eng, dist, N, z)
+                p.state[:, 0] = z
+            m.rand(eng, dist, N, z)
+            x = p.state[:, 0]
+            x.mul_(a).add_(z * sx)
+            d = (float(y[t]) - x) / sy
+            p.weight().add_log(d * d * -0.5 + c)
+            lnc.append(p.weight().stats()[1:2].clone())
+
+        s.selection(sel)
+        s.resample(m.SYSTEMATIC)
+        s.resample_threshold(0.5)
+        s.iterate(T)
+        return s, lnc
+
+    run(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s, lnc = run(2)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ll = float(torch.cat(lnc).sum().item())
+    res = int((s.ess_history() < 0.5 * N).sum())
+    return {"smc_generic_lg_N2^20_T100": {
+        "particle_steps_per_s": N * T / t, "ms": round(t * 1e3, 3),
+        "loglik": ll, "resampling_steps": res,
+        "note": "SMCSampler<T> with torch evaluation objects, FP64 state; "
+                "host clock around the whole run (one ESS read per step)"}}
+
+
+def bench_resample(m, torch):
+    """ResampleAlgorithm + rep -> index on the device (mckl_hip_resample) at
+    N = M = 2^20 on normalised weights exp(N(0, 1)) (a typical SMC step):
+    microseconds per call and particles per second, per scheme."""
+    N = 1 << 20
+    g = torch.Generator(device="cuda").manual_seed(3)
+    w = torch.randn(N, dtype=torch.float64, device="cuda", generator=g).exp()
+    w /= w.sum()
+    rep = torch.empty(N, dtype=torch.int32, device="cuda")
+    idx = torch.empty(N, dtype=torch.int32, device="cuda")
+    out = {}
+    for name, alg in (("multinomial", m.ResampleMultinomial()),
+                      ("stratified", m.ResampleStratified()),
+                      ("systematic", m.ResampleSystematic()),
+                      ("residual_systematic", m.ResampleResidualSystematic())):
+        rng = m.Philox4x32(9)
+        for _ in range(3):
+            alg(N, N, rng, w, rep, idx)
+        torch.cuda.synchronize()
+        a, b = _events()
+        reps = 20
+        a.record()
+        for _ in range(reps):
+            alg(N, N, rng, w, rep, idx)
+        b.record()
+        torch.cuda.synchronize()
+        us = a.elapsed_time(b) / reps * 1e3
+        out[name] = {"us": round(us, 1), "particles_per_s": N / (us * 1e-6)}
+    return {"resample_N2^20": out}
 
 
 def bench_smc_batched(m, torch):
@@ -456,8 +542,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--only", choices=["fills", "saturation", "normals", "smc",
-                                       "smc_plain", "smc_batched", "grad",
-                                       "resample_index"],
+                                       "smc_plain", "smc_generic",
+                                       "smc_batched", "grad",
+                                       "resample_index", "resample"],
                     help="run one secondary leg alone and print its JSON "
                          "(for profiling)")
     args = ap.parse_args()
@@ -483,9 +570,11 @@ def main():
     if args.only:
         leg = {"saturation": bench_saturation, "normals": bench_normals,
                "smc": bench_smc, "smc_plain": bench_smc_plain,
+               "smc_generic": bench_smc_generic,
                "smc_batched": bench_smc_batched,
                "grad": bench_grad,
-               "resample_index": bench_resample_index}[args.only]
+               "resample_index": bench_resample_index,
+               "resample": bench_resample}[args.only]
         print(json.dumps(leg(m, torch)), flush=True)
         return
     c1 = bench_config1(m, torch, args.steps, args.warmup)
@@ -502,6 +591,7 @@ def main():
                           ("normals", bench_normals), ("smc", bench_smc),
                           ("grad", bench_grad),
                           ("resample_index", bench_resample_index),
+                          ("resample", bench_resample),
                           ("cpu", lambda m_, t_: {
                               "cpu": _guarded(cpu_reference_points,
                                               threads)})):

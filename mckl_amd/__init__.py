@@ -319,6 +319,11 @@ class Weight:
         check(lib.mckl_hip_weight_add_log(self.size(), _ptr(v), _ptr(self.data),
                                           _ptr(self._stats), _stream()))
 
+    def stats(self):
+        """The device tensor {ESS, log normaliser increment} of the last
+        weight operation (read it without synchronising)."""
+        return self._stats
+
     def ess(self):
         return float(self._stats[0].item())
 

@@ -27,6 +27,81 @@ namespace {
 
 constexpr int kB = kGroup; // 256 threads, 8 items each: tiles of 2048
 
+// A thread's eight elements i0 .. i0+7: 16-byte accesses when all eight
+// are below n and p + i0 is 16-byte aligned, else element by element
+// (fill past n)
+__device__ __forceinline__ bool vec_ok(const void *p, uint64_t i0, uint64_t n)
+{
+    return i0 + kItems <= n && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
+__device__ __forceinline__ void ld8(const double *p, uint64_t i0, uint64_t n,
+    double (&v)[kItems], double fill)
+{
+    if (vec_ok(p + i0, i0, n)) {
+#pragma unroll
+        for (int k = 0; k < kItems; k += 2) {
+            const double2 a = *reinterpret_cast<const double2 *>(p + i0 + k);
+            v[k] = a.x;
+            v[k + 1] = a.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            v[k] = i0 + k < n ? p[i0 + k] : fill;
+    }
+}
+
+__device__ __forceinline__ void st8(double *p, uint64_t i0, uint64_t n,
+    const double (&v)[kItems])
+{
+    if (vec_ok(p + i0, i0, n)) {
+#pragma unroll
+        for (int k = 0; k < kItems; k += 2)
+            *reinterpret_cast<double2 *>(p + i0 + k) = double2{v[k], v[k + 1]};
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            if (i0 + k < n)
+                p[i0 + k] = v[k];
+    }
+}
+
+__device__ __forceinline__ void ld8(const uint32_t *p, uint64_t i0,
+    uint64_t n, uint32_t (&v)[kItems], uint32_t fill)
+{
+    if (vec_ok(p + i0, i0, n)) {
+#pragma unroll
+        for (int k = 0; k < kItems; k += 4) {
+            const uint4 a = *reinterpret_cast<const uint4 *>(p + i0 + k);
+            v[k] = a.x;
+            v[k + 1] = a.y;
+            v[k + 2] = a.z;
+            v[k + 3] = a.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            v[k] = i0 + k < n ? p[i0 + k] : fill;
+    }
+}
+
+__device__ __forceinline__ void st8(uint32_t *p, uint64_t i0, uint64_t n,
+    const uint32_t (&v)[kItems])
+{
+    if (vec_ok(p + i0, i0, n)) {
+#pragma unroll
+        for (int k = 0; k < kItems; k += 4)
+            *reinterpret_cast<uint4 *>(p + i0 + k) =
+                make_uint4(v[k], v[k + 1], v[k + 2], v[k + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            if (i0 + k < n)
+                p[i0 + k] = v[k];
+    }
+}
+
 __global__ __launch_bounds__(kB) void k_weight_pre(int mode, uint64_t n,
     const double *v, double *w, double *pmax)
 {
@@ -34,14 +109,19 @@ __global__ __launch_bounds__(kB) void k_weight_pre(int mode, uint64_t n,
     __shared__ double sm[kB / 64];
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     double m = -__builtin_huge_val();
+    double a[kItems], b[kItems];
+    ld8(v, i0, n, a, 0.0);
+    if (mode != 0)
+        ld8(w, i0, n, b, 1.0);
+#pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = i0 + k;
-        if (i < n) {
-            const double l = mode == 0 ? v[i] : __dadd_rn(v[i], log(w[i]));
-            w[i] = l;
+        if (i0 + k < n) {
+            const double l = mode == 0 ? a[k] : __dadd_rn(a[k], log(b[k]));
+            a[k] = l;
             m = m < l ? l : m;
         }
     }
+    st8(w, i0, n, a);
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         const double o = __shfl_xor(m, d, 64);
@@ -89,15 +169,18 @@ __global__ __launch_bounds__(kB) void k_weight_exp(uint64_t n, double *w,
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     const double m = *lmax;
     double a = 0.0, b = 0.0;
+    double x[kItems];
+    ld8(w, i0, n, x, 0.0);
+#pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = i0 + k;
-        if (i < n) {
-            const double e = exp(__dsub_rn(w[i], m));
-            w[i] = e;
+        if (i0 + k < n) {
+            const double e = exp(__dsub_rn(x[k], m));
+            x[k] = e;
             a = __dadd_rn(a, e);
             b = __dadd_rn(b, __dmul_rn(e, e));
         }
     }
+    st8(w, i0, n, x);
     group_sum2(a, b, threadIdx.x, sd);
     if (threadIdx.x == 0) {
         ps[blockIdx.x] = a;
@@ -141,11 +224,12 @@ __global__ void k_weight_equal(uint64_t n, double *w, double *stats)
 }
 
 // ---- double-double scan of an array (tiles of 2048) ----
-// term(i): w[i] (mode 0) or log(c[i]) / (N - i) (mode 1, sorted uniforms)
-__device__ __forceinline__ double scan_term(int mode, const double *a,
-    uint64_t i, uint64_t N)
+// term of element i with value a: a (mode 0, weights) or log(a) / (N - i)
+// (mode 1, sorted uniforms)
+__device__ __forceinline__ double scan_term(int mode, double a, uint64_t i,
+    uint64_t N)
 {
-    return mode == 0 ? a[i] : __ddiv_rn(log(a[i]), (double)(N - i));
+    return mode == 0 ? a : __ddiv_rn(log(a), (double)(N - i));
 }
 
 __global__ __launch_bounds__(kB) void k_dd_tiles(int mode, uint64_t n,
@@ -154,10 +238,12 @@ __global__ __launch_bounds__(kB) void k_dd_tiles(int mode, uint64_t n,
     __shared__ double sd[8];
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     dd loc{0.0, 0.0};
+    double v[kItems];
+    ld8(a, i0, n, v, 1.0);
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         if (i < n)
-            loc = dd_add(loc, dd{scan_term(mode, a, i, N), 0.0});
+            loc = dd_add(loc, dd{scan_term(mode, v[k], i, N), 0.0});
     }
     dd tot;
     (void)group_excl_scan_dd(loc, threadIdx.x, sd, tot);
@@ -186,10 +272,11 @@ __device__ __forceinline__ void tile_prefix(int mode, uint64_t n, uint64_t N,
         ? (uint64_t)(tile + 1) * kTile
         : n;
     double t[kItems];
+    ld8(a, i0, n, t, 1.0);
     dd loc{0.0, 0.0};
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        t[k] = i < n ? scan_term(mode, a, i, N) : 0.0;
+        t[k] = i < n ? scan_term(mode, t[k], i, N) : 0.0;
         loc = dd_add(loc, dd{t[k], 0.0});
     }
     dd tot;
@@ -212,18 +299,50 @@ __global__ __launch_bounds__(kB) void k_sorted_finish(uint64_t n,
     tile_prefix(1, n, n, c, oh, ol, P);
     __syncthreads();
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = i0 + k;
-        if (i < n)
-            r[i] = __dsub_rn(1.0, exp(P[k]));
-    }
+    double v[kItems];
+    for (int k = 0; k < kItems; ++k)
+        v[k] = __dsub_rn(1.0, exp(P[k]));
+    st8(r, i0, n, v);
 }
 
-// #{j < M : u_j < S} for ascending u
-__device__ __forceinline__ uint32_t count_below(
-    const double *u, uint64_t M, double S)
+// #{j < M : u_j < S} for non-decreasing u, searched from a guess g in
+// [0, M]: steps of 1, 2, 4, ... away from g until the partition point is
+// bracketed, then bisection of the bracket.  The result is the partition
+// point whatever g is; a guess d away from it costs about 2 log2(d) + 2
+// loads instead of log2(M).
+__device__ __forceinline__ uint64_t count_from(
+    const double *u, uint64_t M, double S, uint64_t g)
 {
-    uint64_t lo = 0, hi = M;
+    uint64_t lo, hi; // the partition point lies in [lo, hi]
+    if (g < M && u[g] < S) {
+        lo = g + 1;
+        for (uint64_t step = 1;; step <<= 1) {
+            const uint64_t j = lo + step - 1;
+            if (j >= M) {
+                hi = M;
+                break;
+            }
+            if (!(u[j] < S)) {
+                hi = j;
+                break;
+            }
+            lo = j + 1;
+        }
+    } else {
+        hi = g;
+        for (uint64_t step = 1;; step <<= 1) {
+            if (hi < step) {
+                lo = 0;
+                break;
+            }
+            const uint64_t j = hi - step;
+            if (u[j] < S) {
+                lo = j + 1;
+                break;
+            }
+            hi = j;
+        }
+    }
     while (lo < hi) {
         const uint64_t mid = lo + (hi - lo) / 2;
         if (u[mid] < S)
@@ -231,36 +350,14 @@ __device__ __forceinline__ uint32_t count_below(
         else
             hi = mid;
     }
-    return (uint32_t)lo;
+    return lo;
 }
 
-// cnt_i = #{u < S_i} for i < N-1, cnt_{N-1} = M
-__global__ __launch_bounds__(kB) void k_counts(uint64_t N, uint64_t M,
-    const double *w, const double *u, const double *oh, const double *ol,
-    uint32_t *cnt)
+// floor(x) clamped to [0, m] (NaN -> 0)
+__device__ __forceinline__ uint64_t clamp_floor(double x, uint64_t m)
 {
-    double P[kItems];
-    tile_prefix(0, N, N, w, oh, ol, P);
-    const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = i0 + k;
-        if (i < N)
-            cnt[i] = i == N - 1 ? (uint32_t)M : count_below(u, M, P[k]);
-    }
-}
-
-__global__ void k_rep_small(uint64_t N, uint64_t M, uint32_t *rep)
-{
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        rep[i] = (N == 1 && i == 0) ? (uint32_t)M : 0u;
-}
-
-__global__ void k_rep_from_counts(uint64_t N, const uint32_t *cnt, uint32_t *rep)
-{
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        rep[i] = cnt[i] - (i == 0 ? 0u : cnt[i - 1]);
+    const double f = floor(x);
+    return f > 0.0 ? (f < (double)m ? (uint64_t)f : m) : 0;
 }
 
 // ---- rep -> index, general M (resample.hpp:416-466) ----
@@ -271,17 +368,93 @@ __device__ __forceinline__ uint32_t extras(uint32_t rep, uint64_t i, uint64_t K)
     return i < K ? (rep > 1 ? rep - 1 : 0u) : rep;
 }
 
+// Replication numbers rep_i = cnt_i - cnt_{i-1} with cnt_i = #{u < S_i}
+// for i < N-1 and cnt_{N-1} = M (resample.hpp:374-407).  A thread's eight
+// counts are searched in order, each from the previous count advanced by
+// M (S_i - S_{i-1}) (the first from M S_i): near the answer for every
+// scheme's uniforms.  The count before a thread's first particle is the
+// previous thread's last, or for a tile's first thread the count at the
+// previous tile's last particle, whose cumulative weight is this tile's
+// offset.  With te != nullptr the tile's extra-copy and zero-slot totals
+// of the rep -> index transform (K = min(N, M)) are written as well.
+__global__ __launch_bounds__(kB) void k_counts(uint64_t N, uint64_t M,
+    const double *w, const double *u, const double *oh, const double *ol,
+    uint32_t *rep, uint64_t K, uint32_t *te, uint32_t *tz)
+{
+    __shared__ uint32_t last[kB];
+    __shared__ uint32_t su[8];
+    double P[kItems];
+    tile_prefix(0, N, N, w, oh, ol, P);
+    const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
+    const double Md = (double)M;
+    uint32_t c[kItems];
+    uint64_t cc = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        c[k] = (uint32_t)M;
+        if (i < N - 1) {
+            const uint64_t g = k == 0
+                ? clamp_floor(__dmul_rn(P[0], Md), M)
+                : cc + clamp_floor(
+                           __dmul_rn(__dsub_rn(P[k], P[k - 1]), Md), M - cc);
+            cc = count_from(u, M, P[k], g);
+            c[k] = (uint32_t)cc;
+        }
+    }
+    last[threadIdx.x] = c[kItems - 1];
+    uint32_t prev = 0;
+    if (threadIdx.x == 0 && blockIdx.x > 0) {
+        const double S = oh[blockIdx.x];
+        prev = (uint32_t)count_from(u, M, S, clamp_floor(__dmul_rn(S, Md), M));
+    }
+    __syncthreads();
+    if (threadIdx.x > 0)
+        prev = last[threadIdx.x - 1];
+    uint32_t r[kItems];
+    uint32_t es = 0, zs = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        r[k] = c[k] - prev;
+        prev = c[k];
+        if (i < N) {
+            es += extras(r[k], i, K);
+            zs += (i < K && r[k] == 0) ? 1u : 0u;
+        }
+    }
+    st8(rep, i0, N, r);
+    if (te) {
+        uint32_t a, b2;
+        (void)group_excl_scan_u(es, threadIdx.x, su, a);
+        (void)group_excl_scan_u(zs, threadIdx.x, su + 4, b2);
+        if (threadIdx.x == 0) {
+            te[blockIdx.x] = a;
+            tz[blockIdx.x] = b2;
+        }
+    }
+}
+
+__global__ void k_rep_small(uint64_t N, uint64_t M, uint32_t *rep)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        rep[i] = (N == 1 && i == 0) ? (uint32_t)M : 0u;
+}
+
 __global__ __launch_bounds__(kB) void k_ri_tiles(uint64_t N, uint64_t K,
     const uint32_t *rep, uint32_t *te, uint32_t *tz)
 {
     __shared__ uint32_t su[8];
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     uint32_t es = 0, zs = 0;
+    uint32_t r[kItems];
+    ld8(rep, i0, N, r, 0u);
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         if (i < N) {
-            es += extras(rep[i], i, K);
-            zs += (i < K && rep[i] == 0) ? 1u : 0u;
+            es += extras(r[k], i, K);
+            zs += (i < K && r[k] == 0) ? 1u : 0u;
         }
     }
     uint32_t a, b;
@@ -300,17 +473,20 @@ __global__ __launch_bounds__(kB) void k_ri_offsets(const uint32_t *te,
     tiles_scan_u2(true, te, tz, (int)nt, teo, tzo, threadIdx.x, su);
 }
 
+// Inclusive extra-copy prefix E_i; idx_i = i for every slot i < K (the
+// zero slots' entries are overwritten by k_ri_deal); zpos[z] = i for the
+// zero slot i of rank z.
 __global__ __launch_bounds__(kB) void k_ri_write(uint64_t N, uint64_t K,
     const uint32_t *rep, const uint32_t *teo, const uint32_t *tzo,
-    uint32_t *Eincl, uint32_t *idx)
+    uint32_t *Eincl, uint32_t *idx, uint32_t *zpos)
 {
     __shared__ uint32_t su[8];
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     uint32_t es = 0, zs = 0;
     uint32_t r[kItems];
+    ld8(rep, i0, N, r, 0u);
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        r[k] = i < N ? rep[i] : 0u;
         if (i < N) {
             es += extras(r[k], i, K);
             zs += (i < K && r[k] == 0) ? 1u : 0u;
@@ -319,49 +495,53 @@ __global__ __launch_bounds__(kB) void k_ri_write(uint64_t N, uint64_t K,
     uint32_t a, b;
     uint32_t ex = group_excl_scan_u(es, threadIdx.x, su, a) + teo[blockIdx.x];
     uint32_t zx = group_excl_scan_u(zs, threadIdx.x, su + 4, b) + tzo[blockIdx.x];
+    uint32_t E[kItems], own[kItems];
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        if (i < N) {
-            ex += extras(r[k], i, K);
-            Eincl[i] = ex;
-            if (i < K) {
-                if (r[k] > 0) {
-                    idx[i] = (uint32_t)i;
-                } else {
-                    idx[i] = 0x80000000u | zx;
-                    ++zx;
-                }
-            }
+        ex += i < N ? extras(r[k], i, K) : 0u;
+        E[k] = ex;
+        own[k] = (uint32_t)i;
+        if (i < K && r[k] == 0) {
+            zpos[zx] = (uint32_t)i;
+            ++zx;
         }
     }
+    st8(Eincl, i0, N, E);
+    st8(idx, i0, K, own);
 }
 
-__device__ __forceinline__ uint32_t ub(const uint32_t *E, uint64_t n, uint32_t z)
+// The extra copies in ascending source order take the demands in order:
+// extra copy r goes to the zero slot of rank r (r < ztot) or to slot
+// K + r - ztot.  One workgroup per tile of sources: the tile's E in LDS, its
+// ranks [E_{t0-1}, E_{t0+n-1}) dealt round-robin to the threads, each
+// source found by bisection in LDS (the first j with E_j > r).  Slots past
+// M (a replication vector that does not sum to M) are not written.
+__global__ __launch_bounds__(kB) void k_ri_deal(uint64_t N, uint64_t K,
+    uint64_t M, const uint32_t *Eincl, const uint32_t *zpos,
+    const uint32_t *tzo, uint32_t nt, uint32_t *idx)
 {
-    uint64_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint64_t mid = lo + (hi - lo) / 2;
-        if (E[mid] > z)
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    return (uint32_t)lo;
-}
-
-__global__ void k_ri_resolve(uint64_t N, uint64_t K, uint64_t M,
-    const uint32_t *Eincl, const uint32_t *tzo, uint32_t ntiles, uint32_t *idx)
-{
-    const uint32_t ztot = tzo[ntiles];
-    for (uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; d < M;
-         d += (uint64_t)gridDim.x * blockDim.x) {
-        if (d < K) {
-            const uint32_t v = idx[d];
-            if (v & 0x80000000u)
-                idx[d] = ub(Eincl, N, v & 0x7FFFFFFFu);
-        } else {
-            idx[d] = ub(Eincl, N, ztot + (uint32_t)(d - K));
+    __shared__ uint32_t Es[kTile];
+    const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
+    const uint32_t n = (uint32_t)(N - t0 < (uint64_t)kTile ? N - t0 : kTile);
+    for (uint32_t q = threadIdx.x; q < n; q += kB)
+        Es[q] = Eincl[t0 + q];
+    const uint32_t beg = t0 == 0 ? 0u : Eincl[t0 - 1];
+    const uint32_t ztot = tzo[nt];
+    __syncthreads();
+    const uint32_t end = Es[n - 1];
+    for (uint32_t r = beg + threadIdx.x; r < end; r += kB) {
+        uint32_t lo = 0, hi = n - 1; // Es[n - 1] = end > r
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (Es[mid] > r)
+                hi = mid;
+            else
+                lo = mid + 1;
         }
+        const uint64_t dest =
+            r < ztot ? (uint64_t)zpos[r] : K + (uint64_t)(r - ztot);
+        if (dest < M)
+            idx[dest] = (uint32_t)(t0 + lo);
     }
 }
 
@@ -453,60 +633,129 @@ int dd_offsets(int mode, size_t n, const double *a, double *th, double *tl,
     return MCKL_HIP_OK;
 }
 
-int sequence(int scheme, mckl_hip_engine *e, size_t n, double *r, hipStream_t st)
+// Device workspace of one resampling call, carved from one allocation:
+// double-double tile totals and offsets (the weights' and the sorted
+// uniforms' scans), the rep -> index tile totals and offsets, the extra-copy
+// prefix and zero-slot positions, the systematic draw and, for sorted
+// uniforms, a copy of the draws.
+struct RsWork {
+    double *th, *tl, *oh, *ol;
+    uint32_t *te, *tz, *teo, *tzo;
+    uint32_t *E, *zpos;
+    double *u01;
+    double *c;
+};
+
+size_t rs_work_bytes(size_t N, size_t M, bool sorted)
 {
-    if (scheme == MCKL_HIP_SYSTEMATIC) {
-        // one U01Canonical<double> draw (two words), resample.hpp:258-268
-        Scratch s(st);
-        int rc = s.alloc(256);
-        if (rc)
-            return rc;
-        double *u01 = static_cast<double *>(s.p);
-        rc = rand_canonical_f64(e, 1, u01, st);
-        if (rc)
-            return rc;
-        if (n == 0)
+    const size_t nt = std::max(ceil_div(N, kTile), ceil_div(M, kTile)) + 1;
+    return 4 * align256(nt * sizeof(double)) +
+        4 * align256(nt * sizeof(uint32_t)) +
+        2 * align256(std::max<size_t>(N, 1) * sizeof(uint32_t)) + 256 +
+        (sorted ? align256(std::max<size_t>(M, 1) * sizeof(double)) : 0);
+}
+
+void rs_work_carve(void *p, size_t N, size_t M, bool sorted, RsWork &w)
+{
+    const size_t nt = std::max(ceil_div(N, kTile), ceil_div(M, kTile)) + 1;
+    const size_t od = align256(nt * sizeof(double));
+    const size_t ou = align256(nt * sizeof(uint32_t));
+    const size_t on = align256(std::max<size_t>(N, 1) * sizeof(uint32_t));
+    char *b = static_cast<char *>(p);
+    w.th = reinterpret_cast<double *>(b);
+    w.tl = reinterpret_cast<double *>(b + od);
+    w.oh = reinterpret_cast<double *>(b + 2 * od);
+    w.ol = reinterpret_cast<double *>(b + 3 * od);
+    b += 4 * od;
+    w.te = reinterpret_cast<uint32_t *>(b);
+    w.tz = reinterpret_cast<uint32_t *>(b + ou);
+    w.teo = reinterpret_cast<uint32_t *>(b + 2 * ou);
+    w.tzo = reinterpret_cast<uint32_t *>(b + 3 * ou);
+    b += 4 * ou;
+    w.E = reinterpret_cast<uint32_t *>(b);
+    w.zpos = reinterpret_cast<uint32_t *>(b + on);
+    b += 2 * on;
+    w.u01 = reinterpret_cast<double *>(b);
+    b += 256;
+    w.c = sorted ? reinterpret_cast<double *>(b) : nullptr;
+}
+
+// The helpers below take a workspace from the caller or allocate their own.
+struct OwnWork {
+    Scratch s;
+    RsWork w;
+    explicit OwnWork(hipStream_t st) : s(st) {}
+    int get(const RsWork *given, size_t N, size_t M, bool sorted,
+        const RsWork *&out)
+    {
+        if (given) {
+            out = given;
             return MCKL_HIP_OK;
-        k_systematic<<<grid_for(n), 256, 0, st>>>(n, u01, r);
-        MCKL_HIP_CHECK_LAUNCH("k_systematic");
+        }
+        const int rc = s.alloc(rs_work_bytes(N, M, sorted));
+        if (rc)
+            return rc;
+        rs_work_carve(s.p, N, M, sorted, w);
+        out = &w;
         return MCKL_HIP_OK;
     }
-    if (n == 0)
-        return MCKL_HIP_OK;
-    int rc = rand_canonical_f64(e, n, r, st);
-    if (rc)
-        return rc;
-    if (scheme == MCKL_HIP_STRATIFIED) {
+};
+
+int sequence(int scheme, mckl_hip_engine *e, size_t n, double *r,
+    hipStream_t st, const RsWork *given = nullptr)
+{
+    const bool sorted = scheme == MCKL_HIP_MULTINOMIAL;
+    if (!sorted && scheme != MCKL_HIP_SYSTEMATIC) {
+        if (n == 0)
+            return MCKL_HIP_OK;
+        const int rc = rand_canonical_f64(e, n, r, st);
+        if (rc)
+            return rc;
         k_stratified<<<grid_for(n), 256, 0, st>>>(n, r);
         MCKL_HIP_CHECK_LAUNCH("k_stratified");
         return MCKL_HIP_OK;
     }
-    // sorted (multinomial)
+    if (sorted && n == 0)
+        return MCKL_HIP_OK;
+    OwnWork own(st);
+    const RsWork *w = nullptr;
+    int rc = own.get(given, sorted ? n : 0, n, sorted, w);
+    if (rc)
+        return rc;
+    if (!sorted) {
+        // one U01Canonical<double> draw (two words), resample.hpp:258-268
+        rc = rand_canonical_f64(e, 1, w->u01, st);
+        if (rc)
+            return rc;
+        if (n == 0)
+            return MCKL_HIP_OK;
+        k_systematic<<<grid_for(n), 256, 0, st>>>(n, w->u01, r);
+        MCKL_HIP_CHECK_LAUNCH("k_systematic");
+        return MCKL_HIP_OK;
+    }
+    rc = rand_canonical_f64(e, n, r, st);
+    if (rc)
+        return rc;
     const uint32_t nt = (uint32_t)ceil_div(n, kTile);
-    Scratch s(st);
-    const size_t o = align256((nt + 1) * sizeof(double));
-    rc = s.alloc(4 * o + align256(n * sizeof(double)));
-    if (rc)
-        return rc;
-    char *base = static_cast<char *>(s.p);
-    double *th = reinterpret_cast<double *>(base);
-    double *tl = reinterpret_cast<double *>(base + o);
-    double *oh = reinterpret_cast<double *>(base + 2 * o);
-    double *ol = reinterpret_cast<double *>(base + 3 * o);
-    double *c = reinterpret_cast<double *>(base + 4 * o);
-    MCKL_HIP_TRY(hipMemcpyAsync(c, r, n * sizeof(double),
+    MCKL_HIP_TRY(hipMemcpyAsync(w->c, r, n * sizeof(double),
         hipMemcpyDeviceToDevice, st));
-    rc = dd_offsets(1, n, c, th, tl, oh, ol, st);
+    rc = dd_offsets(1, n, w->c, w->th, w->tl, w->oh, w->ol, st);
     if (rc)
         return rc;
-    k_sorted_finish<<<nt, kB, 0, st>>>(n, c, oh, ol, r);
+    k_sorted_finish<<<nt, kB, 0, st>>>(n, w->c, w->oh, w->ol, r);
     MCKL_HIP_CHECK_LAUNCH("k_sorted_finish");
     return MCKL_HIP_OK;
 }
 
-int trans_u01_rep(size_t N, size_t M, const double *w, const double *u,
-    uint32_t *rep, hipStream_t st)
+// rep from the weights and M ascending uniforms; with tiles, the
+// rep -> index tile totals go to w->te/w->tz as well (returns in *tiles
+// whether they were written)
+int trans_u01_rep(size_t N, size_t M, const double *wt, const double *u,
+    uint32_t *rep, hipStream_t st, const RsWork *given = nullptr,
+    bool tiles = false, bool *tiles_done = nullptr)
 {
+    if (tiles_done)
+        *tiles_done = false;
     if (N == 0)
         return MCKL_HIP_OK;
     if (N == 1 || M == 0) {
@@ -515,49 +764,42 @@ int trans_u01_rep(size_t N, size_t M, const double *w, const double *u,
         MCKL_HIP_CHECK_LAUNCH("k_rep_small");
         return MCKL_HIP_OK;
     }
+    OwnWork own(st);
+    const RsWork *w = nullptr;
+    int rc = own.get(given, N, M, false, w);
+    if (rc)
+        return rc;
     const uint32_t nt = (uint32_t)ceil_div(N, kTile);
-    Scratch s(st);
-    const size_t o = align256((nt + 1) * sizeof(double));
-    int rc = s.alloc(4 * o + align256(N * sizeof(uint32_t)));
+    rc = dd_offsets(0, N, wt, w->th, w->tl, w->oh, w->ol, st);
     if (rc)
         return rc;
-    char *base = static_cast<char *>(s.p);
-    double *th = reinterpret_cast<double *>(base);
-    double *tl = reinterpret_cast<double *>(base + o);
-    double *oh = reinterpret_cast<double *>(base + 2 * o);
-    double *ol = reinterpret_cast<double *>(base + 3 * o);
-    uint32_t *cnt = reinterpret_cast<uint32_t *>(base + 4 * o);
-    rc = dd_offsets(0, N, w, th, tl, oh, ol, st);
-    if (rc)
-        return rc;
-    k_counts<<<nt, kB, 0, st>>>(N, M, w, u, oh, ol, cnt);
-    k_rep_from_counts<<<grid_for(N), 256, 0, st>>>(N, cnt, rep);
-    MCKL_HIP_CHECK_LAUNCH("replication kernels");
+    k_counts<<<nt, kB, 0, st>>>(N, M, wt, u, w->oh, w->ol, rep,
+        std::min(N, M), tiles ? w->te : nullptr, tiles ? w->tz : nullptr);
+    MCKL_HIP_CHECK_LAUNCH("k_counts");
+    if (tiles_done)
+        *tiles_done = tiles;
     return MCKL_HIP_OK;
 }
 
+// stable rep -> index; have_tiles: w->te/w->tz already hold the tile totals
 int rep_index(size_t N, size_t M, const uint32_t *rep, uint32_t *idx,
-    hipStream_t st)
+    hipStream_t st, const RsWork *given = nullptr, bool have_tiles = false)
 {
     if (N == 0 || M == 0)
         return MCKL_HIP_OK;
     const size_t K = std::min(N, M);
     const uint32_t nt = (uint32_t)ceil_div(N, kTile);
-    Scratch s(st);
-    const size_t o = align256((nt + 1) * sizeof(uint32_t));
-    int rc = s.alloc(4 * o + align256(N * sizeof(uint32_t)));
+    OwnWork own(st);
+    const RsWork *w = nullptr;
+    int rc = own.get(given, N, M, false, w);
     if (rc)
         return rc;
-    char *base = static_cast<char *>(s.p);
-    uint32_t *te = reinterpret_cast<uint32_t *>(base);
-    uint32_t *tz = reinterpret_cast<uint32_t *>(base + o);
-    uint32_t *teo = reinterpret_cast<uint32_t *>(base + 2 * o);
-    uint32_t *tzo = reinterpret_cast<uint32_t *>(base + 3 * o);
-    uint32_t *E = reinterpret_cast<uint32_t *>(base + 4 * o);
-    k_ri_tiles<<<nt, kB, 0, st>>>(N, K, rep, te, tz);
-    k_ri_offsets<<<1, kB, 0, st>>>(te, tz, nt, teo, tzo);
-    k_ri_write<<<nt, kB, 0, st>>>(N, K, rep, teo, tzo, E, idx);
-    k_ri_resolve<<<grid_for(M), 256, 0, st>>>(N, K, M, E, tzo, nt, idx);
+    if (!have_tiles)
+        k_ri_tiles<<<nt, kB, 0, st>>>(N, K, rep, w->te, w->tz);
+    k_ri_offsets<<<1, kB, 0, st>>>(w->te, w->tz, nt, w->teo, w->tzo);
+    k_ri_write<<<nt, kB, 0, st>>>(
+        N, K, rep, w->teo, w->tzo, w->E, idx, w->zpos);
+    k_ri_deal<<<nt, kB, 0, st>>>(N, K, M, w->E, w->zpos, w->tzo, nt, idx);
     MCKL_HIP_CHECK_LAUNCH("rep->index kernels");
     return MCKL_HIP_OK;
 }
@@ -582,17 +824,22 @@ __global__ __launch_bounds__(kB) void k_residual_tiles(uint64_t N, double Md,
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     dd loc{0.0, 0.0};
     unsigned long long isum = 0;
+    double x[kItems];
+    uint32_t ig[kItems];
+    ld8(w, i0, N, x, 0.0);
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
+        double ip = 0.0, r = 0.0;
         if (i < N) {
-            double ip;
-            const double r = modf(__dmul_rn(Md, w[i]), &ip);
-            resid[i] = r;
-            integ[i] = (uint32_t)ip;
+            r = modf(__dmul_rn(Md, x[k]), &ip);
             loc = dd_add(loc, dd{r, 0.0});
             isum += (unsigned long long)ip;
         }
+        x[k] = r;
+        ig[k] = (uint32_t)ip;
     }
+    st8(resid, i0, N, x);
+    st8(integ, i0, N, ig);
     dd tot;
     (void)group_excl_scan_dd(loc, threadIdx.x, sd, tot);
 #pragma unroll
@@ -658,18 +905,27 @@ __global__ void k_add_integ(uint64_t N, uint32_t *rep, const uint32_t *integ)
 // ResampleAlgorithm<U01Seq, true>::eval (resample.hpp:555-576): residuals,
 // R uniforms of the base scheme over them, plus the integral parts.  R is
 // read back to the host (the uniform sequence advances the engine by R).
+// ws: a workspace of at least residual_ws_bytes(N, M, base) bytes, whose
+// RsWork part the caller may reuse for rep -> index afterwards.
+size_t residual_ws_bytes(size_t N, size_t M, int base)
+{
+    const size_t nt = ceil_div(N, kTile) + 1;
+    return 3 * align256(nt * sizeof(double)) +
+        align256(std::max<size_t>(N, 1) * sizeof(double)) +
+        align256(std::max<size_t>(N, 1) * sizeof(uint32_t)) + 512 +
+        align256(std::max<size_t>(M, 1) * sizeof(double)) +
+        rs_work_bytes(N, M, base == MCKL_HIP_MULTINOMIAL);
+}
+
 int residual_rep(int base, size_t N, size_t M, mckl_hip_engine *e,
-    const double *w, uint32_t *rep, hipStream_t st)
+    const double *w, uint32_t *rep, hipStream_t st, void *ws, RsWork &work)
 {
     const uint32_t nt = (uint32_t)ceil_div(N, kTile);
-    Scratch s(st);
-    const size_t o = align256(nt * sizeof(double));
-    const size_t on = align256(N * sizeof(double));
-    const size_t oi = align256(N * sizeof(uint32_t));
-    int rc = s.alloc(3 * o + on + oi + 512);
-    if (rc)
-        return rc;
-    char *base_p = static_cast<char *>(s.p);
+    const size_t o = align256((nt + 1) * sizeof(double));
+    const size_t on = align256(std::max<size_t>(N, 1) * sizeof(double));
+    const size_t oi = align256(std::max<size_t>(N, 1) * sizeof(uint32_t));
+    const size_t om = align256(std::max<size_t>(M, 1) * sizeof(double));
+    char *base_p = static_cast<char *>(ws);
     double *th = reinterpret_cast<double *>(base_p);
     double *tl = reinterpret_cast<double *>(base_p + o);
     unsigned long long *ti =
@@ -679,6 +935,9 @@ int residual_rep(int base, size_t N, size_t M, mckl_hip_engine *e,
     double *mul = reinterpret_cast<double *>(base_p + 3 * o + on + oi);
     unsigned long long *dR = reinterpret_cast<unsigned long long *>(
         base_p + 3 * o + on + oi + 256);
+    double *u = reinterpret_cast<double *>(base_p + 3 * o + on + oi + 512);
+    rs_work_carve(base_p + 3 * o + on + oi + 512 + om, N, M,
+        base == MCKL_HIP_MULTINOMIAL, work);
     k_residual_tiles<<<nt, kB, 0, st>>>(
         N, (double)M, w, resid, integ, th, tl, ti);
     k_residual_total<<<1, kB, 0, st>>>(th, tl, ti, nt, M, mul, dR);
@@ -690,15 +949,10 @@ int residual_rep(int base, size_t N, size_t M, mckl_hip_engine *e,
     if (R > M)
         return set_error(MCKL_HIP_EINVAL,
             "weights do not sum to one (integral parts exceed M)");
-    Scratch su(st);
-    rc = su.alloc(align256(std::max<size_t>(R, 1) * sizeof(double)));
+    int rc = sequence(base, e, R, u, st, &work);
     if (rc)
         return rc;
-    double *u = static_cast<double *>(su.p);
-    rc = sequence(base, e, R, u, st);
-    if (rc)
-        return rc;
-    rc = trans_u01_rep(N, R, resid, u, rep, st);
+    rc = trans_u01_rep(N, R, resid, u, rep, st, &work);
     if (rc)
         return rc;
     k_add_integ<<<grid_for(N), 256, 0, st>>>(N, rep, integ);
@@ -921,26 +1175,38 @@ int mckl_hip_resample(int scheme, size_t N, size_t M, mckl_hip_engine *e,
     if (residual) {
         if (N == 0)
             return MCKL_HIP_OK;
-        rc = residual_rep(scheme - MCKL_HIP_RESIDUAL, N, M, e, d_w, d_rep, st);
+        const int base = scheme - MCKL_HIP_RESIDUAL;
+        Scratch s(st);
+        rc = s.alloc(residual_ws_bytes(N, M, base));
+        if (rc)
+            return rc;
+        RsWork w;
+        rc = residual_rep(base, N, M, e, d_w, d_rep, st, s.p, w);
         if (rc)
             return rc;
         if (d_idx)
-            rc = rep_index(N, M, d_rep, d_idx, st);
+            rc = rep_index(N, M, d_rep, d_idx, st, &w);
         return rc;
     }
+    // one workspace for the uniforms and every helper
+    const bool sorted = scheme == MCKL_HIP_MULTINOMIAL;
+    const size_t ou = align256(std::max<size_t>(M, 1) * sizeof(double));
     Scratch s(st);
-    rc = s.alloc(align256(std::max<size_t>(M, 1) * sizeof(double)));
+    rc = s.alloc(ou + rs_work_bytes(N, M, sorted));
     if (rc)
         return rc;
     double *u = static_cast<double *>(s.p);
-    rc = sequence(scheme, e, M, u, st);
+    RsWork w;
+    rs_work_carve(static_cast<char *>(s.p) + ou, N, M, sorted, w);
+    rc = sequence(scheme, e, M, u, st, &w);
     if (rc)
         return rc;
-    rc = trans_u01_rep(N, M, d_w, u, d_rep, st);
+    bool tiles = false;
+    rc = trans_u01_rep(N, M, d_w, u, d_rep, st, &w, d_idx != nullptr, &tiles);
     if (rc)
         return rc;
     if (d_idx)
-        rc = rep_index(N, M, d_rep, d_idx, st);
+        rc = rep_index(N, M, d_rep, d_idx, st, &w, tiles);
     return rc;
 }
 
