@@ -369,10 +369,14 @@ __device__ __forceinline__ uint32_t extras(uint32_t rep, uint64_t i, uint64_t K)
 }
 
 // Replication numbers rep_i = cnt_i - cnt_{i-1} with cnt_i = #{u < S_i}
-// for i < N-1 and cnt_{N-1} = M (resample.hpp:374-407).  A thread's eight
-// counts are searched in order, each from the previous count advanced by
-// M (S_i - S_{i-1}) (the first from M S_i): near the answer for every
-// scheme's uniforms.  The count before a thread's first particle is the
+// for i < N-1 and cnt_{N-1} = M (resample.hpp:374-407).  For each of a
+// thread's eight cumulative weights the uniforms g-1, g, g+1 around the
+// guess g = floor(M S_i) are read first, all 24 loads in flight: a count
+// the window brackets is final (always so for stratified and systematic
+// uniforms away from rounding ties, where u_j lies in [j/M, (j+1)/M]).
+// The other counts are searched in order (count_from), each from the
+// previous count advanced by M (S_i - S_{i-1}): near the answer for sorted
+// uniforms too.  The count before a thread's first particle is the
 // previous thread's last, or for a tile's first thread the count at the
 // previous tile's last particle, whose cumulative weight is this tile's
 // offset.  With te != nullptr the tile's extra-copy and zero-slot totals
@@ -387,6 +391,17 @@ __global__ __launch_bounds__(kB) void k_counts(uint64_t N, uint64_t M,
     tile_prefix(0, N, N, w, oh, ol, P);
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     const double Md = (double)M;
+    const double inf = __builtin_huge_val();
+    uint64_t g[kItems];
+    double ua[kItems], ub[kItems], uc[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const bool live = i0 + k < N - 1;
+        g[k] = clamp_floor(__dmul_rn(P[k], Md), M);
+        ua[k] = live && g[k] >= 1 ? u[g[k] - 1] : -inf;
+        ub[k] = live && g[k] < M ? u[g[k]] : inf;
+        uc[k] = live && g[k] + 1 < M ? u[g[k] + 1] : inf;
+    }
     uint32_t c[kItems];
     uint64_t cc = 0;
 #pragma unroll
@@ -394,11 +409,18 @@ __global__ __launch_bounds__(kB) void k_counts(uint64_t N, uint64_t M,
         const uint64_t i = i0 + k;
         c[k] = (uint32_t)M;
         if (i < N - 1) {
-            const uint64_t g = k == 0
-                ? clamp_floor(__dmul_rn(P[0], Md), M)
-                : cc + clamp_floor(
-                           __dmul_rn(__dsub_rn(P[k], P[k - 1]), Md), M - cc);
-            cc = count_from(u, M, P[k], g);
+            const double S = P[k];
+            if (ua[k] < S && !(ub[k] < S)) {
+                cc = g[k];
+            } else if (ub[k] < S && !(uc[k] < S)) {
+                cc = g[k] + 1;
+            } else {
+                const uint64_t h = k == 0
+                    ? g[0]
+                    : cc + clamp_floor(
+                               __dmul_rn(__dsub_rn(S, P[k - 1]), Md), M - cc);
+                cc = count_from(u, M, S, h);
+            }
             c[k] = (uint32_t)cc;
         }
     }
