@@ -138,12 +138,12 @@ __global__ __launch_bounds__(kB) void k_weight_pre(int mode, uint64_t n,
     }
 }
 
-__global__ __launch_bounds__(kB) void k_weight_max(
-    const double *pmax, uint32_t nb, double *lmax)
+// max over the n block maxima (every thread gets it)
+__device__ __forceinline__ double blocks_max(const double *pmax, uint32_t n,
+    double *sm)
 {
-    __shared__ double sm[kB / 64];
     double m = -__builtin_huge_val();
-    for (uint32_t i = threadIdx.x; i < nb; i += kB)
+    for (uint32_t i = threadIdx.x; i < n; i += kB)
         m = m < pmax[i] ? pmax[i] : m;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -153,21 +153,23 @@ __global__ __launch_bounds__(kB) void k_weight_max(
     if ((threadIdx.x & 63) == 0)
         sm[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = sm[0];
-        for (int i = 1; i < kB / 64; ++i)
-            r = r < sm[i] ? sm[i] : r;
-        *lmax = r;
-    }
+    double r = sm[0];
+    for (int i = 1; i < kB / 64; ++i)
+        r = r < sm[i] ? sm[i] : r;
+    __syncthreads();
+    return r;
 }
 
-// w = exp(w - lmax); block partial sums of w and w^2
+// w = exp(w - lmax), lmax the max of the block maxima (taken by every
+// workgroup; block 0 keeps it in *lmax); block partial sums of w and w^2
 __global__ __launch_bounds__(kB) void k_weight_exp(uint64_t n, double *w,
-    const double *lmax, double *ps, double *ps2)
+    const double *pmax, uint32_t nb, double *lmax, double *ps, double *ps2)
 {
     __shared__ double sd[8];
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
-    const double m = *lmax;
+    const double m = blocks_max(pmax, nb, sd);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *lmax = m;
     double a = 0.0, b = 0.0;
     double x[kItems];
     ld8(w, i0, n, x, 0.0);
@@ -188,27 +190,28 @@ __global__ __launch_bounds__(kB) void k_weight_exp(uint64_t n, double *w,
     }
 }
 
-// stats = {ESS, lmax + log(accw)}, scale = 1/accw
-__global__ __launch_bounds__(kB) void k_weight_total(const double *ps,
-    const double *ps2, uint32_t nb, const double *lmax, double *stats,
-    double *scale)
+// accw and the sum of squares from the block partials (the same sums in
+// every workgroup), w *= 1/accw on the workgroup's tile; block 0 writes
+// stats = {ESS, lmax + log(accw)}
+__global__ __launch_bounds__(kB) void k_weight_scale(uint64_t n, double *w,
+    const double *ps, const double *ps2, uint32_t nb, const double *lmax,
+    double *stats)
 {
     __shared__ double sd[8];
     double a, b;
     tiles_sum2(ps, ps2, (int)nb, threadIdx.x, sd, a, b);
-    if (threadIdx.x == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         stats[0] = __ddiv_rn(__dmul_rn(a, a), b);
         stats[1] = __dadd_rn(*lmax, log(a));
-        *scale = __ddiv_rn(1.0, a);
     }
-}
-
-__global__ void k_weight_scale(uint64_t n, double *w, const double *scale)
-{
-    const double s = *scale;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        w[i] = __dmul_rn(s, w[i]);
+    const double sc = __ddiv_rn(1.0, a);
+    const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
+    double x[kItems];
+    ld8(w, i0, n, x, 0.0);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        x[k] = __dmul_rn(sc, x[k]);
+    st8(w, i0, n, x);
 }
 
 __global__ void k_weight_equal(uint64_t n, double *w, double *stats)
@@ -253,20 +256,22 @@ __global__ __launch_bounds__(kB) void k_dd_tiles(int mode, uint64_t n,
     }
 }
 
-__global__ __launch_bounds__(kB) void k_dd_offsets(const double *th,
-    const double *tl, uint32_t nt, double *oh, double *ol)
-{
-    __shared__ double sd[8];
-    tiles_scan_dd(true, th, tl, (int)nt, oh, ol, threadIdx.x, sd);
-}
-
 // Inclusive prefix of element i, rounded: the last element of a tile takes
-// the next tile's offset (the same convention as the SMC samplers).
+// the next tile's offset (the same convention as the SMC samplers).  The
+// tile's offsets come from the nt tile totals th/tl (tiles_offset_dd: the
+// operations of a scan over the tiles, taken by every workgroup for
+// itself); off_hi: this tile's offset rounded (the cumulative weight of the
+// previous tile's last element).
 __device__ __forceinline__ void tile_prefix(int mode, uint64_t n, uint64_t N,
-    const double *a, const double *oh, const double *ol, double *P)
+    const double *a, const double *th, const double *tl, uint32_t nt,
+    double *P, double &off_hi)
 {
-    __shared__ double sd[8];
+    __shared__ double sd[16];
     const uint32_t tile = blockIdx.x;
+    dd toff, tnext;
+    tiles_offset_dd(th, tl, (int)nt, (int)tile, (int)tile + 1, threadIdx.x, sd,
+        toff, tnext);
+    off_hi = toff.hi;
     const uint64_t i0 = (uint64_t)tile * kTile + threadIdx.x * kItems;
     const uint64_t tend = (uint64_t)(tile + 1) * kTile < n
         ? (uint64_t)(tile + 1) * kTile
@@ -281,22 +286,24 @@ __device__ __forceinline__ void tile_prefix(int mode, uint64_t n, uint64_t N,
     }
     dd tot;
     const dd off = group_excl_scan_dd(loc, threadIdx.x, sd, tot);
-    const dd base = dd_add(dd{oh[tile], ol[tile]}, off);
+    const dd base = dd_add(toff, off);
     dd run{0.0, 0.0};
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         run = dd_add(run, dd{t[k], 0.0});
-        P[k] = i == tend - 1 ? oh[tile + 1] : dd_add(base, run).hi;
+        P[k] = i == tend - 1 ? tnext.hi : dd_add(base, run).hi;
     }
 }
 
 // sorted uniforms: r_j = 1 - exp(lm_j), lm = prefix of log(c_j)/(N-j)
 // (resample.hpp:63-81)
 __global__ __launch_bounds__(kB) void k_sorted_finish(uint64_t n,
-    const double *c, const double *oh, const double *ol, double *r)
+    const double *c, const double *th, const double *tl, uint32_t nt,
+    double *r)
 {
     double P[kItems];
-    tile_prefix(1, n, n, c, oh, ol, P);
+    double off_hi;
+    tile_prefix(1, n, n, c, th, tl, nt, P, off_hi);
     __syncthreads();
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     double v[kItems];
@@ -382,13 +389,14 @@ __device__ __forceinline__ uint32_t extras(uint32_t rep, uint64_t i, uint64_t K)
 // offset.  With te != nullptr the tile's extra-copy and zero-slot totals
 // of the rep -> index transform (K = min(N, M)) are written as well.
 __global__ __launch_bounds__(kB) void k_counts(uint64_t N, uint64_t M,
-    const double *w, const double *u, const double *oh, const double *ol,
-    uint32_t *rep, uint64_t K, uint32_t *te, uint32_t *tz)
+    const double *w, const double *u, const double *th, const double *tl,
+    uint32_t nt, uint32_t *rep, uint64_t K, uint32_t *te, uint32_t *tz)
 {
     __shared__ uint32_t last[kB];
     __shared__ uint32_t su[8];
     double P[kItems];
-    tile_prefix(0, N, N, w, oh, ol, P);
+    double off_hi;
+    tile_prefix(0, N, N, w, th, tl, nt, P, off_hi);
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     const double Md = (double)M;
     const double inf = __builtin_huge_val();
@@ -427,7 +435,7 @@ __global__ __launch_bounds__(kB) void k_counts(uint64_t N, uint64_t M,
     last[threadIdx.x] = c[kItems - 1];
     uint32_t prev = 0;
     if (threadIdx.x == 0 && blockIdx.x > 0) {
-        const double S = oh[blockIdx.x];
+        const double S = off_hi;
         prev = (uint32_t)count_from(u, M, S, clamp_floor(__dmul_rn(S, Md), M));
     }
     __syncthreads();
@@ -488,21 +496,32 @@ __global__ __launch_bounds__(kB) void k_ri_tiles(uint64_t N, uint64_t K,
     }
 }
 
-__global__ __launch_bounds__(kB) void k_ri_offsets(const uint32_t *te,
-    const uint32_t *tz, uint32_t nt, uint32_t *teo, uint32_t *tzo)
+// sums of a[0..j) and b[0..j) and of all n entries of b (every thread)
+__device__ __forceinline__ void tiles_before_u2(const uint32_t *a,
+    const uint32_t *b, uint32_t n, uint32_t j, uint32_t *su, uint32_t &ab,
+    uint32_t &bb, uint32_t &btot)
 {
-    __shared__ uint32_t su[8];
-    tiles_scan_u2(true, te, tz, (int)nt, teo, tzo, threadIdx.x, su);
+    uint32_t x = 0, y = 0, z = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += kB) {
+        x += i < j ? a[i] : 0u;
+        y += i < j ? b[i] : 0u;
+        z += b[i];
+    }
+    (void)group_excl_scan_u(x, threadIdx.x, su, ab);
+    (void)group_excl_scan_u(y, threadIdx.x, su + 4, bb);
+    (void)group_excl_scan_u(z, threadIdx.x, su + 8, btot);
 }
 
 // Inclusive extra-copy prefix E_i; idx_i = i for every slot i < K (the
 // zero slots' entries are overwritten by k_ri_deal); zpos[z] = i for the
 // zero slot i of rank z.
 __global__ __launch_bounds__(kB) void k_ri_write(uint64_t N, uint64_t K,
-    const uint32_t *rep, const uint32_t *teo, const uint32_t *tzo,
+    const uint32_t *rep, const uint32_t *te, const uint32_t *tz, uint32_t nt,
     uint32_t *Eincl, uint32_t *idx, uint32_t *zpos)
 {
-    __shared__ uint32_t su[8];
+    __shared__ uint32_t su[12];
+    uint32_t eoff, zoff, ztot;
+    tiles_before_u2(te, tz, nt, blockIdx.x, su, eoff, zoff, ztot);
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     uint32_t es = 0, zs = 0;
     uint32_t r[kItems];
@@ -515,8 +534,8 @@ __global__ __launch_bounds__(kB) void k_ri_write(uint64_t N, uint64_t K,
         }
     }
     uint32_t a, b;
-    uint32_t ex = group_excl_scan_u(es, threadIdx.x, su, a) + teo[blockIdx.x];
-    uint32_t zx = group_excl_scan_u(zs, threadIdx.x, su + 4, b) + tzo[blockIdx.x];
+    uint32_t ex = group_excl_scan_u(es, threadIdx.x, su, a) + eoff;
+    uint32_t zx = group_excl_scan_u(zs, threadIdx.x, su + 4, b) + zoff;
     uint32_t E[kItems], own[kItems];
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
@@ -540,15 +559,20 @@ __global__ __launch_bounds__(kB) void k_ri_write(uint64_t N, uint64_t K,
 // M (a replication vector that does not sum to M) are not written.
 __global__ __launch_bounds__(kB) void k_ri_deal(uint64_t N, uint64_t K,
     uint64_t M, const uint32_t *Eincl, const uint32_t *zpos,
-    const uint32_t *tzo, uint32_t nt, uint32_t *idx)
+    const uint32_t *tz, uint32_t nt, uint32_t *idx)
 {
     __shared__ uint32_t Es[kTile];
+    __shared__ uint32_t su[4];
+    uint32_t z = 0;
+    for (uint32_t i = threadIdx.x; i < nt; i += kB)
+        z += tz[i];
+    uint32_t ztot;
+    (void)group_excl_scan_u(z, threadIdx.x, su, ztot);
     const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
     const uint32_t n = (uint32_t)(N - t0 < (uint64_t)kTile ? N - t0 : kTile);
     for (uint32_t q = threadIdx.x; q < n; q += kB)
         Es[q] = Eincl[t0 + q];
     const uint32_t beg = t0 == 0 ? 0u : Eincl[t0 - 1];
-    const uint32_t ztot = tzo[nt];
     __syncthreads();
     const uint32_t end = Es[n - 1];
     for (uint32_t r = beg + threadIdx.x; r < end; r += kB) {
@@ -634,35 +658,32 @@ int weight_normalize(int mode, size_t n, const double *v, double *w,
     double *ps = reinterpret_cast<double *>(base + o1);
     double *ps2 = reinterpret_cast<double *>(base + 2 * o1);
     double *lmax = reinterpret_cast<double *>(base + 3 * o1);
-    double *scale = lmax + 1;
     k_weight_pre<<<nt, kB, 0, st>>>(mode, n, v, w, pmax);
-    k_weight_max<<<1, kB, 0, st>>>(pmax, nt, lmax);
-    k_weight_exp<<<nt, kB, 0, st>>>(n, w, lmax, ps, ps2);
-    k_weight_total<<<1, kB, 0, st>>>(ps, ps2, nt, lmax, stats, scale);
-    k_weight_scale<<<grid_for(n), 256, 0, st>>>(n, w, scale);
+    k_weight_exp<<<nt, kB, 0, st>>>(n, w, pmax, nt, lmax, ps, ps2);
+    k_weight_scale<<<nt, kB, 0, st>>>(n, w, ps, ps2, nt, lmax, stats);
     MCKL_HIP_CHECK_LAUNCH("weight kernels");
     return MCKL_HIP_OK;
 }
 
-// dd prefix offsets of the scan terms; returns offsets in oh/ol (nt+1)
-int dd_offsets(int mode, size_t n, const double *a, double *th, double *tl,
-    double *oh, double *ol, hipStream_t st)
+// double-double tile totals of the scan terms (nt of them); the consumers
+// take their tile offsets from these
+int dd_tiles(int mode, size_t n, const double *a, double *th, double *tl,
+    hipStream_t st)
 {
     const uint32_t nt = (uint32_t)ceil_div(n, kTile);
     k_dd_tiles<<<nt, kB, 0, st>>>(mode, n, n, a, th, tl);
-    k_dd_offsets<<<1, kB, 0, st>>>(th, tl, nt, oh, ol);
-    MCKL_HIP_CHECK_LAUNCH("dd scan kernels");
+    MCKL_HIP_CHECK_LAUNCH("k_dd_tiles");
     return MCKL_HIP_OK;
 }
 
 // Device workspace of one resampling call, carved from one allocation:
-// double-double tile totals and offsets (the weights' and the sorted
-// uniforms' scans), the rep -> index tile totals and offsets, the extra-copy
-// prefix and zero-slot positions, the systematic draw and, for sorted
-// uniforms, a copy of the draws.
+// double-double tile totals (the weights' and the sorted uniforms' scans),
+// the rep -> index tile totals, the extra-copy prefix and zero-slot
+// positions, the systematic draw and, for sorted uniforms, a copy of the
+// draws.
 struct RsWork {
-    double *th, *tl, *oh, *ol;
-    uint32_t *te, *tz, *teo, *tzo;
+    double *th, *tl;
+    uint32_t *te, *tz;
     uint32_t *E, *zpos;
     double *u01;
     double *c;
@@ -671,8 +692,8 @@ struct RsWork {
 size_t rs_work_bytes(size_t N, size_t M, bool sorted)
 {
     const size_t nt = std::max(ceil_div(N, kTile), ceil_div(M, kTile)) + 1;
-    return 4 * align256(nt * sizeof(double)) +
-        4 * align256(nt * sizeof(uint32_t)) +
+    return 2 * align256(nt * sizeof(double)) +
+        2 * align256(nt * sizeof(uint32_t)) +
         2 * align256(std::max<size_t>(N, 1) * sizeof(uint32_t)) + 256 +
         (sorted ? align256(std::max<size_t>(M, 1) * sizeof(double)) : 0);
 }
@@ -686,14 +707,10 @@ void rs_work_carve(void *p, size_t N, size_t M, bool sorted, RsWork &w)
     char *b = static_cast<char *>(p);
     w.th = reinterpret_cast<double *>(b);
     w.tl = reinterpret_cast<double *>(b + od);
-    w.oh = reinterpret_cast<double *>(b + 2 * od);
-    w.ol = reinterpret_cast<double *>(b + 3 * od);
-    b += 4 * od;
+    b += 2 * od;
     w.te = reinterpret_cast<uint32_t *>(b);
     w.tz = reinterpret_cast<uint32_t *>(b + ou);
-    w.teo = reinterpret_cast<uint32_t *>(b + 2 * ou);
-    w.tzo = reinterpret_cast<uint32_t *>(b + 3 * ou);
-    b += 4 * ou;
+    b += 2 * ou;
     w.E = reinterpret_cast<uint32_t *>(b);
     w.zpos = reinterpret_cast<uint32_t *>(b + on);
     b += 2 * on;
@@ -761,10 +778,10 @@ int sequence(int scheme, mckl_hip_engine *e, size_t n, double *r,
     const uint32_t nt = (uint32_t)ceil_div(n, kTile);
     MCKL_HIP_TRY(hipMemcpyAsync(w->c, r, n * sizeof(double),
         hipMemcpyDeviceToDevice, st));
-    rc = dd_offsets(1, n, w->c, w->th, w->tl, w->oh, w->ol, st);
+    rc = dd_tiles(1, n, w->c, w->th, w->tl, st);
     if (rc)
         return rc;
-    k_sorted_finish<<<nt, kB, 0, st>>>(n, w->c, w->oh, w->ol, r);
+    k_sorted_finish<<<nt, kB, 0, st>>>(n, w->c, w->th, w->tl, nt, r);
     MCKL_HIP_CHECK_LAUNCH("k_sorted_finish");
     return MCKL_HIP_OK;
 }
@@ -792,10 +809,10 @@ int trans_u01_rep(size_t N, size_t M, const double *wt, const double *u,
     if (rc)
         return rc;
     const uint32_t nt = (uint32_t)ceil_div(N, kTile);
-    rc = dd_offsets(0, N, wt, w->th, w->tl, w->oh, w->ol, st);
+    rc = dd_tiles(0, N, wt, w->th, w->tl, st);
     if (rc)
         return rc;
-    k_counts<<<nt, kB, 0, st>>>(N, M, wt, u, w->oh, w->ol, rep,
+    k_counts<<<nt, kB, 0, st>>>(N, M, wt, u, w->th, w->tl, nt, rep,
         std::min(N, M), tiles ? w->te : nullptr, tiles ? w->tz : nullptr);
     MCKL_HIP_CHECK_LAUNCH("k_counts");
     if (tiles_done)
@@ -818,10 +835,9 @@ int rep_index(size_t N, size_t M, const uint32_t *rep, uint32_t *idx,
         return rc;
     if (!have_tiles)
         k_ri_tiles<<<nt, kB, 0, st>>>(N, K, rep, w->te, w->tz);
-    k_ri_offsets<<<1, kB, 0, st>>>(w->te, w->tz, nt, w->teo, w->tzo);
     k_ri_write<<<nt, kB, 0, st>>>(
-        N, K, rep, w->teo, w->tzo, w->E, idx, w->zpos);
-    k_ri_deal<<<nt, kB, 0, st>>>(N, K, M, w->E, w->zpos, w->tzo, nt, idx);
+        N, K, rep, w->te, w->tz, nt, w->E, idx, w->zpos);
+    k_ri_deal<<<nt, kB, 0, st>>>(N, K, M, w->E, w->zpos, w->tz, nt, idx);
     MCKL_HIP_CHECK_LAUNCH("rep->index kernels");
     return MCKL_HIP_OK;
 }
