@@ -771,6 +771,7 @@ class RNGSetVector:
 
 
 from . import pmcmc  # noqa: E402,F401  (PMCMCMutation, PMCMCBatch)
-from .pmcmc import PMCMCMutation, PMCMCBatch  # noqa: E402,F401
+from .pmcmc import (PMCMCMutation, PMCMCBatch,  # noqa: E402,F401
+                    GenericPMCMCMutation)
 from .sampler import (Particle, ResampleEval, SMCEstimator,  # noqa: E402,F401
-                      GenericSMCSampler)
+                      GenericSMCSampler, PMCMCParticle)

@@ -1250,6 +1250,110 @@ class SMCSampler
     }
 };
 
+/// PMCMCStateMatrix's parameter and log normalising constant
+/// (algorithm/pmcmc.hpp:46-66), kept beside the device particles: the
+/// filter's evaluation objects read param() and call add_log_nc()
+template <typename Param>
+class PMCMCState
+{
+  public:
+    using param_type = Param;
+    double log_nc() const { return log_nc_; }
+    void add_log_nc(double nc) { log_nc_ += nc; }
+    const param_type &param() const { return param_; }
+    void reset(const param_type &p, double nc = 0)
+    {
+        param_ = p;
+        log_nc_ = nc;
+    }
+
+  private:
+    param_type param_{};
+    double log_nc_ = 0;
+};
+
+/// PMCMCMutation<Param, T> (algorithm/pmcmc.hpp:71-158) over SMCSampler<T>
+/// with user evaluation objects, step for step: iteration 0 runs the filter
+/// at the starting parameter; afterwards the proposal's filter always runs
+/// (the prior is added, not tested), the acceptance draw is
+/// log(U01Canonical<double>) from the particle's engine, and a rejected
+/// proposal restores the parameter and log normalising constant but keeps
+/// the particles.  state() is the PMCMCState the evaluation objects use.
+template <typename Param, typename T, typename RNG = Philox4x32>
+class PMCMCMutation
+{
+  public:
+    using param_type = Param;
+    using eval_type = std::function<double(RNG &, param_type &)>;
+    using prior_type = std::function<double(const param_type &)>;
+    using pf_type = SMCSampler<T, RNG>;
+
+    PMCMCMutation(std::size_t N, std::size_t M, prior_type prior,
+        std::size_t dim = 1, RNG rng = RNG())
+        : M_(M), prior_(std::move(prior)), pf_(N, dim, rng), u_(1)
+    {
+    }
+
+    pf_type &pf() { return pf_; }
+    PMCMCState<Param> &state() { return state_; }
+    void prior(prior_type p) { prior_ = std::move(p); }
+    std::size_t update(eval_type e)
+    {
+        eval_.push_back(std::move(e));
+        return eval_.size() - 1;
+    }
+    void reset()
+    {
+        eval_.clear();
+        pf_.reset();
+    }
+
+    std::size_t operator()(std::size_t iter, param_type &param)
+    {
+        if (iter == 0) {
+            run(param);
+            return 0;
+        }
+        const double lnc = state_.log_nc();
+        double prob = -prior_(param) - lnc;
+        param_type p(param);
+        for (auto &e : eval_)
+            prob += e(pf_.particle().rng(), p);
+        prob += prior_(p);
+        run(p);
+        prob += state_.log_nc();
+        hip_check(mckl_hip_rand_u01_f64(&pf_.particle().rng().state(),
+            MCKL_HIP_U01_CANONICAL, 1, u_.data(), current_stream()));
+        double u = 0;
+        if (hipStreamSynchronize(static_cast<hipStream_t>(current_stream())) !=
+                hipSuccess ||
+            hipMemcpy(&u, u_.data(), sizeof(u), hipMemcpyDeviceToHost) !=
+                hipSuccess)
+            throw RuntimeAssert("mckl_hip: reading the acceptance draw failed");
+        if (std::log(u) < prob) {
+            param = std::move(p);
+            return 1;
+        }
+        state_.reset(param, lnc);
+        return 0;
+    }
+
+  private:
+    std::size_t M_;
+    prior_type prior_;
+    std::vector<eval_type> eval_;
+    pf_type pf_;
+    PMCMCState<Param> state_;
+    DeviceArray<double> u_;
+
+    void run(const param_type &p)
+    {
+        pf_.clear();
+        state_.reset(p, 0);
+        pf_.iterate(M_);
+    }
+};
+
 } // namespace mckl
 
 #endif

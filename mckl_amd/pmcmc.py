@@ -12,6 +12,9 @@ random/u01_distribution.hpp:42,225-233).  The filter is a one-filter
 SMCBatch (a single persistent launch per run) whose log-likelihood is the
 reference's log_nc.
 
+``GenericPMCMCMutation`` is PMCMCMutation over a GenericSMCSampler with user
+evaluation objects (any model), mirroring the reference's loop exactly.
+
 ``PMCMCBatch`` advances C independent chains of the same kind together:
 one batched SMC launch evaluates all C proposals, the acceptance uniforms
 of all chains come from one device fill.  It has no counterpart in the
@@ -204,3 +207,69 @@ class PMCMCBatch:
         params[acc] = P[acc]
         self.log_nc = np.where(acc, new, lnc)
         return acc.astype(np.int32)
+
+
+class GenericPMCMCMutation:
+    """PMCMCMutation<Param, T> (algorithm/pmcmc.hpp:71-158) over a
+    GenericSMCSampler with user evaluation objects, step for step.
+
+    ``pf()`` is the particle filter: its particle is a PMCMCParticle, whose
+    ``param`` the evaluation objects read and whose log normalising
+    constant they accumulate with ``add_log_nc``.  M: filter iterations per
+    run.  ``update(eval)`` adds eval(rng, p) -> log proposal ratio, moving
+    the parameter array p in place with the particle's engine.  At
+    iteration 0 the filter runs at the starting parameter; afterwards the
+    proposal's filter always runs (the prior is added, not tested, as in
+    the reference), the acceptance draw is log(U01Canonical<double>) from
+    the particle's engine, and a rejected proposal restores the parameter
+    and log normalising constant but keeps the particles, as the reference
+    does.  ``mutation(iter, param)`` moves param in place and returns 1 if
+    the proposal was accepted."""
+
+    def __init__(self, N, M, prior, dim=1, dtype=None, rng=None, seed=1):
+        from .sampler import GenericSMCSampler, PMCMCParticle
+        self.M = int(M)
+        self._prior = prior
+        self._eval = []
+        part = PMCMCParticle(N, dim, dtype, rng, seed)
+        self._pf = GenericSMCSampler(N, particle=part)
+
+    def pf(self):
+        return self._pf
+
+    def prior(self, prior):
+        self._prior = prior
+
+    def update(self, ev):
+        self._eval.append(ev)
+        return len(self._eval) - 1
+
+    def reset(self):
+        self._eval = []
+        self._pf.reset()
+
+    def _run(self, p):
+        part = self._pf.particle()
+        self._pf.clear()
+        part.reset(p, 0.0)
+        self._pf.iterate(self.M)
+
+    def __call__(self, iter, param):
+        part = self._pf.particle()
+        if iter == 0:
+            self._run(param)
+            return 0
+        lnc = part.log_nc()
+        prob = -self._prior(param) - lnc
+        p = np.array(param, np.float64)
+        for ev in self._eval:
+            prob += ev(part.rng(), p)
+        prob += self._prior(p)
+        self._run(p)
+        prob += part.log_nc()
+        u = math.log(float(_u01_canonical(part.rng(), 1)[0]))
+        if u < prob:
+            param[...] = p
+            return 1
+        part.reset(param, lnc)
+        return 0

@@ -18,6 +18,8 @@ work (torch operations or their own kernels) on the current stream.
                     weighted sums are the device's exactly rounded sums
                     (the reference calls an external BLAS dgemv)
 ``GenericSMCSampler``  SMCSampler<T> (algorithm/smc.hpp:151-421)
+``PMCMCParticle``   a Particle carrying PMCMCStateMatrix's parameter and log
+                    normalising constant (algorithm/pmcmc.hpp:46-66)
 
 The ESS test reads the ESS to the host once per iteration, as the
 reference does; the built-in-model samplers (SMCSampler, SMCBatch) take
@@ -95,6 +97,34 @@ class Particle:
         else:
             self._spare, self.state = self.state, dst
             self._weight.set_equal()
+
+
+class PMCMCParticle(Particle):
+    """A Particle whose state also carries PMCMCStateMatrix's parameter
+    and log normalising constant (algorithm/pmcmc.hpp:46-66): ``param``,
+    ``log_nc()``, ``add_log_nc(nc)`` (a float, or a 0-d device tensor that
+    is summed on the device and read by log_nc()), ``reset(p, nc)``."""
+
+    def __init__(self, N, dim=1, dtype=None, rng=None, seed=1):
+        super().__init__(N, dim, dtype, rng, seed)
+        self.param = None
+        self._nc = 0.0
+        self._dev = None
+
+    def log_nc(self):
+        return self._nc + (float(self._dev.item()) if self._dev is not None
+                           else 0.0)
+
+    def add_log_nc(self, nc):
+        if hasattr(nc, "is_cuda"):
+            self._dev = nc if self._dev is None else self._dev + nc
+        else:
+            self._nc += float(nc)
+
+    def reset(self, p, nc=0.0):
+        self.param = np.array(p, np.float64)
+        self._nc = float(nc)
+        self._dev = None
 
 
 class ResampleEval:
@@ -219,8 +249,9 @@ class GenericSMCSampler:
     ESS < N * threshold, resample estimators, mutation evals, mutation
     estimators.  The threshold starts at resample_threshold_never()."""
 
-    def __init__(self, N, dim=1, dtype=None, rng=None, seed=1):
-        self._particle = Particle(N, dim, dtype, rng, seed)
+    def __init__(self, N, dim=1, dtype=None, rng=None, seed=1, particle=None):
+        self._particle = (particle if particle is not None
+                          else Particle(N, dim, dtype, rng, seed))
         self._evals = [[], [], []]
         self._ests = [[], [], []]
         self._iter = 0

@@ -558,6 +558,26 @@ def resample_eval(scheme):
     return ev
 
 
+class PMCMCParticle(Particle):
+    """A Particle whose state also carries PMCMCStateMatrix's parameter and
+    log normalising constant (algorithm/pmcmc.hpp:46-66)."""
+
+    def __init__(self, N, dim=1, key=(1, 0)):
+        super().__init__(N, dim, key)
+        self.param = None
+        self._log_nc = 0.0
+
+    def log_nc(self):
+        return self._log_nc
+
+    def add_log_nc(self, nc):
+        self._log_nc += nc
+
+    def reset(self, p, nc=0.0):
+        self.param = np.array(p, np.float64)
+        self._log_nc = nc
+
+
 class GenericSMC:
     """SMCSampler<T>::iterate/do_iterate (algorithm/smc.hpp:352-406):
     selection evals, selection estimators, record the size and the ESS,
@@ -565,8 +585,9 @@ class GenericSMC:
     evals, mutation estimators.  Estimators are callables est(iter, p)
     whose results are appended to self.estimates[slot]."""
 
-    def __init__(self, N, dim=1, key=(1, 0)):
-        self.particle = Particle(N, dim, key)
+    def __init__(self, N, dim=1, key=(1, 0), particle=None):
+        self.particle = particle if particle is not None else Particle(
+            N, dim, key)
         self.evals = [[], [], []]
         self.estimators = [[], [], []]
         self.estimates = [[], [], []]
@@ -574,6 +595,14 @@ class GenericSMC:
         self.size_history = []
         self.ess_history = []
         self.iter = 0
+
+    def clear(self):
+        """SMCSampler::clear (smc.hpp:205-212): the history only; the
+        particle system is kept."""
+        self.iter = 0
+        self.size_history = []
+        self.ess_history = []
+        self.estimates = [[], [], []]
 
     def iterate(self, n=1):
         p = self.particle
@@ -630,3 +659,93 @@ def lg_table_model(smc, y, z, a, sx, sy, mut=1.0):
         def mu(t, p):
             p.x *= mut
         smc.evals[2].append(mu)
+
+
+def generic_pmmh_lg(N, T, iters, scheme, threshold, key, scale, theta0, y, z):
+    """PMCMCMutation<Param, T>::operator() (algorithm/pmcmc.hpp:125-158)
+    over GenericSMC on the model of ref_smc.cpp's ref_pmcmc_lg: the filter's
+    selection eval moves with the table noise at the state's parameter,
+    adds log(sum_i W_i exp(v_i)) (in order, libm exp and log) to the log
+    normalising constant, then add_log(v); the update eval is a Gaussian
+    random walk from rand(rng, Normal(0, 1), 3); the acceptance draw is
+    log(U01Canonical<double>) from the same engine.  Iteration 0 runs the
+    filter at the starting parameter; a rejected step restores the old
+    parameter and log normalising constant but keeps the particles, as the
+    reference does.  Returns (accepted, params, log_nc) per iteration."""
+    import math
+    part = PMCMCParticle(N, 1, key)
+    smc = GenericSMC(N, 1, key, particle=part)
+    y = np.asarray(y, np.float64)
+    z = np.asarray(z, np.float64)
+
+    def sel(t, p):
+        th = p.param
+        x = p.x[:, 0]
+        x = z[:N].copy() if t == 0 else th[0] * x + th[1] * z[t * N:(t + 1) * N]
+        p.x[:, 0] = x
+        d = (y[t] - x) / th[2]
+        v = -0.5 * (d * d)
+        s = 0.0
+        W = p.W
+        for i in range(N):
+            s += float(W[i]) * math.exp(float(v[i]))
+        p.add_log_nc(math.log(s))
+        p.add_log(v)
+
+    smc.evals[0].append(sel)
+    smc.evals[1].append(resample_eval(scheme))
+    smc.threshold = threshold
+    scale = np.asarray(scale, np.float64)
+
+    def prior(q):
+        return 0.0 if abs(q[0]) < 1 and q[1] > 0 and q[2] > 0 else -np.inf
+
+    param = np.array(theta0, np.float64)
+    acc = np.zeros(iters, np.int32)
+    params = np.zeros((iters, 3))
+    lnc_out = np.zeros(iters)
+    for it in range(iters):
+        if it == 0:
+            smc.clear()
+            part.reset(param, 0.0)
+            smc.iterate(T)
+            a = 0
+        else:
+            lnc = part.log_nc()
+            prob = -prior(param) - lnc
+            q = param.copy()
+            zz = rand_normal(part.rng, 3, 0.0, 1.0, np.float64)
+            q = q + scale * zz
+            prob += 0.0
+            prob += prior(q)
+            smc.clear()
+            part.reset(q, 0.0)
+            smc.iterate(T)
+            prob += part.log_nc()
+            u = math.log(float(rand_u01(part.rng, CANONICAL, 1, np.float64)[0]))
+            if u < prob:
+                param = q
+                a = 1
+            else:
+                part.reset(param, lnc)
+                a = 0
+        acc[it] = a
+        params[it] = param
+        lnc_out[it] = part.log_nc()
+    return acc, params, lnc_out
+
+
+def ref_pmcmc_lg(N, T, iters, scheme, threshold, key, scale, theta0, y, z):
+    """The reference's own PMCMCMutation on the same model (ref_smc.cpp)."""
+    y = np.ascontiguousarray(y, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    sc = np.ascontiguousarray(scale, np.float64)
+    th = np.ascontiguousarray(theta0, np.float64)
+    acc = np.zeros(iters, np.int32)
+    params = np.zeros(iters * 3)
+    lnc = np.zeros(iters)
+    k = (ctypes.c_uint64 * 2)(*[int(v) for v in key])
+    REF_SMC.ref_pmcmc_lg(_sz(N), _sz(T), _sz(iters), scheme,
+                         ctypes.c_double(threshold), k, P(sc), P(th), P(y),
+                         P(z), P(acc), P(params), P(lnc))
+    return acc, params.reshape(iters, 3), lnc

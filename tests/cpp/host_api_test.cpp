@@ -343,6 +343,54 @@ int main(int argc, char **argv)
         CHECK(gx == x.host(N), "SMCSampler<double> state == the primitives'");
         CHECK(same_est, "SMCSampler<double> estimates == the primitives'");
     }
+    // 11. PMCMCMutation<Param, T> over SMCSampler<T>: the log normalising
+    // constant is the parameter's own function here (a fixed log weight
+    // increment per step), so it can be checked exactly, and a rejected step
+    // restores the previous parameter and constant
+    {
+        using Param = std::array<double, 1>;
+        const std::size_t N = 1000, M = 4;
+        auto prior = [](const Param &p) {
+            return p[0] > -3 && p[0] < 3 ? 0.0 : -HUGE_VAL;
+        };
+        mckl::PMCMCMutation<Param, double> mut(
+            N, M, prior, 1, mckl::Philox4x32(13));
+        std::vector<double> hv(N);
+        Dev<double> dv(N);
+        // log likelihood -p^2 / 2 per step in total: each step adds
+        // log(sum W exp(v)) with v constant = -p^2 / (2 M)
+        mut.pf().selection([&](std::size_t, mckl::Particle<double> &p) {
+            const double c = -mut.state().param()[0] *
+                mut.state().param()[0] / (2.0 * M);
+            std::fill(hv.begin(), hv.end(), c);
+            (void)hipMemcpy(dv.p, hv.data(), N * sizeof(double),
+                hipMemcpyHostToDevice);
+            p.weight().add_log(dv.p);
+            mut.state().add_log_nc(c);
+        });
+        mut.pf().resample(mckl::Stratified);
+        mut.pf().resample_threshold(0.5);
+        mut.update([](mckl::Philox4x32 &rng, Param &p) {
+            mckl::NormalDistribution<double> normal(0, 0.7);
+            Dev<double> z(1);
+            mckl::rand(rng, normal, 1, z.p);
+            p[0] += z.host(1)[0];
+            return 0.0;
+        });
+        Param param{{1.5}};
+        std::size_t acc = 0;
+        bool ok = true;
+        for (std::size_t it = 0; it != 200; ++it) {
+            const Param before = param;
+            const std::size_t a = mut(it, param);
+            acc += a;
+            const double want = -param[0] * param[0] / 2.0;
+            ok = ok && std::fabs(mut.state().log_nc() - want) < 1e-12 &&
+                (a == 1 || param == before);
+        }
+        CHECK(ok && acc > 20 && acc < 190,
+            "PMCMCMutation<Param, double> over SMCSampler<double>");
+    }
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;
 }

@@ -242,3 +242,54 @@ def test_particle_select_and_errors(dev):
         s.selection(3.5)
     assert s.resample_threshold() == -math.inf
     assert m.GenericSMCSampler.resample_threshold_always() == math.inf
+
+
+def test_generic_pmmh_matches_oracle(dev):
+    """GenericPMCMCMutation (PMCMCMutation over the generic sampler, the
+    table-noise model as torch evaluation objects) against the oracle's
+    restatement, which is pinned bit for bit to the reference's own
+    PMCMCMutation: the same acceptance decisions, parameters to the Normal
+    fill's rounding and log normalising constants to 1e-10."""
+    import torch
+    from mckl_amd import pmcmc
+    m = _m()
+    N, T, iters = 300, 12, 25
+    scheme, thr = orc.SYSTEMATIC, 0.5
+    y, z = _case(N, T, 40 + scheme)
+    key = (777, scheme)
+
+    def prior(q):
+        return 0.0 if abs(q[0]) < 1 and q[1] > 0 and q[2] > 0 else -math.inf
+
+    mut = m.GenericPMCMCMutation(N, T, prior, rng=m.Philox4x32(key=key))
+    zd = torch.from_numpy(z).cuda()
+
+    def sel(t, p):
+        th = p.param
+        x = p.state[:, 0]
+        xn = (zd[:N].clone() if t == 0
+              else x * float(th[0]) + zd[t * N:(t + 1) * N] * float(th[1]))
+        p.state[:, 0] = xn
+        d = (float(y[t]) - xn) / float(th[2])
+        v = (d * d) * -0.5
+        p.add_log_nc(torch.log((p.weight().data * torch.exp(v)).sum()))
+        p.weight().add_log(v)
+
+    mut.pf().selection(sel)
+    mut.pf().resample(scheme)
+    mut.pf().resample_threshold(thr)
+    mut.update(pmcmc.random_walk((0.08, 0.08, 0.08)))
+    param = np.array([0.7, 0.9, 0.6])
+    acc, pars, lncs = [], [], []
+    for it in range(iters):
+        acc.append(mut(it, param))
+        pars.append(param.copy())
+        lncs.append(mut.pf().particle().log_nc())
+    oacc, opar, olnc = orc.generic_pmmh_lg(
+        N, T, iters, scheme, thr, key, (0.08, 0.08, 0.08), (0.7, 0.9, 0.6),
+        y, z)
+    print("accepted", sum(acc), "log_nc", lncs[-1], olnc[-1])
+    assert acc == oacc.tolist()
+    assert np.allclose(np.array(pars), opar, rtol=1e-13, atol=0)
+    assert np.allclose(lncs, olnc, rtol=1e-10, atol=1e-10)
+    assert 0 < sum(acc) < iters - 1

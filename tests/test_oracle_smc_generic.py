@@ -78,3 +78,25 @@ def test_generic_oracle_never_resamples():
         w = np.exp(lw - lw.max())
         w /= w.sum()
         assert abs(ess[t] - 1.0 / (w * w).sum()) < 1e-9 * N
+
+
+@needs_ref
+@pytest.mark.parametrize("scheme,thr", [(orc.SYSTEMATIC, 0.5),
+                                        (orc.STRATIFIED, 0.8),
+                                        (orc.RESIDUAL, np.inf)])
+def test_generic_pmmh_oracle_matches_reference(scheme, thr):
+    """The oracle's PMCMCMutation restatement over GenericSMC against the
+    reference's own PMCMCMutation<Param, PMCMCStateMatrix<...>> on the
+    table-noise model: acceptance flags, parameters and log normalising
+    constants bit for bit over 25 iterations (including the particles a
+    rejected step leaves behind)."""
+    N, T, iters = 300, 12, 25
+    y, z = _case(N, T, 40 + scheme)
+    args = (N, T, iters, scheme, thr, (777, scheme), (0.08, 0.08, 0.08),
+            (0.7, 0.9, 0.6), y, z)
+    acc, par, lnc = orc.generic_pmmh_lg(*args)
+    racc, rpar, rlnc = orc.ref_pmcmc_lg(*args)
+    assert np.array_equal(acc, racc)
+    assert np.array_equal(par, rpar)
+    assert np.array_equal(lnc, rlnc)
+    assert 0 < acc.sum() < iters - 1
