@@ -167,3 +167,49 @@ def test_select(dev):
     i = idx.cpu().numpy()
     want = np.stack([3 * i, 3 * i + 1, 3 * i + 2], 1).reshape(-1)
     assert np.array_equal(dst.cpu().numpy(), want.astype(np.float32))
+
+
+@pytest.mark.parametrize("scheme", range(6))
+@pytest.mark.parametrize("N,M", [((1 << 20) + 12345, (1 << 20) + 12345),
+                                 (600001, 900007), (900007, 600001)])
+def test_resample_large_vs_oracle(dev, scheme, N, M):
+    """Sizes past 256 tiles of 2048, where every workgroup's cross-tile
+    loops (tile offsets of the weights, the rep -> index offsets, the zero
+    total) take several tiles per thread: rep and idx against the oracle
+    (the residual family exactly; the others up to replication changes at
+    a uniform within rounding of a cumulative weight), and idx a stable
+    realisation of rep."""
+    import torch
+    m = _m()
+    rng = np.random.default_rng(N + 7 * scheme)
+    w = rng.random(N) ** 4
+    w /= w.sum()
+    key = [9000 + scheme, 0]
+    rep = torch.zeros(N, dtype=torch.int32, device=dev)
+    idx = torch.zeros(M, dtype=torch.int32, device=dev)
+    alg = {0: m.ResampleMultinomial, 1: m.ResampleStratified,
+           2: m.ResampleSystematic, 3: m.ResampleResidual,
+           4: m.ResampleResidualStratified,
+           5: m.ResampleResidualSystematic}[scheme]()
+    alg(N, M, m.Philox4x32(key=key), _t(w, dev), rep, idx)
+    rg = rep.cpu().numpy().view(np.uint32).astype(np.uint64)
+    ig = idx.cpu().numpy().view(np.uint32).astype(np.uint64)
+    e = orc.engine(orc.PHILOX4X32, key)
+    orep, oidx = orc.resample(scheme, e, w, M)
+    assert rg.sum() == M
+    # idx realises rep: slot d < min(N, M) keeps d when rep_d > 0
+    assert np.array_equal(np.bincount(ig.astype(np.int64), minlength=N), rg)
+    K = min(N, M)
+    keep = rg[:K] > 0
+    assert np.array_equal(ig[:K][keep], np.arange(K)[keep])
+    if scheme >= 3:
+        assert np.array_equal(rg, orep)
+        assert np.array_equal(ig, oidx)
+        return
+    e2 = orc.engine(orc.PHILOX4X32, key)
+    u = orc.u01_sequence(scheme, e2, M)
+    near, _ = boundary_explained(w, u, orep, rg)
+    print("scheme %d N=%d M=%d boundary differences %d" % (scheme, N, M, near))
+    if near == 0:
+        assert np.array_equal(rg, orep)
+        assert np.array_equal(ig, oidx)
