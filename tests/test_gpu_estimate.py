@@ -53,7 +53,7 @@ def test_weighted_sums_exact(dev, layout, udt):
     m = _m()
     rng = np.random.default_rng(4)
     for n, dim in ((1, 1), (7, 3), (2048, 8), (2049, 9), (100003, 2),
-                   (5000, 17), (0, 2)):
+                   (5000, 17), (0, 2), ((1 << 20) + 5, 3)):
         u = rng.normal(size=(n, dim)) * rng.choice([1e-3, 1.0, 1e4], (n, dim))
         u = u.astype(udt)
         w = rng.random(n)

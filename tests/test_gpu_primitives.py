@@ -213,3 +213,24 @@ def test_resample_large_vs_oracle(dev, scheme, N, M):
     if near == 0:
         assert np.array_equal(rg, orep)
         assert np.array_equal(ig, oidx)
+
+
+@pytest.mark.parametrize("N", [(1 << 20) + 3, 3 * (1 << 19) + 1])
+def test_weight_large_vs_oracle(dev, N):
+    """Weight::set_log / add_log past 256 tiles (the block maxima and the
+    tile sums taken in every workgroup over several tiles per thread)
+    against the oracle, at the fixture test's tolerances."""
+    m = _m()
+    rng = np.random.default_rng(N)
+    v = rng.normal(size=N) * 3.0
+    v2 = rng.normal(size=N) * 2.0
+    W = m.Weight(N)
+    W.set_log(_t(v, dev))
+    w1, ess1 = orc.weight_set_log(v)
+    assert np.allclose(W.data.cpu().numpy(), w1, rtol=1e-13, atol=0)
+    assert abs(W.ess() - ess1) <= 1e-12 * N
+    W.add_log(_t(v2, dev))
+    w2, ess2, lnc = orc.weight_add_log(w1, v2)
+    assert np.allclose(W.data.cpu().numpy(), w2, rtol=1e-12, atol=0)
+    assert abs(W.ess() - ess2) <= 1e-11 * N
+    assert abs(W.log_nc() - lnc) < 1e-12 * max(1.0, abs(lnc))
