@@ -126,20 +126,54 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     const int chunk = (nt + kGroup - 1) / kGroup;
     const int b = gt * chunk;
     const int e = b + chunk < nt ? b + chunk : nt;
+    // up to 1024 tiles (chunks of at most kPre per thread) the thread's
+    // tile values are loaded before the group max, in one round trip with
+    // the tile maxima; the arithmetic below is the same either way
+    constexpr int kPre = 4;
+    const bool pre = chunk <= kPre;
+    float pt[kPre];
+    double ps[kPre], ps2[kPre], ph[kPre], pl[kPre];
     float mx = -__builtin_huge_valf();
-    for (int i = gt; i < nt; i += kGroup)
-        mx = fmaxf(mx, tmax[i]);
+    if (pre) {
+#pragma unroll
+        for (int q = 0; q < kPre; ++q) {
+            const int i = b + q;
+            if (i < e) {
+                pt[q] = tmax[i];
+                ps[q] = ts[i];
+                ps2[q] = ts2[i];
+                ph[q] = B.tdh[i];
+                pl[q] = B.tdl[i];
+                mx = fmaxf(mx, pt[q]);
+            }
+        }
+    } else {
+        for (int i = gt; i < nt; i += kGroup)
+            mx = fmaxf(mx, tmax[i]);
+    }
     const float m = group_max(mx, gt, L.f);
     // sums of the scaled tile sums: r_i S_i and r_i^2 S2_i (the decision,
     // log Z and the next normaliser), r_i T_i in double-double (the offsets
     // and, rounded once, the total that normalises the cumulative weights)
     double sa = 0.0, sb = 0.0;
     dd loc{0.0, 0.0};
-    for (int i = b; i < e; ++i) {
-        const double r = tile_scale(tmax[i], m);
-        sa = __dadd_rn(sa, __dmul_rn(r, ts[i]));
-        sb = __dadd_rn(sb, __dmul_rn(__dmul_rn(r, r), ts2[i]));
-        loc = dd_add(loc, dd_mul_d(dd{B.tdh[i], B.tdl[i]}, r));
+    if (pre) {
+#pragma unroll
+        for (int q = 0; q < kPre; ++q) {
+            if (b + q < e) {
+                const double r = tile_scale(pt[q], m);
+                sa = __dadd_rn(sa, __dmul_rn(r, ps[q]));
+                sb = __dadd_rn(sb, __dmul_rn(__dmul_rn(r, r), ps2[q]));
+                loc = dd_add(loc, dd_mul_d(dd{ph[q], pl[q]}, r));
+            }
+        }
+    } else {
+        for (int i = b; i < e; ++i) {
+            const double r = tile_scale(tmax[i], m);
+            sa = __dadd_rn(sa, __dmul_rn(r, ts[i]));
+            sb = __dadd_rn(sb, __dmul_rn(__dmul_rn(r, r), ts2[i]));
+            loc = dd_add(loc, dd_mul_d(dd{B.tdh[i], B.tdl[i]}, r));
+        }
     }
     group_sum2(sa, sb, gt, L.d);
     const double rs = sa, rs2 = sb;
