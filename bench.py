@@ -503,15 +503,26 @@ def cpu_reference_points(threads):
     return res
 
 
-def cpu_baseline(steps=10):
+def cpu_baseline(seconds=10.0):
     """The reference's own bulk fills (compiled from its headers), one
-    thread; else the oracle port.  Sample: the full config-1 step."""
+    thread; else the oracle port.  Sample: the full config-1 step, repeated
+    for about `seconds` of CPU time (at least 10 runs), median per step."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     n = 1 << 24
     u32 = np.empty(n, np.uint32)
     f64 = np.empty(n, np.float64)
+    if orc.REF is not None:
+        key = (ctypes.c_uint64 * 4)(*(KEY1 + [0, 0]))
+        t1 = orc.REF.ref_time_philox_u01(key, ctypes.c_size_t(n), 1,
+                                         orc.P(u32), orc.P(f64))
+    else:
+        t0 = time.perf_counter()
+        e = orc.engine(orc.PHILOX4X32, KEY1)
+        orc.O.mo_rand_u32(ctypes.byref(e), ctypes.c_size_t(n), orc.P(u32))
+        t1 = time.perf_counter() - t0
+    steps = int(min(1000, max(10, seconds / max(t1, 1e-6))))
     if orc.REF is not None:
         key = (ctypes.c_uint64 * 4)(*(KEY1 + [0, 0]))
         t = orc.REF.ref_time_philox_u01(key, ctypes.c_size_t(n), steps,
