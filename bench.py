@@ -264,6 +264,26 @@ def bench_resample(m, torch):
     return {"resample_N2^20": out}
 
 
+def bench_smc_batched_run(m, torch):
+    """The batched configuration alone: one warm-up and one timed graph run
+    of the 4096 SV filters (no per-filter comparison), for counter runs."""
+    from mckl_amd import workloads as wl
+    import numpy as np
+    theta, Nf, yb, seeds = wl.sv_batch(4096, 50)
+    bt = m.SMCBatch(m.MODEL_SV, Nf, theta, yb, seeds, 0.5, m.STRATIFIED)
+    bt.run(use_graph=True)
+    torch.cuda.synchronize()
+    a, b = _events()
+    a.record()
+    bt.run(use_graph=True)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b)
+    ps = float(Nf.astype(np.float64).sum()) * 50
+    return {"smc_batched_sv_4096_T50_run": {
+        "ms": round(ms, 3), "particle_steps_per_s": ps / (ms * 1e-3)}}
+
+
 def bench_smc_batched(m, torch):
     from mckl_amd import workloads as wl
     import numpy as np
@@ -554,7 +574,8 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--only", choices=["fills", "saturation", "normals", "smc",
                                        "smc_plain", "smc_generic",
-                                       "smc_batched", "grad",
+                                       "smc_batched", "smc_batched_run",
+                                       "grad",
                                        "resample_index", "resample"],
                     help="run one secondary leg alone and print its JSON "
                          "(for profiling)")
@@ -583,6 +604,7 @@ def main():
                "smc": bench_smc, "smc_plain": bench_smc_plain,
                "smc_generic": bench_smc_generic,
                "smc_batched": bench_smc_batched,
+               "smc_batched_run": bench_smc_batched_run,
                "grad": bench_grad,
                "resample_index": bench_resample_index,
                "resample": bench_resample}[args.only]
