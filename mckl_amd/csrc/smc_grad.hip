@@ -10,6 +10,7 @@ struct GLds {
     double ts[kMaxTilesG], ts2[kMaxTilesG];
     double tdh[kMaxTilesG], tdl[kMaxTilesG];
     double toh[kMaxTilesG + 1], tol[kMaxTilesG + 1];
+    double tmh[kMaxTilesG], tml[kMaxTilesG]; // multinomial: log spacings
     uint32_t te[kMaxTilesG], tz[kMaxTilesG];
     uint32_t teo[kMaxTilesG + 1], tzo[kMaxTilesG + 1];
     double red[kMaxTilesG];
@@ -21,6 +22,7 @@ struct GLds {
 struct GradSlotBufs {
     double *ck, *seg, *xb, *lwb, *xpb, *cb;
     uint32_t *cnt;
+    double *u; // multinomial: the step's sorted uniforms (else null)
 };
 
 __global__ void k_grad_reset(uint32_t *queue)
@@ -51,11 +53,84 @@ __device__ void copy_slot(const SlotPtr &d, const SlotPtr &s, uint32_t Np)
     __syncthreads();
 }
 
+// Multinomial: the step's sorted uniforms into u (the samplers'
+// log-spacing tiles, tile_mterms / tile_muniforms).  Only the multinomial
+// instantiation of the kernel contains it: compiled into the others it
+// raises their register use past what fits without spills.  Every thread
+// of the workgroup calls it.
+template <int G>
+__device__ __forceinline__ void grad_muniforms(const GradArgs &A,
+    uint32_t t, double *u, GLds &S, const GroupLds &L, int grp, int gt)
+{
+    const int nt = (int)A.ntiles;
+    const int rounds = (nt + G - 1) / G;
+    SmcArgs ma{};
+    ma.N = A.N;
+    ma.k0 = A.k0;
+    ma.k1 = A.k1;
+    for (int r = 0; r < rounds; ++r) {
+        const int k = r * G + grp;
+        dd mt;
+        tile_mterms(ma, (uint32_t)(k < nt ? k : nt), t, gt, L.d, mt);
+        if (gt == 0 && k < nt) {
+            S.tmh[k] = mt.hi;
+            S.tml[k] = mt.lo;
+        }
+        __syncthreads();
+    }
+    tiles_scan_dd(grp == 0, S.tmh, S.tml, nt, S.toh, S.tol, gt, L.d);
+    for (int r = 0; r < rounds; ++r) {
+        const int k = r * G + grp;
+        const int kk = k < nt ? k : nt;
+        tile_muniforms(
+            ma, (uint32_t)kk, t, dd{S.toh[kk], S.tol[kk]}, u, gt, L.d);
+    }
+    __syncthreads();
+}
+
+// Multinomial: the step's sorted uniforms, the weights' tile offsets, then
+// the counts of each tile against the uniforms (binary search), leaving
+// the tiles' extra-copy and zero-slot totals in S.te / S.tz (multinomial
+// instantiation only, as grad_muniforms).  Every thread of the workgroup
+// calls it.
+template <int G>
+__device__ __forceinline__ void grad_mcounts(const GradArgs &A,
+    uint32_t t, const double *lw, double m, double invS, double *u,
+    uint32_t *cnt, GLds &S, const GroupLds &L, int grp, int gt)
+{
+    const int nt = (int)A.ntiles;
+    const int rounds = (nt + G - 1) / G;
+    grad_muniforms<G>(A, t, u, S, L, grp, gt); // uses S.toh/S.tol
+    tiles_scan_dd(grp == 0, S.tdh, S.tdl, nt, S.toh, S.tol, gt, L.d);
+    UniformSeq U;
+    U.u = u;
+    U.scheme = MCKL_HIP_MULTINOMIAL;
+    U.k0 = A.k0;
+    U.k1 = A.k1;
+    U.t = t;
+    U.M = A.N;
+    U.delta = 1.0 / (double)A.N;
+    U.u0d = 0.0;
+    for (int r = 0; r < rounds; ++r) {
+        const int k = r * G + grp;
+        uint32_t es, zs;
+        gtile_counts(A, (uint32_t)(k < nt ? k : nt), lw, m, invS, S.toh,
+            S.tol, U, cnt, gt, L.d, L.u, es, zs);
+        if (gt == 0 && k < nt) {
+            S.te[k] = es;
+            S.tz[k] = zs;
+        }
+        __syncthreads();
+    }
+}
+
 // One forward step t from slot `prev` to slot `cur`.  Returns the step's
 // contribution to F.  `rt` is the tape row pointer set for record/replay.
-template <int G>
+// kMult: multinomial resampling (u: the step's sorted uniforms).
+template <int G, bool kMult>
 __device__ double fwd_step(const GradArgs &A, uint32_t t, const SlotPtr &prev,
-    const SlotPtr &cur, uint32_t *cnt, int mode, const double *lwn_row,
+    const SlotPtr &cur, uint32_t *cnt, double *u, int mode,
+    const double *lwn_row,
     const uint32_t *par_row, int32_t *tape_r, double *tape_lwn_out,
     uint32_t *tape_par_out, GLds &S, const GroupLds &L, int grp, int gt,
     uint32_t *Es)
@@ -138,25 +213,32 @@ __device__ double fwd_step(const GradArgs &A, uint32_t t, const SlotPtr &prev,
     __syncthreads();
     if (R && !replay) {
         const double invS = 1.0 / rs;
-        tiles_scan_dd(grp == 0, S.tdh, S.tdl, nt, S.toh, S.tol, gt, L.d);
-        UniformSeq U;
-        U.scheme = A.scheme;
-        U.k0 = A.k0;
-        U.k1 = A.k1;
-        U.t = t;
-        U.M = A.N;
-        U.delta = 1.0 / (double)A.N;
-        U.u0d = A.scheme == MCKL_HIP_SYSTEMATIC ? smc_uniform(A.k0, A.k1, t, 0) * U.delta : 0.0;
-        for (int r = 0; r < rounds; ++r) {
-            const int k = r * G + grp;
-            uint32_t es, zs;
-            gtile_counts(A, (uint32_t)(k < nt ? k : nt), cur.lw, m, invS, S.toh,
-                S.tol, U, cnt, gt, L.d, L.u, es, zs);
-            if (gt == 0 && k < nt) {
-                S.te[k] = es;
-                S.tz[k] = zs;
+        if constexpr (kMult) {
+            grad_mcounts<G>(A, t, cur.lw, m, invS, u, cnt, S, L, grp, gt);
+        } else {
+            tiles_scan_dd(grp == 0, S.tdh, S.tdl, nt, S.toh, S.tol, gt, L.d);
+            UniformSeq U;
+            U.u = nullptr;
+            U.scheme = A.scheme;
+            U.k0 = A.k0;
+            U.k1 = A.k1;
+            U.t = t;
+            U.M = A.N;
+            U.delta = 1.0 / (double)A.N;
+            U.u0d = A.scheme == MCKL_HIP_SYSTEMATIC
+                ? smc_uniform(A.k0, A.k1, t, 0) * U.delta
+                : 0.0;
+            for (int r = 0; r < rounds; ++r) {
+                const int k = r * G + grp;
+                uint32_t es, zs;
+                gtile_counts(A, (uint32_t)(k < nt ? k : nt), cur.lw, m, invS,
+                    S.toh, S.tol, U, cnt, gt, L.d, L.u, es, zs);
+                if (gt == 0 && k < nt) {
+                    S.te[k] = es;
+                    S.tz[k] = zs;
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
         tiles_scan_u2(grp == 0, S.te, S.tz, nt, S.teo, S.tzo, gt, L.u);
         for (int r = 0; r < rounds; ++r) {
@@ -184,7 +266,7 @@ __device__ double fwd_step(const GradArgs &A, uint32_t t, const SlotPtr &prev,
     return inc;
 }
 
-template <int G>
+template <int G, bool kMult>
 __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
     uint32_t npoints, uint32_t *queue, GradSlotBufs B, size_t slot_bytes,
     size_t ck_stride, size_t seg_stride, size_t vec_stride, int mode,
@@ -205,6 +287,7 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
     double *xpb = B.xpb + slot * vec_stride;
     double *cb = B.cb + slot * vec_stride;
     uint32_t *cnt = B.cnt + slot * vec_stride;
+    double *ub = B.u ? B.u + slot * vec_stride : nullptr;
     for (;;) {
         if (threadIdx.x == 0) {
             const uint32_t q = atomicAdd(queue, 1u);
@@ -246,7 +329,7 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
         for (uint32_t t = 1; t <= A.T; ++t) {
             const SlotPtr &prev = (t & 1) ? s0 : s1;
             const SlotPtr &cur = (t & 1) ? s1 : s0;
-            F += fwd_step<G>(A, t, prev, cur, cnt, mode,
+            F += fwd_step<G, kMult>(A, t, prev, cur, cnt, ub, mode,
                 tl ? tl + (size_t)(t - 1) * rowN : nullptr,
                 tp ? tp + (size_t)(t - 1) * rowN : nullptr, tr,
                 tl ? tl + (size_t)t * rowN : nullptr,
@@ -275,8 +358,10 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
             copy_slot<G>(slot_at(seg, slot_bytes, 0, Np),
                 slot_at(ck, slot_bytes, (uint32_t)c, Np), Np);
             for (uint32_t t = t0 + 1; t <= t1; ++t)
-                (void)fwd_step<G>(A, t, slot_at(seg, slot_bytes, t - t0 - 1, Np),
-                    slot_at(seg, slot_bytes, t - t0, Np), cnt, kGradRun, nullptr,
+                (void)fwd_step<G, kMult>(A, t,
+                    slot_at(seg, slot_bytes, t - t0 - 1, Np),
+                    slot_at(seg, slot_bytes, t - t0, Np), cnt, ub, kGradRun,
+                    nullptr,
                     nullptr, nullptr, nullptr, nullptr, S, L, grp, gt,
                     Es + grp * kTile);
             for (uint32_t t = t1; t > t0; --t) {
@@ -426,7 +511,8 @@ struct mckl_hip_smc_grad {
 static void grad_free(mckl_hip_smc_grad *g)
 {
     void *ptrs[] = {g->y, g->args, g->queue, g->B.ck, g->B.seg, g->B.xb,
-        g->B.lwb, g->B.xpb, g->B.cb, g->B.cnt, g->logz, g->grad, g->tape_lwn,
+        g->B.lwb, g->B.xpb, g->B.cb, g->B.cnt, g->B.u, g->logz, g->grad,
+        g->tape_lwn,
         g->tape_par, g->tape_r};
     for (void *p : ptrs)
         if (p)
@@ -485,7 +571,9 @@ static int grad_launch(mckl_hip_smc_grad *g, const double *h_theta, int mode,
         args.size() * sizeof(GradArgs), hipMemcpyHostToDevice, st));
     k_grad_reset<<<1, 64, 0, st>>>(g->queue);
     const bool tape = mode != kGradRun;
-    k_grad<kGradGroups><<<g->nslots, kGradGroups * kGroup, 0, st>>>(g->args,
+    auto kern = g->scheme == MCKL_HIP_MULTINOMIAL ? k_grad<kGradGroups, true>
+                                                   : k_grad<kGradGroups, false>;
+    kern<<<g->nslots, kGradGroups * kGroup, 0, st>>>(g->args,
         (uint32_t)g->nbatch, g->queue, g->B, g->slot_bytes, g->ck_stride,
         g->seg_stride, g->vec_stride, mode, tape ? g->tape_lwn : nullptr,
         tape ? g->tape_par : nullptr, tape ? g->tape_r : nullptr, g->logz,
@@ -513,8 +601,10 @@ int mckl_hip_smc_grad_create(int model, size_t nbatch, size_t N, size_t T,
         return set_error(MCKL_HIP_EINVAL,
             "the gradient sampler holds at most %d particles per point",
             kMaxTilesG * kTile);
-    if (scheme != MCKL_HIP_STRATIFIED && scheme != MCKL_HIP_SYSTEMATIC)
-        return set_error(MCKL_HIP_EINVAL, "stratified or systematic only");
+    if (scheme != MCKL_HIP_STRATIFIED && scheme != MCKL_HIP_SYSTEMATIC &&
+        scheme != MCKL_HIP_MULTINOMIAL)
+        return set_error(MCKL_HIP_EINVAL,
+            "multinomial, stratified or systematic only");
     if (std::isnan(threshold))
         return set_error(MCKL_HIP_EINVAL, "threshold is NaN");
     mckl_hip_smc_grad *g = new mckl_hip_smc_grad();
@@ -564,6 +654,8 @@ int mckl_hip_smc_grad_create(int model, size_t nbatch, size_t N, size_t T,
     MCKL_ALLOC(g->B.xpb, nslots * Np);
     MCKL_ALLOC(g->B.cb, nslots * Np);
     MCKL_ALLOC(g->B.cnt, nslots * Np);
+    if (scheme == MCKL_HIP_MULTINOMIAL)
+        MCKL_ALLOC(g->B.u, nslots * Np);
     MCKL_ALLOC(g->logz, nbatch);
     MCKL_ALLOC(g->grad, 3 * nbatch);
     if (tape) {

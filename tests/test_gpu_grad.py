@@ -19,7 +19,7 @@ import kalman
 
 pytestmark = pytest.mark.gpu
 
-STRAT, SYS = 1, 2  # ResampleScheme numbering
+MULT, STRAT, SYS = 0, 1, 2  # ResampleScheme numbering
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 LG = (0.9, 1.0, 0.5)
@@ -50,7 +50,7 @@ def _fd(g, th, h=1e-4):
 
 @pytest.mark.parametrize("model,N,K,scheme", [
     (1, 3000, 7, SYS), (2, 4096, 10, STRAT), (2, 2049, 60, SYS),
-    (1, 5, 3, SYS)])
+    (1, 5, 3, SYS), (1, 3000, 9, MULT), (2, 4100, 13, MULT)])
 def test_replay_and_fd(dev, model, N, K, scheme):
     m = _m()
     y = _obs()["lg_y"][:60] if model == 1 else _obs()["sv_y"][:60]
@@ -79,13 +79,14 @@ def test_replay_and_fd(dev, model, N, K, scheme):
 def test_checkpoint_interval_invariance(dev):
     m = _m()
     y = _obs()["sv_y"][:40]
-    res = []
-    for K in (1, 3, 13, 40, 100):
-        g = m.SMCGradient(2, 2, 5000, y, [8, 9], 0.5, SYS, K=K)
-        res.append(g.run([SV, (0.9, 0.25, 0.6)]))
-    for ll, gr in res[1:]:
-        assert np.array_equal(ll, res[0][0])
-        assert np.array_equal(gr, res[0][1])
+    for scheme in (SYS, MULT):
+        res = []
+        for K in (1, 3, 13, 40, 100):
+            g = m.SMCGradient(2, 2, 5000, y, [8, 9], 0.5, scheme, K=K)
+            res.append(g.run([SV, (0.9, 0.25, 0.6)]))
+        for ll, gr in res[1:]:
+            assert np.array_equal(ll, res[0][0])
+            assert np.array_equal(gr, res[0][1])
 
 
 def test_per_point_observations_and_one_shot(dev):
@@ -153,7 +154,7 @@ def test_grad_errors(dev):
     with pytest.raises(m.MCKLError):
         m.SMCGradient(1, 1, 1 << 18, np.zeros(3), [1])
     with pytest.raises(m.MCKLError):
-        m.SMCGradient(1, 1, 100, np.zeros(3), [1], scheme=m.MULTINOMIAL)
+        m.SMCGradient(1, 1, 100, np.zeros(3), [1], scheme=m.RESIDUAL)
     g = m.SMCGradient(2, 1, 100, np.zeros(3), [1])
     with pytest.raises(m.MCKLError):
         g.replay(SV)
