@@ -15,6 +15,7 @@
 #include <mckl_hip.h>
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstddef>
@@ -656,6 +657,126 @@ class SMCSamplerHIP
             hip_check(mckl_hip_smc_create(model_, theta_.data(), y_.size(),
                 y_.data(), N_, seed_, threshold_, scheme_, &s_));
     }
+};
+
+/// Batched SMC for the built-in models (no reference counterpart): nfilters
+/// independent filters of uneven sizes in one persistent launch.  theta: 3
+/// values per filter, y: T per filter, one seed per filter; filter f gives
+/// the log-likelihood SMCSamplerHIP gives with the same inputs, bit for bit.
+class SMCBatchHIP
+{
+  public:
+    SMCBatchHIP(int model, const std::vector<std::uint32_t> &N,
+        const std::vector<double> &theta, std::size_t T,
+        const std::vector<double> &y, const std::vector<std::uint64_t> &seed,
+        double threshold = 0.5, int scheme = MCKL_HIP_STRATIFIED)
+        : n_(N.size())
+    {
+        if (theta.size() != 3 * n_ || y.size() != T * n_ ||
+            seed.size() != n_)
+            throw RuntimeAssert("SMCBatchHIP: inconsistent sizes");
+        hip_check(mckl_hip_smc_batch_create(model, n_, N.data(),
+            theta.data(), T, y.data(), seed.data(), threshold, scheme, &b_));
+    }
+    ~SMCBatchHIP()
+    {
+        if (b_)
+            mckl_hip_smc_batch_destroy(b_);
+    }
+    SMCBatchHIP(const SMCBatchHIP &) = delete;
+    SMCBatchHIP &operator=(const SMCBatchHIP &) = delete;
+
+    std::size_t size() const { return n_; }
+    /// workgroup size and occupancy cap of the persistent launch
+    void set_tune(int block, int waves_per_eu = 0)
+    {
+        mckl_hip_tune t;
+        t.block = block;
+        t.waves_per_eu = waves_per_eu;
+        hip_check(mckl_hip_smc_batch_set_tune(b_, &t));
+    }
+    /// new parameters and seeds for every filter (sizes, observations,
+    /// threshold and scheme stay)
+    void set_params(const std::vector<double> &theta,
+        const std::vector<std::uint64_t> &seed)
+    {
+        if (theta.size() != 3 * n_ || seed.size() != n_)
+            throw RuntimeAssert("SMCBatchHIP: inconsistent sizes");
+        hip_check(mckl_hip_smc_batch_set_params(b_, theta.data(), seed.data()));
+    }
+    /// all filters, asynchronous on the current stream
+    void run(bool use_graph = true)
+    {
+        hip_check(mckl_hip_smc_batch_run(b_, use_graph ? 1 : 0,
+            current_stream()));
+    }
+    std::vector<double> log_likelihood()
+    {
+        std::vector<double> ll(n_);
+        hip_check(mckl_hip_smc_batch_result(b_, ll.data()));
+        return ll;
+    }
+
+  private:
+    std::size_t n_;
+    mckl_hip_smc_batch *b_ = nullptr;
+};
+
+/// Gradient of the SMC log-likelihood with respect to the built-in models'
+/// theta by a hand-written adjoint (no reference counterpart): nbatch
+/// points, N particles each, ancestors and decisions of the forward pass
+/// held fixed.  y: T values shared by all points, or T per point.
+/// replay(theta) (with tape) evaluates the differentiated function.
+class SMCGradientHIP
+{
+  public:
+    SMCGradientHIP(int model, std::size_t nbatch, std::size_t N,
+        std::size_t T, const std::vector<double> &y, bool y_per_point,
+        const std::vector<std::uint64_t> &seed, double threshold = 0.5,
+        int scheme = MCKL_HIP_SYSTEMATIC, std::size_t K = 50,
+        bool tape = false)
+        : n_(nbatch)
+    {
+        if (y.size() != (y_per_point ? T * nbatch : T) ||
+            seed.size() != nbatch)
+            throw RuntimeAssert("SMCGradientHIP: inconsistent sizes");
+        hip_check(mckl_hip_smc_grad_create(model, nbatch, N, T, y.data(),
+            y_per_point ? 1 : 0, seed.data(), threshold, scheme, K,
+            tape ? 1 : 0, &g_));
+    }
+    ~SMCGradientHIP()
+    {
+        if (g_)
+            mckl_hip_smc_grad_destroy(g_);
+    }
+    SMCGradientHIP(const SMCGradientHIP &) = delete;
+    SMCGradientHIP &operator=(const SMCGradientHIP &) = delete;
+
+    /// log-likelihoods (nbatch) and gradients (3 per point); synchronises
+    void run(const std::vector<double> &theta, std::vector<double> &loglik,
+        std::vector<double> &grad)
+    {
+        if (theta.size() != 3 * n_)
+            throw RuntimeAssert("SMCGradientHIP: theta holds 3 per point");
+        loglik.resize(n_);
+        grad.resize(3 * n_);
+        hip_check(mckl_hip_smc_grad_run(g_, theta.data(), loglik.data(),
+            grad.data(), current_stream()));
+    }
+    /// the differentiated function at theta (needs tape = true)
+    std::vector<double> replay(const std::vector<double> &theta)
+    {
+        if (theta.size() != 3 * n_)
+            throw RuntimeAssert("SMCGradientHIP: theta holds 3 per point");
+        std::vector<double> v(n_);
+        hip_check(mckl_hip_smc_grad_replay(g_, theta.data(), v.data(),
+            current_stream()));
+        return v;
+    }
+
+  private:
+    std::size_t n_;
+    mckl_hip_smc_grad *g_ = nullptr;
 };
 
 /// PMCMCMutation<Param, T> (algorithm/pmcmc.hpp:71-158) for the built-in

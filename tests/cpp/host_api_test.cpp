@@ -391,6 +391,56 @@ int main(int argc, char **argv)
         CHECK(ok && acc > 20 && acc < 190,
             "PMCMCMutation<Param, double> over SMCSampler<double>");
     }
+    // 12. SMCBatchHIP equals SMCSamplerHIP filter by filter; SMCGradientHIP's
+    // gradient matches central differences of its replayed function
+    {
+        std::vector<double> y(25);
+        for (std::size_t t = 0; t != y.size(); ++t)
+            y[t] = 0.8 * std::sin(0.4 * t);
+        const std::vector<std::uint32_t> N{3000, 2049, 517};
+        std::vector<double> th{0.9, 1.0, 0.5, 0.8, 0.7, 0.6, 0.95, 1.2, 0.4};
+        std::vector<double> yy;
+        for (int f = 0; f != 3; ++f)
+            yy.insert(yy.end(), y.begin(), y.end());
+        const std::vector<std::uint64_t> seed{11, 12, 13};
+        mckl::SMCBatchHIP b(MCKL_HIP_MODEL_LG, N, th, y.size(), yy, seed, 0.5,
+            MCKL_HIP_SYSTEMATIC);
+        b.run();
+        const std::vector<double> lb = b.log_likelihood();
+        bool same = lb.size() == 3;
+        for (int f = 0; f != 3 && same; ++f) {
+            mckl::SMCSamplerHIP s(MCKL_HIP_MODEL_LG,
+                {{th[3 * f], th[3 * f + 1], th[3 * f + 2]}}, y, N[f], seed[f]);
+            s.resample_threshold(0.5);
+            s.resample(mckl::Systematic);
+            s.iterate();
+            same = s.log_likelihood() == lb[f];
+        }
+        CHECK(same, "SMCBatchHIP == SMCSamplerHIP per filter");
+
+        const std::vector<double> th2{0.9, 1.0, 0.5, 0.85, 1.1, 0.45};
+        mckl::SMCGradientHIP g(MCKL_HIP_MODEL_LG, 2, 2000, y.size(), y, false,
+            {3, 4}, 0.5, MCKL_HIP_STRATIFIED, 7, true);
+        std::vector<double> ll, gr;
+        g.run(th2, ll, gr);
+        const std::vector<double> v = g.replay(th2);
+        bool ok = v == ll;
+        const double h = 1e-5;
+        for (int j = 0; j != 3 && ok; ++j) {
+            std::vector<double> tp(th2), tm(th2);
+            for (int p = 0; p != 2; ++p) {
+                tp[3 * p + j] += h;
+                tm[3 * p + j] -= h;
+            }
+            const std::vector<double> fp = g.replay(tp), fm = g.replay(tm);
+            for (int p = 0; p != 2; ++p) {
+                const double fd = (fp[p] - fm[p]) / (2 * h);
+                ok = ok && std::fabs(fd - gr[3 * p + j]) <=
+                        1e-4 * std::max(1.0, std::fabs(gr[3 * p + j]));
+            }
+        }
+        CHECK(ok, "SMCGradientHIP replay == forward and gradient == FD");
+    }
     std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
     return failures ? 1 : 0;
 }
