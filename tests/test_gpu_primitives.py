@@ -234,3 +234,29 @@ def test_weight_large_vs_oracle(dev, N):
     assert np.allclose(W.data.cpu().numpy(), w2, rtol=1e-12, atol=0)
     assert abs(W.ess() - ess2) <= 1e-11 * N
     assert abs(W.log_nc() - lnc) < 1e-12 * max(1.0, abs(lnc))
+
+
+@pytest.mark.parametrize("scheme", range(6))
+def test_resample_tiny_sizes(dev, scheme):
+    """N = 1 (every copy of the one particle), M = 1 and N = M = 2 through
+    mckl_hip_resample against the oracle: rep sums to M and idx realises
+    it."""
+    import torch
+    m = _m()
+    alg = {0: m.ResampleMultinomial, 1: m.ResampleStratified,
+           2: m.ResampleSystematic, 3: m.ResampleResidual,
+           4: m.ResampleResidualStratified,
+           5: m.ResampleResidualSystematic}[scheme]()
+    for N, M in ((1, 5), (5, 1), (2, 2), (1, 1)):
+        w = np.arange(1, N + 1, dtype=np.float64)
+        w /= w.sum()
+        rep = torch.zeros(N, dtype=torch.int32, device=dev)
+        idx = torch.zeros(M, dtype=torch.int32, device=dev)
+        key = [31 + N, M]
+        alg(N, M, m.Philox4x32(key=key), _t(w, dev), rep, idx)
+        orep, oidx = orc.resample(scheme, orc.engine(orc.PHILOX4X32, key), w,
+                                  M)
+        rg = rep.cpu().numpy().astype(np.uint64)
+        ig = idx.cpu().numpy().astype(np.uint64)
+        assert rg.sum() == M and np.array_equal(rg, orep), (N, M, rg, orep)
+        assert np.array_equal(ig, oidx), (N, M, ig, oidx)
