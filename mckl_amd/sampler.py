@@ -75,14 +75,22 @@ class Particle:
     def rng(self):
         return self._rng
 
-    def select(self, n, idx):
+    def select(self, n, idx, validate=True):
         """Particle::select (core/particle.hpp:324-331): row j of the new
         state is row idx[j] of the old one (j < n; idx an int32 device
-        tensor of valid row numbers), then equal weights of size n."""
+        tensor), then equal weights of size n.  With validate (the default)
+        the row numbers are verified on the host first, so that a bad index
+        raises instead of reading outside the state; ResampleEval, whose
+        indices come from the library, skips it."""
         torch = _torch()
         if idx.numel() < n or idx.dtype != torch.int32 or not idx.is_cuda:
             raise MCKLError("Particle.select: idx must be an int32 device "
                             "tensor of at least n entries")
+        if validate and n > 0:
+            lo, hi = torch.aminmax(idx[:n])
+            if int(lo) < 0 or int(hi) >= self.size():
+                raise MCKLError("Particle.select: row numbers must lie in "
+                                "[0, %d)" % self.size())
         resize = n != self.size()
         dst = (torch.empty((n, self.dim()), dtype=self.state.dtype,
                            device="cuda") if resize else self._spare)
@@ -150,7 +158,7 @@ class ResampleEval:
             self._idx = torch.empty(N, dtype=torch.int32, device="cuda")
         self._alg(N, N, particle.rng(), particle.weight().data, self._rep,
                   self._idx)
-        particle.select(N, self._idx)
+        particle.select(N, self._idx, validate=False)
 
 
 class SMCEstimator:

@@ -233,6 +233,10 @@ def test_particle_select_and_errors(dev):
     assert p.weight().ess() == 4.0
     with pytest.raises(m.MCKLError):
         p.select(4, idx.to(torch.int64))
+    with pytest.raises(m.MCKLError):   # row 4 does not exist any more
+        p.select(2, torch.tensor([0, 4], dtype=torch.int32, device="cuda"))
+    with pytest.raises(m.MCKLError):
+        p.select(1, torch.tensor([-1], dtype=torch.int32, device="cuda"))
     with pytest.raises(m.MCKLError):
         m.Particle(4, 1, torch.int32)
     s = m.GenericSMCSampler(8)
