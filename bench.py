@@ -473,7 +473,19 @@ def cpu_reference_points(threads):
     from mckl_amd import workloads as wl
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    res = {"threads": threads}
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    res = {"threads": threads, "nproc": os.cpu_count(), "cpu_model": model,
+           "build": "oracle port: gcc -O2 -ffp-contract=off -fopenmp (its "
+                    "pinned flags); reference: g++ -O2 -ffp-contract=off, "
+                    "no -march"}
     if orc.REF is not None:
         n = 1 << 22
         for ename, kind, dname, dt, mean, sd in (
