@@ -214,6 +214,19 @@ int mckl_hip_resample_trans_rep_index(size_t N, size_t M,
  * ResampleEval::operator() resample.hpp:494-507 */
 int mckl_hip_resample(int scheme, size_t N, size_t M, mckl_hip_engine *e,
     const double *d_w, uint32_t *d_rep, uint32_t *d_idx, void *stream);
+/* No counterpart in the reference.  mckl_hip_resample for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops): the same results and engine advance, with the workspace taken from
+ * a device array of the library (about 36 MiB, held by each device the
+ * library runs on until the process ends) instead of the stream-ordered
+ * allocator.  N, M <= 2^20 (1048576), else MCKL_HIP_EINVAL.  Every call on
+ * one device shares that array: calls that overlap on one device (other
+ * streams, other threads) race and must be serialised by the caller.  The
+ * residual schemes read their residual count back to the host (a stream
+ * synchronisation), as mckl_hip_resample does. */
+int mckl_hip_resample_noalloc(int scheme, size_t N, size_t M,
+    mckl_hip_engine *e, const double *d_w, uint32_t *d_rep, uint32_t *d_idx,
+    void *stream);
 /* SMCEstimator::estimate's weighted sums (algorithm/smc.hpp:89-118, a dgemv
  * in the reference): d_r[j] = sum_i d_w[i] u(i, j), j < dim, with u(i, j) =
  * d_u[i*dim + j] (MCKL_HIP_ROW_MAJOR) or d_u[j*n + i] (MCKL_HIP_COL_MAJOR);

@@ -91,6 +91,8 @@ SIGNATURES = {
     "mckl_hip_resample_trans_u01_rep": (_int, [_sz, _sz, _vp, _vp, _vp, _vp]),
     "mckl_hip_resample_trans_rep_index": (_int, [_sz, _sz, _vp, _vp, _vp]),
     "mckl_hip_resample": (_int, [_int, _sz, _sz, _EP, _vp, _vp, _vp, _vp]),
+    "mckl_hip_resample_noalloc": (
+        _int, [_int, _sz, _sz, _EP, _vp, _vp, _vp, _vp]),
     "mckl_hip_select_f32": (_int, [_sz, _sz, _vp, _vp, _vp, _vp]),
     "mckl_hip_select_f64": (_int, [_sz, _sz, _vp, _vp, _vp, _vp]),
     "mckl_hip_resample_index_create": (_int, [ctypes.POINTER(_vp)]),

@@ -50,7 +50,7 @@ inline int hip_status(hipError_t err, const char *what)
 
 inline hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 
-inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+constexpr uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // Makes sure per-device constant tables are uploaded (AES T0).  Returns a
 // status code.

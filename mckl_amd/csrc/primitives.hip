@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace mckl_hip {
 
@@ -642,7 +643,7 @@ struct Scratch {
     }
 };
 
-size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 int weight_normalize(int mode, size_t n, const double *v, double *w,
     double *stats, hipStream_t st)
@@ -689,7 +690,7 @@ struct RsWork {
     double *c;
 };
 
-size_t rs_work_bytes(size_t N, size_t M, bool sorted)
+constexpr size_t rs_work_bytes(size_t N, size_t M, bool sorted)
 {
     const size_t nt = std::max(ceil_div(N, kTile), ceil_div(M, kTile)) + 1;
     return 2 * align256(nt * sizeof(double)) +
@@ -945,7 +946,7 @@ __global__ void k_add_integ(uint64_t N, uint32_t *rep, const uint32_t *integ)
 // read back to the host (the uniform sequence advances the engine by R).
 // ws: a workspace of at least residual_ws_bytes(N, M, base) bytes, whose
 // RsWork part the caller may reuse for rep -> index afterwards.
-size_t residual_ws_bytes(size_t N, size_t M, int base)
+constexpr size_t residual_ws_bytes(size_t N, size_t M, int base)
 {
     const size_t nt = ceil_div(N, kTile) + 1;
     return 3 * align256(nt * sizeof(double)) +
@@ -1096,6 +1097,81 @@ int select_impl(size_t M, size_t dim, const uint32_t *idx, const T *src,
     return MCKL_HIP_OK;
 }
 
+// Workspace bytes of one ResampleAlgorithm call: the M uniforms and the
+// helpers' arrays (plain schemes), or the residual form's arrays.
+constexpr size_t resample_ws_bytes(int scheme, size_t N, size_t M)
+{
+    return scheme >= MCKL_HIP_RESIDUAL
+        ? residual_ws_bytes(N, M, scheme - MCKL_HIP_RESIDUAL)
+        : align256(std::max<size_t>(M, 1) * sizeof(double)) +
+            rs_work_bytes(N, M, scheme == MCKL_HIP_MULTINOMIAL);
+}
+
+int resample_check(int scheme, size_t N, size_t M, const mckl_hip_engine *e,
+    const double *w, const uint32_t *rep)
+{
+    const bool residual = scheme >= MCKL_HIP_RESIDUAL &&
+        scheme <= MCKL_HIP_RESIDUAL_SYSTEMATIC;
+    const int rc = check_scheme(residual ? scheme - MCKL_HIP_RESIDUAL : scheme);
+    if (rc)
+        return rc;
+    if (!e || !rep)
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    // N == 1 and M == 0 never read the weights (rep = {M} or zeros)
+    if (!w && N > 1 && M > 0)
+        return set_error(MCKL_HIP_EINVAL, "NULL weights");
+    if (N >= 0x7FFFFFFFu || M >= 0x7FFFFFFFu)
+        return set_error(MCKL_HIP_EINVAL, "sizes must be < 2^31");
+    return MCKL_HIP_OK;
+}
+
+// ResampleAlgorithm::operator() + rep -> index on a workspace of
+// resample_ws_bytes(scheme, N, M) bytes; ws == NULL: allocate one on the
+// stream.  Every kernel of the call writes its part of the workspace before a
+// later kernel of the same call reads it, so nothing needs to start at zero.
+int resample_impl(int scheme, size_t N, size_t M, mckl_hip_engine *e,
+    const double *d_w, uint32_t *d_rep, uint32_t *d_idx, hipStream_t st,
+    void *ws)
+{
+    Scratch s(st);
+    const bool residual = scheme >= MCKL_HIP_RESIDUAL;
+    if (residual && N == 0)
+        return MCKL_HIP_OK;
+    if (!ws) {
+        const int rc = s.alloc(resample_ws_bytes(scheme, N, M));
+        if (rc)
+            return rc;
+        ws = s.p;
+    }
+    int rc;
+    if (residual) {
+        RsWork w;
+        rc = residual_rep(scheme - MCKL_HIP_RESIDUAL, N, M, e, d_w, d_rep, st,
+            ws, w);
+        if (rc)
+            return rc;
+        if (d_idx)
+            rc = rep_index(N, M, d_rep, d_idx, st, &w);
+        return rc;
+    }
+    // one workspace for the uniforms and every helper
+    const bool sorted = scheme == MCKL_HIP_MULTINOMIAL;
+    const size_t ou = align256(std::max<size_t>(M, 1) * sizeof(double));
+    double *u = static_cast<double *>(ws);
+    RsWork w;
+    rs_work_carve(static_cast<char *>(ws) + ou, N, M, sorted, w);
+    rc = sequence(scheme, e, M, u, st, &w);
+    if (rc)
+        return rc;
+    bool tiles = false;
+    rc = trans_u01_rep(N, M, d_w, u, d_rep, st, &w, d_idx != nullptr, &tiles);
+    if (rc)
+        return rc;
+    if (d_idx)
+        rc = rep_index(N, M, d_rep, d_idx, st, &w, tiles);
+    return rc;
+}
+
 template <typename U>
 int estimate_impl(size_t n, size_t dim, int layout, const U *u,
     const double *w, double *r, hipStream_t st)
@@ -1122,6 +1198,51 @@ int estimate_impl(size_t n, size_t dim, int layout, const U *u,
     MCKL_HIP_CHECK_LAUNCH("k_est_tiles");
     k_est_final<<<(uint32_t)dim, kB, 0, st>>>(nt, phi, plo, r);
     MCKL_HIP_CHECK_LAUNCH("k_est_final");
+    return MCKL_HIP_OK;
+}
+
+} // namespace
+
+// ---- resampling workspace for callers that cannot allocate ----
+// The workspace of mckl_hip_resample_noalloc lives in this array, sized for
+// N, M <= kNoAllocMax under every scheme (about 36 MiB).  Each device holds
+// its own copy until the process ends.  In host code `scratch` names a host
+// stand-in, so its device address is looked up once per device
+// (noalloc_workspace) and only that address is handed to kernels.
+constexpr size_t kNoAllocMax = size_t(1) << 20;
+constexpr size_t noalloc_bytes()
+{
+    size_t b = 0;
+    for (int s = MCKL_HIP_MULTINOMIAL; s <= MCKL_HIP_RESIDUAL_SYSTEMATIC; ++s)
+        b = std::max(b, resample_ws_bytes(s, kNoAllocMax, kNoAllocMax));
+    return b;
+}
+constexpr size_t kNoAllocBytes = noalloc_bytes();
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices];
+
+// the current device's address of `scratch`, looked up on first use
+int noalloc_workspace(void **out)
+{
+    int dev = -1;
+    MCKL_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(MCKL_HIP_EINVAL, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        MCKL_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            return set_error(MCKL_HIP_EHIP, "no device address for scratch");
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
     return MCKL_HIP_OK;
 }
 
@@ -1200,52 +1321,30 @@ int mckl_hip_resample_trans_rep_index(size_t N, size_t M,
 int mckl_hip_resample(int scheme, size_t N, size_t M, mckl_hip_engine *e,
     const double *d_w, uint32_t *d_rep, uint32_t *d_idx, void *stream)
 {
-    const bool residual = scheme >= MCKL_HIP_RESIDUAL &&
-        scheme <= MCKL_HIP_RESIDUAL_SYSTEMATIC;
-    int rc = check_scheme(residual ? scheme - MCKL_HIP_RESIDUAL : scheme);
+    const int rc = resample_check(scheme, N, M, e, d_w, d_rep);
     if (rc)
         return rc;
-    if (!e || !d_rep)
-        return set_error(MCKL_HIP_EINVAL, "NULL argument");
-    if (N >= 0x7FFFFFFFu || M >= 0x7FFFFFFFu)
-        return set_error(MCKL_HIP_EINVAL, "sizes must be < 2^31");
-    hipStream_t st = as_stream(stream);
-    if (residual) {
-        if (N == 0)
-            return MCKL_HIP_OK;
-        const int base = scheme - MCKL_HIP_RESIDUAL;
-        Scratch s(st);
-        rc = s.alloc(residual_ws_bytes(N, M, base));
-        if (rc)
-            return rc;
-        RsWork w;
-        rc = residual_rep(base, N, M, e, d_w, d_rep, st, s.p, w);
-        if (rc)
-            return rc;
-        if (d_idx)
-            rc = rep_index(N, M, d_rep, d_idx, st, &w);
-        return rc;
-    }
-    // one workspace for the uniforms and every helper
-    const bool sorted = scheme == MCKL_HIP_MULTINOMIAL;
-    const size_t ou = align256(std::max<size_t>(M, 1) * sizeof(double));
-    Scratch s(st);
-    rc = s.alloc(ou + rs_work_bytes(N, M, sorted));
+    return resample_impl(scheme, N, M, e, d_w, d_rep, d_idx,
+        as_stream(stream), nullptr);
+}
+
+int mckl_hip_resample_noalloc(int scheme, size_t N, size_t M,
+    mckl_hip_engine *e, const double *d_w, uint32_t *d_rep, uint32_t *d_idx,
+    void *stream)
+{
+    int rc = resample_check(scheme, N, M, e, d_w, d_rep);
     if (rc)
         return rc;
-    double *u = static_cast<double *>(s.p);
-    RsWork w;
-    rs_work_carve(static_cast<char *>(s.p) + ou, N, M, sorted, w);
-    rc = sequence(scheme, e, M, u, st, &w);
+    if (N > kNoAllocMax || M > kNoAllocMax)
+        return set_error(MCKL_HIP_EINVAL,
+            "resample_noalloc: N = %zu, M = %zu exceed the workspace limit "
+            "%zu", N, M, kNoAllocMax);
+    void *ws = nullptr;
+    rc = noalloc_workspace(&ws);
     if (rc)
         return rc;
-    bool tiles = false;
-    rc = trans_u01_rep(N, M, d_w, u, d_rep, st, &w, d_idx != nullptr, &tiles);
-    if (rc)
-        return rc;
-    if (d_idx)
-        rc = rep_index(N, M, d_rep, d_idx, st, &w, tiles);
-    return rc;
+    return resample_impl(scheme, N, M, e, d_w, d_rep, d_idx,
+        as_stream(stream), ws);
 }
 
 int mckl_hip_estimate_f64(size_t n, size_t dim, int layout, const double *d_u,
