@@ -83,9 +83,21 @@ def build_host_test():
     return out
 
 
+def build_crmath_host():
+    """Host build of csrc/crmath.hpp for the CPU tests (tests/cpp)."""
+    src = os.path.join(ROOT, "tests", "cpp", "crmath_host.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "libcrmath_host.so")
+    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math",
+          "-fPIC", "-shared", "-Wall", "-o", out, src])
+    return out
+
+
 def main():
     build_lib()
     build_host_test()
+    build_crmath_host()
 
 
 if __name__ == "__main__":

@@ -8,6 +8,7 @@
 // engine's buffered block is handled by indexing in "block space": block 0
 // is the buffered block, and the first `skip` elements are masked.
 #include "common.hpp"
+#include "crmath.hpp"
 #include "engines.hpp"
 
 #include <algorithm>
@@ -25,6 +26,8 @@ std::string &last_error_ref()
 }
 
 __constant__ uint32_t c_aes_t0[256];
+// tables of the correctly rounded double log/sin/cos (crmath.hpp)
+__device__ crm::Tables d_crm_tables;
 
 // ---------------------------------------------------------------------------
 // AES T0 table from GF(2^8) arithmetic (FIPS-197 5.1.1); layout as
@@ -76,6 +79,14 @@ int ensure_device_tables()
         uint32_t t0[256];
         build_t0(t0);
         MCKL_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_aes_t0), t0, sizeof(t0)));
+        static crm::Tables crm_host;
+        static bool crm_built = false;
+        if (!crm_built) {
+            crm::build_tables(&crm_host);
+            crm_built = true;
+        }
+        MCKL_HIP_TRY(hipMemcpyToSymbol(
+            HIP_SYMBOL(d_crm_tables), &crm_host, sizeof(crm_host)));
         done[dev] = true;
     }
     return MCKL_HIP_OK;
@@ -363,17 +374,19 @@ __device__ __forceinline__ void box_muller(uint32_t ws, uint32_t wt,
     }
 }
 
+// Double: log, sin and cos correctly rounded (crmath.hpp), so a result can
+// differ from the reference's only where its libm misrounds.
 __device__ __forceinline__ void box_muller(uint32_t ws, uint32_t wt,
     double mean, double sd, double &o_sin, double &o_cos)
 {
     const double u1 = u01_f64(2, ws);
     const double u2 = u01_f64(2, wt);
-    double s = log(u1);
+    double s = crm::log_cr(u1, d_crm_tables.log_c);
     s = __dmul_rn(s, -2.0);
     s = sqrt_rn(s);
     const double th = __dmul_rn(6.283185307179586, u2);
     double a, b;
-    sincos(th, &a, &b);
+    crm::sincos_cr(th, d_crm_tables.sin_a, d_crm_tables.cos_a, a, b);
     if (sd != 1.0)
         s = __dmul_rn(sd, s);
     if (mean != 0.0) {
@@ -401,8 +414,11 @@ __device__ __forceinline__ double box_muller_tail(
     uint32_t wu, uint32_t wv, double mean, double sd)
 {
     const double u = u01_f64(2, wu), v = u01_f64(2, wv);
-    const double r = sqrt_rn(__dmul_rn(-2.0, log(u)));
-    const double c = cos(__dmul_rn(6.283185307179586, v));
+    const double r =
+        sqrt_rn(__dmul_rn(-2.0, crm::log_cr(u, d_crm_tables.log_c)));
+    double sn, c;
+    crm::sincos_cr(__dmul_rn(6.283185307179586, v), d_crm_tables.sin_a,
+        d_crm_tables.cos_a, sn, c);
     return __dadd_rn(mean, __dmul_rn(__dmul_rn(sd, r), c));
 }
 

@@ -12,6 +12,7 @@
 #include "mckl_oracle.h"
 
 #include <math.h>
+#include <quadmath.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -698,15 +699,56 @@ void mo_rand_uniform_real_f32(mo_engine *e, size_t n, float a, float b,
 /* draws); math/vmf.hpp:1052-1077 (sincos), :670-722 (muladd = a*b+c      */
 /* without FMA); math/constants.hpp:39-51,111 (2*pi constant).            */
 /* ===================================================================== */
+/* glibc's result y = f(x) against the correctly rounded value (the quad-  */
+/* precision libquadmath value rounded once): 1 when they agree.  These    */
+/* flags say which outputs depend only on correctly rounded libm results.  */
+#define MO_CR_LOG 0
+#define MO_CR_SIN 1
+#define MO_CR_COS 2
+static __float128 mo_quad_fn(int fn, __float128 x)
+{
+    return fn == MO_CR_LOG ? logq(x) : (fn == MO_CR_SIN ? sinq(x) : cosq(x));
+}
+static int mo_cr_f32(int fn, float x, float y)
+{
+    return y == (float)mo_quad_fn(fn, (__float128)x);
+}
+static int mo_cr_f64(int fn, double x, double y)
+{
+    return y == (double)mo_quad_fn(fn, (__float128)x);
+}
+
+void mo_quad_eval(int fn, size_t n, const double *x, double *y)
+{
+    for (size_t i = 0; i < n; ++i)
+        y[i] = (double)mo_quad_fn(fn, (__float128)x[i]);
+}
+
+/* x = hi + lo exactly as a quad value, minus f(x0) in quad, in units of   */
+/* f(x0)'s magnitude: the relative error of a double-double table entry    */
+void mo_quad_dd_relerr(int fn, size_t n, const double *x0, const double *hi,
+    const double *lo, double *err)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const __float128 v = mo_quad_fn(fn, (__float128)x0[i]);
+        const __float128 d = ((__float128)hi[i] + (__float128)lo[i]) - v;
+        err[i] = v == 0 ? (double)fabsq(d) : (double)fabsq(d / v);
+    }
+}
+
 static void mo_normal_chunk_f32(
-    mo_engine *e, size_t n, float *r, float mean, float sd)
+    mo_engine *e, size_t n, float *r, float mean, float sd, uint8_t *cl)
 {
     float s[1024];
+    uint8_t okl[1024], oks[1024], okc[1024];
     const size_t nu = n / 2;
     float *u1 = r, *u2 = r + nu;
     mo_rand_u01_f32(e, MO_OC, n, r);
     for (size_t i = 0; i < nu; ++i)
         s[i] = logf(u1[i]);
+    if (cl)
+        for (size_t i = 0; i < nu; ++i)
+            okl[i] = (uint8_t)mo_cr_f32(MO_CR_LOG, u1[i], s[i]);
     for (size_t i = 0; i < nu; ++i)
         s[i] = s[i] * -2.0f;
     for (size_t i = 0; i < nu; ++i)
@@ -717,8 +759,20 @@ static void mo_normal_chunk_f32(
      * sincos call may round differently) */
     for (size_t i = 0; i < nu; ++i)
         u1[i] = sinf(u2[i]);
-    for (size_t i = 0; i < nu; ++i)
-        u2[i] = cosf(u2[i]);
+    if (cl)
+        for (size_t i = 0; i < nu; ++i)
+            oks[i] = (uint8_t)mo_cr_f32(MO_CR_SIN, u2[i], u1[i]);
+    for (size_t i = 0; i < nu; ++i) {
+        const float th = u2[i];
+        u2[i] = cosf(th);
+        if (cl)
+            okc[i] = (uint8_t)mo_cr_f32(MO_CR_COS, th, u2[i]);
+    }
+    if (cl)
+        for (size_t i = 0; i < nu; ++i) {
+            cl[i] = okl[i] & oks[i];
+            cl[nu + i] = okl[i] & okc[i];
+        }
     if (sd != 1.0f)
         for (size_t i = 0; i < nu; ++i)
             s[i] = sd * s[i];
@@ -735,31 +789,47 @@ static void mo_normal_chunk_f32(
     }
 }
 
-void mo_rand_normal_f32(mo_engine *e, size_t n, float mean, float sd, float *r)
+/* clean (may be NULL): per output, 1 when every logf/sinf/cosf result it
+ * depends on is correctly rounded */
+void mo_rand_normal_f32_cr(mo_engine *e, size_t n, float mean, float sd,
+    float *r, uint8_t *clean)
 {
     const size_t k = 2048;
     const size_t m = n / k, l = n % k;
     for (size_t i = 0; i < m; ++i, r += k)
-        mo_normal_chunk_f32(e, k, r, mean, sd);
-    mo_normal_chunk_f32(e, l, r, mean, sd);
+        mo_normal_chunk_f32(e, k, r, mean, sd, clean ? clean + i * k : NULL);
+    mo_normal_chunk_f32(e, l, r, mean, sd, clean ? clean + m * k : NULL);
     if (n % 2 != 0) {
         float u = mo_u01_u32_f32(MO_OC, mo_next_u32(e));
         float v = mo_u01_u32_f32(MO_OC, mo_next_u32(e));
-        r[l - 1] = mean +
-            sd * sqrtf(-2 * logf(u)) *
-                cosf(6.283185307179586476925286766559005768394f * v);
+        const float lu = logf(u);
+        const float th = 6.283185307179586476925286766559005768394f * v;
+        const float c = cosf(th);
+        r[l - 1] = mean + sd * sqrtf(-2 * lu) * c;
+        if (clean)
+            clean[n - 1] = (uint8_t)(mo_cr_f32(MO_CR_LOG, u, lu) &
+                mo_cr_f32(MO_CR_COS, th, c));
     }
 }
 
+void mo_rand_normal_f32(mo_engine *e, size_t n, float mean, float sd, float *r)
+{
+    mo_rand_normal_f32_cr(e, n, mean, sd, r, NULL);
+}
+
 static void mo_normal_chunk_f64(
-    mo_engine *e, size_t n, double *r, double mean, double sd)
+    mo_engine *e, size_t n, double *r, double mean, double sd, uint8_t *cl)
 {
     double s[512];
+    uint8_t okl[512], oks[512], okc[512];
     const size_t nu = n / 2;
     double *u1 = r, *u2 = r + nu;
     mo_rand_u01_f64(e, MO_OC, n, r);
     for (size_t i = 0; i < nu; ++i)
         s[i] = log(u1[i]);
+    if (cl)
+        for (size_t i = 0; i < nu; ++i)
+            okl[i] = (uint8_t)mo_cr_f64(MO_CR_LOG, u1[i], s[i]);
     for (size_t i = 0; i < nu; ++i)
         s[i] = s[i] * -2.0;
     for (size_t i = 0; i < nu; ++i)
@@ -768,8 +838,20 @@ static void mo_normal_chunk_f64(
         u2[i] = 6.283185307179586 * u2[i];
     for (size_t i = 0; i < nu; ++i)
         u1[i] = sin(u2[i]);
-    for (size_t i = 0; i < nu; ++i)
-        u2[i] = cos(u2[i]);
+    if (cl)
+        for (size_t i = 0; i < nu; ++i)
+            oks[i] = (uint8_t)mo_cr_f64(MO_CR_SIN, u2[i], u1[i]);
+    for (size_t i = 0; i < nu; ++i) {
+        const double th = u2[i];
+        u2[i] = cos(th);
+        if (cl)
+            okc[i] = (uint8_t)mo_cr_f64(MO_CR_COS, th, u2[i]);
+    }
+    if (cl)
+        for (size_t i = 0; i < nu; ++i) {
+            cl[i] = okl[i] & oks[i];
+            cl[nu + i] = okl[i] & okc[i];
+        }
     if (sd != 1.0)
         for (size_t i = 0; i < nu; ++i)
             s[i] = sd * s[i];
@@ -786,20 +868,31 @@ static void mo_normal_chunk_f64(
     }
 }
 
-void mo_rand_normal_f64(
-    mo_engine *e, size_t n, double mean, double sd, double *r)
+void mo_rand_normal_f64_cr(mo_engine *e, size_t n, double mean, double sd,
+    double *r, uint8_t *clean)
 {
     const size_t k = 1024;
     const size_t m = n / k, l = n % k;
     for (size_t i = 0; i < m; ++i, r += k)
-        mo_normal_chunk_f64(e, k, r, mean, sd);
-    mo_normal_chunk_f64(e, l, r, mean, sd);
+        mo_normal_chunk_f64(e, k, r, mean, sd, clean ? clean + i * k : NULL);
+    mo_normal_chunk_f64(e, l, r, mean, sd, clean ? clean + m * k : NULL);
     if (n % 2 != 0) {
         double u = mo_u01_u32_f64(MO_OC, mo_next_u32(e));
         double v = mo_u01_u32_f64(MO_OC, mo_next_u32(e));
-        r[l - 1] = mean +
-            sd * sqrt(-2 * log(u)) * cos(6.283185307179586 * v);
+        const double lu = log(u);
+        const double th = 6.283185307179586 * v;
+        const double c = cos(th);
+        r[l - 1] = mean + sd * sqrt(-2 * lu) * c;
+        if (clean)
+            clean[n - 1] = (uint8_t)(mo_cr_f64(MO_CR_LOG, u, lu) &
+                mo_cr_f64(MO_CR_COS, th, c));
     }
+}
+
+void mo_rand_normal_f64(
+    mo_engine *e, size_t n, double mean, double sd, double *r)
+{
+    mo_rand_normal_f64_cr(e, n, mean, sd, r, NULL);
 }
 
 /* ===================================================================== */

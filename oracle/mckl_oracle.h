@@ -110,6 +110,17 @@ void mo_rand_uniform_real_f32(mo_engine *e, size_t n, float a, float b,
 void mo_rand_normal_f32(mo_engine *e, size_t n, float mean, float sd, float *r);
 void mo_rand_normal_f64(
     mo_engine *e, size_t n, double mean, double sd, double *r);
+/* the same fills; clean[i] = 1 when every libm result output i depends on
+ * (log, and sin or cos) equals the correctly rounded value */
+void mo_rand_normal_f32_cr(mo_engine *e, size_t n, float mean, float sd,
+    float *r, uint8_t *clean);
+void mo_rand_normal_f64_cr(mo_engine *e, size_t n, double mean, double sd,
+    double *r, uint8_t *clean);
+/* quad-precision log (fn 0), sin (1), cos (2) rounded once to double */
+void mo_quad_eval(int fn, size_t n, const double *x, double *y);
+/* |(hi + lo) - f(x0)| / |f(x0)| in quad precision */
+void mo_quad_dd_relerr(int fn, size_t n, const double *x0, const double *hi,
+    const double *lo, double *err);
 
 /* ---- weight ---- */
 void mo_weight_set_log(size_t n, const double *v, double *w, double *ess);

@@ -127,6 +127,38 @@ def rand_normal(e, n, mean, sd, dtype):
     return out[:n]
 
 
+def rand_normal_cr(e, n, mean, sd, dtype):
+    """rand_normal plus, per output, whether every libm result it depends
+    on (log, and sin or cos) is the correctly rounded value."""
+    out = np.zeros(max(n, 1), dtype)
+    clean = np.zeros(max(n, 1), np.uint8)
+    if np.dtype(dtype) == np.float64:
+        O.mo_rand_normal_f64_cr(ctypes.byref(e), _sz(n), ctypes.c_double(mean),
+                                ctypes.c_double(sd), P(out), P(clean))
+    else:
+        O.mo_rand_normal_f32_cr(ctypes.byref(e), _sz(n), ctypes.c_float(mean),
+                                ctypes.c_float(sd), P(out), P(clean))
+    return out[:n], clean[:n].astype(bool)
+
+
+CR_LOG, CR_SIN, CR_COS = 0, 1, 2
+
+
+def quad_eval(fn, x):
+    """log / sin / cos of doubles in quad precision, rounded once."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(max(len(x), 1))
+    O.mo_quad_eval(fn, _sz(len(x)), P(x), P(y))
+    return y[:len(x)]
+
+
+def quad_dd_relerr(fn, x0, hi, lo):
+    x0, hi, lo = (np.ascontiguousarray(a, np.float64) for a in (x0, hi, lo))
+    err = np.zeros(max(len(x0), 1))
+    O.mo_quad_dd_relerr(fn, _sz(len(x0)), P(x0), P(hi), P(lo), P(err))
+    return err[:len(x0)]
+
+
 def philox_block(key, ctr):
     k = (ctypes.c_uint32 * 2)(*key)
     c = (ctypes.c_uint64 * 2)(*ctr)

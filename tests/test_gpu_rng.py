@@ -6,9 +6,11 @@ The HIP path (libmckl_hip.so through its C ABI) is compared with
   * fixtures written from the compiled reference (tests/golden), and
   * the CPU oracle, which is itself pinned bit-exact to the reference by
     tests/test_oracle_*.py.
-Integer and U01 outputs must be bit-exact; Normal outputs within 1 ulp of the
-reference except for a reported, tiny fraction at 2 ulp (glibc's logf/sinf/
-cosf are not correctly rounded; the device evaluates them in FP64).
+Integer and U01 outputs must be bit-exact.  Normal outputs: the device takes
+the correctly rounded log, sin and cos, the reference glibc's, which misround
+a small fraction of inputs.  Every value whose glibc results are all
+correctly rounded (checked in quad precision by the oracle) must be
+bit-exact; the others are counted as explained and must stay within 2 ulp.
 """
 import ctypes
 import hashlib
@@ -306,14 +308,29 @@ def term_ulps(got, want, mean):
     return np.abs(g - w) / np.spacing(scale).astype(np.float64)
 
 
+def normal_parity(got, want, clean, mean):
+    """(explained differences, histogram of their ulp distances); asserts
+    that every value glibc computes from correctly rounded libm results is
+    bit-exact and that the explained ones are within 2 ulp."""
+    d = term_ulps(got, want, mean) if mean != 0.0 else ulp_diff(got, want)
+    bad = np.nonzero((d > 0) & clean)[0]
+    assert len(bad) == 0, ("unexplained differences", bad[:10],
+                           got[bad[:10]], want[bad[:10]])
+    expl = d[~clean]
+    assert expl.size == 0 or expl.max() <= 2.0, expl.max()
+    hist = np.bincount(np.ceil(expl).astype(np.int64), minlength=3)[:3]
+    return int((expl > 0).sum()), hist
+
+
 @pytest.mark.parametrize("kind", [1, 2, 3])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_normal_fills(dev, kind, dtype):
     m = _m()
     isz = np.dtype(dtype).itemsize
-    worst = 0
-    mism = 0
+    nexpl = 0
     total = 0
+    hist = np.zeros(3, np.int64)
+    nclean = 0
     for pre, n, mean, sd in NORMAL_CASES:
         key = [pre + 7, 1, 2, 3][:2 if kind == 1 else 4]
         rng = _cls(kind, 32)(key=key)
@@ -325,50 +342,58 @@ def test_normal_fills(dev, kind, dtype):
         out = _tensor(n, isz, dev, fp=True)
         m.rand(rng, m.NormalDistribution(mean, sd, dtype), n, out)
         got = out.cpu().numpy()[:n]
-        want = orc.rand_normal(e, n, mean, sd, dtype)
-        d = term_ulps(got, want, mean) if mean != 0.0 else ulp_diff(got, want)
-        print("  pre=%d n=%d mean=%g sd=%g: max %.2f ulp, %d differ" %
-              (pre, n, mean, sd, float(d.max()), int((d > 0).sum())))
-        worst = max(worst, float(d.max()))
-        mism += int((d > 0).sum())
+        want, clean = orc.rand_normal_cr(e, n, mean, sd, dtype)
+        k, h = normal_parity(got, want, clean, mean)
+        nexpl += k
+        hist += h
+        nclean += int(clean.sum())
         total += n
         assert rng.ctr() == orc.state(e)[0] and rng.index == orc.state(e)[1]
-    print("normal kind=%d %s: max %.2f ulp, %d of %d differ" %
-          (kind, np.dtype(dtype).name, worst, mism, total))
-    assert worst <= 2.0
-    assert mism <= max(10, total // 10)
+    print("normal kind=%d %s: %d of %d values from correctly rounded libm "
+          "results, all bit-exact; %d explained differences (ulp histogram "
+          "0/1/2: %s)" % (kind, np.dtype(dtype).name, nclean, total, nexpl,
+                          hist.tolist()))
 
 
 @pytest.mark.parametrize("ename,kind", [("threefry", 2), ("ars", 3)])
 @pytest.mark.parametrize("dname", ["f32", "f64"])
 def test_config2_streams(dev, ename, kind, dname):
     """64 streams x 2^22 Normal values in one launch: windows against the
-    reference fixtures (ulp), engine states exact, moments against theory."""
+    reference fixtures (bit-exact where glibc rounds correctly, explained
+    otherwise), engine states exact, moments against theory."""
     import torch
     m = _m()
     fx = np.load(os.path.join(GOLDEN, "normal_streams.npz"))
     n = 1 << 22
     dt, mean, sd = ((torch.float32, 0.0, 1.0) if dname == "f32"
                     else (torch.float64, 2.5, 0.3))
+    npdt = np.float32 if dname == "f32" else np.float64
     cls = m.Threefry4x64 if kind == 2 else m.ARS
     engines = [cls(key=[s, 0, 0, 0]) for s in range(1, 65)]
     out = torch.empty(64 * n, dtype=dt, device=dev)
     m.streams_normal(engines, n, mean, sd, out)
     r = out.view(64, n)
-    worst = 0
+    nexpl = 0
+    hist = np.zeros(3, np.int64)
     for s in (1, 2, 64):
         p = "%s_%s_%d" % (ename, dname, s)
         row = r[s - 1].cpu().numpy()
-        for part, sl in (("head", slice(0, 4096)),
-                         ("mid", slice(n // 2 - 2048, n // 2 + 2048)),
-                         ("tail", slice(n - 4096, n))):
-            d = (ulp_diff(row[sl], fx[p + "_" + part]) if mean == 0.0 else
-                 term_ulps(row[sl], fx[p + "_" + part], mean))
-            worst = max(worst, float(d.max()))
+        for part, start in (("head", 0), ("mid", n // 2 - 2048),
+                            ("tail", n - 4096)):
+            # windows start on chunk boundaries (2048 floats, 1024
+            # doubles), so the oracle regenerates them after a discard
+            e = orc.engine(kind, [s, 0, 0, 0])
+            orc.O.mo_discard(ctypes.byref(e), ctypes.c_uint64(start))
+            want, clean = orc.rand_normal_cr(e, 4096, mean, sd, npdt)
+            assert np.array_equal(want, fx[p + "_" + part]), (p, part)
+            k, h = normal_parity(row[start:start + 4096], want, clean, mean)
+            nexpl += k
+            hist += h
         st = fx[p + "_state"]
         e = engines[s - 1]
         assert e.ctr() == [int(v) for v in st[:-1]] and e.index == int(st[-1])
-    assert worst <= 2
+    print("config 2 %s %s windows: %d explained differences (0/1/2 ulp: %s)"
+          % (ename, dname, nexpl, hist.tolist()))
     x = r.double()
     mu = x.mean().item()
     var = x.var().item()
