@@ -398,7 +398,42 @@ def bench_grad(m, torch):
                                                    np.abs(g2))),
         "mean_grad": gr.mean(0).tolist(),
     }
+    res.update(bench_grad_kalman(m))
     return res
+
+
+def bench_grad_kalman(m):
+    """The adjoint's mean gradient on the linear-Gaussian model against the
+    Kalman FP64 gradient (values in tests/golden/kalman_lg.json, written by
+    tests/golden/make_kalman.py from the oracle's Kalman filter): the
+    relative error of the mean and its standard error, per parameter, at
+    N = 2^16 with 256 seeds on the T = 100 observations at (0.9, 1, 0.5),
+    and with 16384 seeds on their first 10 at (0.7, 1.3, 0.7)."""
+    import numpy as np
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "tests", "golden", "kalman_lg.json")) as f:
+        ref = json.load(f)
+    out = {}
+    for name, nb, K in (("lg_T100", 256, 50), ("lg_T10_offset", 16384, 5)):
+        r = ref[name]
+        y = np.asarray(r["y"])
+        ex = np.asarray(r["grad"])
+        g = m.SMCGradient(m.MODEL_LG, nb, 1 << 16, y, np.arange(1, nb + 1),
+                          0.5, m.SYSTEMATIC, K=K)
+        t0 = time.perf_counter()
+        _, gr = g.run(r["theta"])
+        t = time.perf_counter() - t0
+        del g
+        mean = gr.mean(0)
+        se = gr.std(0, ddof=1) / np.sqrt(nb)
+        out["smc_grad_vs_kalman_%s" % name] = {
+            "theta": r["theta"], "T": r["T"], "N": 1 << 16, "seeds": nb,
+            "kalman_grad": ex.tolist(), "mean_grad": mean.tolist(),
+            "rel_err": (np.abs(mean - ex) / np.abs(ex)).tolist(),
+            "rel_se": (se / np.abs(ex)).tolist(),
+            "ms": round(t * 1e3, 1),
+        }
+    return out
 
 
 def bench_resample_index(m, torch, N=1 << 20, R=256, reps=5):

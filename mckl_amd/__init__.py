@@ -28,6 +28,14 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _init_device():
+    """Initialise torch's HIP context before the library's first device call
+    (a handle allocates device memory when it is created); torch cannot
+    attach to the device afterwards on this runtime."""
+    import torch
+    torch.cuda.init()
+
+
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
@@ -427,6 +435,7 @@ class ResampleIndex:
 
     def __init__(self):
         h = ctypes.c_void_p()
+        _init_device()
         check(lib.mckl_hip_resample_index_create(ctypes.byref(h)))
         self._h = h
 
@@ -533,6 +542,7 @@ class SMCSampler:
         self.T = len(self._y)
         self.N = int(N)
         h = ctypes.c_void_p()
+        _init_device()
         check(lib.mckl_hip_smc_create(model, _dptr(self._th), self.T,
                                       _dptr(self._y), self.N, int(seed),
                                       float(resample_threshold), scheme,
@@ -592,6 +602,7 @@ class SMCBatch:
         self.nfilters = len(self._N)
         self.T = self._y.shape[-1]
         h = ctypes.c_void_p()
+        _init_device()
         check(lib.mckl_hip_smc_batch_create(
             model, self.nfilters,
             self._N.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
@@ -650,6 +661,7 @@ class SMCGradient:
         self._seeds = np.ascontiguousarray(
             np.broadcast_to(np.asarray(seeds, np.uint64), (self.nbatch,)))
         h = ctypes.c_void_p()
+        _init_device()
         check(lib.mckl_hip_smc_grad_create(
             model, self.nbatch, int(N), self.T, _dptr(self._y.reshape(-1)),
             int(per_point),

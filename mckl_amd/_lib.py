@@ -48,6 +48,11 @@ if not os.path.exists(LIB_PATH):
         "libmckl_hip.so is not built (expected at %s); run "
         "`python -m mckl_amd.build`" % LIB_PATH)
 
+# torch ships its own libamdhip64.so.7 (the same soname as ROCm's): import it
+# first so that this library binds to the HIP runtime torch already uses;
+# loading ROCm's copy first leaves torch without a device.
+import torch  # noqa: E402,F401
+
 lib = ctypes.CDLL(LIB_PATH)
 
 _vp = ctypes.c_void_p

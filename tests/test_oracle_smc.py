@@ -84,6 +84,20 @@ def test_kalman_gradient_matches_finite_differences():
         assert abs(fd - g[k]) < 1e-5 * max(1.0, abs(g[k]))
 
 
+def test_kalman_fixture_of_the_bench():
+    """tests/golden/kalman_lg.json (the bench's reference values) holds the
+    Kalman log-likelihood and gradient of the committed observations."""
+    import json
+    ref = json.load(open(os.path.join(GOLDEN, "kalman_lg.json")))
+    y = np.load(os.path.join(GOLDEN, "observations.npz"))["lg_y"]
+    for r in ref.values():
+        T = r["T"]
+        assert np.array_equal(np.asarray(r["y"]), y[:T])
+        assert r["loglik"] == kalman.loglik(y[:T], *r["theta"])
+        assert np.array_equal(np.asarray(r["grad"]),
+                              kalman.grad(y[:T], *r["theta"]))
+
+
 def test_smc_oracle_vs_kalman():
     """Bootstrap PF log-likelihood on the LG configuration (T = 100,
     observations from the committed fixture) within Monte Carlo error of the
