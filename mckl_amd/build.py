@@ -84,14 +84,19 @@ def build_host_test():
 
 
 def build_crmath_host():
-    """Host build of csrc/crmath.hpp for the CPU tests (tests/cpp)."""
-    src = os.path.join(ROOT, "tests", "cpp", "crmath_host.cpp")
-    if not os.path.exists(src):
-        return None
-    out = os.path.join(ROOT, "tests", "cpp", "libcrmath_host.so")
-    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math",
-          "-fPIC", "-shared", "-Wall", "-o", out, src])
-    return out
+    """Host builds of device headers for the CPU tests (tests/cpp):
+    csrc/crmath.hpp (correctly rounded log/sin/cos) and csrc/smc_index.hpp
+    (the resampling kernels' index arithmetic)."""
+    outs = []
+    for name in ("crmath_host", "rs_guard_host"):
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        if not os.path.exists(src):
+            continue
+        out = os.path.join(ROOT, "tests", "cpp", "lib%s.so" % name)
+        _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off",
+              "-fno-fast-math", "-fPIC", "-shared", "-Wall", "-o", out, src])
+        outs.append(out)
+    return outs
 
 
 def main():

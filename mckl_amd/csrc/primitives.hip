@@ -13,6 +13,7 @@
 // cumulative-weight boundary.  Temporary buffers come from the stream-ordered
 // allocator of the caller's stream.
 #include "common.hpp"
+#include "smc_index.hpp"
 #include "smc_math.hpp"
 
 #include <algorithm>
@@ -575,16 +576,11 @@ __global__ __launch_bounds__(kB) void k_ri_deal(uint64_t N, uint64_t K,
         Es[q] = Eincl[t0 + q];
     const uint32_t beg = t0 == 0 ? 0u : Eincl[t0 - 1];
     __syncthreads();
-    const uint32_t end = Es[n - 1];
+    // ranks past ztot + (M - K) have no destination: a vector that is not a
+    // replication vector (extras beyond M) ends its loop there
+    const uint32_t end = rank_end(Es[n - 1], (uint64_t)ztot + (M - K));
     for (uint32_t r = beg + threadIdx.x; r < end; r += kB) {
-        uint32_t lo = 0, hi = n - 1; // Es[n - 1] = end > r
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (Es[mid] > r)
-                hi = mid;
-            else
-                lo = mid + 1;
-        }
+        const uint32_t lo = rank_source(Es, n, r); // Es[n - 1] >= end > r
         const uint64_t dest =
             r < ztot ? (uint64_t)zpos[r] : K + (uint64_t)(r - ztot);
         if (dest < M)

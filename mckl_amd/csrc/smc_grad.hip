@@ -433,12 +433,15 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
                         const bool own = prev.par[j] == (uint32_t)j;
                         double xs = own ? xpb[j] : 0.0;
                         double cs = own ? cb[j] : 0.0;
-                        const uint32_t Ej = prev.E[j];
+                        // ranks of j's extra copies, kept inside zpos
+                        const uint32_t Ej = rank_end(prev.E[j], A.N);
                         const uint32_t Ejm = j == 0 ? 0u : prev.E[j - 1];
                         for (uint32_t rk = Ejm; rk < Ej; ++rk) {
                             const uint32_t ch = prev.zpos[rk];
-                            xs += xpb[ch];
-                            cs += cb[ch];
+                            if (ch < A.N) {
+                                xs += xpb[ch];
+                                cs += cb[ch];
+                            }
                         }
                         const double Wp = exp(prev.lw[j] - lse_p);
                         xb[j] = xs;

@@ -192,37 +192,43 @@ __device__ __forceinline__ void gtile_counts(const GradArgs &G, uint32_t tile,
     dd total;
     const dd toff = group_excl_scan_dd(loc, gt, ldsd, total);
     const dd base = dd_add(dd{toffh[tile], toffl[tile]}, toff);
+    // boundary counts shared with the neighbouring tiles; the counts are
+    // made non-decreasing inside [clo, chi] (as in tile_counts_l), so the
+    // replication numbers are never negative whatever the rounding
+    const uint32_t clo = tile == 0 ? 0u : U.count_fast(toffh[tile] * invS);
+    const uint32_t chi =
+        tend == G.N ? U.M : U.count_fast(toffh[tile + 1] * invS);
     uint32_t c[kItems];
     dd run{0.0, 0.0};
+    uint32_t cmax = clo;
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         run = dd_add(run, dd{W[k], 0.0});
         c[k] = 0;
         if (i < G.N) {
-            if (i == (uint64_t)G.N - 1) {
-                c[k] = U.M;
-            } else {
-                const double P = i == tend - 1 ? toffh[tile + 1]
-                                               : dd_add(base, run).hi;
-                c[k] = U.count_fast(P * invS);
-            }
+            const uint32_t v = i == tend - 1
+                ? chi
+                : U.count_fast(dd_add(base, run).hi * invS);
+            cmax = cmax > clamp_count(v, clo, chi) ? cmax
+                                                   : clamp_count(v, clo, chi);
+            c[k] = cmax;
+        }
+    }
+    const uint32_t before = group_excl_max_u(cmax, clo, gt, ldsu);
+    uint32_t prev = before;
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        if (i < G.N) {
+            c[k] = c[k] < before ? before : c[k];
             cnt[i] = c[k];
         }
     }
-    ldsu[gt] = c[kItems - 1];
-    __syncthreads();
-    uint32_t prev;
-    if (gt > 0)
-        prev = ldsu[gt - 1];
-    else
-        prev = tile == 0 ? 0u : U.count_fast(toffh[tile] * invS);
-    __syncthreads();
     uint32_t es = 0, zs = 0;
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         if (i < G.N) {
-            const uint32_t rep = c[k] - prev;
-            es += rep > 1 ? rep - 1 : 0;
+            const uint32_t rep = rep_from_counts(c[k], prev);
+            es += extra_copies(rep);
             zs += rep == 0 ? 1 : 0;
         }
         prev = c[k];
@@ -249,8 +255,8 @@ __device__ __forceinline__ void gtile_index(const GradArgs &G, uint32_t tile,
         rep[k] = 0;
         if (i < G.N) {
             const uint32_t p = i == 0 ? 0u : cnt[i - 1];
-            rep[k] = cnt[i] - p;
-            es += rep[k] > 1 ? rep[k] - 1 : 0;
+            rep[k] = rep_from_counts(cnt[i], p);
+            es += extra_copies(rep[k]);
             zs += rep[k] == 0 ? 1 : 0;
         }
     }
@@ -260,7 +266,7 @@ __device__ __forceinline__ void gtile_index(const GradArgs &G, uint32_t tile,
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         if (i < G.N) {
-            ex += rep[k] > 1 ? rep[k] - 1 : 0;
+            ex += extra_copies(rep[k]);
             sl.E[i] = ex;
             sl.par[i] = (uint32_t)i; // zero slots: overwritten by the scatter
             if (rep[k] == 0) {

@@ -20,6 +20,7 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "smc_index.hpp"
 #include "smc_math.hpp"
 
 namespace mckl_hip {
@@ -464,7 +465,7 @@ __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
                 const double P = dd_add(tile_off, dd_mul_d(run, rk)).hi;
                 v = U.count_fast(__dmul_rn(P, invS));
             }
-            v = v < clo ? clo : (v > chi ? chi : v);
+            v = clamp_count(v, clo, chi);
             cmax = v > cmax ? v : cmax;
             c[k] = cmax;
         }
@@ -482,8 +483,8 @@ __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         if (i < A.N) {
-            const uint32_t rep = c[k] - prev;
-            es += rep > 1 ? rep - 1 : 0;
+            const uint32_t rep = rep_from_counts(c[k], prev);
+            es += extra_copies(rep);
             zs += rep == 0 ? 1 : 0;
         }
         prev = c[k];
@@ -606,7 +607,7 @@ __device__ __forceinline__ void tile_rcounts(const SmcArgs &A, uint32_t tile,
                 v = 0;
             else
                 v = U.count_fast(__dmul_rn(dd_add(roff, run).hi, invR));
-            v = v < clo ? clo : (v > chi ? chi : v);
+            v = clamp_count(v, clo, chi);
             cmax = v > cmax ? v : cmax;
             cr[k] = cmax;
         }
@@ -626,8 +627,8 @@ __device__ __forceinline__ void tile_rcounts(const SmcArgs &A, uint32_t tile,
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         if (i0 + k < A.N) {
-            const uint32_t rep = c[k] - prev;
-            es += rep > 1 ? rep - 1 : 0;
+            const uint32_t rep = rep_from_counts(c[k], prev);
+            es += extra_copies(rep);
             zs += rep == 0 ? 1 : 0;
         }
         prev = c[k];
@@ -663,9 +664,9 @@ __device__ __forceinline__ void tile_index_c(const SmcArgs &A, uint32_t tile,
         const uint64_t i = i0 + k;
         rep[k] = 0;
         if (i < A.N) {
-            rep[k] = c[k] - prev;
+            rep[k] = rep_from_counts(c[k], prev);
             prev = c[k];
-            es += rep[k] > 1 ? rep[k] - 1 : 0;
+            es += extra_copies(rep[k]);
             zs += rep[k] == 0 ? 1 : 0;
         }
     }
@@ -676,7 +677,7 @@ __device__ __forceinline__ void tile_index_c(const SmcArgs &A, uint32_t tile,
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        ex += rep[k] > 1 ? rep[k] - 1 : 0;
+        ex += extra_copies(rep[k]);
         E[k] = ex;
         own[k] = (uint32_t)i; // zero slots: overwritten by tile_scatter_ranks
         if (i < A.N && rep[k] == 0) {
@@ -726,17 +727,9 @@ __device__ __forceinline__ void tile_scatter_ranks(const SmcArgs &A,
     __syncthreads();
     // ranks and slots stay inside the arrays even if the counts were not a
     // replication vector (they are: the counts are clamped monotone)
-    uint32_t end = live ? Es[n - 1] : 0u;
-    end = end < A.N ? end : A.N;
+    const uint32_t end = live ? rank_end(Es[n - 1], A.N) : 0u;
     for (uint32_t r = beg + gt; r < end; r += kGroup) {
-        uint32_t lo = 0, hi = n - 1; // Es[n - 1] >= end > r
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (Es[mid] > r)
-                hi = mid;
-            else
-                lo = mid + 1;
-        }
+        const uint32_t lo = rank_source(Es, n, r); // Es[n - 1] >= end > r
         const uint32_t z = zpos[r];
         if (z < A.N)
             idx[z] = (uint32_t)(t0 + lo);
