@@ -40,6 +40,33 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+_FLOAT = {8: "float64", 4: "float32"}
+
+
+def _dev(t, n, itemsize, what, floating=None):
+    """Device pointer of tensor t after checking it: a contiguous tensor in
+    GPU memory with elements of `itemsize` bytes (floating point when
+    `floating` is True, integer when False) holding at least n of them.
+    Raises MCKLError before any launch otherwise."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise MCKLError("%s: expected a torch tensor" % what)
+    if not t.is_cuda:
+        raise MCKLError("%s: tensor is not in GPU memory" % what)
+    if not t.is_contiguous():
+        raise MCKLError("%s: tensor is not contiguous" % what)
+    if itemsize is not None and t.element_size() != itemsize:
+        raise MCKLError("%s: %d-byte elements, expected %d" %
+                        (what, t.element_size(), itemsize))
+    if floating is not None and t.dtype.is_floating_point != floating:
+        raise MCKLError("%s: expected %s elements, got %s" %
+                        (what, _FLOAT.get(itemsize, "floating") if floating
+                         else "integer", t.dtype))
+    if t.numel() < n:
+        raise MCKLError("%s: holds %d values, needs %d" % (what, t.numel(), n))
+    return ctypes.c_void_p(t.data_ptr())
+
+
 class _CounterEngine:
     """CounterEngine<ResultType, Generator> (random/counter.hpp:61-288)."""
 
@@ -186,7 +213,8 @@ class _U01:
     def fill(self, rng, n, r):
         fn = (lib.mckl_hip_rand_u01_f64 if self.dtype == np.float64
               else lib.mckl_hip_rand_u01_f32)
-        check(fn(ctypes.byref(rng.state), self.INTERVAL, n, _ptr(r), _stream()))
+        p = _dev(r, n, self.dtype.itemsize, "U01 output", True)
+        check(fn(ctypes.byref(rng.state), self.INTERVAL, n, p, _stream()))
 
 
 class U01CCDistribution(_U01):
@@ -218,13 +246,14 @@ class NormalDistribution:
         self.dtype = np.dtype(dtype)
 
     def fill(self, rng, n, r):
+        p = _dev(r, n, self.dtype.itemsize, "Normal output", True)
         if self.dtype == np.float64:
             check(lib.mckl_hip_rand_normal_f64(
-                ctypes.byref(rng.state), n, self.mean, self.stddev, _ptr(r),
+                ctypes.byref(rng.state), n, self.mean, self.stddev, p,
                 _stream()))
         else:
             check(lib.mckl_hip_rand_normal_f32(
-                ctypes.byref(rng.state), n, self.mean, self.stddev, _ptr(r),
+                ctypes.byref(rng.state), n, self.mean, self.stddev, p,
                 _stream()))
 
 
@@ -249,8 +278,8 @@ class UniformRealDistribution:
     def fill(self, rng, n, r):
         fn = (lib.mckl_hip_rand_uniform_real_f64 if self.dtype == np.float64
               else lib.mckl_hip_rand_uniform_real_f32)
-        check(fn(ctypes.byref(rng.state), n, self.a, self.b, _ptr(r),
-                 _stream()))
+        p = _dev(r, n, self.dtype.itemsize, "UniformReal output", True)
+        check(fn(ctypes.byref(rng.state), n, self.a, self.b, p, _stream()))
 
 
 def rand(rng, *args):
@@ -260,13 +289,10 @@ def rand(rng, *args):
     """
     if len(args) == 2:
         n, r = args
-        if r.numel() < n:
-            raise ValueError("output tensor too small")
-        check(lib.mckl_hip_rand(ctypes.byref(rng.state), n, _ptr(r), _stream()))
+        p = _dev(r, n, rng.RBITS // 8, "rand output", False)
+        check(lib.mckl_hip_rand(ctypes.byref(rng.state), n, p, _stream()))
         return r
     dist, n, r = args
-    if r.numel() < n:
-        raise ValueError("output tensor too small")
     dist.fill(rng, n, r)
     return r
 
@@ -276,11 +302,12 @@ def streams_normal(engines, n, mean, stddev, r):
     arr = (_lib.Engine * len(engines))()
     for i, e in enumerate(engines):
         arr[i] = e.state
-    if r.dtype.itemsize == 8:
+    p = _dev(r, len(engines) * n, r.element_size(), "Normal output", True)
+    if r.element_size() == 8:
         fn = lib.mckl_hip_rand_normal_f64_streams
     else:
         fn = lib.mckl_hip_rand_normal_f32_streams
-    check(fn(arr, len(engines), n, mean, stddev, _ptr(r), _stream()))
+    check(fn(arr, len(engines), n, mean, stddev, p, _stream()))
     for i, e in enumerate(engines):
         e.state = arr[i]
     return r
@@ -320,11 +347,13 @@ class Weight:
                                             _ptr(self._stats), _stream()))
 
     def set_log(self, v):
-        check(lib.mckl_hip_weight_set_log(self.size(), _ptr(v), _ptr(self.data),
+        p = _dev(v, self.size(), 8, "Weight.set_log values", True)
+        check(lib.mckl_hip_weight_set_log(self.size(), p, _ptr(self.data),
                                           _ptr(self._stats), _stream()))
 
     def add_log(self, v):
-        check(lib.mckl_hip_weight_add_log(self.size(), _ptr(v), _ptr(self.data),
+        p = _dev(v, self.size(), 8, "Weight.add_log values", True)
+        check(lib.mckl_hip_weight_add_log(self.size(), p, _ptr(self.data),
                                           _ptr(self._stats), _stream()))
 
     def stats(self):
@@ -340,20 +369,24 @@ class Weight:
 
 
 def u01_rand_sequence(scheme, rng, n, r):
+    p = _dev(r, n, 8, "uniform sequence output", True)
     check(lib.mckl_hip_u01_rand_sequence(scheme, ctypes.byref(rng.state), n,
-                                         _ptr(r), _stream()))
+                                         p, _stream()))
     return r
 
 
 def resample_trans_u01_rep(N, M, w, u, rep):
-    check(lib.mckl_hip_resample_trans_u01_rep(N, M, _ptr(w), _ptr(u),
-                                              _ptr(rep), _stream()))
+    pw = _dev(w, N, 8, "weights", True)
+    pu = _dev(u, M, 8, "uniforms", True)
+    pr = _dev(rep, N, 4, "replication numbers", False)
+    check(lib.mckl_hip_resample_trans_u01_rep(N, M, pw, pu, pr, _stream()))
     return rep
 
 
 def resample_trans_rep_index(N, M, rep, idx):
-    check(lib.mckl_hip_resample_trans_rep_index(N, M, _ptr(rep), _ptr(idx),
-                                                _stream()))
+    pr = _dev(rep, N, 4, "replication numbers", False)
+    pi = _dev(idx, M, 4, "parent indices", False)
+    check(lib.mckl_hip_resample_trans_rep_index(N, M, pr, pi, _stream()))
     return idx
 
 
@@ -364,10 +397,12 @@ class _ResampleAlgorithm:
     SCHEME = None
 
     def __call__(self, N, M, rng, w, rep, idx=None):
+        pw = _dev(w, N, 8, "weights", True)
+        pr = _dev(rep, N, 4, "replication numbers", False)
+        pi = (_dev(idx, M, 4, "parent indices", False) if idx is not None
+              else None)
         check(lib.mckl_hip_resample(self.SCHEME, N, M, ctypes.byref(rng.state),
-                                    _ptr(w), _ptr(rep),
-                                    _ptr(idx) if idx is not None else None,
-                                    _stream()))
+                                    pw, pr, pi, _stream()))
         return rep
 
 
@@ -404,21 +439,25 @@ def estimate(u, w, dim, layout=None, out=None):
     import torch
     layout = _lib.ROW_MAJOR if layout is None else layout
     n = w.numel()
-    if u.numel() < n * dim:
-        raise MCKLError("estimate: u holds fewer than n * dim values")
+    pw = _dev(w, n, 8, "estimate weights", True)
+    pu = _dev(u, n * dim, u.element_size() if u.element_size() in (4, 8)
+              else 8, "estimate values", True)
     if out is None:
         out = torch.empty(max(dim, 1), dtype=torch.float64, device=w.device)
+    po = _dev(out, dim, 8, "estimate output", True)
     fn = (lib.mckl_hip_estimate_f64 if u.dtype == torch.float64
           else lib.mckl_hip_estimate_f32)
-    check(fn(n, dim, layout, _ptr(u), _ptr(w), _ptr(out), _stream()))
+    check(fn(n, dim, layout, pu, pw, po, _stream()))
     return out[:dim]
 
 
 def select(idx, src, dst, dim=1):
     """StateMatrix<RowMajor, dim, float>::select, out of place."""
     M = idx.numel()
-    check(lib.mckl_hip_select_f32(M, dim, _ptr(idx), _ptr(src), _ptr(dst),
-                                  _stream()))
+    pi = _dev(idx, M, 4, "select indices", False)
+    ps = _dev(src, 0, 4, "select source", True)
+    pd = _dev(dst, M * dim, 4, "select destination", True)
+    check(lib.mckl_hip_select_f32(M, dim, pi, ps, pd, _stream()))
     return dst
 
 
@@ -465,10 +504,9 @@ class ResampleIndex:
     def push_back(self, N, idx=None):
         """push_back(N) (identity) or push_back(N, idx) with idx a device
         tensor of at least N 32-bit entries."""
-        if idx is not None and idx.numel() < N:
-            raise MCKLError("push_back: index vector shorter than N")
-        check(lib.mckl_hip_resample_index_push_back(
-            self._h, N, _ptr(idx) if idx is not None else None, _stream()))
+        p = (_dev(idx, N, 4, "push_back indices", False) if idx is not None
+             else None)
+        check(lib.mckl_hip_resample_index_push_back(self._h, N, p, _stream()))
 
     def _back(self, iter_back):
         return (self.num_iter() - 1) % (2 ** 64) if iter_back is None \
@@ -482,9 +520,10 @@ class ResampleIndex:
         scalar = not torch.is_tensor(id)
         ids = torch.tensor([id], dtype=torch.int32, device="cuda") \
             if scalar else id
+        pid = _dev(ids, ids.numel(), 4, "index ids", False)
         out = torch.empty(ids.numel(), dtype=torch.int32, device=ids.device)
         check(lib.mckl_hip_resample_index_index(
-            self._h, ids.numel(), _ptr(ids), ib, iter, _ptr(out), _stream()))
+            self._h, ids.numel(), pid, ib, iter, _ptr(out), _stream()))
         if scalar:
             return int(out.cpu().numpy().view(np.uint32)[0])
         return out
@@ -504,11 +543,9 @@ class ResampleIndex:
     def read_index_matrix(self, layout, out, iter_back=None, iter=0):
         ib = self._back(iter_back)
         nr, nc = self._shape(ib, iter)
-        if out.numel() < nr * nc:
-            raise MCKLError("read_index_matrix: output smaller than %d x %d"
-                            % (nr, nc))
+        p = _dev(out, nr * nc, 4, "read_index_matrix output", False)
         check(lib.mckl_hip_resample_index_read_matrix(
-            self._h, layout, ib, iter, _ptr(out), _stream()))
+            self._h, layout, ib, iter, p, _stream()))
         return out
 
     def index_matrix(self, layout, iter_back=None, iter=0):
@@ -768,17 +805,19 @@ class RNGSetVector:
         return self.engine_cls(key=k[:self.engine_cls.KEY_WORDS])
 
     def rand(self, per, out):
+        p = _dev(out, self.n * per, 4, "RNG set output", False)
         check(lib.mckl_hip_rngset_rand(self.engine_cls.KIND, self.n,
-                                       _ptr(self.keys), per, _ptr(out),
-                                       _stream()))
+                                       _ptr(self.keys), per, p, _stream()))
         return out
 
     def u01(self, per, out, interval=_lib.U01_OO):
         import torch
         fn = (lib.mckl_hip_rngset_u01_f64 if out.dtype == torch.float64
               else lib.mckl_hip_rngset_u01_f32)
+        p = _dev(out, self.n * per, 8 if out.dtype == torch.float64 else 4,
+                 "RNG set output", True)
         check(fn(self.engine_cls.KIND, interval, self.n, _ptr(self.keys),
-                 per, _ptr(out), _stream()))
+                 per, p, _stream()))
         return out
 
 

@@ -394,8 +394,8 @@ __global__ __launch_bounds__(kB) void k_counts(uint64_t N, uint64_t M,
     const double *w, const double *u, const double *th, const double *tl,
     uint32_t nt, uint32_t *rep, uint64_t K, uint32_t *te, uint32_t *tz)
 {
-    __shared__ uint32_t last[kB];
     __shared__ uint32_t su[8];
+    __shared__ uint32_t bnd[2];
     double P[kItems];
     double off_hi;
     tile_prefix(0, N, N, w, th, tl, nt, P, off_hi);
@@ -434,21 +434,45 @@ __global__ __launch_bounds__(kB) void k_counts(uint64_t N, uint64_t M,
             c[k] = (uint32_t)cc;
         }
     }
-    last[threadIdx.x] = c[kItems - 1];
-    uint32_t prev = 0;
-    if (threadIdx.x == 0 && blockIdx.x > 0) {
+    // the counts at the tile's two boundaries (the previous tile's last
+    // count and this tile's last), then the counts made non-decreasing
+    // inside them, so no replication number is negative whatever the
+    // rounding of the prefixes (smc_index.hpp)
+    const uint64_t tend = (uint64_t)(blockIdx.x + 1) * kTile < N
+        ? (uint64_t)(blockIdx.x + 1) * kTile
+        : N;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        if (i0 + k == tend - 1)
+            bnd[1] = c[k];
+    if (threadIdx.x == 0) {
         const double S = off_hi;
-        prev = (uint32_t)count_from(u, M, S, clamp_floor(__dmul_rn(S, Md), M));
+        bnd[0] = blockIdx.x == 0 ? 0u
+            : (uint32_t)count_from(u, M, S, clamp_floor(__dmul_rn(S, Md), M));
     }
     __syncthreads();
-    if (threadIdx.x > 0)
-        prev = last[threadIdx.x - 1];
+    const uint32_t clo = bnd[0], chi = bnd[1];
+    uint32_t cmax = clo;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (i0 + k < N) {
+            const uint32_t v = clamp_count(c[k], clo, chi);
+            cmax = cmax > v ? cmax : v;
+            c[k] = cmax;
+        }
+    }
+    const uint32_t before = group_excl_max_u(cmax, clo, threadIdx.x, su);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        if (i0 + k < N && c[k] < before)
+            c[k] = before;
+    uint32_t prev = before;
     uint32_t r[kItems];
     uint32_t es = 0, zs = 0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        r[k] = c[k] - prev;
+        r[k] = rep_from_counts(c[k], prev);
         prev = c[k];
         if (i < N) {
             es += extras(r[k], i, K);

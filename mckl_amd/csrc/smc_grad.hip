@@ -632,10 +632,30 @@ int mckl_hip_smc_grad_create(int model, size_t nbatch, size_t N, size_t T,
         &ncu, hipDeviceAttributeMultiprocessorCount, dev));
     size_t freeb = 0, totalb = 0;
     MCKL_HIP_TRY(hipMemGetInfo(&freeb, &totalb));
-    const size_t per_slot = (nck + K + 1) * g->slot_bytes + Np * 8 * 4 + Np * 4;
+    // per resident slot: checkpoints, segment, four adjoint vectors, the
+    // counts and, for multinomial, the sorted uniforms
+    const size_t per_slot = (nck + K + 1) * g->slot_bytes + Np * 8 * 4 +
+        Np * 4 + (scheme == MCKL_HIP_MULTINOMIAL ? Np * 8 : 0);
     size_t nslots = std::min<size_t>(nbatch, (size_t)ncu);
     while (nslots > 1 && nslots * per_slot > freeb / 2)
         nslots /= 2;
+    if (nslots * per_slot > freeb) {
+        grad_free(g);
+        return set_error(MCKL_HIP_ENOMEM,
+            "smc_grad: one slot needs %zu bytes, %zu free", per_slot, freeb);
+    }
+    if (tape) {
+        // (T + 1) N entries per point of 8 (log weights) + 4 (parents) bytes
+        const uint64_t per_point = ((uint64_t)T + 1) * (uint64_t)N;
+        if (per_point > UINT64_MAX / 12 / nbatch ||
+            per_point * 12 * nbatch > (uint64_t)freeb / 2) {
+            grad_free(g);
+            return set_error(MCKL_HIP_EINVAL,
+                "smc_grad: the ancestor tape (%zu points x %llu entries x 12 "
+                "bytes) does not fit in half of the %zu free bytes", nbatch,
+                (unsigned long long)per_point, freeb);
+        }
+    }
     g->nslots = (int)nslots;
     int rc = MCKL_HIP_OK;
 #define MCKL_ALLOC(p, n)                                                      \

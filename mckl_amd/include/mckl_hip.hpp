@@ -993,9 +993,25 @@ class Particle
 
     /// Particle::select (core/particle.hpp:324-331): row j of the new state
     /// is row d_idx[j] of the old one (j < n; d_idx a device array of row
-    /// numbers below size()), then equal weights of size n
-    void select(size_type n, const std::uint32_t *d_idx)
+    /// numbers below size()), then equal weights of size n.  With validate
+    /// (the default) the row numbers are read back and checked first, and a
+    /// number not below size() throws RuntimeAssert before the gather;
+    /// ResampleEval, whose indices are parents by construction, skips it.
+    void select(size_type n, const std::uint32_t *d_idx, bool validate = true)
     {
+        if (validate && n > 0) {
+            std::vector<std::uint32_t> h(n);
+            const auto st = static_cast<hipStream_t>(current_stream());
+            if (hipMemcpyAsync(h.data(), d_idx, n * sizeof(std::uint32_t),
+                    hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                throw RuntimeAssert(
+                    "Particle::select: reading the row numbers failed");
+            for (std::uint32_t v : h)
+                if (v >= n_)
+                    throw RuntimeAssert(
+                        "Particle::select: row numbers must lie below size()");
+        }
         if (n == n_) {
             select_rows(n, dim_, d_idx, x_.data(), spare_.data());
             std::swap(x_, spare_);
@@ -1052,7 +1068,7 @@ class ResampleEval
         hip_check(mckl_hip_resample(scheme_, N, N, &particle.rng().state(),
             particle.weight().data(), buf_->rep.data(), buf_->idx.data(),
             current_stream()));
-        particle.select(N, buf_->idx.data());
+        particle.select(N, buf_->idx.data(), false);
     }
 
   private:

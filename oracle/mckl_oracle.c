@@ -1032,6 +1032,37 @@ void mo_resample_trans_u01_rep(
     rep[N - 1] = M - j;
 }
 
+/* The same merge with the cumulative weights correctly rounded instead of
+ * summed sequentially: S_i = the exact sum of w_0..w_i rounded once (a quad
+ * accumulator, 113 bits, rounded to double at each step).  This is the
+ * device's definition (a double-double prefix rounded once); it differs
+ * from the reference's sequential sums only at uniforms within their
+ * rounding error of a boundary. */
+void mo_resample_trans_u01_rep_cr(
+    size_t N, size_t M, const double *w, const double *u01, uint64_t *rep)
+{
+    if (N == 0)
+        return;
+    if (N == 1) {
+        rep[0] = M;
+        return;
+    }
+    memset(rep, 0, N * sizeof(uint64_t));
+    if (M == 0)
+        return;
+    __float128 acc = 0;
+    size_t j = 0;
+    for (size_t i = 0; i != N - 1; ++i) {
+        acc += (__float128)w[i];
+        const double S = (double)acc;
+        while (j != M && u01[j] < S) {
+            rep[i] += 1;
+            ++j;
+        }
+    }
+    rep[N - 1] = M - j;
+}
+
 void mo_resample_trans_rep_index(
     size_t N, size_t M, const uint64_t *rep, uint64_t *idx)
 {

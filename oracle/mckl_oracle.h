@@ -133,6 +133,10 @@ void mo_u01_rand_stratified(mo_engine *e, size_t n, double *r);
 void mo_u01_rand_systematic(mo_engine *e, size_t n, double *r);
 void mo_resample_trans_u01_rep(size_t n, size_t m, const double *w,
     const double *u01, uint64_t *rep);
+/* cumulative weights correctly rounded (quad accumulator) instead of summed
+ * sequentially: the device's definition */
+void mo_resample_trans_u01_rep_cr(
+    size_t N, size_t M, const double *w, const double *u01, uint64_t *rep);
 void mo_resample_trans_rep_index(
     size_t n, size_t m, const uint64_t *rep, uint64_t *idx);
 size_t mo_resample_trans_residual(size_t N, size_t M, const double *w,

@@ -217,13 +217,18 @@ def u01_sequence(scheme, e, n):
     return out[:n]
 
 
-def trans_u01_rep(w, u, N=None, M=None):
+def trans_u01_rep(w, u, N=None, M=None, exact_prefix=False):
+    """resample_trans_u01_rep: sequential FP64 cumulative weights (the
+    reference's), or with exact_prefix the exact prefix sums rounded once
+    (the device's definition)."""
     w = np.ascontiguousarray(w, np.float64)
     u = np.ascontiguousarray(u, np.float64)
     N = len(w) if N is None else N
     M = len(u) if M is None else M
     rep = np.zeros(max(N, 1), np.uint64)
-    O.mo_resample_trans_u01_rep(_sz(N), _sz(M), P(w), P(u), P(rep))
+    fn = (O.mo_resample_trans_u01_rep_cr if exact_prefix
+          else O.mo_resample_trans_u01_rep)
+    fn(_sz(N), _sz(M), P(w), P(u), P(rep))
     return rep[:N]
 
 

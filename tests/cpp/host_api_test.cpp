@@ -145,6 +145,26 @@ int main(int argc, char **argv)
         }
         CHECK(thrown, "invalid stddev raises RuntimeAssert");
     }
+    // 5b. Particle::select checks the row numbers before the gather
+    {
+        mckl::Particle<double> p(8);
+        std::vector<std::uint32_t> h = {0, 1, 2, 3, 4, 5, 6, 8};
+        Dev<std::uint32_t> idx(8);
+        (void)hipMemcpy(idx.p, h.data(), 8 * sizeof(std::uint32_t),
+            hipMemcpyHostToDevice);
+        bool thrown = false;
+        try {
+            p.select(8, idx.p);
+        } catch (const mckl::RuntimeAssert &) {
+            thrown = true;
+        }
+        h[7] = 7;
+        (void)hipMemcpy(idx.p, h.data(), 8 * sizeof(std::uint32_t),
+            hipMemcpyHostToDevice);
+        p.select(8, idx.p);
+        CHECK(thrown && p.size() == 8,
+            "Particle::select rejects a row number >= size()");
+    }
     // 6. plain sampler on the linear-Gaussian model
     {
         std::vector<double> y(30);

@@ -124,3 +124,46 @@ def test_resample_index_bookkeeping_host_only():
     m.check(L.mckl_hip_resample_index_clear(h))
     m.check(L.mckl_hip_resample_index_push_back(h, 0, None, None))
     m.check(L.mckl_hip_resample_index_destroy(h))
+
+
+def test_resample_rejects_null_weights():
+    """N > 1 with M > 0 reads the weights: a NULL weight pointer is
+    MCKL_HIP_EINVAL before any launch (N == 1 and M == 0 never read them)."""
+    import mckl_amd as m
+    e = m._lib.Engine()
+    k = (ctypes.c_uint64 * 4)(1, 0, 0, 0)
+    m.check(m.lib.mckl_hip_engine_init(ctypes.byref(e), 1, 32, k))
+    host = (ctypes.c_uint32 * 8)()  # never reached by a kernel
+    for fn in (m.lib.mckl_hip_resample, m.lib.mckl_hip_resample_noalloc):
+        for scheme in range(6):
+            rc = fn(scheme, 5, 5, ctypes.byref(e), None, host, None, None)
+            assert rc == m._lib.EINVAL
+            assert b"NULL weights" in m.lib.mckl_hip_last_error()
+    assert list(e.ctr[:2]) == [0, 0]
+
+
+def test_python_wrappers_reject_host_tensors():
+    """Every wrapper checks its tensors before a launch: host memory,
+    non-tensors and short tensors raise MCKLError."""
+    import torch
+    import mckl_amd as m
+    rng = m.Philox4x32()
+    cpu = torch.zeros(16, dtype=torch.int32)
+    with pytest.raises(m.MCKLError, match="not in GPU memory"):
+        m.rand(rng, 16, cpu)
+    with pytest.raises(m.MCKLError, match="expected a torch tensor"):
+        m.rand(rng, 16, [0] * 16)
+    with pytest.raises(m.MCKLError, match="not in GPU memory"):
+        m.rand(rng, m.NormalDistribution(0.0, 1.0, "float32"), 4,
+               torch.zeros(4, dtype=torch.float32))
+    with pytest.raises(m.MCKLError, match="not in GPU memory"):
+        m.rand(rng, m.U01OODistribution("float64"), 4,
+               torch.zeros(4, dtype=torch.float64))
+    with pytest.raises(m.MCKLError, match="not in GPU memory"):
+        m.ResampleSystematic()(4, 4, rng, torch.full((4,), 0.25,
+                                                     dtype=torch.float64),
+                               torch.zeros(4, dtype=torch.int32))
+    with pytest.raises(m.MCKLError, match="not in GPU memory"):
+        m.resample_trans_rep_index(4, 4, torch.ones(4, dtype=torch.int32),
+                                   torch.zeros(4, dtype=torch.int32))
+    assert rng.ctr() == [0, 0] and rng.index == 4

@@ -1,12 +1,17 @@
 """GPU Weight and resampling primitives against the compiled-reference
 fixtures and the oracle.
 
-Replication counts come from FP64 prefix sums.  The device sums in
-double-double (correctly rounded), the reference sums sequentially in FP64;
-a count can therefore differ only where a uniform lies within the
-reference's own rounding error of a cumulative-weight boundary.  Each case
-reports the number of such differences and asserts every one is explained
-that way; the rep->index transform must then be bit-exact given rep.
+Replication counts come from FP64 prefix sums.  The device's cumulative
+weights are the exact prefix sums rounded once (a double-double prefix),
+the reference's are sequential FP64 sums.  Parity is therefore checked two
+ways:
+  * bit-exact: rep against the oracle's merge on the exactly rounded prefix
+    (the device's definition) of the same weights and the device's own
+    uniforms, and idx against the oracle's resample_trans_rep_index of it;
+  * against the reference's own rep (fixtures, or the oracle's sequential
+    form): every difference must be a uniform within the sequential sum's
+    error (N eps) of an exact cumulative weight, and the cases within 1e-15
+    relative and beyond it are counted and reported.
 """
 import math
 import os
@@ -31,26 +36,41 @@ def _t(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
 
-def boundary_explained(w, u, rep_ref, rep_gpu):
-    """Every replication difference is a uniform within the sequential
-    sum's error bound (N * eps * S) of an exact cumulative weight."""
+def boundary_report(w, us, rep_ref, rep_gpu, tol=None):
+    """Differences between the reference's rep and the device's: each must
+    have a uniform (of any sequence in `us`) within `tol` (default N eps)
+    relative of an exact cumulative weight at that particle.  Returns
+    (differences, of them within 1e-15 relative, largest relative
+    distance)."""
     diff = np.nonzero(rep_ref.astype(np.int64) != rep_gpu.astype(np.int64))[0]
     if len(diff) == 0:
-        return 0, 0
+        return 0, 0, 0.0
     exact = np.cumsum(np.asarray(w, np.longdouble))
     N = len(w)
-    tol = N * np.finfo(np.float64).eps
-    near = 0
-    within_1e15 = 0
+    tol = N * np.finfo(np.float64).eps if tol is None else tol
+    within = 0
+    worst = 0.0
     for i in diff:
-        S = float(exact[min(i, N - 1)])
-        Sp = float(exact[i - 1]) if i > 0 else 0.0
-        d = min(np.min(np.abs(u - S)) / S if S else 1.0,
-                np.min(np.abs(u - Sp)) / Sp if Sp else 1.0)
+        d = 1.0
+        for u in us:
+            S = float(exact[min(i, N - 1)])
+            Sp = float(exact[i - 1]) if i > 0 else 0.0
+            d = min(d, np.min(np.abs(u - S)) / S if S else 1.0,
+                    np.min(np.abs(u - Sp)) / Sp if Sp else 1.0)
         assert d <= tol, (i, d)
-        near += 1
-        within_1e15 += d <= 1e-15
-    return near, within_1e15
+        within += d <= 1e-15
+        worst = max(worst, d)
+    return len(diff), within, worst
+
+
+def strict_rep_idx(w, u, M, rep_gpu, idx_gpu):
+    """rep bit-exact against the exactly rounded prefix merge, idx against
+    the stable rep -> index of it."""
+    want = orc.trans_u01_rep(w, u, len(w), M, exact_prefix=True)
+    assert np.array_equal(rep_gpu.astype(np.uint64), want)
+    if idx_gpu is not None:
+        assert np.array_equal(idx_gpu.astype(np.uint64),
+                              orc.rep_index(want, M))
 
 
 def test_resample_fixtures(dev):
@@ -58,7 +78,7 @@ def test_resample_fixtures(dev):
     m = _m()
     fx = np.load(os.path.join(GOLDEN, "resample_cases.npz"))
     k = 0
-    total_near = 0
+    tot = np.zeros(3, np.int64)
     while "c%d_meta" % k in fx:
         p = "c%d" % k
         scheme, N, M, k0 = [int(v) for v in fx[p + "_meta"]]
@@ -81,25 +101,23 @@ def test_resample_fixtures(dev):
             assert np.allclose(ug, fx[p + "_u"], rtol=1e-12, atol=1e-15)
         else:
             assert np.array_equal(ug, fx[p + "_u"])
-        rg = rep.cpu().numpy().astype(np.uint64)
+        rg = rep.cpu().numpy().view(np.uint32).astype(np.uint64)
+        ig = idx.cpu().numpy().view(np.uint32).astype(np.uint64)
         if scheme >= 3:
             # residual family: integral parts exact, R uniforms over the
             # normalised residuals
             assert np.array_equal(rg, fx[p + "_rep"]), p
-            assert np.array_equal(idx.cpu().numpy().astype(np.uint64),
-                                  fx[p + "_idx"]), p
+            assert np.array_equal(ig, fx[p + "_idx"]), p
             k += 1
             continue
-        near, _ = boundary_explained(fx[p + "_w"], fx[p + "_u"], fx[p + "_rep"],
-                                     rg)
-        total_near += near
-        if near == 0:
-            assert np.array_equal(rg, fx[p + "_rep"])
-            assert np.array_equal(idx.cpu().numpy().astype(np.uint64),
-                                  fx[p + "_idx"])
+        strict_rep_idx(fx[p + "_w"], ug, M, rg, ig)
+        tol = None if scheme else N * np.finfo(np.float64).eps + 1e-12
+        tot += boundary_report(fx[p + "_w"], (fx[p + "_u"], ug),
+                               fx[p + "_rep"], rg, tol)[:2] + (0,)
         k += 1
-    print("replication differences at cumulative-weight boundaries:",
-          total_near)
+    print("replication differences from the reference (all at cumulative-"
+          "weight boundaries): %d, of them within 1e-15 relative: %d" %
+          (tot[0], tot[1]))
 
 
 @pytest.mark.parametrize("N,M", [(1, 1), (1, 5), (5, 1), (2, 2), (1000, 1000),
@@ -128,11 +146,10 @@ def test_trans_u01_rep_vs_oracle(dev):
         rep = torch.zeros(N, dtype=torch.int32, device=dev)
         m.resample_trans_u01_rep(N, M, _t(w, dev), _t(u, dev), rep)
         rg = rep.cpu().numpy().astype(np.uint64)
+        strict_rep_idx(w, u, M, rg, None)
         ro = orc.trans_u01_rep(w, u)
-        near, _ = boundary_explained(w, u, ro, rg)
+        boundary_report(w, (u,), ro, rg)
         assert int(rg.sum()) == M
-        if near == 0:
-            assert np.array_equal(rg, ro)
 
 
 def test_weight_fixtures(dev):
@@ -208,11 +225,14 @@ def test_resample_large_vs_oracle(dev, scheme, N, M):
         return
     e2 = orc.engine(orc.PHILOX4X32, key)
     u = orc.u01_sequence(scheme, e2, M)
-    near, _ = boundary_explained(w, u, orep, rg)
-    print("scheme %d N=%d M=%d boundary differences %d" % (scheme, N, M, near))
-    if near == 0:
-        assert np.array_equal(rg, orep)
-        assert np.array_equal(ig, oidx)
+    ud = torch.zeros(M, dtype=torch.float64, device=dev)
+    m.u01_rand_sequence(scheme, m.Philox4x32(key=key), M, ud)
+    ud = ud.cpu().numpy()
+    strict_rep_idx(w, ud, M, rg, ig)
+    tol = None if scheme else N * np.finfo(np.float64).eps + 1e-12
+    nd, n15, worst = boundary_report(w, (u, ud), orep, rg, tol)
+    print("scheme %d N=%d M=%d differences from the reference %d (within "
+          "1e-15: %d, largest distance %.3g)" % (scheme, N, M, nd, n15, worst))
 
 
 @pytest.mark.parametrize("N", [(1 << 20) + 3, 3 * (1 << 19) + 1])

@@ -405,6 +405,35 @@ def test_config2_streams(dev, ename, kind, dname):
     assert 5.0 < zmax < 7.5
 
 
+def test_wrappers_check_tensors(dev):
+    """Element size, element type, contiguity and length are checked before
+    a launch, and the engine does not advance."""
+    import torch
+    m = _m()
+    rng = m.Philox4x32()
+    f32 = torch.empty(8, dtype=torch.float32, device=dev)
+    with pytest.raises(m.MCKLError, match="8"):
+        m.rand(rng, m.NormalDistribution(0.0, 1.0, "float64"), 8, f32)
+    with pytest.raises(m.MCKLError, match="integer"):
+        m.rand(rng, 8, f32)
+    with pytest.raises(m.MCKLError, match="8-byte|4-byte"):
+        m.rand(m.Philox4x32_64(), 4, torch.empty(8, dtype=torch.int32,
+                                                  device=dev))
+    with pytest.raises(m.MCKLError, match="contiguous"):
+        m.rand(rng, 4, torch.empty((4, 2), dtype=torch.int32,
+                                   device=dev)[:, 0])
+    with pytest.raises(m.MCKLError, match="needs 9"):
+        m.rand(rng, m.U01OODistribution("float32"), 9, f32)
+    w = torch.full((8,), 0.125, dtype=torch.float64, device=dev)
+    with pytest.raises(m.MCKLError, match="expected 4"):
+        m.ResampleSystematic()(8, 8, rng, w,
+                               torch.empty(8, dtype=torch.int64, device=dev))
+    with pytest.raises(m.MCKLError, match="expected 8"):
+        m.ResampleSystematic()(8, 8, rng, w.float(),
+                               torch.empty(8, dtype=torch.int32, device=dev))
+    assert rng.ctr() == [0, 0] and rng.index == 4
+
+
 def test_engine_errors():
     m = _m()
     e = m._lib.Engine()

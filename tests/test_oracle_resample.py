@@ -120,3 +120,41 @@ def test_weight_closed_form():
     W, ess = orc.weight_set_log(np.log(p))
     assert np.allclose(W, p, rtol=1e-15)
     assert abs(ess - 1.0 / np.sum(p * p)) < 1e-12
+
+
+def test_exact_prefix_merge_against_rational_arithmetic():
+    """The oracle's exactly-rounded-prefix merge (the device's definition of
+    the cumulative weights): S_i = the exact rational sum of w_0..w_i
+    rounded once to double, rep_i = #{j : S_{i-1} <= u_j < S_i}, last
+    rep = M - j (resample.hpp:374-407).  Weights spanning 2^-60..1 make the
+    sequential FP64 sums differ from the exact ones, and uniforms placed on
+    and next to the exact boundaries exercise every comparison."""
+    from fractions import Fraction
+    rng = np.random.default_rng(5)
+    differs = 0
+    for N, M in ((2, 3), (17, 40), (200, 150), (300, 300)):
+        w = rng.random(N) * np.exp2(-rng.integers(0, 60, N))
+        w /= w.sum()
+        acc = Fraction(0)
+        S = []
+        for x in w[:-1]:
+            acc += Fraction(float(x))
+            S.append(float(acc))
+        pool = np.concatenate([rng.random(M), np.array(S),
+                               np.nextafter(np.array(S), 0.0)])
+        u = np.sort(rng.choice(pool, M, replace=False))
+        want = np.zeros(N, np.uint64)
+        j = 0
+        for i, s in enumerate(S):
+            while j < M and u[j] < s:
+                want[i] += 1
+                j += 1
+        want[N - 1] = M - j
+        got = orc.trans_u01_rep(w, u, exact_prefix=True)
+        assert np.array_equal(got, want), (N, M)
+        seq = orc.trans_u01_rep(w, u)
+        assert int(seq.sum()) == M
+        differs += int(not np.array_equal(seq, want))
+    # the sequential sums do round differently here, so a merge on them
+    # would fail the check above
+    assert differs > 0
