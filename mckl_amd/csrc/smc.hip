@@ -877,6 +877,11 @@ struct mckl_hip_smc {
     double *eph = nullptr, *epl = nullptr, *est = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // recorded on the caller's stream after each run: result calls wait for
+    // it, not for the whole device, and read through a non-blocking stream
+    // of their own (the null stream would wait for every blocking stream)
+    hipEvent_t done = nullptr;
+    hipStream_t rs = nullptr;
 };
 
 static void smc_drop_graph(mckl_hip_smc *s)
@@ -892,6 +897,10 @@ static void smc_drop_graph(mckl_hip_smc *s)
 static void smc_free(mckl_hip_smc *s)
 {
     smc_drop_graph(s);
+    if (s->done)
+        (void)hipEventDestroy(s->done);
+    if (s->rs)
+        (void)hipStreamDestroy(s->rs);
     void *ptrs[] = {s->x[0], s->x[1], s->lw, s->obs, s->tmax, s->idx, s->ts,
         s->ts2, s->B.tdh, s->B.tdl, s->B.te, s->B.tz, s->B.cnt, s->B.Eincl,
         s->B.zpos, s->B.tmh, s->B.tml, s->B.u, s->B.uh, s->B.ul, s->B.ti,
@@ -976,6 +985,10 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
         hipMemcpyHostToDevice);
     if (err == hipSuccess)
         err = hipMemset(s->st, 0, sizeof(SmcState));
+    if (err == hipSuccess)
+        err = hipEventCreateWithFlags(&s->done, hipEventDisableTiming);
+    if (err == hipSuccess)
+        err = hipStreamCreateWithFlags(&s->rs, hipStreamNonBlocking);
     if (err != hipSuccess) {
         rc = hip_status(err, "mckl_hip_smc_create");
         smc_free(s);
@@ -1033,7 +1046,7 @@ int mckl_hip_smc_estimate_moments(mckl_hip_smc *s, int k)
     if (k < 0 || k > kMaxMoments)
         return set_error(MCKL_HIP_EINVAL,
             "moments: k = %d outside 0..%d", k, kMaxMoments);
-    MCKL_HIP_TRY(hipDeviceSynchronize());
+    MCKL_HIP_TRY(hipEventSynchronize(s->done));
     smc_drop_graph(s);
     for (double **p : {&s->eph, &s->epl, &s->est}) {
         if (*p)
@@ -1062,9 +1075,9 @@ int mckl_hip_smc_estimates(mckl_hip_smc *s, double *h_est)
     if (!s->kmom)
         return set_error(MCKL_HIP_EINVAL,
             "no estimator: call mckl_hip_smc_estimate_moments first");
-    MCKL_HIP_TRY(hipDeviceSynchronize());
-    MCKL_HIP_TRY(hipMemcpy(h_est, s->est, s->T * s->kmom * sizeof(double),
-        hipMemcpyDeviceToHost));
+    MCKL_HIP_TRY(hipEventSynchronize(s->done));
+    MCKL_HIP_TRY(hipMemcpyWithStream(h_est, s->est,
+        s->T * s->kmom * sizeof(double), hipMemcpyDeviceToHost, s->rs));
     return MCKL_HIP_OK;
 }
 
@@ -1095,6 +1108,7 @@ int mckl_hip_smc_run(mckl_hip_smc *s, void *stream)
         MCKL_HIP_TRY(err);
     }
     MCKL_HIP_TRY(hipGraphLaunch(s->exec, st));
+    MCKL_HIP_TRY(hipEventRecord(s->done, st));
     return MCKL_HIP_OK;
 }
 
@@ -1103,18 +1117,19 @@ int mckl_hip_smc_result(mckl_hip_smc *s, double *h_loglik, double *h_ess,
 {
     if (!s)
         return set_error(MCKL_HIP_EINVAL, "sampler is NULL");
-    MCKL_HIP_TRY(hipDeviceSynchronize());
+    MCKL_HIP_TRY(hipEventSynchronize(s->done));
     if (h_loglik) {
         SmcState hs;
-        MCKL_HIP_TRY(hipMemcpy(&hs, s->st, sizeof(hs), hipMemcpyDeviceToHost));
+        MCKL_HIP_TRY(hipMemcpyWithStream(&hs, s->st, sizeof(hs),
+            hipMemcpyDeviceToHost, s->rs));
         *h_loglik = hs.logz;
     }
     if (h_ess)
-        MCKL_HIP_TRY(hipMemcpy(h_ess, s->ess, s->T * sizeof(double),
-            hipMemcpyDeviceToHost));
+        MCKL_HIP_TRY(hipMemcpyWithStream(h_ess, s->ess, s->T * sizeof(double),
+            hipMemcpyDeviceToHost, s->rs));
     if (h_resampled)
-        MCKL_HIP_TRY(hipMemcpy(h_resampled, s->res, s->T * sizeof(int32_t),
-            hipMemcpyDeviceToHost));
+        MCKL_HIP_TRY(hipMemcpyWithStream(h_resampled, s->res,
+            s->T * sizeof(int32_t), hipMemcpyDeviceToHost, s->rs));
     return MCKL_HIP_OK;
 }
 
@@ -1146,6 +1161,10 @@ struct mckl_hip_smc_batch {
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
     int graph_tune = -1;
+    // recorded on the caller's stream after each run, and the stream the
+    // results are read through (see mckl_hip_smc)
+    hipEvent_t done = nullptr;
+    hipStream_t rs = nullptr;
     // host copies for set_params: descriptors, model, observations
     int model = 0;
     std::vector<FilterDesc> fdh;
@@ -1154,6 +1173,10 @@ struct mckl_hip_smc_batch {
 
 static void batch_free(mckl_hip_smc_batch *b)
 {
+    if (b->done)
+        (void)hipEventDestroy(b->done);
+    if (b->rs)
+        (void)hipStreamDestroy(b->rs);
     if (b->exec)
         (void)hipGraphExecDestroy(b->exec);
     if (b->graph)
@@ -1316,6 +1339,12 @@ int mckl_hip_smc_batch_create(int model, size_t nfilters, const uint32_t *h_N,
         return rc;
     }
     rc = set_persistent(b);
+    if (rc == MCKL_HIP_OK)
+        rc = hip_status(hipEventCreateWithFlags(&b->done,
+            hipEventDisableTiming), "mckl_hip_smc_batch_create");
+    if (rc == MCKL_HIP_OK)
+        rc = hip_status(hipStreamCreateWithFlags(&b->rs,
+            hipStreamNonBlocking), "mckl_hip_smc_batch_create");
     if (rc) {
         batch_free(b);
         return rc;
@@ -1347,11 +1376,11 @@ int mckl_hip_smc_batch_set_params(mckl_hip_smc_batch *b,
             obs.data() + T * f);
     }
     // a run still in flight reads the descriptors: wait for it
-    MCKL_HIP_TRY(hipDeviceSynchronize());
-    MCKL_HIP_TRY(hipMemcpy(b->fd, b->fdh.data(), nf * sizeof(FilterDesc),
-        hipMemcpyHostToDevice));
-    MCKL_HIP_TRY(hipMemcpy(b->obs, obs.data(), obs.size() * sizeof(float),
-        hipMemcpyHostToDevice));
+    MCKL_HIP_TRY(hipEventSynchronize(b->done));
+    MCKL_HIP_TRY(hipMemcpyWithStream(b->fd, b->fdh.data(),
+        nf * sizeof(FilterDesc), hipMemcpyHostToDevice, b->rs));
+    MCKL_HIP_TRY(hipMemcpyWithStream(b->obs, obs.data(),
+        obs.size() * sizeof(float), hipMemcpyHostToDevice, b->rs));
     return MCKL_HIP_OK;
 }
 
@@ -1372,8 +1401,13 @@ int mckl_hip_smc_batch_run(mckl_hip_smc_batch *b, int use_graph, void *stream)
     if (!b)
         return set_error(MCKL_HIP_EINVAL, "batch is NULL");
     hipStream_t st = as_stream(stream);
-    if (!use_graph)
-        return batch_launch(b, st);
+    if (!use_graph) {
+        const int rc = batch_launch(b, st);
+        if (rc)
+            return rc;
+        MCKL_HIP_TRY(hipEventRecord(b->done, st));
+        return MCKL_HIP_OK;
+    }
     const int key = b->tune.block * 16 + b->tune.waves_per_eu;
     if (!b->exec || b->graph_tune != key) {
         if (b->exec) {
@@ -1401,6 +1435,7 @@ int mckl_hip_smc_batch_run(mckl_hip_smc_batch *b, int use_graph, void *stream)
         b->graph_tune = key;
     }
     MCKL_HIP_TRY(hipGraphLaunch(b->exec, st));
+    MCKL_HIP_TRY(hipEventRecord(b->done, st));
     return MCKL_HIP_OK;
 }
 
@@ -1408,9 +1443,9 @@ int mckl_hip_smc_batch_result(mckl_hip_smc_batch *b, double *h_loglik)
 {
     if (!b || !h_loglik)
         return set_error(MCKL_HIP_EINVAL, "NULL argument");
-    MCKL_HIP_TRY(hipDeviceSynchronize());
-    MCKL_HIP_TRY(hipMemcpy(h_loglik, b->logz, b->nfilters * sizeof(double),
-        hipMemcpyDeviceToHost));
+    MCKL_HIP_TRY(hipEventSynchronize(b->done));
+    MCKL_HIP_TRY(hipMemcpyWithStream(h_loglik, b->logz,
+        b->nfilters * sizeof(double), hipMemcpyDeviceToHost, b->rs));
     return MCKL_HIP_OK;
 }
 

@@ -709,12 +709,13 @@ int mckl_hip_smc_grad_run(mckl_hip_smc_grad *g, const double *h_theta,
     if (rc)
         return rc;
     g->have_tape = g->tape_lwn != nullptr;
+    // grad_launch synchronised st: read on it (not on the null stream)
     if (h_loglik)
-        MCKL_HIP_TRY(hipMemcpy(h_loglik, g->logz, g->nbatch * sizeof(double),
-            hipMemcpyDeviceToHost));
+        MCKL_HIP_TRY(hipMemcpyWithStream(h_loglik, g->logz,
+            g->nbatch * sizeof(double), hipMemcpyDeviceToHost, st));
     if (h_grad)
-        MCKL_HIP_TRY(hipMemcpy(h_grad, g->grad, 3 * g->nbatch * sizeof(double),
-            hipMemcpyDeviceToHost));
+        MCKL_HIP_TRY(hipMemcpyWithStream(h_grad, g->grad,
+            3 * g->nbatch * sizeof(double), hipMemcpyDeviceToHost, st));
     return MCKL_HIP_OK;
 }
 
@@ -729,8 +730,8 @@ int mckl_hip_smc_grad_replay(mckl_hip_smc_grad *g, const double *h_theta,
     int rc = grad_launch(g, h_theta, kGradReplay, as_stream(stream));
     if (rc)
         return rc;
-    MCKL_HIP_TRY(hipMemcpy(h_value, g->logz, g->nbatch * sizeof(double),
-        hipMemcpyDeviceToHost));
+    MCKL_HIP_TRY(hipMemcpyWithStream(h_value, g->logz,
+        g->nbatch * sizeof(double), hipMemcpyDeviceToHost, as_stream(stream)));
     return MCKL_HIP_OK;
 }
 

@@ -295,3 +295,30 @@ def test_batched_extreme_sizes(dev):
             lp = m.smc_loglik(2, theta[f], y[f], int(N[f]), int(seeds[f]), 0.5,
                               scheme)
             assert lb[f] == lp, (scheme, f, lb[f], lp)
+
+
+def test_results_wait_for_their_own_run(dev):
+    """A run enqueued on a side stream, read right away: the result call
+    waits for that run's completion event (not the whole device) and
+    returns the same bits as a run on the default stream; so does the
+    batched form."""
+    import torch
+    m = _m()
+    y = np.linspace(-1.0, 1.0, 40)
+    want = m.smc_loglik(m.MODEL_LG, (0.9, 1.0, 0.5), y, 50000, seed=3)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        s = m.SMCSampler(m.MODEL_LG, (0.9, 1.0, 0.5), y, 50000, seed=3)
+        s.iterate()
+        got = s.log_likelihood()
+    assert got == want
+    N = np.array([3000, 700], np.uint32)
+    th = np.array([[0.95, 0.2, 0.7], [0.9, 0.25, 0.6]])
+    yy = np.tile(y[:20], (2, 1))
+    b0 = m.SMCBatch(m.MODEL_SV, N, th, yy, [5, 6])
+    b0.run()
+    with torch.cuda.stream(side):
+        b1 = m.SMCBatch(m.MODEL_SV, N, th, yy, [5, 6])
+        b1.run()
+        r1 = b1.log_likelihood()
+    assert np.array_equal(r1, b0.log_likelihood())
