@@ -149,22 +149,26 @@ int mckl_hip_ars_kernel(
  * reference's bulk call advances it. */
 int mckl_hip_rand(mckl_hip_engine *e, size_t n, void *d_r, void *stream);
 
-/* rand(rng, U01xxDistribution<T>, n, r) for 32-bit engines:
- * random/u01_distribution.hpp:79-142, random/internal/u01_generic.hpp:48-198,
- * random/internal/common.hpp:69-86 (chunking by BufferSize<T>). */
+/* rand(rng, U01xxDistribution<T>, n, r): random/u01_distribution.hpp:79-142,
+ * random/internal/u01_generic.hpp:48-198, random/internal/common.hpp:69-86
+ * (chunking by BufferSize<T>).  32-bit engines take one u32 per value (two
+ * for Canonical<double>); 64-bit-result engines one u64 per value
+ * (U01UIntType, u01_distribution.hpp:105-113). */
 int mckl_hip_rand_u01_f32(
     mckl_hip_engine *e, int interval, size_t n, float *d_r, void *stream);
 int mckl_hip_rand_u01_f64(
     mckl_hip_engine *e, int interval, size_t n, double *d_r, void *stream);
 
-/* rand(rng, NormalDistribution<T>(mean, sd), n, r) for 32-bit engines:
- * random/normal_distribution.hpp:50-101 (chunked Box-Muller, odd tail). */
+/* rand(rng, NormalDistribution<T>(mean, sd), n, r):
+ * random/normal_distribution.hpp:50-101 (chunked Box-Muller, odd tail), on
+ * U01OC values of one result word each (u32 or u64).  log, sin and cos are
+ * the correctly rounded values. */
 int mckl_hip_rand_normal_f32(mckl_hip_engine *e, size_t n, float mean,
     float sd, float *d_r, void *stream);
 int mckl_hip_rand_normal_f64(mckl_hip_engine *e, size_t n, double mean,
     double sd, double *d_r, void *stream);
 
-/* rand(rng, UniformRealDistribution<T>(a, b), n, r) for 32-bit engines:
+/* rand(rng, UniformRealDistribution<T>(a, b), n, r), either result width:
  * random/uniform_real_distribution.hpp:48-58 (U01CO, then r*(b-a)+a rounded
  * twice, the reference's muladd without FMA, math/vmf.hpp:697-722); a < b
  * (:43-46) or MCKL_HIP_EINVAL. */

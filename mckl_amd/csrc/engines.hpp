@@ -356,4 +356,78 @@ __device__ __forceinline__ float u01_canonical_f32(uint32_t u)
     return __fadd_rn(__fmul_rn((float)u, 0x1p-32f), 0.0f);
 }
 
+// ------------------------------------------------------------------------
+// The same from one u64 word: 64-bit-result engines take one u64 per value
+// for every real type (U01UIntType, random/u01_distribution.hpp:105-113),
+// with W = 64 in u01_generic.hpp:48-164 (all exact) and Canonical
+// (:166-198) one rounding of the 64-bit integer.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double u01_f64_u64(int iv, uint64_t u)
+{
+    switch (iv) {
+        case 0: {
+            const uint64_t v = (u << 1) >> 10; // 54 bits
+            return __dmul_rn((double)((v & 1u) + v), 0x1p-54);
+        }
+        case 1:
+            return __dmul_rn((double)(u >> 11), 0x1p-53);
+        case 2:
+            return __dadd_rn(__dmul_rn((double)(u >> 11), 0x1p-53), 0x1p-53);
+        default:
+            return __dadd_rn(__dmul_rn((double)(u >> 12), 0x1p-52), 0x1p-53);
+    }
+}
+
+__device__ __forceinline__ float u01_f32_u64(int iv, uint64_t u)
+{
+    switch (iv) {
+        case 0: {
+            const uint32_t v = (uint32_t)((u << 1) >> 39); // 25 bits
+            return __fmul_rn((float)((v & 1u) + v), 0x1p-25f);
+        }
+        case 1:
+            return __fmul_rn((float)(uint32_t)(u >> 40), 0x1p-24f);
+        case 2:
+            return __fadd_rn(
+                __fmul_rn((float)(uint32_t)(u >> 40), 0x1p-24f), 0x1p-24f);
+        default:
+            return __fadd_rn(
+                __fmul_rn((float)(uint32_t)(u >> 41), 0x1p-23f), 0x1p-24f);
+    }
+}
+
+// u rounded to the nearest double (ties to even): the two halves are exact
+// in double and their sum rounds once
+__device__ __forceinline__ double u64_to_f64_rn(uint64_t u)
+{
+    return __dadd_rn(__dmul_rn((double)(uint32_t)(u >> 32), 0x1p32),
+        (double)(uint32_t)u);
+}
+
+// u rounded to the nearest float (ties to even), by integer rounding of the
+// top 24 bits
+__device__ __forceinline__ float u64_to_f32_rn(uint64_t u)
+{
+    if (u < (1ull << 24))
+        return (float)(uint32_t)u;
+    const int lz = __clzll((long long)u);
+    const uint64_t v = u << lz; // leading one at bit 63
+    uint32_t m = (uint32_t)(v >> 40);
+    const uint64_t rest = v & ((1ull << 40) - 1);
+    const uint64_t half = 1ull << 39;
+    if (rest > half || (rest == half && (m & 1u)))
+        ++m; // may reach 2^24, still exact
+    return __builtin_ldexpf((float)m, 40 - lz);
+}
+
+__device__ __forceinline__ double u01_canonical_f64_u64(uint64_t u)
+{
+    return __dmul_rn(u64_to_f64_rn(u), 0x1p-64);
+}
+
+__device__ __forceinline__ float u01_canonical_f32_u64(uint64_t u)
+{
+    return __fmul_rn(u64_to_f32_rn(u), 0x1p-64f);
+}
+
 } // namespace mckl_hip

@@ -190,6 +190,67 @@ struct ConvCanF32 {
     }
 };
 
+// One u64 word per value (64-bit-result engines): two u32 of the block,
+// little-endian, as the engine's result words are laid out
+__device__ __forceinline__ uint64_t word64(const uint32_t *w)
+{
+    return (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+}
+struct ConvU01F64W {
+    using T = double;
+    static constexpr int Q = 2;
+    int iv;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return u01_f64_u64(iv, word64(w));
+    }
+};
+struct ConvU01F32W {
+    using T = float;
+    static constexpr int Q = 2;
+    int iv;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return u01_f32_u64(iv, word64(w));
+    }
+};
+struct ConvCanF64W {
+    using T = double;
+    static constexpr int Q = 2;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return u01_canonical_f64_u64(word64(w));
+    }
+};
+struct ConvCanF32W {
+    using T = float;
+    static constexpr int Q = 2;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return u01_canonical_f32_u64(word64(w));
+    }
+};
+struct ConvUniformF64W {
+    using T = double;
+    static constexpr int Q = 2;
+    double a, d;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return __dadd_rn(__dmul_rn(u01_f64_u64(MCKL_HIP_U01_CO, word64(w)), d),
+            a);
+    }
+};
+struct ConvUniformF32W {
+    using T = float;
+    static constexpr int Q = 2;
+    float a, d;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return __fadd_rn(__fmul_rn(u01_f32_u64(MCKL_HIP_U01_CO, word64(w)), d),
+            a);
+    }
+};
+
 // 16-byte vector stores through L2 (plain, not non-temporal): with 32 or 64
 // bytes per lane a store instruction covers only part of each cache line,
 // and L2 must merge the pieces of a line before it goes to HBM; measured on
@@ -295,15 +356,19 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(FillBatch batch,
             if (b >= nblocks)
                 break;
             const int64_t e0 = (int64_t)(b * EPB) - skip;
-            if (vec_ok && e0 >= 0 && e0 + EPB <= (int64_t)n) {
-                store_vec(o + e0, v[u]);
-            } else {
-#pragma unroll
-                for (int j = 0; j < EPB; ++j) {
-                    const int64_t e = e0 + j;
-                    if (e >= 0 && e < (int64_t)n)
-                        o[e] = v[u][j];
+            // a block of 8 output bytes (floats from 64-bit words) is stored
+            // element by element
+            if constexpr ((sizeof(T) * EPB) % 16 == 0) {
+                if (vec_ok && e0 >= 0 && e0 + EPB <= (int64_t)n) {
+                    store_vec(o + e0, v[u]);
+                    continue;
                 }
+            }
+#pragma unroll
+            for (int j = 0; j < EPB; ++j) {
+                const int64_t e = e0 + j;
+                if (e >= 0 && e < (int64_t)n)
+                    o[e] = v[u][j];
             }
         }
     }
@@ -348,14 +413,36 @@ struct NormalTraits<double> {
     static constexpr int K = 1024; // BufferSize<double>
 };
 
-// Box-Muller on one (s-word, theta-word) pair with the reference's op order
-// (normal_distribution.hpp:58-77).  Float transcendentals are evaluated in
-// double and rounded once, which is within 1 ulp of glibc's logf/sinf/cosf.
-__device__ __forceinline__ void box_muller(uint32_t ws, uint32_t wt,
+// U01OC of a value's result word(s): one u32, or one u64 (Q = 2)
+template <typename T, int Q>
+__device__ __forceinline__ T u01oc(const uint32_t *w);
+template <>
+__device__ __forceinline__ float u01oc<float, 1>(const uint32_t *w)
+{
+    return u01_f32(MCKL_HIP_U01_OC, w[0]);
+}
+template <>
+__device__ __forceinline__ double u01oc<double, 1>(const uint32_t *w)
+{
+    return u01_f64(MCKL_HIP_U01_OC, w[0]);
+}
+template <>
+__device__ __forceinline__ float u01oc<float, 2>(const uint32_t *w)
+{
+    return u01_f32_u64(MCKL_HIP_U01_OC, word64(w));
+}
+template <>
+__device__ __forceinline__ double u01oc<double, 2>(const uint32_t *w)
+{
+    return u01_f64_u64(MCKL_HIP_U01_OC, word64(w));
+}
+
+// Box-Muller on one (s, theta) pair of U01OC values with the reference's op
+// order (normal_distribution.hpp:58-77).  Float transcendentals are
+// evaluated in double and rounded once (the correctly rounded value).
+__device__ __forceinline__ void box_muller(float u1, float u2,
     float mean, float sd, float &o_sin, float &o_cos)
 {
-    const float u1 = u01_f32(2, ws);
-    const float u2 = u01_f32(2, wt);
     float s = (float)log((double)u1);
     s = __fmul_rn(s, -2.0f);
     s = sqrt_rn(s);
@@ -376,11 +463,9 @@ __device__ __forceinline__ void box_muller(uint32_t ws, uint32_t wt,
 
 // Double: log, sin and cos correctly rounded (crmath.hpp), so a result can
 // differ from the reference's only where its libm misrounds.
-__device__ __forceinline__ void box_muller(uint32_t ws, uint32_t wt,
+__device__ __forceinline__ void box_muller(double u1, double u2,
     double mean, double sd, double &o_sin, double &o_cos)
 {
-    const double u1 = u01_f64(2, ws);
-    const double u2 = u01_f64(2, wt);
     double s = crm::log_cr(u1, d_crm_tables.log_c);
     s = __dmul_rn(s, -2.0);
     s = sqrt_rn(s);
@@ -400,9 +485,8 @@ __device__ __forceinline__ void box_muller(uint32_t ws, uint32_t wt,
 
 // odd tail: normal_distribution.hpp:93-100
 __device__ __forceinline__ float box_muller_tail(
-    uint32_t wu, uint32_t wv, float mean, float sd)
+    float u, float v, float mean, float sd)
 {
-    const float u = u01_f32(2, wu), v = u01_f32(2, wv);
     const float l = (float)log((double)u);
     const float r = sqrt_rn(__fmul_rn(-2.0f, l));
     const float c = (float)cos(
@@ -411,9 +495,8 @@ __device__ __forceinline__ float box_muller_tail(
 }
 
 __device__ __forceinline__ double box_muller_tail(
-    uint32_t wu, uint32_t wv, double mean, double sd)
+    double u, double v, double mean, double sd)
 {
-    const double u = u01_f64(2, wu), v = u01_f64(2, wv);
     const double r =
         sqrt_rn(__dmul_rn(-2.0, crm::log_cr(u, d_crm_tables.log_c)));
     double sn, c;
@@ -439,17 +522,31 @@ __device__ __forceinline__ uint32_t word_at(const Gen &g, uint64_t skip,
     return r;
 }
 
-// Full chunks.  Task (c, g): pairs g*W .. g*W+W-1 of chunk c, W = words per
-// block.  With skip == 0 the s-words and theta-words of a task are exactly
-// one block each; otherwise each window straddles two blocks.
-template <class Gen, typename T>
+// The result words of value p (relative to the call's first value) of a
+// stream whose first u32 word is word `skip` of block 0: Q u32 words
+template <class Gen, int Q>
+__device__ __forceinline__ void value_words(const Gen &g, uint64_t skip,
+    uint64_t p, uint32_t (&w)[Q])
+{
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        w[q] = word_at(g, skip, p * Q + q);
+}
+
+// Full chunks of K values.  Q u32 words per value (1, or 2 for one u64 word
+// of a 64-bit-result engine), V = W32 / Q values per block.  Task (c, g):
+// pairs g*V .. g*V+V-1 of chunk c.  With skip == 0 the s-words and
+// theta-words of a task are exactly one block each; otherwise each window
+// straddles two blocks (skip counts u32 words).
+template <class Gen, typename T, int Q>
 __global__ __launch_bounds__(kBlock) void normal_kernel(FillBatch batch,
     uint64_t nchunks, uint64_t n, T mean, T sd, T *out)
 {
     constexpr int K = NormalTraits<T>::K;
     constexpr int H = K / 2;
     constexpr int W = Gen::W32;
-    constexpr int G = H / W; // tasks per chunk
+    constexpr int V = W / Q;
+    constexpr int G = H / V; // tasks per chunk
     __shared__ uint32_t lds[LdsWords<Gen>::value];
     const FillArg &arg = batch.a[blockIdx.y];
     Gen g;
@@ -463,8 +560,8 @@ __global__ __launch_bounds__(kBlock) void normal_kernel(FillBatch batch,
          t < ntask; t += stride) {
         const uint64_t c = t / G;
         const uint32_t gi = (uint32_t)(t % G);
-        const uint64_t bs = c * (K / W) + gi;  // block of first s-word
-        const uint64_t bt = bs + H / W;        // block of first theta-word
+        const uint64_t bs = c * (K / V) + gi;  // block of first s-word
+        const uint64_t bt = bs + H / V;        // block of first theta-word
         uint32_t ws[W], wt[W];
         if (skip == 0) {
             g.block(bs, ws);
@@ -482,26 +579,30 @@ __global__ __launch_bounds__(kBlock) void normal_kernel(FillBatch batch,
             for (int j = 0; j < W; ++j)
                 wt[j] = j + skip < (uint32_t)W ? a[j + skip] : b[j + skip - W];
         }
-        T vs[W], vc[W];
+        T vs[V], vc[V];
 #pragma unroll
-        for (int j = 0; j < W; ++j)
-            box_muller(ws[j], wt[j], mean, sd, vs[j], vc[j]);
-        T *os = o + c * K + (uint64_t)gi * W;
-        if (arg.vec_ok) {
-            store_vec(os, vs);
-            store_vec(os + H, vc);
-        } else {
-#pragma unroll
-            for (int j = 0; j < W; ++j) {
-                os[j] = vs[j];
-                os[H + j] = vc[j];
+        for (int j = 0; j < V; ++j)
+            box_muller(u01oc<T, Q>(ws + j * Q), u01oc<T, Q>(wt + j * Q), mean,
+                sd, vs[j], vc[j]);
+        T *os = o + c * K + (uint64_t)gi * V;
+        if constexpr ((sizeof(T) * V) % 16 == 0) {
+            if (arg.vec_ok) {
+                store_vec(os, vs);
+                store_vec(os + H, vc);
+                continue;
             }
+        }
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            os[j] = vs[j];
+            os[H + j] = vc[j];
         }
     }
 }
 
-// Last partial chunk (L = n % K values, nu = L/2 pairs) and the odd tail.
-template <class Gen, typename T>
+// Last partial chunk (L = n % K values, nu = L/2 pairs) and the odd tail
+// (two scalar U01OC draws, one result word each).
+template <class Gen, typename T, int Q>
 __global__ __launch_bounds__(kBlock) void normal_tail_kernel(FillBatch batch,
     uint64_t nchunks, uint64_t n, T mean, T sd, T *out)
 {
@@ -516,17 +617,19 @@ __global__ __launch_bounds__(kBlock) void normal_tail_kernel(FillBatch batch,
     const uint64_t L = n - base;
     const uint64_t nu = L / 2;
     for (uint64_t p = threadIdx.x; p <= nu; p += blockDim.x) {
+        uint32_t wa[Q], wb[Q];
         if (p < nu) {
-            const uint32_t ws = word_at(g, arg.skip, base + p);
-            const uint32_t wt = word_at(g, arg.skip, base + nu + p);
+            value_words<Gen, Q>(g, arg.skip, base + p, wa);
+            value_words<Gen, Q>(g, arg.skip, base + nu + p, wb);
             T a, b;
-            box_muller(ws, wt, mean, sd, a, b);
+            box_muller(u01oc<T, Q>(wa), u01oc<T, Q>(wb), mean, sd, a, b);
             o[base + p] = a;
             o[base + nu + p] = b;
         } else if (L % 2 == 1) {
-            const uint32_t wu = word_at(g, arg.skip, base + L);
-            const uint32_t wv = word_at(g, arg.skip, base + L + 1);
-            o[base + L - 1] = box_muller_tail(wu, wv, mean, sd);
+            value_words<Gen, Q>(g, arg.skip, base + L, wa);
+            value_words<Gen, Q>(g, arg.skip, base + L + 1, wb);
+            o[base + L - 1] =
+                box_muller_tail(u01oc<T, Q>(wa), u01oc<T, Q>(wb), mean, sd);
         }
     }
 }
@@ -698,12 +801,12 @@ static int launch_straddle(const std::vector<FillArg> &args, uint64_t n,
 }
 
 template <class Gen, typename T>
-static int launch_normal(const std::vector<FillArg> &args, uint64_t n, T mean,
-    T sd, T *out, hipStream_t st)
+static int launch_normal(const std::vector<FillArg> &args, int Q, uint64_t n,
+    T mean, T sd, T *out, hipStream_t st)
 {
     constexpr int K = NormalTraits<T>::K;
     const uint64_t nchunks = n / K;
-    const uint64_t ntask = nchunks * (K / 2 / Gen::W32);
+    const uint64_t ntask = nchunks * (K / 2 / (Gen::W32 / Q));
     for (size_t b0 = 0; b0 < args.size(); b0 += kMaxBatch) {
         FillBatch batch;
         std::memset(&batch, 0, sizeof(batch));
@@ -716,13 +819,25 @@ static int launch_normal(const std::vector<FillArg> &args, uint64_t n, T mean,
                     std::max<size_t>(1, cnt));
             const uint32_t gx = (uint32_t)std::max<uint64_t>(
                 1, std::min<uint64_t>(ceil_div(ntask, kBlock), cap));
-            normal_kernel<Gen, T><<<dim3(gx, (uint32_t)cnt), kBlock, 0, st>>>(
-                batch, nchunks, n, mean, sd, out + b0 * n);
+            if (Q == 2)
+                normal_kernel<Gen, T, 2>
+                    <<<dim3(gx, (uint32_t)cnt), kBlock, 0, st>>>(
+                        batch, nchunks, n, mean, sd, out + b0 * n);
+            else
+                normal_kernel<Gen, T, 1>
+                    <<<dim3(gx, (uint32_t)cnt), kBlock, 0, st>>>(
+                        batch, nchunks, n, mean, sd, out + b0 * n);
             MCKL_HIP_CHECK_LAUNCH("normal_kernel");
         }
         if (n % K != 0) {
-            normal_tail_kernel<Gen, T><<<dim3(1, (uint32_t)cnt), kBlock, 0, st>>>(
-                batch, nchunks, n, mean, sd, out + b0 * n);
+            if (Q == 2)
+                normal_tail_kernel<Gen, T, 2>
+                    <<<dim3(1, (uint32_t)cnt), kBlock, 0, st>>>(
+                        batch, nchunks, n, mean, sd, out + b0 * n);
+            else
+                normal_tail_kernel<Gen, T, 1>
+                    <<<dim3(1, (uint32_t)cnt), kBlock, 0, st>>>(
+                        batch, nchunks, n, mean, sd, out + b0 * n);
             MCKL_HIP_CHECK_LAUNCH("normal_tail_kernel");
         }
     }
@@ -766,10 +881,10 @@ struct StraddleOp {
 template <class Gen>
 struct NormalOp {
     template <typename T>
-    static int run(const std::vector<FillArg> &args, uint64_t n, T mean, T sd,
-        T *out, hipStream_t st)
+    static int run(const std::vector<FillArg> &args, int Q, uint64_t n, T mean,
+        T sd, T *out, hipStream_t st)
     {
-        return launch_normal<Gen, T>(args, n, mean, sd, out, st);
+        return launch_normal<Gen, T>(args, Q, n, mean, sd, out, st);
     }
 };
 
@@ -854,9 +969,6 @@ static int normal_fill(mckl_hip_engine *e, size_t nstreams, uint64_t n, T mean,
     int rc = common_checks(e, nstreams);
     if (rc)
         return rc;
-    if (e[0].rbits != 32)
-        return set_error(MCKL_HIP_EINVAL,
-            "Normal fills are provided for 32-bit engines");
     if (!(sd > 0))
         return set_error(MCKL_HIP_EINVAL,
             "NormalDistribution requires stddev > 0"); // normal_distribution.hpp:45-48
@@ -877,7 +989,8 @@ static int normal_fill(mckl_hip_engine *e, size_t nstreams, uint64_t n, T mean,
         args[s].vec_ok = (base % 16 == 0) ? 1u : 0u;
         args[s].pad = 0;
     }
-    rc = by_kind<NormalOp>(e[0].kind, args, n, mean, sd, out, st);
+    rc = by_kind<NormalOp>(e[0].kind, args, e[0].rbits / 32, n, mean, sd,
+        out, st);
     if (rc)
         return rc;
     for (size_t s = 0; s < nstreams; ++s) {
@@ -1127,11 +1240,18 @@ static int u01_f64_impl(mckl_hip_engine *e, size_t ns, int interval, size_t n,
 {
     if (interval < 0 || interval > 4)
         return set_error(MCKL_HIP_EINVAL, "unknown U01 interval %d", interval);
-    for (size_t s = 0; s < ns; ++s)
-        if (e && e[s].rbits != 32)
-            return set_error(
-                MCKL_HIP_EINVAL, "U01 fills are provided for 32-bit engines");
-    // BufferSize<double> = 1024 elements per bulk call (common.hpp:69-86)
+    if (!e)
+        return set_error(MCKL_HIP_EINVAL, "engine is NULL");
+    // BufferSize<double> = 1024 elements per bulk call (common.hpp:69-86);
+    // 64-bit-result engines take one u64 word per value, Canonical included
+    if (e[0].rbits == 64) {
+        if (interval == MCKL_HIP_U01_CANONICAL)
+            return element_fill(e, ns, n, d_r, ConvCanF64W(), 1024,
+                as_stream(stream));
+        ConvU01F64W c;
+        c.iv = interval;
+        return element_fill(e, ns, n, d_r, c, 1024, as_stream(stream));
+    }
     if (interval == MCKL_HIP_U01_CANONICAL)
         return element_fill(e, ns, n, d_r, ConvCanF64(), 2048, as_stream(stream));
     ConvU01F64 c;
@@ -1156,9 +1276,14 @@ int mckl_hip_rand_u01_f32(
 {
     if (interval < 0 || interval > 4)
         return set_error(MCKL_HIP_EINVAL, "unknown U01 interval %d", interval);
-    if (e && e->rbits != 32)
-        return set_error(
-            MCKL_HIP_EINVAL, "U01 fills are provided for 32-bit engines");
+    if (e && e->rbits == 64) {
+        if (interval == MCKL_HIP_U01_CANONICAL)
+            return element_fill(e, 1, n, d_r, ConvCanF32W(), 2048,
+                as_stream(stream));
+        ConvU01F32W c;
+        c.iv = interval;
+        return element_fill(e, 1, n, d_r, c, 2048, as_stream(stream));
+    }
     if (interval == MCKL_HIP_U01_CANONICAL)
         return element_fill(e, 1, n, d_r, ConvCanF32(), 2048, as_stream(stream));
     ConvU01F32 c;
@@ -1172,9 +1297,12 @@ int mckl_hip_rand_uniform_real_f64(mckl_hip_engine *e, size_t n, double a,
     if (!(a < b))
         return set_error(MCKL_HIP_EINVAL,
             "**UniformRealDistribution** constructed with invalid arguments");
-    if (e && e->rbits != 32)
-        return set_error(MCKL_HIP_EINVAL,
-            "UniformReal fills are provided for 32-bit engines");
+    if (e && e->rbits == 64) {
+        ConvUniformF64W c;
+        c.a = a;
+        c.d = b - a;
+        return element_fill(e, 1, n, d_r, c, 1024, as_stream(stream));
+    }
     ConvUniformF64 c;
     c.a = a;
     c.d = b - a;
@@ -1187,9 +1315,12 @@ int mckl_hip_rand_uniform_real_f32(mckl_hip_engine *e, size_t n, float a,
     if (!(a < b))
         return set_error(MCKL_HIP_EINVAL,
             "**UniformRealDistribution** constructed with invalid arguments");
-    if (e && e->rbits != 32)
-        return set_error(MCKL_HIP_EINVAL,
-            "UniformReal fills are provided for 32-bit engines");
+    if (e && e->rbits == 64) {
+        ConvUniformF32W c;
+        c.a = a;
+        c.d = b - a;
+        return element_fill(e, 1, n, d_r, c, 2048, as_stream(stream));
+    }
     ConvUniformF32 c;
     c.a = a;
     c.d = b - a;

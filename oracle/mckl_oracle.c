@@ -565,6 +565,20 @@ uint32_t mo_next_u32(mo_engine *e)
     return v;
 }
 
+/* scalar operator() of a 64-bit-result engine (counter.hpp:134-142) */
+uint64_t mo_next_u64(mo_engine *e)
+{
+    const unsigned M = mo_engine_words(e);
+    if (e->index == M) {
+        mo_gen_next(e, e->buf);
+        e->index = 0;
+    }
+    uint64_t v;
+    memcpy(&v, e->buf + (size_t)e->index * 8, 8);
+    e->index++;
+    return v;
+}
+
 void mo_discard(mo_engine *e, uint64_t nskip)
 {
     /* counter.hpp:178-199; rate = sizeof(result block)/sizeof(ctr) == 1 */
@@ -630,11 +644,66 @@ float mo_u01_canonical_f32(uint32_t u)
     return (float)u * 0x1p-32f + 0.0f;
 }
 
+/* The same conversions from one u64 word, the word type the reference
+ * uses for every real type on a 64-bit-result engine
+ * (U01UIntType, u01_distribution.hpp:105-113): W = 64 in
+ * u01_generic.hpp:48-164, all exact; Canonical (:166-198) with Q = 1 is
+ * one rounding of the 64-bit integer. */
+double mo_u01_u64_f64(int interval, uint64_t u)
+{
+    switch (interval) {
+        case MO_CC: {
+            const uint64_t v = (u << 1) >> 10; /* P = 53, L = 1, R = 9 */
+            return (double)((v & 1) + v) * 0x1p-54;
+        }
+        case MO_CO:
+            return (double)(u >> 11) * 0x1p-53;
+        case MO_OC:
+            return (double)(u >> 11) * 0x1p-53 + 0x1p-53;
+        default:
+            return (double)(u >> 12) * 0x1p-52 + 0x1p-53;
+    }
+}
+
+float mo_u01_u64_f32(int interval, uint64_t u)
+{
+    switch (interval) {
+        case MO_CC: {
+            const uint32_t v = (uint32_t)((u << 1) >> 39); /* P = 24, R = 38 */
+            return (float)((v & 1) + v) * 0x1p-25f;
+        }
+        case MO_CO:
+            return (float)(uint32_t)(u >> 40) * 0x1p-24f;
+        case MO_OC:
+            return (float)(uint32_t)(u >> 40) * 0x1p-24f + 0x1p-24f;
+        default:
+            return (float)(uint32_t)(u >> 41) * 0x1p-23f + 0x1p-24f;
+    }
+}
+
+double mo_u01_canonical64_f64(uint64_t u) { return (double)u * 0x1p-64; }
+
+float mo_u01_canonical64_f32(uint64_t u) { return (float)u * 0x1p-64f; }
+
 /* random/internal/common.hpp:69-86 (BATCH_0 chunking by BufferSize),
  * internal/defines.hpp:62-67 (BufferSize = 8192 / sizeof(T)). */
 void mo_rand_u01_f64(mo_engine *e, int interval, size_t n, double *r)
 {
     const size_t K = 1024;
+    if (e->rbits == 64) {
+        uint64_t w[1024];
+        for (;;) {
+            size_t k = n < K ? n : K;
+            mo_rand_u64(e, k, w);
+            for (size_t i = 0; i < k; ++i)
+                r[i] = interval == MO_CANONICAL ? mo_u01_canonical64_f64(w[i])
+                                                : mo_u01_u64_f64(interval, w[i]);
+            r += k;
+            n -= k;
+            if (k < K)
+                return;
+        }
+    }
     uint32_t s[2 * 1024];
     const size_t Q = interval == MO_CANONICAL ? 2 : 1;
     for (;;) {
@@ -653,6 +722,20 @@ void mo_rand_u01_f64(mo_engine *e, int interval, size_t n, double *r)
 void mo_rand_u01_f32(mo_engine *e, int interval, size_t n, float *r)
 {
     const size_t K = 2048;
+    if (e->rbits == 64) {
+        uint64_t w[2048];
+        for (;;) {
+            size_t k = n < K ? n : K;
+            mo_rand_u64(e, k, w);
+            for (size_t i = 0; i < k; ++i)
+                r[i] = interval == MO_CANONICAL ? mo_u01_canonical64_f32(w[i])
+                                                : mo_u01_u64_f32(interval, w[i]);
+            r += k;
+            n -= k;
+            if (k < K)
+                return;
+        }
+    }
     uint32_t s[2048];
     for (;;) {
         size_t k = n < K ? n : K;
@@ -800,8 +883,11 @@ void mo_rand_normal_f32_cr(mo_engine *e, size_t n, float mean, float sd,
         mo_normal_chunk_f32(e, k, r, mean, sd, clean ? clean + i * k : NULL);
     mo_normal_chunk_f32(e, l, r, mean, sd, clean ? clean + m * k : NULL);
     if (n % 2 != 0) {
-        float u = mo_u01_u32_f32(MO_OC, mo_next_u32(e));
-        float v = mo_u01_u32_f32(MO_OC, mo_next_u32(e));
+        /* two scalar U01OC draws: one result word each */
+        float u = e->rbits == 64 ? mo_u01_u64_f32(MO_OC, mo_next_u64(e))
+                                 : mo_u01_u32_f32(MO_OC, mo_next_u32(e));
+        float v = e->rbits == 64 ? mo_u01_u64_f32(MO_OC, mo_next_u64(e))
+                                 : mo_u01_u32_f32(MO_OC, mo_next_u32(e));
         const float lu = logf(u);
         const float th = 6.283185307179586476925286766559005768394f * v;
         const float c = cosf(th);
@@ -877,8 +963,10 @@ void mo_rand_normal_f64_cr(mo_engine *e, size_t n, double mean, double sd,
         mo_normal_chunk_f64(e, k, r, mean, sd, clean ? clean + i * k : NULL);
     mo_normal_chunk_f64(e, l, r, mean, sd, clean ? clean + m * k : NULL);
     if (n % 2 != 0) {
-        double u = mo_u01_u32_f64(MO_OC, mo_next_u32(e));
-        double v = mo_u01_u32_f64(MO_OC, mo_next_u32(e));
+        double u = e->rbits == 64 ? mo_u01_u64_f64(MO_OC, mo_next_u64(e))
+                                  : mo_u01_u32_f64(MO_OC, mo_next_u32(e));
+        double v = e->rbits == 64 ? mo_u01_u64_f64(MO_OC, mo_next_u64(e))
+                                  : mo_u01_u32_f64(MO_OC, mo_next_u32(e));
         const double lu = log(u);
         const double th = 6.283185307179586 * v;
         const double c = cos(th);

@@ -99,6 +99,12 @@ double mo_u01_u32_f64(int interval, uint32_t u);
 float mo_u01_u32_f32(int interval, uint32_t u);
 double mo_u01_canonical_f64(uint32_t u0, uint32_t u1);
 float mo_u01_canonical_f32(uint32_t u);
+/* one u64 word per value (64-bit-result engines) */
+double mo_u01_u64_f64(int interval, uint64_t u);
+float mo_u01_u64_f32(int interval, uint64_t u);
+double mo_u01_canonical64_f64(uint64_t u);
+float mo_u01_canonical64_f32(uint64_t u);
+uint64_t mo_next_u64(mo_engine *e);
 void mo_rand_u01_f64(mo_engine *e, int interval, size_t n, double *r);
 void mo_rand_u01_f32(mo_engine *e, int interval, size_t n, float *r);
 void mo_rand_uniform_real_f64(mo_engine *e, size_t n, double a, double b,

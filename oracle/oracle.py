@@ -104,12 +104,13 @@ def rand_uniform_real(e, n, a, b, dtype):
     return out[:n]
 
 
-def ref_uniform_real(kind, key, dtype, n, a, b, pre=0):
+def ref_uniform_real(kind, key, dtype, n, a, b, pre=0, rbits=32):
     out = np.zeros(n + 1, dtype)
     k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
     ctr = (ctypes.c_uint64 * 4)()
     idx = ctypes.c_uint()
-    REF.ref_uniform_real(kind, k, int(np.dtype(dtype) == np.float64), _sz(pre),
+    REF.ref_uniform_real(kind, rbits, k, int(np.dtype(dtype) == np.float64),
+                         _sz(pre),
                          _sz(n), ctypes.c_double(a), ctypes.c_double(b), P(out),
                          ctr, ctypes.byref(idx))
     nc = 4 if kind == THREEFRY4X64 else 2
@@ -304,23 +305,25 @@ def ref_engine_ops(kind, rbits, key, sizes, ctr0=None):
     return out[:n], list(ctr[:nc]), idx.value
 
 
-def ref_u01(kind, key, interval, dtype, n, pre=0):
+def ref_u01(kind, key, interval, dtype, n, pre=0, rbits=32):
     out = np.zeros(n + 1, dtype)
     k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
     ctr = (ctypes.c_uint64 * 4)()
     idx = ctypes.c_uint()
-    REF.ref_u01(kind, k, interval, int(np.dtype(dtype) == np.float64), _sz(pre),
+    REF.ref_u01(kind, rbits, k, interval, int(np.dtype(dtype) == np.float64),
+                _sz(pre),
                 _sz(n), P(out), ctr, ctypes.byref(idx))
     nc = 4 if kind == THREEFRY4X64 else 2
     return out[:n], list(ctr[:nc]), idx.value
 
 
-def ref_normal(kind, key, dtype, n, mean, sd, pre=0):
+def ref_normal(kind, key, dtype, n, mean, sd, pre=0, rbits=32):
     out = np.zeros(n + 1, dtype)
     k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
     ctr = (ctypes.c_uint64 * 4)()
     idx = ctypes.c_uint()
-    REF.ref_normal(kind, k, int(np.dtype(dtype) == np.float64), _sz(pre), _sz(n),
+    REF.ref_normal(kind, rbits, k, int(np.dtype(dtype) == np.float64), _sz(pre),
+                   _sz(n),
                    ctypes.c_double(mean), ctypes.c_double(sd), P(out), ctr,
                    ctypes.byref(idx))
     nc = 4 if kind == THREEFRY4X64 else 2

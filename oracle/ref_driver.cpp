@@ -160,12 +160,13 @@ int ref_engine_text(int kind, int rbits, const std::uint64_t *key, int ncalls,
     });
 }
 
-// pre raw u32 draws, then rand(rng, U01{CC,CO,OC,OO,Canonical}<T>, n, r)
-int ref_u01(int kind, const std::uint64_t *key, int interval, int dbl,
+// pre raw result words, then rand(rng, U01{CC,CO,OC,OO,Canonical}<T>, n, r)
+// (rbits 32 or 64: the engine's result width)
+int ref_u01(int kind, int rbits, const std::uint64_t *key, int interval, int dbl,
     std::size_t pre, std::size_t n, void *out, std::uint64_t *ctr,
     unsigned *index)
 {
-    return dispatch(kind, 32, key, [&](auto &rng) {
+    return dispatch(kind, rbits, key, [&](auto &rng) {
         using R = typename std::remove_reference_t<decltype(rng)>::result_type;
         std::vector<R> skip(pre + 1);
         mckl::rand(rng, pre, skip.data());
@@ -203,12 +204,12 @@ int ref_u01(int kind, const std::uint64_t *key, int interval, int dbl,
     });
 }
 
-// pre raw u32 draws, then rand(rng, NormalDistribution<T>(mean, sd), n, r)
-int ref_normal(int kind, const std::uint64_t *key, int dbl, std::size_t pre,
+// pre raw result words, then rand(rng, NormalDistribution<T>(mean, sd), n, r)
+int ref_normal(int kind, int rbits, const std::uint64_t *key, int dbl, std::size_t pre,
     std::size_t n, double mean, double sd, void *out, std::uint64_t *ctr,
     unsigned *index)
 {
-    return dispatch(kind, 32, key, [&](auto &rng) {
+    return dispatch(kind, rbits, key, [&](auto &rng) {
         using R = typename std::remove_reference_t<decltype(rng)>::result_type;
         std::vector<R> skip(pre + 1);
         mckl::rand(rng, pre, skip.data());
@@ -225,12 +226,13 @@ int ref_normal(int kind, const std::uint64_t *key, int dbl, std::size_t pre,
     });
 }
 
-// pre raw u32 draws, then rand(rng, UniformRealDistribution<T>(a, b), n, r)
-int ref_uniform_real(int kind, const std::uint64_t *key, int dbl,
+// pre raw result words, then rand(rng, UniformRealDistribution<T>(a, b), n,
+// r)
+int ref_uniform_real(int kind, int rbits, const std::uint64_t *key, int dbl,
     std::size_t pre, std::size_t n, double a, double b, void *out,
     std::uint64_t *ctr, unsigned *index)
 {
-    return dispatch(kind, 32, key, [&](auto &rng) {
+    return dispatch(kind, rbits, key, [&](auto &rng) {
         using R = typename std::remove_reference_t<decltype(rng)>::result_type;
         std::vector<R> skip(pre + 1);
         mckl::rand(rng, pre, skip.data());

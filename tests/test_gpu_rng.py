@@ -62,6 +62,17 @@ def _tensor(n, itemsize, dev, fp=False):
     return torch.full((max(n, 1),), -1, dtype=dt, device=dev)
 
 
+def _predraw(m, rng, e, pre, rbits, dev):
+    """pre raw result words (u32 or u64) from the device engine and the
+    oracle's, before a distribution call"""
+    if pre:
+        m.rand(rng, pre, _tensor(pre, rbits // 8, dev))
+        if rbits == 64:
+            orc.rand_u64(e, pre)
+        else:
+            orc.rand_u32(e, pre)
+
+
 def ulp_diff(a, b):
     if a.dtype == np.float32:
         ia = a.view(np.int32).astype(np.int64)
@@ -213,12 +224,14 @@ def test_raw_kernel_abi(dev, kind):
     assert list(ctr[:2]) == end
 
 
+@pytest.mark.parametrize("rbits", [32, 64])
 @pytest.mark.parametrize("kind", [1, 2, 3])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("interval", [0, 1, 2, 3, 4])
-def test_u01_fills(dev, kind, dtype, interval):
+def test_u01_fills(dev, kind, dtype, interval, rbits):
     """rand(rng, U01xx<T>, n, r) bit-exact, after 0..7 pre-drawn words (odd
-    offsets exercise the straddling Canonical<double> path)."""
+    offsets exercise the straddling Canonical<double> path); the *_64
+    engines take one u64 word per value."""
     m = _m()
     dist = [m.U01CCDistribution, m.U01CODistribution, m.U01OCDistribution,
             m.U01OODistribution, m.U01CanonicalDistribution][interval](dtype)
@@ -226,12 +239,9 @@ def test_u01_fills(dev, kind, dtype, interval):
     for pre, n in ((0, 5000), (1, 4099), (3, 2048), (2, 1), (7, 12345),
                    (5, 0)):
         key = [pre + 1, 2, 3, 4][:2 if kind == 1 else 4]
-        rng = _cls(kind, 32)(key=key)
-        e = orc.engine(kind, key)
-        if pre:
-            tmp = _tensor(pre, 4, dev)
-            m.rand(rng, pre, tmp)
-            orc.rand_u32(e, pre)
+        rng = _cls(kind, rbits)(key=key)
+        e = orc.engine(kind, key, rbits)
+        _predraw(m, rng, e, pre, rbits, dev)
         out = _tensor(n, isz, dev, fp=True)
         m.rand(rng, dist, n, out)
         got = out.cpu().numpy()[:n]
@@ -241,9 +251,10 @@ def test_u01_fills(dev, kind, dtype, interval):
         assert rng.ctr() == orc.state(e)[0] and rng.index == orc.state(e)[1]
 
 
+@pytest.mark.parametrize("rbits", [32, 64])
 @pytest.mark.parametrize("kind", [1, 2, 3])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_uniform_real_fills(dev, kind, dtype):
+def test_uniform_real_fills(dev, kind, dtype, rbits):
     """rand(rng, UniformRealDistribution<T>(a, b), n, r) bit-exact against
     the oracle (pinned to the compiled reference), after pre-drawn words."""
     m = _m()
@@ -252,12 +263,9 @@ def test_uniform_real_fills(dev, kind, dtype):
                          (1, 1, 1e-3, 1e-3 + 1e-9), (0, 40961, -1e6, -3.0),
                          (5, 0, 0.0, 1.0), (2, 3001, 100.0, 1e8)):
         key = [pre + 3, 1, 4, 1][:2 if kind == 1 else 4]
-        rng = _cls(kind, 32)(key=key)
-        e = orc.engine(kind, key)
-        if pre:
-            tmp = _tensor(pre, 4, dev)
-            m.rand(rng, pre, tmp)
-            orc.rand_u32(e, pre)
+        rng = _cls(kind, rbits)(key=key)
+        e = orc.engine(kind, key, rbits)
+        _predraw(m, rng, e, pre, rbits, dev)
         out = _tensor(n, isz, dev, fp=True)
         m.rand(rng, m.UniformRealDistribution(a, b, dtype), n, out)
         got = out.cpu().numpy()[:n]
@@ -322,9 +330,10 @@ def normal_parity(got, want, clean, mean):
     return int((expl > 0).sum()), hist
 
 
+@pytest.mark.parametrize("rbits", [32, 64])
 @pytest.mark.parametrize("kind", [1, 2, 3])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_normal_fills(dev, kind, dtype):
+def test_normal_fills(dev, kind, dtype, rbits):
     m = _m()
     isz = np.dtype(dtype).itemsize
     nexpl = 0
@@ -333,12 +342,9 @@ def test_normal_fills(dev, kind, dtype):
     nclean = 0
     for pre, n, mean, sd in NORMAL_CASES:
         key = [pre + 7, 1, 2, 3][:2 if kind == 1 else 4]
-        rng = _cls(kind, 32)(key=key)
-        e = orc.engine(kind, key)
-        if pre:
-            tmp = _tensor(pre, 4, dev)
-            m.rand(rng, pre, tmp)
-            orc.rand_u32(e, pre)
+        rng = _cls(kind, rbits)(key=key)
+        e = orc.engine(kind, key, rbits)
+        _predraw(m, rng, e, pre, rbits, dev)
         out = _tensor(n, isz, dev, fp=True)
         m.rand(rng, m.NormalDistribution(mean, sd, dtype), n, out)
         got = out.cpu().numpy()[:n]
@@ -349,10 +355,10 @@ def test_normal_fills(dev, kind, dtype):
         nclean += int(clean.sum())
         total += n
         assert rng.ctr() == orc.state(e)[0] and rng.index == orc.state(e)[1]
-    print("normal kind=%d %s: %d of %d values from correctly rounded libm "
-          "results, all bit-exact; %d explained differences (ulp histogram "
-          "0/1/2: %s)" % (kind, np.dtype(dtype).name, nclean, total, nexpl,
-                          hist.tolist()))
+    print("normal kind=%d rbits=%d %s: %d of %d values from correctly "
+          "rounded libm results, all bit-exact; %d explained differences "
+          "(ulp histogram 0/1/2: %s)" % (kind, rbits, np.dtype(dtype).name,
+                                         nclean, total, nexpl, hist.tolist()))
 
 
 @pytest.mark.parametrize("ename,kind", [("threefry", 2), ("ars", 3)])

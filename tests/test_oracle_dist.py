@@ -1,6 +1,7 @@
 """Pin the oracle's U01 and Normal fills: closed forms and edge words for U01,
 the compiled reference (bit-exact) for both, the committed fixtures, and
 moments against theory.  CPU only."""
+import ctypes
 import hashlib
 import json
 import os
@@ -40,16 +41,77 @@ def test_u01_closed_forms_and_edges():
         float(np.float64((2 ** 64 - 1) * 2.0 ** -64))
 
 
+def rn(u, bits):
+    """u rounded to `bits` significant bits, to nearest, ties to even"""
+    e = u.bit_length() - bits
+    if e <= 0:
+        return u
+    q, r = divmod(u, 2 ** e)
+    half = 2 ** (e - 1)
+    if r > half or (r == half and q & 1):
+        q += 1
+    return q * 2 ** e
+
+
+def test_u01_u64_closed_forms_and_edges():
+    """The W = 64 forms (u01_generic.hpp:48-198 with UIntType uint64, the
+    word type of every real type on a 64-bit-result engine,
+    u01_distribution.hpp:105-113) at edge words, against exact rational
+    arithmetic."""
+    from fractions import Fraction as F
+    O = orc.O
+    O.mo_u01_u64_f64.restype = ctypes.c_double
+    O.mo_u01_u64_f32.restype = ctypes.c_float
+    O.mo_u01_canonical64_f64.restype = ctypes.c_double
+    O.mo_u01_canonical64_f32.restype = ctypes.c_float
+    for f in (O.mo_u01_u64_f64, O.mo_u01_u64_f32):
+        f.argtypes = [ctypes.c_int, ctypes.c_uint64]
+    for f in (O.mo_u01_canonical64_f64, O.mo_u01_canonical64_f32):
+        f.argtypes = [ctypes.c_uint64]
+    words = (0, 1, 2 ** 11 - 1, 2 ** 11, 2 ** 40, 2 ** 41 - 1, 2 ** 53 - 1,
+             2 ** 63, 2 ** 64 - 2 ** 11, 2 ** 64 - 1, 0x0123456789ABCDEF,
+             2 ** 24 + 1, (2 ** 24 + 1) * 2 ** 39, (2 ** 24 + 3) * 2 ** 39,
+             2 ** 53 + 1, (2 ** 53 + 1) * 2 ** 10, (2 ** 53 + 3) * 2 ** 10)
+    for u in words:
+        assert O.mo_u01_u64_f64(orc.CO, u) == F(u >> 11, 2 ** 53)
+        assert O.mo_u01_u64_f64(orc.OC, u) == F((u >> 11) + 1, 2 ** 53)
+        assert O.mo_u01_u64_f64(orc.OO, u) == F(2 * (u >> 12) + 1, 2 ** 53)
+        v = ((u << 1) % 2 ** 64) >> 10
+        assert O.mo_u01_u64_f64(orc.CC, u) == F(v + (v & 1), 2 ** 54)
+        assert O.mo_u01_u64_f32(orc.CO, u) == F(u >> 40, 2 ** 24)
+        assert O.mo_u01_u64_f32(orc.OC, u) == F((u >> 40) + 1, 2 ** 24)
+        assert O.mo_u01_u64_f32(orc.OO, u) == F(2 * (u >> 41) + 1, 2 ** 24)
+        v = ((u << 1) % 2 ** 64) >> 39
+        assert O.mo_u01_u64_f32(orc.CC, u) == F(v + (v & 1), 2 ** 25)
+        # Canonical: the integer rounded once to the real type, times 2^-64
+        assert O.mo_u01_canonical64_f64(u) == F(rn(u, 53), 2 ** 64)
+        assert O.mo_u01_canonical64_f32(u) == F(rn(u, 24), 2 ** 64)
+    assert O.mo_u01_u64_f64(orc.OO, 2 ** 64 - 1) < 1.0
+    assert O.mo_u01_u64_f64(orc.OC, 2 ** 64 - 1) == 1.0
+    assert O.mo_u01_u64_f64(orc.CC, 2 ** 64 - 1) == 1.0
+    assert O.mo_u01_u64_f32(orc.OO, 2 ** 64 - 1) < 1.0
+
+
+def _pre(e, pre, rbits):
+    """pre raw result words (u32 or u64) before the distribution call"""
+    if rbits == 64:
+        orc.rand_u64(e, pre)
+    else:
+        orc.rand_u32(e, pre)
+
+
 @needs_ref
+@pytest.mark.parametrize("rbits", [32, 64])
 @pytest.mark.parametrize("kind", [orc.PHILOX4X32, orc.THREEFRY4X64, orc.ARS])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("interval", [orc.CC, orc.CO, orc.OC, orc.OO,
                                       orc.CANONICAL])
-def test_u01_vs_reference(kind, dtype, interval):
+def test_u01_vs_reference(kind, dtype, interval, rbits):
     for pre, n in ((0, 5000), (3, 2049), (1, 1), (7, 0), (2, 3000)):
-        ref, rctr, ridx = orc.ref_u01(kind, [5], interval, dtype, n, pre)
-        e = orc.engine(kind, [5])
-        orc.rand_u32(e, pre)
+        ref, rctr, ridx = orc.ref_u01(kind, [5], interval, dtype, n, pre,
+                                      rbits)
+        e = orc.engine(kind, [5], rbits)
+        _pre(e, pre, rbits)
         got = orc.rand_u01(e, interval, n, dtype)
         assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
         assert orc.state(e) == (rctr, ridx)
@@ -61,16 +123,18 @@ UNIFORM_CASES = ((0, 5000, 0.0, 1.0), (3, 2049, -2.5, 7.25),
 
 
 @needs_ref
+@pytest.mark.parametrize("rbits", [32, 64])
 @pytest.mark.parametrize("kind", [orc.PHILOX4X32, orc.THREEFRY4X64, orc.ARS])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_uniform_real_vs_reference(kind, dtype):
+def test_uniform_real_vs_reference(kind, dtype, rbits):
     """UniformRealDistribution (uniform_real_distribution.hpp:48-95) bit-exact
     against the compiled reference: chunked U01CO, r * (b - a) + a rounded
     twice, engine state after the call."""
     for pre, n, a, b in UNIFORM_CASES:
-        ref, rctr, ridx = orc.ref_uniform_real(kind, [6], dtype, n, a, b, pre)
-        e = orc.engine(kind, [6])
-        orc.rand_u32(e, pre)
+        ref, rctr, ridx = orc.ref_uniform_real(kind, [6], dtype, n, a, b, pre,
+                                               rbits)
+        e = orc.engine(kind, [6], rbits)
+        _pre(e, pre, rbits)
         got = orc.rand_uniform_real(e, n, a, b, dtype)
         assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), \
             (pre, n, a, b)
@@ -80,17 +144,19 @@ def test_uniform_real_vs_reference(kind, dtype):
 
 
 @needs_ref
+@pytest.mark.parametrize("rbits", [32, 64])
 @pytest.mark.parametrize("kind", [orc.PHILOX4X32, orc.THREEFRY4X64, orc.ARS])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_normal_vs_reference(kind, dtype):
+def test_normal_vs_reference(kind, dtype, rbits):
     """Chunked Box-Muller (normal_distribution.hpp:50-101) bit-exact, including
     partial chunks, the odd tail and an unaligned start."""
     for pre, n, mean, sd in ((0, 8192, 0.0, 1.0), (1, 5001, 2.5, 0.3),
                              (0, 1, 0.0, 1.0), (3, 4097, 1.0, 2.0),
                              (0, 2048 * 3 + 7, 0.0, 1.0), (2, 1025, -1.0, 0.5)):
-        ref, rctr, ridx = orc.ref_normal(kind, [9], dtype, n, mean, sd, pre)
-        e = orc.engine(kind, [9])
-        orc.rand_u32(e, pre)
+        ref, rctr, ridx = orc.ref_normal(kind, [9], dtype, n, mean, sd, pre,
+                                         rbits)
+        e = orc.engine(kind, [9], rbits)
+        _pre(e, pre, rbits)
         got = orc.rand_normal(e, n, mean, sd, dtype)
         assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), (pre, n)
         assert orc.state(e) == (rctr, ridx)
