@@ -149,6 +149,14 @@ int mckl_hip_ars_kernel(
  * reference's bulk call advances it. */
 int mckl_hip_rand(mckl_hip_engine *e, size_t n, void *d_r, void *stream);
 
+/* rand(rng, UniformBitsDistribution<UIntType>, n, r) with UIntType of ubits
+ * = 16, 32 or 64 bits: random/uniform_bits_distribution.hpp:60-107.  Wider
+ * than the engine's result (u64 from a 32-bit engine): the result words
+ * themselves, little-endian, ubits/rbits per value; narrower: the low ubits
+ * of one result word per value, in chunks of BufferSize<UIntType>. */
+int mckl_hip_rand_uniform_bits(
+    mckl_hip_engine *e, int ubits, size_t n, void *d_r, void *stream);
+
 /* rand(rng, U01xxDistribution<T>, n, r): random/u01_distribution.hpp:79-142,
  * random/internal/u01_generic.hpp:48-198, random/internal/common.hpp:69-86
  * (chunking by BufferSize<T>).  32-bit engines take one u32 per value (two

@@ -18,7 +18,8 @@ __all__ = [
     "Philox4x32", "Philox4x32_64", "Threefry4x64", "Threefry4x64_64", "ARS",
     "ARS_64", "rand", "U01CCDistribution", "U01CODistribution",
     "U01OCDistribution", "U01OODistribution", "U01CanonicalDistribution",
-    "NormalDistribution", "UniformRealDistribution", "MCKLError",
+    "NormalDistribution", "UniformRealDistribution",
+    "UniformBitsDistribution", "MCKLError",
     "Particle", "ResampleEval", "SMCEstimator", "GenericSMCSampler",
 ]
 
@@ -255,6 +256,23 @@ class NormalDistribution:
             check(lib.mckl_hip_rand_normal_f32(
                 ctypes.byref(rng.state), n, self.mean, self.stddev, p,
                 _stream()))
+
+
+class UniformBitsDistribution:
+    """UniformBitsDistribution<UIntType> for UIntType of 16, 32 or 64 bits
+    (random/uniform_bits_distribution.hpp:120-210): the engine's result
+    words themselves when UIntType is at least as wide, else the low bits of
+    one result word per value."""
+
+    def __init__(self, bits=32):
+        if bits not in (16, 32, 64):
+            raise MCKLError("UniformBitsDistribution needs 16, 32 or 64 bits")
+        self.bits = int(bits)
+
+    def fill(self, rng, n, r):
+        p = _dev(r, n, self.bits // 8, "UniformBits output", False)
+        check(lib.mckl_hip_rand_uniform_bits(ctypes.byref(rng.state),
+                                             self.bits, n, p, _stream()))
 
 
 class UniformRealDistribution:

@@ -78,6 +78,7 @@ SIGNATURES = {
     "mckl_hip_threefry4x64_kernel": (_int, [_U64P, _sz, _vp, _U64P, _vp]),
     "mckl_hip_ars_kernel": (_int, [_U64P, _sz, _vp, _U32P, _vp]),
     "mckl_hip_rand": (_int, [_EP, _sz, _vp, _vp]),
+    "mckl_hip_rand_uniform_bits": (_int, [_EP, _int, _sz, _vp, _vp]),
     "mckl_hip_rand_u01_f32": (_int, [_EP, _int, _sz, _vp, _vp]),
     "mckl_hip_rand_u01_f64": (_int, [_EP, _int, _sz, _vp, _vp]),
     "mckl_hip_rand_normal_f32": (_int, [_EP, _sz, _flt, _flt, _vp, _vp]),

@@ -251,6 +251,32 @@ struct ConvUniformF32W {
     }
 };
 
+// UniformBits narrower than the result word: its low bits
+struct ConvLow16 {
+    using T = uint16_t;
+    static constexpr int Q = 1;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return (uint16_t)w[0];
+    }
+};
+struct ConvLow16W {
+    using T = uint16_t;
+    static constexpr int Q = 2;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return (uint16_t)w[0];
+    }
+};
+struct ConvLow32W {
+    using T = uint32_t;
+    static constexpr int Q = 2;
+    __device__ __forceinline__ T operator()(const uint32_t *w) const
+    {
+        return w[0];
+    }
+};
+
 // 16-byte vector stores through L2 (plain, not non-temporal): with 32 or 64
 // bytes per lane a store instruction covers only part of each cache line,
 // and L2 must merge the pieces of a line before it goes to HBM; measured on
@@ -1233,6 +1259,33 @@ int mckl_hip_rand(mckl_hip_engine *e, size_t n, void *d_r, void *stream)
             as_stream(stream));
     return element_fill(
         e, 1, n, static_cast<uint64_t *>(d_r), ConvU64(), 0, as_stream(stream));
+}
+
+int mckl_hip_rand_uniform_bits(
+    mckl_hip_engine *e, int ubits, size_t n, void *d_r, void *stream)
+{
+    int rc = check_engine(e);
+    if (rc)
+        return rc;
+    if (ubits != 16 && ubits != 32 && ubits != 64)
+        return set_error(MCKL_HIP_EINVAL,
+            "UniformBits needs 16, 32 or 64 bits, not %d", ubits);
+    // at least as wide as the result: the result words, reinterpreted
+    // (uniform_bits_distribution.hpp:72-80)
+    if (ubits >= e->rbits)
+        return mckl_hip_rand(e, n * (size_t)(ubits / e->rbits), d_r, stream);
+    // narrower: one result word per value, chunks of BufferSize<UIntType>
+    // = 8192 / sizeof(UIntType) result words (:82-97)
+    const uint64_t k = 8192 / (uint64_t)(ubits / 8);
+    hipStream_t st = as_stream(stream);
+    if (ubits == 16 && e->rbits == 32)
+        return element_fill(e, 1, n, static_cast<uint16_t *>(d_r),
+            ConvLow16(), k, st);
+    if (ubits == 16)
+        return element_fill(e, 1, n, static_cast<uint16_t *>(d_r),
+            ConvLow16W(), k, st);
+    return element_fill(e, 1, n, static_cast<uint32_t *>(d_r), ConvLow32W(),
+        k, st);
 }
 
 static int u01_f64_impl(mckl_hip_engine *e, size_t ns, int interval, size_t n,

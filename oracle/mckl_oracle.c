@@ -644,6 +644,48 @@ float mo_u01_canonical_f32(uint32_t u)
     return (float)u * 0x1p-32f + 0.0f;
 }
 
+/* uniform_bits_distribution<UIntType>(rng, n, r), ubits = 16, 32, 64:
+ * random/uniform_bits_distribution.hpp:60-107.  At least as wide as the
+ * result word: rand(rng, n * ubits / rbits, result words) into the same
+ * bytes (:72-80); narrower: chunks of BufferSize<UIntType> = 8192 /
+ * sizeof(UIntType) result words, each truncated to UIntType (:82-97). */
+void mo_rand_uniform_bits(mo_engine *e, int ubits, size_t n, void *r)
+{
+    if (ubits >= e->rbits) {
+        if (e->rbits == 32)
+            mo_rand_u32(e, n * (size_t)(ubits / 32), (uint32_t *)r);
+        else
+            mo_rand_u64(e, n, (uint64_t *)r);
+        return;
+    }
+    const size_t K = 8192 / (size_t)(ubits / 8);
+    uint8_t *out = (uint8_t *)r;
+    uint64_t s64[4096];
+    uint32_t s32[4096];
+    for (;;) {
+        /* one bulk call per chunk (the last one possibly empty) */
+        const size_t k = n < K ? n : K;
+        if (e->rbits == 32)
+            mo_rand_u32(e, k, s32);
+        else
+            mo_rand_u64(e, k, s64);
+        for (size_t i = 0; i < k; ++i) {
+            const uint64_t w = e->rbits == 32 ? (uint64_t)s32[i] : s64[i];
+            if (ubits == 16) {
+                const uint16_t v = (uint16_t)w;
+                memcpy(out + 2 * i, &v, 2);
+            } else {
+                const uint32_t v = (uint32_t)w;
+                memcpy(out + 4 * i, &v, 4);
+            }
+        }
+        out += k * (size_t)(ubits / 8);
+        n -= k;
+        if (k < K)
+            break;
+    }
+}
+
 /* The same conversions from one u64 word, the word type the reference
  * uses for every real type on a 64-bit-result engine
  * (U01UIntType, u01_distribution.hpp:105-113): W = 64 in

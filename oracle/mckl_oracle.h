@@ -99,6 +99,8 @@ double mo_u01_u32_f64(int interval, uint32_t u);
 float mo_u01_u32_f32(int interval, uint32_t u);
 double mo_u01_canonical_f64(uint32_t u0, uint32_t u1);
 float mo_u01_canonical_f32(uint32_t u);
+/* uniform_bits_distribution<UIntType>, ubits = 16, 32 or 64 */
+void mo_rand_uniform_bits(mo_engine *e, int ubits, size_t n, void *r);
 /* one u64 word per value (64-bit-result engines) */
 double mo_u01_u64_f64(int interval, uint64_t u);
 float mo_u01_u64_f32(int interval, uint64_t u);

@@ -317,6 +317,26 @@ def ref_u01(kind, key, interval, dtype, n, pre=0, rbits=32):
     return out[:n], list(ctr[:nc]), idx.value
 
 
+_UBITS = {16: np.uint16, 32: np.uint32, 64: np.uint64}
+
+
+def rand_uniform_bits(e, ubits, n):
+    out = np.zeros(max(n, 1), _UBITS[ubits])
+    O.mo_rand_uniform_bits(ctypes.byref(e), ubits, _sz(n), P(out))
+    return out[:n]
+
+
+def ref_uniform_bits(kind, key, ubits, n, pre=0, rbits=32):
+    out = np.zeros(n + 1, _UBITS[ubits])
+    k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))
+    ctr = (ctypes.c_uint64 * 4)()
+    idx = ctypes.c_uint()
+    REF.ref_uniform_bits(kind, rbits, k, ubits, _sz(pre), _sz(n), P(out), ctr,
+                         ctypes.byref(idx))
+    nc = 4 if kind == THREEFRY4X64 else 2
+    return out[:n], list(ctr[:nc]), idx.value
+
+
 def ref_normal(kind, key, dtype, n, mean, sd, pre=0, rbits=32):
     out = np.zeros(n + 1, dtype)
     k = (ctypes.c_uint64 * 4)(*(list(key) + [0] * (4 - len(key))))

@@ -15,6 +15,7 @@
 #include <mckl/core/weight.hpp>
 #include <mckl/random/aes.hpp>
 #include <mckl/random/normal_distribution.hpp>
+#include <mckl/random/uniform_bits_distribution.hpp>
 #include <mckl/random/philox.hpp>
 #include <mckl/random/threefry.hpp>
 #include <mckl/random/u01_distribution.hpp>
@@ -199,6 +200,32 @@ int ref_u01(int kind, int rbits, const std::uint64_t *key, int interval, int dbl
             run(static_cast<double *>(out));
         else
             run(static_cast<float *>(out));
+        get_state(rng, ctr, index);
+        return 0;
+    });
+}
+
+// pre raw result words, then rand(rng, UniformBitsDistribution<U>, n, r)
+// for U of ubits = 16, 32 or 64 bits
+int ref_uniform_bits(int kind, int rbits, const std::uint64_t *key,
+    int ubits, std::size_t pre, std::size_t n, void *out, std::uint64_t *ctr,
+    unsigned *index)
+{
+    return dispatch(kind, rbits, key, [&](auto &rng) {
+        using R = typename std::remove_reference_t<decltype(rng)>::result_type;
+        std::vector<R> skip(pre + 1);
+        mckl::rand(rng, pre, skip.data());
+        auto run = [&](auto *r) {
+            using U = std::remove_pointer_t<decltype(r)>;
+            mckl::UniformBitsDistribution<U> d;
+            mckl::rand(rng, d, n, r);
+        };
+        if (ubits == 16)
+            run(static_cast<std::uint16_t *>(out));
+        else if (ubits == 32)
+            run(static_cast<std::uint32_t *>(out));
+        else
+            run(static_cast<std::uint64_t *>(out));
         get_state(rng, ctr, index);
         return 0;
     });

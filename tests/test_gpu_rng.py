@@ -411,6 +411,58 @@ def test_config2_streams(dev, ename, kind, dname):
     assert 5.0 < zmax < 7.5
 
 
+@pytest.mark.parametrize("rbits", [32, 64])
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_uniform_bits_fills(dev, kind, rbits):
+    """rand(rng, UniformBitsDistribution<U>, n, r) for U of 16, 32 and 64
+    bits bit-exact against the oracle (pinned to the compiled reference),
+    and the reference's own determinism check
+    (example/random/include/random_rng.hpp:234-305): for random sizes K,
+    K scalar draws equal one bulk draw of K values, for the raw result and
+    all three widths."""
+    import torch
+    m = _m()
+    tdt = {16: torch.int16, 32: torch.int32, 64: torch.int64}
+    for ubits in (16, 32, 64):
+        for pre, n in ((0, 5000), (3, 4097), (1, 1), (2, 9001)):
+            key = [pre + 5, 6, 7, 8][:2 if kind == 1 else 4]
+            rng = _cls(kind, rbits)(key=key)
+            e = orc.engine(kind, key, rbits)
+            _predraw(m, rng, e, pre, rbits, dev)
+            out = torch.full((n,), -1, dtype=tdt[ubits], device=dev)
+            m.rand(rng, m.UniformBitsDistribution(ubits), n, out)
+            want = orc.rand_uniform_bits(e, ubits, n)
+            got = out.cpu().numpy().view(want.dtype)
+            assert np.array_equal(got, want), (ubits, pre, n)
+            assert rng.ctr() == orc.state(e)[0] and \
+                rng.index == orc.state(e)[1]
+    # scalar draws (engine operator()) against bulk draws, sizes from the
+    # engine as in random_rng_d
+    r1 = _cls(kind, rbits)(key=[9, 9, 9, 9][:2 if kind == 1 else 4])
+    r2 = r1.copy()
+    sizes = np.random.default_rng(kind * rbits).integers(0, 300, 12)
+    mask = (1 << rbits) - 1
+    for K in sizes:
+        K = int(K)
+        scal = [r1() & mask for _ in range(K)]
+        out = _tensor(K, rbits // 8, dev)
+        m.rand(r2, K, out)
+        assert [int(v) for v in _u(out)[:K]] == scal
+        for ubits in (16, 32, 64):
+            if ubits >= rbits:
+                per = ubits // rbits
+                words = [r1() & mask for _ in range(K * per)]
+                scal = [sum(words[i * per + j] << (rbits * j)
+                            for j in range(per)) for i in range(K)]
+            else:
+                scal = [(r1() & mask) & ((1 << ubits) - 1) for _ in range(K)]
+            out = torch.full((max(K, 1),), 0, dtype=tdt[ubits], device=dev)
+            m.rand(r2, m.UniformBitsDistribution(ubits), K, out)
+            a = out.cpu().numpy()[:K]
+            a = a.view({16: np.uint16, 32: np.uint32, 64: np.uint64}[ubits])
+            assert [int(v) for v in a] == scal, (K, ubits)
+
+
 def test_wrappers_check_tensors(dev):
     """Element size, element type, contiguity and length are checked before
     a launch, and the engine does not advance."""

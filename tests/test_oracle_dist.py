@@ -203,3 +203,44 @@ def test_normal_moments():
     assert abs(z.var() - 1) < 5 * np.sqrt(2.0 / n)
     # fourth moment of N(0,1) is 3
     assert abs((z ** 4).mean() - 3) < 0.05
+
+
+@needs_ref
+@pytest.mark.parametrize("rbits", [32, 64])
+@pytest.mark.parametrize("kind", [orc.PHILOX4X32, orc.THREEFRY4X64, orc.ARS])
+@pytest.mark.parametrize("ubits", [16, 32, 64])
+def test_uniform_bits_vs_reference(kind, rbits, ubits):
+    """rand(rng, UniformBitsDistribution<U>, n, r) for U of 16, 32, 64 bits
+    (uniform_bits_distribution.hpp:60-107) bit-exact against the compiled
+    reference, engine state included, across chunk boundaries (4096 u16 or
+    2048 u32 per chunk when narrower than the result word)."""
+    for pre, n in ((0, 5000), (3, 4097), (1, 1), (7, 0), (2, 9001)):
+        ref, rctr, ridx = orc.ref_uniform_bits(kind, [8], ubits, n, pre,
+                                               rbits)
+        e = orc.engine(kind, [8], rbits)
+        _pre(e, pre, rbits)
+        got = orc.rand_uniform_bits(e, ubits, n)
+        assert np.array_equal(got, ref), (pre, n)
+        assert orc.state(e) == (rctr, ridx)
+
+
+@pytest.mark.parametrize("rbits", [32, 64])
+def test_uniform_bits_closed_form(rbits):
+    """The words themselves: u64 from a 32-bit engine is two consecutive
+    result words, low first; narrower values are the low bits of one word
+    each (the reference's scalar UniformBits, :128-175, gives the same)."""
+    w = orc.rand_u32(orc.engine(orc.PHILOX4X32, [4], 32), 40) \
+        if rbits == 32 else \
+        orc.rand_u64(orc.engine(orc.PHILOX4X32, [4], 64), 20)
+    e = orc.engine(orc.PHILOX4X32, [4], rbits)
+    u16 = orc.rand_uniform_bits(e, 16, 20)
+    assert np.array_equal(u16, (w[:20] & 0xFFFF).astype(np.uint16))
+    if rbits == 32:
+        e = orc.engine(orc.PHILOX4X32, [4], 32)
+        u64 = orc.rand_uniform_bits(e, 64, 20)
+        want = w[0::2].astype(np.uint64) | (w[1::2].astype(np.uint64) << 32)
+        assert np.array_equal(u64, want)
+    else:
+        e = orc.engine(orc.PHILOX4X32, [4], 64)
+        u32 = orc.rand_uniform_bits(e, 32, 20)
+        assert np.array_equal(u32, (w & 0xFFFFFFFF).astype(np.uint32))
