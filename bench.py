@@ -293,24 +293,9 @@ def bench_smc_batched(m, torch):
     T = 50
     ps = float(Nf.astype(np.float64).sum()) * T
     bt = m.SMCBatch(m.MODEL_SV, Nf, theta, yb, seeds, 0.5, m.STRATIFIED)
-    best = None
-    tune_log = []
-    for block in (256, 512):
-        for wpe in (0, 2):
-            bt.set_tune(block, wpe)
-            bt.run(use_graph=True)
-            torch.cuda.synchronize()
-            a, b = _events()
-            a.record()
-            bt.run(use_graph=True)
-            b.record()
-            torch.cuda.synchronize()
-            tt = a.elapsed_time(b) / 1e3
-            tune_log.append({"block": block, "waves_per_eu": wpe,
-                             "ms": round(tt * 1e3, 3)})
-            if best is None or tt < best[0]:
-                best = (tt, block, wpe)
-    bt.set_tune(best[1], best[2])
+    # the library's autotuner (its candidates checked bit-identical); the
+    # chosen configuration is applied to the handle
+    tune, tune_ms = bt.autotune()
     bt.run(use_graph=True)
     torch.cuda.synchronize()
     a, b = _events()
@@ -335,15 +320,15 @@ def bench_smc_batched(m, torch):
     b.record()
     torch.cuda.synchronize()
     tp = a.elapsed_time(b) / 1e3
-    lp = np.array([sm.log_likelihood() for sm in samplers[:64]])
+    lp = np.array([sm.log_likelihood() for sm in samplers])
     res["smc_batched_sv_4096_T50"] = {
         "particle_steps": ps,
         "batched_particle_steps_per_s": ps / tb,
         "per_filter_particle_steps_per_s": ps / tp,
         "speedup_vs_per_filter": tp / tb,
         "batched_ms": round(tb * 1e3, 3), "per_filter_ms": round(tp * 1e3, 1),
-        "tune": {"block": best[1], "waves_per_eu": best[2], "sweep": tune_log},
-        "bit_identical_first64": bool(np.array_equal(lb[:64], lp)),
+        "tune": dict(tune, autotune_ms=round(tune_ms, 3)),
+        "bit_identical_all_4096": bool(np.array_equal(lb, lp)),
     }
     return res
 

@@ -336,9 +336,16 @@ int mckl_hip_smc_estimates(mckl_hip_smc *s, double *h_est);
  * All six schemes; at most 131072 particles per filter.
  */
 typedef struct mckl_hip_smc_batch mckl_hip_smc_batch;
+/* Launch configuration of a batched run.  No field changes a bit of the
+ * results.  split_tiles > 0 runs the filters of at least split_tiles
+ * 2048-particle tiles in a persistent launch of `block`-thread workgroups
+ * and the others concurrently in one of `small_block`-thread workgroups;
+ * 0 runs every filter in one launch of `block`. */
 typedef struct mckl_hip_tune {
-    int32_t block;       /* threads per workgroup */
-    int32_t waves_per_eu; /* occupancy hint: 0 = default */
+    int32_t block;        /* threads per workgroup: 256, 512 or 1024 */
+    int32_t waves_per_eu; /* occupancy cap: 0 = as many as fit */
+    int32_t split_tiles;  /* 0 = one launch */
+    int32_t small_block;  /* threads per workgroup of the small filters */
 } mckl_hip_tune;
 int mckl_hip_smc_batch_create(int model, size_t nfilters,
     const uint32_t *h_N, const double *h_theta, size_t T, const double *h_y,
@@ -346,6 +353,16 @@ int mckl_hip_smc_batch_create(int model, size_t nfilters,
     mckl_hip_smc_batch **out);
 int mckl_hip_smc_batch_set_tune(mckl_hip_smc_batch *b, const mckl_hip_tune *t);
 int mckl_hip_smc_batch_run(mckl_hip_smc_batch *b, int use_graph, void *stream);
+/* Autotuner (no reference counterpart): times a fixed list of launch
+ * configurations (workgroup width, occupancy cap, large/small split) with
+ * graph runs on `stream`, checks that each reproduces the first one's
+ * log-likelihoods bit for bit (MCKL_HIP_EHIP otherwise), and keeps the
+ * fastest.  With cache_path, a configuration already recorded there for the
+ * same workload signature (device, model, scheme, T, threshold, filter
+ * sizes) is taken without runs, and a new choice is appended (one JSON
+ * object per line).  best and best_ms (ms per run) may be NULL. */
+int mckl_hip_smc_batch_autotune(mckl_hip_smc_batch *b, const char *cache_path,
+    mckl_hip_tune *best, double *best_ms, void *stream);
 int mckl_hip_smc_batch_result(mckl_hip_smc_batch *b, double *h_loglik);
 /* New parameters theta[3f..3f+2] and seeds for every filter of an existing
  * batch (same model, sizes, observations, threshold and scheme); the next

@@ -8,6 +8,7 @@ in include/mckl_hip.h (libmckl_hip.so).  Names follow the reference:
 device memory; torch supplies allocation and the current HIP stream only.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -496,10 +497,14 @@ class ResampleIndex:
         check(lib.mckl_hip_resample_index_create(ctypes.byref(h)))
         self._h = h
 
+    # the library's destroy function, bound while the module is alive (at
+    # interpreter exit the module's globals may already be cleared)
+    _destroy = staticmethod(lib.mckl_hip_resample_index_destroy)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib.mckl_hip_resample_index_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def num_iter(self):
@@ -632,10 +637,14 @@ class SMCSampler:
         check(lib.mckl_hip_smc_estimates(self._h, _dptr(out)))
         return out[:, :k]
 
+    # the library's destroy function, bound while the module is alive (at
+    # interpreter exit the module's globals may already be cleared)
+    _destroy = staticmethod(lib.mckl_hip_smc_destroy)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib.mckl_hip_smc_destroy(h)
+            self._destroy(h)
             self._h = None
 
 
@@ -666,9 +675,27 @@ class SMCBatch:
             float(resample_threshold), scheme, ctypes.byref(h)))
         self._h = h
 
-    def set_tune(self, block=256, waves_per_eu=0):
-        t = _lib.Tune(block, waves_per_eu)
+    def set_tune(self, block=256, waves_per_eu=0, split_tiles=0,
+                 small_block=256):
+        """Launch configuration (changes no result bit): workgroup width,
+        occupancy cap, and with split_tiles > 0 a second concurrent launch
+        of small_block-thread workgroups for the filters of fewer tiles."""
+        t = _lib.Tune(block, waves_per_eu, split_tiles, small_block)
         check(lib.mckl_hip_smc_batch_set_tune(self._h, ctypes.byref(t)))
+
+    def autotune(self, cache=None):
+        """The library's autotuner: times its candidate configurations,
+        checks they give identical bits, keeps the fastest; with `cache` (a
+        file path) reuses or records the choice for this workload.  Returns
+        (dict of the configuration, ms per run)."""
+        t = _lib.Tune()
+        ms = ctypes.c_double()
+        path = None if cache is None else os.fsencode(cache)
+        check(lib.mckl_hip_smc_batch_autotune(self._h, path, ctypes.byref(t),
+                                              ctypes.byref(ms), _stream()))
+        return ({"block": t.block, "waves_per_eu": t.waves_per_eu,
+                 "split_tiles": t.split_tiles, "small_block": t.small_block},
+                ms.value)
 
     def run(self, use_graph=True):
         check(lib.mckl_hip_smc_batch_run(self._h, int(bool(use_graph)),
@@ -689,10 +716,14 @@ class SMCBatch:
             self._h, _dptr(th),
             sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
 
+    # the library's destroy function, bound while the module is alive (at
+    # interpreter exit the module's globals may already be cleared)
+    _destroy = staticmethod(lib.mckl_hip_smc_batch_destroy)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib.mckl_hip_smc_batch_destroy(h)
+            self._destroy(h)
             self._h = None
 
 
@@ -747,10 +778,14 @@ class SMCGradient:
                                            _stream()))
         return v
 
+    # the library's destroy function, bound while the module is alive (at
+    # interpreter exit the module's globals may already be cleared)
+    _destroy = staticmethod(lib.mckl_hip_smc_grad_destroy)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib.mckl_hip_smc_grad_destroy(h)
+            self._destroy(h)
             self._h = None
 
 

@@ -40,7 +40,9 @@ class Engine(ctypes.Structure):
 
 
 class Tune(ctypes.Structure):
-    _fields_ = [("block", ctypes.c_int32), ("waves_per_eu", ctypes.c_int32)]
+    _fields_ = [("block", ctypes.c_int32), ("waves_per_eu", ctypes.c_int32),
+                ("split_tiles", ctypes.c_int32),
+                ("small_block", ctypes.c_int32)]
 
 
 if not os.path.exists(LIB_PATH):
@@ -130,6 +132,9 @@ SIGNATURES = {
                ctypes.POINTER(_dbl), _U64P, _dbl, _int, ctypes.POINTER(_vp)]),
     "mckl_hip_smc_batch_set_tune": (_int, [_vp, ctypes.POINTER(Tune)]),
     "mckl_hip_smc_batch_run": (_int, [_vp, _int, _vp]),
+    "mckl_hip_smc_batch_autotune": (
+        _int, [_vp, ctypes.c_char_p, ctypes.POINTER(Tune),
+               ctypes.POINTER(_dbl), _vp]),
     "mckl_hip_smc_batch_result": (_int, [_vp, ctypes.POINTER(_dbl)]),
     "mckl_hip_smc_batch_destroy": (_int, [_vp]),
     "mckl_hip_smc_batch_set_params": (

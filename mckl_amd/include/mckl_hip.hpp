@@ -687,13 +687,29 @@ class SMCBatchHIP
     SMCBatchHIP &operator=(const SMCBatchHIP &) = delete;
 
     std::size_t size() const { return n_; }
-    /// workgroup size and occupancy cap of the persistent launch
-    void set_tune(int block, int waves_per_eu = 0)
+    /// launch configuration (changes no result bit): workgroup width,
+    /// occupancy cap, and with split_tiles > 0 a concurrent launch of
+    /// small_block-thread workgroups for the filters of fewer tiles
+    void set_tune(int block, int waves_per_eu = 0, int split_tiles = 0,
+        int small_block = 256)
     {
         mckl_hip_tune t;
         t.block = block;
         t.waves_per_eu = waves_per_eu;
+        t.split_tiles = split_tiles;
+        t.small_block = small_block;
         hip_check(mckl_hip_smc_batch_set_tune(b_, &t));
+    }
+    /// the library's autotuner (mckl_hip_smc_batch_autotune); cache_path
+    /// may be empty (no cache)
+    mckl_hip_tune autotune(const std::string &cache_path = std::string(),
+        double *ms = nullptr)
+    {
+        mckl_hip_tune t;
+        hip_check(mckl_hip_smc_batch_autotune(b_,
+            cache_path.empty() ? nullptr : cache_path.c_str(), &t, ms,
+            current_stream()));
+        return t;
     }
     /// new parameters and seeds for every filter (sizes, observations,
     /// threshold and scheme stay)

@@ -437,6 +437,11 @@ int main(int argc, char **argv)
             same = s.log_likelihood() == lb[f];
         }
         CHECK(same, "SMCBatchHIP == SMCSamplerHIP per filter");
+        double tms = 0.0;
+        const mckl_hip_tune tt = b.autotune(std::string(), &tms);
+        b.run();
+        CHECK(b.log_likelihood() == lb && tms > 0 && tt.block >= 256,
+            "SMCBatchHIP::autotune keeps the bits");
 
         const std::vector<double> th2{0.9, 1.0, 0.5, 0.85, 1.1, 0.45};
         mckl::SMCGradientHIP g(MCKL_HIP_MODEL_LG, 2, 2000, y.size(), y, false,

@@ -322,3 +322,88 @@ def test_results_wait_for_their_own_run(dev):
         b1.run()
         r1 = b1.log_likelihood()
     assert np.array_equal(r1, b0.log_likelihood())
+
+
+TUNE_CANDIDATES = [(256, 0, 0, 256), (256, 2, 0, 256), (512, 0, 0, 256),
+                   (512, 2, 0, 256), (1024, 0, 0, 256), (512, 0, 2, 256),
+                   (512, 0, 4, 256), (512, 0, 8, 256), (1024, 0, 4, 256),
+                   (1024, 0, 8, 256), (1024, 0, 16, 256), (1024, 0, 24, 256),
+                   (512, 2, 8, 256), (1024, 2, 8, 256), (256, 0, 2, 512),
+                   (1024, 0, 1, 1024)]
+
+
+def test_every_tuned_configuration_equals_plain(dev):
+    """Every configuration the autotuner tries (and a few more: a wide small
+    launch, every filter in the large launch) gives each filter the plain
+    sampler's bits, graph or not: 40 SV filters of 1 to 40 tiles."""
+    m = _m()
+    rng = np.random.default_rng(17)
+    nf = 40
+    N = np.concatenate([[3, 2048, 2049, 40 * 2048],
+                        rng.integers(256, 40 * 2048, nf - 4)]).astype(np.uint32)
+    theta = np.stack([rng.uniform(0.9, 0.99, nf), rng.uniform(0.1, 0.3, nf),
+                      rng.uniform(0.5, 1.0, nf)], 1)
+    y = rng.normal(size=(nf, 12)) * 0.8
+    seeds = np.arange(100, 100 + nf, dtype=np.uint64)
+    plain = np.array([m.smc_loglik(2, theta[f], y[f], int(N[f]),
+                                   int(seeds[f]), 0.5, STRAT)
+                      for f in range(nf)])
+    b = m.SMCBatch(2, N, theta, y, seeds, 0.5, STRAT)
+    for cand in TUNE_CANDIDATES:
+        b.set_tune(*cand)
+        for g in (False, True):
+            b.run(use_graph=g)
+            assert np.array_equal(b.log_likelihood(), plain), (cand, g)
+
+
+def test_autotune_and_cache(dev, tmp_path):
+    """The library's autotuner picks one of its configurations, checks the
+    bits of every one, records the choice, and a second batch of the same
+    workload takes it from the cache without timing runs."""
+    m = _m()
+    ob = _obs()
+    theta = ob["batch64_theta"]
+    N = ob["batch64_N"]
+    y = ob["batch64_y"]
+    seeds = np.arange(1, len(N) + 1, dtype=np.uint64)
+    cache = str(tmp_path / "tune.jsonl")
+    b = m.SMCBatch(2, N, theta, y, seeds, 0.5, STRAT)
+    tune, ms = b.autotune(cache)
+    assert tuple(tune.values()) in [tuple(c) for c in TUNE_CANDIDATES]
+    assert ms > 0
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 1 and '"block": %d' % tune["block"] in lines[0]
+    b.run()
+    lb = b.log_likelihood()
+    b2 = m.SMCBatch(2, N, theta, y, seeds, 0.5, STRAT)
+    tune2, ms2 = b2.autotune(cache)
+    assert tune2 == tune and ms2 == pytest.approx(ms, rel=1e-3)
+    assert len(open(cache).read().splitlines()) == 1
+    b2.run()
+    assert np.array_equal(b2.log_likelihood(), lb)
+    # another workload (other sizes) is not served by that record
+    b3 = m.SMCBatch(2, N[:10], theta[:10], y[:10], seeds[:10], 0.5, STRAT)
+    b3.autotune(cache)
+    assert len(open(cache).read().splitlines()) == 2
+
+
+def test_batched_config4_equals_plain_runs(dev):
+    """The batched configuration at full size -- 4096 stochastic-volatility
+    filters, N log-uniform in [256, 65536], T = 50, stratified -- as one
+    graph launch with the autotuned configuration, against one plain
+    sampler run per filter: all 4096 log-likelihoods bit-identical."""
+    m = _m()
+    from mckl_amd import workloads as wl
+    theta, Nf, yb, seeds = wl.sv_batch(4096, 50)
+    b = m.SMCBatch(m.MODEL_SV, Nf, theta, yb, seeds, 0.5, STRAT)
+    tune, ms = b.autotune()
+    print("autotuned %s: %.3f ms per run" % (tune, ms))
+    b.run(use_graph=True)
+    lb = b.log_likelihood()
+    lp = np.empty(len(Nf))
+    for f in range(len(Nf)):
+        s = m.SMCSampler(m.MODEL_SV, theta[f], yb[f], int(Nf[f]),
+                         int(seeds[f]), 0.5, STRAT)
+        s.iterate()
+        lp[f] = s.log_likelihood()
+    assert np.array_equal(lb, lp), int((lb != lp).sum())
