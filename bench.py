@@ -126,7 +126,20 @@ def bench_saturation(m, torch, reps=10):
     return res
 
 
+# VALU lane instructions per Normal sample of normal_kernel, from its
+# rocprofv3 counters (SQ_INSTS_VALU x 64 / samples per dispatch), and the
+# VALU roof: one wave instruction per cycle per CU, 256 CUs at 2.4 GHz
+NORMAL_VALU_PER_SAMPLE = {("threefry4x64", "f32"): 139.1,
+                          ("threefry4x64", "f64"): 333.6,
+                          ("ars", "f32"): 156.3, ("ars", "f64"): 343.9}
+VALU_PEAK_LANE_OPS = 256 * 2.4e9 * 64
+
+
 def bench_normals(m, torch, reps=3):
+    """Config-2 fills: 64 streams (keys {s,0,0,0}, s = 1..64) of 2^22
+    Normal samples each, per engine and precision; samples per second, HBM
+    bytes written, and the fraction of the VALU roof that bounds them (the
+    profiled instructions per sample times the rate)."""
     n = 1 << 22
     res = {}
     for ename, cls in (("threefry4x64", m.Threefry4x64), ("ars", m.ARS)):
@@ -143,10 +156,17 @@ def bench_normals(m, torch, reps=3):
             b.record()
             torch.cuda.synchronize()
             t = a.elapsed_time(b) / reps / 1e3
+            vps = NORMAL_VALU_PER_SAMPLE[(ename, dname)]
             res["normal_%s_%s_2^28" % (ename, dname)] = {
                 "Gsamples_per_s": round(64 * n / t / 1e9, 2),
                 "GBps": round(out.element_size() * 64 * n / t / 1e9, 1),
-                "ms": round(t * 1e3, 3)}
+                "ms": round(t * 1e3, 3),
+                "roofline": {"bound": "valu",
+                             "valu_lane_instr_per_sample": vps,
+                             "frac": round(64 * n / t * vps /
+                                           VALU_PEAK_LANE_OPS, 3),
+                             "hbm_frac": round(out.element_size() * 64 * n /
+                                               t / 1e9 / HBM_PEAK_GBS, 3)}}
             del out
     return res
 
@@ -236,7 +256,8 @@ def bench_smc_generic(m, torch):
 def bench_resample(m, torch):
     """ResampleAlgorithm + rep -> index on the device (mckl_hip_resample) at
     N = M = 2^20 on normalised weights exp(N(0, 1)) (a typical SMC step):
-    microseconds per call and particles per second, per scheme."""
+    microseconds per call and particles per second, per scheme, and the
+    count of boundary cases among the first call's uniforms."""
     N = 1 << 20
     g = torch.Generator(device="cuda").manual_seed(3)
     w = torch.randn(N, dtype=torch.float64, device="cuda", generator=g).exp()
@@ -244,10 +265,12 @@ def bench_resample(m, torch):
     rep = torch.empty(N, dtype=torch.int32, device="cuda")
     idx = torch.empty(N, dtype=torch.int32, device="cuda")
     out = {}
-    for name, alg in (("multinomial", m.ResampleMultinomial()),
-                      ("stratified", m.ResampleStratified()),
-                      ("systematic", m.ResampleSystematic()),
-                      ("residual_systematic", m.ResampleResidualSystematic())):
+    for name, alg, scheme in (
+            ("multinomial", m.ResampleMultinomial(), m.MULTINOMIAL),
+            ("stratified", m.ResampleStratified(), m.STRATIFIED),
+            ("systematic", m.ResampleSystematic(), m.SYSTEMATIC),
+            ("residual_systematic", m.ResampleResidualSystematic(),
+             m.RESIDUAL_SYSTEMATIC)):
         rng = m.Philox4x32(9)
         for _ in range(3):
             alg(N, N, rng, w, rep, idx)
@@ -261,7 +284,43 @@ def bench_resample(m, torch):
         torch.cuda.synchronize()
         us = a.elapsed_time(b) / reps * 1e3
         out[name] = {"us": round(us, 1), "particles_per_s": N / (us * 1e-6)}
+        out[name]["boundary_cases"] = _boundary_cases(m, torch, scheme, w, N)
     return {"resample_N2^20": out}
+
+
+def _boundary_cases(m, torch, scheme, w, M):
+    """Uniforms of the first call (a fresh Philox4x32(9)) that lie within
+    1e-15, and within N eps, relative of a cumulative-weight boundary: the
+    only places where a resampled ancestor depends on the rounding of the
+    running weight sum, so where the reference's own answer is not fixed by
+    the weights.  Boundaries are torch.cumsum in FP64 of the weights (the
+    residual schemes: of the residual weights, with M - sum floor(M w)
+    uniforms)."""
+    if scheme >= m.RESIDUAL:
+        mw = w * M
+        fl = torch.floor(mw)
+        r = mw - fl
+        w = r / r.sum()
+        M = M - int(fl.sum().item())
+        seq = scheme - m.RESIDUAL if scheme > m.RESIDUAL else m.MULTINOMIAL
+    else:
+        seq = scheme
+    N = w.numel()
+    if M <= 0:
+        return {"uniforms": 0, "within_1e-15": 0, "within_N_eps": 0}
+    u = torch.empty(M, dtype=torch.float64, device="cuda")
+    m.u01_rand_sequence(seq, m.Philox4x32(9), M, u)
+    cw = torch.cumsum(w, 0)
+    k = torch.searchsorted(cw, u).clamp(max=N - 1)
+    hi = cw[k]
+    lo = cw[(k - 1).clamp(min=0)]
+    d = (hi - u).abs() / hi
+    dl = torch.where(k > 0, (u - lo).abs() / lo.clamp(min=1e-300),
+                     torch.full_like(u, 1.0))
+    d = torch.minimum(d, dl)
+    eps = float(torch.finfo(torch.float64).eps)
+    return {"uniforms": M, "within_1e-15": int((d <= 1e-15).sum().item()),
+            "within_N_eps": int((d <= N * eps).sum().item())}
 
 
 def bench_smc_batched_run(m, torch):
@@ -484,38 +543,54 @@ def _guarded(fn, *a):
 
 def cpu_reference_points(threads):
     """CPU figures beside the secondary GPU legs, on this host: the
-    reference compiled from its headers (Normal fills) and the oracle port
-    (SMC; the reference has no counterpart of these samplers).  Bounded
-    samples: 2^22 normals, the plain configuration over its first 10 steps,
+    reference compiled from its headers (fills, Normal streams, the
+    ResampleIndex trace-back), in its pinned build (-O2, SSE2) and its speed
+    build (-O3, x86-64-v4 + AES-NI) where the host has those features, on one
+    thread and on `threads`; and the oracle port for the SMC samplers (the
+    reference has no counterpart of the fused samplers).  Bounded samples:
+    the config-1 step, 2^22 normals per stream-set on one thread and the full
+    64 x 2^22 on `threads`, the plain configuration over its first 10 steps,
     the first 64 filters of the batched configuration, the trace-back of 64
     iterations of 2^20 particles."""
     import numpy as np
     from mckl_amd import workloads as wl
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    model = "unknown"
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    res = {"threads": threads, "nproc": os.cpu_count(), "cpu_model": model,
+    res = {"threads": threads, "nproc": os.cpu_count(), "lscpu": host_cpu(),
            "build": "oracle port: gcc -O2 -ffp-contract=off -fopenmp (its "
-                    "pinned flags); reference: g++ -O2 -ffp-contract=off, "
-                    "no -march"}
-    if orc.REF is not None:
-        n = 1 << 22
-        for ename, kind, dname, dt, mean, sd in (
-                ("threefry4x64", orc.THREEFRY4X64, "f32", np.float32, 0.0, 1.0),
-                ("ars", orc.ARS, "f64", np.float64, 2.5, 0.3)):
-            t0 = time.perf_counter()
-            orc.ref_normal(kind, [1, 0, 0, 0], dt, n, mean, sd)
-            t = time.perf_counter() - t0
-            res["normal_%s_%s_reference_1thread_Gsamples_per_s" % (
-                ename, dname)] = round(n / t / 1e9, 4)
+                    "pinned flags); reference pinned: g++ -O2 "
+                    "-ffp-contract=off, no -march; reference speed: g++ -O3 "
+                    "-march=x86-64-v4 -maes (%s)" % (
+                        "loaded" if orc.REF_V4 is not None else
+                        "not loaded: host lacks AVX-512/AES-NI or not built")}
+    builds = [(nm, lib) for nm, lib in (("pinned", orc.REF),
+                                         ("speed", orc.REF_V4))
+              if lib is not None]
+    n = 1 << 24
+    u32 = np.empty(n, np.uint32)
+    f64 = np.empty(n, np.float64)
+    key = (ctypes.c_uint64 * 4)(*(KEY1 + [0, 0]))
+    for bname, lib in builds:
+        for nt in (1, threads):
+            t = lib.ref_time_philox_u01_mt(key, ctypes.c_size_t(n), 5, nt,
+                                           orc.P(u32), orc.P(f64))
+            res["config1_step_reference_%s_%dthreads_GBps" % (bname, nt)] = \
+                round(12 * n / t / 1e9, 3)
+    del u32, f64
+    for bname, lib in builds:
+        for ename, kind, dname, dbl, mean, sd in (
+                ("threefry4x64", orc.THREEFRY4X64, "f32", 0, 0.0, 1.0),
+                ("threefry4x64", orc.THREEFRY4X64, "f64", 1, 2.5, 0.3),
+                ("ars", orc.ARS, "f32", 0, 0.0, 1.0),
+                ("ars", orc.ARS, "f64", 1, 2.5, 0.3)):
+            for nt, ns in ((1, 1), (threads, 64)):
+                n = 1 << 22
+                out = np.empty(ns * n, np.float64 if dbl else np.float32)
+                t = lib.ref_time_normal_streams(kind, dbl, ns, n, mean, sd, nt,
+                                                orc.P(out))
+                del out
+                res["normal_%s_%s_reference_%s_%dthreads_Gsamples_per_s" % (
+                    ename, dname, bname, nt)] = round(ns * n / t / 1e9, 4)
     y = wl.lg_observations(100, 7)[:10]
     N = 1 << 20
     for nt in (1, threads):
@@ -555,47 +630,96 @@ def cpu_reference_points(threads):
     return res
 
 
+def host_threads():
+    """CPU threads this process may use: its affinity mask, at most 16 (the
+    host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def host_cpu():
+    """lscpu's description of the host CPU (model name, family/model/
+    stepping, topology, caches, SIMD flags)."""
+    info = {}
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True,
+                             timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Flags":
+                fl = set(v.split())
+                info["simd_flags"] = " ".join(
+                    f for f in ("sse4_2", "avx2", "fma", "bmi2", "aes", "vaes",
+                                "avx512f", "avx512bw", "avx512vl",
+                                "avx512dq", "avx512_vnni") if f in fl)
+            elif k in ("Model name", "Vendor ID", "CPU family", "Model",
+                       "Stepping", "CPU(s)", "Thread(s) per core",
+                       "Core(s) per socket", "Socket(s)", "L2 cache",
+                       "L3 cache", "CPU max MHz", "Hypervisor vendor",
+                       "NUMA node(s)"):
+                info[k] = v
+    except (OSError, ValueError):
+        pass
+    # a hypervisor may shorten the model name (to "AMD EPYC"); the family
+    # number still names the core generation
+    if info.get("Vendor ID") == "AuthenticAMD":
+        fam = info.get("CPU family", "")
+        info["core_generation"] = {"23": "Zen 1/2", "25": "Zen 3/4",
+                                   "26": "Zen 5"}.get(fam, "family " + fam)
+    return info
+
+
 def cpu_baseline(seconds=10.0):
-    """The reference's own bulk fills (compiled from its headers), one
-    thread; else the oracle port.  Sample: the full config-1 step, repeated
-    for about `seconds` of CPU time (at least 10 runs), median per step."""
+    """The reference's own bulk fills, compiled from its headers: its speed
+    build (-O3, x86-64-v4 + AES-NI, so its AVX-512 Philox and U01 paths) where
+    the host has those features, else its pinned build, on every host thread
+    allotted (at most 16), the step split by counter range (each thread's
+    engines moved to its range by discard; the outputs equal the
+    single-thread ones).  Else the oracle port on one thread.  Sample: the
+    full config-1 step, repeated for about `seconds` (at least 10 runs),
+    median per step."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     n = 1 << 24
     u32 = np.empty(n, np.uint32)
     f64 = np.empty(n, np.float64)
-    if orc.REF is not None:
+    lib = orc.REF_V4 if orc.REF_V4 is not None else orc.REF
+    threads = host_threads()
+    if lib is not None:
         key = (ctypes.c_uint64 * 4)(*(KEY1 + [0, 0]))
-        t1 = orc.REF.ref_time_philox_u01(key, ctypes.c_size_t(n), 1,
-                                         orc.P(u32), orc.P(f64))
-    else:
+        t1 = lib.ref_time_philox_u01_mt(key, ctypes.c_size_t(n), 3, threads,
+                                        orc.P(u32), orc.P(f64))
+        steps = int(min(20000, max(10, seconds / max(t1, 1e-6))))
+        t = lib.ref_time_philox_u01_mt(key, ctypes.c_size_t(n), steps,
+                                       threads, orc.P(u32), orc.P(f64))
+        build = ("g++ -O3 -march=x86-64-v4 -maes (AVX-512, AES-NI)"
+                 if lib is orc.REF_V4 else "g++ -O2 -ffp-contract=off")
+        return {"value": round(12 * n / t / 1e9, 3), "unit": "GB/s",
+                "cores": threads, "kind": "reference",
+                "sample": "full step: 2^24 u32 + 2^24 U01OO doubles, "
+                          "Philox4x32, reference built %s, %d threads split "
+                          "by counter range, median of %d runs"
+                          % (build, threads, steps)}
+    ts = []
+    for _ in range(10):
         t0 = time.perf_counter()
         e = orc.engine(orc.PHILOX4X32, KEY1)
         orc.O.mo_rand_u32(ctypes.byref(e), ctypes.c_size_t(n), orc.P(u32))
-        t1 = time.perf_counter() - t0
-    steps = int(min(1000, max(10, seconds / max(t1, 1e-6))))
-    if orc.REF is not None:
-        key = (ctypes.c_uint64 * 4)(*(KEY1 + [0, 0]))
-        t = orc.REF.ref_time_philox_u01(key, ctypes.c_size_t(n), steps,
-                                        orc.P(u32), orc.P(f64))
-        kind = "reference"
-    else:
-        ts = []
-        for _ in range(steps):
-            t0 = time.perf_counter()
-            e = orc.engine(orc.PHILOX4X32, KEY1)
-            orc.O.mo_rand_u32(ctypes.byref(e), ctypes.c_size_t(n), orc.P(u32))
-            e = orc.engine(orc.PHILOX4X32, KEY1)
-            orc.O.mo_rand_u01_f64(ctypes.byref(e), 3, ctypes.c_size_t(n),
-                                  orc.P(f64))
-            ts.append(time.perf_counter() - t0)
-        t = sorted(ts)[len(ts) // 2]
-        kind = "port"
+        e = orc.engine(orc.PHILOX4X32, KEY1)
+        orc.O.mo_rand_u01_f64(ctypes.byref(e), 3, ctypes.c_size_t(n),
+                              orc.P(f64))
+        ts.append(time.perf_counter() - t0)
+    t = sorted(ts)[len(ts) // 2]
     return {"value": round(12 * n / t / 1e9, 3), "unit": "GB/s", "cores": 1,
-            "kind": kind,
+            "kind": "port",
             "sample": "full step: 2^24 u32 + 2^24 U01OO doubles, Philox4x32, "
-                      "median of %d runs, single thread" % steps}
+                      "oracle port, median of 10 runs, single thread"}
 
 
 def main():
@@ -651,7 +775,7 @@ def main():
                  "u01oo_f64_fill_GBps": round(u01_gbs, 1),
                  "u01oo_f64_fill_ms": round(c1["u01_ms"], 4)}
     if not args.no_secondary:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_threads()
         for name, leg in (("saturation", bench_saturation),
                           ("normals", bench_normals), ("smc", bench_smc),
                           ("grad", bench_grad),

@@ -563,8 +563,10 @@ __device__ __forceinline__ void value_words(const Gen &g, uint64_t skip,
 // of a 64-bit-result engine), V = W32 / Q values per block.  Task (c, g):
 // pairs g*V .. g*V+V-1 of chunk c.  With skip == 0 the s-words and
 // theta-words of a task are exactly one block each; otherwise each window
-// straddles two blocks (skip counts u32 words).
-template <class Gen, typename T, int Q>
+// straddles two blocks (skip counts u32 words).  Straddle instantiates the
+// second form; a launch whose streams all have skip == 0 uses the first,
+// which then holds no runtime-indexed word arrays (those live in scratch).
+template <class Gen, typename T, int Q, bool Straddle>
 __global__ __launch_bounds__(kBlock) void normal_kernel(FillBatch batch,
     uint64_t nchunks, uint64_t n, T mean, T sd, T *out)
 {
@@ -589,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void normal_kernel(FillBatch batch,
         const uint64_t bs = c * (K / V) + gi;  // block of first s-word
         const uint64_t bt = bs + H / V;        // block of first theta-word
         uint32_t ws[W], wt[W];
-        if (skip == 0) {
+        if constexpr (!Straddle) {
             g.block(bs, ws);
             g.block(bt, wt);
         } else {
@@ -845,14 +847,23 @@ static int launch_normal(const std::vector<FillArg> &args, int Q, uint64_t n,
                     std::max<size_t>(1, cnt));
             const uint32_t gx = (uint32_t)std::max<uint64_t>(
                 1, std::min<uint64_t>(ceil_div(ntask, kBlock), cap));
-            if (Q == 2)
-                normal_kernel<Gen, T, 2>
-                    <<<dim3(gx, (uint32_t)cnt), kBlock, 0, st>>>(
-                        batch, nchunks, n, mean, sd, out + b0 * n);
+            bool straddle = false;
+            for (size_t i = 0; i < cnt; ++i)
+                straddle = straddle || batch.a[i].skip != 0;
+            const dim3 grid(gx, (uint32_t)cnt);
+            T *o = out + b0 * n;
+            if (Q == 2 && straddle)
+                normal_kernel<Gen, T, 2, true><<<grid, kBlock, 0, st>>>(
+                    batch, nchunks, n, mean, sd, o);
+            else if (Q == 2)
+                normal_kernel<Gen, T, 2, false><<<grid, kBlock, 0, st>>>(
+                    batch, nchunks, n, mean, sd, o);
+            else if (straddle)
+                normal_kernel<Gen, T, 1, true><<<grid, kBlock, 0, st>>>(
+                    batch, nchunks, n, mean, sd, o);
             else
-                normal_kernel<Gen, T, 1>
-                    <<<dim3(gx, (uint32_t)cnt), kBlock, 0, st>>>(
-                        batch, nchunks, n, mean, sd, out + b0 * n);
+                normal_kernel<Gen, T, 1, false><<<grid, kBlock, 0, st>>>(
+                    batch, nchunks, n, mean, sd, o);
             MCKL_HIP_CHECK_LAUNCH("normal_kernel");
         }
         if (n % K != 0) {

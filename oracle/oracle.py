@@ -6,6 +6,10 @@
 ``REF_SMC`` -- oracle/_ref/libmckl_ref_smc.so, the reference's SMCSampler
            behind ref_smc.cpp (lazy binding: its BLAS symbol is never
            called), when built
+``REF_V4`` -- oracle/_ref/libmckl_ref_v4.so, the same driver as ``REF``
+           built -O3 for x86-64-v4 with AES-NI (the reference's AVX-512 and
+           AES-NI paths), loaded only on a host that has those features;
+           timing baselines only
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
 """
@@ -18,6 +22,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_PATH = os.path.join(HERE, "libmckl_oracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libmckl_ref.so")
 REF_SMC_PATH = os.path.join(HERE, "_ref", "libmckl_ref_smc.so")
+REF_V4_PATH = os.path.join(HERE, "_ref", "libmckl_ref_v4.so")
+V4_FLAGS = ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl", "avx2",
+            "bmi2", "fma", "aes")
 
 PHILOX4X32, THREEFRY4X64, ARS = 1, 2, 3
 CC, CO, OC, OO, CANONICAL = 0, 1, 2, 3, 4
@@ -29,6 +36,23 @@ O = ctypes.CDLL(ORACLE_PATH)
 REF = ctypes.CDLL(REF_PATH) if os.path.exists(REF_PATH) else None
 REF_SMC = (ctypes.CDLL(REF_SMC_PATH, mode=os.RTLD_LAZY)
            if os.path.exists(REF_SMC_PATH) else None)
+
+
+def cpu_flags():
+    """The feature flags of this host's first CPU (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+REF_V4 = (ctypes.CDLL(REF_V4_PATH)
+          if os.path.exists(REF_V4_PATH) and
+          all(f in cpu_flags() for f in V4_FLAGS) else None)
 
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
@@ -58,8 +82,14 @@ O.mo_u01_u32_f64.restype = ctypes.c_double
 O.mo_u01_u32_f32.restype = ctypes.c_float
 O.mo_u01_canonical_f64.restype = ctypes.c_double
 O.mo_u01_canonical_f32.restype = ctypes.c_float
-if REF is not None:
-    REF.ref_time_philox_u01.restype = ctypes.c_double
+for _lib in (REF, REF_V4):
+    if _lib is not None:
+        _lib.ref_time_philox_u01.restype = ctypes.c_double
+        _lib.ref_time_philox_u01_mt.restype = ctypes.c_double
+        _lib.ref_time_normal_streams.restype = ctypes.c_double
+        _lib.ref_time_normal_streams.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, _sz, ctypes.c_double,
+            ctypes.c_double, ctypes.c_int, _vp]
 
 
 def engine(kind, key, rbits=32):

@@ -26,6 +26,7 @@
 #include <cstring>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -445,6 +446,91 @@ double ref_time_philox_u01(const std::uint64_t *key, std::size_t n, int reps,
     }
     std::sort(t.begin(), t.end());
     return t[t.size() / 2];
+}
+
+// The same step split over nthreads threads.  Thread i fills elements
+// [lo, hi) of both outputs from its own engines, each moved to element lo by
+// discard (one 32-bit result word per value: U01 doubles from a 32-bit engine
+// take one word each, u01_distribution.hpp:95-113); lo is a multiple of
+// 8, so every thread starts on a counter-block boundary and the outputs equal
+// the single-thread ones.  Returns seconds per rep (median), thread start-up
+// included.
+double ref_time_philox_u01_mt(const std::uint64_t *key, std::size_t n,
+    int reps, int nthreads, std::uint32_t *u32, double *f64)
+{
+    const std::size_t nt = static_cast<std::size_t>(nthreads < 1 ? 1 : nthreads);
+    const std::size_t chunk = ((n + nt - 1) / nt + 7) / 8 * 8;
+    std::vector<double> t;
+    for (int k = 0; k < reps; ++k) {
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<std::thread> th;
+        for (std::size_t i = 0; i < nt; ++i) {
+            const std::size_t lo = std::min(n, i * chunk);
+            const std::size_t hi = std::min(n, lo + chunk);
+            th.emplace_back([=]() {
+                mckl::Philox4x32 a;
+                set_key(a, key);
+                a.discard(lo);
+                mckl::rand(a, hi - lo, u32 + lo);
+                mckl::Philox4x32 b;
+                set_key(b, key);
+                b.discard(lo);
+                mckl::U01OODistribution<double> d;
+                mckl::rand(b, d, hi - lo, f64 + lo);
+            });
+        }
+        for (auto &x : th)
+            x.join();
+        auto t1 = std::chrono::steady_clock::now();
+        t.push_back(std::chrono::duration<double>(t1 - t0).count());
+    }
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+}
+
+// Independent Normal streams on nthreads threads: stream s (key {s + 1, 0,
+// 0, 0}, 32-bit results) fills out[s * n, (s + 1) * n) with rand(rng,
+// NormalDistribution<T>(mean, sd), n, r); thread i takes streams i, i +
+// nthreads, ...  Returns wall seconds for the whole pass, or -1 on a bad
+// engine kind.
+double ref_time_normal_streams(int kind, int dbl, int nstreams, std::size_t n,
+    double mean, double sd, int nthreads, void *out)
+{
+    const int nt = nthreads < 1 ? 1 : nthreads;
+    std::vector<int> rc(static_cast<std::size_t>(nt), 0);
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; ++i) {
+        th.emplace_back([=, &rc]() {
+            for (int s = i; s < nstreams; s += nt) {
+                const std::uint64_t key[4] = {
+                    static_cast<std::uint64_t>(s + 1), 0, 0, 0};
+                const std::size_t off = static_cast<std::size_t>(s) * n;
+                rc[static_cast<std::size_t>(i)] |=
+                    dispatch(kind, 32, key, [&](auto &rng) {
+                        if (dbl) {
+                            mckl::NormalDistribution<double> d(mean, sd);
+                            mckl::rand(rng, d, n,
+                                static_cast<double *>(out) + off);
+                        } else {
+                            mckl::NormalDistribution<float> d(
+                                static_cast<float>(mean),
+                                static_cast<float>(sd));
+                            mckl::rand(rng, d, n,
+                                static_cast<float *>(out) + off);
+                        }
+                        return 0;
+                    });
+            }
+        });
+    }
+    for (auto &x : th)
+        x.join();
+    auto t1 = std::chrono::steady_clock::now();
+    for (int r : rc)
+        if (r != 0)
+            return -1.0;
+    return std::chrono::duration<double>(t1 - t0).count();
 }
 
 } // extern "C"

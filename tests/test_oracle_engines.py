@@ -145,3 +145,60 @@ def test_engine_text_fixture():
                 else:
                     orc.rand_u64(e, 1)
         assert orc.engine_text(e) == c["text"], c
+
+
+def _baseline_step(lib, n, nthreads):
+    key = (ctypes.c_uint64 * 4)(0x0123456789ABCDEF, 0, 0, 0)
+    u32 = np.zeros(n, np.uint32)
+    f64 = np.zeros(n, np.float64)
+    if nthreads is None:
+        lib.ref_time_philox_u01(key, ctypes.c_size_t(n), 1, orc.P(u32),
+                                orc.P(f64))
+    else:
+        lib.ref_time_philox_u01_mt(key, ctypes.c_size_t(n), 1, nthreads,
+                                   orc.P(u32), orc.P(f64))
+    return u32, f64
+
+
+@needs_ref
+@pytest.mark.parametrize("n,nthreads", [(1 << 16, 4), (1000, 3), (37, 8)])
+def test_baseline_threads_match_single(n, nthreads):
+    """The CPU baseline's threaded split of the full step (raw u32 fill plus
+    U01OO doubles, each thread moved to its range by discard) writes the
+    single-thread outputs, and the speed build (AVX-512 / AES-NI paths of the
+    reference) writes the pinned build's."""
+    u1, f1 = _baseline_step(orc.REF, n, None)
+    libs = [orc.REF] + ([orc.REF_V4] if orc.REF_V4 is not None else [])
+    for lib in libs:
+        u, f = _baseline_step(lib, n, nthreads)
+        assert np.array_equal(u, u1)
+        assert np.array_equal(f.view(np.uint64), f1.view(np.uint64))
+    e = orc.engine(orc.PHILOX4X32, [0x0123456789ABCDEF])
+    assert np.array_equal(u1, orc.rand_u32(e, n))
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", [orc.THREEFRY4X64, orc.ARS])
+@pytest.mark.parametrize("dbl", [0, 1])
+def test_baseline_normal_streams(kind, dbl):
+    """The threaded Normal-streams baseline fills stream s with the
+    reference's rand(NormalDistribution) from key {s + 1, 0, 0, 0}; the speed
+    build agrees with the pinned one to a few ulp (its log/sin/cos and
+    contraction may differ)."""
+    ns, n = 5, 3001
+    dt = np.float64 if dbl else np.float32
+    mean, sd = (2.5, 0.3) if dbl else (0.0, 1.0)
+    outs = []
+    libs = [orc.REF] + ([orc.REF_V4] if orc.REF_V4 is not None else [])
+    for lib in libs:
+        out = np.zeros(ns * n, dt)
+        t = lib.ref_time_normal_streams(kind, dbl, ns, n, mean, sd, 3,
+                                        orc.P(out))
+        assert t >= 0
+        outs.append(out)
+    for s in range(ns):
+        want = orc.ref_normal(kind, [s + 1, 0, 0, 0], dt, n, mean, sd)[0]
+        assert np.array_equal(outs[0][s * n:(s + 1) * n], want)
+    if len(outs) > 1:
+        a, b = outs
+        assert np.allclose(a, b, rtol=8 * np.finfo(dt).eps, atol=0)
