@@ -323,6 +323,34 @@ def _boundary_cases(m, torch, scheme, w, M):
             "within_N_eps": int((d <= N * eps).sum().item())}
 
 
+def bench_pf_example(m, torch):
+    """The reference's own particle-filter example (example/algorithm/
+    include/algorithm_pf.hpp: 4-D state, Student-t weights through log1p,
+    Normal<double> moves, Stratified resampling at 0.5) at its own N =
+    100000 over its 100 observations, on the device API
+    (tests/cpp/algorithm_pf, a separate process): the example's timed
+    second run (clear(), then iterate), host clock around iterate."""
+    prog = os.path.join(ROOT, "tests", "cpp", "algorithm_pf")
+    if not os.path.exists(prog):
+        return {"pf_example_N1e5_T100": "tests/cpp/algorithm_pf not built"}
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run(
+            [prog, os.path.join(ROOT, "tests", "golden", "algorithm_pf.data"),
+             "100000", "1", "1000", os.path.join(d, "pf.bin"), "time"],
+            capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError("algorithm_pf failed: " + r.stderr[-500:])
+    sec = float(r.stdout.split("iterate_seconds")[1].split()[0])
+    return {"pf_example_N1e5_T100": {
+        "seconds": round(sec, 4),
+        "particle_steps_per_s": 1e5 * 100 / sec,
+        "note": "generic SMCSampler<double> (ColMajor 4-column state) with "
+                "device functors for the selection arithmetic; one ESS read "
+                "to the host per step"}}
+
+
 def bench_smc_batched_run(m, torch):
     """The batched configuration alone: one warm-up and one timed graph run
     of the 4096 SV filters (no per-filter comparison), for counter runs."""
@@ -610,6 +638,27 @@ def cpu_reference_points(threads):
     t = time.perf_counter() - t0
     res["smc_batched_sv_port_%dthreads_particle_steps_per_s" % threads] = \
         ps / t
+    pf_lib = os.path.join(orc.HERE, "_ref", "libmckl_ref_pf.so")
+    if os.path.exists(pf_lib):
+        # the reference's particle-filter example itself, BackendSEQ, one
+        # thread, its own N = 100000 over its 100 observations (sampler
+        # construction and the data read included)
+        lib = ctypes.CDLL(pf_lib, mode=os.RTLD_LAZY)
+        N = 100000
+        Tn = ctypes.c_size_t()
+        ess = np.zeros(1000)
+        size = np.zeros(1000, np.uint64)
+        st = np.zeros(4 * N)
+        wv = np.zeros(N)
+        keys = np.zeros(4, np.uint64)
+        t0 = time.perf_counter()
+        rc = lib.ref_pf(ctypes.c_size_t(N), ctypes.c_uint64(1),
+                        os.path.join(ROOT, "tests", "golden").encode(),
+                        ctypes.c_size_t(1000), ctypes.byref(Tn), orc.P(ess),
+                        orc.P(size), orc.P(st), orc.P(wv), orc.P(keys))
+        t = time.perf_counter() - t0
+        if rc == 0:
+            res["pf_example_N1e5_T100_reference_seq_1thread_s"] = round(t, 3)
     if orc.REF is not None:
         # the reference's read_index_matrix (column-major) over 64 iterations
         # of 2^20 uniformly random parents, 8 algorithmic bytes per entry
@@ -731,7 +780,7 @@ def main():
     ap.add_argument("--only", choices=["fills", "saturation", "normals", "smc",
                                        "smc_plain", "smc_generic",
                                        "smc_batched", "smc_batched_run",
-                                       "grad",
+                                       "grad", "pf_example",
                                        "resample_index", "resample"],
                     help="run one secondary leg alone and print its JSON "
                          "(for profiling)")
@@ -762,6 +811,7 @@ def main():
                "smc_batched": bench_smc_batched,
                "smc_batched_run": bench_smc_batched_run,
                "grad": bench_grad,
+               "pf_example": bench_pf_example,
                "resample_index": bench_resample_index,
                "resample": bench_resample}[args.only]
         print(json.dumps(leg(m, torch)), flush=True)
@@ -779,6 +829,7 @@ def main():
         for name, leg in (("saturation", bench_saturation),
                           ("normals", bench_normals), ("smc", bench_smc),
                           ("grad", bench_grad),
+                          ("pf_example", bench_pf_example),
                           ("resample_index", bench_resample_index),
                           ("resample", bench_resample),
                           ("cpu", lambda m_, t_: {

@@ -157,6 +157,14 @@ int mckl_hip_rand(mckl_hip_engine *e, size_t n, void *d_r, void *stream);
 int mckl_hip_rand_uniform_bits(
     mckl_hip_engine *e, int ubits, size_t n, void *d_r, void *stream);
 
+/* The device addresses, on the current device, of the tables that the
+ * correctly rounded log, log1p, sin and cos of device evaluation objects
+ * read (mckl_hip_eval.hpp's DeviceMath).  No counterpart in the reference:
+ * its evaluation objects call the host libm through math/vmf.hpp; these are
+ * the device stand-ins, rounded once. */
+int mckl_hip_device_math(
+    const double **d_log_c, const double **d_sin_a, const double **d_cos_a);
+
 /* rand(rng, U01xxDistribution<T>, n, r): random/u01_distribution.hpp:79-142,
  * random/internal/u01_generic.hpp:48-198, random/internal/common.hpp:69-86
  * (chunking by BufferSize<T>).  32-bit engines take one u32 per value (two
@@ -257,6 +265,13 @@ int mckl_hip_select_f32(size_t M, size_t dim, const uint32_t *d_idx,
     const float *d_src, float *d_dst, void *stream);
 int mckl_hip_select_f64(size_t M, size_t dim, const uint32_t *d_idx,
     const double *d_src, double *d_dst, void *stream);
+/* StateMatrix<ColMajor,dim,T>::select: core/state_matrix.hpp:240-265 (out
+ * of place: column c of d_dst, M rows, is column c of d_src, N rows,
+ * gathered by d_idx; the indices must lie below N) */
+int mckl_hip_select_cols_f32(size_t M, size_t N, size_t dim,
+    const uint32_t *d_idx, const float *d_src, float *d_dst, void *stream);
+int mckl_hip_select_cols_f64(size_t M, size_t N, size_t dim,
+    const uint32_t *d_idx, const double *d_src, double *d_dst, void *stream);
 
 /*
  * ResampleIndex<uint32_t> (algorithm/resample.hpp:597-832): the history of

@@ -83,6 +83,22 @@ def build_host_test():
     return out
 
 
+def build_pf_program():
+    """The reference's particle-filter example on the device API
+    (tests/cpp/algorithm_pf.hip), a hipcc program linked to the library."""
+    src = os.path.join(ROOT, "tests", "cpp", "algorithm_pf.hip")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "algorithm_pf")
+    _run([hipcc()] + [f for f in HIPCC_FLAGS if f != "-fPIC"] + [
+        "-I", os.path.join(ROOT, "include"),
+        "-I", os.path.join(HERE, "include"),
+        src, "-o", out,
+        "-L", HERE, "-lmckl_hip", "-Wl,-rpath,$ORIGIN/../../mckl_amd",
+    ])
+    return out
+
+
 def build_crmath_host():
     """Host builds of device headers for the CPU tests (tests/cpp):
     csrc/crmath.hpp (correctly rounded log/sin/cos) and csrc/smc_index.hpp
@@ -102,6 +118,7 @@ def build_crmath_host():
 def main():
     build_lib()
     build_host_test()
+    build_pf_program()
     build_crmath_host()
 
 

@@ -1,5 +1,5 @@
-// crmath.hpp -- correctly rounded double-precision log, sin and cos for the
-// Normal fills.
+// crmath.hpp -- correctly rounded double-precision log, log1p, sin and cos
+// for the Normal fills and for device evaluation objects.
 //
 // The reference's NormalDistribution<double> takes std::log, std::sin and
 // std::cos of each draw (random/normal_distribution.hpp:50-78,
@@ -17,6 +17,8 @@
 //           c_j = j/1024 nearest to f, r = (f - c_j)/c_j, |r| <= 2^-10.58,
 //           log1p by its Taylor series (10 terms, the first five in
 //           double-double), log c_j from a table built by the atanh series.
+//   log1p x = the series for |x| < 2^-11, else log(s) + log1p(e/s) where
+//           1 + x = s + e exactly.
 //   sin/cos x, 0 <= x <= 8: x = k pi/2 + r (pi/2 in three 50-bit pieces,
 //           so r is exact to 2^-150), r = a_j + s with a_j = j/64,
 //           |s| <= 1/128, sin/cos s by Taylor series, the angle-addition
@@ -132,6 +134,25 @@ constexpr double kPio2b = 0x1.1a62633145c00p-54; // with 50 significant bits
 constexpr double kPio2c = 0x1.b839a252049c1p-104;
 constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
 
+// log1p(r) for a double-double |r| <= 2^-10.58, double-double result:
+// r (1 - r/2 + r^2/3 - ... - r^9/10), the terms r^6/6 and beyond in double
+// (the first omitted term is below 2^-116 relative)
+__host__ __device__ inline ddv log1p_small(ddv r)
+{
+    double p = -0x1.999999999999ap-4;          // -1/10
+    p = __builtin_fma(p, r.hi, 0x1.c71c71c71c71cp-4);  // 1/9
+    p = __builtin_fma(p, r.hi, -0.125);                // -1/8
+    p = __builtin_fma(p, r.hi, 0x1.2492492492492p-3);  // 1/7
+    p = __builtin_fma(p, r.hi, -0x1.5555555555555p-3); // -1/6
+    ddv s = add(ddv{0x1.999999999999ap-3, -0x1.999999999999ap-57},
+        mul_d(r, p));                                   // 1/5
+    s = add(ddv{-0.25, 0.0}, mul(r, s));
+    s = add(ddv{0x1.5555555555555p-2, 0x1.5555555555555p-56}, mul(r, s)); // 1/3
+    s = add(ddv{-0.5, 0.0}, mul(r, s));
+    s = add(ddv{1.0, 0.0}, mul(r, s));
+    return mul(r, s);
+}
+
 // log x for a positive normal double x, double-double result
 __host__ __device__ inline ddv log_dd(double x, const double *log_c)
 {
@@ -149,19 +170,7 @@ __host__ __device__ inline ddv log_dd(double x, const double *log_c)
     // r = d / c in double-double: the remainder d - q c is exact
     const double q = d / c;
     const ddv r = ddv{q, __builtin_fma(-q, c, d) / c};
-    // log1p(r) = r (1 - r/2 + r^2/3 - ...): terms r^6/6 and beyond in double
-    double p = -0x1.999999999999ap-4;          // -1/10
-    p = __builtin_fma(p, r.hi, 0x1.c71c71c71c71cp-4);  // 1/9
-    p = __builtin_fma(p, r.hi, -0.125);                // -1/8
-    p = __builtin_fma(p, r.hi, 0x1.2492492492492p-3);  // 1/7
-    p = __builtin_fma(p, r.hi, -0x1.5555555555555p-3); // -1/6
-    ddv s = add(ddv{0x1.999999999999ap-3, -0x1.999999999999ap-57},
-        mul_d(r, p));                                   // 1/5
-    s = add(ddv{-0.25, 0.0}, mul(r, s));
-    s = add(ddv{0x1.5555555555555p-2, 0x1.5555555555555p-56}, mul(r, s)); // 1/3
-    s = add(ddv{-0.5, 0.0}, mul(r, s));
-    s = add(ddv{1.0, 0.0}, mul(r, s));
-    const ddv l1p = mul(r, s);
+    const ddv l1p = log1p_small(r);
     const ddv t{log_c[2 * (j - kLogJ0)], log_c[2 * (j - kLogJ0) + 1]};
     const double ed = (double)e;
     ddv el = two_prod(ed, kLn2Hi);
@@ -176,6 +185,33 @@ __host__ __device__ inline ddv log_dd(double x, const double *log_c)
 __host__ __device__ inline double log_cr(double x, const double *log_c)
 {
     const ddv v = log_dd(x, log_c);
+    return v.hi + v.lo;
+}
+
+// log1p x for a finite double x > -1, double-double result.  |x| < 2^-11:
+// the series.  Otherwise 1 + x = s + e exactly (two_sum; s >= 2^-53 is
+// normal), log1p x = log s + log1p(e/s) with |e/s| <= 2^-53, so
+// log1p(e/s) = t - t^2/2 to 2^-159, t = e/s in double-double (the
+// remainder e - q s is exact).
+__host__ __device__ inline ddv log1p_dd(double x, const double *log_c)
+{
+    if (__builtin_fabs(x) < 0x1p-11)
+        return log1p_small(ddv{x, 0.0});
+    const ddv s = two_sum(1.0, x);
+    const ddv l = log_dd(s.hi, log_c);
+    if (s.lo == 0.0)
+        return l;
+    const double q = s.lo / s.hi;
+    const double tl = __builtin_fma(-q, s.hi, s.lo) / s.hi;
+    return add(l, ddv{q, tl - 0.5 * (q * q)});
+}
+
+// correctly rounded log1p x (x > -1, finite)
+__host__ __device__ inline double log1p_cr(double x, const double *log_c)
+{
+    if (x == 0.0)
+        return x; // log1p(+-0) = +-0
+    const ddv v = log1p_dd(x, log_c);
     return v.hi + v.lo;
 }
 

@@ -641,6 +641,20 @@ __global__ void k_select(uint64_t M, uint64_t dim, const uint32_t *idx,
     }
 }
 
+// column-major: column c of the new state (M rows) is column c of the old
+// one (N rows) gathered by idx
+template <typename T>
+__global__ void k_select_cols(uint64_t M, uint64_t N, uint64_t dim,
+    const uint32_t *idx, const T *src, T *dst)
+{
+    const uint64_t total = M * dim;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+         q < total; q += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = q / M, j = q % M;
+        dst[q] = src[c * N + idx[j]];
+    }
+}
+
 uint32_t grid_for(uint64_t n, uint32_t cap = 8192)
 {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256), cap));
@@ -1117,6 +1131,26 @@ int select_impl(size_t M, size_t dim, const uint32_t *idx, const T *src,
     return MCKL_HIP_OK;
 }
 
+template <typename T>
+int select_cols_impl(size_t M, size_t N, size_t dim, const uint32_t *idx,
+    const T *src, T *dst, hipStream_t st)
+{
+    if (M == 0 || dim == 0)
+        return MCKL_HIP_OK;
+    if (!idx || !src || !dst)
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    if (src == dst)
+        return set_error(MCKL_HIP_EINVAL, "select is out of place");
+    if (N == 0 || N > UINT32_MAX)
+        return set_error(MCKL_HIP_EINVAL, "select: N out of range");
+    if (dim > ((uint64_t)1 << 40) / std::max(M, N))
+        return set_error(MCKL_HIP_EINVAL, "select: size out of range");
+    k_select_cols<T><<<grid_for(M * dim), 256, 0, st>>>(M, N, dim, idx, src,
+        dst);
+    MCKL_HIP_CHECK_LAUNCH("k_select_cols");
+    return MCKL_HIP_OK;
+}
+
 // Workspace bytes of one ResampleAlgorithm call: the M uniforms and the
 // helpers' arrays (plain schemes), or the residual form's arrays.
 constexpr size_t resample_ws_bytes(int scheme, size_t N, size_t M)
@@ -1389,6 +1423,18 @@ int mckl_hip_select_f64(size_t M, size_t dim, const uint32_t *d_idx,
     const double *d_src, double *d_dst, void *stream)
 {
     return select_impl(M, dim, d_idx, d_src, d_dst, as_stream(stream));
+}
+
+int mckl_hip_select_cols_f32(size_t M, size_t N, size_t dim,
+    const uint32_t *d_idx, const float *d_src, float *d_dst, void *stream)
+{
+    return select_cols_impl(M, N, dim, d_idx, d_src, d_dst, as_stream(stream));
+}
+
+int mckl_hip_select_cols_f64(size_t M, size_t N, size_t dim,
+    const uint32_t *d_idx, const double *d_src, double *d_dst, void *stream)
+{
+    return select_cols_impl(M, N, dim, d_idx, d_src, d_dst, as_stream(stream));
 }
 
 } // extern "C"

@@ -1132,6 +1132,23 @@ extern "C" {
 
 const char *mckl_hip_last_error(void) { return last_error_ref().c_str(); }
 
+int mckl_hip_device_math(
+    const double **d_log_c, const double **d_sin_a, const double **d_cos_a)
+{
+    if (!d_log_c || !d_sin_a || !d_cos_a)
+        return set_error(MCKL_HIP_EINVAL, "NULL argument");
+    int rc = ensure_device_tables();
+    if (rc)
+        return rc;
+    crm::Tables *t = nullptr;
+    MCKL_HIP_TRY(hipGetSymbolAddress(
+        reinterpret_cast<void **>(&t), HIP_SYMBOL(d_crm_tables)));
+    *d_log_c = t->log_c;
+    *d_sin_a = t->sin_a;
+    *d_cos_a = t->cos_a;
+    return MCKL_HIP_OK;
+}
+
 int mckl_hip_engine_words(const mckl_hip_engine *e)
 {
     int rc = check_engine(e);

@@ -516,7 +516,7 @@ class ResampleIndexHIP
             hipMemcpy(&r, d + 1, sizeof(r), hipMemcpyDeviceToHost) !=
                 hipSuccess)
             rc = MCKL_HIP_EHIP;
-        hipFree(d);
+        (void)hipFree(d);
         hip_check(rc);
         return r;
     }
@@ -865,7 +865,7 @@ class PMCMCMutationHIP
         if (rc == MCKL_HIP_OK &&
             hipMemcpy(&u, d, sizeof(u), hipMemcpyDeviceToHost) != hipSuccess)
             rc = MCKL_HIP_EHIP;
-        hipFree(d);
+        (void)hipFree(d);
         hip_check(rc);
         if (std::log(u) < prob) {
             param = p;
@@ -972,10 +972,26 @@ inline void select_rows(std::size_t n, std::size_t dim,
     hip_check(mckl_hip_select_f64(n, dim, d_idx, src, dst, current_stream()));
 }
 
+inline void select_cols(std::size_t n, std::size_t N, std::size_t dim,
+    const std::uint32_t *d_idx, const float *src, float *dst)
+{
+    hip_check(mckl_hip_select_cols_f32(
+        n, N, dim, d_idx, src, dst, current_stream()));
+}
+
+inline void select_cols(std::size_t n, std::size_t N, std::size_t dim,
+    const std::uint32_t *d_idx, const double *src, double *dst)
+{
+    hip_check(mckl_hip_select_cols_f64(
+        n, N, dim, d_idx, src, dst, current_stream()));
+}
+
 /// Particle<T> (core/particle.hpp:277-360) in device memory: the state is
 /// N rows of dim values of T (float or double), row-major as
-/// StateMatrix<RowMajor, Dynamic, T>; weight() is the device Weight; rng()
-/// is the engine ResampleEval draws from.
+/// StateMatrix<RowMajor, Dynamic, T> or, with layout ColMajor, dim columns
+/// of N values as StateMatrix<ColMajor, Dynamic, T> (col_data(j) is column
+/// j); weight() is the device Weight; rng() is the engine ResampleEval
+/// draws from.
 template <typename T, typename RNG = Philox4x32>
 class Particle
 {
@@ -988,8 +1004,9 @@ class Particle
     using state_type = T;
     using rng_type = RNG;
 
-    explicit Particle(std::size_t N, std::size_t dim = 1, RNG rng = RNG())
-        : dim_(dim), rng_(rng)
+    explicit Particle(std::size_t N, std::size_t dim = 1, RNG rng = RNG(),
+        MatrixLayout layout = RowMajor)
+        : dim_(dim), layout_(layout), rng_(rng)
     {
         if (N == 0 || dim == 0)
             throw RuntimeAssert("Particle: N and dim must be positive");
@@ -1000,9 +1017,18 @@ class Particle
 
     size_type size() const { return n_; }
     std::size_t dim() const { return dim_; }
-    /// device pointer to the N x dim state, row-major
+    MatrixLayout layout() const { return layout_; }
+    /// device pointer to the N x dim state in layout()
     T *state() { return x_.data(); }
     const T *state() const { return x_.data(); }
+    /// device pointer to column j (ColMajor layout only)
+    T *col_data(std::size_t j)
+    {
+        if (layout_ != ColMajor || j >= dim_)
+            throw RuntimeAssert(
+                "Particle::col_data: ColMajor layout and j < dim() required");
+        return x_.data() + j * n_;
+    }
     Weight &weight() { return weight_; }
     const Weight &weight() const { return weight_; }
     RNG &rng() { return rng_; }
@@ -1029,13 +1055,13 @@ class Particle
                         "Particle::select: row numbers must lie below size()");
         }
         if (n == n_) {
-            select_rows(n, dim_, d_idx, x_.data(), spare_.data());
+            gather(n, d_idx, spare_.data());
             std::swap(x_, spare_);
             weight_.set_equal();
             return;
         }
         DeviceArray<T> nx(n * dim_);
-        select_rows(n, dim_, d_idx, x_.data(), nx.data());
+        gather(n, d_idx, nx.data());
         x_ = std::move(nx);
         spare_.resize(n * dim_);
         w_.resize(n);
@@ -1046,8 +1072,17 @@ class Particle
   private:
     std::size_t n_ = 0;
     std::size_t dim_;
+    MatrixLayout layout_;
     RNG rng_;
     DeviceArray<T> x_, spare_;
+
+    void gather(size_type n, const std::uint32_t *d_idx, T *dst)
+    {
+        if (layout_ == ColMajor)
+            select_cols(n, n_, dim_, d_idx, x_.data(), dst);
+        else
+            select_rows(n, dim_, d_idx, x_.data(), dst);
+    }
     DeviceArray<double> w_, stats_;
     Weight weight_{0, nullptr, nullptr};
 
@@ -1222,8 +1257,9 @@ class SMCSampler
     using eval_type = std::function<void(std::size_t, Particle<T, RNG> &)>;
     using estimator_type = SMCEstimator<T, RNG>;
 
-    explicit SMCSampler(std::size_t N, std::size_t dim = 1, RNG rng = RNG())
-        : particle_(N, dim, rng)
+    explicit SMCSampler(std::size_t N, std::size_t dim = 1, RNG rng = RNG(),
+        MatrixLayout layout = RowMajor)
+        : particle_(N, dim, rng, layout)
     {
     }
 

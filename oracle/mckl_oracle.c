@@ -830,9 +830,15 @@ void mo_rand_uniform_real_f32(mo_engine *e, size_t n, float a, float b,
 #define MO_CR_LOG 0
 #define MO_CR_SIN 1
 #define MO_CR_COS 2
+#define MO_CR_LOG1P 3
 static __float128 mo_quad_fn(int fn, __float128 x)
 {
-    return fn == MO_CR_LOG ? logq(x) : (fn == MO_CR_SIN ? sinq(x) : cosq(x));
+    switch (fn) {
+    case MO_CR_LOG: return logq(x);
+    case MO_CR_SIN: return sinq(x);
+    case MO_CR_COS: return cosq(x);
+    default: return log1pq(x);
+    }
 }
 static int mo_cr_f32(int fn, float x, float y)
 {

@@ -172,7 +172,7 @@ def rand_normal_cr(e, n, mean, sd, dtype):
     return out[:n], clean[:n].astype(bool)
 
 
-CR_LOG, CR_SIN, CR_COS = 0, 1, 2
+CR_LOG, CR_SIN, CR_COS, CR_LOG1P = 0, 1, 2, 3
 
 
 def quad_eval(fn, x):

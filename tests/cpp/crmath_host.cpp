@@ -1,5 +1,5 @@
 // Host build of mckl_amd/csrc/crmath.hpp (the device's correctly rounded
-// log/sin/cos) for the CPU tests: the same source, compiled by g++ with
+// log/log1p/sin/cos) for the CPU tests: the same source, compiled by g++ with
 // -ffp-contract=off, so its results can be compared with quad precision
 // without a GPU.  Test infrastructure only.
 #define __host__
@@ -29,6 +29,13 @@ void crm_host_log(size_t n, const double *x, double *y)
     const Tables &t = tables();
     for (size_t i = 0; i < n; ++i)
         y[i] = log_cr(x[i], t.log_c);
+}
+
+void crm_host_log1p(size_t n, const double *x, double *y)
+{
+    const Tables &t = tables();
+    for (size_t i = 0; i < n; ++i)
+        y[i] = log1p_cr(x[i], t.log_c);
 }
 
 void crm_host_sincos(size_t n, const double *x, double *s, double *c)

@@ -1,4 +1,4 @@
-"""The device's correctly rounded log/sin/cos (mckl_amd/csrc/crmath.hpp),
+"""The device's correctly rounded log/log1p/sin/cos (mckl_amd/csrc/crmath.hpp),
 compiled for the host from the same source, against quad-precision values
 (libquadmath through the oracle) rounded once.
 
@@ -6,7 +6,9 @@ Pins: the double-double tables of log(j/1024) and sin/cos(j/64) to 2^-100
 relative; log of every kind of U01OC double the Normal fill draws (2^20
 random ones plus the extremes), of doubles across the whole exponent range
 and next to 1; sin and cos of the Normal fill's angles 2*pi*u and of
-arguments next to each multiple of pi/2 up to 2*pi.  Every result must equal
+arguments next to each multiple of pi/2 up to 2*pi; log1p across (-1, 1e300)
+including both sides of its 2^-11 switch and the particle-filter example's
+arguments 0.1 (10 d)^2.  Every result must equal
 the correctly rounded value.
 """
 import ctypes
@@ -99,6 +101,31 @@ def test_log_general(crm):
     got = _log(crm, x)
     want = orc.quad_eval(orc.CR_LOG, x)
     assert np.array_equal(got, want), int((got != want).sum())
+
+
+def _log1p(crm, x):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(len(x))
+    crm.crm_host_log1p(ctypes.c_size_t(len(x)), P(x), P(y))
+    return y
+
+
+def test_log1p(crm):
+    rng = np.random.default_rng(11)
+    d = rng.normal(0.0, 1.0, 200000) * np.exp(rng.uniform(-12, 3, 200000))
+    t = 2.0 ** -11
+    x = np.concatenate([
+        0.1 * (10.0 * d) ** 2,                      # the PF example's form
+        np.exp(rng.uniform(-700.0, 690.0, 100000)),
+        -np.exp(rng.uniform(-700.0, -1e-12, 100000)),  # (-1, 0)
+        t + np.arange(-300, 301) * 2.0 ** -63,         # both sides of 2^-11
+        -t + np.arange(-300, 301) * 2.0 ** -63,
+        np.nextafter(-1.0, 0.0) + np.arange(0, 64) * 2.0 ** -53,
+        np.array([0.0, -0.0, 5e-324, -5e-324, 1.0, -0.5, 1e300])])
+    got = _log1p(crm, x)
+    want = orc.quad_eval(orc.CR_LOG1P, x)
+    assert np.array_equal(got, want), int((got != want).sum())
+    assert np.signbit(_log1p(crm, np.array([-0.0])))[0]
 
 
 def test_sincos_normal_angles(crm):
