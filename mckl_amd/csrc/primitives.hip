@@ -124,11 +124,7 @@ __global__ __launch_bounds__(kB) void k_weight_pre(int mode, uint64_t n,
         }
     }
     st8(w, i0, n, a);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        const double o = __shfl_xor(m, d, 64);
-        m = m < o ? o : m;
-    }
+    m = wave_max_d(m);
     if ((threadIdx.x & 63) == 0)
         sm[threadIdx.x >> 6] = m;
     __syncthreads();
@@ -147,11 +143,7 @@ __device__ __forceinline__ double blocks_max(const double *pmax, uint32_t n,
     double m = -__builtin_huge_val();
     for (uint32_t i = threadIdx.x; i < n; i += kB)
         m = m < pmax[i] ? pmax[i] : m;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        const double o = __shfl_xor(m, d, 64);
-        m = m < o ? o : m;
-    }
+    m = wave_max_d(m);
     if ((threadIdx.x & 63) == 0)
         sm[threadIdx.x >> 6] = m;
     __syncthreads();
@@ -915,9 +907,7 @@ __global__ __launch_bounds__(kB) void k_residual_tiles(uint64_t N, double Md,
     st8(integ, i0, N, ig);
     dd tot;
     (void)group_excl_scan_dd(loc, threadIdx.x, sd, tot);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1)
-        isum += __shfl_xor(isum, d, 64);
+    isum = wave_sum_u(isum);
     if ((threadIdx.x & 63) == 0)
         su[threadIdx.x >> 6] = isum;
     __syncthreads();
@@ -948,9 +938,7 @@ __global__ __launch_bounds__(kB) void k_residual_total(const double *th,
     }
     dd tot;
     (void)group_excl_scan_dd(loc, gt, sd, tot);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1)
-        isum += __shfl_xor(isum, d, 64);
+    isum = wave_sum_u(isum);
     if ((gt & 63) == 0)
         su[gt >> 6] = isum;
     __syncthreads();

@@ -33,37 +33,47 @@ struct SmcState {
     double C[2];      // normaliser of the carried weights for the next step
     float m[2];       // max log weight of the step
     int resampled[2]; // the step resampled (applies to the next move)
+    int err;          // a look-back wait gave up (results are invalid)
 };
 
 // -------------------------------------------------------------------------
 // plain sampler kernels (grid = ntiles workgroups of 256 threads).  No
-// device-wide fences or atomics: a value every tile needs (max, sums, the
-// resampling decision, tile offsets) is recomputed by each workgroup from
-// the per-tile arrays with the canonical cross-tile functions, and only
-// workgroup 0 writes it to the state.
+// device-wide fences: a value every tile needs (max, sums, the resampling
+// decision, tile offsets) is recomputed by each workgroup from the per-tile
+// arrays with the canonical cross-tile functions, and only workgroup 0
+// writes it to the state; the one cross-tile prefix that depends on the
+// step's counts (the tiles' extra-copy and zero-slot totals) is a
+// decoupled look-back on relaxed atomic words.
 // -------------------------------------------------------------------------
-__global__ __launch_bounds__(kGroup) void k_init(
-    SmcArgs A, float *x, float *lw, SmcState *st)
-{
-    tile_init(A, blockIdx.x, x, lw, threadIdx.x);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->logz = 0.0;
-        st->C[0] = (double)A.N;
-        st->m[0] = 0.0f;
-        st->resampled[0] = 0;
-    }
-}
-
 struct RsBuf {
     double *tdh, *tdl; // tile dd totals of a (unscaled)
     uint32_t *te, *tz; // tile extra-copy / zero-slot counts
-    uint32_t *cnt, *Eincl, *zpos;
+    uint32_t *cnt;     // cumulative counts (multinomial, residual)
     double *tmh, *tml; // multinomial: tile dd totals of the log spacings
     double *u;         // multinomial: the step's sorted uniforms
     double *uh, *ul;   // multinomial: scaled tile dd totals r T
     uint32_t *ti;      // residual: tile sums of the integral parts
     double *trh, *trl; // residual: tile dd totals of the residuals
+    uint32_t *src;     // source of each extra-copy rank (tile_index_src)
+    uint64_t *lbw;     // look-back words, 2 x ntiles (by step parity)
 };
+
+__global__ __launch_bounds__(kGroup) void k_init(
+    SmcArgs A, float *x, float *lw, SmcState *st, RsBuf B)
+{
+    tile_init(A, blockIdx.x, x, lw, threadIdx.x);
+    if (threadIdx.x == 0) {
+        B.lbw[blockIdx.x] = 0;
+        B.lbw[A.ntiles + blockIdx.x] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->logz = 0.0;
+        st->C[0] = (double)A.N;
+        st->m[0] = 0.0f;
+        st->resampled[0] = 0;
+        st->err = 0;
+    }
+}
 
 // move + weight of a tile, then the tile's sums of a_i = exp_f(lw_i - m_k)
 // about its own max m_k (the weights' scale r_k needs the max over tiles and
@@ -76,7 +86,7 @@ __global__ __launch_bounds__(kGroup) void k_move(SmcArgs A, uint32_t t,
     const GroupLds L = group_lds(lds);
     const int p = (t - 1) & 1;
     float l[kItems];
-    const float mt = tile_move(A, t, blockIdx.x, xin, xout, lw, idx,
+    const float mt = tile_move(A, t, blockIdx.x, xin, xout, lw, idx, B.src,
         st->resampled[p], st->m[p], threadIdx.x, L.f, l);
     double S, S2;
     dd et;
@@ -113,16 +123,120 @@ __host__ __device__ inline bool is_residual(int scheme)
         scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC;
 }
 
+// Decoupled look-back (single-pass prefix over the tiles of one launch):
+// tile k publishes its (extra-copy, zero-slot) totals as an aggregate, sums
+// its predecessors' words back to the nearest inclusive prefix, then
+// publishes its own inclusive prefix.  A word packs a status (bit 62
+// aggregate, bit 63 inclusive) with two 31-bit totals, so one relaxed
+// agent-scope atomic load or store carries value and status together and no
+// fence is needed.  Workgroups start in index order and every one of a
+// launch is resident (one per tile, a few per CU), so a predecessor always
+// makes progress; the wait is bounded all the same and, if it gives up,
+// sets st->err and counts the missing tiles as empty (every index the
+// kernels then derive still stays below N).
+constexpr uint64_t kLbAgg = 1ull << 62;
+constexpr uint64_t kLbIncl = 1ull << 63;
+constexpr uint32_t kLbSpinLimit = 1u << 20;
+
+__device__ __forceinline__ uint64_t lb_pack(
+    uint64_t status, uint32_t e, uint32_t z)
+{
+    return status | ((uint64_t)(e & 0x7FFFFFFFu) << 31) |
+        (uint64_t)(z & 0x7FFFFFFFu);
+}
+
+__device__ __forceinline__ uint64_t lb_load(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void lb_store(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// exclusive prefix (eoff, zoff) of tile k's totals (te, tz) over the tiles
+// before it.  Each round reads kLbQ predecessor words per thread (a window
+// of kLbQ x 256 tiles, one memory round trip), waits until each holds a
+// status, and adds the words up to the nearest inclusive one; a launch of at
+// most 1024 tiles finishes in one round.  Every thread of the workgroup
+// calls it.
+constexpr int kLbQ = 4;
+
+__device__ __forceinline__ void tile_lookback(uint64_t *w, uint32_t k,
+    uint32_t te, uint32_t tz, int gt, uint32_t *ldsu, int *err,
+    uint32_t &eoff, uint32_t &zoff)
+{
+    if (gt == 0)
+        lb_store(&w[k], lb_pack(k == 0 ? kLbIncl : kLbAgg, te, tz));
+    uint32_t es = 0, zs = 0, spins = 0;
+    bool failed = false;
+    int64_t top = (int64_t)k - 1; // the window's nearest predecessor
+    while (top >= 0) {
+        uint64_t v[kLbQ];
+#pragma unroll
+        for (int q = 0; q < kLbQ; ++q) {
+            const int64_t j = top - (gt + q * kGroup);
+            v[q] = j >= 0 ? lb_load(&w[j]) : kLbIncl; // before tile 0: zero
+        }
+#pragma unroll
+        for (int q = 0; q < kLbQ; ++q) {
+            const int64_t j = top - (gt + q * kGroup);
+            while (!(v[q] & (kLbAgg | kLbIncl))) {
+                if (++spins > kLbSpinLimit) {
+                    failed = true;
+                    v[q] = kLbIncl;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+                v[q] = lb_load(&w[j]);
+            }
+        }
+        // distance of the nearest inclusive word; words up to it count
+        uint32_t dmin = 0xFFFFFFFFu;
+#pragma unroll
+        for (int q = 0; q < kLbQ; ++q) {
+            const uint32_t d = (uint32_t)(gt + q * kGroup);
+            if ((v[q] & kLbIncl) && d < dmin)
+                dmin = d;
+        }
+        dmin = group_min_u(dmin, gt, ldsu + 8);
+#pragma unroll
+        for (int q = 0; q < kLbQ; ++q) {
+            if ((uint32_t)(gt + q * kGroup) <= dmin) {
+                es += (uint32_t)((v[q] >> 31) & 0x7FFFFFFFu);
+                zs += (uint32_t)(v[q] & 0x7FFFFFFFu);
+            }
+        }
+        if (dmin != 0xFFFFFFFFu)
+            break;
+        top -= (int64_t)kLbQ * kGroup;
+    }
+    if (failed)
+        *err = 1;
+    group_sum2_u(es, zs, gt, ldsu);
+    if (gt == 0 && k != 0)
+        lb_store(&w[k], lb_pack(kLbIncl, es + te, zs + tz));
+    eoff = es;
+    zoff = zs;
+}
+
 __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     const float *lw, const float *tmax, const double *ts, const double *ts2,
-    RsBuf B, SmcState *st, double *ess_hist, int32_t *res_hist)
+    RsBuf B, SmcState *st, double *ess_hist, int32_t *res_hist,
+    uint32_t *idx)
 {
     __shared__ __align__(16) unsigned char lds[kScratch];
+    __shared__ __align__(16) uint32_t Es[kTile];
     const GroupLds L = group_lds(lds);
     const int gt = threadIdx.x;
     const int nt = (int)A.ntiles;
     const uint32_t k = blockIdx.x;
     const int p = (t - 1) & 1, c = t & 1;
+    // this tile's look-back word of the next step (last read by the
+    // previous step's launch, which has finished)
+    if (gt == 0)
+        lb_store(&B.lbw[(size_t)(c ^ 1) * nt + k], 0);
     float l[kItems];
     load8(lw, tile_first(k, gt), A.N, l, 0.0f);
     const double Cprev = st->C[p];
@@ -254,14 +368,17 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
         }
         return;
     }
+    // systematic / stratified: the counts, the tile's place among the
+    // extra copies and zero slots by look-back, and the rep -> index
+    // transform (parents as codes, resolved by the next move)
     const UniformSeq U = make_uniform_seq(A, t);
-    uint32_t es, zs;
-    tile_counts_l(A, k, l, tmax[k], rk, invS, toff, next_hi, U, B.cnt, gt,
-        L.d, L.u, es, zs);
-    if (gt == 0) {
-        B.te[k] = es;
-        B.tz[k] = zs;
-    }
+    uint32_t es, zs, cs[kItems], prev;
+    tile_counts_c(A, k, l, tmax[k], rk, invS, toff, next_hi, U, gt, L.d,
+        L.u, cs, prev, es, zs);
+    uint32_t eoff, zoff;
+    tile_lookback(B.lbw + (size_t)c * nt, k, es, zs, gt, L.u, &st->err, eoff,
+        zoff);
+    tile_index_src(A, k, cs, prev, eoff, zoff, idx, B.src, gt, L.u, Es);
 }
 
 // R = M - sum of the tile sums of the integral parts (all threads)
@@ -428,28 +545,22 @@ __device__ __forceinline__ void tiles_prefix_u2(const uint32_t *a,
     pb = sb;
 }
 
+// rep -> index of the multinomial and residual schemes, whose counts come
+// from kernels of their own (the tile offsets from the tile totals)
 __global__ __launch_bounds__(kGroup) void k_rs_index(SmcArgs A, uint32_t t,
     const SmcState *st, RsBuf B, uint32_t *idx)
 {
     if (!st->resampled[t & 1])
         return;
     __shared__ __align__(16) unsigned char lds[kScratch];
+    __shared__ __align__(16) uint32_t Es[kTile];
     const GroupLds L = group_lds(lds);
     uint32_t c[kItems], prev;
     load_counts(A, blockIdx.x, B.cnt, threadIdx.x, c, prev);
     uint32_t eoff, zoff;
     tiles_prefix_u2(B.te, B.tz, blockIdx.x, threadIdx.x, L.u, eoff, zoff);
-    tile_index_c(A, blockIdx.x, c, prev, eoff, zoff, B.Eincl, idx, B.zpos,
-        threadIdx.x, L.u);
-}
-
-__global__ __launch_bounds__(kGroup) void k_rs_scatter(SmcArgs A, uint32_t t,
-    const SmcState *st, RsBuf B, uint32_t *idx)
-{
-    if (!st->resampled[t & 1])
-        return;
-    __shared__ uint32_t Es[kTile];
-    tile_scatter_ranks(A, blockIdx.x, B.Eincl, B.zpos, idx, threadIdx.x, Es);
+    tile_index_src(A, blockIdx.x, c, prev, eoff, zoff, idx, B.src,
+        threadIdx.x, L.u, Es);
 }
 
 // -------------------------------------------------------------------------
@@ -622,7 +733,7 @@ __global__ __launch_bounds__(G *kGroup, G == 1 ? 2 : 1) void k_batch(const Filte
                 const uint32_t kk = (uint32_t)(k < nt ? k : nt);
                 float l[kItems];
                 const float mt = tile_move(A, t, kk, xin, xout, lw, idx,
-                    resampled, mprev, gt, L.f, l);
+                    nullptr, resampled, mprev, gt, L.f, l);
                 double s, s2;
                 dd et;
                 tile_sums_l(A, kk, l, mt, gt, L.d, s, s2, et);
@@ -905,9 +1016,9 @@ static void smc_free(mckl_hip_smc *s)
     if (s->rs)
         (void)hipStreamDestroy(s->rs);
     void *ptrs[] = {s->x[0], s->x[1], s->lw, s->obs, s->tmax, s->idx, s->ts,
-        s->ts2, s->B.tdh, s->B.tdl, s->B.te, s->B.tz, s->B.cnt, s->B.Eincl,
-        s->B.zpos, s->B.tmh, s->B.tml, s->B.u, s->B.uh, s->B.ul, s->B.ti,
-        s->B.trh, s->B.trl, s->st, s->ess, s->res, s->eph, s->epl, s->est};
+        s->ts2, s->B.tdh, s->B.tdl, s->B.te, s->B.tz, s->B.cnt, s->B.tmh, s->B.tml, s->B.u, s->B.uh, s->B.ul, s->B.ti,
+        s->B.trh, s->B.trl, s->B.src, s->B.lbw, s->st, s->ess, s->res, s->eph,
+        s->epl, s->est};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -964,8 +1075,8 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
     MCKL_ALLOC(s->B.te, nt);
     MCKL_ALLOC(s->B.tz, nt);
     MCKL_ALLOC(s->B.cnt, Np);
-    MCKL_ALLOC(s->B.Eincl, Np);
-    MCKL_ALLOC(s->B.zpos, Np);
+    MCKL_ALLOC(s->B.src, Np);
+    MCKL_ALLOC(s->B.lbw, 2 * nt);
     if (is_residual(scheme)) {
         MCKL_ALLOC(s->B.ti, nt);
         MCKL_ALLOC(s->B.trh, nt);
@@ -988,6 +1099,8 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
         hipMemcpyHostToDevice);
     if (err == hipSuccess)
         err = hipMemset(s->st, 0, sizeof(SmcState));
+    if (err == hipSuccess) // every parent a code may name stays below N
+        err = hipMemset(s->B.src, 0, Np * sizeof(uint32_t));
     if (err == hipSuccess)
         err = hipEventCreateWithFlags(&s->done, hipEventDisableTiming);
     if (err == hipSuccess)
@@ -1002,13 +1115,15 @@ int mckl_hip_smc_create(int model, const double *theta, size_t T,
     return MCKL_HIP_OK;
 }
 
-// Enqueue the whole run: init, then per step move, sums, the decision with
-// the resampling counts, the rep->index transform and the zero-slot scatter
-// (the last three exit at once on steps that do not resample).
+// Enqueue the whole run: init, then per step the move with the tile sums,
+// and the decision with the resampling counts and the rep->index transform
+// (systematic, stratified: one kernel; multinomial and residual: their
+// count kernels, then k_rs_index).  The resampling kernels exit at once on
+// steps that do not resample.
 static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
 {
     const dim3 grid(s->A.ntiles);
-    k_init<<<grid, kGroup, 0, st>>>(s->A, s->x[0], s->lw, s->st);
+    k_init<<<grid, kGroup, 0, st>>>(s->A, s->x[0], s->lw, s->st, s->B);
     MCKL_HIP_CHECK_LAUNCH("k_init");
     for (uint32_t t = 1; t <= s->T; ++t) {
         const float *xin = s->x[(t - 1) & 1];
@@ -1019,7 +1134,7 @@ static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
             k_est_moments<<<grid, kGroup, 0, st>>>(s->A, t, xout, s->lw,
                 s->tmax, s->kmom, s->eph, s->epl);
         k_rs_counts<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->tmax, s->ts,
-            s->ts2, s->B, s->st, s->ess, s->res);
+            s->ts2, s->B, s->st, s->ess, s->res, s->idx);
         if (s->A.scheme == MCKL_HIP_MULTINOMIAL) {
             k_rs_uniforms<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B);
             k_rs_mcounts<<<grid, kGroup, 0, st>>>(
@@ -1032,8 +1147,8 @@ static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
             k_rs_rcounts<<<grid, kGroup, 0, st>>>(
                 s->A, t, s->lw, s->tmax, s->st, s->B);
         }
-        k_rs_index<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);
-        k_rs_scatter<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);
+        if (s->A.scheme == MCKL_HIP_MULTINOMIAL || is_residual(s->A.scheme))
+            k_rs_index<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B, s->idx);
     }
     if (s->kmom)
         k_est_moments_final<<<(uint32_t)s->T, kGroup, 0, st>>>(
@@ -1121,12 +1236,14 @@ int mckl_hip_smc_result(mckl_hip_smc *s, double *h_loglik, double *h_ess,
     if (!s)
         return set_error(MCKL_HIP_EINVAL, "sampler is NULL");
     MCKL_HIP_TRY(hipEventSynchronize(s->done));
-    if (h_loglik) {
-        SmcState hs;
-        MCKL_HIP_TRY(hipMemcpyWithStream(&hs, s->st, sizeof(hs),
-            hipMemcpyDeviceToHost, s->rs));
+    SmcState hs;
+    MCKL_HIP_TRY(hipMemcpyWithStream(&hs, s->st, sizeof(hs),
+        hipMemcpyDeviceToHost, s->rs));
+    if (hs.err)
+        return set_error(MCKL_HIP_EHIP,
+            "smc run: a resampling look-back wait gave up; results invalid");
+    if (h_loglik)
         *h_loglik = hs.logz;
-    }
     if (h_ess)
         MCKL_HIP_TRY(hipMemcpyWithStream(h_ess, s->ess, s->T * sizeof(double),
             hipMemcpyDeviceToHost, s->rs));

@@ -136,11 +136,7 @@ __device__ __forceinline__ double gtile_move(const GradArgs &G, uint32_t t,
         }
     }
     // group max of doubles
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        const double o = __shfl_xor(lmax, d, 64);
-        lmax = lmax < o ? o : lmax;
-    }
+    lmax = wave_max_d(lmax);
     if ((gt & 63) == 0)
         ldsd[gt >> 6] = lmax;
     __syncthreads();

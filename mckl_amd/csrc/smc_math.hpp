@@ -208,46 +208,193 @@ __device__ __forceinline__ double dd_div_d(dd a, double b)
     return __dadd_rn(q, __ddiv_rn(r, b));
 }
 
-// wave-wide dd sum (every lane gets the total)
-__device__ __forceinline__ dd wave_sum_dd(dd v)
+// ---- wave (64-lane) reductions and scans on DPP lane moves ----
+// A DPP move reads a register of another lane of the same row (16 lanes),
+// or broadcasts lane 15 / 31 to the next rows (row_bcast, GFX9), inside a
+// VALU instruction, with no LDS round trip.  Lanes without a source lane, or
+// in rows outside the row mask, take the identity `idn`.  Orders are fixed:
+//   reduce: quad butterflies (xor 1, xor 2), half-row and row mirrors (all
+//           16 lanes of a row hold the row total), rows 0+1 and 2+3 by
+//           row_bcast:15, then the four rows by row_bcast:31; lane 63 holds
+//           the total, which readlane makes uniform;
+//   scan:   row shifts by 1, 2, 4, 8 (inclusive within rows), then
+//           row_bcast:15 and row_bcast:31 carry the row totals forward;
+//           the exclusive form shifts the inclusive one by a lane.
+enum : int {
+    kDppXor1 = 0xB1,     // quad_perm [1, 0, 3, 2]
+    kDppXor2 = 0x4E,     // quad_perm [2, 3, 0, 1]
+    kDppShr1 = 0x111,    // row_shr:1
+    kDppShr2 = 0x112,    // row_shr:2
+    kDppShr4 = 0x114,    // row_shr:4
+    kDppShr8 = 0x118,    // row_shr:8
+    kDppWaveShr1 = 0x138, // wave_shr:1
+    kDppMirror = 0x140,  // row_mirror
+    kDppHalfMirror = 0x141, // row_half_mirror
+    kDppBcast15 = 0x142, // row_bcast:15
+    kDppBcast31 = 0x143, // row_bcast:31
+};
+
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t idn, uint32_t v)
 {
-#pragma unroll
-    for (int k = 32; k >= 1; k >>= 1)
-        v = dd_add(v, dd{__shfl_xor(v.hi, k, 64), __shfl_xor(v.lo, k, 64)});
-    return v;
+    return (uint32_t)__builtin_amdgcn_update_dpp(
+        (int)idn, (int)v, Ctrl, RowMask, 0xF, false);
 }
 
-__device__ __forceinline__ dd dd_shfl_up(dd v, int d)
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ float dpp_f(float idn, float v)
 {
-    return dd{__shfl_up(v.hi, d, 64), __shfl_up(v.lo, d, 64)};
+    return __uint_as_float(dpp_u<Ctrl, RowMask>(
+        __float_as_uint(idn), __float_as_uint(v)));
 }
 
-// ---- group (256-thread) reductions; `lds` is this group's scratch ----
-__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ double dpp_d(double idn, double v)
+{
+    const uint64_t a = (uint64_t)__double_as_longlong(idn);
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = dpp_u<Ctrl, RowMask>((uint32_t)a, (uint32_t)b);
+    const uint32_t hi =
+        dpp_u<Ctrl, RowMask>((uint32_t)(a >> 32), (uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 
+__device__ __forceinline__ uint32_t lane63_u(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ float lane63_f(float v)
+{
+    return __uint_as_float(lane63_u(__float_as_uint(v)));
+}
+
+__device__ __forceinline__ double lane63_d(double v)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = lane63_u((uint32_t)b);
+    const uint32_t hi = lane63_u((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// wave reductions (every lane gets the result)
 __device__ __forceinline__ float wave_max(float v)
 {
-#pragma unroll
-    for (int k = 32; k >= 1; k >>= 1)
-        v = fmaxf(v, __shfl_xor(v, k, 64));
-    return v;
+    const float n = -__builtin_huge_valf();
+    v = fmaxf(v, dpp_f<kDppXor1>(n, v));
+    v = fmaxf(v, dpp_f<kDppXor2>(n, v));
+    v = fmaxf(v, dpp_f<kDppHalfMirror>(n, v));
+    v = fmaxf(v, dpp_f<kDppMirror>(n, v));
+    v = fmaxf(v, dpp_f<kDppBcast15, 0xA>(n, v));
+    v = fmaxf(v, dpp_f<kDppBcast31, 0xC>(n, v));
+    return lane63_f(v);
+}
+
+__device__ __forceinline__ double dmax(double a, double b)
+{
+    return a < b ? b : a;
+}
+
+__device__ __forceinline__ double wave_max_d(double v)
+{
+    const double n = -__builtin_huge_val();
+    v = dmax(v, dpp_d<kDppXor1>(n, v));
+    v = dmax(v, dpp_d<kDppXor2>(n, v));
+    v = dmax(v, dpp_d<kDppHalfMirror>(n, v));
+    v = dmax(v, dpp_d<kDppMirror>(n, v));
+    v = dmax(v, dpp_d<kDppBcast15, 0xA>(n, v));
+    v = dmax(v, dpp_d<kDppBcast31, 0xC>(n, v));
+    return lane63_d(v);
 }
 
 __device__ __forceinline__ double wave_sum_d(double v)
 {
-#pragma unroll
-    for (int k = 32; k >= 1; k >>= 1)
-        v = __dadd_rn(v, __shfl_xor(v, k, 64));
-    return v;
+    v = __dadd_rn(v, dpp_d<kDppXor1>(0.0, v));
+    v = __dadd_rn(v, dpp_d<kDppXor2>(0.0, v));
+    v = __dadd_rn(v, dpp_d<kDppHalfMirror>(0.0, v));
+    v = __dadd_rn(v, dpp_d<kDppMirror>(0.0, v));
+    v = __dadd_rn(v, dpp_d<kDppBcast15, 0xA>(0.0, v));
+    v = __dadd_rn(v, dpp_d<kDppBcast31, 0xC>(0.0, v));
+    return lane63_d(v);
 }
 
 __device__ __forceinline__ uint32_t wave_sum_u(uint32_t v)
 {
-#pragma unroll
-    for (int k = 32; k >= 1; k >>= 1)
-        v += __shfl_xor(v, k, 64);
+    v += dpp_u<kDppXor1>(0u, v);
+    v += dpp_u<kDppXor2>(0u, v);
+    v += dpp_u<kDppHalfMirror>(0u, v);
+    v += dpp_u<kDppMirror>(0u, v);
+    v += dpp_u<kDppBcast15, 0xA>(0u, v);
+    v += dpp_u<kDppBcast31, 0xC>(0u, v);
+    return lane63_u(v);
+}
+
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v)
+{
+    const uint32_t n = 0xFFFFFFFFu;
+    v = min(v, dpp_u<kDppXor1>(n, v));
+    v = min(v, dpp_u<kDppXor2>(n, v));
+    v = min(v, dpp_u<kDppHalfMirror>(n, v));
+    v = min(v, dpp_u<kDppMirror>(n, v));
+    v = min(v, dpp_u<kDppBcast15, 0xA>(n, v));
+    v = min(v, dpp_u<kDppBcast31, 0xC>(n, v));
+    return lane63_u(v);
+}
+
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ dd dpp_dd(dd v)
+{
+    return dd{dpp_d<Ctrl, RowMask>(0.0, v.hi), dpp_d<Ctrl, RowMask>(0.0, v.lo)};
+}
+
+// wave-wide dd sum (every lane gets the total)
+__device__ __forceinline__ dd wave_sum_dd(dd v)
+{
+    v = dd_add(v, dpp_dd<kDppXor1>(v));
+    v = dd_add(v, dpp_dd<kDppXor2>(v));
+    v = dd_add(v, dpp_dd<kDppHalfMirror>(v));
+    v = dd_add(v, dpp_dd<kDppMirror>(v));
+    v = dd_add(v, dpp_dd<kDppBcast15, 0xA>(v));
+    v = dd_add(v, dpp_dd<kDppBcast31, 0xC>(v));
+    return dd{lane63_d(v.hi), lane63_d(v.lo)};
+}
+
+// inclusive wave scans (lane l gets the combination of lanes 0..l)
+__device__ __forceinline__ uint32_t wave_incl_sum_u(uint32_t v)
+{
+    v += dpp_u<kDppShr1>(0u, v);
+    v += dpp_u<kDppShr2>(0u, v);
+    v += dpp_u<kDppShr4>(0u, v);
+    v += dpp_u<kDppShr8>(0u, v);
+    v += dpp_u<kDppBcast15, 0xA>(0u, v);
+    v += dpp_u<kDppBcast31, 0xC>(0u, v);
     return v;
 }
+
+__device__ __forceinline__ uint32_t wave_incl_max_u(uint32_t v)
+{
+    v = max(v, dpp_u<kDppShr1>(0u, v));
+    v = max(v, dpp_u<kDppShr2>(0u, v));
+    v = max(v, dpp_u<kDppShr4>(0u, v));
+    v = max(v, dpp_u<kDppShr8>(0u, v));
+    v = max(v, dpp_u<kDppBcast15, 0xA>(0u, v));
+    v = max(v, dpp_u<kDppBcast31, 0xC>(0u, v));
+    return v;
+}
+
+__device__ __forceinline__ dd wave_incl_sum_dd(dd v)
+{
+    v = dd_add(dpp_dd<kDppShr1>(v), v);
+    v = dd_add(dpp_dd<kDppShr2>(v), v);
+    v = dd_add(dpp_dd<kDppShr4>(v), v);
+    v = dd_add(dpp_dd<kDppShr8>(v), v);
+    v = dd_add(dpp_dd<kDppBcast15, 0xA>(v), v);
+    v = dd_add(dpp_dd<kDppBcast31, 0xC>(v), v);
+    return v;
+}
+
+// ---- group (256-thread) reductions; `lds` is this group's scratch ----
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
 // group max; result valid in all threads of the group (gt = thread index
 // within the group, lds >= 4 floats)
@@ -263,7 +410,7 @@ __device__ __forceinline__ float group_max(float v, int gt, float *lds)
 }
 
 // group sum of two doubles in the canonical order: sequential per thread
-// (done by the caller), butterfly per wave, (w0 + w1) + (w2 + w3).
+// (done by the caller), the wave reduction above, (w0 + w1) + (w2 + w3).
 __device__ __forceinline__ void group_sum2(
     double &a, double &b, int gt, double *lds)
 {
@@ -285,13 +432,7 @@ __device__ __forceinline__ uint32_t group_excl_scan_u(
     uint32_t v, int gt, uint32_t *lds, uint32_t &total)
 {
     const int lane = gt & 63;
-    uint32_t incl = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_up(incl, d, 64);
-        if (lane >= d)
-            incl += o;
-    }
+    const uint32_t incl = wave_incl_sum_u(v);
     if (lane == 63)
         lds[gt >> 6] = incl;
     __syncthreads();
@@ -304,22 +445,29 @@ __device__ __forceinline__ uint32_t group_excl_scan_u(
     return off + incl - v;
 }
 
+// group min of a u32; result in every thread.  lds >= 4 u32.
+__device__ __forceinline__ uint32_t group_min_u(uint32_t v, int gt,
+    uint32_t *lds)
+{
+    v = wave_min_u(v);
+    if ((gt & 63) == 0)
+        lds[gt >> 6] = v;
+    __syncthreads();
+    uint32_t r = lds[0];
+    for (int w = 1; w < kWaves; ++w)
+        r = lds[w] < r ? lds[w] : r;
+    __syncthreads();
+    return r;
+}
+
 // exclusive max-scan of a u32 per thread over the group (init for thread 0).
 // lds >= 4 u32.
 __device__ __forceinline__ uint32_t group_excl_max_u(
     uint32_t v, uint32_t init, int gt, uint32_t *lds)
 {
     const int lane = gt & 63;
-    uint32_t incl = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_up(incl, d, 64);
-        if (lane >= d)
-            incl = incl > o ? incl : o;
-    }
-    uint32_t ex = __shfl_up(incl, 1, 64);
-    if (lane == 0)
-        ex = 0;
+    const uint32_t incl = wave_incl_max_u(v);
+    const uint32_t ex = dpp_u<kDppWaveShr1>(0u, incl); // lane 0: 0
     if (lane == 63)
         lds[gt >> 6] = incl;
     __syncthreads();
@@ -330,23 +478,15 @@ __device__ __forceinline__ uint32_t group_excl_max_u(
     return base > ex ? base : ex;
 }
 
-// exclusive dd scan per thread value over the group (fixed Hillis-Steele
-// per wave, waves in order).  lds >= 8 doubles.
+// exclusive dd scan per thread value over the group (the wave scan above,
+// waves in order).  lds >= 8 doubles.
 __device__ __forceinline__ dd group_excl_scan_dd(
     dd v, int gt, double *lds, dd &total)
 {
     const int lane = gt & 63;
-    dd incl = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const dd o = dd_shfl_up(incl, d);
-        if (lane >= d)
-            incl = dd_add(o, incl);
-    }
+    const dd incl = wave_incl_sum_dd(v);
     // exclusive within the wave: the inclusive value of lane-1
-    dd ex = dd_shfl_up(incl, 1);
-    if (lane == 0)
-        ex = dd{0.0, 0.0};
+    dd ex{dpp_d<kDppWaveShr1>(0.0, incl.hi), dpp_d<kDppWaveShr1>(0.0, incl.lo)};
     if (lane == 63) {
         lds[2 * (gt >> 6)] = incl.hi;
         lds[2 * (gt >> 6) + 1] = incl.lo;

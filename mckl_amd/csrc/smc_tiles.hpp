@@ -193,13 +193,26 @@ __device__ __forceinline__ uint32_t parent_of(
                               : code;
 }
 
+// Parent of slot code a: with src == nullptr the code is the parent itself
+// (idx holds resolved parents); otherwise a zero slot's code is kZeroFlag |
+// its zero-slot rank r and its parent is src[r], the source of extra copy r.
+// Parents are kept below N whatever the codes hold.
+__device__ __forceinline__ uint32_t resolve_parent(
+    uint32_t a, const uint32_t *src, uint32_t N)
+{
+    if (src && (a & kZeroFlag))
+        a = src[a & ~kZeroFlag];
+    return a < N ? a : N - 1;
+}
+
 // move + weight of one tile; returns the tile max of lw (all threads) and
 // the thread's new log weights in lout (0 past the end).  After a
-// resampling step idx holds every slot's resolved parent.
+// resampling step idx holds every slot's parent, or its code when src is
+// given (resolve_parent).
 __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
     uint32_t tile, const float *xin, float *xout, float *lw,
-    const uint32_t *idx, int resampled, float mprev, int gt, float *ldsf,
-    float (&lout)[kItems])
+    const uint32_t *idx, const uint32_t *src, int resampled, float mprev,
+    int gt, float *ldsf, float (&lout)[kItems])
 {
     const uint64_t i0 = tile_first(tile, gt);
     const float obs = A.obs[t - 1];
@@ -216,7 +229,7 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
                 a1.z, a1.w};
 #pragma unroll
             for (int k = 0; k < kItems; ++k) {
-                xp[k] = xin[a[k]];
+                xp[k] = xin[resolve_parent(a[k], src, A.N)];
                 lo[k] = 0.0f;
             }
         } else {
@@ -269,7 +282,7 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
                     break;
                 float xv, carry;
                 if (resampled) {
-                    xv = xin[idx[p]];
+                    xv = xin[resolve_parent(idx[p], src, A.N)];
                     carry = 0.0f;
                 } else {
                     xv = xin[p];
@@ -422,12 +435,14 @@ struct UniformSeq {
 // values the neighbouring tiles use at the shared boundaries, so the
 // replication numbers are never negative and sum to M whatever the rounding
 // of the prefix sums (the clamp changes nothing when those are monotone).
-// Returns the tile sums of the extra-copy and zero-slot counts through
+// Returns the thread's counts in c, the count before its first particle in
+// prev, and the tile sums of the extra-copy and zero-slot counts through
 // esum/zsum.  (l: the thread's 8 log weights, loaded by the caller)
-__device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
+__device__ __forceinline__ void tile_counts_c(const SmcArgs &A, uint32_t tile,
     const float (&l)[kItems], float mk, double rk, double invS, dd tile_off,
-    double next_hi, const UniformSeq &U, uint32_t *cnt, int gt, double *ldsd,
-    uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
+    double next_hi, const UniformSeq &U, int gt, double *ldsd,
+    uint32_t *ldsu, uint32_t (&c)[kItems], uint32_t &prev_out,
+    uint32_t &esum, uint32_t &zsum)
 {
     const uint64_t i0 = tile_first(tile, gt);
     const uint64_t tend = (uint64_t)(tile + 1) * kTile < A.N
@@ -447,7 +462,6 @@ __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
     }
     dd total;
     const dd toff = group_excl_scan_dd(loc, gt, ldsd, total);
-    uint32_t c[kItems];
     dd run = toff;
     uint32_t cmax = clo;
 #pragma unroll
@@ -477,7 +491,7 @@ __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
     for (int k = 0; k < kItems; ++k)
         if (i0 + k < A.N && c[k] < before)
             c[k] = before;
-    store8(cnt, i0, A.N, c);
+    prev_out = before;
     uint32_t es = 0, zs = 0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
@@ -492,6 +506,18 @@ __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
     group_sum2_u(es, zs, gt, ldsu);
     esum = es;
     zsum = zs;
+}
+
+// tile_counts_c, the counts stored to cnt
+__device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
+    const float (&l)[kItems], float mk, double rk, double invS, dd tile_off,
+    double next_hi, const UniformSeq &U, uint32_t *cnt, int gt, double *ldsd,
+    uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
+{
+    uint32_t c[kItems], prev;
+    tile_counts_c(A, tile, l, mk, rk, invS, tile_off, next_hi, U, gt, ldsd,
+        ldsu, c, prev, esum, zsum);
+    store8(cnt, tile_first(tile, gt), A.N, c);
 }
 
 __device__ __forceinline__ void tile_counts(const SmcArgs &A, uint32_t tile,
@@ -696,6 +722,64 @@ __device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
     uint32_t c[kItems], prev;
     load_counts(A, tile, cnt, gt, c, prev);
     tile_index_c(A, tile, c, prev, eoff, zoff, Eincl, idx, zpos, gt, ldsu);
+}
+
+// Stable rep->index with the zero slots' parents left as codes, for the
+// plain sampler: idx_i = i when rep_i > 0, else kZeroFlag | (zoff + the
+// slot's rank among the tile's zero slots); src[eoff + r] = the source of
+// the tile's extra copy r, the first particle j of the tile with E_j > r (E
+// the tile's inclusive extra-copy prefix, held in LDS; the ranks are dealt
+// round-robin to the group's threads).  The next move resolves a code
+// through src (resolve_parent), so the zero slot of rank r takes the source
+// of extra copy r: the order of resample.hpp:416-466.  eoff/zoff: the
+// extra-copy and zero-slot totals of the tiles before this one.  Contains
+// four workgroup barriers: every thread of the workgroup calls it.
+__device__ __forceinline__ void tile_index_src(const SmcArgs &A,
+    uint32_t tile, const uint32_t (&c)[kItems], uint32_t prev, uint32_t eoff,
+    uint32_t zoff, uint32_t *idx, uint32_t *src, int gt, uint32_t *ldsu,
+    uint32_t *Es)
+{
+    const uint64_t t0 = (uint64_t)tile * kTile;
+    const uint64_t i0 = tile_first(tile, gt);
+    uint32_t rep[kItems];
+    uint32_t es = 0, zs = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        rep[k] = 0;
+        if (i0 + k < A.N) {
+            rep[k] = rep_from_counts(c[k], prev);
+            prev = c[k];
+            es += extra_copies(rep[k]);
+            zs += rep[k] == 0 ? 1 : 0;
+        }
+    }
+    uint32_t te, tz;
+    uint32_t ex = group_excl_scan_u(es, gt, ldsu, te);
+    uint32_t zx = group_excl_scan_u(zs, gt, ldsu + 4, tz) + zoff;
+    uint32_t code[kItems], E[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        ex += extra_copies(rep[k]);
+        E[k] = ex;
+        code[k] = (uint32_t)i;
+        if (i < A.N && rep[k] == 0)
+            code[k] = kZeroFlag | zx++;
+    }
+    reinterpret_cast<uint4 *>(Es)[2 * gt] = make_uint4(E[0], E[1], E[2], E[3]);
+    reinterpret_cast<uint4 *>(Es)[2 * gt + 1] =
+        make_uint4(E[4], E[5], E[6], E[7]);
+    store8(idx, i0, A.N, code);
+    __syncthreads();
+    const uint32_t n = t0 >= A.N ? 0u
+        : (uint32_t)(A.N - t0 < (uint64_t)kTile ? A.N - t0 : kTile);
+    const uint32_t end = n ? Es[n - 1] : 0u;
+    for (uint32_t r = gt; r < end; r += kGroup) {
+        const uint64_t g = (uint64_t)eoff + r;
+        if (g < A.N)
+            src[g] = (uint32_t)(t0 + rank_source(Es, n, r)); // Es[n-1] > r
+    }
+    __syncthreads();
 }
 
 // Zero slots take their sources.  Source j with rep_j > 1 owns the extra
