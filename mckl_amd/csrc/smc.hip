@@ -54,7 +54,7 @@ struct RsBuf {
     double *uh, *ul;   // multinomial: scaled tile dd totals r T
     uint32_t *ti;      // residual: tile sums of the integral parts
     double *trh, *trl; // residual: tile dd totals of the residuals
-    uint32_t *src;     // source of each extra-copy rank (tile_index_src)
+    uint32_t *src;     // source of each extra-copy rank (tile_index_store)
     uint64_t *lbw;     // look-back words, 2 x ntiles (by step parity)
 };
 
@@ -116,7 +116,7 @@ __device__ __forceinline__ float tiles_max(
 // tile's log weights, its chunk of every per-tile array), so the kernel
 // waits on memory once; the reductions are those of tiles_max, tiles_sum2
 // and tiles_offset_dd, operation for operation.
-__host__ __device__ inline bool is_residual(int scheme)
+__host__ __device__ constexpr bool is_residual(int scheme)
 {
     return scheme == MCKL_HIP_RESIDUAL ||
         scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
@@ -155,8 +155,16 @@ __device__ __forceinline__ void lb_store(uint64_t *p, uint64_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// tile k's aggregate (or, for tile 0, inclusive) word; one thread calls it
+__device__ __forceinline__ void lb_publish(
+    uint64_t *w, uint32_t k, uint32_t te, uint32_t tz)
+{
+    lb_store(&w[k], lb_pack(k == 0 ? kLbIncl : kLbAgg, te, tz));
+}
+
 // exclusive prefix (eoff, zoff) of tile k's totals (te, tz) over the tiles
-// before it.  Each round reads kLbQ predecessor words per thread (a window
+// before it, after lb_publish.  Each round reads kLbQ predecessor words per
+// thread (a window
 // of kLbQ x 256 tiles, one memory round trip), waits until each holds a
 // status, and adds the words up to the nearest inclusive one; a launch of at
 // most 1024 tiles finishes in one round.  Every thread of the workgroup
@@ -167,8 +175,6 @@ __device__ __forceinline__ void tile_lookback(uint64_t *w, uint32_t k,
     uint32_t te, uint32_t tz, int gt, uint32_t *ldsu, int *err,
     uint32_t &eoff, uint32_t &zoff)
 {
-    if (gt == 0)
-        lb_store(&w[k], lb_pack(k == 0 ? kLbIncl : kLbAgg, te, tz));
     uint32_t es = 0, zs = 0, spins = 0;
     bool failed = false;
     int64_t top = (int64_t)k - 1; // the window's nearest predecessor
@@ -221,13 +227,18 @@ __device__ __forceinline__ void tile_lookback(uint64_t *w, uint32_t k,
     zoff = zs;
 }
 
+// one instantiation per scheme S (a compile-time copy of A.scheme), so each
+// launch carries only its own scheme's code
+template <int S>
 __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     const float *lw, const float *tmax, const double *ts, const double *ts2,
     RsBuf B, SmcState *st, double *ess_hist, int32_t *res_hist,
     uint32_t *idx)
 {
+    A.scheme = S;
     __shared__ __align__(16) unsigned char lds[kScratch];
     __shared__ __align__(16) uint32_t Es[kTile];
+    __shared__ uint32_t Srcl[kTile];
     const GroupLds L = group_lds(lds);
     const int gt = threadIdx.x;
     const int nt = (int)A.ntiles;
@@ -240,6 +251,7 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     float l[kItems];
     load8(lw, tile_first(k, gt), A.N, l, 0.0f);
     const double Cprev = st->C[p];
+    const float tk = tmax[k];
     const int chunk = (nt + kGroup - 1) / kGroup;
     const int b = gt * chunk;
     const int e = b + chunk < nt ? b + chunk : nt;
@@ -274,14 +286,17 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     // and, rounded once, the total that normalises the cumulative weights)
     double sa = 0.0, sb = 0.0;
     dd loc{0.0, 0.0};
+    dd pu[kPre]; // the scaled dd tile totals r_i T_i (pre path)
     if (pre) {
 #pragma unroll
         for (int q = 0; q < kPre; ++q) {
+            pu[q] = dd{0.0, 0.0};
             if (b + q < e) {
                 const double r = tile_scale(pt[q], m);
                 sa = __dadd_rn(sa, __dmul_rn(r, ps[q]));
                 sb = __dadd_rn(sb, __dmul_rn(__dmul_rn(r, r), ps2[q]));
-                loc = dd_add(loc, dd_mul_d(dd{ph[q], pl[q]}, r));
+                pu[q] = dd_mul_d(dd{ph[q], pl[q]}, r);
+                loc = dd_add(loc, pu[q]);
             }
         }
     } else {
@@ -312,8 +327,8 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     }
     if (!o.resample)
         return;
-    const double rk = tile_scale(tmax[k], m);
-    if (A.scheme == MCKL_HIP_MULTINOMIAL) {
+    const double rk = tile_scale(tk, m);
+    if constexpr (S == MCKL_HIP_MULTINOMIAL) {
         // multinomial: tile totals of the log spacings; the uniforms and
         // the counts follow in k_rs_uniforms and k_rs_mcounts, which read
         // this tile's scaled total from B.uh/B.ul
@@ -334,15 +349,31 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     double *x = L.d + 8;
     {
         dd run = off;
-        for (int i = b; i < e; ++i) {
-            if (i == (int)k) {
-                x[0] = run.hi;
-                x[1] = run.lo;
+        if (pre) {
+#pragma unroll
+            for (int q = 0; q < kPre; ++q) {
+                const int i = b + q;
+                if (i < e) {
+                    if (i == (int)k) {
+                        x[0] = run.hi;
+                        x[1] = run.lo;
+                    }
+                    if (i == (int)k + 1)
+                        x[2] = run.hi;
+                    run = dd_add(run, pu[q]);
+                }
             }
-            if (i == (int)k + 1)
-                x[2] = run.hi;
-            run = dd_add(run,
-                dd_mul_d(dd{B.tdh[i], B.tdl[i]}, tile_scale(tmax[i], m)));
+        } else {
+            for (int i = b; i < e; ++i) {
+                if (i == (int)k) {
+                    x[0] = run.hi;
+                    x[1] = run.lo;
+                }
+                if (i == (int)k + 1)
+                    x[2] = run.hi;
+                run = dd_add(run, dd_mul_d(dd{B.tdh[i], B.tdl[i]},
+                                      tile_scale(tmax[i], m)));
+            }
         }
         if (gt == 0 && (int)k + 1 == nt)
             x[2] = total.hi;
@@ -352,13 +383,13 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     const double next_hi = x[2];
     __syncthreads();
     const double invS = __ddiv_rn(1.0, total.hi);
-    if (is_residual(A.scheme)) {
+    if constexpr (is_residual(S)) {
         // residual: the tile's integral parts and residual total; the
         // counts follow in k_rs_rcounts (after k_rs_rterms and
         // k_rs_uniforms for residual multinomial)
         uint32_t isum;
         dd rt;
-        tile_residual_sums(A, k, l, tmax[k], rk, invS, gt, L.d, L.u, isum, rt);
+        tile_residual_sums(A, k, l, tk, rk, invS, gt, L.d, L.u, isum, rt);
         if (gt == 0) {
             B.ti[k] = isum;
             B.trh[k] = rt.hi;
@@ -370,15 +401,20 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     }
     // systematic / stratified: the counts, the tile's place among the
     // extra copies and zero slots by look-back, and the rep -> index
-    // transform (parents as codes, resolved by the next move)
+    // transform (parents as codes, resolved by the next move); the tile's
+    // own part of the transform runs while its predecessors publish
     const UniformSeq U = make_uniform_seq(A, t);
     uint32_t es, zs, cs[kItems], prev;
-    tile_counts_c(A, k, l, tmax[k], rk, invS, toff, next_hi, U, gt, L.d,
+    tile_counts_c(A, k, l, tk, rk, invS, toff, next_hi, U, gt, L.d,
         L.u, cs, prev, es, zs);
+    uint64_t *w = B.lbw + (size_t)c * nt;
+    if (gt == 0)
+        lb_publish(w, k, es, zs);
+    TileIndex ti;
+    tile_index_local(A, k, cs, prev, gt, L.u, Es, Srcl, ti);
     uint32_t eoff, zoff;
-    tile_lookback(B.lbw + (size_t)c * nt, k, es, zs, gt, L.u, &st->err, eoff,
-        zoff);
-    tile_index_src(A, k, cs, prev, eoff, zoff, idx, B.src, gt, L.u, Es);
+    tile_lookback(w, k, es, zs, gt, L.u, &st->err, eoff, zoff);
+    tile_index_store(A, k, ti, eoff, zoff, idx, B.src, gt, Es, Srcl);
 }
 
 // R = M - sum of the tile sums of the integral parts (all threads)
@@ -554,13 +590,16 @@ __global__ __launch_bounds__(kGroup) void k_rs_index(SmcArgs A, uint32_t t,
         return;
     __shared__ __align__(16) unsigned char lds[kScratch];
     __shared__ __align__(16) uint32_t Es[kTile];
+    __shared__ uint32_t Srcl[kTile];
     const GroupLds L = group_lds(lds);
     uint32_t c[kItems], prev;
     load_counts(A, blockIdx.x, B.cnt, threadIdx.x, c, prev);
     uint32_t eoff, zoff;
     tiles_prefix_u2(B.te, B.tz, blockIdx.x, threadIdx.x, L.u, eoff, zoff);
-    tile_index_src(A, blockIdx.x, c, prev, eoff, zoff, idx, B.src,
-        threadIdx.x, L.u, Es);
+    TileIndex ti;
+    tile_index_local(A, blockIdx.x, c, prev, threadIdx.x, L.u, Es, Srcl, ti);
+    tile_index_store(A, blockIdx.x, ti, eoff, zoff, idx, B.src, threadIdx.x,
+        Es, Srcl);
 }
 
 // -------------------------------------------------------------------------
@@ -1025,6 +1064,34 @@ static void smc_free(mckl_hip_smc *s)
     delete s;
 }
 
+template <typename... Args>
+static void launch_rs_counts(int scheme, dim3 grid, hipStream_t st,
+    Args... a)
+{
+    switch (scheme) {
+    case MCKL_HIP_MULTINOMIAL:
+        k_rs_counts<MCKL_HIP_MULTINOMIAL><<<grid, kGroup, 0, st>>>(a...);
+        break;
+    case MCKL_HIP_STRATIFIED:
+        k_rs_counts<MCKL_HIP_STRATIFIED><<<grid, kGroup, 0, st>>>(a...);
+        break;
+    case MCKL_HIP_SYSTEMATIC:
+        k_rs_counts<MCKL_HIP_SYSTEMATIC><<<grid, kGroup, 0, st>>>(a...);
+        break;
+    case MCKL_HIP_RESIDUAL:
+        k_rs_counts<MCKL_HIP_RESIDUAL><<<grid, kGroup, 0, st>>>(a...);
+        break;
+    case MCKL_HIP_RESIDUAL_STRATIFIED:
+        k_rs_counts<MCKL_HIP_RESIDUAL_STRATIFIED><<<grid, kGroup, 0, st>>>(
+            a...);
+        break;
+    default:
+        k_rs_counts<MCKL_HIP_RESIDUAL_SYSTEMATIC><<<grid, kGroup, 0, st>>>(
+            a...);
+        break;
+    }
+}
+
 extern "C" {
 
 int mckl_hip_smc_create(int model, const double *theta, size_t T,
@@ -1133,8 +1200,8 @@ static int smc_enqueue(mckl_hip_smc *s, hipStream_t st)
         if (s->kmom)
             k_est_moments<<<grid, kGroup, 0, st>>>(s->A, t, xout, s->lw,
                 s->tmax, s->kmom, s->eph, s->epl);
-        k_rs_counts<<<grid, kGroup, 0, st>>>(s->A, t, s->lw, s->tmax, s->ts,
-            s->ts2, s->B, s->st, s->ess, s->res, s->idx);
+        launch_rs_counts(s->A.scheme, grid, st, s->A, t, s->lw, s->tmax,
+            s->ts, s->ts2, s->B, s->st, s->ess, s->res, s->idx);
         if (s->A.scheme == MCKL_HIP_MULTINOMIAL) {
             k_rs_uniforms<<<grid, kGroup, 0, st>>>(s->A, t, s->st, s->B);
             k_rs_mcounts<<<grid, kGroup, 0, st>>>(

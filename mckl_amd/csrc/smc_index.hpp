@@ -17,6 +17,7 @@
 //   * the source of a rank is a particle of the tile (< n).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 
 namespace mckl_hip {
@@ -62,6 +63,64 @@ __host__ __device__ inline uint32_t rank_source(
             lo = mid + 1;
     }
     return lo;
+}
+
+// ---- counts of resampling uniforms below a cumulative weight known to
+// within a margin ----
+// The samplers evaluate a particle's cumulative weight S in double (Sf)
+// with a bound mg on |S - Sf|, S the double-double prefix rounded once.
+// When no uniform can lie within mg of Sf the count #{j < M : u_j < S} is
+// the same for every S in [Sf - mg, Sf + mg] and follows from the uniforms'
+// closed form; otherwise these return false and the caller counts against
+// the exact S.  With unit roundoff u = 2^-53 and x = fl(fl(Sf - u0d) M):
+//   systematic u_j = fl(fl(j delta) + u0d), delta = fl(1/M), u0d < delta:
+//     |u_j - (j/M + u0d)| <= 3u (+ O(u^2)) and |x - (Sf - u0d) M| <= 2uM,
+//     so u_j < S <=> j < x + th with |th| <= (mg + 5u) M: if x is farther
+//     than D = (mg + 8u) M from every integer the count is
+//     clamp(ceil(x), 0, M);
+//   stratified u_j = fl(delta fl(U_j + j)), U_j in [0, 1), x = fl(Sf M):
+//     u_j <= (j+1)/M + 3u and u_j >= j/M - 3u, so with x farther than D
+//     from every integer every j < floor(x) counts, no j > floor(x) does,
+//     and j = floor(x) counts when its own u_j (evaluated by the caller)
+//     lies below Sf - mg, not when it lies at or above Sf + mg.
+constexpr double kUnitRoundoff = 1.1102230246251565e-16; // 2^-53
+
+__host__ __device__ inline bool count_sys_guarded(
+    double Sf, double mg, double u0d, uint32_t M, uint32_t &c)
+{
+    const double x = (Sf - u0d) * (double)M;
+    const double D = (mg + 8.0 * kUnitRoundoff) * (double)M;
+    const double r = floor(x + 0.5);
+    if (!(fabs(x - r) > D)) // also catches NaN
+        return false;
+    const double cx = ceil(x);
+    c = cx <= 0.0 ? 0u : (cx >= (double)M ? M : (uint32_t)cx);
+    return true;
+}
+
+// stratified: the index j = floor(x) whose own uniform decides (j < M), or
+// the count itself when every uniform lies on one side (all counts: M).
+// Returns 0 when the margin does not hold, 1 when c is the count, 2 when
+// the caller must compare u_c with Sf -/+ mg (count c + 1 below, c above).
+__host__ __device__ inline int count_strat_guarded(
+    double Sf, double mg, uint32_t M, uint32_t &c)
+{
+    const double x = Sf * (double)M;
+    const double D = (mg + 8.0 * kUnitRoundoff) * (double)M;
+    const double r = floor(x + 0.5);
+    if (!(fabs(x - r) > D))
+        return 0;
+    const double fx = floor(x);
+    if (fx < 0.0) {
+        c = 0;
+        return 1;
+    }
+    if (fx >= (double)M) {
+        c = M;
+        return 1;
+    }
+    c = (uint32_t)fx;
+    return 2;
 }
 
 } // namespace mckl_hip

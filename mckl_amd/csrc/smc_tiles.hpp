@@ -423,7 +423,33 @@ struct UniformSeq {
             return c + in;
         return count_less(S);
     }
+
+    // The count for every S within mg of Sf when they all agree (see
+    // count_sys_guarded / count_strat_guarded); false when a uniform may lie
+    // within mg of Sf, and always for the multinomial scheme.
+    __device__ __forceinline__ bool count_guarded(
+        double Sf, double mg, uint32_t &c) const
+    {
+        if (scheme == MCKL_HIP_SYSTEMATIC)
+            return count_sys_guarded(Sf, mg, u0d, M, c);
+        if (scheme != MCKL_HIP_STRATIFIED)
+            return false;
+        const int g = count_strat_guarded(Sf, mg, M, c);
+        if (g != 2)
+            return g == 1;
+        const double uc = (*this)(c);
+        if (uc < __dsub_rn(Sf, mg)) {
+            ++c;
+            return true;
+        }
+        return uc >= __dadd_rn(Sf, mg);
+    }
 };
+
+// relative bound on |S - Sf| for the double evaluation of a cumulative
+// weight in tile_counts_c: 64 unit roundoffs (the evaluation's own bound is
+// about 17)
+constexpr double kCountMargin = 64.0 * kUnitRoundoff;
 
 // Cumulative normalised weights S_j = round(prefix_j(e)) * (1/S), prefix in
 // double-double: the tile's offset (tile_off, from the exclusive scan of the
@@ -453,31 +479,43 @@ __device__ __forceinline__ void tile_counts_c(const SmcArgs &A, uint32_t tile,
         tile == 0 ? 0u : U.count_fast(__dmul_rn(tile_off.hi, invS));
     const uint32_t chi =
         tend == A.N ? U.M : U.count_fast(__dmul_rn(next_hi, invS));
+    float a[kItems];
     dd loc{0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        const double w = i < A.N ? (double)smc_exp_f(__fsub_rn(l[k], mk)) : 0.0;
-        loc = dd_add(loc, dd{w, 0.0});
+        a[k] = i < A.N ? smc_exp_f(__fsub_rn(l[k], mk)) : 0.0f;
+        loc = dd_add(loc, dd{(double)a[k], 0.0});
     }
     dd total;
     const dd toff = group_excl_scan_dd(loc, gt, ldsd, total);
-    dd run = toff;
+    // P_i = tile_off + r_k (toff + sum_{j <= i} a_j): in double first
+    // (relative error below 17 unit roundoffs, all terms non-negative); the
+    // double-double value rounded once only where a uniform may lie within
+    // the bound of the double one (count_guarded), so the count is that of
+    // the rounded double-double prefix in every case
+    double rd = toff.hi;
     uint32_t cmax = clo;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        // the same a_i as above (recomputed: fewer live registers)
-        const double w = i < A.N ? (double)smc_exp_f(__fsub_rn(l[k], mk)) : 0.0;
-        run = dd_add(run, dd{w, 0.0});
+        rd = __dadd_rn(rd, (double)a[k]);
         c[k] = 0;
         if (i < A.N) {
             uint32_t v;
             if (i == tend - 1) {
                 v = chi;
             } else {
-                const double P = dd_add(tile_off, dd_mul_d(run, rk)).hi;
-                v = U.count_fast(__dmul_rn(P, invS));
+                const double Pf = __dadd_rn(__dmul_rn(rd, rk), tile_off.hi);
+                const double Sf = __dmul_rn(Pf, invS);
+                if (!U.count_guarded(Sf, __dmul_rn(Sf, kCountMargin), v)) {
+                    dd run = toff;
+#pragma unroll
+                    for (int q = 0; q <= k; ++q)
+                        run = dd_add(run, dd{(double)a[q], 0.0});
+                    const double P = dd_add(tile_off, dd_mul_d(run, rk)).hi;
+                    v = U.count_fast(__dmul_rn(P, invS));
+                }
             }
             v = clamp_count(v, clo, chi);
             cmax = v > cmax ? v : cmax;
@@ -728,16 +766,58 @@ __device__ __forceinline__ void tile_index(const SmcArgs &A, uint32_t tile,
 // plain sampler: idx_i = i when rep_i > 0, else kZeroFlag | (zoff + the
 // slot's rank among the tile's zero slots); src[eoff + r] = the source of
 // the tile's extra copy r, the first particle j of the tile with E_j > r (E
-// the tile's inclusive extra-copy prefix, held in LDS; the ranks are dealt
-// round-robin to the group's threads).  The next move resolves a code
-// through src (resolve_parent), so the zero slot of rank r takes the source
-// of extra copy r: the order of resample.hpp:416-466.  eoff/zoff: the
-// extra-copy and zero-slot totals of the tiles before this one.  Contains
-// four workgroup barriers: every thread of the workgroup calls it.
-__device__ __forceinline__ void tile_index_src(const SmcArgs &A,
-    uint32_t tile, const uint32_t (&c)[kItems], uint32_t prev, uint32_t eoff,
-    uint32_t zoff, uint32_t *idx, uint32_t *src, int gt, uint32_t *ldsu,
-    uint32_t *Es)
+// the tile's inclusive extra-copy prefix, held in LDS).  The next move
+// resolves a code through src (resolve_parent), so the zero slot of rank r
+// takes the source of extra copy r: the order of resample.hpp:416-466.
+// eoff/zoff: the extra-copy and zero-slot totals of the tiles before this
+// one.  Two parts, so that a caller can wait for eoff/zoff between them:
+// tile_index_local does everything the offsets do not enter (the codes'
+// local ranks, E in LDS, and, when the tile holds at most kTile extra
+// copies, each rank's source in LDS), tile_index_store adds the offsets
+// and writes idx and src.  Each contains workgroup barriers: every thread
+// of the workgroup calls them.
+struct TileIndex {
+    uint32_t code[kItems]; // own index, or kZeroFlag | local zero rank
+    uint32_t n;            // the tile's particles
+    uint32_t end;          // the tile's extra copies
+};
+
+// four independent rank -> source searches at once (their LDS round trips
+// overlap); ranks at or past `end` are skipped
+__device__ __forceinline__ void rank_sources4(const uint32_t *Es, uint32_t n,
+    uint32_t end, uint32_t r0, uint32_t stride, uint32_t (&j)[4])
+{
+    uint32_t lo[4], hi[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        lo[q] = 0;
+        hi[q] = r0 + q * stride < end ? n - 1 : 0u;
+    }
+    // at most ceil(log2(kTile)) halvings
+    for (int it = 0; it < 12; ++it) {
+        bool more = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (lo[q] < hi[q]) {
+                const uint32_t mid = (lo[q] + hi[q]) >> 1;
+                if (Es[mid] > r0 + q * stride)
+                    hi[q] = mid;
+                else
+                    lo[q] = mid + 1;
+                more = true;
+            }
+        }
+        if (!more)
+            break;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        j[q] = lo[q];
+}
+
+__device__ __forceinline__ void tile_index_local(const SmcArgs &A,
+    uint32_t tile, const uint32_t (&c)[kItems], uint32_t prev, int gt,
+    uint32_t *ldsu, uint32_t *Es, uint32_t *srcl, TileIndex &ti)
 {
     const uint64_t t0 = (uint64_t)tile * kTile;
     const uint64_t i0 = tile_first(tile, gt);
@@ -755,31 +835,83 @@ __device__ __forceinline__ void tile_index_src(const SmcArgs &A,
     }
     uint32_t te, tz;
     uint32_t ex = group_excl_scan_u(es, gt, ldsu, te);
-    uint32_t zx = group_excl_scan_u(zs, gt, ldsu + 4, tz) + zoff;
-    uint32_t code[kItems], E[kItems];
+    uint32_t zx = group_excl_scan_u(zs, gt, ldsu + 4, tz);
+    ti.n = t0 >= A.N ? 0u
+        : (uint32_t)(A.N - t0 < (uint64_t)kTile ? A.N - t0 : kTile);
+    ti.end = te;
+    const bool small = te <= (uint32_t)kTile; // uniform over the group
+    if (small) {
+        reinterpret_cast<uint4 *>(srcl)[2 * gt] = make_uint4(0, 0, 0, 0);
+        reinterpret_cast<uint4 *>(srcl)[2 * gt + 1] = make_uint4(0, 0, 0, 0);
+        __syncthreads();
+    }
+    uint32_t E[kItems];
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
+        // a source's extra copies are the ranks [ex, ex + rep - 1): it
+        // marks the first with its own index
+        if (small && rep[k] > 1 && ex < (uint32_t)kTile)
+            srcl[ex] = (uint32_t)(gt * kItems + k);
         ex += extra_copies(rep[k]);
         E[k] = ex;
-        code[k] = (uint32_t)i;
+        ti.code[k] = (uint32_t)i;
         if (i < A.N && rep[k] == 0)
-            code[k] = kZeroFlag | zx++;
+            ti.code[k] = kZeroFlag | zx++;
     }
     reinterpret_cast<uint4 *>(Es)[2 * gt] = make_uint4(E[0], E[1], E[2], E[3]);
     reinterpret_cast<uint4 *>(Es)[2 * gt + 1] =
         make_uint4(E[4], E[5], E[6], E[7]);
-    store8(idx, i0, A.N, code);
     __syncthreads();
-    const uint32_t n = t0 >= A.N ? 0u
-        : (uint32_t)(A.N - t0 < (uint64_t)kTile ? A.N - t0 : kTile);
-    const uint32_t end = n ? Es[n - 1] : 0u;
-    for (uint32_t r = gt; r < end; r += kGroup) {
-        const uint64_t g = (uint64_t)eoff + r;
-        if (g < A.N)
-            src[g] = (uint32_t)(t0 + rank_source(Es, n, r)); // Es[n-1] > r
+    if (small) {
+        // every rank's source: the last mark at or before it (sources mark
+        // increasing ranks in particle order), by a max-scan over the ranks
+        const uint4 a = reinterpret_cast<const uint4 *>(srcl)[2 * gt];
+        const uint4 b = reinterpret_cast<const uint4 *>(srcl)[2 * gt + 1];
+        uint32_t v[kItems] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 1; k < kItems; ++k)
+            v[k] = v[k] > v[k - 1] ? v[k] : v[k - 1];
+        const uint32_t before = group_excl_max_u(v[kItems - 1], 0u, gt, ldsu);
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            v[k] = v[k] > before ? v[k] : before;
+        reinterpret_cast<uint4 *>(srcl)[2 * gt] =
+            make_uint4(v[0], v[1], v[2], v[3]);
+        reinterpret_cast<uint4 *>(srcl)[2 * gt + 1] =
+            make_uint4(v[4], v[5], v[6], v[7]);
+        __syncthreads();
     }
-    __syncthreads();
+}
+
+__device__ __forceinline__ void tile_index_store(const SmcArgs &A,
+    uint32_t tile, const TileIndex &ti, uint32_t eoff, uint32_t zoff,
+    uint32_t *idx, uint32_t *src, int gt, const uint32_t *Es,
+    const uint32_t *srcl)
+{
+    const uint64_t t0 = (uint64_t)tile * kTile;
+    const uint64_t i0 = tile_first(tile, gt);
+    uint32_t code[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        code[k] = (ti.code[k] & kZeroFlag) ? ti.code[k] + zoff : ti.code[k];
+    store8(idx, i0, A.N, code);
+    // ranks with a slot in src: below N - eoff (the loop stays bounded even
+    // for counts that are not a replication vector)
+    const uint32_t end = rank_end(ti.end, eoff < A.N ? A.N - eoff : 0u);
+    if (ti.end <= (uint32_t)kTile) { // sources from tile_index_local
+        for (uint32_t r = gt; r < end; r += kGroup)
+            src[eoff + r] = (uint32_t)(t0 + srcl[r]);
+    } else {
+        for (uint32_t r0 = gt; r0 < end; r0 += 4 * kGroup) {
+            uint32_t j[4];
+            rank_sources4(Es, ti.n, end, r0, kGroup, j);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (r0 + q * kGroup < end)
+                    src[eoff + r0 + q * kGroup] = (uint32_t)(t0 + j[q]);
+        }
+    }
 }
 
 // Zero slots take their sources.  Source j with rep_j > 1 owns the extra

@@ -10,6 +10,13 @@ the tile stages around it and checks every array access).
   resample_trans_rep_index), and no access leaves its array.
 * With boundary counts that disagree between neighbouring tiles (which the
   kernels never produce), no access leaves its array either.
+* The plain sampler's form (zero slots coded by rank, resolved through a
+  rank -> source table written at each tile's look-back offset) gives the
+  same parents and stays inside its arrays on the same inputs.
+* The guarded counts (a cumulative weight known in double to within a
+  margin, counted from the uniforms' closed form) equal brute-force counts
+  of the systematic and stratified uniforms at every weight they claim,
+  with weights placed on, beside and between uniforms.
 * The form before the clamp and the rank bound, replayed on the CPU, reads
   the zero-slot array past its end on such inputs: the out-of-range access
   behind the GPU fault of an earlier resampling-kernel variant.
@@ -35,10 +42,13 @@ def rg():
     if (not os.path.exists(LIB) or
             os.path.getmtime(LIB) < max(os.path.getmtime(SRC),
                                         os.path.getmtime(HDR))):
-        subprocess.run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared",
-                        "-Wall", "-o", LIB, SRC], check=True)
+        subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off",
+                        "-fPIC", "-shared", "-Wall", "-o", LIB, SRC],
+                       check=True)
     lib = ctypes.CDLL(LIB)
     lib.rs_guard_run.restype = ctypes.c_uint64
+    lib.rs_guard_src_run.restype = ctypes.c_uint64
+    lib.guard_count_check.restype = ctypes.c_uint64
     return lib
 
 
@@ -141,3 +151,73 @@ def test_unguarded_form_reads_past_the_zero_slots(rg):
     assert rep.astype(np.int64).sum() == N and rep.max() <= N
     assert np.array_equal(idx.astype(np.uint64),
                           orc.rep_index(rep.astype(np.uint64), N))
+
+
+def run_src(rg, raw, clo, chi):
+    N = len(raw)
+    raw = np.ascontiguousarray(raw, np.uint32)
+    clo = np.ascontiguousarray(clo, np.uint32)
+    chi = np.ascontiguousarray(chi, np.uint32)
+    idx = np.zeros(max(N, 1), np.uint32)
+    rep = np.zeros(max(N, 1), np.uint32)
+    bad = rg.rs_guard_src_run(ctypes.c_uint64(N), P(raw), P(clo), P(chi),
+                              P(idx), P(rep))
+    return int(bad), idx[:N], rep[:N]
+
+
+@pytest.mark.parametrize("N", SIZES + [5 * 2048 + 17])
+def test_rank_source_form_equals_the_reference_transform(rg, N):
+    rng = np.random.default_rng(3 * N + 1)
+    for name, raw in patterns(N, rng):
+        for jitter in (None, 1, N):
+            clo, chi = boundaries(raw, N, jitter, rng)
+            bad, idx, rep = run_src(rg, raw, clo, chi)
+            assert bad == 0, (name, jitter)
+            assert np.all(idx < N), (name, jitter)
+            if jitter is None:
+                assert rep.astype(np.int64).sum() == N, name
+                want = orc.rep_index(rep.astype(np.uint64), N)
+                assert np.array_equal(idx.astype(np.uint64), want), name
+                _, idx_old, rep_old = run(rg, raw, clo, chi)
+                assert np.array_equal(rep, rep_old), name
+                assert np.array_equal(idx, idx_old), name
+
+
+MARGIN = 64 * 2.0 ** -53  # kCountMargin
+
+
+def weights_near_uniforms(u, rng, n):
+    """Cumulative weights on uniforms, a few ulps either side, at the
+    margin's edges, between uniforms and past both ends."""
+    j = rng.integers(0, len(u), n)
+    base = u[j]
+    k = rng.integers(-300, 301, n).astype(np.float64)
+    out = [base * (1.0 + k * 2.0 ** -53),
+           base * (1.0 + MARGIN * rng.uniform(-3, 3, n)),
+           rng.uniform(0.0, 1.0, n),
+           np.array([0.0, 1e-300, u[0], u[-1], 1.0, 1.5])]
+    return np.ascontiguousarray(np.concatenate(out))
+
+
+@pytest.mark.parametrize("stratified", [0, 1])
+@pytest.mark.parametrize("M", [1, 2, 3, 7, 1000, 2048, 4101, 1 << 20,
+                               (1 << 20) + 12345])
+def test_guarded_counts_equal_brute_force(rg, stratified, M):
+    rng = np.random.default_rng(M * 2 + stratified)
+    u01 = float(rng.uniform())
+    us = np.ascontiguousarray(rng.uniform(0.0, 1.0, M))
+    delta = 1.0 / M
+    if stratified:
+        u = delta * (us + np.arange(M, dtype=np.float64))
+    else:
+        u = np.arange(M, dtype=np.float64) * delta + u01 * delta
+    S = weights_near_uniforms(u, rng, 20000)
+    nok = ctypes.c_uint64(0)
+    bad = rg.guard_count_check(stratified, ctypes.c_uint32(M),
+                               ctypes.c_double(u01), P(us), P(S),
+                               ctypes.c_uint64(len(S)),
+                               ctypes.c_double(MARGIN), ctypes.byref(nok))
+    assert bad == 0
+    # the guard decides most weights itself (those on or within a few ulps
+    # of a uniform go to the exact path)
+    assert nok.value > len(S) // 4
