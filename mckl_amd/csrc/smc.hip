@@ -86,8 +86,8 @@ __global__ __launch_bounds__(kGroup) void k_move(SmcArgs A, uint32_t t,
     const GroupLds L = group_lds(lds);
     const int p = (t - 1) & 1;
     float l[kItems];
-    const float mt = tile_move(A, t, blockIdx.x, xin, xout, lw, idx, B.src,
-        st->resampled[p], st->m[p], threadIdx.x, L.f, l);
+    const float mt = tile_move<true>(A, t, blockIdx.x, xin, xout, lw, idx,
+        B.src, st->resampled[p], st->m[p], threadIdx.x, L.f, l);
     double S, S2;
     dd et;
     tile_sums_l(A, blockIdx.x, l, mt, threadIdx.x, L.d, S, S2, et);
@@ -405,8 +405,11 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     // own part of the transform runs while its predecessors publish
     const UniformSeq U = make_uniform_seq(A, t);
     uint32_t es, zs, cs[kItems], prev;
-    tile_counts_c(A, k, l, tk, rk, invS, toff, next_hi, U, gt, L.d,
-        L.u, cs, prev, es, zs);
+    float ta[kItems];
+    dd ttoff;
+    tile_counts_local(A, k, l, tk, gt, L.d, ta, ttoff);
+    tile_counts_c(A, k, ta, ttoff, rk, invS, toff, next_hi, U, gt, L.u, cs,
+        prev, es, zs);
     uint64_t *w = B.lbw + (size_t)c * nt;
     if (gt == 0)
         lb_publish(w, k, es, zs);
@@ -771,8 +774,8 @@ __global__ __launch_bounds__(G *kGroup, G == 1 ? 2 : 1) void k_batch(const Filte
                 const int k = r * G + grp;
                 const uint32_t kk = (uint32_t)(k < nt ? k : nt);
                 float l[kItems];
-                const float mt = tile_move(A, t, kk, xin, xout, lw, idx,
-                    nullptr, resampled, mprev, gt, L.f, l);
+                const float mt = tile_move<false>(A, t, kk, xin, xout, lw,
+                    idx, nullptr, resampled, mprev, gt, L.f, l);
                 double s, s2;
                 dd et;
                 tile_sums_l(A, kk, l, mt, gt, L.d, s, s2, et);

@@ -54,13 +54,10 @@ __device__ __forceinline__ float smc_log_f(float x)
     return __fmaf_rn(ef, 0x1.62e430p-1f, __fmaf_rn(ef, -0x1.05c610p-29f, r));
 }
 
-// exp(x); 0 below -87 (deterministic underflow), +inf above 88.
+// exp(x); 0 below -87 (deterministic underflow), +inf above 88.  Both
+// ends are selects over the evaluated polynomial (no divergent branch).
 __device__ __forceinline__ float smc_exp_f(float x)
 {
-    if (x < -87.0f)
-        return 0.0f;
-    if (x > 88.0f)
-        return __builtin_huge_valf();
     const float k = rintf(__fmul_rn(x, 0x1.715476p+0f));
     float r = __fmaf_rn(-k, 0x1.62e430p-1f, x);
     r = __fmaf_rn(-k, -0x1.05c610p-29f, r);
@@ -71,7 +68,8 @@ __device__ __forceinline__ float smc_exp_f(float x)
     q = __fmaf_rn(q, r, 0x1.555556p-3f);
     q = __fmaf_rn(q, r, 0x1.000000p-1f);
     const float p = __fadd_rn(1.0f, __fmaf_rn(__fmul_rn(r, r), q, r));
-    return __fmul_rn(p, f_as((uint32_t)((int)k + 127) << 23));
+    const float v = __fmul_rn(p, f_as((uint32_t)((int)k + 127) << 23));
+    return x < -87.0f ? 0.0f : (x > 88.0f ? __builtin_huge_valf() : v);
 }
 
 // sin(2 pi u), cos(2 pi u), u in [0, 1]
@@ -92,24 +90,14 @@ __device__ __forceinline__ void smc_sincos2pi_f(float u, float &s, float &c)
     cp = __fmaf_rn(cp, p2, 0x1.555556p-5f);
     cp = __fmaf_rn(cp, p2, -0x1.000000p-1f);
     const float cs = __fmaf_rn(p2, cp, 1.0f);
-    switch ((int)q & 3) {
-        case 0:
-            s = sn;
-            c = cs;
-            break;
-        case 1:
-            s = cs;
-            c = -sn;
-            break;
-        case 2:
-            s = -sn;
-            c = -cs;
-            break;
-        default:
-            s = -cs;
-            c = sn;
-            break;
-    }
+    // quadrant q mod 4: (s, c) = (sn, cs), (cs, -sn), (-sn, -cs), (-cs, sn),
+    // by selects and sign flips
+    const uint32_t q3 = (uint32_t)(int)q & 3u;
+    const bool swap = (q3 & 1u) != 0;
+    const float bs = swap ? cs : sn;
+    const float bc = swap ? sn : cs;
+    s = f_as(u_as(bs) ^ ((q3 & 2u) << 30));
+    c = f_as(u_as(bc) ^ (((q3 + 1u) & 2u) << 30));
 }
 
 // Normal pair of particles 2k, 2k+1 at step t: Philox4x32 block at counter

@@ -130,6 +130,28 @@ __device__ __forceinline__ void store8(uint32_t *p, uint64_t i0, uint64_t N,
     }
 }
 
+// Non-temporal form of the two 16-byte stores of a thread's 8 values, for
+// outputs the next launch reads: they bypass L2 residency, so the end of
+// the launch has fewer dirty lines to write back.
+template <typename T>
+__device__ __forceinline__ void store8_nt(T *p, uint64_t i0, uint64_t N,
+    const T (&v)[kItems])
+{
+    static_assert(sizeof(T) == 4, "32-bit elements");
+    typedef T t4 __attribute__((ext_vector_type(4)));
+    if (i0 + kItems <= N) {
+        const t4 a = {v[0], v[1], v[2], v[3]};
+        const t4 b = {v[4], v[5], v[6], v[7]};
+        __builtin_nontemporal_store(a, reinterpret_cast<t4 *>(p + i0));
+        __builtin_nontemporal_store(b, reinterpret_cast<t4 *>(p + i0 + 4));
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            if (i0 + k < N)
+                p[i0 + k] = v[k];
+    }
+}
+
 // group sums of two u32 per thread; results in every thread
 __device__ __forceinline__ void group_sum2_u(
     uint32_t &a, uint32_t &b, int gt, uint32_t *ldsu)
@@ -207,8 +229,9 @@ __device__ __forceinline__ uint32_t resolve_parent(
 
 // move + weight of one tile; returns the tile max of lw (all threads) and
 // the thread's new log weights in lout (0 past the end).  After a
-// resampling step idx holds every slot's parent, or its code when src is
-// given (resolve_parent).
+// resampling step idx holds every slot's parent (Codes false), or its code
+// resolved through src (Codes true, resolve_parent).
+template <bool Codes>
 __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
     uint32_t tile, const float *xin, float *xout, float *lw,
     const uint32_t *idx, const uint32_t *src, int resampled, float mprev,
@@ -221,15 +244,43 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
     for (int k = 0; k < kItems; ++k)
         lout[k] = 0.0f;
     if (i0 + kItems <= A.N) {
-        float xp[kItems], lo[kItems];
+        float xp[kItems], lo[kItems], e[kItems];
         if (resampled) {
             const uint4 a0 = *reinterpret_cast<const uint4 *>(idx + i0);
             const uint4 a1 = *reinterpret_cast<const uint4 *>(idx + i0 + 4);
-            const uint32_t a[kItems] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y,
+            // the normals do not depend on the parents: half are generated
+            // while the codes load, half while the sources load
+#pragma unroll
+            for (int k = 0; k < kItems / 2; k += 2)
+                smc_normal_pair(A.k0, A.k1, (i0 + k) >> 1, t, e[k], e[k + 1]);
+            uint32_t a[kItems] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y,
                 a1.z, a1.w};
+            // branch-free resolution: every slot reads src (a non-zero code
+            // is the slot's own index, also below N), so the eight source
+            // loads, then the eight state loads, are each in flight at once
+            if constexpr (Codes) {
+                uint32_t sv[kItems];
+#pragma unroll
+                for (int k = 0; k < kItems; ++k) {
+                    const uint32_t r = a[k] & ~kZeroFlag;
+                    sv[k] = src[r < A.N ? r : A.N - 1];
+                }
+#pragma unroll
+                for (int k = kItems / 2; k < kItems; k += 2)
+                    smc_normal_pair(
+                        A.k0, A.k1, (i0 + k) >> 1, t, e[k], e[k + 1]);
+#pragma unroll
+                for (int k = 0; k < kItems; ++k)
+                    a[k] = (a[k] & kZeroFlag) ? sv[k] : a[k];
+            } else {
+#pragma unroll
+                for (int k = kItems / 2; k < kItems; k += 2)
+                    smc_normal_pair(
+                        A.k0, A.k1, (i0 + k) >> 1, t, e[k], e[k + 1]);
+            }
 #pragma unroll
             for (int k = 0; k < kItems; ++k) {
-                xp[k] = xin[resolve_parent(a[k], src, A.N)];
+                xp[k] = xin[a[k] < A.N ? a[k] : A.N - 1];
                 lo[k] = 0.0f;
             }
         } else {
@@ -242,6 +293,9 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
             const float ws[kItems] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y,
                 w1.z, w1.w};
 #pragma unroll
+            for (int k = 0; k < kItems; k += 2)
+                smc_normal_pair(A.k0, A.k1, (i0 + k) >> 1, t, e[k], e[k + 1]);
+#pragma unroll
             for (int k = 0; k < kItems; ++k) {
                 xp[k] = xs[k];
                 lo[k] = __fsub_rn(ws[k], mprev);
@@ -250,10 +304,8 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
         float xn[kItems], ln[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k += 2) {
-            float e0, e1;
-            smc_normal_pair(A.k0, A.k1, (i0 + k) >> 1, t, e0, e1);
-            xn[k] = smc_move(A.mc, xp[k], e0);
-            xn[k + 1] = smc_move(A.mc, xp[k + 1], e1);
+            xn[k] = smc_move(A.mc, xp[k], e[k]);
+            xn[k + 1] = smc_move(A.mc, xp[k + 1], e[k + 1]);
             ln[k] = __fadd_rn(lo[k], smc_logg(A.mc, obs, xn[k]));
             ln[k + 1] = __fadd_rn(lo[k + 1], smc_logg(A.mc, obs, xn[k + 1]));
             lmax = fmaxf(lmax, fmaxf(ln[k], ln[k + 1]));
@@ -261,14 +313,19 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
 #pragma unroll
         for (int k = 0; k < kItems; ++k)
             lout[k] = ln[k];
-        *reinterpret_cast<float4 *>(xout + i0) =
-            make_float4(xn[0], xn[1], xn[2], xn[3]);
-        *reinterpret_cast<float4 *>(xout + i0 + 4) =
-            make_float4(xn[4], xn[5], xn[6], xn[7]);
-        *reinterpret_cast<float4 *>(lw + i0) =
-            make_float4(ln[0], ln[1], ln[2], ln[3]);
-        *reinterpret_cast<float4 *>(lw + i0 + 4) =
-            make_float4(ln[4], ln[5], ln[6], ln[7]);
+        if constexpr (Codes) { // the plain sampler: read by the next launch
+            store8_nt(xout, i0, A.N, xn);
+            store8_nt(lw, i0, A.N, ln);
+        } else {
+            *reinterpret_cast<float4 *>(xout + i0) =
+                make_float4(xn[0], xn[1], xn[2], xn[3]);
+            *reinterpret_cast<float4 *>(xout + i0 + 4) =
+                make_float4(xn[4], xn[5], xn[6], xn[7]);
+            *reinterpret_cast<float4 *>(lw + i0) =
+                make_float4(ln[0], ln[1], ln[2], ln[3]);
+            *reinterpret_cast<float4 *>(lw + i0 + 4) =
+                make_float4(ln[4], ln[5], ln[6], ln[7]);
+        }
     } else {
         for (int k = 0; k < kItems; k += 2) {
             const uint64_t i = i0 + k;
@@ -282,7 +339,8 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
                     break;
                 float xv, carry;
                 if (resampled) {
-                    xv = xin[resolve_parent(idx[p], src, A.N)];
+                    xv = xin[resolve_parent(idx[p], Codes ? src : nullptr,
+                        A.N)];
                     carry = 0.0f;
                 } else {
                     xv = xin[p];
@@ -451,6 +509,26 @@ struct UniformSeq {
 // about 17)
 constexpr double kCountMargin = 64.0 * kUnitRoundoff;
 
+// The tile-local part of tile_counts_c, which needs no value from other
+// tiles: a_i = exp_f(lw_i - m_k) of the thread's particles and the thread's
+// exclusive double-double prefix of a within the tile (toff).  A sampler
+// can run it while the cross-tile values load.
+__device__ __forceinline__ void tile_counts_local(const SmcArgs &A,
+    uint32_t tile, const float (&l)[kItems], float mk, int gt, double *ldsd,
+    float (&a)[kItems], dd &toff)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    dd loc{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t i = i0 + k;
+        a[k] = i < A.N ? smc_exp_f(__fsub_rn(l[k], mk)) : 0.0f;
+        loc = dd_add(loc, dd{(double)a[k], 0.0});
+    }
+    dd total;
+    toff = group_excl_scan_dd(loc, gt, ldsd, total);
+}
+
 // Cumulative normalised weights S_j = round(prefix_j(e)) * (1/S), prefix in
 // double-double: the tile's offset (tile_off, from the exclusive scan of the
 // scaled tile totals) plus r_k times the in-tile prefix of a_i; the last
@@ -463,10 +541,10 @@ constexpr double kCountMargin = 64.0 * kUnitRoundoff;
 // of the prefix sums (the clamp changes nothing when those are monotone).
 // Returns the thread's counts in c, the count before its first particle in
 // prev, and the tile sums of the extra-copy and zero-slot counts through
-// esum/zsum.  (l: the thread's 8 log weights, loaded by the caller)
+// esum/zsum.  (a, toff: from tile_counts_local)
 __device__ __forceinline__ void tile_counts_c(const SmcArgs &A, uint32_t tile,
-    const float (&l)[kItems], float mk, double rk, double invS, dd tile_off,
-    double next_hi, const UniformSeq &U, int gt, double *ldsd,
+    const float (&a)[kItems], const dd toff, double rk, double invS,
+    dd tile_off, double next_hi, const UniformSeq &U, int gt,
     uint32_t *ldsu, uint32_t (&c)[kItems], uint32_t &prev_out,
     uint32_t &esum, uint32_t &zsum)
 {
@@ -479,16 +557,6 @@ __device__ __forceinline__ void tile_counts_c(const SmcArgs &A, uint32_t tile,
         tile == 0 ? 0u : U.count_fast(__dmul_rn(tile_off.hi, invS));
     const uint32_t chi =
         tend == A.N ? U.M : U.count_fast(__dmul_rn(next_hi, invS));
-    float a[kItems];
-    dd loc{0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = i0 + k;
-        a[k] = i < A.N ? smc_exp_f(__fsub_rn(l[k], mk)) : 0.0f;
-        loc = dd_add(loc, dd{(double)a[k], 0.0});
-    }
-    dd total;
-    const dd toff = group_excl_scan_dd(loc, gt, ldsd, total);
     // P_i = tile_off + r_k (toff + sum_{j <= i} a_j): in double first
     // (relative error below 17 unit roundoffs, all terms non-negative); the
     // double-double value rounded once only where a uniform may lie within
@@ -553,7 +621,10 @@ __device__ __forceinline__ void tile_counts_l(const SmcArgs &A, uint32_t tile,
     uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
 {
     uint32_t c[kItems], prev;
-    tile_counts_c(A, tile, l, mk, rk, invS, tile_off, next_hi, U, gt, ldsd,
+    float a[kItems];
+    dd toff;
+    tile_counts_local(A, tile, l, mk, gt, ldsd, a, toff);
+    tile_counts_c(A, tile, a, toff, rk, invS, tile_off, next_hi, U, gt,
         ldsu, c, prev, esum, zsum);
     store8(cnt, tile_first(tile, gt), A.N, c);
 }
@@ -895,13 +966,14 @@ __device__ __forceinline__ void tile_index_store(const SmcArgs &A,
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
         code[k] = (ti.code[k] & kZeroFlag) ? ti.code[k] + zoff : ti.code[k];
-    store8(idx, i0, A.N, code);
+    store8_nt(idx, i0, A.N, code);
     // ranks with a slot in src: below N - eoff (the loop stays bounded even
     // for counts that are not a replication vector)
     const uint32_t end = rank_end(ti.end, eoff < A.N ? A.N - eoff : 0u);
     if (ti.end <= (uint32_t)kTile) { // sources from tile_index_local
         for (uint32_t r = gt; r < end; r += kGroup)
-            src[eoff + r] = (uint32_t)(t0 + srcl[r]);
+            __builtin_nontemporal_store(
+                (uint32_t)(t0 + srcl[r]), src + eoff + r);
     } else {
         for (uint32_t r0 = gt; r0 < end; r0 += 4 * kGroup) {
             uint32_t j[4];
