@@ -378,6 +378,52 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
                     const uint64_t i0 = tile_first(kk, gt);
                     double csum = 0.0, dummy = 0.0;
                     float ep[2] = {0.0f, 0.0f}; // the normal pair of i, i^1
+                    if (i0 + kItems <= A.N) {
+                        // a whole row: loads issued together, then the
+                        // element loop's arithmetic in the same order
+                        double lwt[kItems], lwp[kItems], lbv[kItems];
+                        double xc[kItems], xbv[kItems], xpv[kItems];
+                        float e[kItems];
+                        uint32_t par[kItems];
+                        load8d(cur.lw, i0, A.N, lwt, 0.0);
+                        load8d(lwb, i0, A.N, lbv, 0.0);
+                        load8d(cur.x, i0, A.N, xc, 0.0);
+                        load8d(xb, i0, A.N, xbv, 0.0);
+                        if (Rprev) {
+                            load8(prev.par, i0, A.N, par, 0u);
+#pragma unroll
+                            for (int q = 0; q < kItems; ++q)
+                                xpv[q] = prev.x[par[q]];
+                        } else {
+                            load8d(prev.lw, i0, A.N, lwp, 0.0);
+                            load8d(prev.x, i0, A.N, xpv, 0.0);
+                        }
+#pragma unroll
+                        for (int q = 0; q < kItems; q += 2)
+                            smc_normal_pair(A.k0, A.k1, (i0 + q) >> 1, t,
+                                e[q], e[q + 1]);
+                        double o1[kItems], o2[kItems];
+#pragma unroll
+                        for (int q = 0; q < kItems; ++q) {
+                            const double Wt = exp(lwt[q] - lse_t);
+                            const double Wc =
+                                Rprev ? invN : exp(lwp[q] - lse_p);
+                            const double lwbar = lbv[q] + Wt;
+                            const double cbar = lwbar - Wc;
+                            double g, gx, gc;
+                            gmodel(A, y, xc[q], g, gx, gc);
+                            const double xbar = xbv[q] + lwbar * gx;
+                            tc += lwbar * gc;
+                            ta += xbar * xpv[q];
+                            tsg += xbar * (double)e[q];
+                            o1[q] = A.a * xbar;
+                            o2[q] = cbar;
+                            if (Rprev)
+                                csum += cbar;
+                        }
+                        store8d(Rprev ? xpb : xb, i0, A.N, o1);
+                        store8d(Rprev ? cb : lwb, i0, A.N, o2);
+                    } else
                     for (int q = 0; q < kItems; ++q) {
                         const uint64_t i = i0 + q;
                         if (i >= A.N)
@@ -426,16 +472,28 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
                     if (k >= nt)
                         continue;
                     const uint64_t i0 = tile_first((uint32_t)k, gt);
+                    if (i0 >= A.N)
+                        continue;
+                    // the row's contiguous values loaded together
+                    uint32_t parv[kItems], Ev[kItems];
+                    double xpv[kItems], cbv[kItems], lpv[kItems];
+                    load8(prev.par, i0, A.N, parv, 0u);
+                    load8(prev.E, i0, A.N, Ev, 0u);
+                    load8d(xpb, i0, A.N, xpv, 0.0);
+                    load8d(cb, i0, A.N, cbv, 0.0);
+                    load8d(prev.lw, i0, A.N, lpv, 0.0);
+                    uint32_t Eprev = i0 == 0 ? 0u : prev.E[i0 - 1];
+                    double xo[kItems], lo[kItems];
+#pragma unroll
                     for (int q = 0; q < kItems; ++q) {
                         const uint64_t j = i0 + q;
-                        if (j >= A.N)
-                            break;
-                        const bool own = prev.par[j] == (uint32_t)j;
-                        double xs = own ? xpb[j] : 0.0;
-                        double cs = own ? cb[j] : 0.0;
+                        const bool own = parv[q] == (uint32_t)j;
+                        double xs = own ? xpv[q] : 0.0;
+                        double cs = own ? cbv[q] : 0.0;
                         // ranks of j's extra copies, kept inside zpos
-                        const uint32_t Ej = rank_end(prev.E[j], A.N);
-                        const uint32_t Ejm = j == 0 ? 0u : prev.E[j - 1];
+                        const uint32_t Ej = rank_end(Ev[q], A.N);
+                        const uint32_t Ejm = Eprev;
+                        Eprev = Ev[q];
                         for (uint32_t rk = Ejm; rk < Ej; ++rk) {
                             const uint32_t ch = prev.zpos[rk];
                             if (ch < A.N) {
@@ -443,10 +501,12 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
                                 cs += cb[ch];
                             }
                         }
-                        const double Wp = exp(prev.lw[j] - lse_p);
-                        xb[j] = xs;
-                        lwb[j] = cs - Wp * total;
+                        const double Wp = exp(lpv[q] - lse_p);
+                        xo[q] = xs;
+                        lo[q] = cs - Wp * total;
                     }
+                    store8d(xb, i0, A.N, xo);
+                    store8d(lwb, i0, A.N, lo);
                 }
                 __syncthreads();
             }

@@ -95,6 +95,44 @@ __device__ __forceinline__ double eps_at(const GradArgs &G, uint64_t i,
     return (double)((i & 1) ? e1 : e0);
 }
 
+// The 8 doubles of a thread's particles: four 16-byte loads when all are
+// below N (slot arrays are whole tiles, 16-byte aligned), else element loads
+// with `fill` past N.  Issuing a thread's loads together (then computing)
+// keeps them in flight at once instead of one round trip per particle.
+__device__ __forceinline__ void load8d(const double *p, uint64_t i0,
+    uint64_t N, double (&v)[kItems], double fill)
+{
+    if (i0 + kItems <= N) {
+        const double2 *q = reinterpret_cast<const double2 *>(p + i0);
+#pragma unroll
+        for (int j = 0; j < kItems / 2; ++j) {
+            const double2 a = q[j];
+            v[2 * j] = a.x;
+            v[2 * j + 1] = a.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            v[k] = i0 + k < N ? p[i0 + k] : fill;
+    }
+}
+
+__device__ __forceinline__ void store8d(double *p, uint64_t i0, uint64_t N,
+    const double (&v)[kItems])
+{
+    if (i0 + kItems <= N) {
+        double2 *q = reinterpret_cast<double2 *>(p + i0);
+#pragma unroll
+        for (int j = 0; j < kItems / 2; ++j)
+            q[j] = make_double2(v[2 * j], v[2 * j + 1]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            if (i0 + k < N)
+                p[i0 + k] = v[k];
+    }
+}
+
 // ---- forward pieces (FP64 counterparts of smc_tiles.hpp) ----
 __device__ __forceinline__ double gtile_move(const GradArgs &G, uint32_t t,
     uint32_t tile, const SlotPtr &prev, const SlotPtr &cur, int Rprev,
@@ -104,6 +142,62 @@ __device__ __forceinline__ double gtile_move(const GradArgs &G, uint32_t t,
     const uint64_t i0 = tile_first(tile, gt);
     const double y = G.y[t - 1];
     double lmax = -__builtin_huge_val();
+    if (i0 + kItems <= G.N) {
+        // a whole row of 8: the loads (and the parents' gathers) issued
+        // together, the normals drawn while they are in flight; the same
+        // arithmetic as the element loop below
+        double xp[kItems], carry[kItems];
+        float e[kItems];
+        if (Rprev) {
+            uint32_t p[kItems];
+            load8(tpar ? tpar : prev.par, i0, G.N, p, 0u);
+#pragma unroll
+            for (int k = 0; k < kItems; k += 2)
+                smc_normal_pair(G.k0, G.k1, (i0 + k) >> 1, t, e[k], e[k + 1]);
+#pragma unroll
+            for (int k = 0; k < kItems; ++k)
+                xp[k] = prev.x[p[k]];
+            if (lwn_ref) {
+                const double l2 = prev.scal[2];
+                double a[kItems], b[kItems];
+#pragma unroll
+                for (int k = 0; k < kItems; ++k) {
+                    a[k] = prev.lw[p[k]];
+                    b[k] = lwn_ref[p[k]];
+                }
+#pragma unroll
+                for (int k = 0; k < kItems; ++k)
+                    carry[k] = (a[k] - l2) - b[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < kItems; ++k)
+                    carry[k] = 0.0;
+            }
+        } else {
+            double lv[kItems];
+            load8d(prev.x, i0, G.N, xp, 0.0);
+            load8d(prev.lw, i0, G.N, lv, 0.0);
+#pragma unroll
+            for (int k = 0; k < kItems; k += 2)
+                smc_normal_pair(G.k0, G.k1, (i0 + k) >> 1, t, e[k], e[k + 1]);
+#pragma unroll
+            for (int k = 0; k < kItems; ++k)
+                carry[k] = lv[k] - mprev;
+        }
+        double xo[kItems], lo[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const double x = G.a * xp[k] + G.s * (double)e[k];
+            double g, gx, gc;
+            gmodel(G, y, x, g, gx, gc);
+            const double l = carry[k] + g;
+            xo[k] = x;
+            lo[k] = l;
+            lmax = lmax < l ? l : lmax;
+        }
+        store8d(cur.x, i0, G.N, xo);
+        store8d(cur.lw, i0, G.N, lo);
+    } else
     // particles i0.. are consecutive from an even index: one normal pair per
     // two particles (eps_at's values, each pair drawn once)
     for (int k = 0; k < kItems; k += 2) {
@@ -154,10 +248,13 @@ __device__ __forceinline__ void gtile_sums(const GradArgs &G, uint32_t tile,
     const uint64_t i0 = tile_first(tile, gt);
     double s = 0.0, s2 = 0.0;
     dd loc{0.0, 0.0};
+    double lv[kItems];
+    load8d(lw, i0, G.N, lv, 0.0);
+#pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         if (i < G.N) {
-            const double e = exp(lw[i] - m);
+            const double e = exp(lv[k] - m);
             s += e;
             s2 += e * e;
             loc = dd_add(loc, dd{e, 0.0});
@@ -180,9 +277,11 @@ __device__ __forceinline__ void gtile_counts(const GradArgs &G, uint32_t tile,
         : G.N;
     double W[kItems];
     dd loc{0.0, 0.0};
+    load8d(lw, i0, G.N, W, 0.0);
+#pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
-        W[k] = i < G.N ? exp(lw[i] - m) : 0.0;
+        W[k] = i < G.N ? exp(W[k] - m) : 0.0;
         loc = dd_add(loc, dd{W[k], 0.0});
     }
     dd total;
