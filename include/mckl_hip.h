@@ -397,8 +397,8 @@ int mckl_hip_smc_batch_destroy(mckl_hip_smc_batch *b);
  * and stay fixed in the reverse sweep; forward states are checkpointed every
  * K steps and recomputed segment by segment.  y holds T observations shared
  * by all points (y_per_point = 0) or T per point.  theta, grad: 3 per point
- * (LG: a, sx, sy; SV: phi, sigma, beta).  Schemes: multinomial, stratified,
- * systematic.
+ * (LG: a, sx, sy; SV: phi, sigma, beta).  Schemes: all six (the three base
+ * schemes and their residual forms, resample.hpp:581-593).
  * With tape = 1 run() also records the ancestors, normalised log-weights and
  * decisions (nbatch * (T+1) * N * 12 bytes), and replay() evaluates the
  * differentiated function at another theta with them held fixed: the

@@ -13,6 +13,8 @@ struct GLds {
     double tmh[kMaxTilesG], tml[kMaxTilesG]; // multinomial: log spacings
     uint32_t te[kMaxTilesG], tz[kMaxTilesG];
     uint32_t teo[kMaxTilesG + 1], tzo[kMaxTilesG + 1];
+    uint32_t ti[kMaxTilesG];               // residual: integral parts
+    double trh[kMaxTilesG], trl[kMaxTilesG]; // residual: residual totals
     double red[kMaxTilesG];
     double csum[kMaxTilesG];
     double acc[3 * 4];
@@ -24,6 +26,34 @@ struct GradSlotBufs {
     uint32_t *cnt;
     double *u; // multinomial: the step's sorted uniforms (else null)
 };
+
+__host__ __device__ constexpr bool is_residual_g(int scheme)
+{
+    return scheme == MCKL_HIP_RESIDUAL ||
+        scheme == MCKL_HIP_RESIDUAL_STRATIFIED ||
+        scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC;
+}
+
+// the schemes that draw sorted (multinomial) uniforms
+__host__ __device__ constexpr bool needs_sorted(int scheme)
+{
+    return scheme == MCKL_HIP_MULTINOMIAL || scheme == MCKL_HIP_RESIDUAL;
+}
+
+// kernel instantiations by the resampling code they contain (each only its
+// own, which keeps the others' register use down)
+constexpr int kGradBase = 0;      // stratified, systematic
+constexpr int kGradMult = 1;      // multinomial
+constexpr int kGradResid = 2;     // residual stratified, residual systematic
+constexpr int kGradResidMult = 3; // residual multinomial
+
+__host__ __device__ constexpr int grad_kind(int scheme)
+{
+    return scheme == MCKL_HIP_MULTINOMIAL ? kGradMult
+        : scheme == MCKL_HIP_RESIDUAL     ? kGradResidMult
+        : is_residual_g(scheme)           ? kGradResid
+                                          : kGradBase;
+}
 
 __global__ void k_grad_reset(uint32_t *queue)
 {
@@ -60,7 +90,8 @@ __device__ void copy_slot(const SlotPtr &d, const SlotPtr &s, uint32_t Np)
 // of the workgroup calls it.
 template <int G>
 __device__ __forceinline__ void grad_muniforms(const GradArgs &A,
-    uint32_t t, double *u, GLds &S, const GroupLds &L, int grp, int gt)
+    uint32_t t, double *u, GLds &S, const GroupLds &L, int grp, int gt,
+    uint32_t M)
 {
     const int nt = (int)A.ntiles;
     const int rounds = (nt + G - 1) / G;
@@ -71,7 +102,7 @@ __device__ __forceinline__ void grad_muniforms(const GradArgs &A,
     for (int r = 0; r < rounds; ++r) {
         const int k = r * G + grp;
         dd mt;
-        tile_mterms(ma, (uint32_t)(k < nt ? k : nt), t, gt, L.d, mt);
+        tile_mterms(ma, (uint32_t)(k < nt ? k : nt), t, gt, L.d, mt, M);
         if (gt == 0 && k < nt) {
             S.tmh[k] = mt.hi;
             S.tml[k] = mt.lo;
@@ -83,9 +114,91 @@ __device__ __forceinline__ void grad_muniforms(const GradArgs &A,
         const int k = r * G + grp;
         const int kk = k < nt ? k : nt;
         tile_muniforms(
-            ma, (uint32_t)kk, t, dd{S.toh[kk], S.tol[kk]}, u, gt, L.d);
+            ma, (uint32_t)kk, t, dd{S.toh[kk], S.tol[kk]}, u, gt, L.d, M);
     }
     __syncthreads();
+}
+
+// Residual resampling (ResampleAlgorithm<U01Seq, true>, resample.hpp:
+// 333-364, 515-593) on the FP64 weights W_i = exp(lw_i - m): the counts of
+// the FP32 samplers' residual step (tile_residual_sums_e, the R uniforms of
+// the base scheme, tile_rcounts_e), leaving the tiles' extra-copy and
+// zero-slot totals in S.te / S.tz.  The sorted uniforms of residual
+// multinomial need the kMult instantiation.  Every thread of the workgroup
+// calls it.
+template <int G, bool kMult>
+__device__ __forceinline__ void grad_rcounts(const GradArgs &A, uint32_t t,
+    const double *lw, double m, double invS, double *u, uint32_t *cnt,
+    GLds &S, const GroupLds &L, int grp, int gt)
+{
+    const int nt = (int)A.ntiles;
+    const int rounds = (nt + G - 1) / G;
+    SmcArgs ra{};
+    ra.N = A.N;
+    ra.k0 = A.k0;
+    ra.k1 = A.k1;
+    for (int r = 0; r < rounds; ++r) {
+        const int k = r * G + grp;
+        const uint32_t kk = (uint32_t)(k < nt ? k : nt);
+        double e[kItems];
+        load8d(lw, tile_first(kk, gt), A.N, e, 0.0);
+#pragma unroll
+        for (int q = 0; q < kItems; ++q)
+            e[q] = tile_first(kk, gt) + q < A.N ? exp(e[q] - m) : 0.0;
+        uint32_t isum;
+        dd rt;
+        tile_residual_sums_e(ra, kk, e, invS, gt, L.d, L.u, isum, rt);
+        if (gt == 0 && k < nt) {
+            S.ti[k] = isum;
+            S.trh[k] = rt.hi;
+            S.trl[k] = rt.lo;
+        }
+        __syncthreads();
+    }
+    uint32_t isum = 0;
+    for (int i = 0; i < nt; ++i)
+        isum += S.ti[i];
+    const uint32_t R = isum <= A.N ? A.N - isum : 0u;
+    if constexpr (kMult) {
+        if (A.scheme == MCKL_HIP_RESIDUAL)
+            grad_muniforms<G>(A, t, u, S, L, grp, gt, R);
+    }
+    tiles_scan_dd(grp == 0, S.trh, S.trl, nt, S.toh, S.tol, gt, L.d);
+    UniformSeq U;
+    U.u = A.scheme == MCKL_HIP_RESIDUAL ? u : nullptr;
+    U.scheme = A.scheme == MCKL_HIP_RESIDUAL_SYSTEMATIC ? MCKL_HIP_SYSTEMATIC
+        : A.scheme == MCKL_HIP_RESIDUAL_STRATIFIED      ? MCKL_HIP_STRATIFIED
+                                                        : MCKL_HIP_MULTINOMIAL;
+    U.k0 = A.k0;
+    U.k1 = A.k1;
+    U.t = t;
+    U.M = R;
+    U.delta = R ? __ddiv_rn(1.0, (double)R) : 0.0;
+    U.u0d = U.scheme == MCKL_HIP_SYSTEMATIC
+        ? __dmul_rn(smc_uniform(A.k0, A.k1, t, 0), U.delta)
+        : 0.0;
+    const double invR = R ? __ddiv_rn(1.0, S.toh[nt]) : 0.0;
+    for (int r = 0; r < rounds; ++r) {
+        const int k = r * G + grp;
+        const uint32_t kk = (uint32_t)(k < nt ? k : nt);
+        uint32_t ioff = 0;
+        for (uint32_t i = 0; i < kk; ++i)
+            ioff += S.ti[i];
+        double e[kItems];
+        load8d(lw, tile_first(kk, gt), A.N, e, 0.0);
+#pragma unroll
+        for (int q = 0; q < kItems; ++q)
+            e[q] = tile_first(kk, gt) + q < A.N ? exp(e[q] - m) : 0.0;
+        uint32_t es, zs;
+        tile_rcounts_e(ra, kk, e, invS, ioff, dd{S.toh[kk], S.tol[kk]},
+            (int)kk < nt ? S.toh[kk + 1] : 0.0, invR, U, cnt, gt, L.d, L.u,
+            es, zs);
+        if (gt == 0 && k < nt) {
+            S.te[k] = es;
+            S.tz[k] = zs;
+        }
+        __syncthreads();
+    }
 }
 
 // Multinomial: the step's sorted uniforms, the weights' tile offsets, then
@@ -100,7 +213,7 @@ __device__ __forceinline__ void grad_mcounts(const GradArgs &A,
 {
     const int nt = (int)A.ntiles;
     const int rounds = (nt + G - 1) / G;
-    grad_muniforms<G>(A, t, u, S, L, grp, gt); // uses S.toh/S.tol
+    grad_muniforms<G>(A, t, u, S, L, grp, gt, A.N); // uses S.toh/S.tol
     tiles_scan_dd(grp == 0, S.tdh, S.tdl, nt, S.toh, S.tol, gt, L.d);
     UniformSeq U;
     U.u = u;
@@ -126,8 +239,9 @@ __device__ __forceinline__ void grad_mcounts(const GradArgs &A,
 
 // One forward step t from slot `prev` to slot `cur`.  Returns the step's
 // contribution to F.  `rt` is the tape row pointer set for record/replay.
-// kMult: multinomial resampling (u: the step's sorted uniforms).
-template <int G, bool kMult>
+// Kind: the instantiation (grad_kind; u: the step's sorted uniforms for the
+// kinds that draw them).
+template <int G, int Kind>
 __device__ double fwd_step(const GradArgs &A, uint32_t t, const SlotPtr &prev,
     const SlotPtr &cur, uint32_t *cnt, double *u, int mode,
     const double *lwn_row,
@@ -213,7 +327,10 @@ __device__ double fwd_step(const GradArgs &A, uint32_t t, const SlotPtr &prev,
     __syncthreads();
     if (R && !replay) {
         const double invS = 1.0 / rs;
-        if constexpr (kMult) {
+        if constexpr (Kind == kGradResid || Kind == kGradResidMult) {
+            grad_rcounts<G, Kind == kGradResidMult>(A, t, cur.lw, m, invS, u,
+                cnt, S, L, grp, gt);
+        } else if constexpr (Kind == kGradMult) {
             grad_mcounts<G>(A, t, cur.lw, m, invS, u, cnt, S, L, grp, gt);
         } else {
             tiles_scan_dd(grp == 0, S.tdh, S.tdl, nt, S.toh, S.tol, gt, L.d);
@@ -266,7 +383,7 @@ __device__ double fwd_step(const GradArgs &A, uint32_t t, const SlotPtr &prev,
     return inc;
 }
 
-template <int G, bool kMult>
+template <int G, int Kind>
 __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
     uint32_t npoints, uint32_t *queue, GradSlotBufs B, size_t slot_bytes,
     size_t ck_stride, size_t seg_stride, size_t vec_stride, int mode,
@@ -329,7 +446,7 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
         for (uint32_t t = 1; t <= A.T; ++t) {
             const SlotPtr &prev = (t & 1) ? s0 : s1;
             const SlotPtr &cur = (t & 1) ? s1 : s0;
-            F += fwd_step<G, kMult>(A, t, prev, cur, cnt, ub, mode,
+            F += fwd_step<G, Kind>(A, t, prev, cur, cnt, ub, mode,
                 tl ? tl + (size_t)(t - 1) * rowN : nullptr,
                 tp ? tp + (size_t)(t - 1) * rowN : nullptr, tr,
                 tl ? tl + (size_t)t * rowN : nullptr,
@@ -358,7 +475,7 @@ __global__ __launch_bounds__(G *kGroup) void k_grad(const GradArgs *args,
             copy_slot<G>(slot_at(seg, slot_bytes, 0, Np),
                 slot_at(ck, slot_bytes, (uint32_t)c, Np), Np);
             for (uint32_t t = t0 + 1; t <= t1; ++t)
-                (void)fwd_step<G, kMult>(A, t,
+                (void)fwd_step<G, Kind>(A, t,
                     slot_at(seg, slot_bytes, t - t0 - 1, Np),
                     slot_at(seg, slot_bytes, t - t0, Np), cnt, ub, kGradRun,
                     nullptr,
@@ -634,8 +751,11 @@ static int grad_launch(mckl_hip_smc_grad *g, const double *h_theta, int mode,
         args.size() * sizeof(GradArgs), hipMemcpyHostToDevice, st));
     k_grad_reset<<<1, 64, 0, st>>>(g->queue);
     const bool tape = mode != kGradRun;
-    auto kern = g->scheme == MCKL_HIP_MULTINOMIAL ? k_grad<kGradGroups, true>
-                                                   : k_grad<kGradGroups, false>;
+    const int kind = grad_kind(g->scheme);
+    auto kern = kind == kGradMult ? k_grad<kGradGroups, kGradMult>
+        : kind == kGradResid      ? k_grad<kGradGroups, kGradResid>
+        : kind == kGradResidMult  ? k_grad<kGradGroups, kGradResidMult>
+                                  : k_grad<kGradGroups, kGradBase>;
     kern<<<g->nslots, kGradGroups * kGroup, 0, st>>>(g->args,
         (uint32_t)g->nbatch, g->queue, g->B, g->slot_bytes, g->ck_stride,
         g->seg_stride, g->vec_stride, mode, tape ? g->tape_lwn : nullptr,
@@ -664,10 +784,9 @@ int mckl_hip_smc_grad_create(int model, size_t nbatch, size_t N, size_t T,
         return set_error(MCKL_HIP_EINVAL,
             "the gradient sampler holds at most %d particles per point",
             kMaxTilesG * kTile);
-    if (scheme != MCKL_HIP_STRATIFIED && scheme != MCKL_HIP_SYSTEMATIC &&
-        scheme != MCKL_HIP_MULTINOMIAL)
-        return set_error(MCKL_HIP_EINVAL,
-            "multinomial, stratified or systematic only");
+    if (scheme < MCKL_HIP_MULTINOMIAL || scheme > MCKL_HIP_RESIDUAL_SYSTEMATIC)
+        return set_error(MCKL_HIP_EINVAL, "unknown resampling scheme %d",
+            scheme);
     if (std::isnan(threshold))
         return set_error(MCKL_HIP_EINVAL, "threshold is NaN");
     mckl_hip_smc_grad *g = new mckl_hip_smc_grad();
@@ -693,9 +812,10 @@ int mckl_hip_smc_grad_create(int model, size_t nbatch, size_t N, size_t T,
     size_t freeb = 0, totalb = 0;
     MCKL_HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     // per resident slot: checkpoints, segment, four adjoint vectors, the
-    // counts and, for multinomial, the sorted uniforms
+    // counts and, for multinomial and residual multinomial, the sorted
+    // uniforms
     const size_t per_slot = (nck + K + 1) * g->slot_bytes + Np * 8 * 4 +
-        Np * 4 + (scheme == MCKL_HIP_MULTINOMIAL ? Np * 8 : 0);
+        Np * 4 + (needs_sorted(scheme) ? Np * 8 : 0);
     size_t nslots = std::min<size_t>(nbatch, (size_t)ncu);
     while (nslots > 1 && nslots * per_slot > freeb / 2)
         nslots /= 2;
@@ -737,7 +857,7 @@ int mckl_hip_smc_grad_create(int model, size_t nbatch, size_t N, size_t T,
     MCKL_ALLOC(g->B.xpb, nslots * Np);
     MCKL_ALLOC(g->B.cb, nslots * Np);
     MCKL_ALLOC(g->B.cnt, nslots * Np);
-    if (scheme == MCKL_HIP_MULTINOMIAL)
+    if (needs_sorted(scheme))
         MCKL_ALLOC(g->B.u, nslots * Np);
     MCKL_ALLOC(g->logz, nbatch);
     MCKL_ALLOC(g->grad, 3 * nbatch);

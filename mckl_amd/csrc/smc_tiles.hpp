@@ -656,10 +656,24 @@ __device__ __forceinline__ void resid_part(
     r = __dsub_rn(w, fw);
 }
 
-// the tile's sum of I_i and double-double total of r_i
-__device__ __forceinline__ void tile_residual_sums(const SmcArgs &A,
-    uint32_t tile, const float (&l)[kItems], float mk, double rk, double invS,
-    int gt, double *ldsd, uint32_t *ldsu, uint32_t &isum, dd &rtot)
+// the samplers' weights e_i = a_i r_k of a thread's particles (0 past N)
+__device__ __forceinline__ void tile_weights(const SmcArgs &A, uint32_t tile,
+    const float (&l)[kItems], float mk, double rk, int gt,
+    double (&e)[kItems])
+{
+    const uint64_t i0 = tile_first(tile, gt);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        e[k] = i0 + k < A.N
+            ? __dmul_rn((double)smc_exp_f(__fsub_rn(l[k], mk)), rk)
+            : 0.0;
+}
+
+// the tile's sum of I_i and double-double total of r_i, from the thread's
+// weights e (the FP32 samplers' tile_weights, the gradient's FP64 weights)
+__device__ __forceinline__ void tile_residual_sums_e(const SmcArgs &A,
+    uint32_t tile, const double (&ew)[kItems], double invS, int gt,
+    double *ldsd, uint32_t *ldsu, uint32_t &isum, dd &rtot)
 {
     const uint64_t i0 = tile_first(tile, gt);
     const double M = (double)A.N;
@@ -668,8 +682,7 @@ __device__ __forceinline__ void tile_residual_sums(const SmcArgs &A,
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         if (i0 + k < A.N) {
-            const double e =
-                __dmul_rn((double)smc_exp_f(__fsub_rn(l[k], mk)), rk);
+            const double e = ew[k];
             uint32_t I;
             double r;
             resid_part(e, invS, M, I, r);
@@ -683,14 +696,23 @@ __device__ __forceinline__ void tile_residual_sums(const SmcArgs &A,
     (void)group_excl_scan_dd(loc, gt, ldsd, rtot);
 }
 
+__device__ __forceinline__ void tile_residual_sums(const SmcArgs &A,
+    uint32_t tile, const float (&l)[kItems], float mk, double rk, double invS,
+    int gt, double *ldsd, uint32_t *ldsu, uint32_t &isum, dd &rtot)
+{
+    double e[kItems];
+    tile_weights(A, tile, l, mk, rk, gt, e);
+    tile_residual_sums_e(A, tile, e, invS, gt, ldsd, ldsu, isum, rtot);
+}
+
 // Cumulative counts of a residual step: cnt_i = (sum_{j <= i} I_j) +
 // (residual count of i), the residual counts #{u_j < P^r_i (1/Rtot)} over
 // the R uniforms of U (U.M = R), with P^r the double-double prefix of the
 // residuals (roff: the tile's offset, rnext_hi: the next tile's) and the
 // same boundary clamp as tile_counts_l.  Ioff: sum of I over the tiles
 // before this one.  Returns the tile's extra-copy and zero-slot counts.
-__device__ __forceinline__ void tile_rcounts(const SmcArgs &A, uint32_t tile,
-    const float (&l)[kItems], float mk, double rk, double invS, uint32_t Ioff,
+__device__ __forceinline__ void tile_rcounts_e(const SmcArgs &A,
+    uint32_t tile, const double (&ew)[kItems], double invS, uint32_t Ioff,
     dd roff, double rnext_hi, double invR, const UniformSeq &U, uint32_t *cnt,
     int gt, double *ldsd, uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
 {
@@ -714,11 +736,8 @@ __device__ __forceinline__ void tile_rcounts(const SmcArgs &A, uint32_t tile,
     for (int k = 0; k < kItems; ++k) {
         I[k] = 0;
         r[k] = 0.0;
-        if (i0 + k < A.N) {
-            const double e =
-                __dmul_rn((double)smc_exp_f(__fsub_rn(l[k], mk)), rk);
-            resid_part(e, invS, M, I[k], r[k]);
-        }
+        if (i0 + k < A.N)
+            resid_part(ew[k], invS, M, I[k], r[k]);
         is += I[k];
         loc = dd_add(loc, dd{r[k], 0.0});
     }
@@ -771,6 +790,17 @@ __device__ __forceinline__ void tile_rcounts(const SmcArgs &A, uint32_t tile,
     group_sum2_u(es, zs, gt, ldsu);
     esum = es;
     zsum = zs;
+}
+
+__device__ __forceinline__ void tile_rcounts(const SmcArgs &A, uint32_t tile,
+    const float (&l)[kItems], float mk, double rk, double invS, uint32_t Ioff,
+    dd roff, double rnext_hi, double invR, const UniformSeq &U, uint32_t *cnt,
+    int gt, double *ldsd, uint32_t *ldsu, uint32_t &esum, uint32_t &zsum)
+{
+    double e[kItems];
+    tile_weights(A, tile, l, mk, rk, gt, e);
+    tile_rcounts_e(A, tile, e, invS, Ioff, roff, rnext_hi, invR, U, cnt, gt,
+        ldsd, ldsu, esum, zsum);
 }
 
 // Stable rep->index (resample.hpp:416-466, M = N): idx_i = i when rep_i > 0;

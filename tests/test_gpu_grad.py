@@ -22,6 +22,7 @@ import kalman
 pytestmark = pytest.mark.gpu
 
 MULT, STRAT, SYS = 0, 1, 2  # ResampleScheme numbering
+RES, RES_STRAT, RES_SYS = 3, 4, 5  # their residual forms
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 LG = (0.9, 1.0, 0.5)
@@ -52,7 +53,9 @@ def _fd(g, th, h=1e-4):
 
 @pytest.mark.parametrize("model,N,K,scheme", [
     (1, 3000, 7, SYS), (2, 4096, 10, STRAT), (2, 2049, 60, SYS),
-    (1, 5, 3, SYS), (1, 3000, 9, MULT), (2, 4100, 13, MULT)])
+    (1, 5, 3, SYS), (1, 3000, 9, MULT), (2, 4100, 13, MULT),
+    (1, 3000, 7, RES_SYS), (2, 4100, 11, RES_STRAT), (2, 2049, 9, RES),
+    (1, 7, 3, RES)])
 def test_replay_and_fd(dev, model, N, K, scheme):
     m = _m()
     y = _obs()["lg_y"][:60] if model == 1 else _obs()["sv_y"][:60]
@@ -81,7 +84,7 @@ def test_replay_and_fd(dev, model, N, K, scheme):
 def test_checkpoint_interval_invariance(dev):
     m = _m()
     y = _obs()["sv_y"][:40]
-    for scheme in (SYS, MULT):
+    for scheme in (SYS, MULT, RES_SYS, RES):
         res = []
         for K in (1, 3, 13, 40, 100):
             g = m.SMCGradient(2, 2, 5000, y, [8, 9], 0.5, scheme, K=K)
@@ -107,7 +110,8 @@ def test_per_point_observations_and_one_shot(dev):
         assert llp[0] == ll[p] and np.array_equal(grp[0], gr[p])
 
 
-def test_forward_close_to_plain_sampler(dev):
+@pytest.mark.parametrize("scheme", [SYS, RES_SYS])
+def test_forward_close_to_plain_sampler(dev, scheme):
     """FP64 forward of the gradient sampler vs the FP32 plain sampler on the
     same noise: without resampling (threshold 0) the two log-likelihoods
     agree up to the FP32 rounding of the plain states over 15 steps.  With
@@ -125,21 +129,24 @@ def test_forward_close_to_plain_sampler(dev):
         print("seed %d no resampling: fp64 %.9f fp32 %.9f" % (seed, ll[0], lp))
         assert abs(ll[0] - lp) < 1e-4
     seeds = np.arange(1, 17)
-    g = m.SMCGradient(1, len(seeds), N, y, seeds, 0.5, SYS, K=5)
+    g = m.SMCGradient(1, len(seeds), N, y, seeds, 0.5, scheme, K=5)
     ll, _ = g.run(np.tile(LG, (len(seeds), 1)))
-    lp = np.array([m.smc_loglik(1, LG, y, N, int(s), 0.5, SYS) for s in seeds])
+    lp = np.array([m.smc_loglik(1, LG, y, N, int(s), 0.5, scheme)
+                   for s in seeds])
     d = ll - lp
     print("paired differences with resampling:", np.round(d, 5))
     assert abs(d.mean()) < 4 * max(d.std(ddof=1), 1e-6) / np.sqrt(len(d))
 
 
-def test_lg_mean_gradient_vs_kalman(dev):
+@pytest.mark.parametrize("scheme", [SYS, RES_STRAT])
+def test_lg_mean_gradient_vs_kalman(dev, scheme):
     """Linear-Gaussian (0.9, 1, 0.5), T = 100 observations from seed 7,
     N = 2^16, 64 seeds: |mean - Kalman| within 4 standard errors."""
     m = _m()
     y = _obs()["lg_y"]
     nb = 64
-    g = m.SMCGradient(1, nb, 1 << 16, y, np.arange(1, nb + 1), 0.5, SYS, K=50)
+    g = m.SMCGradient(1, nb, 1 << 16, y, np.arange(1, nb + 1), 0.5, scheme,
+                      K=50)
     ll, gr = g.run(LG)
     ex = kalman.grad(y, *LG)
     mean = gr.mean(0)
@@ -180,7 +187,7 @@ def test_grad_errors(dev):
     with pytest.raises(m.MCKLError):
         m.SMCGradient(1, 1, 1 << 18, np.zeros(3), [1])
     with pytest.raises(m.MCKLError):
-        m.SMCGradient(1, 1, 100, np.zeros(3), [1], scheme=m.RESIDUAL)
+        m.SMCGradient(1, 1, 100, np.zeros(3), [1], scheme=6)
     g = m.SMCGradient(2, 1, 100, np.zeros(3), [1])
     with pytest.raises(m.MCKLError):
         g.replay(SV)
