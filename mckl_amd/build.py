@@ -99,6 +99,24 @@ def build_pf_program():
     return out
 
 
+def build_fused_test():
+    """Caller-written models on the fused samplers through
+    mckl_amd/include/mckl_hip_fused.hpp (tests/cpp/fused_model_test.hip), a
+    hipcc shared library linked to libmckl_hip.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "fused_model_test.hip")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "libfused_model_test.so")
+    _run([hipcc()] + HIPCC_FLAGS + [
+        "-shared",
+        "-I", os.path.join(ROOT, "include"),
+        "-I", os.path.join(HERE, "include"),
+        src, "-o", out,
+        "-L", HERE, "-lmckl_hip", "-Wl,-rpath,$ORIGIN/../../mckl_amd",
+    ])
+    return out
+
+
 def build_crmath_host():
     """Host builds of device headers for the CPU tests (tests/cpp):
     csrc/crmath.hpp (correctly rounded log/sin/cos) and csrc/smc_index.hpp
@@ -119,6 +137,7 @@ def main():
     build_lib()
     build_host_test()
     build_pf_program()
+    build_fused_test()
     build_crmath_host()
 
 

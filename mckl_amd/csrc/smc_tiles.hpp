@@ -168,9 +168,57 @@ __device__ __forceinline__ void group_sum2_u(
     __syncthreads();
 }
 
-// t = 0: x_i = sd0 eps(i, 0), lw_i = 0
-__device__ __forceinline__ void tile_init(
-    const SmcArgs &A, uint32_t tile, float *x, float *lw, int gt)
+// ---- models ----
+// A fused sampler takes its model as a value of a type M with
+//   __device__ float init(float e) const         x_0 from a standard normal e
+//   __device__ Step at(uint32_t t) const         the model at step t = 1..T
+// where Step (any type) has
+//   __device__ float move(float x, float e) const   x_t from x_{t-1} and a
+//                                                  standard normal e
+//   __device__ float log_weight(float x) const      log g(y_t | x_t)
+// The roles of the reference's evaluation object (smp/backend_base.hpp:
+// 113-183): at(t) is eval_first's per-step set-up, move/log_weight are
+// eval_each on one particle.  M is passed to the kernels by value, so it
+// holds plain values and device pointers.  The noise is the samplers'
+// counter-based stream (smc_normal_pair: particle i, step t), the states
+// FP32, the bootstrap weights lw_t = carry + log_weight(x_t).
+//
+// The built-in linear-Gaussian and stochastic-volatility models in that
+// form: ModelConst and the per-step observation constants (obs_const).
+struct BuiltinModel {
+    ModelConst mc;
+    const float *obs; // T observation constants
+    struct Step {
+        ModelConst mc;
+        float obs;
+        __device__ __forceinline__ float move(float x, float e) const
+        {
+            return smc_move(mc, x, e);
+        }
+        __device__ __forceinline__ float log_weight(float x) const
+        {
+            return smc_logg(mc, obs, x);
+        }
+    };
+    __device__ __forceinline__ float init(float e) const
+    {
+        return __fmul_rn(mc.sd0, e);
+    }
+    __device__ __forceinline__ Step at(uint32_t t) const
+    {
+        return Step{mc, obs[t - 1]};
+    }
+};
+
+__host__ __device__ inline BuiltinModel builtin_model(const SmcArgs &A)
+{
+    return BuiltinModel{A.mc, A.obs};
+}
+
+// t = 0: x_i = init(eps(i, 0)), lw_i = 0
+template <class M>
+__device__ __forceinline__ void tile_init(const SmcArgs &A, const M &model,
+    uint32_t tile, float *x, float *lw, int gt)
 {
     const uint64_t i0 = tile_first(tile, gt);
 #pragma unroll
@@ -180,10 +228,10 @@ __device__ __forceinline__ void tile_init(
             break;
         float e0, e1;
         smc_normal_pair(A.k0, A.k1, i >> 1, 0, e0, e1);
-        x[i] = __fmul_rn(A.mc.sd0, e0);
+        x[i] = model.init(e0);
         lw[i] = 0.0f;
         if (i + 1 < A.N) {
-            x[i + 1] = __fmul_rn(A.mc.sd0, e1);
+            x[i + 1] = model.init(e1);
             lw[i + 1] = 0.0f;
         }
     }
@@ -231,14 +279,14 @@ __device__ __forceinline__ uint32_t resolve_parent(
 // the thread's new log weights in lout (0 past the end).  After a
 // resampling step idx holds every slot's parent (Codes false), or its code
 // resolved through src (Codes true, resolve_parent).
-template <bool Codes>
-__device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
-    uint32_t tile, const float *xin, float *xout, float *lw,
+template <bool Codes, class M>
+__device__ __forceinline__ float tile_move(const SmcArgs &A, const M &model,
+    uint32_t t, uint32_t tile, const float *xin, float *xout, float *lw,
     const uint32_t *idx, const uint32_t *src, int resampled, float mprev,
     int gt, float *ldsf, float (&lout)[kItems])
 {
     const uint64_t i0 = tile_first(tile, gt);
-    const float obs = A.obs[t - 1];
+    const auto step = model.at(t);
     float lmax = -__builtin_huge_valf();
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
@@ -304,10 +352,10 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
         float xn[kItems], ln[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k += 2) {
-            xn[k] = smc_move(A.mc, xp[k], e[k]);
-            xn[k + 1] = smc_move(A.mc, xp[k + 1], e[k + 1]);
-            ln[k] = __fadd_rn(lo[k], smc_logg(A.mc, obs, xn[k]));
-            ln[k + 1] = __fadd_rn(lo[k + 1], smc_logg(A.mc, obs, xn[k + 1]));
+            xn[k] = step.move(xp[k], e[k]);
+            xn[k + 1] = step.move(xp[k + 1], e[k + 1]);
+            ln[k] = __fadd_rn(lo[k], step.log_weight(xn[k]));
+            ln[k + 1] = __fadd_rn(lo[k + 1], step.log_weight(xn[k + 1]));
             lmax = fmaxf(lmax, fmaxf(ln[k], ln[k + 1]));
         }
 #pragma unroll
@@ -346,8 +394,8 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, uint32_t t,
                     xv = xin[p];
                     carry = __fsub_rn(lw[p], mprev);
                 }
-                const float xnew = smc_move(A.mc, xv, e[q]);
-                const float l = __fadd_rn(carry, smc_logg(A.mc, obs, xnew));
+                const float xnew = step.move(xv, e[q]);
+                const float l = __fadd_rn(carry, step.log_weight(xnew));
                 xout[p] = xnew;
                 lw[p] = l;
                 lout[k + q] = l;

@@ -1442,6 +1442,14 @@ typedef struct {
 static void mo_model_init(int model, const double *th, mo_model *m)
 {
     const double l2pi = log(2.0 * 3.141592653589793);
+    if (model == MO_MODEL_GROWTH) {
+        m->a = 0.0f;
+        m->s = (float)th[0];
+        m->inv_sy = (float)(1.0 / th[1]);
+        m->cst = (float)(-log(th[1]) - 0.5 * l2pi);
+        m->sd0 = (float)th[2];
+        return;
+    }
     m->a = (float)th[0];
     m->s = (float)th[1];
     m->cst = (float)(-log(th[2]) - 0.5 * l2pi);
@@ -1457,6 +1465,8 @@ static void mo_model_init(int model, const double *th, mo_model *m)
 /* per-step observation constant: LG (float)y, SV (float)(y^2 / beta^2) */
 float mo_smc_obs(int model, const double *th, double y)
 {
+    if (model == MO_MODEL_GROWTH)
+        return (float)y;
     return model == MO_MODEL_LG ? (float)y : (float)(y * y / (th[2] * th[2]));
 }
 
@@ -1464,6 +1474,12 @@ float mo_smc_logg(int model, const double *th, float obs, float x)
 {
     mo_model m;
     mo_model_init(model, th, &m);
+    if (model == MO_MODEL_GROWTH) {
+        const float h = (x * x) * 0.05f;
+        const float d = obs - h;
+        const float z = d * m.inv_sy;
+        return -0.5f * (z * z) + m.cst;
+    }
     if (model == MO_MODEL_LG) {
         const float d = obs - x;
         const float z = d * m.inv_sy;
@@ -1529,6 +1545,8 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
     double logz = 0;
     for (size_t t = 1; t <= T; ++t) {
         const float obs = mo_smc_obs(model, theta, y[t - 1]);
+        const float drift = model == MO_MODEL_GROWTH ? (float)y[T + t - 1]
+                                                     : 0.0f;
         const double C = resampled ? (double)N : Sprev;
 #pragma omp parallel for schedule(static)
         for (long long i = 0; i < Ns; ++i) {
@@ -1541,7 +1559,13 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
                 carry = lw[i] - mprev;
             }
             const float eps = mo_smc_normal(seed, (uint64_t)i, t);
-            const float xv = m.a * xp + m.s * eps;
+            float xv;
+            if (model == MO_MODEL_GROWTH) {
+                const float q = xp / (1.0f + xp * xp);
+                xv = ((0.5f * xp + 25.0f * q) + drift) + m.s * eps;
+            } else {
+                xv = m.a * xp + m.s * eps;
+            }
             xn[i] = xv;
             lw[i] = carry + mo_smc_logg(model, theta, obs, xv);
         }

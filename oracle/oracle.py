@@ -306,7 +306,8 @@ def smc_run(model, theta, y, N, seed, threshold=0.5, scheme=SYSTEMATIC,
     with moments=k > 0 also the (T, k) selection estimates of x^1..x^k)."""
     th = np.ascontiguousarray(theta, np.float64)
     y = np.ascontiguousarray(y, np.float64)
-    T = len(y)
+    # the growth model (3) takes the T observations then the T drifts
+    T = len(y) // 2 if model == 3 else len(y)
     ess = np.zeros(T, np.float64)
     res = np.zeros(T, np.int32)
     est = np.zeros((T, max(moments, 1)), np.float64)

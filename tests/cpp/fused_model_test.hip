@@ -1,0 +1,186 @@
+// Models written as a caller of mckl_amd/include/mckl_hip_fused.hpp would
+// write them, run through the fused plain and batched samplers, behind a C
+// interface for tests/test_gpu_fused.py:
+//  * UserLG: the linear-Gaussian model's arithmetic, written out here (it
+//    must give the built-in sampler's bits);
+//  * Growth: the nonstationary growth model, x_t = (0.5 x + 25 x / (1 +
+//    x^2)) + u_t + s e, y_t = x^2 / 20 + sy w (the oracle's model 3).
+// Test infrastructure only.
+#include "mckl_hip_fused.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct UserLG {
+    float a, s, sd0, inv_sy, cst;
+    const float *obs;
+    struct Step {
+        float a, s, inv_sy, cst, y;
+        __device__ float move(float x, float e) const
+        {
+            return __fadd_rn(__fmul_rn(a, x), __fmul_rn(s, e));
+        }
+        __device__ float log_weight(float x) const
+        {
+            const float z = __fmul_rn(__fsub_rn(y, x), inv_sy);
+            return __fadd_rn(__fmul_rn(-0.5f, __fmul_rn(z, z)), cst);
+        }
+    };
+    __device__ float init(float e) const { return __fmul_rn(sd0, e); }
+    __device__ Step at(uint32_t t) const
+    {
+        return Step{a, s, inv_sy, cst, obs[t - 1]};
+    }
+};
+
+struct Growth {
+    float s, sd0, inv_sy, cst;
+    const float *obs, *drift;
+    struct Step {
+        float s, inv_sy, cst, y, u;
+        __device__ float move(float x, float e) const
+        {
+            const float q = __fdiv_rn(x, __fadd_rn(1.0f, __fmul_rn(x, x)));
+            const float mean = __fadd_rn(
+                __fadd_rn(__fmul_rn(0.5f, x), __fmul_rn(25.0f, q)), u);
+            return __fadd_rn(mean, __fmul_rn(s, e));
+        }
+        __device__ float log_weight(float x) const
+        {
+            const float h = __fmul_rn(__fmul_rn(x, x), 0.05f);
+            const float z = __fmul_rn(__fsub_rn(y, h), inv_sy);
+            return __fadd_rn(__fmul_rn(-0.5f, __fmul_rn(z, z)), cst);
+        }
+    };
+    __device__ float init(float e) const { return __fmul_rn(sd0, e); }
+    __device__ Step at(uint32_t t) const
+    {
+        return Step{s, inv_sy, cst, obs[t - 1], drift[t - 1]};
+    }
+};
+
+const double kL2pi = std::log(2.0 * 3.141592653589793);
+
+UserLG make_lg(const double *th, const float *d_obs)
+{
+    return UserLG{(float)th[0], (float)th[1], 1.0f, (float)(1.0 / th[2]),
+        (float)(-std::log(th[2]) - 0.5 * kL2pi), d_obs};
+}
+
+// theta = (s, sy, sd0)
+Growth make_growth(const double *th, const float *d_obs, const float *d_u)
+{
+    return Growth{(float)th[0], (float)th[2], (float)(1.0 / th[1]),
+        (float)(-std::log(th[1]) - 0.5 * kL2pi), d_obs, d_u};
+}
+
+std::string g_msg;
+
+// per filter: T observations (model 1) or T observations and T drifts
+// (model 3) from y, as floats on the device
+struct Obs {
+    std::vector<float *> p;
+    ~Obs()
+    {
+        for (float *q : p)
+            (void)hipFree(q);
+    }
+    const float *put(const double *y, size_t n)
+    {
+        std::vector<float> h(n);
+        for (size_t i = 0; i < n; ++i)
+            h[i] = (float)y[i];
+        float *d = nullptr;
+        if (hipMalloc(&d, n * sizeof(float)) != hipSuccess)
+            throw mckl::RuntimeAssert("hipMalloc");
+        p.push_back(d);
+        if (hipMemcpy(d, h.data(), n * sizeof(float), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            throw mckl::RuntimeAssert("hipMemcpy");
+        return d;
+    }
+};
+
+template <typename M>
+void plain(const M &model, size_t T, size_t N, uint64_t seed,
+    double threshold, int scheme, double *ll, double *ess, int32_t *res)
+{
+    mckl::FusedSMCSampler<M> s(model, N, T, seed, threshold, scheme);
+    s.iterate();
+    s.iterate(); // a replay of the captured graph gives the same run
+    *ll = s.log_likelihood();
+    const std::vector<double> e = s.ess_history();
+    const std::vector<int32_t> r = s.resampled_history();
+    for (size_t t = 0; t < T; ++t) {
+        ess[t] = e[t];
+        res[t] = r[t];
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+const char *fused_last_error() { return g_msg.c_str(); }
+
+// model 1 (UserLG) or 3 (Growth); y: T (model 1) or 2 T (model 3) values
+int fused_plain_run(int model, const double *theta, const double *y,
+    size_t T, size_t N, uint64_t seed, double threshold, int scheme,
+    double *ll, double *ess, int32_t *res)
+{
+    try {
+        Obs o;
+        if (model == 1)
+            plain(make_lg(theta, o.put(y, T)), T, N, seed, threshold, scheme,
+                ll, ess, res);
+        else
+            plain(make_growth(theta, o.put(y, T), o.put(y + T, T)), T, N,
+                seed, threshold, scheme, ll, ess, res);
+        return 0;
+    } catch (const std::exception &e) {
+        g_msg = e.what();
+        return -1;
+    }
+}
+
+// nf filters with their own theta (3 each), y (T or 2 T each), N and seed
+int fused_batch_run(int model, size_t nf, const uint32_t *N,
+    const double *theta, const double *y, size_t T, const uint64_t *seeds,
+    double threshold, int scheme, double *ll)
+{
+    try {
+        Obs o;
+        const std::vector<uint32_t> n(N, N + nf);
+        const std::vector<uint64_t> sd(seeds, seeds + nf);
+        std::vector<double> out;
+        if (model == 1) {
+            std::vector<UserLG> m;
+            for (size_t f = 0; f < nf; ++f)
+                m.push_back(make_lg(theta + 3 * f, o.put(y + T * f, T)));
+            mckl::FusedSMCBatch<UserLG> b(m, n, T, sd, threshold, scheme);
+            b.iterate();
+            out = b.log_likelihood();
+        } else {
+            std::vector<Growth> m;
+            for (size_t f = 0; f < nf; ++f)
+                m.push_back(make_growth(theta + 3 * f,
+                    o.put(y + 2 * T * f, T), o.put(y + 2 * T * f + T, T)));
+            mckl::FusedSMCBatch<Growth> b(m, n, T, sd, threshold, scheme);
+            b.iterate();
+            b.iterate();
+            out = b.log_likelihood();
+        }
+        for (size_t f = 0; f < nf; ++f)
+            ll[f] = out[f];
+        return 0;
+    } catch (const std::exception &e) {
+        g_msg = e.what();
+        return -1;
+    }
+}
+
+} // extern "C"

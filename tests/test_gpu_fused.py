@@ -1,0 +1,146 @@
+"""The fused samplers on models written by the caller
+(mckl_amd/include/mckl_hip_fused.hpp; the models are in
+tests/cpp/fused_model_test.hip):
+
+* a caller-written linear-Gaussian model gives the built-in sampler's bits,
+  plain and batched, for every scheme: the model enters the kernels through
+  the model type alone;
+* the nonstationary growth model (a model the library does not have) on
+  the plain fused sampler agrees with the oracle's restatement (model 3):
+  the same resampling decisions, ESS to 1e-12 and log-likelihood to 1e-10,
+  as the built-in models' parity tests ask;
+* the batched form of the growth model gives each filter the plain run's
+  bits.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "tests", "cpp", "libfused_model_test.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+SCHEMES = [0, 1, 2, 3, 4, 5]
+LG = (0.9, 1.0, 0.5)
+GROWTH = (np.sqrt(10.0), 1.0, np.sqrt(5.0))  # (s, sy, sd0)
+
+
+def P(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@pytest.fixture(scope="module")
+def fl(dev):
+    import mckl_amd  # noqa: F401  (loads the library and the HIP runtime)
+    lib = ctypes.CDLL(LIB)
+    lib.fused_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+def growth_data(T, seed):
+    """Observations of the growth model and its drifts u_t = 8 cos(1.2 t),
+    concatenated (the oracle's model 3 layout)."""
+    rng = np.random.default_rng(seed)
+    s, sy, sd0 = GROWTH
+    x = sd0 * rng.standard_normal()
+    y = np.zeros(T)
+    u = 8.0 * np.cos(1.2 * np.arange(1, T + 1))
+    for t in range(T):
+        x = 0.5 * x + 25.0 * x / (1.0 + x * x) + u[t] + s * rng.standard_normal()
+        y[t] = x * x / 20.0 + sy * rng.standard_normal()
+    return np.concatenate([y, u])
+
+
+def plain(fl, model, th, y, T, N, seed, thr, scheme):
+    th = np.ascontiguousarray(th, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    ll = ctypes.c_double()
+    ess = np.zeros(T)
+    res = np.zeros(T, np.int32)
+    rc = fl.fused_plain_run(model, P(th), P(y), ctypes.c_size_t(T),
+                            ctypes.c_size_t(N), ctypes.c_uint64(seed),
+                            ctypes.c_double(thr), scheme, ctypes.byref(ll),
+                            P(ess), P(res))
+    assert rc == 0, fl.fused_last_error()
+    return ll.value, ess, res
+
+
+def batch(fl, model, th, y, T, N, seeds, thr, scheme):
+    nf = len(N)
+    th = np.ascontiguousarray(th, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    Nv = np.ascontiguousarray(N, np.uint32)
+    sv = np.ascontiguousarray(seeds, np.uint64)
+    ll = np.zeros(nf)
+    rc = fl.fused_batch_run(model, ctypes.c_size_t(nf), P(Nv), P(th), P(y),
+                            ctypes.c_size_t(T), P(sv), ctypes.c_double(thr),
+                            scheme, P(ll))
+    assert rc == 0, fl.fused_last_error()
+    return ll
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_user_lg_equals_builtin(fl, scheme):
+    import mckl_amd as m
+    y = np.load(os.path.join(GOLDEN, "observations.npz"))["lg_y"][:40]
+    for N, seed in ((4096, 3), (5000, 4), (1 << 16, 5)):
+        ll, ess, res = plain(fl, 1, LG, y, len(y), N, seed, 0.5, scheme)
+        s = m.SMCSampler(m.MODEL_LG, LG, y, N, seed, 0.5, scheme)
+        s.iterate()
+        bl, bess, bres = s.results()
+        print(scheme, N, ll, bl, int(res.sum()))
+        assert ll == bl
+        assert np.array_equal(ess, bess) and np.array_equal(res, bres)
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 0, 5])
+def test_user_lg_batch_equals_builtin(fl, scheme):
+    import mckl_amd as m
+    y = np.load(os.path.join(GOLDEN, "observations.npz"))["lg_y"][:30]
+    T = len(y)
+    N = [300, 2048, 2049, 5000, 17000]
+    seeds = [11, 12, 13, 14, 15]
+    th = np.array([LG, (0.8, 1.1, 0.6), LG, (0.95, 0.7, 0.4), LG])
+    ll = batch(fl, 1, th, np.tile(y, len(N)), T, N, seeds, 0.5, scheme)
+    b = m.SMCBatch(m.MODEL_LG, N, th, np.tile(y, (len(N), 1)), seeds, 0.5,
+                   scheme)
+    b.run()
+    want = b.log_likelihood()
+    print(ll, want)
+    assert np.array_equal(ll, want)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_growth_model_vs_oracle(fl, scheme):
+    T = 50
+    y = growth_data(T, 21)
+    for N, seed in ((4096, 7), (3000, 8)):
+        ll, ess, res = plain(fl, 3, GROWTH, y, T, N, seed, 0.5, scheme)
+        oll, oess, ores = orc.smc_run(3, GROWTH, y, N, seed, 0.5, scheme,
+                                      nthreads=8, history=True)
+        print(scheme, N, ll, oll, int(res.sum()))
+        assert 0 < res.sum() <= T
+        assert np.array_equal(res, ores)
+        assert np.allclose(ess, oess, rtol=1e-12, atol=0)
+        assert abs(ll - oll) < 1e-10 * max(1.0, abs(oll))
+
+
+def test_growth_batch_equals_plain(fl):
+    T = 40
+    N = [256, 2048, 4097, 9000, 640]
+    seeds = [31, 32, 33, 34, 35]
+    ys = [growth_data(T, 40 + f) for f in range(len(N))]
+    th = np.array([GROWTH] * len(N))
+    th[1, 0] = 2.5
+    th[3, 1] = 1.5
+    for scheme in (1, 2, 4):
+        ll = batch(fl, 3, th, np.concatenate(ys), T, N, seeds, 0.5, scheme)
+        for f in range(len(N)):
+            lp, _, _ = plain(fl, 3, th[f], ys[f], T, N[f], seeds[f], 0.5,
+                             scheme)
+            assert ll[f] == lp, (scheme, f, ll[f], lp)
