@@ -104,21 +104,29 @@ def test_plain_graph_replays(dev):
 
 def test_lg_config_vs_kalman(dev):
     """Config: N = 2^20 FP32 particles, T = 100 observations simulated from
-    seed 7, systematic resampling when ESS < N/2."""
+    seed 7, systematic resampling when ESS < N/2.  Over 64 seeds the mean
+    log Z-hat is within 4 standard errors of the Kalman FP64 value (the
+    standard error is about 1e-3), and every seed within 6 of the seeds'
+    standard deviations of it."""
     m = _m()
     y = _obs()["lg_y"]
     exact = kalman.loglik(y, 0.9, 1.0, 0.5)
     ests = []
-    for seed in range(1, 5):
+    for seed in range(1, 65):
         s = m.SMCSampler(1, (0.9, 1.0, 0.5), y, 1 << 20, seed, 0.5, SYS)
         s.iterate()
         ll, ess, res = s.results()
         ests.append(ll)
         assert 0 < res.sum() < 100
+        del s
     ests = np.array(ests)
-    # Monte Carlo sd of log Z-hat at N = 2^20 is ~0.01 for this model
-    print("kalman %.6f  smc %s" % (exact, ests))
-    assert np.all(np.abs(ests - exact) < 0.1)
+    sd = ests.std(ddof=1)
+    se = sd / np.sqrt(len(ests))
+    print("kalman %.6f  smc mean %.6f sd %.5f se %.5f z %.2f" % (
+        exact, ests.mean(), sd, se, (ests.mean() - exact) / se))
+    assert se < 3e-3
+    assert abs(ests.mean() - exact) < 4 * se
+    assert np.all(np.abs(ests - exact) < 6 * sd)
 
 
 def test_batched_equals_plain(dev):
