@@ -57,7 +57,23 @@ struct RsBuf {
     double *trh, *trl; // residual: tile dd totals of the residuals
     uint32_t *src;     // source of each extra-copy rank (tile_index_store)
     uint64_t *lbw;     // look-back words, 2 x ntiles (by step parity)
+    double *tpre;      // systematic/stratified: each thread's exclusive dd
+                       // prefix of a within its tile (hi, lo), from k_move
 };
+
+__device__ __forceinline__ void store_dd_nt(double *p, size_t i, dd v)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 w = {v.hi, v.lo};
+    __builtin_nontemporal_store(w, reinterpret_cast<d2 *>(p) + i);
+}
+
+__device__ __forceinline__ dd load_dd(const double *p, size_t i)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 w = reinterpret_cast<const d2 *>(p)[i];
+    return dd{w.x, w.y};
+}
 
 template <class M>
 __global__ __launch_bounds__(kGroup) void k_init(
@@ -93,7 +109,10 @@ __global__ __launch_bounds__(kGroup) void k_move(SmcArgs A, M model,
         idx, B.src, st->resampled[p], st->m[p], threadIdx.x, L.f, l);
     double S, S2;
     dd et;
-    tile_sums_l(A, blockIdx.x, l, mt, threadIdx.x, L.d, S, S2, et);
+    const dd lp =
+        tile_sums_l(A, blockIdx.x, l, mt, threadIdx.x, L.d, S, S2, et);
+    if (B.tpre)
+        store_dd_nt(B.tpre, (size_t)blockIdx.x * kGroup + threadIdx.x, lp);
     if (threadIdx.x == 0) {
         tmax[blockIdx.x] = mt;
         ts[blockIdx.x] = S;
@@ -251,8 +270,13 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     // previous step's launch, which has finished)
     if (gt == 0)
         lb_store(&B.lbw[(size_t)(c ^ 1) * nt + k], 0);
+    constexpr bool kSorted =
+        S == MCKL_HIP_SYSTEMATIC || S == MCKL_HIP_STRATIFIED;
     float l[kItems];
     load8(lw, tile_first(k, gt), A.N, l, 0.0f);
+    dd ttoff{0.0, 0.0}; // the thread's prefix of a within the tile
+    if constexpr (kSorted)
+        ttoff = load_dd(B.tpre, (size_t)k * kGroup + gt);
     const double Cprev = st->C[p];
     const float tk = tmax[k];
     const int chunk = (nt + kGroup - 1) / kGroup;
@@ -283,6 +307,10 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
         for (int i = gt; i < nt; i += kGroup)
             mx = fmaxf(mx, tmax[i]);
     }
+    // the step's uniform sequence (its key work overlaps the loads)
+    UniformSeq U{};
+    if constexpr (kSorted)
+        U = make_uniform_seq(A, t);
     const float m = group_max(mx, gt, L.f);
     // sums of the scaled tile sums: r_i S_i and r_i^2 S2_i (the decision,
     // log Z and the next normaliser), r_i T_i in double-double (the offsets
@@ -331,6 +359,9 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     if (!o.resample)
         return;
     const double rk = tile_scale(tk, m);
+    float ta[kItems]; // a_i = exp_f(lw_i - m_k)
+    if constexpr (kSorted)
+        tile_a(A, k, l, tk, gt, ta);
     if constexpr (S == MCKL_HIP_MULTINOMIAL) {
         // multinomial: tile totals of the log spacings; the uniforms and
         // the counts follow in k_rs_uniforms and k_rs_mcounts, which read
@@ -406,11 +437,7 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     // extra copies and zero slots by look-back, and the rep -> index
     // transform (parents as codes, resolved by the next move); the tile's
     // own part of the transform runs while its predecessors publish
-    const UniformSeq U = make_uniform_seq(A, t);
     uint32_t es, zs, cs[kItems], prev;
-    float ta[kItems];
-    dd ttoff;
-    tile_counts_local(A, k, l, tk, gt, L.d, ta, ttoff);
     tile_counts_c(A, k, ta, ttoff, rk, invS, toff, next_hi, U, gt, L.u, cs,
         prev, es, zs);
     uint64_t *w = B.lbw + (size_t)c * nt;

@@ -104,6 +104,7 @@ inline void plain_release(PlainSampler<M> *s)
     drop(s->B.trl);
     drop(s->B.src);
     drop(s->B.lbw);
+    drop(s->B.tpre);
     drop(s->st);
     drop(s->ess);
     drop(s->res);
@@ -155,6 +156,8 @@ inline int plain_init(PlainSampler<M> *s, const M &model, size_t N, size_t T,
     MCKL_ALLOC(s->B.cnt, Np);
     MCKL_ALLOC(s->B.src, Np);
     MCKL_ALLOC(s->B.lbw, 2 * nt);
+    if (scheme == MCKL_HIP_SYSTEMATIC || scheme == MCKL_HIP_STRATIFIED)
+        MCKL_ALLOC(s->B.tpre, 2 * nt * (size_t)kGroup);
     if (is_residual(scheme)) {
         MCKL_ALLOC(s->B.ti, nt);
         MCKL_ALLOC(s->B.trh, nt);

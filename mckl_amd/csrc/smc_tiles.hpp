@@ -417,8 +417,10 @@ __device__ __forceinline__ double tile_scale(float mk, float m)
     return (double)smc_exp_f(__fsub_rn(mk, m));
 }
 
-// the tile's sums of a_i = exp_f(lw_i - mk) (see above)
-__device__ __forceinline__ void tile_sums_l(const SmcArgs &A, uint32_t tile,
+// the tile's sums of a_i = exp_f(lw_i - mk) (see above); returns the
+// thread's exclusive double-double prefix of a within the tile, the toff of
+// tile_counts_local, bit for bit
+__device__ __forceinline__ dd tile_sums_l(const SmcArgs &A, uint32_t tile,
     const float (&l)[kItems], float m, int gt, double *ldsd, double &S,
     double &S2, dd &etot)
 {
@@ -438,7 +440,7 @@ __device__ __forceinline__ void tile_sums_l(const SmcArgs &A, uint32_t tile,
     group_sum2(s, s2, gt, ldsd);
     S = s;
     S2 = s2;
-    (void)group_excl_scan_dd(loc, gt, ldsd, etot);
+    return group_excl_scan_dd(loc, gt, ldsd, etot);
 }
 
 // Uniform sequences of step t (sorted, M = N):
@@ -558,21 +560,29 @@ struct UniformSeq {
 constexpr double kCountMargin = 64.0 * kUnitRoundoff;
 
 // The tile-local part of tile_counts_c, which needs no value from other
-// tiles: a_i = exp_f(lw_i - m_k) of the thread's particles and the thread's
-// exclusive double-double prefix of a within the tile (toff).  A sampler
-// can run it while the cross-tile values load.
+// tiles: a_i = exp_f(lw_i - m_k) of the thread's particles (tile_a; 0 past
+// N) and the thread's exclusive double-double prefix of a within the tile
+// (toff; tile_sums_l returns the same prefix).
+__device__ __forceinline__ void tile_a(const SmcArgs &A, uint32_t tile,
+    const float (&l)[kItems], float mk, int gt, float (&a)[kItems])
+{
+    const uint64_t i0 = tile_first(tile, gt);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const float e = smc_exp_f(__fsub_rn(l[k], mk));
+        a[k] = i0 + k < A.N ? e : 0.0f;
+    }
+}
+
 __device__ __forceinline__ void tile_counts_local(const SmcArgs &A,
     uint32_t tile, const float (&l)[kItems], float mk, int gt, double *ldsd,
     float (&a)[kItems], dd &toff)
 {
-    const uint64_t i0 = tile_first(tile, gt);
+    tile_a(A, tile, l, mk, gt, a);
     dd loc{0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t i = i0 + k;
-        a[k] = i < A.N ? smc_exp_f(__fsub_rn(l[k], mk)) : 0.0f;
+    for (int k = 0; k < kItems; ++k)
         loc = dd_add(loc, dd{(double)a[k], 0.0});
-    }
     dd total;
     toff = group_excl_scan_dd(loc, gt, ldsd, total);
 }
@@ -600,42 +610,65 @@ __device__ __forceinline__ void tile_counts_c(const SmcArgs &A, uint32_t tile,
     const uint64_t tend = (uint64_t)(tile + 1) * kTile < A.N
         ? (uint64_t)(tile + 1) * kTile
         : A.N;
-    // boundary counts shared with the neighbouring tiles
-    const uint32_t clo =
-        tile == 0 ? 0u : U.count_fast(__dmul_rn(tile_off.hi, invS));
-    const uint32_t chi =
-        tend == A.N ? U.M : U.count_fast(__dmul_rn(next_hi, invS));
+    // boundary counts shared with the neighbouring tiles (lane 0: the
+    // tile's first boundary, lane 1: its last; one copy of the count code)
+    const uint32_t cb = U.count_fast(
+        __dmul_rn((lane_id() & 1) ? next_hi : tile_off.hi, invS));
+    const uint32_t clo = tile == 0
+        ? 0u
+        : (uint32_t)__builtin_amdgcn_readlane((int)cb, 0);
+    const uint32_t chi = tend == A.N
+        ? U.M
+        : (uint32_t)__builtin_amdgcn_readlane((int)cb, 1);
     // P_i = tile_off + r_k (toff + sum_{j <= i} a_j): in double first
     // (relative error below 17 unit roundoffs, all terms non-negative); the
     // double-double value rounded once only where a uniform may lie within
     // the bound of the double one (count_guarded), so the count is that of
-    // the rounded double-double prefix in every case
+    // the rounded double-double prefix in every case.  Those (rare) items
+    // are marked in `redo` and counted afterwards in one rolled loop, so the
+    // exact path's code is not repeated per item.
     double rd = toff.hi;
-    uint32_t cmax = clo;
+    uint32_t redo = 0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint64_t i = i0 + k;
         rd = __dadd_rn(rd, (double)a[k]);
-        c[k] = 0;
-        if (i < A.N) {
-            uint32_t v;
-            if (i == tend - 1) {
-                v = chi;
-            } else {
-                const double Pf = __dadd_rn(__dmul_rn(rd, rk), tile_off.hi);
-                const double Sf = __dmul_rn(Pf, invS);
-                if (!U.count_guarded(Sf, __dmul_rn(Sf, kCountMargin), v)) {
-                    dd run = toff;
+        uint32_t v = chi;
+        if (i < tend - 1) {
+            const double Pf = __dadd_rn(__dmul_rn(rd, rk), tile_off.hi);
+            const double Sf = __dmul_rn(Pf, invS);
+            if (!U.count_guarded(Sf, __dmul_rn(Sf, kCountMargin), v))
+                redo |= 1u << k;
+        }
+        c[k] = v;
+    }
+    if (redo) {
+        dd run = toff;
+#pragma unroll 1
+        for (int k = 0; k < kItems; ++k) {
+            float ak = a[0];
 #pragma unroll
-                    for (int q = 0; q <= k; ++q)
-                        run = dd_add(run, dd{(double)a[q], 0.0});
-                    const double P = dd_add(tile_off, dd_mul_d(run, rk)).hi;
-                    v = U.count_fast(__dmul_rn(P, invS));
-                }
+            for (int q = 1; q < kItems; ++q)
+                ak = k == q ? a[q] : ak;
+            run = dd_add(run, dd{(double)ak, 0.0});
+            if ((redo >> k) & 1u) {
+                const double P = dd_add(tile_off, dd_mul_d(run, rk)).hi;
+                const uint32_t v = U.count_fast(__dmul_rn(P, invS));
+#pragma unroll
+                for (int q = 0; q < kItems; ++q)
+                    c[q] = k == q ? v : c[q];
             }
-            v = clamp_count(v, clo, chi);
+        }
+    }
+    uint32_t cmax = clo;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (i0 + k < A.N) {
+            const uint32_t v = clamp_count(c[k], clo, chi);
             cmax = v > cmax ? v : cmax;
             c[k] = cmax;
+        } else {
+            c[k] = 0;
         }
     }
     // counts of the threads before this one: make the sequence monotone
@@ -943,6 +976,7 @@ __device__ __forceinline__ void rank_sources4(const uint32_t *Es, uint32_t n,
         hi[q] = r0 + q * stride < end ? n - 1 : 0u;
     }
     // at most ceil(log2(kTile)) halvings
+#pragma unroll 1
     for (int it = 0; it < 12; ++it) {
         bool more = false;
 #pragma unroll
