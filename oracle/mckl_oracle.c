@@ -1510,9 +1510,21 @@ static void mo_dd_add_d(double *hi, double *lo, double d)
  * and weighting (SMCEstimator at step 0, algorithm/smc.hpp:89-118,
  * :389-394): est[(t-1) kmom + j-1] = sum_i W_i x_i^j, W_i = e_i * (1/S) as
  * the resampling weights below, summed in order (the reference's dgemv). */
-double mo_smc_run_est(int model, const double *theta, const double *y,
+/* a request to record the resampling inputs and outputs of one step (the
+ * first resampling step at or after t, non-residual schemes) and stop */
+typedef struct {
+    size_t t;      /* in: earliest step; out: the step recorded (0: none) */
+    double *u01;   /* the step's uniform draws (systematic: one) */
+    double *W;     /* normalised weights e_i (1/S) */
+    double *u;     /* the uniform sequence the counts use */
+    uint64_t *rep; /* replication numbers */
+    int done;
+} mo_smc_dump;
+
+static double smc_run_impl(int model, const double *theta, const double *y,
     size_t T, size_t N, uint64_t seed, double threshold, int scheme,
-    int nthreads, double *ess_hist, int *resampled_hist, int kmom, double *est)
+    int nthreads, double *ess_hist, int *resampled_hist, int kmom, double *est,
+    mo_smc_dump *dump)
 {
     mo_model m;
     mo_model_init(model, theta, &m);
@@ -1695,6 +1707,17 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
                     u[j] = 1.0 - exp(lm);
                 }
             }
+            if (dump && t >= dump->t) {
+                if (scheme == MO_SYSTEMATIC)
+                    dump->u01[0] = mo_smc_uniform(seed, t, 0);
+                else
+                    for (size_t j = 0; j < N; ++j)
+                        dump->u01[j] = mo_smc_uniform(seed, t, j);
+                for (size_t j = 0; j < N; ++j) {
+                    dump->W[j] = e[j] * inv;
+                    dump->u[j] = u[j];
+                }
+            }
             /* counts from the cumulative weights rounded once: the device
              * sums the weights in double-double (within tiles, then over
              * tile offsets), so its cumulative weight of particle i is the
@@ -1718,6 +1741,12 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
                     prevc = c;
                 }
             }
+            if (dump && t >= dump->t) {
+                memcpy(dump->rep, rep, N * sizeof(uint64_t));
+                dump->t = t;
+                dump->done = 1;
+                break; /* nothing after the recorded step is needed */
+            }
             mo_resample_trans_rep_index(N, N, rep, idx);
         }
     }
@@ -1731,6 +1760,27 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
     free(rep);
     free(idx);
     return logz;
+}
+
+double mo_smc_run_est(int model, const double *theta, const double *y,
+    size_t T, size_t N, uint64_t seed, double threshold, int scheme,
+    int nthreads, double *ess_hist, int *resampled_hist, int kmom, double *est)
+{
+    return smc_run_impl(model, theta, y, T, N, seed, threshold, scheme,
+        nthreads, ess_hist, resampled_hist, kmom, est, NULL);
+}
+
+size_t mo_smc_resample_dump(int model, const double *theta, const double *y,
+    size_t T, size_t N, uint64_t seed, double threshold, int scheme,
+    size_t t_first, double *u01, double *W, double *u, uint64_t *rep)
+{
+    if (scheme != MO_MULTINOMIAL && scheme != MO_STRATIFIED &&
+        scheme != MO_SYSTEMATIC)
+        return 0;
+    mo_smc_dump d = {t_first ? t_first : 1, u01, W, u, rep, 0};
+    (void)smc_run_impl(model, theta, y, T, N, seed, threshold, scheme, 1,
+        NULL, NULL, 0, NULL, &d);
+    return d.done ? d.t : 0;
 }
 
 double mo_smc_run(int model, const double *theta, const double *y, size_t T,

@@ -320,7 +320,55 @@ def smc_run(model, theta, y, N, seed, threshold=0.5, scheme=SYSTEMATIC,
     return ll
 
 
+O.mo_smc_resample_dump.restype = _sz
+O.mo_smc_resample_dump.argtypes = [ctypes.c_int, _vp, _vp, _sz, _sz,
+                                   ctypes.c_uint64, ctypes.c_double,
+                                   ctypes.c_int, _sz, _vp, _vp, _vp, _vp]
+
+
+def smc_resample_dump(model, theta, y, N, seed, threshold, scheme, t_first=1):
+    """The fused sampler's first resampling step at or after t_first
+    (multinomial, stratified or systematic): (t, u01, W, u, rep) with the
+    step's uniform draws u01 (systematic: one), the normalised weights W,
+    the uniform sequence u and the replication numbers; None when no step
+    from t_first on resamples."""
+    th = np.ascontiguousarray(theta, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    T = len(y) // 2 if model == 3 else len(y)
+    u01 = np.zeros(N, np.float64)
+    W = np.zeros(N, np.float64)
+    u = np.zeros(N, np.float64)
+    rep = np.zeros(N, np.uint64)
+    t = O.mo_smc_resample_dump(model, P(th), P(y), _sz(T), _sz(N), seed,
+                               threshold, scheme, _sz(t_first), P(u01), P(W),
+                               P(u), P(rep))
+    if t == 0:
+        return None
+    return t, (u01[:1] if scheme == SYSTEMATIC else u01), W, u, rep
+
+
 # ---- reference (compiled from the reference headers), when available ----
+def ref_u01_trans(scheme, u01, N):
+    """u01_trans_sorted / stratified / systematic of given uniforms (scheme
+    0, 1, 2; systematic reads u01[0])."""
+    u01 = np.ascontiguousarray(u01, np.float64)
+    r = np.zeros(max(N, 1), np.float64)
+    REF.ref_u01_trans(scheme, _sz(N), P(u01), P(r))
+    return r[:N]
+
+
+def ref_trans_u01_rep(w, u, M=None):
+    """resample_trans_u01_rep of normalised weights w and ordered uniforms u
+    (the reference's sequential FP64 cumulative sum)."""
+    w = np.ascontiguousarray(w, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    N = len(w)
+    M = len(u) if M is None else M
+    rep = np.zeros(max(N, 1), np.uint64)
+    REF.ref_trans_u01_rep(_sz(N), _sz(M), P(w), P(u), P(rep))
+    return rep[:N]
+
+
 def ref_engine_ops(kind, rbits, key, sizes, ctr0=None):
     n = sum(s if s > 0 else (1 if s == 0 else 0) for s in sizes)
     out = np.zeros(max(n, 1), np.uint32 if rbits == 32 else np.uint64)

@@ -310,6 +310,20 @@ int ref_u01_sequence(int scheme, int kind, const std::uint64_t *key,
     });
 }
 
+// the transforms of given standard uniforms into the schemes' sequences
+// (u01_trans_sorted/stratified/systematic, algorithm/resample.hpp:141-212);
+// scheme numbering as above, systematic reads u01[0] only
+int ref_u01_trans(int scheme, std::size_t n, const double *u01, double *r)
+{
+    if (scheme == 0)
+        mckl::u01_trans_sorted(n, u01, r);
+    else if (scheme == 1)
+        mckl::u01_trans_stratified(n, u01, r);
+    else
+        mckl::u01_trans_systematic(n, u01, r);
+    return 0;
+}
+
 int ref_resample(int scheme, std::size_t N, std::size_t M, int kind,
     const std::uint64_t *key, const double *w, std::uint64_t *rep,
     std::uint64_t *idx)
