@@ -1670,6 +1670,17 @@ static double smc_run_impl(int model, const double *theta, const double *y,
                 for (size_t j = 0; j < R; ++j)
                     u[j] = dR * (mo_smc_uniform(seed, t, j) + (double)j);
             }
+            if (dump && t >= dump->t) {
+                if (scheme == MO_RESIDUAL_SYSTEMATIC)
+                    dump->u01[0] = mo_smc_uniform(seed, t, 0);
+                else
+                    for (size_t j = 0; j < R; ++j)
+                        dump->u01[j] = mo_smc_uniform(seed, t, j);
+                for (size_t j = 0; j < N; ++j)
+                    dump->W[j] = e[j] * inv;
+                for (size_t j = 0; j < R; ++j)
+                    dump->u[j] = u[j];
+            }
             const double invR = R ? 1.0 / th : 0.0;
             double hi = 0.0, lo = 0.0;
             size_t j = 0;
@@ -1688,6 +1699,12 @@ static double smc_run_impl(int model, const double *theta, const double *y,
                 }
                 rep[i] += c - prevc;
                 prevc = c;
+            }
+            if (dump && t >= dump->t) {
+                memcpy(dump->rep, rep, N * sizeof(uint64_t));
+                dump->t = t;
+                dump->done = 1;
+                break; /* nothing after the recorded step is needed */
             }
             mo_resample_trans_rep_index(N, N, rep, idx);
         } else if (resampled) {
@@ -1774,8 +1791,7 @@ size_t mo_smc_resample_dump(int model, const double *theta, const double *y,
     size_t T, size_t N, uint64_t seed, double threshold, int scheme,
     size_t t_first, double *u01, double *W, double *u, uint64_t *rep)
 {
-    if (scheme != MO_MULTINOMIAL && scheme != MO_STRATIFIED &&
-        scheme != MO_SYSTEMATIC)
+    if (scheme < MO_MULTINOMIAL || scheme > MO_RESIDUAL_SYSTEMATIC)
         return 0;
     mo_smc_dump d = {t_first ? t_first : 1, u01, W, u, rep, 0};
     (void)smc_run_impl(model, theta, y, T, N, seed, threshold, scheme, 1,

@@ -175,10 +175,11 @@ double mo_smc_run_est(int model, const double *theta, const double *y,
     double *est);
 /* The fused sampler's resampling step, for checks against the reference's
  * own transforms: runs mo_smc_run's algorithm up to its first resampling
- * step at or after t_first (multinomial, stratified or systematic) and
- * writes that step's uniform draws u01 (systematic: u01[0] only), the
- * normalised weights W (N), the uniform sequence u (N) and rep (N).
- * Returns the step, or 0 when no step from t_first on resamples. */
+ * step at or after t_first (any of the six schemes) and writes that step's
+ * uniform draws u01 (systematic forms: u01[0] only), the normalised
+ * weights W (N), the uniform sequence u (N; residual schemes: the R
+ * uniforms of the remainder) and rep (N).  Returns the step, or 0 when no
+ * step from t_first on resamples. */
 size_t mo_smc_resample_dump(int model, const double *theta, const double *y,
     size_t T, size_t N, uint64_t seed, double threshold, int scheme,
     size_t t_first, double *u01, double *W, double *u, uint64_t *rep);

@@ -84,6 +84,7 @@ O.mo_u01_canonical_f64.restype = ctypes.c_double
 O.mo_u01_canonical_f32.restype = ctypes.c_float
 for _lib in (REF, REF_V4):
     if _lib is not None:
+        _lib.ref_trans_residual.restype = _sz
         _lib.ref_time_philox_u01.restype = ctypes.c_double
         _lib.ref_time_philox_u01_mt.restype = ctypes.c_double
         _lib.ref_time_normal_streams.restype = ctypes.c_double
@@ -327,11 +328,12 @@ O.mo_smc_resample_dump.argtypes = [ctypes.c_int, _vp, _vp, _sz, _sz,
 
 
 def smc_resample_dump(model, theta, y, N, seed, threshold, scheme, t_first=1):
-    """The fused sampler's first resampling step at or after t_first
-    (multinomial, stratified or systematic): (t, u01, W, u, rep) with the
-    step's uniform draws u01 (systematic: one), the normalised weights W,
-    the uniform sequence u and the replication numbers; None when no step
-    from t_first on resamples."""
+    """The fused sampler's first resampling step at or after t_first (any
+    scheme): (t, u01, W, u, rep) with the step's uniform draws u01
+    (systematic forms: one), the normalised weights W, the uniform sequence
+    u (residual schemes: the R uniforms of the remainder, R = N - sum of
+    floor(N W_i)) and the replication numbers; None when no step from
+    t_first on resamples."""
     th = np.ascontiguousarray(theta, np.float64)
     y = np.ascontiguousarray(y, np.float64)
     T = len(y) // 2 if model == 3 else len(y)
@@ -344,7 +346,12 @@ def smc_resample_dump(model, theta, y, N, seed, threshold, scheme, t_first=1):
                                P(u), P(rep))
     if t == 0:
         return None
-    return t, (u01[:1] if scheme == SYSTEMATIC else u01), W, u, rep
+    n = N
+    if scheme >= RESIDUAL:
+        n = N - int(np.floor(float(N) * W).sum())
+    if scheme in (SYSTEMATIC, RESIDUAL_SYSTEMATIC):
+        return t, u01[:1], W, u[:n], rep
+    return t, u01[:n], W, u[:n], rep
 
 
 # ---- reference (compiled from the reference headers), when available ----
@@ -355,6 +362,17 @@ def ref_u01_trans(scheme, u01, N):
     r = np.zeros(max(N, 1), np.float64)
     REF.ref_u01_trans(scheme, _sz(N), P(u01), P(r))
     return r[:N]
+
+
+def ref_trans_residual(w, M=None):
+    """resample_trans_residual: (normalised residuals, integral parts, R)."""
+    w = np.ascontiguousarray(w, np.float64)
+    N = len(w)
+    M = N if M is None else M
+    resid = np.zeros(max(N, 1), np.float64)
+    integ = np.zeros(max(N, 1), np.uint64)
+    R = REF.ref_trans_residual(_sz(N), _sz(M), P(w), P(resid), P(integ))
+    return resid[:N], integ[:N], int(R)
 
 
 def ref_trans_u01_rep(w, u, M=None):

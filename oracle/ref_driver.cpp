@@ -324,6 +324,19 @@ int ref_u01_trans(int scheme, std::size_t n, const double *u01, double *r)
     return 0;
 }
 
+// resample_trans_residual (algorithm/resample.hpp:333-364): normalised
+// residuals and integral parts of M W_i; returns R = M - sum of the parts
+std::size_t ref_trans_residual(std::size_t N, std::size_t M, const double *w,
+    double *resid, std::uint64_t *integ)
+{
+    std::vector<std::size_t> in(N);
+    const std::size_t R =
+        mckl::resample_trans_residual(N, M, w, resid, in.data());
+    for (std::size_t i = 0; i != N; ++i)
+        integ[i] = in[i];
+    return R;
+}
+
 int ref_resample(int scheme, std::size_t N, std::size_t M, int kind,
     const std::uint64_t *key, const double *w, std::uint64_t *rep,
     std::uint64_t *idx)

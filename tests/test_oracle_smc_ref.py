@@ -15,7 +15,8 @@ compiled from its headers (oracle/_ref):
 
 Models: linear-Gaussian and stochastic volatility (the built-in ones) and
 the growth model (a caller's own in the fused-sampler tests), at several
-steps and sizes, all three base schemes.
+steps and sizes, all six schemes (the residual ones through
+resample_trans_residual, the integral parts exact).
 """
 import math
 import os
@@ -94,6 +95,36 @@ def test_resampling_step_vs_reference(model, scheme):
         steps += 1
     print("model %d scheme %d: %d steps, rep differences %d (within 1e-15: "
           "%d)" % (model, scheme, steps, total, within))
+
+
+@pytest.mark.parametrize("model", [1, 2, 3])
+@pytest.mark.parametrize("scheme", [orc.RESIDUAL, orc.RESIDUAL_STRATIFIED,
+                                    orc.RESIDUAL_SYSTEMATIC])
+def test_residual_step_vs_reference(model, scheme):
+    """Residual schemes (algorithm/resample.hpp:333-364, 557-574): the
+    integral parts of N W_i equal resample_trans_residual's, the R uniforms
+    equal the base scheme's transform of the same draws, and the
+    replication numbers equal resample_trans_u01_rep on the reference's
+    normalised residuals plus the integral parts, except at rounding-edge
+    uniforms of the residuals' cumulative sums."""
+    th, y = model_data(model)
+    base = scheme - orc.RESIDUAL
+    total = within = 0
+    for N, seed, t_first in ((5000, 6, 1), (70001, 7, 4), (2049, 8, 9)):
+        d = orc.smc_resample_dump(model, th, y, N, seed, 0.5, scheme, t_first)
+        assert d is not None
+        t, u01, W, u, rep = d
+        resid, integ, R = orc.ref_trans_residual(W)
+        assert R == len(u) and R > 0
+        ru = orc.ref_u01_trans(base, u01, R)
+        assert np.array_equal(ru, u), (N, t)
+        rr = orc.ref_trans_u01_rep(resid, ru) + integ
+        assert int(rep.sum()) == N and int(rr.sum()) == N
+        nd, n15 = unexplained(resid, u, rr, rep)
+        total += nd
+        within += n15
+    print("model %d scheme %d: rep differences %d (within 1e-15: %d)"
+          % (model, scheme, total, within))
 
 
 def test_resampling_step_large_vs_reference():
