@@ -121,6 +121,19 @@ void plain(const M &model, size_t T, size_t N, uint64_t seed,
     }
 }
 
+template <typename M>
+void plain_moments(const M &model, size_t T, size_t N, uint64_t seed,
+    double threshold, int scheme, int k, double *ll, double *est)
+{
+    mckl::FusedSMCSampler<M> s(model, N, T, seed, threshold, scheme);
+    s.estimate_moments(k);
+    s.iterate();
+    *ll = s.log_likelihood();
+    const std::vector<double> e = s.estimates();
+    for (size_t i = 0; i < e.size(); ++i)
+        est[i] = e[i];
+}
+
 } // namespace
 
 extern "C" {
@@ -140,6 +153,26 @@ int fused_plain_run(int model, const double *theta, const double *y,
         else
             plain(make_growth(theta, o.put(y, T), o.put(y + T, T)), T, N,
                 seed, threshold, scheme, ll, ess, res);
+        return 0;
+    } catch (const std::exception &e) {
+        g_msg = e.what();
+        return -1;
+    }
+}
+
+// the run's selection estimates of x^1..x^k, T x k step-major
+int fused_plain_moments(int model, const double *theta, const double *y,
+    size_t T, size_t N, uint64_t seed, double threshold, int scheme, int k,
+    double *ll, double *est)
+{
+    try {
+        Obs o;
+        if (model == 1)
+            plain_moments(make_lg(theta, o.put(y, T)), T, N, seed, threshold,
+                scheme, k, ll, est);
+        else
+            plain_moments(make_growth(theta, o.put(y, T), o.put(y + T, T)),
+                T, N, seed, threshold, scheme, k, ll, est);
         return 0;
     } catch (const std::exception &e) {
         g_msg = e.what();

@@ -10,7 +10,9 @@ tests/cpp/fused_model_test.hip):
   the same resampling decisions, ESS to 1e-12 and log-likelihood to 1e-10,
   as the built-in models' parity tests ask;
 * the batched form of the growth model gives each filter the plain run's
-  bits.
+  bits;
+* the plain sampler's moment estimates on the growth model agree with the
+  oracle's.
 """
 import ctypes
 import os
@@ -128,6 +130,34 @@ def test_growth_model_vs_oracle(fl, scheme):
         assert np.array_equal(res, ores)
         assert np.allclose(ess, oess, rtol=1e-12, atol=0)
         assert abs(ll - oll) < 1e-10 * max(1.0, abs(oll))
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 0])
+def test_growth_moments_vs_oracle(fl, scheme):
+    """FusedSMCSampler::estimate_moments on the caller's growth model: the
+    selection estimates of x, x^2, x^3 at every step agree with the oracle's
+    (same particles and weights; the oracle sums W_i x_i^j in order, the
+    device in double-double)."""
+    T, k = 30, 3
+    y = growth_data(T, 23)
+    for N in (1, 3000, 4096):
+        th = np.ascontiguousarray(GROWTH, np.float64)
+        yy = np.ascontiguousarray(y, np.float64)
+        ll = ctypes.c_double()
+        est = np.zeros(T * k)
+        rc = fl.fused_plain_moments(3, P(th), P(yy), ctypes.c_size_t(T),
+                                    ctypes.c_size_t(N), ctypes.c_uint64(9),
+                                    ctypes.c_double(0.5), scheme, k,
+                                    ctypes.byref(ll), P(est))
+        assert rc == 0, fl.fused_last_error()
+        oll, _, _, oest = orc.smc_run(3, GROWTH, y, N, 9, 0.5, scheme,
+                                      nthreads=8, moments=k)
+        est = est.reshape(T, k)
+        assert abs(ll.value - oll) < 1e-10 * max(1.0, abs(oll))
+        scale = np.abs(oest) + 1.0
+        assert np.all(np.abs(est - oest) < 1e-9 * scale), (N, scheme)
+        if N == 1:
+            assert np.allclose(est[:, 1], est[:, 0] ** 2, rtol=1e-6)
 
 
 def test_growth_batch_equals_plain(fl):
