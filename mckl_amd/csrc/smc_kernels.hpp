@@ -126,6 +126,7 @@ __device__ __forceinline__ float tiles_max(
     const float *tmax, uint32_t nt, int gt, float *ldsf)
 {
     float m = -__builtin_huge_valf();
+#pragma unroll 4
     for (uint32_t i = gt; i < nt; i += kGroup)
         m = fmaxf(m, tmax[i]);
     return group_max(m, gt, ldsf);
@@ -304,6 +305,8 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
             }
         }
     } else {
+        // loads of four iterations in flight together (unrolled)
+#pragma unroll 4
         for (int i = gt; i < nt; i += kGroup)
             mx = fmaxf(mx, tmax[i]);
     }
@@ -331,6 +334,7 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
             }
         }
     } else {
+#pragma unroll 4
         for (int i = b; i < e; ++i) {
             const double r = tile_scale(tmax[i], m);
             sa = __dadd_rn(sa, __dmul_rn(r, ts[i]));
@@ -398,6 +402,7 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
                 }
             }
         } else {
+#pragma unroll 4
             for (int i = b; i < e; ++i) {
                 if (i == (int)k) {
                     x[0] = run.hi;
@@ -547,6 +552,7 @@ __global__ __launch_bounds__(kGroup) void k_rs_rcounts(SmcArgs A, uint32_t t,
     const int e = b + chunk < nt ? b + chunk : nt;
     uint32_t isum = 0, ioff = 0;
     dd loc{0.0, 0.0};
+#pragma unroll 4
     for (int i = b; i < e; ++i) {
         isum += B.ti[i];
         ioff += i < (int)k ? B.ti[i] : 0u;
@@ -559,6 +565,7 @@ __global__ __launch_bounds__(kGroup) void k_rs_rcounts(SmcArgs A, uint32_t t,
     double *x = L.d + 8;
     {
         dd run = off;
+#pragma unroll 4
         for (int i = b; i < e; ++i) {
             if (i == (int)k) {
                 x[0] = run.hi;

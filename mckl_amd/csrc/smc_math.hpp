@@ -507,6 +507,7 @@ __device__ __forceinline__ void tiles_sum2(const double *s, const double *s2,
     const int b = gt * chunk;
     const int e = b + chunk < n ? b + chunk : n;
     double a = 0.0, c = 0.0;
+#pragma unroll 4
     for (int i = b; i < e; ++i) {
         a = __dadd_rn(a, s[i]);
         c = __dadd_rn(c, s2[i]);
@@ -525,6 +526,7 @@ __device__ __forceinline__ dd tiles_total_dd(
     const int b = gt * chunk;
     const int e = b + chunk < n ? b + chunk : n;
     dd loc{0.0, 0.0};
+#pragma unroll 4
     for (int i = b; i < e; ++i)
         loc = dd_add(loc, dd{hi[i], lo[i]});
     dd total;
@@ -541,11 +543,13 @@ __device__ __forceinline__ void tiles_scan_dd(bool write, const double *hi,
     const int b = gt * chunk;
     const int e = b + chunk < n ? b + chunk : n;
     dd loc{0.0, 0.0};
+#pragma unroll 4
     for (int i = b; i < e; ++i)
         loc = dd_add(loc, dd{hi[i], lo[i]});
     dd total;
     dd off = group_excl_scan_dd(loc, gt, lds, total);
     if (write) {
+#pragma unroll 4
         for (int i = b; i < e; ++i) {
             ohi[i] = off.hi;
             olo[i] = off.lo;
@@ -572,6 +576,7 @@ __device__ __forceinline__ void tiles_offset_dd(const double *hi,
     const int b = gt * chunk;
     const int e = b + chunk < n ? b + chunk : n;
     dd loc{0.0, 0.0};
+#pragma unroll 4
     for (int i = b; i < e; ++i)
         loc = dd_add(loc, dd{hi[i], lo[i]});
     dd total;
@@ -579,6 +584,7 @@ __device__ __forceinline__ void tiles_offset_dd(const double *hi,
     double *x = lds + 8;
     if (b < n || gt == 0) {
         dd run = off;
+#pragma unroll 4
         for (int i = b; i < e; ++i) {
             if (i == j0) {
                 x[0] = run.hi;
@@ -616,6 +622,7 @@ __device__ __forceinline__ void tiles_scan_u2(bool write, const uint32_t *a,
     const int b = gt * chunk;
     const int e = b + chunk < n ? b + chunk : n;
     uint32_t la = 0, lb = 0;
+#pragma unroll 4
     for (int i = b; i < e; ++i) {
         la += a[i];
         lb += b2[i];
@@ -624,6 +631,7 @@ __device__ __forceinline__ void tiles_scan_u2(bool write, const uint32_t *a,
     uint32_t xa = group_excl_scan_u(la, gt, lds, ta);
     uint32_t xb = group_excl_scan_u(lb, gt, lds + 4, tb);
     if (write) {
+#pragma unroll 4
         for (int i = b; i < e; ++i) {
             oa[i] = xa;
             ob[i] = xb;
