@@ -264,11 +264,14 @@ def test_residual_systematic_equals_systematic(dev):
 @pytest.mark.parametrize("N,T,scheme", [((1 << 21) + 77, 8, SYS),
                                         (1, 12, STRAT), (2, 12, MULT),
                                         ((1 << 21) + 77, 4, RES),
-                                        (2048, 1, SYS)])
+                                        (2048, 1, SYS),
+                                        ((1 << 23) + 5, 3, STRAT),
+                                        ((1 << 24) + 5, 3, SYS)])
 def test_plain_sizes_vs_oracle(dev, N, T, scheme):
     """Sizes the configurations do not reach: more than 1024 tiles (several
-    tiles per thread in every cross-tile reduction), one and two particles,
-    a single step."""
+    tiles per thread in every cross-tile reduction; at 2^24 + 5, 8193
+    tiles: 33 per thread, a look-back of nine rounds and more workgroups
+    than the GPU holds at once), one and two particles, a single step."""
     m = _m()
     y = _obs()["lg_y"][:T]
     th = (0.9, 1.0, 0.5)
