@@ -95,7 +95,9 @@ __global__ __launch_bounds__(kGroup) void k_init(
 
 // move + weight of a tile, then the tile's sums of a_i = exp_f(lw_i - m_k)
 // about its own max m_k (the weights' scale r_k needs the max over tiles and
-// is applied by the next kernel)
+// is applied by the next kernel); with B.tpre (systematic, stratified) each
+// thread also stores its exclusive double-double prefix of a within the
+// tile, which k_rs_counts would otherwise recompute
 template <class M>
 __global__ __launch_bounds__(kGroup) void k_move(SmcArgs A, M model,
     uint32_t t, const float *xin, float *xout, float *lw, const uint32_t *idx,
@@ -136,9 +138,9 @@ __device__ __forceinline__ float tiles_max(
 // workgroup 0 records it.  When the step resamples, each workgroup derives
 // its tile's prefix offsets and counts the uniforms below its cumulative
 // weights.  All global loads the workgroup needs are issued up front (its
-// tile's log weights, its chunk of every per-tile array), so the kernel
-// waits on memory once; the reductions are those of tiles_max, tiles_sum2
-// and tiles_offset_dd, operation for operation.
+// tile's log weights and thread prefixes, its chunk of every per-tile
+// array), so the kernel waits on memory once; the reductions are those of
+// tiles_max, tiles_sum2 and tiles_offset_dd, operation for operation.
 __host__ __device__ constexpr bool is_residual(int scheme)
 {
     return scheme == MCKL_HIP_RESIDUAL ||
