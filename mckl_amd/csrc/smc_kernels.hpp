@@ -794,7 +794,10 @@ __global__ __launch_bounds__(G *kGroup, G == 1 ? 2 : 1) void k_batch(
         const SmcArgs A = fd[f].A;
         const M model = fd[f].model;
         const uint64_t off = fd[f].off;
-        float *x0 = xa + off, *x1 = xb + off, *lw = lwb + off;
+        // states by component: model_dim blocks of the filter's padded
+        // size (its share of xa/xb is model_dim times its offset)
+        constexpr uint64_t D = model_dim<M>::value;
+        float *x0 = xa + D * off, *x1 = xb + D * off, *lw = lwb + off;
         uint32_t *idx = idxb + off, *cnt = cntb + off, *Eincl = Eb + off;
         uint32_t *zpos = zb + off;
         const int nt = (int)A.ntiles;

@@ -142,8 +142,9 @@ inline int plain_init(PlainSampler<M> *s, const M &model, size_t N, size_t T,
             return rc;                                                        \
         }                                                                     \
     } while (0)
-    MCKL_ALLOC(s->x[0], Np);
-    MCKL_ALLOC(s->x[1], Np);
+    // states by component (model_dim of them), each a stride of Np
+    MCKL_ALLOC(s->x[0], Np * (size_t)model_dim<M>::value);
+    MCKL_ALLOC(s->x[1], Np * (size_t)model_dim<M>::value);
     MCKL_ALLOC(s->lw, Np);
     MCKL_ALLOC(s->idx, Np);
     MCKL_ALLOC(s->tmax, nt);

@@ -18,6 +18,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 #include "smc_index.hpp"
@@ -183,6 +184,26 @@ __device__ __forceinline__ void group_sum2_u(
 // counter-based stream (smc_normal_pair: particle i, step t), the states
 // FP32, the bootstrap weights lw_t = carry + log_weight(x_t).
 //
+// A model whose state is a vector of D > 1 floats declares
+//   static constexpr int dim = D;
+// and takes and returns the state through arrays of D floats:
+//   __device__ void init(const float *e, float *x) const   x_0 from D normals
+//   __device__ void move(const float *xp, const float *e, float *xn) const
+//   __device__ float log_weight(const float *x) const
+// (move and log_weight on the Step).  Component d of particle i draws the
+// stream's normal of index i D + d, so a model of dimension 1 written either
+// way draws the same noise.  States are stored by component (component d of
+// every particle contiguous, a stride of whole tiles), so each component's
+// loads and stores stay coalesced.
+template <class M, class = void>
+struct model_dim {
+    static constexpr int value = 1;
+};
+template <class M>
+struct model_dim<M, std::void_t<decltype(M::dim)>> {
+    static constexpr int value = M::dim;
+};
+
 // The built-in linear-Gaussian and stochastic-volatility models in that
 // form: ModelConst and the per-step observation constants (obs_const).
 struct BuiltinModel {
@@ -215,9 +236,68 @@ __host__ __device__ inline BuiltinModel builtin_model(const SmcArgs &A)
     return BuiltinModel{A.mc, A.obs};
 }
 
-// t = 0: x_i = init(eps(i, 0)), lw_i = 0
+// the stream's normals of indices [j0, j0 + 2 n) at step t (j0 even): the
+// pairs of smc_normal_pair
+template <int n2>
+__device__ __forceinline__ void normals_from(
+    uint32_t k0, uint32_t k1, uint64_t j0, uint64_t t, float *e)
+{
+#pragma unroll
+    for (int q = 0; q < n2; q += 2)
+        smc_normal_pair(k0, k1, (j0 + q) >> 1, t, e[q], e[q + 1]);
+}
+
+// a thread's 8 values of one state component (or of lw), plain or
+// non-temporal stores, element stores past N
+template <bool NT>
+__device__ __forceinline__ void store8f(
+    float *p, uint64_t i0, uint64_t N, const float (&v)[kItems])
+{
+    if constexpr (NT) {
+        store8_nt(p, i0, N, v);
+    } else if (i0 + kItems <= N) {
+        *reinterpret_cast<float4 *>(p + i0) =
+            make_float4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<float4 *>(p + i0 + 4) =
+            make_float4(v[4], v[5], v[6], v[7]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            if (i0 + k < N)
+                p[i0 + k] = v[k];
+    }
+}
+
+// vector states (model_dim > 1): component d of particle i at x[d Np + i],
+// Np = ntiles * kTile
 template <class M>
-__device__ __forceinline__ void tile_init(const SmcArgs &A, const M &model,
+__device__ __forceinline__ void tile_init_v(const SmcArgs &A, const M &model,
+    uint32_t tile, float *x, float *lw, int gt)
+{
+    constexpr int D = model_dim<M>::value;
+    const uint64_t i0 = tile_first(tile, gt);
+    const uint64_t Np = (uint64_t)A.ntiles * kTile;
+    float e[kItems * D];
+    normals_from<kItems * D>(A.k0, A.k1, i0 * D, 0, e);
+    float xs[D][kItems], zero[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        float xv[D];
+        model.init(e + k * D, xv);
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            xs[d][k] = xv[d];
+        zero[k] = 0.0f;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        store8f<false>(x + d * Np, i0, A.N, xs[d]);
+    store8f<false>(lw, i0, A.N, zero);
+}
+
+// t = 0: x_i = init(eps(i, 0)), lw_i = 0 (scalar states)
+template <class M>
+__device__ __forceinline__ void tile_init_s(const SmcArgs &A, const M &model,
     uint32_t tile, float *x, float *lw, int gt)
 {
     const uint64_t i0 = tile_first(tile, gt);
@@ -235,6 +315,17 @@ __device__ __forceinline__ void tile_init(const SmcArgs &A, const M &model,
             lw[i + 1] = 0.0f;
         }
     }
+}
+
+// t = 0: x_i = init(eps(i, 0)), lw_i = 0
+template <class M>
+__device__ __forceinline__ void tile_init(const SmcArgs &A, const M &model,
+    uint32_t tile, float *x, float *lw, int gt)
+{
+    if constexpr (model_dim<M>::value > 1)
+        tile_init_v(A, model, tile, x, lw, gt);
+    else
+        tile_init_s(A, model, tile, x, lw, gt);
 }
 
 // first j with E[j] > z (E nondecreasing, length n)
@@ -275,12 +366,69 @@ __device__ __forceinline__ uint32_t resolve_parent(
     return a < N ? a : N - 1;
 }
 
-// move + weight of one tile; returns the tile max of lw (all threads) and
-// the thread's new log weights in lout (0 past the end).  After a
-// resampling step idx holds every slot's parent (Codes false), or its code
-// resolved through src (Codes true, resolve_parent).
+// tile_move for vector states (model_dim > 1)
 template <bool Codes, class M>
-__device__ __forceinline__ float tile_move(const SmcArgs &A, const M &model,
+__device__ __forceinline__ float tile_move_v(const SmcArgs &A, const M &model,
+    uint32_t t, uint32_t tile, const float *xin, float *xout, float *lw,
+    const uint32_t *idx, const uint32_t *src, int resampled, float mprev,
+    int gt, float *ldsf, float (&lout)[kItems])
+{
+    constexpr int D = model_dim<M>::value;
+    const uint64_t i0 = tile_first(tile, gt);
+    const uint64_t Np = (uint64_t)A.ntiles * kTile;
+    const auto step = model.at(t);
+    uint32_t par[kItems];
+    float carry[kItems];
+    if (resampled) {
+        uint32_t a[kItems];
+        load8(idx, i0, A.N, a, 0u);
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            par[k] = resolve_parent(a[k], Codes ? src : nullptr, A.N);
+            carry[k] = 0.0f;
+        }
+    } else {
+        float w[kItems];
+        load8(lw, i0, A.N, w, 0.0f);
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            par[k] = i0 + k < A.N ? (uint32_t)(i0 + k) : A.N - 1;
+            carry[k] = __fsub_rn(w[k], mprev);
+        }
+    }
+    // the normals do not depend on the parents: drawn while they load
+    float e[kItems * D];
+    normals_from<kItems * D>(A.k0, A.k1, i0 * D, t, e);
+    float xp[kItems * D];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            xp[k * D + d] = xin[d * Np + par[k]];
+    float xs[D][kItems];
+    float lmax = -__builtin_huge_valf();
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        float xn[D];
+        step.move(xp + k * D, e + k * D, xn);
+        const float l = __fadd_rn(carry[k], step.log_weight(xn));
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            xs[d][k] = xn[d];
+        const bool live = i0 + k < A.N;
+        lout[k] = live ? l : 0.0f;
+        lmax = live ? fmaxf(lmax, l) : lmax;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        store8f<Codes>(xout + d * Np, i0, A.N, xs[d]);
+    store8f<Codes>(lw, i0, A.N, lout);
+    return group_max(lmax, gt, ldsf);
+}
+
+// tile_move for scalar states
+template <bool Codes, class M>
+__device__ __forceinline__ float tile_move_s(const SmcArgs &A, const M &model,
     uint32_t t, uint32_t tile, const float *xin, float *xout, float *lw,
     const uint32_t *idx, const uint32_t *src, int resampled, float mprev,
     int gt, float *ldsf, float (&lout)[kItems])
@@ -404,6 +552,24 @@ __device__ __forceinline__ float tile_move(const SmcArgs &A, const M &model,
         }
     }
     return group_max(lmax, gt, ldsf);
+}
+
+// move + weight of one tile; returns the tile max of lw (all threads) and
+// the thread's new log weights in lout (0 past the end).  After a
+// resampling step idx holds every slot's parent (Codes false), or its code
+// resolved through src (Codes true, resolve_parent).
+template <bool Codes, class M>
+__device__ __forceinline__ float tile_move(const SmcArgs &A, const M &model,
+    uint32_t t, uint32_t tile, const float *xin, float *xout, float *lw,
+    const uint32_t *idx, const uint32_t *src, int resampled, float mprev,
+    int gt, float *ldsf, float (&lout)[kItems])
+{
+    if constexpr (model_dim<M>::value > 1)
+        return tile_move_v<Codes>(A, model, t, tile, xin, xout, lw, idx,
+            src, resampled, mprev, gt, ldsf, lout);
+    else
+        return tile_move_s<Codes>(A, model, t, tile, xin, xout, lw, idx,
+            src, resampled, mprev, gt, ldsf, lout);
 }
 
 // Weights.  With m_k the tile's max of lw and m the max over tiles, particle

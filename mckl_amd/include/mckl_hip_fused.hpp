@@ -17,9 +17,24 @@
 //       __device__ Step at(uint32_t t) const;                // step t = 1..T
 //   };
 //
+// or, for a state of D > 1 floats,
+//
+//   struct MyModel {
+//       static constexpr int dim = D;
+//       struct Step {
+//           __device__ void move(const float *xp, const float *e,
+//               float *xn) const;                            // D values each
+//           __device__ float log_weight(const float *x) const;
+//       };
+//       __device__ void init(const float *e, float *x) const;
+//       __device__ Step at(uint32_t t) const;
+//   };
+//
 // e is a standard normal draw from the samplers' counter-based stream
-// (particle i, step t; the same for every model), states are FP32, weights
-// follow the bootstrap filter: after the move, lw_t = carry + log_weight.
+// (particle i, step t, component d: the stream's normal i D + d; the same
+// for every model), states are FP32, weights follow the bootstrap filter:
+// after the move, lw_t = carry + log_weight.  The moment estimates of a
+// vector state are those of its first component.
 // The samplers are the library's own (plain: two launches per step captured
 // into one HIP graph; batched: one persistent launch for all filters), with
 // all six resampling schemes; a model that reproduces the built-in
@@ -161,8 +176,8 @@ class FusedSMCBatch
             alloc(fd_, nf_);
             alloc(order_, nf_);
             alloc(queue_, 2);
-            alloc(xa_, total);
-            alloc(xb_, total);
+            alloc(xa_, total * (std::size_t)model_dim<M>::value);
+            alloc(xb_, total * (std::size_t)model_dim<M>::value);
             alloc(lw_, total);
             alloc(idx_, total);
             alloc(cnt_, total);

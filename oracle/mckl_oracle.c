@@ -1442,6 +1442,16 @@ typedef struct {
 static void mo_model_init(int model, const double *th, mo_model *m)
 {
     const double l2pi = log(2.0 * 3.141592653589793);
+    if (model == MO_MODEL_CV2) {
+        /* theta = (sp, sv, sy): position and velocity noise scales, the
+         * observation scale; both components start N(0, 1) */
+        m->s = (float)th[0];
+        m->a = (float)th[1];
+        m->sd0 = 1.0f;
+        m->inv_sy = (float)(1.0 / th[2]);
+        m->cst = (float)(-log(th[2]) - 0.5 * l2pi);
+        return;
+    }
     if (model == MO_MODEL_GROWTH) {
         m->a = 0.0f;
         m->s = (float)th[0];
@@ -1465,7 +1475,7 @@ static void mo_model_init(int model, const double *th, mo_model *m)
 /* per-step observation constant: LG (float)y, SV (float)(y^2 / beta^2) */
 float mo_smc_obs(int model, const double *th, double y)
 {
-    if (model == MO_MODEL_GROWTH)
+    if (model == MO_MODEL_GROWTH || model == MO_MODEL_CV2)
         return (float)y;
     return model == MO_MODEL_LG ? (float)y : (float)(y * y / (th[2] * th[2]));
 }
@@ -1480,7 +1490,8 @@ float mo_smc_logg(int model, const double *th, float obs, float x)
         const float z = d * m.inv_sy;
         return -0.5f * (z * z) + m.cst;
     }
-    if (model == MO_MODEL_LG) {
+    if (model == MO_MODEL_LG || model == MO_MODEL_CV2) {
+        /* CV2: x is the position component */
         const float d = obs - x;
         const float z = d * m.inv_sy;
         return -0.5f * (z * z) + m.cst;
@@ -1528,8 +1539,11 @@ static double smc_run_impl(int model, const double *theta, const double *y,
 {
     mo_model m;
     mo_model_init(model, theta, &m);
-    float *x = (float *)malloc(N * sizeof(float));
-    float *xn = (float *)malloc(N * sizeof(float));
+    /* CV2 keeps (position, velocity): component d of particle i at
+     * x[d N + i], its noise the stream's normal 2 i + d */
+    const size_t D = model == MO_MODEL_CV2 ? 2 : 1;
+    float *x = (float *)malloc(D * N * sizeof(float));
+    float *xn = (float *)malloc(D * N * sizeof(float));
     float *lw = (float *)malloc(N * sizeof(float));
     double *e = (double *)malloc(N * sizeof(double));
     double *resid = (double *)malloc(N * sizeof(double));
@@ -1547,7 +1561,12 @@ static double smc_run_impl(int model, const double *theta, const double *y,
     const long long Ns = (long long)N;
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < Ns; ++i) {
-        x[i] = m.sd0 * mo_smc_normal(seed, (uint64_t)i, 0);
+        if (D == 2) {
+            x[i] = mo_smc_normal(seed, 2 * (uint64_t)i, 0);
+            x[N + i] = mo_smc_normal(seed, 2 * (uint64_t)i + 1, 0);
+        } else {
+            x[i] = m.sd0 * mo_smc_normal(seed, (uint64_t)i, 0);
+        }
         lw[i] = 0.0f;
     }
     /* t = 1 carries lw = 0 with m_0 = 0, so C_1 = S_0 = N */
@@ -1563,12 +1582,24 @@ static double smc_run_impl(int model, const double *theta, const double *y,
 #pragma omp parallel for schedule(static)
         for (long long i = 0; i < Ns; ++i) {
             float xp, carry;
+            const size_t src = resampled ? (size_t)idx[i] : (size_t)i;
             if (resampled) {
                 xp = x[idx[i]];
                 carry = 0.0f;
             } else {
                 xp = x[i];
                 carry = lw[i] - mprev;
+            }
+            if (D == 2) {
+                /* position += velocity + sp e0, velocity += sv e1 */
+                const float e0 = mo_smc_normal(seed, 2 * (uint64_t)i, t);
+                const float e1 = mo_smc_normal(seed, 2 * (uint64_t)i + 1, t);
+                const float vp = x[N + src];
+                const float x0 = (xp + vp) + m.s * e0;
+                xn[i] = x0;
+                xn[N + i] = vp + m.a * e1;
+                lw[i] = carry + mo_smc_logg(model, theta, obs, x0);
+                continue;
             }
             const float eps = mo_smc_normal(seed, (uint64_t)i, t);
             float xv;

@@ -158,7 +158,7 @@ void mo_resample(int scheme, size_t n, size_t m, mo_engine *e,
  * fused samplers on a caller's own model): theta = (s, sy, sd0),
  * x_t = (0.5 x + 25 x / (1 + x^2)) + u_t + s e, y_t = x^2 / 20 + sy w, the
  * T observations followed by the T drift values u_t in y */
-enum { MO_MODEL_LG = 1, MO_MODEL_SV = 2, MO_MODEL_GROWTH = 3 };
+enum { MO_MODEL_LG = 1, MO_MODEL_SV = 2, MO_MODEL_GROWTH = 3, MO_MODEL_CV2 = 4 };
 float mo_log_f(float x);
 float mo_exp_f(float x);
 void mo_sincos2pi_f(float u, float *s, float *c);

@@ -4,7 +4,8 @@
 //  * UserLG: the linear-Gaussian model's arithmetic, written out here (it
 //    must give the built-in sampler's bits);
 //  * Growth: the nonstationary growth model, x_t = (0.5 x + 25 x / (1 +
-//    x^2)) + u_t + s e, y_t = x^2 / 20 + sy w (the oracle's model 3).
+//    x^2)) + u_t + s e, y_t = x^2 / 20 + sy w (the oracle's model 3);
+//  * CV2: a two-component state (position, velocity), the oracle's model 4.
 // Test infrastructure only.
 #include "mckl_hip_fused.hpp"
 
@@ -63,6 +64,36 @@ struct Growth {
     }
 };
 
+// a vector state: (position, velocity), position += velocity + sp e0,
+// velocity += sv e1, y = position + sy w (the oracle's model 4)
+struct CV2 {
+    static constexpr int dim = 2;
+    float sp, sv, inv_sy, cst;
+    const float *obs;
+    struct Step {
+        float sp, sv, inv_sy, cst, y;
+        __device__ void move(const float *xp, const float *e, float *xn) const
+        {
+            xn[0] = __fadd_rn(__fadd_rn(xp[0], xp[1]), __fmul_rn(sp, e[0]));
+            xn[1] = __fadd_rn(xp[1], __fmul_rn(sv, e[1]));
+        }
+        __device__ float log_weight(const float *x) const
+        {
+            const float z = __fmul_rn(__fsub_rn(y, x[0]), inv_sy);
+            return __fadd_rn(__fmul_rn(-0.5f, __fmul_rn(z, z)), cst);
+        }
+    };
+    __device__ void init(const float *e, float *x) const
+    {
+        x[0] = e[0];
+        x[1] = e[1];
+    }
+    __device__ Step at(uint32_t t) const
+    {
+        return Step{sp, sv, inv_sy, cst, obs[t - 1]};
+    }
+};
+
 const double kL2pi = std::log(2.0 * 3.141592653589793);
 
 UserLG make_lg(const double *th, const float *d_obs)
@@ -76,6 +107,13 @@ Growth make_growth(const double *th, const float *d_obs, const float *d_u)
 {
     return Growth{(float)th[0], (float)th[2], (float)(1.0 / th[1]),
         (float)(-std::log(th[1]) - 0.5 * kL2pi), d_obs, d_u};
+}
+
+// theta = (sp, sv, sy)
+CV2 make_cv2(const double *th, const float *d_obs)
+{
+    return CV2{(float)th[0], (float)th[1], (float)(1.0 / th[2]),
+        (float)(-std::log(th[2]) - 0.5 * kL2pi), d_obs};
 }
 
 std::string g_msg;
@@ -140,7 +178,8 @@ extern "C" {
 
 const char *fused_last_error() { return g_msg.c_str(); }
 
-// model 1 (UserLG) or 3 (Growth); y: T (model 1) or 2 T (model 3) values
+// model 1 (UserLG), 3 (Growth) or 4 (CV2); y: T (models 1, 4) or 2 T
+// (model 3) values
 int fused_plain_run(int model, const double *theta, const double *y,
     size_t T, size_t N, uint64_t seed, double threshold, int scheme,
     double *ll, double *ess, int32_t *res)
@@ -149,6 +188,9 @@ int fused_plain_run(int model, const double *theta, const double *y,
         Obs o;
         if (model == 1)
             plain(make_lg(theta, o.put(y, T)), T, N, seed, threshold, scheme,
+                ll, ess, res);
+        else if (model == 4)
+            plain(make_cv2(theta, o.put(y, T)), T, N, seed, threshold, scheme,
                 ll, ess, res);
         else
             plain(make_growth(theta, o.put(y, T), o.put(y + T, T)), T, N,
@@ -170,6 +212,9 @@ int fused_plain_moments(int model, const double *theta, const double *y,
         if (model == 1)
             plain_moments(make_lg(theta, o.put(y, T)), T, N, seed, threshold,
                 scheme, k, ll, est);
+        else if (model == 4)
+            plain_moments(make_cv2(theta, o.put(y, T)), T, N, seed,
+                threshold, scheme, k, ll, est);
         else
             plain_moments(make_growth(theta, o.put(y, T), o.put(y + T, T)),
                 T, N, seed, threshold, scheme, k, ll, est);
@@ -181,6 +226,7 @@ int fused_plain_moments(int model, const double *theta, const double *y,
 }
 
 // nf filters with their own theta (3 each), y (T or 2 T each), N and seed
+// (models 1, 3, 4)
 int fused_batch_run(int model, size_t nf, const uint32_t *N,
     const double *theta, const double *y, size_t T, const uint64_t *seeds,
     double threshold, int scheme, double *ll)
@@ -195,6 +241,14 @@ int fused_batch_run(int model, size_t nf, const uint32_t *N,
             for (size_t f = 0; f < nf; ++f)
                 m.push_back(make_lg(theta + 3 * f, o.put(y + T * f, T)));
             mckl::FusedSMCBatch<UserLG> b(m, n, T, sd, threshold, scheme);
+            b.iterate();
+            out = b.log_likelihood();
+        } else if (model == 4) {
+            std::vector<CV2> m;
+            for (size_t f = 0; f < nf; ++f)
+                m.push_back(make_cv2(theta + 3 * f, o.put(y + T * f, T)));
+            mckl::FusedSMCBatch<CV2> b(m, n, T, sd, threshold, scheme);
+            b.iterate();
             b.iterate();
             out = b.log_likelihood();
         } else {

@@ -12,7 +12,9 @@ tests/cpp/fused_model_test.hip):
 * the batched form of the growth model gives each filter the plain run's
   bits;
 * the plain sampler's moment estimates on the growth model agree with the
-  oracle's.
+  oracle's;
+* a model with a two-float state (the oracle's model 4, pinned by a
+  Kalman filter) gives the oracle's runs, plain and batched.
 """
 import ctypes
 import os
@@ -130,6 +132,78 @@ def test_growth_model_vs_oracle(fl, scheme):
         assert np.array_equal(res, ores)
         assert np.allclose(ess, oess, rtol=1e-12, atol=0)
         assert abs(ll - oll) < 1e-10 * max(1.0, abs(oll))
+
+
+CV2 = (0.3, 0.2, 0.8)  # (sp, sv, sy)
+
+
+def cv2_data(T, seed):
+    """Observations of the two-component constant-velocity model."""
+    rng = np.random.default_rng(seed)
+    sp, sv, sy = CV2
+    x = rng.standard_normal(2)
+    y = np.zeros(T)
+    for t in range(T):
+        x = np.array([x[0] + x[1], x[1]]) + np.array([sp, sv]) * \
+            rng.standard_normal(2)
+        y[t] = x[0] + sy * rng.standard_normal()
+    return y
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_vector_state_vs_oracle(fl, scheme):
+    """A model with a two-float state (dim = 2: states by component, the
+    normals i D + d) gives the oracle's model 4 run: same decisions, ESS to
+    1e-12, log Z to 1e-10."""
+    T = 40
+    y = cv2_data(T, 31)
+    for N, seed in ((4096, 7), (3000, 8), (2049, 9)):
+        ll, ess, res = plain(fl, 4, CV2, y, T, N, seed, 0.5, scheme)
+        oll, oess, ores = orc.smc_run(4, CV2, y, N, seed, 0.5, scheme,
+                                      nthreads=8, history=True)
+        print(scheme, N, ll, oll, int(res.sum()))
+        assert 0 < res.sum() <= T
+        assert np.array_equal(res, ores)
+        assert np.allclose(ess, oess, rtol=1e-12, atol=0)
+        assert abs(ll - oll) < 1e-10 * max(1.0, abs(oll))
+
+
+def test_vector_state_batch_equals_plain(fl):
+    T = 30
+    N = [256, 2048, 4097, 9000, 640]
+    seeds = [41, 42, 43, 44, 45]
+    ys = [cv2_data(T, 50 + f) for f in range(len(N))]
+    th = np.array([CV2] * len(N))
+    th[1, 1] = 0.4
+    th[3, 2] = 1.2
+    for scheme in (1, 2, 5):
+        ll = batch(fl, 4, th, np.concatenate(ys), T, N, seeds, 0.5, scheme)
+        for f in range(len(N)):
+            lp, _, _ = plain(fl, 4, th[f], ys[f], T, N[f], seeds[f], 0.5,
+                             scheme)
+            assert ll[f] == lp, (scheme, f, ll[f], lp)
+
+
+def test_vector_state_moments_vs_oracle(fl):
+    """The moment estimates of a vector state are those of its first
+    component (the oracle's too)."""
+    T, k = 20, 2
+    y = cv2_data(T, 33)
+    th = np.ascontiguousarray(CV2, np.float64)
+    yy = np.ascontiguousarray(y, np.float64)
+    for N in (1, 3000):
+        ll = ctypes.c_double()
+        est = np.zeros(T * k)
+        rc = fl.fused_plain_moments(4, P(th), P(yy), ctypes.c_size_t(T),
+                                    ctypes.c_size_t(N), ctypes.c_uint64(9),
+                                    ctypes.c_double(0.5), 2, k,
+                                    ctypes.byref(ll), P(est))
+        assert rc == 0, fl.fused_last_error()
+        oll, _, _, oest = orc.smc_run(4, CV2, y, N, 9, 0.5, 2, nthreads=8,
+                                      moments=k)
+        est = est.reshape(T, k)
+        assert abs(ll.value - oll) < 1e-10 * max(1.0, abs(oll))
+        assert np.all(np.abs(est - oest) < 1e-9 * (np.abs(oest) + 1.0))
 
 
 @pytest.mark.parametrize("scheme", [1, 2, 0])
