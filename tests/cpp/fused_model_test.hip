@@ -94,6 +94,52 @@ struct CV2 {
     }
 };
 
+// The reference's particle-filter example (example/algorithm/include/
+// algorithm_pf.hpp:91-150) as a four-float state (p, q, s, t): iteration 0
+// draws p, q ~ N(0, 2), s, t ~ N(0, 1) and weighs the first observation;
+// every later iteration moves p += 0.1 s + sqrt(0.02) e, s += sqrt(0.001) e
+// (then q, t likewise) and weighs by -5.5 (log1p(0.1 (10 (p - x))^2) +
+// log1p(0.1 (10 (q - y))^2)).  The fused samplers move before every
+// weighting, so step 1's move is the identity.  FP32 states and the
+// samplers' own noise stream: the same model, not the reference's bits.
+struct PF4 {
+    static constexpr int dim = 4;
+    const float *ox, *oy;
+    struct Step {
+        float ox, oy;
+        bool first;
+        __device__ void move(const float *xp, const float *e, float *xn) const
+        {
+            if (first) {
+                for (int d = 0; d < 4; ++d)
+                    xn[d] = xp[d];
+                return;
+            }
+            const float sp = 0.141421356f, sv = 0.0316227766f;
+            xn[0] = xp[0] + (0.1f * xp[2] + sp * e[0]);
+            xn[2] = xp[2] + sv * e[1];
+            xn[1] = xp[1] + (0.1f * xp[3] + sp * e[2]);
+            xn[3] = xp[3] + sv * e[3];
+        }
+        __device__ float log_weight(const float *x) const
+        {
+            const float a = 10.0f * (x[0] - ox), b = 10.0f * (x[1] - oy);
+            return -5.5f * (log1pf(0.1f * a * a) + log1pf(0.1f * b * b));
+        }
+    };
+    __device__ void init(const float *e, float *x) const
+    {
+        x[0] = 2.0f * e[0];
+        x[1] = 2.0f * e[1];
+        x[2] = e[2];
+        x[3] = e[3];
+    }
+    __device__ Step at(uint32_t t) const
+    {
+        return Step{ox[t - 1], oy[t - 1], t == 1};
+    }
+};
+
 const double kL2pi = std::log(2.0 * 3.141592653589793);
 
 UserLG make_lg(const double *th, const float *d_obs)
@@ -218,6 +264,59 @@ int fused_plain_moments(int model, const double *theta, const double *y,
         else
             plain_moments(make_growth(theta, o.put(y, T), o.put(y + T, T)),
                 T, N, seed, threshold, scheme, k, ll, est);
+        return 0;
+    } catch (const std::exception &e) {
+        g_msg = e.what();
+        return -1;
+    }
+}
+
+// the particle-filter example on the fused plain sampler (stratified
+// resampling below ESS N/2, as the example): xy holds the T observations as
+// (x, y) pairs; writes log Z-hat, the ESS and resampling histories, the
+// last step's posterior mean and second moment of p (post[0], post[1]) and
+// the time of one iterate() in ms
+int fused_pf_run(const double *xy, size_t T, size_t N, uint64_t seed,
+    double *ll, double *ess, int32_t *res, double *post, double *ms)
+{
+    try {
+        Obs o;
+        std::vector<double> xs(T), ys(T);
+        for (size_t t = 0; t < T; ++t) {
+            xs[t] = xy[2 * t];
+            ys[t] = xy[2 * t + 1];
+        }
+        const PF4 model{o.put(xs.data(), T), o.put(ys.data(), T)};
+        mckl::FusedSMCSampler<PF4> s(model, N, T, seed, 0.5,
+            MCKL_HIP_STRATIFIED);
+        s.iterate();
+        (void)s.log_likelihood();
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) !=
+            hipSuccess)
+            throw mckl::RuntimeAssert("hipEventCreate");
+        (void)hipEventRecord(a, nullptr);
+        s.iterate();
+        (void)hipEventRecord(b, nullptr);
+        (void)hipEventSynchronize(b);
+        float f = 0.0f;
+        (void)hipEventElapsedTime(&f, a, b);
+        *ms = f;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        *ll = s.log_likelihood();
+        const std::vector<double> e = s.ess_history();
+        const std::vector<int32_t> r = s.resampled_history();
+        for (size_t t = 0; t < T; ++t) {
+            ess[t] = e[t];
+            res[t] = r[t];
+        }
+        // moments of the first component (p) at every step
+        s.estimate_moments(2);
+        s.iterate();
+        const std::vector<double> m = s.estimates();
+        post[0] = m[2 * (T - 1)];
+        post[1] = m[2 * (T - 1) + 1];
         return 0;
     } catch (const std::exception &e) {
         g_msg = e.what();

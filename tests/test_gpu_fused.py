@@ -14,7 +14,9 @@ tests/cpp/fused_model_test.hip):
 * the plain sampler's moment estimates on the growth model agree with the
   oracle's;
 * a model with a two-float state (the oracle's model 4, pinned by a
-  Kalman filter) gives the oracle's runs, plain and batched.
+  Kalman filter) gives the oracle's runs, plain and batched;
+* the reference's own particle-filter model (a four-float state) agrees
+  with the reference's run statistically.
 """
 import ctypes
 import os
@@ -204,6 +206,46 @@ def test_vector_state_moments_vs_oracle(fl):
         est = est.reshape(T, k)
         assert abs(ll.value - oll) < 1e-10 * max(1.0, abs(oll))
         assert np.all(np.abs(est - oest) < 1e-9 * (np.abs(oest) + 1.0))
+
+
+def test_reference_pf_model(fl):
+    """The reference's particle-filter example model (four-float state,
+    algorithm_pf.hpp:91-150) on the fused sampler, against the reference's
+    own run (pf_reference.npz, N = 10^4, BackendSEQ): the last step's
+    posterior mean of the position p from 2^20 particles lies within 4
+    standard deviations of the fused estimate at N = 10^4 (measured over 16
+    seeds) of the reference's weighted mean.  Same model, other noise and
+    FP32 states: a statistical check."""
+    xy = np.loadtxt(os.path.join(GOLDEN, "algorithm_pf.data")).ravel()
+    T = len(xy) // 2
+    fix = np.load(os.path.join(GOLDEN, "pf_reference.npz"))
+    p, w = fix["N10000_s7_state"][0], fix["N10000_s7_weight"]
+    ref = float(np.sum(w * p) / np.sum(w))
+
+    def run(N, seed):
+        ll = ctypes.c_double()
+        ess = np.zeros(T)
+        res = np.zeros(T, np.int32)
+        post = np.zeros(2)
+        ms = ctypes.c_double()
+        xx = np.ascontiguousarray(xy, np.float64)
+        rc = fl.fused_pf_run(P(xx), ctypes.c_size_t(T), ctypes.c_size_t(N),
+                             ctypes.c_uint64(seed), ctypes.byref(ll), P(ess),
+                             P(res), P(post), ctypes.byref(ms))
+        assert rc == 0, fl.fused_last_error()
+        return ll.value, ess, res, post, ms.value
+
+    small = np.array([run(10000, 100 + s)[3][0] for s in range(16)])
+    sd = small.std(ddof=1)
+    ll, ess, res, post, ms = run(1 << 20, 5)
+    print("p mean: fused 2^20 %.6f (%.3f ms per run, %d resampling steps), "
+          "reference N=1e4 %.6f, sd at 1e4 %.2e" % (post[0], ms,
+                                                   int(res.sum()), ref, sd))
+    assert np.all(ess > 0) and 0 < res.sum() <= T
+    assert post[1] >= post[0] ** 2 * (1 - 1e-6)
+    assert abs(post[0] - ref) < 4 * sd
+    assert abs(small.mean() - post[0]) < 4 * sd / np.sqrt(len(small)) + \
+        1e-3 * abs(post[0])
 
 
 @pytest.mark.parametrize("scheme", [1, 2, 0])
