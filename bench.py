@@ -343,12 +343,50 @@ def bench_pf_example(m, torch):
     if r.returncode != 0:
         raise RuntimeError("algorithm_pf failed: " + r.stderr[-500:])
     sec = float(r.stdout.split("iterate_seconds")[1].split()[0])
-    return {"pf_example_N1e5_T100": {
+    res = {"pf_example_N1e5_T100": {
         "seconds": round(sec, 4),
         "particle_steps_per_s": 1e5 * 100 / sec,
         "note": "generic SMCSampler<double> (ColMajor 4-column state) with "
                 "device functors for the selection arithmetic; one ESS read "
                 "to the host per step"}}
+    res.update(_pf_fused(torch))
+    return res
+
+
+def _pf_fused(torch):
+    """The same model as a four-float device model on the fused plain
+    sampler (tests/cpp/fused_model_test.hip, PF4: FP32 states, the
+    samplers' own noise, stratified resampling below ESS N/2), one
+    captured 100-step run on the event clock."""
+    import ctypes
+    import numpy as np
+    lib_path = os.path.join(ROOT, "tests", "cpp", "libfused_model_test.so")
+    if not os.path.exists(lib_path):
+        return {"pf_example_fused_T100": "libfused_model_test.so not built"}
+    fl = ctypes.CDLL(lib_path)
+    fl.fused_last_error.restype = ctypes.c_char_p
+    xy = np.ascontiguousarray(np.loadtxt(
+        os.path.join(ROOT, "tests", "golden", "algorithm_pf.data")).ravel())
+    T = len(xy) // 2
+    out = {}
+    for N in (100000, 1 << 20):
+        ll = ctypes.c_double()
+        ess = np.zeros(T)
+        res = np.zeros(T, np.int32)
+        post = np.zeros(2)
+        ms = ctypes.c_double()
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        rc = fl.fused_pf_run(P(xy), ctypes.c_size_t(T), ctypes.c_size_t(N),
+                             ctypes.c_uint64(5), ctypes.byref(ll), P(ess),
+                             P(res), P(post), ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(fl.fused_last_error().decode())
+        out["pf_example_fused_N%d_T100" % N] = {
+            "ms": round(ms.value, 3),
+            "particle_steps_per_s": N * T / (ms.value * 1e-3),
+            "resampling_steps": int(res.sum()),
+            "posterior_mean_p_last": float(post[0])}
+    return out
 
 
 def bench_smc_batched_run(m, torch):
