@@ -822,15 +822,13 @@ __global__ __launch_bounds__(G *kGroup, G == 1 ? 2 : 1) void k_batch(
                 float l[kItems];
                 const float mt = tile_move<false>(A, model, t, kk, xin, xout,
                     lw, idx, nullptr, resampled, mprev, gt, L.f, l);
+                // the double-double totals only when the step resamples
                 double s, s2;
-                dd et;
-                tile_sums_l(A, kk, l, mt, gt, L.d, s, s2, et);
+                tile_sums2(A, kk, l, mt, gt, L.d, s, s2);
                 if (gt == 0 && k < nt) {
                     S.tmax[k] = mt;
                     S.ts[k] = s;
                     S.ts2[k] = s2;
-                    S.tdh[k] = et.hi;
-                    S.tdl[k] = et.lo;
                 }
                 __syncthreads();
             }
@@ -844,9 +842,6 @@ __global__ __launch_bounds__(G *kGroup, G == 1 ? 2 : 1) void k_batch(
                 S.r[i] = rr;
                 S.ts[i] = __dmul_rn(rr, S.ts[i]);
                 S.ts2[i] = __dmul_rn(__dmul_rn(rr, rr), S.ts2[i]);
-                const dd u = dd_mul_d(dd{S.tdh[i], S.tdl[i]}, rr);
-                S.tdh[i] = u.hi;
-                S.tdl[i] = u.lo;
             }
             __syncthreads();
             double rs, rs2;
@@ -857,6 +852,22 @@ __global__ __launch_bounds__(G *kGroup, G == 1 ? 2 : 1) void k_batch(
             mprev = m;
             resampled = o.resample;
             if (resampled) {
+                // the tiles' double-double totals of a (tile_sums_l's etot,
+                // from the stored log weights), scaled by r_k
+                for (int r = 0; r < rounds; ++r) {
+                    const int k = r * G + grp;
+                    const uint32_t kk = (uint32_t)(k < nt ? k : nt);
+                    float l[kItems];
+                    load8(lw, tile_first(kk, gt), A.N, l, 0.0f);
+                    const dd et = tile_dd_total(
+                        A, kk, l, k < nt ? S.tmax[k] : 0.0f, gt, L.d);
+                    if (gt == 0 && k < nt) {
+                        const dd u = dd_mul_d(et, S.r[k]);
+                        S.tdh[k] = u.hi;
+                        S.tdl[k] = u.lo;
+                    }
+                    __syncthreads();
+                }
                 double *u = ub ? ub + off : nullptr;
                 if (A.scheme == MCKL_HIP_MULTINOMIAL) // sorted uniforms first
                     batch_muniforms<G>(A, t, nt, rounds, grp, gt, L, S, u, A.N);

@@ -609,6 +609,45 @@ __device__ __forceinline__ dd tile_sums_l(const SmcArgs &A, uint32_t tile,
     return group_excl_scan_dd(loc, gt, ldsd, etot);
 }
 
+// tile_sums_l's S and S2 alone (same operations), for a sampler that takes
+// the double-double total only on the steps that resample (tile_dd_total)
+__device__ __forceinline__ void tile_sums2(const SmcArgs &A, uint32_t tile,
+    const float (&l)[kItems], float m, int gt, double *ldsd, double &S,
+    double &S2)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    double s = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (i0 + k < A.N) {
+            const double e = (double)smc_exp_f(__fsub_rn(l[k], m));
+            s = __dadd_rn(s, e);
+            s2 = __dadd_rn(s2, __dmul_rn(e, e));
+        }
+    }
+    group_sum2(s, s2, gt, ldsd);
+    S = s;
+    S2 = s2;
+}
+
+// tile_sums_l's double-double total etot alone (same operations)
+__device__ __forceinline__ dd tile_dd_total(const SmcArgs &A, uint32_t tile,
+    const float (&l)[kItems], float m, int gt, double *ldsd)
+{
+    const uint64_t i0 = tile_first(tile, gt);
+    dd loc{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (i0 + k < A.N) {
+            const double e = (double)smc_exp_f(__fsub_rn(l[k], m));
+            loc = dd_add(loc, dd{e, 0.0});
+        }
+    }
+    dd etot;
+    (void)group_excl_scan_dd(loc, gt, ldsd, etot);
+    return etot;
+}
+
 // Uniform sequences of step t (sorted, M = N):
 //   systematic u_j = j*delta + u0d  (resample.hpp:122-135), u0d = u01*delta
 //   stratified u_j = delta*(c_j + j) (resample.hpp:95-120)
