@@ -445,13 +445,14 @@ __global__ __launch_bounds__(kGroup) void k_rs_counts(SmcArgs A, uint32_t t,
     // transform (parents as codes, resolved by the next move); the tile's
     // own part of the transform runs while its predecessors publish
     uint32_t es, zs, cs[kItems], prev;
+    TileScan sc;
     tile_counts_c(A, k, ta, ttoff, rk, invS, toff, next_hi, U, gt, L.u, cs,
-        prev, es, zs);
+        prev, es, zs, &sc);
     uint64_t *w = B.lbw + (size_t)c * nt;
     if (gt == 0)
         lb_publish(w, k, es, zs);
     TileIndex ti;
-    tile_index_local(A, k, cs, prev, gt, L.u, Es, Srcl, ti);
+    tile_index_local(A, k, cs, prev, gt, L.u, Es, Srcl, ti, &sc);
     uint32_t eoff, zoff;
     tile_lookback(w, k, es, zs, gt, L.u, &st->err, eoff, zoff);
     tile_index_store(A, k, ti, eoff, zoff, idx, B.src, gt, Es, Srcl);

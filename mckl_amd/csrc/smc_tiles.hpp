@@ -792,6 +792,13 @@ __device__ __forceinline__ void tile_counts_local(const SmcArgs &A,
     toff = group_excl_scan_dd(loc, gt, ldsd, total);
 }
 
+// a thread's exclusive prefixes of the tile's extra-copy and zero-slot
+// counts and the tile's totals (tile_counts_c can hand them to
+// tile_index_local, which would otherwise scan them again)
+struct TileScan {
+    uint32_t ex, zx, te, tz;
+};
+
 // Cumulative normalised weights S_j = round(prefix_j(e)) * (1/S), prefix in
 // double-double: the tile's offset (tile_off, from the exclusive scan of the
 // scaled tile totals) plus r_k times the in-tile prefix of a_i; the last
@@ -809,7 +816,7 @@ __device__ __forceinline__ void tile_counts_c(const SmcArgs &A, uint32_t tile,
     const float (&a)[kItems], const dd toff, double rk, double invS,
     dd tile_off, double next_hi, const UniformSeq &U, int gt,
     uint32_t *ldsu, uint32_t (&c)[kItems], uint32_t &prev_out,
-    uint32_t &esum, uint32_t &zsum)
+    uint32_t &esum, uint32_t &zsum, TileScan *scan = nullptr)
 {
     const uint64_t i0 = tile_first(tile, gt);
     const uint64_t tend = (uint64_t)(tile + 1) * kTile < A.N
@@ -894,6 +901,13 @@ __device__ __forceinline__ void tile_counts_c(const SmcArgs &A, uint32_t tile,
             zs += rep == 0 ? 1 : 0;
         }
         prev = c[k];
+    }
+    if (scan) { // uniform over the group
+        scan->ex = group_excl_scan_u(es, gt, ldsu, scan->te);
+        scan->zx = group_excl_scan_u(zs, gt, ldsu + 4, scan->tz);
+        esum = scan->te;
+        zsum = scan->tz;
+        return;
     }
     group_sum2_u(es, zs, gt, ldsu);
     esum = es;
@@ -1205,7 +1219,8 @@ __device__ __forceinline__ void rank_sources4(const uint32_t *Es, uint32_t n,
 
 __device__ __forceinline__ void tile_index_local(const SmcArgs &A,
     uint32_t tile, const uint32_t (&c)[kItems], uint32_t prev, int gt,
-    uint32_t *ldsu, uint32_t *Es, uint32_t *srcl, TileIndex &ti)
+    uint32_t *ldsu, uint32_t *Es, uint32_t *srcl, TileIndex &ti,
+    const TileScan *scan = nullptr)
 {
     const uint64_t t0 = (uint64_t)tile * kTile;
     const uint64_t i0 = tile_first(tile, gt);
@@ -1221,9 +1236,16 @@ __device__ __forceinline__ void tile_index_local(const SmcArgs &A,
             zs += rep[k] == 0 ? 1 : 0;
         }
     }
-    uint32_t te, tz;
-    uint32_t ex = group_excl_scan_u(es, gt, ldsu, te);
-    uint32_t zx = group_excl_scan_u(zs, gt, ldsu + 4, tz);
+    uint32_t te, tz, ex, zx;
+    if (scan) { // the counting stage's scans of the same counts
+        ex = scan->ex;
+        zx = scan->zx;
+        te = scan->te;
+        tz = scan->tz;
+    } else {
+        ex = group_excl_scan_u(es, gt, ldsu, te);
+        zx = group_excl_scan_u(zs, gt, ldsu + 4, tz);
+    }
     ti.n = t0 >= A.N ? 0u
         : (uint32_t)(A.N - t0 < (uint64_t)kTile ? A.N - t0 : kTile);
     ti.end = te;
