@@ -573,13 +573,17 @@ __global__ __launch_bounds__(kB) void k_ri_write(uint64_t N, uint64_t K,
 // extra copy r goes to the zero slot of rank r (r < ztot) or to slot
 // K + r - ztot.  One workgroup per tile of sources: the tile's E in LDS, its
 // ranks [E_{t0-1}, E_{t0+n-1}) dealt round-robin to the threads, each
-// source found by bisection in LDS (the first j with E_j > r).  Slots past
-// M (a replication vector that does not sum to M) are not written.
+// rank's source (the first j with E_j > r) from LDS: when the tile holds at
+// most kTile ranks, every source marks its first rank and a max-scan over
+// the ranks fills in the rest (sources are in rank order); otherwise by
+// bisection.  Slots past M (a replication vector that does not sum to M)
+// are not written.
 __global__ __launch_bounds__(kB) void k_ri_deal(uint64_t N, uint64_t K,
     uint64_t M, const uint32_t *Eincl, const uint32_t *zpos,
     const uint32_t *tz, uint32_t nt, uint32_t *idx)
 {
     __shared__ uint32_t Es[kTile];
+    __shared__ uint32_t Src[kTile];
     __shared__ uint32_t su[4];
     uint32_t z = 0;
     for (uint32_t i = threadIdx.x; i < nt; i += kB)
@@ -595,6 +599,45 @@ __global__ __launch_bounds__(kB) void k_ri_deal(uint64_t N, uint64_t K,
     // ranks past ztot + (M - K) have no destination: a vector that is not a
     // replication vector (extras beyond M) ends its loop there
     const uint32_t end = rank_end(Es[n - 1], (uint64_t)ztot + (M - K));
+    if (end > beg && end - beg <= (uint32_t)kTile) { // uniform over the group
+        // marks: source q's first rank E_{q-1} (q's extras are the ranks
+        // [E_{q-1}, E_q)); rank 0 of the tile is always marked
+        const uint32_t q0 = threadIdx.x * kItems;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            Src[q0 + k] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const uint32_t q = q0 + k;
+            if (q < n) {
+                const uint32_t first = q == 0 ? beg : Es[q - 1];
+                if (Es[q] > first && first < end)
+                    Src[first - beg] = q;
+            }
+        }
+        __syncthreads();
+        uint32_t v[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            v[k] = Src[q0 + k];
+#pragma unroll
+        for (int k = 1; k < kItems; ++k)
+            v[k] = v[k] > v[k - 1] ? v[k] : v[k - 1];
+        const uint32_t before =
+            group_excl_max_u(v[kItems - 1], 0u, threadIdx.x, su);
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            Src[q0 + k] = v[k] > before ? v[k] : before;
+        __syncthreads();
+        for (uint32_t r = beg + threadIdx.x; r < end; r += kB) {
+            const uint64_t dest =
+                r < ztot ? (uint64_t)zpos[r] : K + (uint64_t)(r - ztot);
+            if (dest < M)
+                idx[dest] = (uint32_t)(t0 + Src[r - beg]);
+        }
+        return;
+    }
     for (uint32_t r = beg + threadIdx.x; r < end; r += kB) {
         const uint32_t lo = rank_source(Es, n, r); // Es[n - 1] >= end > r
         const uint64_t dest =
