@@ -253,6 +253,35 @@ def bench_smc_generic(m, torch):
                 "host clock around the whole run (one ESS read per step)"}}
 
 
+def bench_weight(m, torch):
+    """Weight::add_log (core/weight.hpp:143-148, 198-227) on the device at
+    N = 2^20 and 2^24: W_i = exp(log W_i + v_i - max) / sum, ESS, log of
+    the normaliser.  Algorithmic bytes: v and W read, W written (24 B per
+    weight); the three passes (log + max, exp + sums, scale) move 48."""
+    out = {}
+    for lg in (20, 24):
+        N = 1 << lg
+        g = torch.Generator(device="cuda").manual_seed(5)
+        w = m.Weight(N)
+        v = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+        for _ in range(3):
+            w.add_log(v)
+        torch.cuda.synchronize()
+        a, b = _events()
+        reps = 20
+        a.record()
+        for _ in range(reps):
+            w.add_log(v)
+        b.record()
+        torch.cuda.synchronize()
+        us = a.elapsed_time(b) / reps * 1e3
+        gbs = 24.0 * N / (us * 1e-6) / 1e9
+        out["weight_add_log_N2^%d" % lg] = {
+            "us": round(us, 1), "GBps_algorithmic": round(gbs, 1),
+            "frac_of_hbm": round(gbs / 8000.0, 3)}
+    return out
+
+
 def bench_resample(m, torch):
     """ResampleAlgorithm + rep -> index on the device (mckl_hip_resample) at
     N = M = 2^20 on normalised weights exp(N(0, 1)) (a typical SMC step):
@@ -819,7 +848,8 @@ def main():
                                        "smc_plain", "smc_generic",
                                        "smc_batched", "smc_batched_run",
                                        "grad", "pf_example",
-                                       "resample_index", "resample"],
+                                       "resample_index", "resample",
+                                       "weight"],
                     help="run one secondary leg alone and print its JSON "
                          "(for profiling)")
     args = ap.parse_args()
@@ -850,6 +880,7 @@ def main():
                "smc_batched_run": bench_smc_batched_run,
                "grad": bench_grad,
                "pf_example": bench_pf_example,
+               "weight": bench_weight,
                "resample_index": bench_resample_index,
                "resample": bench_resample}[args.only]
         print(json.dumps(leg(m, torch)), flush=True)
@@ -870,6 +901,7 @@ def main():
                           ("pf_example", bench_pf_example),
                           ("resample_index", bench_resample_index),
                           ("resample", bench_resample),
+                          ("weight", bench_weight),
                           ("cpu", lambda m_, t_: {
                               "cpu": _guarded(cpu_reference_points,
                                               threads)})):

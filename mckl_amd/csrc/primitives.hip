@@ -141,6 +141,7 @@ __device__ __forceinline__ double blocks_max(const double *pmax, uint32_t n,
     double *sm)
 {
     double m = -__builtin_huge_val();
+#pragma unroll 4
     for (uint32_t i = threadIdx.x; i < n; i += kB)
         m = m < pmax[i] ? pmax[i] : m;
     m = wave_max_d(m);
@@ -154,14 +155,44 @@ __device__ __forceinline__ double blocks_max(const double *pmax, uint32_t n,
     return r;
 }
 
+// Many tiles: one workgroup reduces the block maxima (gmax[0]), or later
+// the partial sums (gsum[0..1], and stats), and the other kernels read the
+// result instead of each reducing every tile's value again (the same
+// canonical functions, so the same bits).
+__global__ __launch_bounds__(kB) void k_weight_gmax(
+    const double *pmax, uint32_t nb, double *gmax)
+{
+    __shared__ double sd[8];
+    const double m = blocks_max(pmax, nb, sd);
+    if (threadIdx.x == 0)
+        gmax[0] = m;
+}
+
+__global__ __launch_bounds__(kB) void k_weight_gsum(const double *ps,
+    const double *ps2, uint32_t nb, const double *lmax, double *gsum,
+    double *stats)
+{
+    __shared__ double sd[8];
+    double a, b;
+    tiles_sum2(ps, ps2, (int)nb, threadIdx.x, sd, a, b);
+    if (threadIdx.x == 0) {
+        gsum[0] = a;
+        gsum[1] = b;
+        stats[0] = __ddiv_rn(__dmul_rn(a, a), b);
+        stats[1] = __dadd_rn(*lmax, log(a));
+    }
+}
+
 // w = exp(w - lmax), lmax the max of the block maxima (taken by every
-// workgroup; block 0 keeps it in *lmax); block partial sums of w and w^2
+// workgroup, or from gmax; block 0 keeps it in *lmax); block partial sums of
+// w and w^2
 __global__ __launch_bounds__(kB) void k_weight_exp(uint64_t n, double *w,
-    const double *pmax, uint32_t nb, double *lmax, double *ps, double *ps2)
+    const double *pmax, uint32_t nb, const double *gmax, double *lmax,
+    double *ps, double *ps2)
 {
     __shared__ double sd[8];
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
-    const double m = blocks_max(pmax, nb, sd);
+    const double m = gmax ? gmax[0] : blocks_max(pmax, nb, sd);
     if (blockIdx.x == 0 && threadIdx.x == 0)
         *lmax = m;
     double a = 0.0, b = 0.0;
@@ -188,15 +219,20 @@ __global__ __launch_bounds__(kB) void k_weight_exp(uint64_t n, double *w,
 // every workgroup), w *= 1/accw on the workgroup's tile; block 0 writes
 // stats = {ESS, lmax + log(accw)}
 __global__ __launch_bounds__(kB) void k_weight_scale(uint64_t n, double *w,
-    const double *ps, const double *ps2, uint32_t nb, const double *lmax,
-    double *stats)
+    const double *ps, const double *ps2, uint32_t nb, const double *gsum,
+    const double *lmax, double *stats)
 {
     __shared__ double sd[8];
     double a, b;
-    tiles_sum2(ps, ps2, (int)nb, threadIdx.x, sd, a, b);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        stats[0] = __ddiv_rn(__dmul_rn(a, a), b);
-        stats[1] = __dadd_rn(*lmax, log(a));
+    if (gsum) { // k_weight_gsum wrote the sums and stats
+        a = gsum[0];
+        b = gsum[1];
+    } else {
+        tiles_sum2(ps, ps2, (int)nb, threadIdx.x, sd, a, b);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            stats[0] = __ddiv_rn(__dmul_rn(a, a), b);
+            stats[1] = __dadd_rn(*lmax, log(a));
+        }
     }
     const double sc = __ddiv_rn(1.0, a);
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
@@ -256,15 +292,39 @@ __global__ __launch_bounds__(kB) void k_dd_tiles(int mode, uint64_t n,
 // operations of a scan over the tiles, taken by every workgroup for
 // itself); off_hi: this tile's offset rounded (the cumulative weight of the
 // previous tile's last element).
+// Many tiles: one workgroup writes every tile's offsets (tiles_scan_dd,
+// tiles_scan_u2: the operations each workgroup would otherwise repeat over
+// all tiles), and the tile kernels read their own.
+constexpr uint32_t kTwoLevelTiles = 1024;
+
+__global__ __launch_bounds__(kB) void k_tiles_scan_dd(const double *th,
+    const double *tl, uint32_t nt, double *oh, double *ol)
+{
+    __shared__ double sd[16];
+    tiles_scan_dd(true, th, tl, (int)nt, oh, ol, threadIdx.x, sd);
+}
+
+__global__ __launch_bounds__(kB) void k_tiles_scan_u2(const uint32_t *te,
+    const uint32_t *tz, uint32_t nt, uint32_t *teo, uint32_t *tzo)
+{
+    __shared__ uint32_t su[8];
+    tiles_scan_u2(true, te, tz, (int)nt, teo, tzo, threadIdx.x, su);
+}
+
 __device__ __forceinline__ void tile_prefix(int mode, uint64_t n, uint64_t N,
     const double *a, const double *th, const double *tl, uint32_t nt,
-    double *P, double &off_hi)
+    const double *oh, const double *ol, double *P, double &off_hi)
 {
     __shared__ double sd[16];
     const uint32_t tile = blockIdx.x;
     dd toff, tnext;
-    tiles_offset_dd(th, tl, (int)nt, (int)tile, (int)tile + 1, threadIdx.x, sd,
-        toff, tnext);
+    if (oh) { // the offsets k_tiles_scan_dd wrote (the same operations)
+        toff = dd{oh[tile], ol[tile]};
+        tnext = dd{oh[tile + 1], ol[tile + 1]};
+    } else {
+        tiles_offset_dd(th, tl, (int)nt, (int)tile, (int)tile + 1,
+            threadIdx.x, sd, toff, tnext);
+    }
     off_hi = toff.hi;
     const uint64_t i0 = (uint64_t)tile * kTile + threadIdx.x * kItems;
     const uint64_t tend = (uint64_t)(tile + 1) * kTile < n
@@ -293,11 +353,11 @@ __device__ __forceinline__ void tile_prefix(int mode, uint64_t n, uint64_t N,
 // (resample.hpp:63-81)
 __global__ __launch_bounds__(kB) void k_sorted_finish(uint64_t n,
     const double *c, const double *th, const double *tl, uint32_t nt,
-    double *r)
+    const double *oh, const double *ol, double *r)
 {
     double P[kItems];
     double off_hi;
-    tile_prefix(1, n, n, c, th, tl, nt, P, off_hi);
+    tile_prefix(1, n, n, c, th, tl, nt, oh, ol, P, off_hi);
     __syncthreads();
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     double v[kItems];
@@ -384,13 +444,14 @@ __device__ __forceinline__ uint32_t extras(uint32_t rep, uint64_t i, uint64_t K)
 // of the rep -> index transform (K = min(N, M)) are written as well.
 __global__ __launch_bounds__(kB) void k_counts(uint64_t N, uint64_t M,
     const double *w, const double *u, const double *th, const double *tl,
-    uint32_t nt, uint32_t *rep, uint64_t K, uint32_t *te, uint32_t *tz)
+    uint32_t nt, const double *oh, const double *ol, uint32_t *rep,
+    uint64_t K, uint32_t *te, uint32_t *tz)
 {
     __shared__ uint32_t su[8];
     __shared__ uint32_t bnd[2];
     double P[kItems];
     double off_hi;
-    tile_prefix(0, N, N, w, th, tl, nt, P, off_hi);
+    tile_prefix(0, N, N, w, th, tl, nt, oh, ol, P, off_hi);
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     const double Md = (double)M;
     const double inf = __builtin_huge_val();
@@ -535,11 +596,18 @@ __device__ __forceinline__ void tiles_before_u2(const uint32_t *a,
 // zero slot i of rank z.
 __global__ __launch_bounds__(kB) void k_ri_write(uint64_t N, uint64_t K,
     const uint32_t *rep, const uint32_t *te, const uint32_t *tz, uint32_t nt,
-    uint32_t *Eincl, uint32_t *idx, uint32_t *zpos)
+    const uint32_t *teo, const uint32_t *tzo, uint32_t *Eincl, uint32_t *idx,
+    uint32_t *zpos)
 {
     __shared__ uint32_t su[12];
     uint32_t eoff, zoff, ztot;
-    tiles_before_u2(te, tz, nt, blockIdx.x, su, eoff, zoff, ztot);
+    if (teo) { // the offsets k_tiles_scan_u2 wrote
+        eoff = teo[blockIdx.x];
+        zoff = tzo[blockIdx.x];
+        ztot = tzo[nt];
+    } else {
+        tiles_before_u2(te, tz, nt, blockIdx.x, su, eoff, zoff, ztot);
+    }
     const uint64_t i0 = (uint64_t)blockIdx.x * kTile + threadIdx.x * kItems;
     uint32_t es = 0, zs = 0;
     uint32_t r[kItems];
@@ -580,16 +648,20 @@ __global__ __launch_bounds__(kB) void k_ri_write(uint64_t N, uint64_t K,
 // are not written.
 __global__ __launch_bounds__(kB) void k_ri_deal(uint64_t N, uint64_t K,
     uint64_t M, const uint32_t *Eincl, const uint32_t *zpos,
-    const uint32_t *tz, uint32_t nt, uint32_t *idx)
+    const uint32_t *tz, uint32_t nt, const uint32_t *tzo, uint32_t *idx)
 {
     __shared__ uint32_t Es[kTile];
     __shared__ uint32_t Src[kTile];
     __shared__ uint32_t su[4];
-    uint32_t z = 0;
-    for (uint32_t i = threadIdx.x; i < nt; i += kB)
-        z += tz[i];
     uint32_t ztot;
-    (void)group_excl_scan_u(z, threadIdx.x, su, ztot);
+    if (tzo) {
+        ztot = tzo[nt];
+    } else {
+        uint32_t z = 0;
+        for (uint32_t i = threadIdx.x; i < nt; i += kB)
+            z += tz[i];
+        (void)group_excl_scan_u(z, threadIdx.x, su, ztot);
+    }
     const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
     const uint32_t n = (uint32_t)(N - t0 < (uint64_t)kTile ? N - t0 : kTile);
     for (uint32_t q = threadIdx.x; q < n; q += kB)
@@ -728,9 +800,18 @@ int weight_normalize(int mode, size_t n, const double *v, double *w,
     double *ps = reinterpret_cast<double *>(base + o1);
     double *ps2 = reinterpret_cast<double *>(base + 2 * o1);
     double *lmax = reinterpret_cast<double *>(base + 3 * o1);
+    // past 1024 tiles every workgroup reducing every tile's values again
+    // costs more than two one-workgroup launches
+    const bool two_level = nt > 1024;
+    double *gmax = two_level ? lmax + 1 : nullptr;
+    double *gsum = two_level ? lmax + 2 : nullptr;
     k_weight_pre<<<nt, kB, 0, st>>>(mode, n, v, w, pmax);
-    k_weight_exp<<<nt, kB, 0, st>>>(n, w, pmax, nt, lmax, ps, ps2);
-    k_weight_scale<<<nt, kB, 0, st>>>(n, w, ps, ps2, nt, lmax, stats);
+    if (two_level)
+        k_weight_gmax<<<1, kB, 0, st>>>(pmax, nt, gmax);
+    k_weight_exp<<<nt, kB, 0, st>>>(n, w, pmax, nt, gmax, lmax, ps, ps2);
+    if (two_level)
+        k_weight_gsum<<<1, kB, 0, st>>>(ps, ps2, nt, lmax, gsum, stats);
+    k_weight_scale<<<nt, kB, 0, st>>>(n, w, ps, ps2, nt, gsum, lmax, stats);
     MCKL_HIP_CHECK_LAUNCH("weight kernels");
     return MCKL_HIP_OK;
 }
@@ -757,13 +838,15 @@ struct RsWork {
     uint32_t *E, *zpos;
     double *u01;
     double *c;
+    double *oh, *ol;     // tile offsets (many tiles: k_tiles_scan_dd)
+    uint32_t *teo, *tzo; // tile offsets (many tiles: k_tiles_scan_u2)
 };
 
 constexpr size_t rs_work_bytes(size_t N, size_t M, bool sorted)
 {
     const size_t nt = std::max(ceil_div(N, kTile), ceil_div(M, kTile)) + 1;
-    return 2 * align256(nt * sizeof(double)) +
-        2 * align256(nt * sizeof(uint32_t)) +
+    return 4 * align256(nt * sizeof(double)) +
+        4 * align256(nt * sizeof(uint32_t)) +
         2 * align256(std::max<size_t>(N, 1) * sizeof(uint32_t)) + 256 +
         (sorted ? align256(std::max<size_t>(M, 1) * sizeof(double)) : 0);
 }
@@ -780,6 +863,12 @@ void rs_work_carve(void *p, size_t N, size_t M, bool sorted, RsWork &w)
     b += 2 * od;
     w.te = reinterpret_cast<uint32_t *>(b);
     w.tz = reinterpret_cast<uint32_t *>(b + ou);
+    b += 2 * ou;
+    w.oh = reinterpret_cast<double *>(b);
+    w.ol = reinterpret_cast<double *>(b + od);
+    b += 2 * od;
+    w.teo = reinterpret_cast<uint32_t *>(b);
+    w.tzo = reinterpret_cast<uint32_t *>(b + ou);
     b += 2 * ou;
     w.E = reinterpret_cast<uint32_t *>(b);
     w.zpos = reinterpret_cast<uint32_t *>(b + on);
@@ -851,7 +940,11 @@ int sequence(int scheme, mckl_hip_engine *e, size_t n, double *r,
     rc = dd_tiles(1, n, w->c, w->th, w->tl, st);
     if (rc)
         return rc;
-    k_sorted_finish<<<nt, kB, 0, st>>>(n, w->c, w->th, w->tl, nt, r);
+    const bool two = nt > kTwoLevelTiles;
+    if (two)
+        k_tiles_scan_dd<<<1, kB, 0, st>>>(w->th, w->tl, nt, w->oh, w->ol);
+    k_sorted_finish<<<nt, kB, 0, st>>>(n, w->c, w->th, w->tl, nt,
+        two ? w->oh : nullptr, two ? w->ol : nullptr, r);
     MCKL_HIP_CHECK_LAUNCH("k_sorted_finish");
     return MCKL_HIP_OK;
 }
@@ -882,8 +975,12 @@ int trans_u01_rep(size_t N, size_t M, const double *wt, const double *u,
     rc = dd_tiles(0, N, wt, w->th, w->tl, st);
     if (rc)
         return rc;
-    k_counts<<<nt, kB, 0, st>>>(N, M, wt, u, w->th, w->tl, nt, rep,
-        std::min(N, M), tiles ? w->te : nullptr, tiles ? w->tz : nullptr);
+    const bool two = nt > kTwoLevelTiles;
+    if (two)
+        k_tiles_scan_dd<<<1, kB, 0, st>>>(w->th, w->tl, nt, w->oh, w->ol);
+    k_counts<<<nt, kB, 0, st>>>(N, M, wt, u, w->th, w->tl, nt,
+        two ? w->oh : nullptr, two ? w->ol : nullptr, rep, std::min(N, M),
+        tiles ? w->te : nullptr, tiles ? w->tz : nullptr);
     MCKL_HIP_CHECK_LAUNCH("k_counts");
     if (tiles_done)
         *tiles_done = tiles;
@@ -905,9 +1002,13 @@ int rep_index(size_t N, size_t M, const uint32_t *rep, uint32_t *idx,
         return rc;
     if (!have_tiles)
         k_ri_tiles<<<nt, kB, 0, st>>>(N, K, rep, w->te, w->tz);
-    k_ri_write<<<nt, kB, 0, st>>>(
-        N, K, rep, w->te, w->tz, nt, w->E, idx, w->zpos);
-    k_ri_deal<<<nt, kB, 0, st>>>(N, K, M, w->E, w->zpos, w->tz, nt, idx);
+    const bool two = nt > kTwoLevelTiles;
+    if (two)
+        k_tiles_scan_u2<<<1, kB, 0, st>>>(w->te, w->tz, nt, w->teo, w->tzo);
+    k_ri_write<<<nt, kB, 0, st>>>(N, K, rep, w->te, w->tz, nt,
+        two ? w->teo : nullptr, two ? w->tzo : nullptr, w->E, idx, w->zpos);
+    k_ri_deal<<<nt, kB, 0, st>>>(N, K, M, w->E, w->zpos, w->tz, nt,
+        two ? w->tzo : nullptr, idx);
     MCKL_HIP_CHECK_LAUNCH("rep->index kernels");
     return MCKL_HIP_OK;
 }

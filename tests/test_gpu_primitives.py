@@ -188,14 +188,17 @@ def test_select(dev):
 
 @pytest.mark.parametrize("scheme", range(6))
 @pytest.mark.parametrize("N,M", [((1 << 20) + 12345, (1 << 20) + 12345),
-                                 (600001, 900007), (900007, 600001)])
+                                 (600001, 900007), (900007, 600001),
+                                 ((1 << 22) + 777, (1 << 22) + 777),
+                                 ((1 << 22) + 5, 3000001)])
 def test_resample_large_vs_oracle(dev, scheme, N, M):
     """Sizes past 256 tiles of 2048, where every workgroup's cross-tile
     loops (tile offsets of the weights, the rep -> index offsets, the zero
-    total) take several tiles per thread: rep and idx against the oracle
-    (the residual family exactly; the others up to replication changes at
-    a uniform within rounding of a cumulative weight), and idx a stable
-    realisation of rep."""
+    total) take several tiles per thread, and past 1024 tiles, where one
+    workgroup writes every tile's offsets for the others to read: rep and
+    idx against the oracle (the residual family exactly; the others up to
+    replication changes at a uniform within rounding of a cumulative
+    weight), and idx a stable realisation of rep."""
     import torch
     m = _m()
     rng = np.random.default_rng(N + 7 * scheme)
