@@ -44,6 +44,18 @@ def _events():
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
+def _warm(torch, fn, ms=50.0, cap=2000):
+    """Calls fn until ms of wall time have passed (at most cap calls), then
+    synchronises: a short timed loop that follows host-only work otherwise
+    starts on idle GPU clocks."""
+    t0 = time.perf_counter()
+    for _ in range(cap):
+        fn()
+        if (time.perf_counter() - t0) * 1e3 >= ms:
+            break
+    torch.cuda.synchronize()
+
+
 def bench_config1(m, torch, steps, warmup):
     """One step = rand(rng, n, u32) then rand(rng, U01OO<double>, n, f64),
     each from a fresh Philox4x32 with the configuration key, through the C
@@ -264,11 +276,9 @@ def bench_weight(m, torch):
         g = torch.Generator(device="cuda").manual_seed(5)
         w = m.Weight(N)
         v = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
-        for _ in range(3):
-            w.add_log(v)
-        torch.cuda.synchronize()
+        _warm(torch, lambda: w.add_log(v))
         a, b = _events()
-        reps = 20
+        reps = 50
         a.record()
         for _ in range(reps):
             w.add_log(v)
@@ -301,11 +311,9 @@ def bench_resample(m, torch):
             ("residual_systematic", m.ResampleResidualSystematic(),
              m.RESIDUAL_SYSTEMATIC)):
         rng = m.Philox4x32(9)
-        for _ in range(3):
-            alg(N, N, rng, w, rep, idx)
-        torch.cuda.synchronize()
+        _warm(torch, lambda: alg(N, N, rng, w, rep, idx))
         a, b = _events()
-        reps = 20
+        reps = 50
         a.record()
         for _ in range(reps):
             alg(N, N, rng, w, rep, idx)

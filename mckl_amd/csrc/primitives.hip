@@ -931,12 +931,12 @@ int sequence(int scheme, mckl_hip_engine *e, size_t n, double *r,
         MCKL_HIP_CHECK_LAUNCH("k_systematic");
         return MCKL_HIP_OK;
     }
-    rc = rand_canonical_f64(e, n, r, st);
+    // the draws go straight to the workspace (k_sorted_finish reads them
+    // there and writes r)
+    rc = rand_canonical_f64(e, n, w->c, st);
     if (rc)
         return rc;
     const uint32_t nt = (uint32_t)ceil_div(n, kTile);
-    MCKL_HIP_TRY(hipMemcpyAsync(w->c, r, n * sizeof(double),
-        hipMemcpyDeviceToDevice, st));
     rc = dd_tiles(1, n, w->c, w->th, w->tl, st);
     if (rc)
         return rc;
