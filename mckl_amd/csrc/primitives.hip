@@ -767,14 +767,58 @@ uint32_t grid_for(uint64_t n, uint32_t cap = 8192)
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256), cap));
 }
 
-// RAII for stream-ordered scratch
+// The library's own stream-ordered pool on each device, created on first
+// use and kept until the process ends.  Its release threshold is unbounded:
+// memory freed by a call stays in the pool for the next one, also across a
+// stream synchronisation (the residual schemes synchronise inside every call
+// to read R), instead of going back to the driver and being mapped again.
+// The pool holds at most the largest scratch one call has needed.  The
+// process's default pool is left as the caller set it.
+constexpr int kMaxPoolDevices = 64;
+std::mutex g_pool_mu;
+hipMemPool_t g_pool[kMaxPoolDevices] = {};
+
+int scratch_pool(hipMemPool_t &pool)
+{
+    int dev = 0;
+    MCKL_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxPoolDevices)
+        return set_error(MCKL_HIP_EINVAL, "device ordinal beyond 64");
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (!g_pool[dev]) {
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t p = nullptr;
+        MCKL_HIP_TRY(hipMemPoolCreate(&p, &props));
+        uint64_t keep = UINT64_MAX;
+        const hipError_t err =
+            hipMemPoolSetAttribute(p, hipMemPoolAttrReleaseThreshold, &keep);
+        if (err != hipSuccess) {
+            (void)hipMemPoolDestroy(p);
+            MCKL_HIP_TRY(err);
+        }
+        g_pool[dev] = p;
+    }
+    pool = g_pool[dev];
+    return MCKL_HIP_OK;
+}
+
+// RAII for stream-ordered scratch from the library's pool
 struct Scratch {
     hipStream_t st;
     void *p = nullptr;
     explicit Scratch(hipStream_t s) : st(s) {}
     int alloc(size_t bytes)
     {
-        MCKL_HIP_TRY(hipMallocAsync(&p, std::max<size_t>(bytes, 256), st));
+        hipMemPool_t pool = nullptr;
+        const int rc = scratch_pool(pool);
+        if (rc)
+            return rc;
+        MCKL_HIP_TRY(hipMallocFromPoolAsync(
+            &p, std::max<size_t>(bytes, 256), pool, st));
         return MCKL_HIP_OK;
     }
     ~Scratch()
